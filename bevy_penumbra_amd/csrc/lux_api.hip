@@ -1,0 +1,472 @@
+// lux_api.hip -- context, scene upload and BVH orchestration behind the C ABI.
+//
+// The context owns every device allocation (the reference's ViewBuffers /
+// SceneData ownership, src/render.rs:5077-5160); caller host arrays are
+// copied on call.  Errors never abort: they return a negative status and the
+// text is kept for lux_last_error (forwarded to
+// RadianceCascadesStatus::record_failure, src/lib.rs:225-230).
+#include <cstring>
+#include <cstdio>
+#include "lux_internal.h"
+
+int lux_fail(lux_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->error = msg;
+    return code;
+}
+
+int lux_hip_check(lux_ctx* ctx, hipError_t e, const char* what) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "%s failed: %s", what, hipGetErrorString(e));
+    return lux_fail(ctx, e == hipErrorOutOfMemory ? LUX_ERR_OOM : LUX_ERR_HIP, buf);
+}
+
+void lux_release(lux_buffer& b) {
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+int lux_ensure(lux_ctx* ctx, lux_buffer& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.bytes >= bytes) return LUX_OK;
+    if (b.ptr) {
+        // the buffer may still be read by queued work
+        LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        lux_release(b);
+    }
+    size_t cap = bytes + bytes / 8;
+    LUX_HIP(ctx, hipMalloc(&b.ptr, cap));
+    b.bytes = cap;
+    return LUX_OK;
+}
+
+namespace {
+
+__device__ __forceinline__ float sel_min(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float sel_max(float a, float b) { return b > a ? b : a; }
+
+// Triangle boxes of one mesh (input order).
+__global__ void k_tri_boxes(const float* pos, uint32_t n, float4* lo, float4* hi) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* p = pos + 9ull * i;
+    float4 l, h;
+    l.x = sel_min(sel_min(p[0], p[3]), p[6]);
+    l.y = sel_min(sel_min(p[1], p[4]), p[7]);
+    l.z = sel_min(sel_min(p[2], p[5]), p[8]);
+    h.x = sel_max(sel_max(p[0], p[3]), p[6]);
+    h.y = sel_max(sel_max(p[1], p[4]), p[7]);
+    h.z = sel_max(sel_max(p[2], p[5]), p[8]);
+    l.w = h.w = 0.0f;
+    lo[i] = l;
+    hi[i] = h;
+}
+
+// Gather triangles into BLAS leaf order.
+__global__ void k_tri_gather(const float* pos, const uint32_t* mat, const uint32_t* perm, uint32_t n,
+                             float4* A, float4* B, float4* C, uint32_t* order, uint32_t tri_base) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t s = perm[j];
+    const float* p = pos + 9ull * s;
+    A[j] = make_float4(p[0], p[1], p[2], __uint_as_float(s));
+    B[j] = make_float4(p[3], p[4], p[5], __uint_as_float(mat ? mat[s] : 0u));
+    C[j] = make_float4(p[6], p[7], p[8], 0.0f);
+    order[j] = tri_base + s;
+}
+
+struct InstDesc {
+    float inv[12];
+    float fwd[12];
+    uint32_t mesh, flags;
+};
+
+__global__ void k_inst_boxes(const InstDesc* d, uint32_t n, const float4* blas,
+                             const uint32_t* mesh_blas_base, float4* lo, float4* hi) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const InstDesc& in = d[i];
+    uint32_t root = mesh_blas_base[in.mesh];
+    float4 a = blas[2u * root], b = blas[2u * root + 1u];
+    float mn[3] = {a.x, a.y, a.z}, mx[3] = {b.x, b.y, b.z};
+    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < 8; ++k) {
+        float p[3] = {(k & 1) ? mx[0] : mn[0], (k & 2) ? mx[1] : mn[1], (k & 4) ? mx[2] : mn[2]};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const float* w = in.fwd + 4 * r;
+            float v = ((w[0] * p[0] + w[1] * p[1]) + w[2] * p[2]) + w[3] * 1.0f;
+            l[r] = sel_min(l[r], v);
+            h[r] = sel_max(h[r], v);
+        }
+    }
+    float4 lo4, hi4;
+    float* lp = &lo4.x;
+    float* hp = &hi4.x;
+#pragma unroll
+    for (int a2 = 0; a2 < 3; ++a2) {
+        float m = fmaxf(fabsf(l[a2]), fabsf(h[a2]));
+        float pad = m * 4.76837158e-7f + 1e-30f;
+        lp[a2] = l[a2] - pad;
+        hp[a2] = h[a2] + pad;
+    }
+    lo4.w = hi4.w = 0.0f;
+    lo[i] = lo4;
+    hi[i] = hi4;
+}
+
+__global__ void k_inst_records(const InstDesc* d, const uint32_t* perm, uint32_t n,
+                               const uint32_t* mesh_blas_base, const uint32_t* mesh_tri_base,
+                               const uint32_t* mesh_tri_count, float4* rec, uint32_t* order) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t s = perm[k];
+    const InstDesc& in = d[s];
+    float4* r = rec + 8u * k;
+    for (int i = 0; i < 3; ++i) {
+        r[i] = make_float4(in.inv[4 * i], in.inv[4 * i + 1], in.inv[4 * i + 2], in.inv[4 * i + 3]);
+        r[3 + i] = make_float4(in.fwd[4 * i], in.fwd[4 * i + 1], in.fwd[4 * i + 2], in.fwd[4 * i + 3]);
+    }
+    uint32_t m = in.mesh;
+    r[6] = make_float4(__uint_as_float(in.flags), __uint_as_float(mesh_blas_base[m]),
+                       __uint_as_float(mesh_tri_base[m]), __uint_as_float(mesh_tri_count[m]));
+    r[7] = make_float4(__uint_as_float(s), __uint_as_float(m), 0.0f, 0.0f);
+    order[k] = s;
+}
+
+inline uint32_t nblk(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
+
+}  // namespace
+
+extern "C" int lux_ctx_create(int device, lux_ctx** out) {
+    if (!out) return LUX_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return LUX_ERR_HIP;
+    if (device < 0 || device >= count) return LUX_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return LUX_ERR_HIP;
+    lux_ctx* ctx = new lux_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        delete ctx;
+        return LUX_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->cu_count = prop.multiProcessorCount;
+    *out = ctx;
+    return LUX_OK;
+}
+
+extern "C" void lux_ctx_destroy(lux_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    lux_buffer* bufs[] = {&ctx->raw_pos, &ctx->raw_mat, &ctx->inst_desc, &ctx->materials,
+                          &ctx->triA, &ctx->triB, &ctx->triC, &ctx->tri_order, &ctx->blas,
+                          &ctx->tlas, &ctx->inst, &ctx->inst_order, &ctx->scratch,
+                          &ctx->ray_buf, &ctx->hit_buf};
+    for (lux_buffer* b : bufs) lux_release(*b);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char* lux_last_error(const lux_ctx* ctx) {
+    return ctx ? ctx->error.c_str() : "null context";
+}
+
+extern "C" void* lux_ctx_stream(lux_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+extern "C" int lux_ctx_synchronize(lux_ctx* ctx) {
+    if (!ctx) return LUX_ERR_INVALID;
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LUX_OK;
+}
+
+extern "C" int lux_scene_upload_soa(lux_ctx* ctx, const lux_mesh_desc* meshes, uint32_t mesh_count,
+                                    const lux_instance_desc* instances, uint32_t instance_count,
+                                    const lux_material* materials, uint32_t material_count) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if ((mesh_count && !meshes) || (instance_count && !instances) || (material_count && !materials))
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_soa: null array");
+    uint64_t total = 0;
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        if (meshes[m].triangle_count == 0)
+            return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_soa: mesh without triangles");
+        if (!meshes[m].positions)
+            return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_soa: mesh without positions");
+        total += meshes[m].triangle_count;
+    }
+    if (total >= (1ull << 31))
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_soa: too many triangles");
+    for (uint32_t i = 0; i < instance_count; ++i)
+        if (instances[i].mesh >= mesh_count)
+            return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_soa: instance mesh out of range");
+    LUX_HIP(ctx, hipSetDevice(ctx->device));
+    ctx->n_meshes = mesh_count;
+    ctx->n_tri = (uint32_t)total;
+    ctx->n_inst = instance_count;
+    ctx->n_materials = material_count;
+    ctx->mesh_tri_count.assign(mesh_count, 0);
+    ctx->mesh_tri_base.assign(mesh_count, 0);
+    ctx->mesh_blas_base.assign(mesh_count, 0);
+    uint32_t tb = 0, bb = 0;
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        uint32_t n = meshes[m].triangle_count;
+        ctx->mesh_tri_count[m] = n;
+        ctx->mesh_tri_base[m] = tb;
+        ctx->mesh_blas_base[m] = bb;
+        tb += n;
+        bb += n > 1 ? 2 * n - 1 : 1;
+    }
+    ctx->n_blas = bb;
+    if (int rc = lux_ensure(ctx, ctx->raw_pos, 36ull * total)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->raw_mat, 4ull * total)) return rc;
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        uint32_t n = meshes[m].triangle_count;
+        float* dp = static_cast<float*>(ctx->raw_pos.ptr) + 9ull * ctx->mesh_tri_base[m];
+        uint32_t* dm = static_cast<uint32_t*>(ctx->raw_mat.ptr) + ctx->mesh_tri_base[m];
+        LUX_HIP(ctx, hipMemcpy(dp, meshes[m].positions, 36ull * n, hipMemcpyHostToDevice));
+        if (meshes[m].materials) {
+            LUX_HIP(ctx, hipMemcpy(dm, meshes[m].materials, 4ull * n, hipMemcpyHostToDevice));
+        } else {
+            LUX_HIP(ctx, hipMemset(dm, 0, 4ull * n));
+        }
+    }
+    ctx->instances.assign(instances, instances + instance_count);
+    std::vector<InstDesc> idesc(instance_count);
+    for (uint32_t i = 0; i < instance_count; ++i) {
+        memcpy(idesc[i].inv, instances[i].local_from_world, 48);
+        memcpy(idesc[i].fwd, instances[i].world_from_local, 48);
+        idesc[i].mesh = instances[i].mesh;
+        idesc[i].flags = instances[i].flags;
+    }
+    if (int rc = lux_ensure(ctx, ctx->inst_desc, sizeof(InstDesc) * instance_count + 16)) return rc;
+    if (instance_count)
+        LUX_HIP(ctx, hipMemcpy(ctx->inst_desc.ptr, idesc.data(), sizeof(InstDesc) * instance_count,
+                               hipMemcpyHostToDevice));
+    if (int rc = lux_ensure(ctx, ctx->materials, sizeof(lux_material) * material_count + 16)) return rc;
+    if (material_count)
+        LUX_HIP(ctx, hipMemcpy(ctx->materials.ptr, materials, sizeof(lux_material) * material_count,
+                               hipMemcpyHostToDevice));
+    ctx->blas_built = false;
+    ctx->tlas_built = false;
+    return LUX_OK;
+}
+
+extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_world,
+                                        const float* world_from_local, uint32_t instance_count) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (instance_count != ctx->n_inst)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_set_transforms: instance count changed");
+    if (instance_count && (!local_from_world || !world_from_local))
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_set_transforms: null array");
+    std::vector<InstDesc> idesc(instance_count);
+    for (uint32_t i = 0; i < instance_count; ++i) {
+        memcpy(idesc[i].inv, local_from_world + 12ull * i, 48);
+        memcpy(idesc[i].fwd, world_from_local + 12ull * i, 48);
+        idesc[i].mesh = ctx->instances[i].mesh;
+        idesc[i].flags = ctx->instances[i].flags;
+        memcpy(ctx->instances[i].local_from_world, local_from_world + 12ull * i, 48);
+        memcpy(ctx->instances[i].world_from_local, world_from_local + 12ull * i, 48);
+    }
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (instance_count)
+        LUX_HIP(ctx, hipMemcpy(ctx->inst_desc.ptr, idesc.data(), sizeof(InstDesc) * instance_count,
+                               hipMemcpyHostToDevice));
+    ctx->tlas_built = false;
+    return LUX_OK;
+}
+
+static int build_blas(lux_ctx* ctx) {
+    hipStream_t st = ctx->stream;
+    uint32_t nt = ctx->n_tri;
+    if (int rc = lux_ensure(ctx, ctx->triA, 16ull * nt)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->triB, 16ull * nt)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->triC, 16ull * nt)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->tri_order, 4ull * nt)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->blas, 32ull * ctx->n_blas)) return rc;
+    // per-mesh temporaries: boxes + perm (largest mesh)
+    uint32_t maxn = 1;
+    for (uint32_t n : ctx->mesh_tri_count) maxn = n > maxn ? n : maxn;
+    lux_buffer tmp;
+    if (hipMalloc(&tmp.ptr, 36ull * maxn + 64) != hipSuccess)
+        return lux_fail(ctx, LUX_ERR_OOM, "build_blas: temporary allocation failed");
+    float4* lo = static_cast<float4*>(tmp.ptr);
+    float4* hi = lo + maxn;
+    uint32_t* perm = reinterpret_cast<uint32_t*>(hi + maxn);
+    int rc = LUX_OK;
+    for (uint32_t m = 0; m < ctx->n_meshes && rc == LUX_OK; ++m) {
+        uint32_t n = ctx->mesh_tri_count[m], tb = ctx->mesh_tri_base[m], bb = ctx->mesh_blas_base[m];
+        const float* pos = static_cast<const float*>(ctx->raw_pos.ptr) + 9ull * tb;
+        const uint32_t* mat = static_cast<const uint32_t*>(ctx->raw_mat.ptr) + tb;
+        hipLaunchKernelGGL(k_tri_boxes, dim3(nblk(n, 256)), dim3(256), 0, st, pos, n, lo, hi);
+        float4* nodes = static_cast<float4*>(ctx->blas.ptr) + 2ull * bb;
+        rc = lux_lbvh_build_device(ctx, lo, hi, n, 4u, nodes, perm, bb, tb);
+        if (rc) break;
+        hipLaunchKernelGGL(k_tri_gather, dim3(nblk(n, 256)), dim3(256), 0, st, pos, mat, perm, n,
+                           static_cast<float4*>(ctx->triA.ptr) + tb,
+                           static_cast<float4*>(ctx->triB.ptr) + tb,
+                           static_cast<float4*>(ctx->triC.ptr) + tb,
+                           static_cast<uint32_t*>(ctx->tri_order.ptr) + tb, tb);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = lux_hip_check(ctx, e, "build_blas launch");
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(tmp.ptr);
+    if (rc == LUX_OK) ctx->blas_built = true;
+    return rc;
+}
+
+static int build_tlas(lux_ctx* ctx) {
+    hipStream_t st = ctx->stream;
+    uint32_t ni = ctx->n_inst;
+    ctx->n_tlas = ni > 1 ? 2 * ni - 1 : 1;
+    if (int rc = lux_ensure(ctx, ctx->tlas, 32ull * ctx->n_tlas)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->inst, 128ull * (ni ? ni : 1))) return rc;
+    if (int rc = lux_ensure(ctx, ctx->inst_order, 4ull * (ni ? ni : 1))) return rc;
+    if (ni == 0) {
+        ctx->tlas_built = true;
+        return LUX_OK;
+    }
+    size_t mbytes = 12ull * ctx->n_meshes;
+    lux_buffer tmp;
+    if (hipMalloc(&tmp.ptr, 36ull * ni + mbytes + 64) != hipSuccess)
+        return lux_fail(ctx, LUX_ERR_OOM, "build_tlas: temporary allocation failed");
+    float4* lo = static_cast<float4*>(tmp.ptr);
+    float4* hi = lo + ni;
+    uint32_t* perm = reinterpret_cast<uint32_t*>(hi + ni);
+    uint32_t* mb = perm + ni;
+    uint32_t* mt = mb + ctx->n_meshes;
+    uint32_t* mc = mt + ctx->n_meshes;
+    int rc = LUX_OK;
+    do {
+        hipError_t e;
+        if ((e = hipMemcpyAsync(mb, ctx->mesh_blas_base.data(), 4ull * ctx->n_meshes,
+                                hipMemcpyHostToDevice, st)) != hipSuccess ||
+            (e = hipMemcpyAsync(mt, ctx->mesh_tri_base.data(), 4ull * ctx->n_meshes,
+                                hipMemcpyHostToDevice, st)) != hipSuccess ||
+            (e = hipMemcpyAsync(mc, ctx->mesh_tri_count.data(), 4ull * ctx->n_meshes,
+                                hipMemcpyHostToDevice, st)) != hipSuccess) {
+            rc = lux_hip_check(ctx, e, "build_tlas copy");
+            break;
+        }
+        const InstDesc* d = static_cast<const InstDesc*>(ctx->inst_desc.ptr);
+        hipLaunchKernelGGL(k_inst_boxes, dim3(nblk(ni, 128)), dim3(128), 0, st, d, ni,
+                           static_cast<const float4*>(ctx->blas.ptr), mb, lo, hi);
+        rc = lux_lbvh_build_device(ctx, lo, hi, ni, 4u, static_cast<float4*>(ctx->tlas.ptr), perm,
+                                   0u, 0u);
+        if (rc) break;
+        hipLaunchKernelGGL(k_inst_records, dim3(nblk(ni, 128)), dim3(128), 0, st, d, perm, ni, mb, mt,
+                           mc, static_cast<float4*>(ctx->inst.ptr),
+                           static_cast<uint32_t*>(ctx->inst_order.ptr));
+        if ((e = hipGetLastError()) != hipSuccess) rc = lux_hip_check(ctx, e, "build_tlas launch");
+    } while (0);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(tmp.ptr);
+    if (rc == LUX_OK) ctx->tlas_built = true;
+    return rc;
+}
+
+extern "C" int lux_bvh_build(lux_ctx* ctx, uint32_t mode) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (mode != LUX_BVH_MORTON30) return lux_fail(ctx, LUX_ERR_INVALID, "lux_bvh_build: unknown mode");
+    if (ctx->n_meshes == 0 && ctx->n_inst == 0)
+        return lux_fail(ctx, LUX_ERR_STATE, "lux_bvh_build: no scene uploaded");
+    LUX_HIP(ctx, hipSetDevice(ctx->device));
+    if (!ctx->blas_built) {
+        if (int rc = build_blas(ctx)) return rc;
+        ctx->tlas_built = false;
+    }
+    if (!ctx->tlas_built)
+        if (int rc = build_tlas(ctx)) return rc;
+    return LUX_OK;
+}
+
+extern "C" int lux_bvh_info(lux_ctx* ctx, uint32_t* blas_nodes, uint32_t* tlas_nodes,
+                            uint32_t* triangles, uint32_t* instances) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (blas_nodes) *blas_nodes = ctx->n_blas;
+    if (tlas_nodes) *tlas_nodes = ctx->n_tlas;
+    if (triangles) *triangles = ctx->n_tri;
+    if (instances) *instances = ctx->n_inst;
+    return LUX_OK;
+}
+
+extern "C" int lux_bvh_download(lux_ctx* ctx, lux_bvh_node* blas, lux_bvh_node* tlas,
+                                uint32_t* tri_order, uint32_t* inst_order) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!ctx->tlas_built) return lux_fail(ctx, LUX_ERR_STATE, "lux_bvh_download: BVH not built");
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (blas) LUX_HIP(ctx, hipMemcpy(blas, ctx->blas.ptr, 32ull * ctx->n_blas, hipMemcpyDeviceToHost));
+    if (tlas && ctx->n_inst)
+        LUX_HIP(ctx, hipMemcpy(tlas, ctx->tlas.ptr, 32ull * ctx->n_tlas, hipMemcpyDeviceToHost));
+    if (tri_order)
+        LUX_HIP(ctx, hipMemcpy(tri_order, ctx->tri_order.ptr, 4ull * ctx->n_tri, hipMemcpyDeviceToHost));
+    if (inst_order && ctx->n_inst)
+        LUX_HIP(ctx, hipMemcpy(inst_order, ctx->inst_order.ptr, 4ull * ctx->n_inst, hipMemcpyDeviceToHost));
+    return LUX_OK;
+}
+
+extern "C" int lux_sort_pairs_device(lux_ctx* ctx, const uint32_t* keys_in, uint32_t n,
+                                     uint32_t* keys_out, uint32_t* index_out) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (n == 0) return LUX_OK;
+    if (!keys_in || !keys_out || !index_out)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_sort_pairs_device: null buffer");
+    size_t hw = lux_sort_hist_words(n);
+    lux_buffer tmp;
+    if (hipMalloc(&tmp.ptr, 8ull * n + 4ull * hw + 64) != hipSuccess)
+        return lux_fail(ctx, LUX_ERR_OOM, "lux_sort_pairs_device: allocation failed");
+    uint32_t* tk = static_cast<uint32_t*>(tmp.ptr);
+    uint32_t* tv = tk + n;
+    uint32_t* hist = tv + n;
+    int rc = lux_sort_pairs_impl(ctx, keys_in, n, keys_out, index_out, tk, tv, hist);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp.ptr);
+    return rc;
+}
+
+extern "C" int lux_scene_upload_built(lux_ctx* ctx, const lux_bvh_node* blas, uint32_t blas_count,
+                                      const lux_bvh_node* tlas, uint32_t tlas_count,
+                                      const float* triangles, uint32_t triangle_count,
+                                      const lux_instance_record* instances,
+                                      uint32_t instance_count) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!blas || !tlas || !triangles || !instances || !blas_count || !tlas_count ||
+        !triangle_count || !instance_count)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_built: empty or null array");
+    static_assert(sizeof(lux_instance_record) == 128, "instance record layout");
+    static_assert(sizeof(lux_bvh_node) == 32, "node layout");
+    LUX_HIP(ctx, hipSetDevice(ctx->device));
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc = lux_ensure(ctx, ctx->blas, 32ull * blas_count)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->tlas, 32ull * tlas_count)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->inst, 128ull * instance_count)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->triA, 16ull * triangle_count)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->triB, 16ull * triangle_count)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->triC, 16ull * triangle_count)) return rc;
+    LUX_HIP(ctx, hipMemcpy(ctx->blas.ptr, blas, 32ull * blas_count, hipMemcpyHostToDevice));
+    LUX_HIP(ctx, hipMemcpy(ctx->tlas.ptr, tlas, 32ull * tlas_count, hipMemcpyHostToDevice));
+    LUX_HIP(ctx, hipMemcpy(ctx->inst.ptr, instances, 128ull * instance_count, hipMemcpyHostToDevice));
+    // 12 floats per triangle -> three float4 arrays
+    LUX_HIP(ctx, hipMemcpy2D(ctx->triA.ptr, 16, triangles, 48, 16, triangle_count, hipMemcpyHostToDevice));
+    LUX_HIP(ctx, hipMemcpy2D(ctx->triB.ptr, 16, triangles + 4, 48, 16, triangle_count, hipMemcpyHostToDevice));
+    LUX_HIP(ctx, hipMemcpy2D(ctx->triC.ptr, 16, triangles + 8, 48, 16, triangle_count, hipMemcpyHostToDevice));
+    ctx->n_blas = blas_count;
+    ctx->n_tlas = tlas_count;
+    ctx->n_inst = instance_count;
+    ctx->n_tri = triangle_count;
+    ctx->n_meshes = 0;
+    ctx->mesh_tri_count.clear();
+    ctx->mesh_tri_base.clear();
+    ctx->mesh_blas_base.clear();
+    ctx->instances.clear();
+    ctx->blas_built = true;
+    ctx->tlas_built = true;
+    return LUX_OK;
+}

@@ -1,0 +1,602 @@
+// lux_device.h -- device-side geometry and two-level BVH traversal for gfx950.
+//
+// Semantics follow the reference's software traversal operation for
+// operation (shaders/traversal.slang:349-443 box/triangle tests, :474-566
+// traceInstance, :696-811 traceRayTracked, :826-1046 occlusion).  All
+// arithmetic is IEEE f32 with contraction disabled at compile time
+// (-ffp-contract=off) and correctly rounded division, so results are
+// bit-identical to a scalar CPU evaluation of the same expressions.
+//
+// Memory layout in HBM (structure of arrays, 16-byte loads):
+//   nodes      float4 pair per node: (min.xyz, first) (max.xyz, count)
+//   triangles  three float4 arrays A, B, C in BLAS leaf order;
+//              A.w = triangle index within its mesh, B.w = material index
+//   instances  8 float4 per instance in TLAS order: inv rows[3], fwd rows[3],
+//              (flags, blas_root, tri_first, tri_count), (orig, mesh, 0, 0)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lux {
+
+constexpr uint32_t kEmpty = 0xffffffffu;
+constexpr uint32_t kMaxSteps = 4096u;
+constexpr uint32_t kMaxLeafTriangles = 256u;
+constexpr uint32_t kMaxLeafInstances = 4096u;
+constexpr uint32_t kStackCap = 49u;
+constexpr uint32_t kErrBlasOverflow = 1u;
+constexpr uint32_t kErrTlasOverflow = 2u;
+constexpr uint32_t kErrUnsupported = 4u;
+constexpr uint32_t kHitOccluded = 0x100u;
+constexpr uint32_t kHitRoot = 0x200u;
+constexpr uint32_t kRayAnyHit = 1u;
+
+struct DevScene {
+    const float4* tlas;
+    const float4* inst;
+    const float4* blas;
+    const float4* triA;
+    const float4* triB;
+    const float4* triC;
+    uint32_t n_tlas, n_inst, n_blas, n_tri;
+};
+
+struct Node {
+    float3 mn;
+    uint32_t first;
+    float3 mx;
+    uint32_t count;
+};
+
+__device__ __forceinline__ Node load_node(const float4* base, uint32_t n, uint32_t idx) {
+    // Clamped read: a malformed index reads the last node instead of faulting
+    // (the reference clamps scene word reads, traversal.slang:114-118).
+    uint32_t i = idx < n ? idx : n - 1u;
+    float4 a = base[2u * i];
+    float4 b = base[2u * i + 1u];
+    Node r;
+    r.mn = make_float3(a.x, a.y, a.z);
+    r.first = __float_as_uint(a.w);
+    r.mx = make_float3(b.x, b.y, b.z);
+    r.count = __float_as_uint(b.w);
+    return r;
+}
+
+__device__ __forceinline__ bool isfinite_bits(float f) {
+    return (__float_as_uint(f) & 0x7f800000u) != 0x7f800000u;
+}
+
+// boxRayInterval (traversal.slang:349-369).
+__device__ __forceinline__ bool box_interval(float3 o, float3 d, float3 mn, float3 mx, float limit,
+                                             float& entry, float& exit_) {
+    float near_ = 0.0f, far_ = limit;
+    const float os[3] = {o.x, o.y, o.z}, ds[3] = {d.x, d.y, d.z};
+    const float lo[3] = {mn.x, mn.y, mn.z}, hi[3] = {mx.x, mx.y, mx.z};
+    bool ok = true;
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        if (ds[axis] == 0.0f) {
+            if (os[axis] < lo[axis] || os[axis] > hi[axis]) ok = false;
+        } else {
+            float a = (lo[axis] - os[axis]) / ds[axis];
+            float b = (hi[axis] - os[axis]) / ds[axis];
+            float ax = fmaxf(a, b);
+            near_ = fmaxf(near_, fminf(a, b));
+            float pad = fabsf(ax) * 3.5762791e-7f;
+            far_ = fminf(far_, ax + pad);
+        }
+    }
+    if (!ok) {
+        entry = 0.0f;
+        exit_ = 0.0f;
+        return false;
+    }
+    entry = near_;
+    exit_ = far_;
+    return near_ <= far_;
+}
+
+__device__ __forceinline__ float box_entry(float3 o, float3 d, const Node& n, float limit) {
+    float e, x;
+    return box_interval(o, d, n.mn, n.mx, limit, e, x) ? e : -1.0f;
+}
+
+// orientedEdge (traversal.slang:379-392).
+__device__ __forceinline__ float oriented_edge(float ax, float ay, float bx, float by) {
+    bool reverse = ax > bx || (ax == bx && ay > by);
+    float fx = reverse ? bx : ax, fy = reverse ? by : ay;
+    float sx = reverse ? ax : bx, sy = reverse ? ay : by;
+    float pa = fx * sy;
+    float pb = fy * sx;
+    float edge = pa - pb;
+    float err = (fabsf(pa) + fabsf(pb)) * 2.3841858e-7f;
+    if (fabsf(edge) <= err) edge = 0.0f;
+    return reverse ? -edge : edge;
+}
+
+struct TriRay {
+    int kx, ky, kz;
+    float sx, sy;
+    bool valid;
+};
+
+__device__ __forceinline__ float comp(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// triangleRay (traversal.slang:399-413).
+__device__ __forceinline__ TriRay make_tri_ray(float3 d) {
+    TriRay r;
+    r.kx = r.ky = 0;
+    r.sx = r.sy = 0.0f;
+    float m0 = fabsf(d.x), m1 = fabsf(d.y), m2 = fabsf(d.z);
+    r.kz = m0 > m1 ? (m0 > m2 ? 0 : 2) : (m1 > m2 ? 1 : 2);
+    float dz = comp(d, r.kz);
+    r.valid = dz != 0.0f;
+    if (!r.valid) return r;
+    r.kx = (r.kz + 1) % 3;
+    r.ky = (r.kx + 1) % 3;
+    if (dz < 0.0f) {
+        int s = r.kx;
+        r.kx = r.ky;
+        r.ky = s;
+    }
+    r.sx = comp(d, r.kx) / dz;
+    r.sy = comp(d, r.ky) / dz;
+    return r;
+}
+
+// intersectTriangle (traversal.slang:415-443).  Returns (t, w1, w2), t = -1 on miss.
+__device__ __forceinline__ float3 intersect_tri(float3 o, float3 d, const TriRay& r, float3 ta,
+                                                float3 tb, float3 tc) {
+    float3 miss = make_float3(-1.0f, 0.0f, 0.0f);
+    if (!r.valid) return miss;
+    float3 A = make_float3(ta.x - o.x, ta.y - o.y, ta.z - o.z);
+    float3 B = make_float3(tb.x - o.x, tb.y - o.y, tb.z - o.z);
+    float3 C = make_float3(tc.x - o.x, tc.y - o.y, tc.z - o.z);
+    float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+    float pax = comp(A, r.kx) - r.sx * Akz, pay = comp(A, r.ky) - r.sy * Akz;
+    float pbx = comp(B, r.kx) - r.sx * Bkz, pby = comp(B, r.ky) - r.sy * Bkz;
+    float pcx = comp(C, r.kx) - r.sx * Ckz, pcy = comp(C, r.ky) - r.sy * Ckz;
+    float largest = fmaxf(fmaxf(fmaxf(fabsf(pax), fabsf(pay)), fmaxf(fabsf(pbx), fabsf(pby))),
+                          fmaxf(fabsf(pcx), fabsf(pcy)));
+    if (largest == 0.0f || !isfinite_bits(largest)) return miss;
+    uint32_t exponent = (__float_as_uint(largest) >> 23) & 255u;
+    uint32_t ce = exponent < 1u ? 1u : (exponent > 253u ? 253u : exponent);
+    float scale = __uint_as_float((254u - ce) << 23);
+    pax *= scale; pay *= scale; pbx *= scale; pby *= scale; pcx *= scale; pcy *= scale;
+    float e0 = oriented_edge(pbx, pby, pcx, pcy);
+    float e1 = oriented_edge(pcx, pcy, pax, pay);
+    float e2 = oriented_edge(pax, pay, pbx, pby);
+    bool anyneg = e0 < 0.0f || e1 < 0.0f || e2 < 0.0f;
+    bool anypos = e0 > 0.0f || e1 > 0.0f || e2 > 0.0f;
+    if (anyneg && anypos) return miss;
+    float det = (e0 + e1) + e2;
+    if (det == 0.0f || !isfinite_bits(det)) return miss;
+    float w0 = e0 / det, w1 = e1 / det, w2 = e2 / det;
+    float dist = ((w0 * Akz + w1 * Bkz) + w2 * Ckz) / comp(d, r.kz);
+    if (!isfinite_bits(dist)) return miss;
+    return make_float3(dist, w1, w2);
+}
+
+__device__ __forceinline__ bool culled(float3 a, float3 b, float3 c, float3 d, uint32_t flags) {
+    if ((flags & 3u) == 0u) return false;
+    float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
+    float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
+    float cx = e1y * e2z - e1z * e2y;
+    float cy = e1z * e2x - e1x * e2z;
+    float cz = e1x * e2y - e1y * e2x;
+    float facing = ((cx * d.x + cy * d.y) + cz * d.z) * ((flags & 4u) != 0u ? -1.0f : 1.0f);
+    return ((flags & 1u) != 0u && facing >= 0.0f) || ((flags & 2u) != 0u && facing <= 0.0f);
+}
+
+__device__ __forceinline__ float3 xform_point(float4 r0, float4 r1, float4 r2, float3 p) {
+    return make_float3(((r0.x * p.x + r0.y * p.y) + r0.z * p.z) + r0.w * 1.0f,
+                       ((r1.x * p.x + r1.y * p.y) + r1.z * p.z) + r1.w * 1.0f,
+                       ((r2.x * p.x + r2.y * p.y) + r2.z * p.z) + r2.w * 1.0f);
+}
+__device__ __forceinline__ float3 xform_dir(float4 r0, float4 r1, float4 r2, float3 v) {
+    return make_float3(((r0.x * v.x + r0.y * v.y) + r0.z * v.z) + r0.w * 0.0f,
+                       ((r1.x * v.x + r1.y * v.y) + r1.z * v.z) + r1.w * 0.0f,
+                       ((r2.x * v.x + r2.y * v.y) + r2.z * v.z) + r2.w * 0.0f);
+}
+
+// Traversal work counters: NoStats compiles away; Counting tallies the node
+// fetches, triangle tests and instance visits that define the algorithmic
+// bytes of a ray (the oracle counts the same events).
+struct NoStats {
+    __device__ __forceinline__ void node(uint32_t) {}
+    __device__ __forceinline__ void tri() {}
+    __device__ __forceinline__ void inst() {}
+};
+struct Counting {
+    uint32_t nodes = 0, tris = 0, insts = 0;
+    __device__ __forceinline__ void node(uint32_t k) { nodes += k; }
+    __device__ __forceinline__ void tri() { tris += 1; }
+    __device__ __forceinline__ void inst() { insts += 1; }
+};
+
+struct HitState {
+    float t;
+    uint32_t tri, inst, err;
+    float b1, b2;
+};
+
+// Per-lane traversal stack: the first LdsDepth slots live in LDS
+// (lane-contiguous columns, so a wave's access is one conflict-free row),
+// deeper slots spill to a private array.  TLAS entries occupy the bottom of
+// the stack and the BLAS walk of the current instance stacks on top of them,
+// so the reference's two 49-entry stacks fit in one 98-slot stack.
+template <int LdsDepth, int Wg>
+struct LaneStack {
+    uint2* col;  // &lds[0][lane]
+    uint2 spill[2 * kStackCap - LdsDepth > 0 ? 2 * kStackCap - LdsDepth : 1];
+    __device__ __forceinline__ void put(uint32_t slot, uint2 v) {
+        if (slot < (uint32_t)LdsDepth) col[slot * Wg] = v;
+        else spill[slot - LdsDepth] = v;
+    }
+    __device__ __forceinline__ uint2 get(uint32_t slot) {
+        if (slot < (uint32_t)LdsDepth) return col[slot * Wg];
+        return spill[slot - LdsDepth];
+    }
+};
+
+// traceInstance, closest hit (traversal.slang:474-566).  Stack slots
+// [base, base+49) belong to this walk.
+template <class Stack, class St>
+__device__ __forceinline__ void trace_instance(const DevScene& s, float3 o, float3 d, uint32_t ii,
+                                               HitState& h, Stack& stk, uint32_t base, St& st) {
+    const float4* rec = s.inst + 8u * ii;
+    float4 i0 = rec[0], i1 = rec[1], i2 = rec[2];
+    uint4 g = *reinterpret_cast<const uint4*>(rec + 6);
+    uint32_t flags = g.x, root = g.y, gfirst = g.z, gcount = g.w;
+    float3 lo = xform_point(i0, i1, i2, o);
+    float3 ld = xform_dir(i0, i1, i2, d);
+    st.inst();
+    uint32_t count = 0;
+    Node node = load_node(s.blas, s.n_blas, root);
+    st.node(1);
+    float e, x;
+    bool ready = box_interval(lo, ld, node.mn, node.mx, h.t, e, x) && e >= 0.0f;
+    TriRay ray;
+    ray.valid = false;
+    ray.kx = ray.ky = ray.kz = 0;
+    ray.sx = ray.sy = 0.0f;
+    if (ready) ray = make_tri_ray(ld);
+    uint32_t steps = 0;
+    while (ready || count != 0) {
+        if (++steps > kMaxSteps) {
+            h.err |= kErrUnsupported;
+            break;
+        }
+        if (!ready) {
+            --count;
+            uint2 ent = stk.get(base + count);
+            if (__uint_as_float(ent.y) > h.t) continue;
+            node = load_node(s.blas, s.n_blas, ent.x);
+            st.node(1);
+        }
+        ready = false;
+        if (node.count == 0) {
+            if (count + 2 > kStackCap) {
+                h.err |= kErrBlasOverflow;
+                node.first = gfirst;
+                node.count = gcount;
+                count = 0;
+            } else {
+                uint32_t li = node.first, ri = li + 1u;
+                Node ln = load_node(s.blas, s.n_blas, li);
+                Node rn = load_node(s.blas, s.n_blas, ri);
+                st.node(2);
+                float le = box_entry(lo, ld, ln, h.t);
+                float re = box_entry(lo, ld, rn, h.t);
+                if (le >= 0.0f && re >= 0.0f) {
+                    if (le <= re) {
+                        stk.put(base + count, make_uint2(ri, __float_as_uint(re)));
+                        node = ln;
+                    } else {
+                        stk.put(base + count, make_uint2(li, __float_as_uint(le)));
+                        node = rn;
+                    }
+                    ++count;
+                    ready = true;
+                } else if (le >= 0.0f) {
+                    node = ln;
+                    ready = true;
+                } else if (re >= 0.0f) {
+                    node = rn;
+                    ready = true;
+                }
+                continue;
+            }
+        }
+        uint32_t tbegin = node.first > gfirst ? node.first : gfirst;
+        uint32_t tlimit = gfirst + gcount;
+        uint32_t req_end = node.first + node.count;
+        uint32_t tend = req_end < tlimit ? req_end : tlimit;
+        bool truncated = node.first < gfirst || req_end < node.first || req_end > tlimit ||
+                         node.count > kMaxLeafTriangles;
+        uint32_t bend = tbegin + kMaxLeafTriangles;
+        if (bend > tend) bend = tend;
+        truncated = truncated || bend < tend;
+        if (truncated) h.err |= kErrUnsupported;
+        for (uint32_t i = tbegin; i < bend; ++i) {
+            uint32_t ti = i < s.n_tri ? i : s.n_tri - 1u;
+            float4 A = s.triA[ti], B = s.triB[ti], C = s.triC[ti];
+            st.tri();
+            float3 a = make_float3(A.x, A.y, A.z), b = make_float3(B.x, B.y, B.z),
+                   c = make_float3(C.x, C.y, C.z);
+            if (culled(a, b, c, ld, flags)) continue;
+            float3 r3 = intersect_tri(lo, ld, ray, a, b, c);
+            if (r3.x > 0.000025f && r3.x < h.t) {
+                h.t = r3.x;
+                h.tri = i;
+                h.inst = ii;
+                h.b1 = r3.y;
+                h.b2 = r3.z;
+            }
+        }
+    }
+}
+
+// traceRayTracked without dependency or cone terms (traversal.slang:696-811).
+template <class Stack, class St>
+__device__ __forceinline__ void trace_closest(const DevScene& s, float3 o, float3 d, float tmax,
+                                              HitState& h, Stack& stk, St& st) {
+    h.t = tmax;
+    h.tri = kEmpty;
+    h.inst = kEmpty;
+    h.err = 0;
+    h.b1 = h.b2 = 0.0f;
+    if (s.n_inst == 0) return;
+    uint32_t count = 0;
+    Node node = load_node(s.tlas, s.n_tlas, 0);
+    st.node(1);
+    float re, rx;
+    if (!box_interval(o, d, node.mn, node.mx, h.t, re, rx)) return;
+    bool ready = re >= 0.0f;
+    uint32_t steps = 0, ninst = s.n_inst;
+    while (ready || count != 0) {
+        if (++steps > kMaxSteps) {
+            h.err |= kErrUnsupported;
+            break;
+        }
+        if (!ready) {
+            --count;
+            uint2 ent = stk.get(count);
+            if (__uint_as_float(ent.y) > h.t) continue;
+            node = load_node(s.tlas, s.n_tlas, ent.x);
+            st.node(1);
+        }
+        ready = false;
+        if (node.count == 0) {
+            if (count + 2 > kStackCap) {
+                h.err |= kErrTlasOverflow;
+                node.first = 0;
+                node.count = ninst;
+                count = 0;
+            } else {
+                uint32_t li = node.first, ri = li + 1u;
+                Node ln = load_node(s.tlas, s.n_tlas, li);
+                Node rn = load_node(s.tlas, s.n_tlas, ri);
+                st.node(2);
+                float le = box_entry(o, d, ln, h.t);
+                float rre = box_entry(o, d, rn, h.t);
+                if (le >= 0.0f && rre >= 0.0f) {
+                    if (le <= rre) {
+                        stk.put(count, make_uint2(ri, __float_as_uint(rre)));
+                        node = ln;
+                    } else {
+                        stk.put(count, make_uint2(li, __float_as_uint(le)));
+                        node = rn;
+                    }
+                    ++count;
+                    ready = true;
+                } else if (le >= 0.0f) {
+                    node = ln;
+                    ready = true;
+                } else if (rre >= 0.0f) {
+                    node = rn;
+                    ready = true;
+                }
+                continue;
+            }
+        }
+        uint32_t req_end = node.first + node.count;
+        bool truncated = req_end < node.first || node.first > ninst || req_end > ninst ||
+                         node.count > kMaxLeafInstances;
+        uint32_t ib = node.first < ninst ? node.first : ninst;
+        uint32_t ie = req_end < ninst ? req_end : ninst;
+        uint32_t be = ib + kMaxLeafInstances;
+        if (be > ie) be = ie;
+        truncated = truncated || be < ie;
+        if (truncated) h.err |= kErrUnsupported;
+        for (uint32_t i = ib; i < be; ++i) trace_instance(s, o, d, i, h, stk, count, st);
+    }
+}
+
+// occludedInstance (traversal.slang:845-932).
+template <class Stack, class St>
+__device__ __forceinline__ bool occluded_instance(const DevScene& s, float3 o, float3 d,
+                                                  uint32_t ii, float limit, bool& supported,
+                                                  HitState& h, Stack& stk, uint32_t base, St& st) {
+    supported = true;
+    const float4* rec = s.inst + 8u * ii;
+    float4 i0 = rec[0], i1 = rec[1], i2 = rec[2];
+    uint4 g = *reinterpret_cast<const uint4*>(rec + 6);
+    uint32_t flags = g.x, root = g.y, gfirst = g.z, gcount = g.w;
+    float3 lo = xform_point(i0, i1, i2, o);
+    float3 ld = xform_dir(i0, i1, i2, d);
+    st.inst();
+    uint32_t count = 0;
+    Node node = load_node(s.blas, s.n_blas, root);
+    st.node(1);
+    float e, x;
+    bool ready = box_interval(lo, ld, node.mn, node.mx, limit, e, x) && e >= 0.0f;
+    TriRay ray;
+    ray.valid = false;
+    ray.kx = ray.ky = ray.kz = 0;
+    ray.sx = ray.sy = 0.0f;
+    if (ready) ray = make_tri_ray(ld);
+    uint32_t steps = 0;
+    bool incomplete = false;
+    while (ready || count != 0) {
+        if (++steps > kMaxSteps) {
+            supported = false;
+            return false;
+        }
+        if (!ready) {
+            --count;
+            node = load_node(s.blas, s.n_blas, stk.get(base + count).x);
+            st.node(1);
+        }
+        ready = false;
+        if (node.count == 0) {
+            if (count + 2 > kStackCap) {
+                node.first = gfirst;
+                node.count = gcount;
+                count = 0;
+            } else {
+                uint32_t li = node.first, ri = li + 1u;
+                Node ln = load_node(s.blas, s.n_blas, li);
+                Node rn = load_node(s.blas, s.n_blas, ri);
+                st.node(2);
+                float le = box_entry(lo, ld, ln, limit);
+                float re = box_entry(lo, ld, rn, limit);
+                if (le >= 0.0f && re >= 0.0f) {
+                    if (le <= re) {
+                        stk.put(base + count, make_uint2(ri, 0u));
+                        node = ln;
+                    } else {
+                        stk.put(base + count, make_uint2(li, 0u));
+                        node = rn;
+                    }
+                    ++count;
+                    ready = true;
+                } else if (le >= 0.0f) {
+                    node = ln;
+                    ready = true;
+                } else if (re >= 0.0f) {
+                    node = rn;
+                    ready = true;
+                }
+                continue;
+            }
+        }
+        uint32_t tbegin = node.first > gfirst ? node.first : gfirst;
+        uint32_t tlimit = gfirst + gcount;
+        uint32_t req_end = node.first + node.count;
+        uint32_t tend = req_end < tlimit ? req_end : tlimit;
+        bool truncated = node.first < gfirst || req_end < node.first || req_end > tlimit ||
+                         node.count > kMaxLeafTriangles;
+        uint32_t bend = tbegin + kMaxLeafTriangles;
+        if (bend > tend) bend = tend;
+        truncated = truncated || bend < tend;
+        incomplete = incomplete || truncated;
+        for (uint32_t i = tbegin; i < bend; ++i) {
+            uint32_t ti = i < s.n_tri ? i : s.n_tri - 1u;
+            float4 A = s.triA[ti], B = s.triB[ti], C = s.triC[ti];
+            st.tri();
+            float3 a = make_float3(A.x, A.y, A.z), b = make_float3(B.x, B.y, B.z),
+                   c = make_float3(C.x, C.y, C.z);
+            if (culled(a, b, c, ld, flags)) continue;
+            float3 r3 = intersect_tri(lo, ld, ray, a, b, c);
+            if (r3.x > 0.000025f && r3.x < limit) {
+                h.t = r3.x;
+                h.tri = i;
+                h.inst = ii;
+                h.b1 = r3.y;
+                h.b2 = r3.z;
+                return true;
+            }
+        }
+    }
+    supported = !incomplete;
+    return false;
+}
+
+// traceOccluded (traversal.slang:955-1046).  Result in h.err bits:
+// kHitOccluded when blocked, kHitRoot when the root box was met,
+// kErrUnsupported when the query could not be completed.
+template <class Stack, class St>
+__device__ __forceinline__ void trace_occluded(const DevScene& s, float3 o, float3 d, float limit,
+                                               HitState& h, Stack& stk, St& st) {
+    h.t = limit;
+    h.tri = kEmpty;
+    h.inst = kEmpty;
+    h.err = 0;
+    h.b1 = h.b2 = 0.0f;
+    if (s.n_inst == 0) return;
+    uint32_t count = 0;
+    Node node = load_node(s.tlas, s.n_tlas, 0);
+    st.node(1);
+    float re, rx;
+    bool root = box_interval(o, d, node.mn, node.mx, limit, re, rx);
+    if (root) h.err |= kHitRoot;
+    bool ready = root && re >= 0.0f;
+    if (!ready) return;
+    uint32_t steps = 0, ninst = s.n_inst;
+    bool incomplete = false;
+    while (ready || count != 0) {
+        if (++steps > kMaxSteps) {
+            h.err |= kErrUnsupported;
+            return;
+        }
+        if (!ready) {
+            --count;
+            node = load_node(s.tlas, s.n_tlas, stk.get(count).x);
+            st.node(1);
+        }
+        ready = false;
+        if (node.count == 0) {
+            if (count + 2 > kStackCap) {
+                node.first = 0;
+                node.count = ninst;
+                count = 0;
+            } else {
+                uint32_t li = node.first, ri = li + 1u;
+                Node ln = load_node(s.tlas, s.n_tlas, li);
+                Node rn = load_node(s.tlas, s.n_tlas, ri);
+                st.node(2);
+                float le = box_entry(o, d, ln, limit);
+                float rre = box_entry(o, d, rn, limit);
+                if (le >= 0.0f && rre >= 0.0f) {
+                    if (le <= rre) {
+                        stk.put(count, make_uint2(ri, 0u));
+                        node = ln;
+                    } else {
+                        stk.put(count, make_uint2(li, 0u));
+                        node = rn;
+                    }
+                    ++count;
+                    ready = true;
+                } else if (le >= 0.0f) {
+                    node = ln;
+                    ready = true;
+                } else if (rre >= 0.0f) {
+                    node = rn;
+                    ready = true;
+                }
+                continue;
+            }
+        }
+        uint32_t req_end = node.first + node.count;
+        bool truncated = req_end < node.first || node.first > ninst || req_end > ninst ||
+                         node.count > kMaxLeafInstances;
+        uint32_t ib = node.first < ninst ? node.first : ninst;
+        uint32_t ie = req_end < ninst ? req_end : ninst;
+        uint32_t be = ib + kMaxLeafInstances;
+        if (be > ie) be = ie;
+        incomplete = incomplete || truncated || be < ie;
+        for (uint32_t i = ib; i < be; ++i) {
+            bool sup;
+            bool blocked = occluded_instance(s, o, d, i, limit, sup, h, stk, count, st);
+            incomplete = incomplete || !sup;
+            if (blocked) {
+                h.err |= kHitOccluded;
+                return;
+            }
+        }
+    }
+    if (incomplete) h.err |= kErrUnsupported;
+}
+
+}  // namespace lux

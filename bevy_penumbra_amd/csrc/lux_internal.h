@@ -1,0 +1,73 @@
+// lux_internal.h -- host-side context shared by the liblux_hip translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/lux.h"
+#include "lux_device.h"
+
+struct lux_buffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct lux_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string error;
+    int cu_count = 256;
+
+    // scene input (device copies of caller arrays)
+    std::vector<uint32_t> mesh_tri_count, mesh_tri_base, mesh_blas_base;
+    uint32_t n_meshes = 0, n_inst = 0, n_tri = 0, n_materials = 0;
+    std::vector<lux_instance_desc> instances;
+    lux_buffer raw_pos;   // float[9*n_tri], input order, meshes concatenated
+    lux_buffer raw_mat;   // uint32[n_tri]
+    lux_buffer inst_desc; // float[24*n_inst] (inv, fwd) + uint32 mesh/flags
+    lux_buffer materials; // lux_material[n_materials]
+
+    // built scene
+    bool blas_built = false, tlas_built = false;
+    uint32_t n_blas = 0, n_tlas = 0;
+    lux_buffer triA, triB, triC;  // float4[n_tri]
+    lux_buffer tri_order;         // uint32[n_tri] sorted -> input triangle (global)
+    lux_buffer blas;              // float4[2*n_blas]
+    lux_buffer tlas;              // float4[2*n_tlas]
+    lux_buffer inst;              // float4[8*n_inst]
+    lux_buffer inst_order;        // uint32[n_inst]
+
+    // scratch
+    lux_buffer scratch;
+    lux_buffer ray_buf, hit_buf;
+
+    // trace tuning
+    uint32_t waves_per_cu = 0, regroup = 0;
+    float last_ms = 0.0f;
+
+    lux::DevScene dev_scene() const;
+};
+
+int lux_fail(lux_ctx* ctx, int code, const std::string& msg);
+int lux_hip_check(lux_ctx* ctx, hipError_t e, const char* what);
+int lux_ensure(lux_ctx* ctx, lux_buffer& b, size_t bytes);
+void lux_release(lux_buffer& b);
+
+#define LUX_HIP(ctx, call)                                         \
+    do {                                                           \
+        hipError_t _e = (call);                                    \
+        if (_e != hipSuccess) return lux_hip_check((ctx), _e, #call); \
+    } while (0)
+
+// LBVH over primitive boxes (device lo/hi float4[n]).  Writes nodes
+// (float4[2*(2n-1)] or 2 for n<=leaf_max... always 2n-1 slots) and perm.
+// node_first_offset is added to internal 'first', leaf_first_offset to
+// leaf 'first'.
+int lux_lbvh_build_device(lux_ctx* ctx, const float4* lo, const float4* hi, uint32_t n,
+                          uint32_t leaf_max, float4* nodes, uint32_t* perm,
+                          uint32_t node_first_offset, uint32_t leaf_first_offset);
+int lux_sort_pairs_impl(lux_ctx* ctx, const uint32_t* keys_in, uint32_t n, uint32_t* keys_out,
+                        uint32_t* vals_out, uint32_t* tmp_keys, uint32_t* tmp_vals,
+                        uint32_t* hist);
+size_t lux_sort_hist_words(uint32_t n);

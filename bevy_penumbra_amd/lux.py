@@ -1,0 +1,236 @@
+"""ctypes binding of liblux_hip.so (the C ABI declared in include/lux.h).
+
+The library must be built (``python -m bevy_penumbra_amd.build``); loading
+fails loudly when it is missing -- there is no CPU fallback for any entry
+point.  Device buffers are plain integers (addresses); torch tensors are
+convenient owners of device memory and are passed as ``t.data_ptr()``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "liblux_hip.so"
+
+LUX_OK = 0
+LUX_HIT_BLAS_OVERFLOW = 1
+LUX_HIT_TLAS_OVERFLOW = 2
+LUX_HIT_UNSUPPORTED = 4
+LUX_HIT_OCCLUDED = 0x100
+LUX_HIT_ROOT = 0x200
+LUX_RAY_ANY_HIT = 1
+LUX_EMPTY = 0xFFFFFFFF
+LUX_BVH_MORTON30 = 0
+
+
+class LuxError(RuntimeError):
+    pass
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("materials", C.c_void_p), ("triangle_count", C.c_uint32)]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [
+        ("local_from_world", C.c_float * 12),
+        ("world_from_local", C.c_float * 12),
+        ("mesh", C.c_uint32),
+        ("flags", C.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Load liblux_hip.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise LuxError(
+            f"{LIB_PATH} is missing: build it with `python -m bevy_penumbra_amd.build` "
+            "(the HIP path has no fallback)"
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int
+    sig = {
+        "lux_ctx_create": (i32, [i32, C.POINTER(vp)]),
+        "lux_ctx_destroy": (None, [vp]),
+        "lux_last_error": (C.c_char_p, [vp]),
+        "lux_ctx_stream": (vp, [vp]),
+        "lux_ctx_synchronize": (i32, [vp]),
+        "lux_scene_upload_soa": (i32, [vp, vp, u32, vp, u32, vp, u32]),
+        "lux_scene_set_transforms": (i32, [vp, vp, vp, u32]),
+        "lux_scene_upload_built": (i32, [vp, vp, u32, vp, u32, vp, u32, vp, u32]),
+        "lux_bvh_build": (i32, [vp, u32]),
+        "lux_bvh_info": (i32, [vp, vp, vp, vp, vp]),
+        "lux_bvh_download": (i32, [vp, vp, vp, vp, vp]),
+        "lux_sort_pairs_device": (i32, [vp, vp, u32, vp, vp]),
+        "lux_morton_codes_device": (i32, [vp, vp, vp, u32, vp]),
+        "lux_trace_rays": (i32, [vp, vp, u32, vp]),
+        "lux_trace_rays_device": (i32, [vp, vp, u32, vp, vp]),
+        "lux_trace_stats_device": (i32, [vp, vp, u32, vp, vp]),
+        "lux_last_kernel_ms": (i32, [vp, C.POINTER(C.c_float)]),
+        "lux_set_trace_tuning": (i32, [vp, u32, u32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a) -> int | None:
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"], "arrays passed to lux must be C-contiguous"
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    raise TypeError(type(a))
+
+
+class Context:
+    """One lux context (device, stream, scene, BVH)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.lux_ctx_create(device, C.byref(h))
+        if rc != LUX_OK:
+            raise LuxError(f"lux_ctx_create({device}) failed with {rc}")
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lux_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc != LUX_OK:
+            msg = self.lib.lux_last_error(self.h)
+            raise LuxError(f"{what}: {rc} ({msg.decode() if msg else ''})")
+
+    @property
+    def stream(self) -> int:
+        return self.lib.lux_ctx_stream(self.h) or 0
+
+    def synchronize(self):
+        self._check(self.lib.lux_ctx_synchronize(self.h), "lux_ctx_synchronize")
+
+    # ---- scene ----
+    def upload_scene(self, scene) -> None:
+        meshes = (MeshDesc * max(1, len(scene.meshes)))()
+        keep = []
+        for i, (pos, mat) in enumerate(zip(scene.meshes, scene.mesh_materials)):
+            pos = np.ascontiguousarray(pos, dtype=np.float32)
+            mat = np.ascontiguousarray(mat, dtype=np.uint32)
+            keep += [pos, mat]
+            meshes[i].positions = pos.ctypes.data
+            meshes[i].materials = mat.ctypes.data
+            meshes[i].triangle_count = pos.shape[0]
+        ni = scene.inst_mesh.shape[0]
+        insts = (InstanceDesc * max(1, ni))()
+        for i in range(ni):
+            insts[i].local_from_world[:] = [float(x) for x in scene.inst_inv[i]]
+            insts[i].world_from_local[:] = [float(x) for x in scene.inst_fwd[i]]
+            insts[i].mesh = int(scene.inst_mesh[i])
+            insts[i].flags = int(scene.inst_flags[i])
+        mats = np.ascontiguousarray(scene.materials, dtype=np.float32)
+        rc = self.lib.lux_scene_upload_soa(
+            self.h, C.addressof(meshes), len(scene.meshes), C.addressof(insts), ni,
+            mats.ctypes.data, mats.shape[0])
+        self._check(rc, "lux_scene_upload_soa")
+
+    def set_transforms(self, inv: np.ndarray, fwd: np.ndarray) -> None:
+        inv = np.ascontiguousarray(inv, dtype=np.float32)
+        fwd = np.ascontiguousarray(fwd, dtype=np.float32)
+        self._check(self.lib.lux_scene_set_transforms(self.h, inv.ctypes.data, fwd.ctypes.data,
+                                                       inv.shape[0]), "lux_scene_set_transforms")
+
+    def upload_built(self, blas: np.ndarray, tlas: np.ndarray, tri: np.ndarray, inst: np.ndarray):
+        """Install a built scene: nodes (N,8) u32 words, tri (T,12) f32, inst (I,32) u32 words."""
+        arrs = [np.ascontiguousarray(a) for a in (blas, tlas, tri, inst)]
+        b, t, tr, ins = arrs
+        rc = self.lib.lux_scene_upload_built(self.h, b.ctypes.data, b.shape[0], t.ctypes.data,
+                                             t.shape[0], tr.ctypes.data, tr.shape[0],
+                                             ins.ctypes.data, ins.shape[0])
+        self._check(rc, "lux_scene_upload_built")
+
+    def build_bvh(self, mode: int = LUX_BVH_MORTON30) -> None:
+        self._check(self.lib.lux_bvh_build(self.h, mode), "lux_bvh_build")
+
+    def bvh_info(self) -> dict:
+        v = [C.c_uint32() for _ in range(4)]
+        self._check(self.lib.lux_bvh_info(self.h, *[C.byref(x) for x in v]), "lux_bvh_info")
+        return dict(blas_nodes=v[0].value, tlas_nodes=v[1].value, triangles=v[2].value,
+                    instances=v[3].value)
+
+    def download_bvh(self) -> dict:
+        info = self.bvh_info()
+        blas = np.zeros((info["blas_nodes"], 8), np.uint32)
+        tlas = np.zeros((max(1, info["tlas_nodes"]), 8), np.uint32)
+        tri_order = np.zeros(max(1, info["triangles"]), np.uint32)
+        inst_order = np.zeros(max(1, info["instances"]), np.uint32)
+        self._check(self.lib.lux_bvh_download(self.h, blas.ctypes.data, tlas.ctypes.data,
+                                              tri_order.ctypes.data, inst_order.ctypes.data),
+                    "lux_bvh_download")
+        return dict(blas=blas, tlas=tlas, tri_order=tri_order[: info["triangles"]],
+                    inst_order=inst_order[: info["instances"]])
+
+    # ---- primitives ----
+    def sort_pairs_device(self, keys_in, n, keys_out, index_out):
+        self._check(self.lib.lux_sort_pairs_device(self.h, _ptr(keys_in), n, _ptr(keys_out),
+                                                   _ptr(index_out)), "lux_sort_pairs_device")
+
+    def morton_codes_device(self, lo4, hi4, n, codes):
+        self._check(self.lib.lux_morton_codes_device(self.h, _ptr(lo4), _ptr(hi4), n, _ptr(codes)),
+                    "lux_morton_codes_device")
+
+    # ---- traversal ----
+    def trace(self, rays: np.ndarray) -> np.ndarray:
+        """Host rays (N,8) float32 -> hits (N,4) uint32 (t bits, tri, inst, flags)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), np.uint32)
+        self._check(self.lib.lux_trace_rays(self.h, rays.ctypes.data, n, hits.ctypes.data),
+                    "lux_trace_rays")
+        return hits
+
+    def trace_device(self, rays, n: int, hits, bary=None):
+        self._check(self.lib.lux_trace_rays_device(self.h, _ptr(rays), n, _ptr(hits), _ptr(bary)),
+                    "lux_trace_rays_device")
+
+    def trace_stats_device(self, rays, n: int, hits, counters):
+        self._check(self.lib.lux_trace_stats_device(self.h, _ptr(rays), n, _ptr(hits),
+                                                    _ptr(counters)), "lux_trace_stats_device")
+
+    def last_kernel_ms(self) -> float:
+        v = C.c_float()
+        self._check(self.lib.lux_last_kernel_ms(self.h, C.byref(v)), "lux_last_kernel_ms")
+        return float(v.value)
+
+    def set_trace_tuning(self, waves_per_cu: int = 0, regroup: int = 0):
+        self._check(self.lib.lux_set_trace_tuning(self.h, waves_per_cu, regroup),
+                    "lux_set_trace_tuning")

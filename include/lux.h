@@ -1,0 +1,180 @@
+/*
+ * lux.h -- C ABI of the MI355X transport hot path (liblux_hip.so).
+ *
+ * This is the boundary a Rust FFI crate (lux-hip-sys) binds with
+ * `extern "C"`.  Every entry point takes plain pointers and sizes; no
+ * PyTorch or HIP types cross it (streams are opaque void*).
+ *
+ * It replaces, inside the reference's render() Split branch
+ * (src/render.rs:9553-9801), the sequence of wgpu compute dispatches that
+ * trace, reduce, merge, pack and resolve radiance intervals, and the CPU BVH
+ * build of src/scene.rs:2374-2554 / :2014-2095.  The Bevy-facing types
+ * (RadianceCascadesPlugin, RadianceCascades, GiMethod,
+ * RadianceCascadesGeometry; src/lib.rs:305-307,431-518,631-669,795-900) are
+ * unchanged and sit above this ABI.
+ *
+ * Status codes: 0 = OK, negative = error (lux_last_error has the text).
+ * A context is thread-compatible: one thread at a time; several contexts may
+ * coexist, each with its own stream.
+ */
+#ifndef LUX_H
+#define LUX_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LUX_OK 0
+#define LUX_ERR_INVALID -1
+#define LUX_ERR_OOM -2
+#define LUX_ERR_HIP -3
+#define LUX_ERR_STATE -4
+
+/* Per-ray result flags.  Bits 1, 2 and 4 are the reference's Hit.errors bits
+ * (shaders/traversal.slang:12-16,500,515,743). */
+#define LUX_HIT_BLAS_OVERFLOW 1u
+#define LUX_HIT_TLAS_OVERFLOW 2u
+#define LUX_HIT_UNSUPPORTED 4u
+#define LUX_HIT_OCCLUDED 0x100u /* any-hit: an accepted blocker was found */
+#define LUX_HIT_ROOT 0x200u     /* any-hit: the ray met the scene root box */
+
+#define LUX_RAY_ANY_HIT 1u /* ray.flags: traceOccluded semantics instead of closest hit */
+
+#define LUX_EMPTY 0xffffffffu
+
+/* 32 B ray: origin + tmax, direction + flags. */
+typedef struct {
+    float ox, oy, oz, tmax;
+    float dx, dy, dz;
+    uint32_t flags;
+} lux_ray;
+
+/* 16 B hit: closest (or first accepted, any-hit) distance, the triangle's
+ * index within its mesh, the caller's instance index, flags. */
+typedef struct {
+    float t;
+    uint32_t triangle;
+    uint32_t instance;
+    uint32_t flags;
+} lux_hit;
+
+/* Reference BvhNode layout (src/scene.rs:76-83), 32 B. */
+typedef struct {
+    float mn[3];
+    uint32_t first;
+    float mx[3];
+    uint32_t count;
+} lux_bvh_node;
+
+/* One mesh of non-indexed triangles: positions[9*i .. 9*i+8] = a, b, c. */
+typedef struct {
+    const float* positions;
+    const uint32_t* materials; /* per triangle material index, or NULL = 0 */
+    uint32_t triangle_count;
+} lux_mesh_desc;
+
+/* One rigid instance.  Both transforms are row-major 3x4 and supplied by
+ * the caller, as the reference's instance record stores both
+ * (src/scene.rs:1071-1122).  flags: 1 cull back, 2 cull front, 4 flip
+ * winding, 16 unlit (reference instance flags). */
+typedef struct {
+    float local_from_world[12];
+    float world_from_local[12];
+    uint32_t mesh;
+    uint32_t flags;
+} lux_instance_desc;
+
+/* Constant material (configs use untextured, opaque materials). */
+typedef struct {
+    float albedo[3];
+    float roughness;
+    float emission[3];
+    float pad;
+} lux_material;
+
+typedef struct lux_ctx lux_ctx;
+
+/* ---- context (replaces render::install's device/pipeline set-up,
+ *      src/render.rs:1641-1760) ---- */
+int lux_ctx_create(int device, lux_ctx** out);
+void lux_ctx_destroy(lux_ctx* ctx);
+const char* lux_last_error(const lux_ctx* ctx);
+/* The context's HIP stream as an opaque handle (hipStream_t). */
+void* lux_ctx_stream(lux_ctx* ctx);
+int lux_ctx_synchronize(lux_ctx* ctx);
+
+/* ---- scene (replaces scene.rs triangle/instance packing and uploads,
+ *      src/scene.rs:809-823,1025-1122,2096-2117) ---- */
+int lux_scene_upload_soa(lux_ctx* ctx, const lux_mesh_desc* meshes, uint32_t mesh_count,
+                         const lux_instance_desc* instances, uint32_t instance_count,
+                         const lux_material* materials, uint32_t material_count);
+/* Replace instance transforms (same count/meshes); the next lux_bvh_build
+ * rebuilds only the TLAS when the BLASes are already built. */
+int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_world,
+                             const float* world_from_local, uint32_t instance_count);
+
+/* ---- BVH build (replaces build_bvh/build_tree, src/scene.rs:2374-2554) ---- */
+#define LUX_BVH_MORTON30 0u
+int lux_bvh_build(lux_ctx* ctx, uint32_t mode);
+/* 128 B instance record of a built scene, in TLAS order (the reference's
+ * 9-word record, src/scene.rs:1071-1122, minus layer/material words). */
+typedef struct {
+    float local_from_world[12];
+    float world_from_local[12];
+    uint32_t flags, blas_root, tri_first, tri_count;
+    uint32_t orig_index, mesh, reserved0, reserved1;
+} lux_instance_record;
+
+/* Install an already-built scene (packed-scene interchange): BLAS/TLAS nodes
+ * in the reference layout, triangles as 12 floats each (A.xyz, index within
+ * mesh as uint bits; B.xyz, material as uint bits; C.xyz, 0), instance
+ * records in TLAS order.  Node and triangle indices are absolute. */
+int lux_scene_upload_built(lux_ctx* ctx, const lux_bvh_node* blas, uint32_t blas_count,
+                           const lux_bvh_node* tlas, uint32_t tlas_count, const float* triangles,
+                           uint32_t triangle_count, const lux_instance_record* instances,
+                           uint32_t instance_count);
+
+/* Node/instance counts of the built scene. */
+int lux_bvh_info(lux_ctx* ctx, uint32_t* blas_nodes, uint32_t* tlas_nodes,
+                 uint32_t* triangles, uint32_t* instances);
+/* Copy the built BLAS/TLAS nodes, the sorted-triangle permutation and the
+ * TLAS instance order to host memory (for parity checks and interchange). */
+int lux_bvh_download(lux_ctx* ctx, lux_bvh_node* blas, lux_bvh_node* tlas, uint32_t* tri_order,
+                     uint32_t* inst_order);
+
+/* Stable (key, index) sort of 32-bit keys on the GPU: the gpuSortLess order
+ * contract (shaders/split.slang:3-10).  Device pointers. */
+int lux_sort_pairs_device(lux_ctx* ctx, const uint32_t* keys_in, uint32_t n, uint32_t* keys_out,
+                          uint32_t* index_out);
+/* Morton-30 codes of primitive boxes (device pointers, lo/hi float4). */
+int lux_morton_codes_device(lux_ctx* ctx, const float* lo4, const float* hi4, uint32_t n,
+                            uint32_t* codes);
+
+/* ---- traversal (replaces traceRayTracked/traceOccluded dispatches,
+ *      shaders/traversal.slang:696-1046; split.slang trace/traceVisibility) ---- */
+/* Host buffers: copies in, traces, copies out, synchronizes. */
+int lux_trace_rays(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits);
+/* Device buffers, asynchronous on the context stream. bary may be NULL
+ * (else 2 floats per ray: w1, w2). */
+int lux_trace_rays_device(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits,
+                          float* bary);
+
+/* Instrumented trace (same results): counters[0] += node fetches,
+ * counters[1] += triangle tests, counters[2] += instance visits, summed over
+ * all rays.  counters is a device pointer to 3 uint64 (zeroed by caller). */
+int lux_trace_stats_device(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits,
+                           unsigned long long* counters);
+
+/* Kernel timing of the last device trace (HIP events on the context stream). */
+int lux_last_kernel_ms(lux_ctx* ctx, float* ms);
+
+/* Launch tuning (autotuner output): persistent-grid waves per CU, refill
+ * threshold (idle lanes before a wave fetches new rays), 0 = defaults. */
+int lux_set_trace_tuning(lux_ctx* ctx, uint32_t waves_per_cu, uint32_t regroup_threshold);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,147 @@
+/*
+ * lux_oracle.h -- CPU oracle for the transport hot path (TEST INFRASTRUCTURE).
+ *
+ * This library is the checker, never the product: only tests/, the smoke
+ * check in __graft_entry__.py and the cpu_baseline leg of bench.py may load
+ * it.  It restates, in plain C with IEEE f32 arithmetic and no floating-point
+ * contraction (-ffp-contract=off), the algorithms of the reference's Split
+ * transport path so that the HIP kernels can be compared bit for bit:
+ *
+ *   traversal      shaders/traversal.slang:349-443 (box/triangle tests),
+ *                  :474-566 (traceInstance), :696-811 (traceRayTracked),
+ *                  :826-1046 (occludedTriangle/occludedInstance/traceOccluded)
+ *   SAH build      src/scene.rs:2374-2554 (build_bvh/build_tree/tree_cost)
+ *   Morton LBVH    new (the reference has no Morton build); order contract is
+ *                  gpuSortLess, shaders/split.slang:3-10
+ *   linked sort    shaders/linked_merge_sort.slang:3-37
+ *   brute force    src/traversal_tests.rs:978-1038 (f64 Moller-Trumbore)
+ *   transport      shaders/split.slang, shaders/cascades.slang:60-268,
+ *                  shaders/world_resolve.slang (see orc_transport.c)
+ *
+ * Parity pins (what each function is checked against) are listed per file.
+ */
+#ifndef LUX_ORACLE_H
+#define LUX_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_EMPTY 0xffffffffu
+#define ORC_ERR_BLAS_OVERFLOW 1u
+#define ORC_ERR_TLAS_OVERFLOW 2u
+#define ORC_ERR_UNSUPPORTED 4u
+#define ORC_HIT_OCCLUDED 0x100u
+#define ORC_HIT_ROOT 0x200u
+#define ORC_RAY_ANY_HIT 1u
+
+/* 32-byte BVH node, reference layout (src/scene.rs:76-83): count > 0 is a
+ * leaf over [first, first+count); count == 0 has children first, first+1. */
+typedef struct {
+    float mn[3];
+    uint32_t first;
+    float mx[3];
+    uint32_t count;
+} orc_node;
+
+/* 128-byte instance record in TLAS order.  inv/fwd are row-major 3x4
+ * (local_from_world rows, world_from_local rows) as in the reference's
+ * 9-word instance record (src/scene.rs:1071-1122). */
+typedef struct {
+    float inv[12];
+    float fwd[12];
+    uint32_t flags, root, first, count;
+    uint32_t orig, mesh, pad0, pad1;
+} orc_instance;
+
+typedef struct {
+    const orc_node* tlas;
+    uint32_t n_tlas;
+    const orc_instance* inst;
+    uint32_t n_inst;
+    const orc_node* blas;
+    uint32_t n_blas;
+    const float* tri; /* 12 floats per triangle: A.xyzw B.xyzw C.xyzw */
+    uint32_t n_tri;
+} orc_scene;
+
+typedef struct {
+    float ox, oy, oz, tmax;
+    float dx, dy, dz;
+    uint32_t flags;
+} orc_ray;
+
+typedef struct {
+    float t;
+    uint32_t triangle;
+    uint32_t instance;
+    uint32_t flags;
+} orc_hit;
+
+typedef struct {
+    uint64_t node_fetches;
+    uint64_t tri_tests;
+    uint64_t inst_visits;
+} orc_stats;
+
+/* ---- geometric primitives (bit-exact restatements) ---- */
+int orc_box_interval(const float o[3], const float d[3], const float mn[3], const float mx[3],
+                     float limit, float* entry, float* exit_);
+/* out = (t, w1, w2); t = -1 on miss */
+void orc_intersect_triangle(const float o[3], const float d[3], const float a[3], const float b[3],
+                            const float c[3], float out[3]);
+
+/* ---- traversal ---- */
+void orc_trace(const orc_scene* s, const orc_ray* rays, uint32_t n, orc_hit* hits,
+               float* bary /* nullable, 2 per ray */, orc_stats* stats /* nullable */,
+               int threads);
+
+/* ---- LBVH (Morton-30) build, restated exactly as the GPU builds it ---- */
+/* boxes: lo[4*n], hi[4*n] (xyz used).  Outputs: codes[n] (unsorted Morton),
+ * perm[n] (sorted order), nodes[max(1,2n-1)].  Returns node count. */
+uint32_t orc_lbvh_build(const float* lo, const float* hi, uint32_t n, uint32_t leaf_max,
+                        uint32_t* codes, uint32_t* perm, orc_node* nodes);
+void orc_morton_codes(const float* lo, const float* hi, uint32_t n, uint32_t* codes);
+void orc_stable_sort_pairs(const uint32_t* keys, uint32_t n, uint32_t* perm);
+
+/* Whole-scene build mirroring lux_bvh_build: meshes of non-indexed
+ * triangles (9 floats each), instances with inv/fwd transforms. */
+typedef struct {
+    uint32_t n_tri;
+    uint32_t n_blas;
+    uint32_t n_tlas;
+    uint32_t n_inst;
+} orc_scene_sizes;
+int orc_scene_build(const float* const* mesh_pos, const uint32_t* const* mesh_mat,
+                    const uint32_t* mesh_tri_count, uint32_t n_meshes, const float* inst_inv,
+                    const float* inst_fwd, const uint32_t* inst_mesh, const uint32_t* inst_flags,
+                    uint32_t n_inst, uint32_t leaf_max, int sah,
+                    /* outputs, caller-allocated with capacities from orc_scene_capacity */
+                    float* tri, orc_node* blas, orc_node* tlas, orc_instance* inst,
+                    orc_scene_sizes* sizes);
+void orc_scene_capacity(const uint32_t* mesh_tri_count, uint32_t n_meshes, uint32_t n_inst,
+                        uint32_t* tri_cap, uint32_t* blas_cap, uint32_t* tlas_cap);
+
+/* ---- binned SAH build_tree restatement (src/scene.rs:2398-2554) ---- */
+/* Reorders perm in place; returns node count (<= 2n-1). */
+uint32_t orc_sah_build(const float* lo, const float* hi, uint32_t n, uint32_t* perm,
+                       orc_node* nodes);
+double orc_tree_cost(const orc_node* nodes, uint32_t n_nodes);
+uint32_t orc_tree_depth(const orc_node* nodes, uint32_t n_nodes);
+
+/* ---- f64 brute force, world space (src/traversal_tests.rs:978-1038) ---- */
+/* Writes up to cap candidate hits sorted by t: (t, global triangle slot,
+ * TLAS-order instance).  Returns the number of candidates. */
+uint32_t orc_brute_force(const orc_scene* s, const orc_ray* ray, double* ts, uint32_t* tris,
+                         uint32_t* insts, uint32_t cap);
+
+/* ---- linked merge sort (shaders/linked_merge_sort.slang:3-37) ---- */
+/* nodes: 2 words per node (key as int32 bits, next). Sorts by (key, index). */
+uint32_t orc_linked_sort(uint32_t* nodes, uint32_t head);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
