@@ -29,7 +29,8 @@ constexpr uint32_t kErrTlasOverflow = 2u;
 constexpr uint32_t kErrUnsupported = 4u;
 constexpr uint32_t kHitOccluded = 0x100u;
 constexpr uint32_t kHitRoot = 0x200u;
-constexpr uint32_t kRayAnyHit = 1u;
+constexpr uint32_t kRayAnyHit = 1u;    // traceOccluded semantics
+constexpr uint32_t kRayFirstHit = 2u;  // traceRay(anyHit = true) semantics
 
 struct DevScene {
     const float4* tlas;
@@ -243,7 +244,8 @@ struct LaneStack {
 // [base, base+49) belong to this walk.
 template <class Stack, class St>
 __device__ __forceinline__ void trace_instance(const DevScene& s, float3 o, float3 d, uint32_t ii,
-                                               HitState& h, Stack& stk, uint32_t base, St& st) {
+                                               HitState& h, Stack& stk, uint32_t base, St& st,
+                                               bool any_hit) {
     const float4* rec = s.inst + 8u * ii;
     float4 i0 = rec[0], i1 = rec[1], i2 = rec[2];
     uint4 g = *reinterpret_cast<const uint4*>(rec + 6);
@@ -333,6 +335,7 @@ __device__ __forceinline__ void trace_instance(const DevScene& s, float3 o, floa
                 h.b1 = r3.y;
                 h.b2 = r3.z;
             }
+            if (any_hit && h.tri != kEmpty) return;
         }
     }
 }
@@ -340,7 +343,7 @@ __device__ __forceinline__ void trace_instance(const DevScene& s, float3 o, floa
 // traceRayTracked without dependency or cone terms (traversal.slang:696-811).
 template <class Stack, class St>
 __device__ __forceinline__ void trace_closest(const DevScene& s, float3 o, float3 d, float tmax,
-                                              HitState& h, Stack& stk, St& st) {
+                                              HitState& h, Stack& stk, St& st, bool any_hit) {
     h.t = tmax;
     h.tri = kEmpty;
     h.inst = kEmpty;
@@ -409,7 +412,15 @@ __device__ __forceinline__ void trace_closest(const DevScene& s, float3 o, float
         if (be > ie) be = ie;
         truncated = truncated || be < ie;
         if (truncated) h.err |= kErrUnsupported;
-        for (uint32_t i = ib; i < be; ++i) trace_instance(s, o, d, i, h, stk, count, st);
+        for (uint32_t i = ib; i < be; ++i) {
+            trace_instance(s, o, d, i, h, stk, count, st, any_hit);
+            if (any_hit && h.tri != kEmpty) {
+                // an accepted hit proves an any-hit query (traversal.slang:786-792)
+                h.err &= ~kErrUnsupported;
+                count = 0;
+                break;
+            }
+        }
     }
 }
 

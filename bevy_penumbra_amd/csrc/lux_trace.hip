@@ -35,7 +35,7 @@ __device__ __forceinline__ void trace_one(const TraceArgs& a, uint32_t idx,
     uint32_t flags = __float_as_uint(r1.w);
     lux::HitState h;
     if (flags & lux::kRayAnyHit) lux::trace_occluded(a.scene, o, d, r0.w, h, stk, st);
-    else lux::trace_closest(a.scene, o, d, r0.w, h, stk, st);
+    else lux::trace_closest(a.scene, o, d, r0.w, h, stk, st, (flags & lux::kRayFirstHit) != 0u);
     uint4 out;
     out.x = __float_as_uint(h.t);
     out.y = h.tri == lux::kEmpty ? lux::kEmpty : __float_as_uint(a.scene.triA[h.tri].w);

@@ -21,6 +21,7 @@ LUX_HIT_UNSUPPORTED = 4
 LUX_HIT_OCCLUDED = 0x100
 LUX_HIT_ROOT = 0x200
 LUX_RAY_ANY_HIT = 1
+LUX_RAY_FIRST_HIT = 2
 LUX_EMPTY = 0xFFFFFFFF
 LUX_BVH_MORTON30 = 0
 

@@ -215,8 +215,12 @@ def _euler_xyz(a, b, c):
 def traversal_fixture() -> Scene:
     """Cube / panel / bare meshes under 24 transforms (cull modes none, back, front)."""
     cube = hexahedron(box_corners((0, 0, 0), (0.5, 0.5, 0.5)))
-    panel = quad((-0.5, -0.5, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0))
-    bare = np.asarray([[0, 0, 0, 1, 0, 0, 0, 1, 0], [1, 0, 0, 1, 1, 0.5, 0, 1, 0]], np.float64)
+    # 2 x 2 rectangle in the XY plane, corners (+,+) (-,+) (-,-) (+,-), two
+    # triangles sharing corner 0 and the diagonal; the bare mesh is the same
+    # rectangle without attributes (src/traversal_tests.rs:2082-2089)
+    rect = np.asarray([[1, 1, 0], [-1, 1, 0], [-1, -1, 0], [1, -1, 0]], np.float64)
+    panel = np.asarray([np.concatenate(rect[[0, 1, 2]]), np.concatenate(rect[[0, 2, 3]])])
+    bare = panel.copy()
     meshes = [cube.astype(np.float32), panel.astype(np.float32), bare.astype(np.float32)]
     mats = [np.zeros(m.shape[0], np.uint32) for m in meshes]
     inv, fwd, mesh, flags = [], [], [], []

@@ -40,7 +40,11 @@ extern "C" {
 #define LUX_HIT_OCCLUDED 0x100u /* any-hit: an accepted blocker was found */
 #define LUX_HIT_ROOT 0x200u     /* any-hit: the ray met the scene root box */
 
-#define LUX_RAY_ANY_HIT 1u /* ray.flags: traceOccluded semantics instead of closest hit */
+/* ray.flags: 0 = closest hit (traceRay), LUX_RAY_ANY_HIT = occlusion query
+ * (traceOccluded), LUX_RAY_FIRST_HIT = first accepted hit (traceRay with
+ * anyHit = true, shaders/traversal.slang:696-811). */
+#define LUX_RAY_ANY_HIT 1u
+#define LUX_RAY_FIRST_HIT 2u
 
 #define LUX_EMPTY 0xffffffffu
 

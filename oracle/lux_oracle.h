@@ -35,7 +35,8 @@ extern "C" {
 #define ORC_ERR_UNSUPPORTED 4u
 #define ORC_HIT_OCCLUDED 0x100u
 #define ORC_HIT_ROOT 0x200u
-#define ORC_RAY_ANY_HIT 1u
+#define ORC_RAY_ANY_HIT 1u   /* traceOccluded */
+#define ORC_RAY_FIRST_HIT 2u /* traceRay(anyHit = true) */
 
 /* 32-byte BVH node, reference layout (src/scene.rs:76-83): count > 0 is a
  * leaf over [first, first+count); count == 0 has children first, first+1. */

@@ -346,16 +346,27 @@ uint32_t orc_sah_build(const float* lo, const float* hi, uint32_t n, uint32_t* p
     return c.n_nodes;
 }
 
+/* tree_cost (src/scene.rs:2383-2396) over the nodes reachable from the root
+ * (the LBVH layout keeps unreachable slots inside collapsed subtrees). */
 double orc_tree_cost(const orc_node* nodes, uint32_t n_nodes) {
     if (n_nodes == 0) return 0.0;
     double area = bounds_area(nodes[0].mn, nodes[0].mx);
     if (area <= 0.0) return 0.0;
     double sum = 0.0;
-    for (uint32_t i = 0; i < n_nodes; ++i) {
-        if (!(nodes[i].mn[0] <= nodes[i].mx[0])) continue; /* unused slot */
+    uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * (n_nodes + 2));
+    uint32_t top = 0;
+    stack[top++] = 0;
+    while (top) {
+        uint32_t i = stack[--top];
+        if (i >= n_nodes) continue;
         uint32_t c = nodes[i].count > 1 ? nodes[i].count : 1;
         sum += bounds_area(nodes[i].mn, nodes[i].mx) * (double)c;
+        if (nodes[i].count == 0 && top + 2 <= n_nodes) {
+            stack[top++] = nodes[i].first;
+            stack[top++] = nodes[i].first + 1;
+        }
     }
+    free(stack);
     return sum / area;
 }
 

@@ -8,7 +8,8 @@
  *   intersectTriangle     shaders/traversal.slang:415-443
  *   traceTriangle         shaders/traversal.slang:445-473 (opaque materials)
  *   traceInstance         shaders/traversal.slang:474-566
- *   traceRayTracked       shaders/traversal.slang:696-811 (no dependency/cone)
+ *   traceRayTracked       shaders/traversal.slang:696-811 (no dependency/cone),
+ *                         closest hit or, with ORC_RAY_FIRST_HIT, its anyHit mode
  *   occludedTriangle      shaders/traversal.slang:826-843
  *   occludedInstance      shaders/traversal.slang:845-932
  *   traceOccluded         shaders/traversal.slang:955-1046
@@ -180,7 +181,7 @@ typedef struct {
 
 /* traceInstance, closest hit (shaders/traversal.slang:474-566). */
 static void trace_instance(const orc_scene* s, const float o[3], const float d[3], uint32_t ii,
-                           hit_state* h, orc_stats* st) {
+                           hit_state* h, orc_stats* st, int any_hit) {
     const orc_instance* in = &s->inst[ii];
     if (st) st->inst_visits++;
     float lo[3], ld[3];
@@ -263,11 +264,13 @@ static void trace_instance(const orc_scene* s, const float o[3], const float d[3
                 h->b1 = r3[1];
                 h->b2 = r3[2];
             }
+            if (any_hit && h->tri != ORC_EMPTY) return;
         }
     }
 }
 
-static void trace_closest(const orc_scene* s, const orc_ray* r, hit_state* h, orc_stats* st) {
+static void trace_closest(const orc_scene* s, const orc_ray* r, hit_state* h, orc_stats* st,
+                          int any_hit) {
     h->t = r->tmax; h->tri = ORC_EMPTY; h->inst = ORC_EMPTY; h->err = 0; h->b1 = h->b2 = 0.0f;
     if (s->n_inst == 0) return;
     const float o[3] = {r->ox, r->oy, r->oz};
@@ -329,7 +332,15 @@ static void trace_closest(const orc_scene* s, const orc_ray* r, hit_state* h, or
         if (be > ie) be = ie;
         truncated = truncated || be < ie;
         if (truncated) h->err |= ORC_ERR_UNSUPPORTED;
-        for (uint32_t i = ib; i < be; ++i) trace_instance(s, o, d, i, h, st);
+        for (uint32_t i = ib; i < be; ++i) {
+            trace_instance(s, o, d, i, h, st, any_hit);
+            if (any_hit && h->tri != ORC_EMPTY) {
+                /* an accepted hit proves an any-hit query (traversal.slang:786-792) */
+                h->err &= ~ORC_ERR_UNSUPPORTED;
+                count = 0;
+                break;
+            }
+        }
     }
 }
 
@@ -499,7 +510,7 @@ static void* trace_worker(void* arg) {
         hit_state h;
         const orc_ray* r = &j->rays[i];
         if (r->flags & ORC_RAY_ANY_HIT) trace_occluded(j->s, r, &h, st);
-        else trace_closest(j->s, r, &h, st);
+        else trace_closest(j->s, r, &h, st, (r->flags & ORC_RAY_FIRST_HIT) != 0);
         orc_hit* o = &j->hits[i];
         o->t = h.t;
         /* Report caller-facing identities: the triangle's index within its

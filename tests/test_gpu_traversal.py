@@ -1,0 +1,176 @@
+"""HIP traversal and LBVH build against the oracle, bit for bit.
+
+Every GPU result is compared with the CPU oracle on the same inputs: BVH node
+words, triangle and instance order, and per-ray (t bits, triangle, instance,
+flags) plus barycentric bits.  Full-size (4M ray) runs are checked through
+size-independent properties (determinism, any-hit == closest-hit-within-tmax).
+"""
+import numpy as np
+import pytest
+
+from bevy_penumbra_amd import lux, scenes
+from oracle import oracle as O
+from tests.gpu_util import dev, empty_u32, host, trace_gpu
+from tests.traversal_check import check_against_brute_force, first_hit_copies, forced_deep
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def colonnade():
+    sc, _ = scenes.colonnade()
+    return sc, O.build_scene(sc)
+
+
+@pytest.fixture(scope="module")
+def fixture():
+    sc = scenes.traversal_fixture()
+    rays = scenes.fixture_rays(sc)
+    rays = np.concatenate([rays, first_hit_copies(rays[rays.shape[0] // 2:])])
+    return sc, O.build_scene(sc), rays
+
+
+def assert_same_scene(ctx, built):
+    g = ctx.download_bvh()
+    assert np.array_equal(g["blas"], built.blas)
+    assert np.array_equal(g["tlas"], built.tlas)
+
+
+def test_gpu_sort_pairs_is_stable(lux_ctx):
+    import torch
+    rng = np.random.default_rng(0)
+    for n in [1, 2, 100, 2048, 2049, 100_000, 1 << 20]:
+        keys = rng.integers(0, max(2, n // 3), n).astype(np.uint32)
+        keys[: n // 4] = rng.integers(0, 2**32, n // 4, dtype=np.uint64).astype(np.uint32)
+        k = dev(keys.view(np.int32))
+        ko = empty_u32(n)
+        io = empty_u32(n)
+        torch.cuda.synchronize()
+        lux_ctx.sort_pairs_device(k, n, ko, io)
+        lux_ctx.synchronize()
+        perm = np.argsort(keys, kind="stable").astype(np.uint32)
+        assert np.array_equal(host(io), perm), n
+        assert np.array_equal(host(ko), keys[perm]), n
+
+
+def test_gpu_morton_codes_match_oracle(lux_ctx):
+    rng = np.random.default_rng(1)
+    n = 50_000
+    lo = np.zeros((n, 4), np.float32)
+    lo[:, :3] = rng.normal(size=(n, 3)) * 5
+    hi = lo + np.float32(0.1) * rng.random((n, 4)).astype(np.float32)
+    hi[:, 3] = 0
+    lo[:10, :3] = 0.0  # ties and signed zeros
+    lo[5:10, :3] = -0.0
+    codes = empty_u32(n)
+    lux_ctx.morton_codes_device(dev(lo), dev(hi), n, codes)
+    assert np.array_equal(host(codes), O.morton_codes(lo, hi))
+
+
+def test_gpu_lbvh_matches_oracle_colonnade(lux_ctx, colonnade):
+    sc, built = colonnade
+    lux_ctx.upload_scene(sc)
+    lux_ctx.build_bvh()
+    assert_same_scene(lux_ctx, built)
+    g = lux_ctx.download_bvh()
+    assert np.array_equal(g["tri_order"], built.tri[:, 3].view(np.uint32))
+
+
+def test_gpu_lbvh_matches_oracle_fixture(lux_ctx, fixture):
+    sc, built, _ = fixture
+    lux_ctx.upload_scene(sc)
+    lux_ctx.build_bvh()
+    assert_same_scene(lux_ctx, built)
+    assert np.array_equal(lux_ctx.download_bvh()["inst_order"], built.inst[:, 28])
+
+
+def test_gpu_trace_fixture_bit_exact(lux_ctx, fixture):
+    sc, built, rays = fixture
+    lux_ctx.upload_scene(sc)
+    lux_ctx.build_bvh()
+    hits, bary = trace_gpu(lux_ctx, rays, bary=True)
+    ohits, obary = built.trace(rays, bary=True)
+    bad = np.nonzero(np.any(hits != ohits, axis=1))[0]
+    assert bad.size == 0, (bad[:5], hits[bad[:5]], ohits[bad[:5]])
+    assert np.array_equal(bary.view(np.uint32), obary.view(np.uint32))
+    failures, counts = check_against_brute_force(built, rays, hits)
+    assert not failures, "\n".join(failures)
+
+
+def test_gpu_host_entry_point_matches_device(lux_ctx, fixture):
+    sc, built, rays = fixture
+    lux_ctx.upload_scene(sc)
+    lux_ctx.build_bvh()
+    assert np.array_equal(lux_ctx.trace(rays), built.trace(rays))
+
+
+@pytest.mark.parametrize("blas", [True, False])
+def test_gpu_forced_deep_chains(lux_ctx, fixture, blas):
+    sc, built, rays = fixture
+    deep = forced_deep(built, blas)
+    lux_ctx.upload_built(deep.blas, deep.tlas, deep.tri, deep.inst)
+    sel = rays[::3]
+    hits = trace_gpu(lux_ctx, sel)
+    assert np.array_equal(hits, deep.trace(sel))
+    closest = (sel.view(np.uint32)[:, 7] == 0) & (hits[:, 1] != 0xFFFFFFFF)
+    assert np.all(hits[closest, 3] == (1 if blas else 2))
+
+
+def test_gpu_traverses_sah_built_scene(lux_ctx, colonnade):
+    sc, _ = colonnade
+    sah = O.build_scene(sc, sah=True)
+    lux_ctx.upload_built(sah.blas, sah.tlas, sah.tri, sah.inst)
+    rays = scenes.microbench_rays(sc.bounds, n=100_000, seed=11)
+    assert np.array_equal(trace_gpu(lux_ctx, rays), sah.trace(rays, threads=16))
+
+
+def test_gpu_trace_colonnade_1m_bit_exact(lux_ctx, colonnade):
+    sc, built = colonnade
+    lux_ctx.upload_scene(sc)
+    lux_ctx.build_bvh()
+    rays = scenes.microbench_rays(sc.bounds, n=1 << 20, seed=1)
+    hits = trace_gpu(lux_ctx, rays)
+    ohits = built.trace(rays, threads=16)
+    bad = np.nonzero(np.any(hits != ohits, axis=1))[0]
+    assert bad.size == 0, (bad[:5], hits[bad[:5]], ohits[bad[:5]])
+
+
+def test_gpu_launch_shapes_agree(lux_ctx, colonnade):
+    sc, _ = colonnade
+    lux_ctx.upload_scene(sc)
+    lux_ctx.build_bvh()
+    rays = scenes.microbench_rays(sc.bounds, n=200_000, seed=3)
+    ref = trace_gpu(lux_ctx, rays)
+    for wpc, rg in [(1000, 0), (4, 0), (32, 16)]:
+        lux_ctx.set_trace_tuning(wpc, rg)
+        assert np.array_equal(trace_gpu(lux_ctx, rays), ref), (wpc, rg)
+    lux_ctx.set_trace_tuning(0, 0)
+
+
+def test_gpu_trace_4m_properties(lux_ctx, colonnade):
+    """Full microbench size: deterministic, and every any-hit answer equals
+    "a closest hit exists before tmax" for the same ray."""
+    sc, _ = colonnade
+    lux_ctx.upload_scene(sc)
+    lux_ctx.build_bvh()
+    rays = scenes.microbench_rays(sc.bounds)
+    a = trace_gpu(lux_ctx, rays)
+    b = trace_gpu(lux_ctx, rays)
+    assert np.array_equal(a, b)
+    half = rays.shape[0] // 2
+    as_closest = rays[half:].copy()
+    as_closest.view(np.uint32)[:, 7] = 0
+    c = trace_gpu(lux_ctx, as_closest)
+    blocked = (a[half:, 3] & lux.LUX_HIT_OCCLUDED) != 0
+    assert np.array_equal(blocked, c[:, 1] != lux.LUX_EMPTY)
+    assert not np.any(a[:, 3] & lux.LUX_HIT_UNSUPPORTED)
+    assert 0.5 < blocked.mean() < 1.0
+
+
+def test_gpu_errors_are_reported(lux_ctx):
+    with pytest.raises(lux.LuxError):
+        lux_ctx.build_bvh(7)
+    ctx2 = lux.Context(0)
+    with pytest.raises(lux.LuxError):
+        ctx2.trace(np.zeros((4, 8), np.float32))  # no scene built
+    ctx2.close()

@@ -1,0 +1,209 @@
+"""Oracle BVH builders and sorts: fixtures, closed forms and invariants.
+
+* linked merge sort: the reference fixture of src/traversal_tests.rs:2824-2905
+  (lengths 0..257, LCG keys in -6..6), expected = sort by (key, index);
+* stable pair sort: numpy's stable argsort;
+* Morton codes: closed-form corner/centre values and bit de-interleaving;
+* Karras LBVH: equal to the textbook top-down construction that splits each
+  range at the highest differing bit of (key, index);
+* SAH: src/scene.rs:3627-3674 (250k triangles, depth <= 48, cost < 3) and
+  src/scene.rs:4440-4466 (cover every triangle once, contain children).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def test_linked_sort_matches_stdlib_sort():
+    for length in [0, 1, 2, 3, 5, 17, 63, 64, 129, 257]:
+        nodes = np.zeros((max(1, length), 2), np.uint32)
+        for i in range(length):
+            k = ((((i * 1664525) + 1013904223) & 0xFFFFFFFF) >> 16) % 13 - 6
+            nodes[i, 0] = np.uint32(k & 0xFFFFFFFF)
+            nodes[i, 1] = 0xFFFFFFFF if i + 1 == length else i + 1
+        expected = sorted(range(length), key=lambda j: (int(nodes[j, 0].view(np.int32)), j))
+        head = O.linked_sort(nodes, 0 if length else 0xFFFFFFFF)
+        order = []
+        while head != 0xFFFFFFFF and len(order) <= length:
+            order.append(head)
+            head = int(nodes[head, 1])
+        assert order == expected
+        assert head == 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 7, 1000, 65537])
+def test_stable_sort_pairs(n):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 50, n).astype(np.uint32)
+    perm = O.stable_sort_pairs(keys)
+    assert np.array_equal(perm, np.argsort(keys, kind="stable"))
+
+
+def deinterleave(code):
+    out = [0, 0, 0]
+    for bit in range(10):
+        for axis in range(3):
+            if code >> (3 * bit + (2 - axis)) & 1:
+                out[axis] |= 1 << bit
+    return out
+
+
+def test_morton_closed_forms():
+    lo = np.zeros((5, 4), np.float32)
+    hi = np.zeros((5, 4), np.float32)
+    pts = [(0, 0, 0), (8, 4, 2), (4, 2, 1), (8, 0, 0), (0, 4, 0)]
+    for i, p in enumerate(pts):
+        lo[i, :3] = p
+        hi[i, :3] = p
+    codes = O.morton_codes(lo, hi)
+    assert codes[0] == 0
+    assert codes[1] == 0x3FFFFFFF
+    assert deinterleave(int(codes[2])) == [512, 512, 512]
+    assert deinterleave(int(codes[3])) == [1023, 0, 0]
+    assert deinterleave(int(codes[4])) == [0, 1023, 0]
+    assert codes[3] == 0x24924924
+
+
+def test_morton_degenerate_axis_is_zero():
+    lo = np.zeros((3, 4), np.float32)
+    lo[:, 0] = [0, 1, 2]
+    codes = O.morton_codes(lo, lo.copy())
+    assert [deinterleave(int(c)) for c in codes] == [[0, 0, 0], [512, 0, 0], [1023, 0, 0]]
+
+
+def reference_hierarchy(keys, lo_i, hi_i, out):
+    """Top-down: split [lo, hi] at the highest bit where (key, index) differ."""
+    if lo_i == hi_i:
+        return
+    def full(i):
+        return (int(keys[i]) << 32) | i
+    a, b = full(lo_i), full(hi_i)
+    prefix = 64 - (a ^ b).bit_length()
+    split = lo_i
+    step = hi_i - lo_i
+    while step > 1:
+        step = (step + 1) >> 1
+        m = split + step
+        if m < hi_i and 64 - (a ^ full(m)).bit_length() > prefix:
+            split = m
+    out[(lo_i, hi_i)] = split
+    reference_hierarchy(keys, lo_i, split, out)
+    reference_hierarchy(keys, split + 1, hi_i, out)
+
+
+def node_ranges(nodes, leaf_max):
+    """Walk the oracle's layout, returning {range: split} for internal nodes
+    and the list of leaf ranges."""
+    splits, leaves = {}, []
+    def rng_of(i):
+        n = nodes[i]
+        if n[7] > 0:
+            return (int(n[3]), int(n[3]) + int(n[7]) - 1)
+        l = rng_of(int(n[3]))
+        r = rng_of(int(n[3]) + 1)
+        splits[(l[0], r[1])] = l[1]
+        return (l[0], r[1])
+    def walk(i):
+        n = nodes[i]
+        if n[7] > 0:
+            leaves.append((int(n[3]), int(n[7])))
+            return
+        walk(int(n[3])); walk(int(n[3]) + 1)
+    rng_of(0)
+    walk(0)
+    return splits, leaves
+
+
+@pytest.mark.parametrize("n,dup", [(2, False), (5, False), (300, False), (300, True), (5000, True)])
+def test_karras_matches_top_down_hierarchy(n, dup):
+    rng = np.random.default_rng(n + dup)
+    c = rng.random((n, 3)).astype(np.float32)
+    if dup:
+        c = np.round(c * 8) / 8
+    lo = np.zeros((n, 4), np.float32); lo[:, :3] = c - 0.01
+    hi = np.zeros((n, 4), np.float32); hi[:, :3] = c + 0.01
+    codes, perm, nodes = O.lbvh(lo, hi, leaf_max=1)
+    sk = codes[perm]
+    expected = {}
+    reference_hierarchy(sk, 0, n - 1, expected)
+    splits, leaves = node_ranges(nodes, 1)
+    assert splits == expected
+    assert sorted(l[0] for l in leaves) == list(range(n))
+
+
+def check_tree(nodes, lo, hi, perm, leaf_max):
+    """Coverage once, exact leaf boxes, children contained (reachable nodes)."""
+    n = lo.shape[0]
+    visits = np.zeros(n, np.int64)
+    stack = [(0, 0)]
+    depth = 0
+    f = nodes.view(np.float32)
+    while stack:
+        i, d = stack.pop()
+        depth = max(depth, d)
+        nd = nodes[i]
+        if nd[7] > 0:
+            assert nd[7] <= leaf_max
+            idx = perm[nd[3]:nd[3] + nd[7]]
+            visits[nd[3]:nd[3] + nd[7]] += 1
+            assert np.all(lo[idx, :3] >= f[i, :3]) and np.all(hi[idx, :3] <= f[i, 4:7])
+        else:
+            for ch in (nd[3], nd[3] + 1):
+                assert np.all(f[ch, :3] >= f[i, :3]) and np.all(f[ch, 4:7] <= f[i, 4:7])
+                stack.append((int(ch), d + 1))
+    assert np.all(visits == 1)
+    return depth
+
+
+def test_lbvh_invariants_random_triangles():
+    rng = np.random.default_rng(3)
+    n = 20000
+    a = rng.random((n, 3)).astype(np.float32) * 10
+    t = a[:, None, :] + rng.normal(size=(n, 3, 3)).astype(np.float32) * 0.05
+    lo = np.zeros((n, 4), np.float32); lo[:, :3] = t.min(1)
+    hi = np.zeros((n, 4), np.float32); hi[:, :3] = t.max(1)
+    codes, perm, nodes = O.lbvh(lo, hi, leaf_max=4)
+    assert check_tree(nodes, lo, hi, perm, 4) < 64
+    # root box is the scene box, exactly
+    f = nodes.view(np.float32)
+    assert np.array_equal(f[0, :3], lo[:, :3].min(0)) and np.array_equal(f[0, 4:7], hi[:, :3].max(0))
+
+
+def sah_fixture_250k():
+    i = np.arange(250_000)
+    x = np.where(i >= 249_900, np.float32(1000.0) + (i - 249_900).astype(np.float32) * np.float32(0.01),
+                 (i % 500).astype(np.float32) * np.float32(0.01)).astype(np.float32)
+    y = (i // 500).astype(np.float32) * np.float32(0.01)
+    tri = np.zeros((i.size, 3, 3), np.float32)
+    tri[:, 0, 0], tri[:, 0, 1] = x, y
+    tri[:, 1, 0], tri[:, 1, 1] = x + np.float32(0.003), y
+    tri[:, 2, 0], tri[:, 2, 1] = x, y + np.float32(0.003)
+    lo = np.zeros((i.size, 4), np.float32); lo[:, :3] = tri.min(1)
+    hi = np.zeros((i.size, 4), np.float32); hi[:, :3] = tri.max(1)
+    return lo, hi
+
+
+def test_sah_bounded_at_world_workload_size():
+    lo, hi = sah_fixture_250k()
+    perm, nodes = O.sah(lo, hi)
+    depth = check_tree(nodes, lo, hi, perm, 250_000)
+    assert depth <= 48
+    assert O.tree_cost(nodes) < 3.0
+
+
+def test_sah_covers_every_triangle_once_and_contains_children():
+    n = 100
+    i = np.arange(n, dtype=np.float32)
+    lo = np.zeros((n, 4), np.float32); hi = np.zeros((n, 4), np.float32)
+    lo[:, 0] = i; hi[:, 0] = i + 0.5; hi[:, 1] = 1.0
+    perm, nodes = O.sah(lo, hi)
+    check_tree(nodes, lo, hi, perm, n)
+    perm0, nodes0 = O.sah(lo[:0], hi[:0])
+    assert nodes0.shape[0] == 0
+
+
+def test_lbvh_on_sah_fixture_is_valid():
+    lo, hi = sah_fixture_250k()
+    codes, perm, nodes = O.lbvh(lo, hi, leaf_max=4)
+    check_tree(nodes, lo, hi, perm, 4)
