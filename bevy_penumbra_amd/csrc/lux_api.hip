@@ -166,7 +166,8 @@ extern "C" void lux_ctx_destroy(lux_ctx* ctx) {
     lux_buffer* bufs[] = {&ctx->raw_pos, &ctx->raw_mat, &ctx->inst_desc, &ctx->materials,
                           &ctx->triA, &ctx->triB, &ctx->triC, &ctx->tri_order, &ctx->blas,
                           &ctx->tlas, &ctx->inst, &ctx->inst_order, &ctx->scratch,
-                          &ctx->ray_buf, &ctx->hit_buf};
+                          &ctx->ray_buf, &ctx->hit_buf, &ctx->gi_intervals, &ctx->gi_merged,
+                          &ctx->gi_probes, &ctx->gi_light_buf};
     for (lux_buffer* b : bufs) lux_release(*b);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -237,6 +238,16 @@ extern "C" int lux_scene_upload_soa(lux_ctx* ctx, const lux_mesh_desc* meshes, u
             LUX_HIP(ctx, hipMemset(dm, 0, 4ull * n));
         }
     }
+    ctx->host_pos.assign(mesh_count, {});
+    ctx->host_mat.assign(mesh_count, {});
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        uint32_t n = meshes[m].triangle_count;
+        ctx->host_pos[m].assign(meshes[m].positions, meshes[m].positions + 9ull * n);
+        if (meshes[m].materials) ctx->host_mat[m].assign(meshes[m].materials, meshes[m].materials + n);
+        else ctx->host_mat[m].assign(n, 0u);
+    }
+    ctx->host_materials.assign(materials, materials + material_count);
+    ctx->gi_ready = false;
     ctx->instances.assign(instances, instances + instance_count);
     std::vector<InstDesc> idesc(instance_count);
     for (uint32_t i = 0; i < instance_count; ++i) {

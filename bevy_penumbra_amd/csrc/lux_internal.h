@@ -42,6 +42,20 @@ struct lux_ctx {
     lux_buffer scratch;
     lux_buffer ray_buf, hit_buf;
 
+    // host copies kept for the emitter table (CPU-side scene packing, as in
+    // the reference's build_emissive_lights)
+    std::vector<std::vector<float>> host_pos;
+    std::vector<std::vector<uint32_t>> host_mat;
+    std::vector<lux_material> host_materials;
+
+    // transport state
+    bool gi_ready = false;
+    lux_gi_params gi{};
+    uint64_t gi_total = 0;
+    uint64_t gi_offset[17] = {};
+    uint32_t gi_probes0 = 0, gi_lights = 0;
+    lux_buffer gi_intervals, gi_merged, gi_probes, gi_light_buf;
+
     // trace tuning
     uint32_t waves_per_cu = 0, regroup = 0;
     float last_ms = 0.0f;

@@ -34,6 +34,35 @@ class MeshDesc(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("materials", C.c_void_p), ("triangle_count", C.c_uint32)]
 
 
+class GiParams(C.Structure):
+    """lux_gi_params (dense probe grid, cascades, intervals, NEE sampling)."""
+    _fields_ = [("origin", C.c_float * 3), ("spacing", C.c_float), ("dims", C.c_uint32 * 3),
+                ("levels", C.c_uint32), ("base_phi", C.c_uint32), ("base_y", C.c_uint32),
+                ("interval_mode", C.c_uint32), ("interval_length", C.c_float),
+                ("normal_bias", C.c_float), ("emissive_samples", C.c_uint32),
+                ("frame", C.c_uint32), ("seed", C.c_uint32), ("rays_per_interval", C.c_uint32),
+                ("terminal_tail", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+def gi_params(origin, spacing, dims, levels, base_phi=8, base_y=4, interval_mode=0,
+              interval_length=0.4, normal_bias=0.005, emissive_samples=16, frame=0, seed=0,
+              rays_per_interval=1, terminal_tail=1) -> GiParams:
+    p = GiParams()
+    p.origin[:] = [float(x) for x in origin]
+    p.spacing = spacing
+    p.dims[:] = [int(x) for x in dims]
+    p.levels = levels
+    p.base_phi, p.base_y = base_phi, base_y
+    p.interval_mode = interval_mode
+    p.interval_length = interval_length
+    p.normal_bias = normal_bias
+    p.emissive_samples = emissive_samples
+    p.frame, p.seed = frame, seed
+    p.rays_per_interval = rays_per_interval
+    p.terminal_tail = terminal_tail
+    return p
+
+
 class InstanceDesc(C.Structure):
     _fields_ = [
         ("local_from_world", C.c_float * 12),
@@ -77,6 +106,17 @@ def load_library() -> C.CDLL:
         "lux_trace_stats_device": (i32, [vp, vp, u32, vp, vp]),
         "lux_last_kernel_ms": (i32, [vp, C.POINTER(C.c_float)]),
         "lux_set_trace_tuning": (i32, [vp, u32, u32]),
+        "lux_gi_configure": (i32, [vp, vp]),
+        "lux_gi_set_frame": (i32, [vp, u32, u32]),
+        "lux_gi_trace": (i32, [vp]),
+        "lux_gi_merge": (i32, [vp]),
+        "lux_gi_pack": (i32, [vp]),
+        "lux_gi_update": (i32, [vp, vp]),
+        "lux_gi_sizes": (i32, [vp, vp, vp, vp]),
+        "lux_gi_download": (i32, [vp, vp, vp, vp, vp]),
+        "lux_gi_upload_intervals": (i32, [vp, vp]),
+        "lux_gi_device_arrays": (i32, [vp, vp, vp, vp]),
+        "lux_world_resolve_device": (i32, [vp, vp, u32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -235,3 +275,56 @@ class Context:
     def set_trace_tuning(self, waves_per_cu: int = 0, regroup: int = 0):
         self._check(self.lib.lux_set_trace_tuning(self.h, waves_per_cu, regroup),
                     "lux_set_trace_tuning")
+
+    # ---- transport (dense-grid GI update) ----
+    def gi_configure(self, params: GiParams):
+        self._gi = params
+        self._check(self.lib.lux_gi_configure(self.h, C.byref(params)), "lux_gi_configure")
+
+    def gi_set_frame(self, frame: int, seed: int):
+        self._check(self.lib.lux_gi_set_frame(self.h, frame, seed), "lux_gi_set_frame")
+
+    def gi_trace(self):
+        self._check(self.lib.lux_gi_trace(self.h), "lux_gi_trace")
+
+    def gi_merge(self):
+        self._check(self.lib.lux_gi_merge(self.h), "lux_gi_merge")
+
+    def gi_pack(self):
+        self._check(self.lib.lux_gi_pack(self.h), "lux_gi_pack")
+
+    def gi_update(self, timed: bool = False):
+        ms = C.c_float()
+        self._check(self.lib.lux_gi_update(self.h, C.byref(ms) if timed else None), "lux_gi_update")
+        return float(ms.value) if timed else None
+
+    def gi_sizes(self):
+        n = C.c_uint64()
+        p = C.c_uint32()
+        l = C.c_uint32()
+        self._check(self.lib.lux_gi_sizes(self.h, C.byref(n), C.byref(p), C.byref(l)), "lux_gi_sizes")
+        return int(n.value), int(p.value), int(l.value)
+
+    def gi_download(self):
+        n, p, l = self.gi_sizes()
+        iv = np.zeros((n, 4), np.float32)
+        mg = np.zeros((n, 4), np.float32)
+        pr = np.zeros((p, 36, 4), np.float32)
+        li = np.zeros((max(1, l), 24), np.float32)
+        self._check(self.lib.lux_gi_download(self.h, iv.ctypes.data, mg.ctypes.data, pr.ctypes.data,
+                                             li.ctypes.data), "lux_gi_download")
+        return dict(intervals=iv, merged=mg, probes=pr, lights=li[:l])
+
+    def gi_upload_intervals(self, intervals: np.ndarray):
+        iv = np.ascontiguousarray(intervals, np.float32)
+        self._check(self.lib.lux_gi_upload_intervals(self.h, iv.ctypes.data), "lux_gi_upload_intervals")
+
+    def gi_device_arrays(self):
+        a, b, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        self._check(self.lib.lux_gi_device_arrays(self.h, C.byref(a), C.byref(b), C.byref(c)),
+                    "lux_gi_device_arrays")
+        return a.value, b.value, c.value
+
+    def world_resolve_device(self, receivers, n: int, out):
+        self._check(self.lib.lux_world_resolve_device(self.h, _ptr(receivers), n, _ptr(out)),
+                    "lux_world_resolve_device")

@@ -171,6 +171,60 @@ int lux_trace_rays_device(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit
 int lux_trace_stats_device(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits,
                            unsigned long long* counters);
 
+/* ---- transport: interval trace, cascade merge, probe packing, world resolve
+ *      (replaces the Split dispatches trace/reduce/merge x L/packProbes/
+ *      worldResolve, src/render.rs:9640-9801; shaders/split.slang:1056-1918,
+ *      shaders/cascades.slang:60-268, shaders/world_resolve.slang) ---- */
+#define LUX_INTERVAL_POW4 0u     /* interval end = length * 4^L (split.slang:224-231) */
+#define LUX_INTERVAL_DOUBLING 1u /* interval end = length * (2^(L+1) - 1) */
+
+/* Dense probe grid.  Level L has ceil(dims / 2^L) probes at spacing * 2^L
+ * and (base_phi * 2^L) x (base_y * 2^L) equal-area direction bins
+ * (reference preset: 8 x 4 = 32 at L0).  One interval = one (probe,
+ * direction, level); its ray starts at the probe centre + start * dir. */
+typedef struct {
+    float origin[3];           /* world position of the grid corner */
+    float spacing;             /* level-0 probe spacing (RadianceCascades probe spacing) */
+    uint32_t dims[3];          /* level-0 probe counts */
+    uint32_t levels;           /* cascades, 1..8 */
+    uint32_t base_phi, base_y; /* level-0 direction bins */
+    uint32_t interval_mode;    /* LUX_INTERVAL_* */
+    float interval_length;     /* level-0 interval end */
+    float normal_bias;         /* surface offset of visibility rays */
+    uint32_t emissive_samples; /* SamplingSettings emissive samples; NEE uses max(1, n/2) */
+    uint32_t frame;            /* frame index (NEE sequence) */
+    uint32_t seed;             /* direction jitter seed */
+    uint32_t rays_per_interval;
+    uint32_t terminal_tail;    /* 1: the last level owns the unbounded tail */
+    uint32_t flags;            /* reserved, 0 */
+    uint32_t reserved;
+} lux_gi_params;
+
+/* Allocate the per-level interval/merged arrays (float4 per interval:
+ * J.rgb + beta, beta = -1 where no supported sample; I.rgb + support), the
+ * packed probes (36 float4 per level-0 probe) and build the emitter table
+ * from the uploaded scene (build_emissive_lights, src/scene.rs:2311-2372). */
+int lux_gi_configure(lux_ctx* ctx, const lux_gi_params* params);
+int lux_gi_set_frame(lux_ctx* ctx, uint32_t frame, uint32_t seed);
+int lux_gi_trace(lux_ctx* ctx);
+int lux_gi_merge(lux_ctx* ctx);
+int lux_gi_pack(lux_ctx* ctx);
+/* trace + merge + pack; *ms (nullable) = GPU time of the three phases. */
+int lux_gi_update(lux_ctx* ctx, float* ms);
+/* Total intervals of all levels and level-0 probe count. */
+int lux_gi_sizes(lux_ctx* ctx, uint64_t* intervals, uint32_t* probes, uint32_t* lights);
+/* Host copies (each nullable): intervals/merged float[4*intervals],
+ * probes float[4*36*probes], lights float[24*lights]. */
+int lux_gi_download(lux_ctx* ctx, float* intervals, float* merged, float* probes, float* lights);
+/* Replace the intervals (host float[4*intervals]) before lux_gi_merge. */
+int lux_gi_upload_intervals(lux_ctx* ctx, const float* intervals);
+/* Device addresses of the context-owned arrays (for zero-copy interop). */
+int lux_gi_device_arrays(lux_ctx* ctx, float** intervals, float** merged, float** probes);
+/* Resolve receivers (device float[6*n]: position, normal) against the packed
+ * probes into device float[4*n]: diffuse irradiance/pi rgb and the support
+ * code (0 not eligible, -1 unsupported, 1 + 0.25 coverage). */
+int lux_world_resolve_device(lux_ctx* ctx, const float* receivers, uint32_t n, float* out);
+
 /* Kernel timing of the last device trace (HIP events on the context stream). */
 int lux_last_kernel_ms(lux_ctx* ctx, float* ms);
 

@@ -99,6 +99,9 @@ void orc_trace(const orc_scene* s, const orc_ray* rays, uint32_t n, orc_hit* hit
                float* bary /* nullable, 2 per ray */, orc_stats* stats /* nullable */,
                int threads);
 
+void orc_trace_ray(const orc_scene* s, const orc_ray* r, orc_hit* out, float* bary2,
+                   uint32_t* tri_slot, uint32_t* inst_slot);
+
 /* ---- LBVH (Morton-30) build, restated exactly as the GPU builds it ---- */
 /* boxes: lo[4*n], hi[4*n] (xyz used).  Outputs: codes[n] (unsorted Morton),
  * perm[n] (sorted order), nodes[max(1,2n-1)].  Returns node count. */
@@ -141,6 +144,75 @@ uint32_t orc_brute_force(const orc_scene* s, const orc_ray* ray, double* ts, uin
 /* ---- linked merge sort (shaders/linked_merge_sort.slang:3-37) ---- */
 /* nodes: 2 words per node (key as int32 bits, next). Sorts by (key, index). */
 uint32_t orc_linked_sort(uint32_t* nodes, uint32_t head);
+
+/* ---- transport: dense-grid interval trace, merge, pack, resolve ---- */
+/* Same layout as lux_gi_params in include/lux.h. */
+typedef struct {
+    float origin[3];
+    float spacing;
+    uint32_t dims[3];
+    uint32_t levels;
+    uint32_t base_phi, base_y;
+    uint32_t interval_mode; /* 0: end = len*4^L; 1: end = len*(2^(L+1)-1) */
+    float interval_length;
+    float normal_bias;
+    uint32_t emissive_samples;
+    uint32_t frame;
+    uint32_t seed;
+    uint32_t rays_per_interval;
+    uint32_t terminal_tail;
+    uint32_t flags;
+    uint32_t reserved;
+} orc_gi_params;
+
+/* Emissive triangle record (GiLight, src/scene.rs:90-108), 96 B. */
+typedef struct {
+    float position[4];  /* a.xyz, -2 double-sided / -1 one-sided */
+    float color[4];     /* emission.rgb, area */
+    float direction[4]; /* unit normal.xyz, CDF boundary */
+    float edge1[4];     /* b - a, discrete probability */
+    float edge2[4];     /* c - a, 0 */
+    uint32_t source[4]; /* TLAS instance slot, triangle slot, 0, 0 */
+} orc_light;
+
+typedef struct {
+    float albedo[3];
+    float roughness;
+    float emission[3];
+    float pad;
+} orc_material;
+
+uint32_t orc_gi_level_dims(const orc_gi_params* p, uint32_t level, uint32_t dims[3]);
+uint64_t orc_gi_level_offset(const orc_gi_params* p, uint32_t level);
+uint32_t orc_gi_level_directions(const orc_gi_params* p, uint32_t level);
+void orc_gi_interval_bounds(const orc_gi_params* p, uint32_t level, float* start, float* end);
+/* Emitter table of a built scene (build_emissive_lights, src/scene.rs:2311-2372).
+ * Returns the light count (<= cap). */
+uint32_t orc_build_lights(const float* const* mesh_pos, const uint32_t* const* mesh_mat,
+                          const uint32_t* mesh_tri_count, uint32_t n_meshes, const float* inst_fwd,
+                          const uint32_t* inst_mesh, const uint32_t* inst_flags, uint32_t n_inst,
+                          const orc_material* mats, uint32_t n_mats, orc_light* out, uint32_t cap);
+void orc_bin_direction(uint32_t index, uint32_t nphi, uint32_t ny, float ox, float oy,
+                       float out[3]);
+void orc_cos_sin_2pi(float u, float* c, float* s);
+/* Intervals (J.rgb, beta; beta = -1 where no supported sample) of every level. */
+void orc_gi_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
+                  const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
+                  float* intervals, int threads);
+void orc_gi_merge(const orc_gi_params* p, const float* intervals, float* merged);
+void orc_gi_merge_f64(const orc_gi_params* p, const double* intervals, double* merged);
+/* 36 octahedral texels (irradiance/pi rgb, coverage) per level-0 probe. */
+void orc_gi_pack(const orc_gi_params* p, const float* merged, float* probes);
+void orc_gi_pack_f64(const orc_gi_params* p, const double* merged, double* probes);
+/* receivers: 6 floats (position, normal); out: rgba (alpha = support code). */
+void orc_world_resolve(const orc_gi_params* p, const float* probes, const float* receivers,
+                       uint32_t n, float* out);
+void orc_world_resolve_f64(const orc_gi_params* p, const double* probes, const float* receivers,
+                           uint32_t n, double* out);
+/* Shading in f64 (FP64 twin): same hits and visibility, f64 radiance. */
+void orc_gi_trace_f64(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
+                      const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
+                      double* intervals, int threads);
 
 #ifdef __cplusplus
 }

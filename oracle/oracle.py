@@ -190,3 +190,158 @@ def box_interval(o, d, mn, mx, limit):
     e, x = C.c_float(), C.c_float()
     hit = load().orc_box_interval(*[a.ctypes.data for a in arrs], float(limit), C.byref(e), C.byref(x))
     return bool(hit), e.value, x.value
+
+
+# ---------------------------------------------------------------- transport
+class GiParams(C.Structure):
+    """Mirror of orc_gi_params / lux_gi_params."""
+    _fields_ = [("origin", C.c_float * 3), ("spacing", C.c_float), ("dims", C.c_uint32 * 3),
+                ("levels", C.c_uint32), ("base_phi", C.c_uint32), ("base_y", C.c_uint32),
+                ("interval_mode", C.c_uint32), ("interval_length", C.c_float),
+                ("normal_bias", C.c_float), ("emissive_samples", C.c_uint32),
+                ("frame", C.c_uint32), ("seed", C.c_uint32), ("rays_per_interval", C.c_uint32),
+                ("terminal_tail", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+def gi_params(origin, spacing, dims, levels, base_phi=8, base_y=4, interval_mode=0,
+              interval_length=0.4, normal_bias=0.005, emissive_samples=16, frame=0, seed=0,
+              rays_per_interval=1, terminal_tail=1) -> GiParams:
+    p = GiParams()
+    p.origin[:] = [float(x) for x in origin]
+    p.spacing = spacing
+    p.dims[:] = [int(x) for x in dims]
+    p.levels = levels
+    p.base_phi, p.base_y = base_phi, base_y
+    p.interval_mode = interval_mode
+    p.interval_length = interval_length
+    p.normal_bias = normal_bias
+    p.emissive_samples = emissive_samples
+    p.frame, p.seed = frame, seed
+    p.rays_per_interval = rays_per_interval
+    p.terminal_tail = terminal_tail
+    return p
+
+
+def _tsig():
+    lib = load()
+    vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int
+    for name, res, args in [
+        ("orc_gi_level_dims", u32, [vp, u32, vp]),
+        ("orc_gi_level_offset", C.c_uint64, [vp, u32]),
+        ("orc_gi_level_directions", u32, [vp, u32]),
+        ("orc_gi_interval_bounds", None, [vp, u32, vp, vp]),
+        ("orc_build_lights", u32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, u32, vp, u32]),
+        ("orc_bin_direction", None, [u32, u32, u32, C.c_float, C.c_float, vp]),
+        ("orc_cos_sin_2pi", None, [C.c_float, vp, vp]),
+        ("orc_gi_trace", None, [vp, vp, u32, vp, u32, vp, vp, i32]),
+        ("orc_gi_trace_f64", None, [vp, vp, u32, vp, u32, vp, vp, i32]),
+        ("orc_gi_merge", None, [vp, vp, vp]),
+        ("orc_gi_merge_f64", None, [vp, vp, vp]),
+        ("orc_gi_pack", None, [vp, vp, vp]),
+        ("orc_gi_pack_f64", None, [vp, vp, vp]),
+        ("orc_world_resolve", None, [vp, vp, vp, u32, vp]),
+        ("orc_world_resolve_f64", None, [vp, vp, vp, u32, vp]),
+    ]:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def level_dims(p: GiParams, level: int):
+    d = (C.c_uint32 * 3)()
+    n = _tsig().orc_gi_level_dims(C.byref(p), level, d)
+    return n, tuple(d)
+
+
+def level_offset(p: GiParams, level: int) -> int:
+    return int(_tsig().orc_gi_level_offset(C.byref(p), level))
+
+
+def level_directions(p: GiParams, level: int) -> int:
+    return int(_tsig().orc_gi_level_directions(C.byref(p), level))
+
+
+def interval_bounds(p: GiParams, level: int):
+    s, e = C.c_float(), C.c_float()
+    _tsig().orc_gi_interval_bounds(C.byref(p), level, C.byref(s), C.byref(e))
+    return s.value, e.value
+
+
+def total_intervals(p: GiParams) -> int:
+    return level_offset(p, p.levels)
+
+
+def bin_direction(index, nphi, ny, ox=0.5, oy=0.5):
+    out = np.zeros(3, np.float32)
+    _tsig().orc_bin_direction(index, nphi, ny, ox, oy, out.ctypes.data)
+    return out
+
+
+def cos_sin_2pi(u):
+    c, s = C.c_float(), C.c_float()
+    _tsig().orc_cos_sin_2pi(float(u), C.byref(c), C.byref(s))
+    return c.value, s.value
+
+
+def build_lights(scene, cap: int = 1 << 20) -> np.ndarray:
+    """(L, 24) float32 light records (uint words in the last 4 columns)."""
+    lib = _tsig()
+    nm = len(scene.meshes)
+    pos = [np.ascontiguousarray(m, np.float32) for m in scene.meshes]
+    mat = [np.ascontiguousarray(m, np.uint32) for m in scene.mesh_materials]
+    counts = np.asarray([p.shape[0] for p in pos], np.uint32)
+    pp = (C.c_void_p * nm)(*[p.ctypes.data for p in pos])
+    mp = (C.c_void_p * nm)(*[m.ctypes.data for m in mat])
+    fwd = np.ascontiguousarray(scene.inst_fwd, np.float32)
+    im = np.ascontiguousarray(scene.inst_mesh, np.uint32)
+    fl = np.ascontiguousarray(scene.inst_flags, np.uint32)
+    mats = np.ascontiguousarray(scene.materials, np.float32)
+    cap = min(cap, int(counts.sum()) * max(1, im.shape[0]) + 1)
+    out = np.zeros((cap, 24), np.float32)
+    n = lib.orc_build_lights(pp, mp, counts.ctypes.data, nm, fwd.ctypes.data, im.ctypes.data,
+                             fl.ctypes.data, im.shape[0], mats.ctypes.data, mats.shape[0],
+                             out.ctypes.data, cap)
+    return out[:n].copy()
+
+
+def gi_trace(built: BuiltScene, lights: np.ndarray, materials: np.ndarray, p: GiParams,
+             threads: int = 8, f64: bool = False) -> np.ndarray:
+    lib = _tsig()
+    lights = np.ascontiguousarray(lights, np.float32)
+    mats = np.ascontiguousarray(materials, np.float32)
+    out = np.zeros((total_intervals(p), 4), np.float64 if f64 else np.float32)
+    fn = lib.orc_gi_trace_f64 if f64 else lib.orc_gi_trace
+    fn(C.byref(built._s), lights.ctypes.data, lights.shape[0], mats.ctypes.data, mats.shape[0],
+       C.byref(p), out.ctypes.data, threads)
+    return out
+
+
+def gi_merge(p: GiParams, intervals: np.ndarray) -> np.ndarray:
+    lib = _tsig()
+    f64 = intervals.dtype == np.float64
+    iv = np.ascontiguousarray(intervals)
+    out = np.zeros_like(iv)
+    (lib.orc_gi_merge_f64 if f64 else lib.orc_gi_merge)(C.byref(p), iv.ctypes.data, out.ctypes.data)
+    return out
+
+
+def gi_pack(p: GiParams, merged: np.ndarray) -> np.ndarray:
+    lib = _tsig()
+    f64 = merged.dtype == np.float64
+    n0, _ = level_dims(p, 0)
+    out = np.zeros((n0, 36, 4), merged.dtype)
+    m = np.ascontiguousarray(merged)
+    (lib.orc_gi_pack_f64 if f64 else lib.orc_gi_pack)(C.byref(p), m.ctypes.data, out.ctypes.data)
+    return out
+
+
+def world_resolve(p: GiParams, probes: np.ndarray, receivers: np.ndarray) -> np.ndarray:
+    lib = _tsig()
+    f64 = probes.dtype == np.float64
+    pr = np.ascontiguousarray(probes)
+    rc = np.ascontiguousarray(receivers, np.float32)
+    out = np.zeros((rc.shape[0], 4), probes.dtype)
+    (lib.orc_world_resolve_f64 if f64 else lib.orc_world_resolve)(
+        C.byref(p), pr.ctypes.data, rc.ctypes.data, rc.shape[0], out.ctypes.data)
+    return out

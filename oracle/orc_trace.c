@@ -674,3 +674,19 @@ uint32_t orc_linked_sort(uint32_t* nodes, uint32_t head) {
     }
     return head;
 }
+
+/* Single-ray entry for the transport oracle: returns the caller-facing hit and
+ * the BVH slots of the hit triangle and instance (ORC_EMPTY on a miss). */
+void orc_trace_ray(const orc_scene* s, const orc_ray* r, orc_hit* out, float* bary2,
+                   uint32_t* tri_slot, uint32_t* inst_slot) {
+    hit_state h;
+    if (r->flags & ORC_RAY_ANY_HIT) trace_occluded(s, r, &h, NULL);
+    else trace_closest(s, r, &h, NULL, (r->flags & ORC_RAY_FIRST_HIT) != 0);
+    out->t = h.t;
+    out->triangle = h.tri == ORC_EMPTY ? ORC_EMPTY : fbits(s->tri[12ull * h.tri + 3]);
+    out->instance = h.inst == ORC_EMPTY ? ORC_EMPTY : s->inst[h.inst].orig;
+    out->flags = h.err;
+    if (bary2) { bary2[0] = h.b1; bary2[1] = h.b2; }
+    if (tri_slot) *tri_slot = h.tri;
+    if (inst_slot) *inst_slot = h.inst;
+}

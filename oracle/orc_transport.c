@@ -1,0 +1,371 @@
+/*
+ * orc_transport.c -- oracle of the dense-grid interval trace, cascade merge,
+ * probe packing and world resolve (TEST INFRASTRUCTURE, never the product).
+ *
+ * Restated from the reference (f32, no contraction, left-to-right sums):
+ *   transportHash, r2, hashKey, cacheKey, rayOffset, decodeDirection,
+ *   encodeNormal, decodeNormal, disneyDiffuseMultiplier
+ *                           shaders/common.slang:282-429
+ *   unitVector, transformDirection, triangleGeometricNormal
+ *                           shaders/traversal.slang:170-254
+ *   outgoingTransportRadianceTracked / emissiveResponseTracked
+ *                           shaders/cascades.slang:60-134,196-268
+ *   deposit (J, beta per interval, terminal tail)
+ *                           shaders/split.slang:1056-1117
+ *   reduce (sample mean)    shaders/split.slang:1386-1476
+ *   mergeEntry without history: beginMergeGroup/addMergeChild/
+ *   finishMergeGroup/mergeRadianceGroups/interpolateMergedChildSources
+ *                           shaders/split.slang:389-479,1523-1622,1624-1725
+ *   packProbes              shaders/split.slang:1788-1918
+ *   irradianceLookup/diffuseProbePage/diffuseAtLod/resolveLodSource/
+ *   resolveSplitDiffuse/rawGiSupportAlpha
+ *                           shaders/world_resolve.slang:1-196, common.slang:324
+ *   build_emissive_lights   src/scene.rs:2311-2372
+ *
+ * Dense-grid restatement (the reference keys sparse probes by hash; here a
+ * probe is a dense grid cell, a missing entry is a cell without a supported
+ * sample, beta = -1 marks "no sample").  The three radiance source groups
+ * share one support in a history-free update, so they are merged as one RGB
+ * group.  The BSDF is the diffuse albedo times the Disney multiplier / pi
+ * (scatteringResponse's diffuse branch, shaders/environment.slang:2-6).
+ * cos/sin of 2*pi*u use a fixed f64 Taylor evaluation (orc_cos_sin_2pi) so
+ * that CPU and GPU directions are bit-identical.
+ *
+ * Pinned by (tests/test_oracle_transport.py): the four-cascade closed form of
+ * src/split_merge_tests.rs:1403-1419 (level means and supports), the dense
+ * direction round trip of src/transport_tests.rs:4040-4071, analytic cases of
+ * the merge and pack formulas, the f64 twin (errors stated against it), and
+ * the Cornell emitter irradiance against numerical quadrature.
+ */
+#include "lux_oracle.h"
+#include <pthread.h>
+#include <tgmath.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_PI_F 3.14159265358979323846f
+
+static inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static inline int finitef_(float f) { return (fbits(f) & 0x7f800000u) != 0x7f800000u; }
+
+/* ---------------- level geometry ---------------- */
+uint32_t orc_gi_level_dims(const orc_gi_params* p, uint32_t level, uint32_t dims[3]) {
+    uint32_t n = 1;
+    for (int a = 0; a < 3; ++a) {
+        uint32_t d = p->dims[a];
+        for (uint32_t l = 0; l < level; ++l) d = (d + 1) / 2;
+        dims[a] = d;
+        n *= d;
+    }
+    return n;
+}
+
+uint32_t orc_gi_level_directions(const orc_gi_params* p, uint32_t level) {
+    return (p->base_phi << level) * (p->base_y << level);
+}
+
+uint64_t orc_gi_level_offset(const orc_gi_params* p, uint32_t level) {
+    uint64_t off = 0;
+    for (uint32_t l = 0; l < level; ++l) {
+        uint32_t d[3];
+        off += (uint64_t)orc_gi_level_dims(p, l, d) * orc_gi_level_directions(p, l);
+    }
+    return off;
+}
+
+static float interval_end(const orc_gi_params* p, uint32_t level) {
+    if (p->interval_mode == 1u)
+        return p->interval_length * (float)((2u << level) - 1u);
+    return p->interval_length * ldexpf(1.0f, (int)(2u * level));
+}
+
+void orc_gi_interval_bounds(const orc_gi_params* p, uint32_t level, float* start, float* end) {
+    *end = interval_end(p, level);
+    *start = level == 0 ? 0.0f : interval_end(p, level - 1);
+}
+
+/* ---------------- integer hashes and sequences (common.slang) ---------------- */
+static inline uint32_t transport_hash(uint32_t v) {
+    v ^= v >> 16;
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return v;
+}
+
+static inline void r2(uint32_t index, float out[2]) {
+    uint32_t px = index * 3242174889u, py = index * 2447445414u;
+    out[0] = (float)(px >> 8) / 16777216.0f;
+    out[1] = (float)(py >> 8) / 16777216.0f;
+}
+
+static inline float fracf_(float x) { return x - floorf(x); }
+
+static inline uint32_t hash_key(int32_t x, int32_t y, int32_t z, int32_t w, uint32_t buckets) {
+    uint32_t h = (uint32_t)x * 0x8da6b343u ^ (uint32_t)y * 0xd8163841u ^
+                 (uint32_t)z * 0xcb1ab31fu ^ (uint32_t)w * 0x165667b1u;
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    return h & (buckets - 1u);
+}
+
+/* ---------------- directions ---------------- */
+/* cos(2 pi u), sin(2 pi u): quadrant split in exact f64, then Taylor series
+ * to t^21 on [0, pi/2), rounded to f32. Only + and * (no contraction). */
+void orc_cos_sin_2pi(float u, float* c, float* s) {
+    double x = (double)u * 4.0;
+    double q = floor(x);
+    double r = x - q;
+    int qi = ((int)q) & 3;
+    double t = r * 1.5707963267948966;
+    double t2 = t * t;
+    double sn = 1.0 / 51090942171709440000.0;           /* 1/21! */
+    sn = sn * t2 * -1.0 + 1.0 / 121645100408832000.0;   /* 1/19! */
+    sn = sn * t2 * -1.0 + 1.0 / 355687428096000.0;      /* 1/17! */
+    sn = sn * t2 * -1.0 + 1.0 / 1307674368000.0;        /* 1/15! */
+    sn = sn * t2 * -1.0 + 1.0 / 6227020800.0;           /* 1/13! */
+    sn = sn * t2 * -1.0 + 1.0 / 39916800.0;             /* 1/11! */
+    sn = sn * t2 * -1.0 + 1.0 / 362880.0;               /* 1/9! */
+    sn = sn * t2 * -1.0 + 1.0 / 5040.0;                 /* 1/7! */
+    sn = sn * t2 * -1.0 + 1.0 / 120.0;                  /* 1/5! */
+    sn = sn * t2 * -1.0 + 1.0 / 6.0;                    /* 1/3! */
+    sn = sn * t2 * -1.0 + 1.0;
+    sn = sn * t;
+    double cs = 1.0 / 2432902008176640000.0;            /* 1/20! */
+    cs = cs * t2 * -1.0 + 1.0 / 6402373705728000.0;     /* 1/18! */
+    cs = cs * t2 * -1.0 + 1.0 / 20922789888000.0;       /* 1/16! */
+    cs = cs * t2 * -1.0 + 1.0 / 87178291200.0;          /* 1/14! */
+    cs = cs * t2 * -1.0 + 1.0 / 479001600.0;            /* 1/12! */
+    cs = cs * t2 * -1.0 + 1.0 / 3628800.0;              /* 1/10! */
+    cs = cs * t2 * -1.0 + 1.0 / 40320.0;                /* 1/8! */
+    cs = cs * t2 * -1.0 + 1.0 / 720.0;                  /* 1/6! */
+    cs = cs * t2 * -1.0 + 1.0 / 24.0;                   /* 1/4! */
+    cs = cs * t2 * -1.0 + 1.0 / 2.0;                    /* 1/2! */
+    cs = cs * t2 * -1.0 + 1.0;
+    double oc, os;
+    switch (qi) {
+        case 0: oc = cs; os = sn; break;
+        case 1: oc = -sn; os = cs; break;
+        case 2: oc = -cs; os = -sn; break;
+        default: oc = sn; os = -cs; break;
+    }
+    *c = (float)oc;
+    *s = (float)os;
+}
+
+/* decodeDirection (common.slang:376-382) with the polynomial cos/sin. */
+static void decode_direction(float ux, float uy, float out[3]) {
+    float c, s;
+    orc_cos_sin_2pi(ux, &c, &s);
+    float y = uy * 2.0f - 1.0f;
+    float q = 1.0f - y * y;
+    float r = sqrtf(q > 0.0f ? q : 0.0f);
+    out[0] = r * c;
+    out[1] = y;
+    out[2] = r * s;
+}
+
+/* Equal-area lat-long bin (binDirection, common.slang:395-400) generalised
+ * to nphi x ny bins, with an in-bin offset (0.5, 0.5 = bin centre). */
+void orc_bin_direction(uint32_t index, uint32_t nphi, uint32_t ny, float ox, float oy,
+                       float out[3]) {
+    float ux = ((float)(index % nphi) + ox) / (float)nphi;
+    float uy = ((float)(index / nphi) + oy) / (float)ny;
+    decode_direction(ux, uy, out);
+}
+
+/* Per-update in-bin offset of ray r: frac(0.5 + jitter + r2(r)) with
+ * jitter = r2(transportHash(seed ^ 0x5a17c9e3)) (split.slang:1135-1143). */
+static void ray_offset_uv(uint32_t seed, uint32_t r, float out[2]) {
+    float j[2], q[2];
+    r2(transport_hash(seed ^ 0x5a17c9e3u), j);
+    r2(r, q);
+    out[0] = fracf_(0.5f + j[0] + q[0]);
+    out[1] = fracf_(0.5f + j[1] + q[1]);
+}
+
+/* rayOffset (common.slang:345-359). */
+static void ray_offset(const float pos[3], const float n[3], float bias, float out[3]) {
+    for (int a = 0; a < 3; ++a) {
+        float origin = pos[a] + n[a] * bias;
+        int32_t step = (int32_t)(n[a] * 256.0f);
+        uint32_t bits = fbits(origin);
+        uint32_t moved = bits + (uint32_t)(origin < 0.0f ? -step : step);
+        float integer_offset = bitsf(moved);
+        float shifted = fabsf(origin) < 0.03125f ? origin + n[a] / 65536.0f : integer_offset;
+        out[a] = finitef_(shifted) ? shifted : pos[a];
+    }
+}
+
+/* ---------------- emitter table (src/scene.rs:2311-2372) ---------------- */
+uint32_t orc_build_lights(const float* const* mesh_pos, const uint32_t* const* mesh_mat,
+                          const uint32_t* mesh_tri_count, uint32_t n_meshes, const float* inst_fwd,
+                          const uint32_t* inst_mesh, const uint32_t* inst_flags, uint32_t n_inst,
+                          const orc_material* mats, uint32_t n_mats, orc_light* out, uint32_t cap) {
+    uint32_t n = 0;
+    double* w = (double*)malloc(sizeof(double) * (cap ? cap : 1));
+    for (uint32_t k = 0; k < n_inst; ++k) {
+        uint32_t m = inst_mesh[k];
+        if (m >= n_meshes) continue;
+        const float* f = inst_fwd + 12ull * k;
+        uint32_t cull = (inst_flags ? inst_flags[k] : 0u) & 3u;
+        for (uint32_t t = 0; t < mesh_tri_count[m]; ++t) {
+            uint32_t mi = mesh_mat && mesh_mat[m] ? mesh_mat[m][t] : 0u;
+            if (n_mats == 0) continue;
+            if (mi >= n_mats) mi = n_mats - 1;
+            const orc_material* mat = &mats[mi];
+            float em[3];
+            for (int c = 0; c < 3; ++c) em[c] = mat->emission[c] > 0.0f ? mat->emission[c] : 0.0f;
+            const float* p = mesh_pos[m] + 9ull * t;
+            float v[3][3];
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < 3; ++r) {
+                    const float* row = f + 4 * r;
+                    v[c][r] = ((row[0] * p[3 * c] + row[1] * p[3 * c + 1]) + row[2] * p[3 * c + 2]) +
+                              row[3];
+                }
+            float e1[3], e2[3];
+            for (int r = 0; r < 3; ++r) { e1[r] = v[1][r] - v[0][r]; e2[r] = v[2][r] - v[0][r]; }
+            float cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                           e1[0] * e2[1] - e1[1] * e2[0]};
+            float len = sqrtf((cr[0] * cr[0] + cr[1] * cr[1]) + cr[2] * cr[2]);
+            float area = len * 0.5f;
+            float lum = (em[0] * 0.2126f + em[1] * 0.7152f) + em[2] * 0.0722f;
+            double power = (double)area * (double)lum * 1.0;
+            if (!isfinite(power) || power <= 0.0) continue;
+            if (n >= cap) continue;
+            orc_light* L = &out[n];
+            memset(L, 0, sizeof *L);
+            L->position[0] = v[0][0]; L->position[1] = v[0][1]; L->position[2] = v[0][2];
+            L->position[3] = cull == 0 ? -2.0f : -1.0f;
+            L->color[0] = em[0]; L->color[1] = em[1]; L->color[2] = em[2]; L->color[3] = area;
+            float inv = 1.0f / len;
+            float sgn = cull == 2 ? -1.0f : 1.0f;
+            for (int r = 0; r < 3; ++r) {
+                L->direction[r] = (cr[r] * inv) * sgn;
+                L->edge1[r] = e1[r];
+                L->edge2[r] = e2[r];
+            }
+            L->source[0] = k;
+            L->source[1] = t;
+            w[n] = power;
+            n++;
+        }
+    }
+    if (n > 0) {
+        const uint32_t steps = 1u << 24;
+        double total = 0.0;
+        for (uint32_t i = 0; i < n; ++i) total += w[i];
+        double cum = 0.0;
+        uint32_t prev = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            cum += w[i];
+            uint32_t boundary;
+            if (i + 1 == n) {
+                boundary = steps;
+            } else {
+                double ideal_d = round(cum / total * (double)steps);
+                uint32_t ideal = ideal_d < 0.0 ? 0u : (ideal_d > 4294967295.0 ? 0xffffffffu : (uint32_t)ideal_d);
+                uint32_t lo = prev + 1, hi = steps - (n - i - 1);
+                boundary = ideal < lo ? lo : (ideal > hi ? hi : ideal);
+            }
+            out[i].direction[3] = (float)boundary / (float)steps;
+            out[i].edge1[3] = (float)(boundary - prev) / (float)steps;
+            prev = boundary;
+        }
+    }
+    free(w);
+    return n;
+}
+
+/* ---------------- shared f32 helpers ---------------- */
+static inline float dot3f(const float a[3], const float b[3]) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static inline float satf(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
+static void unit_vector_f(const float v[3], const float fb[3], float out[3]) {
+    float sc = fmaxf(fabsf(v[0]), fmaxf(fabsf(v[1]), fabsf(v[2])));
+    if (sc == 0.0f || !finitef_(sc)) { memcpy(out, fb, 12); return; }
+    float div = fmaxf(sc * 0.25f, 1.17549435e-38f);
+    float w[3] = {v[0] / div, v[1] / div, v[2] / div};
+    float len = sqrtf(dot3f(w, w));
+    out[0] = w[0] / len; out[1] = w[1] / len; out[2] = w[2] / len;
+}
+
+/* triangleGeometricNormal (traversal.slang:248-251) through the inverse rows. */
+static void geometric_normal(const orc_instance* in, const float* tri, float out[3]) {
+    const float* a = tri;
+    const float* b = tri + 4;
+    const float* c = tri + 8;
+    float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    float cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                   e1[0] * e2[1] - e1[1] * e2[0]};
+    /* transformDirection(base, cr, transpose = true) */
+    float r[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r[i][j] = in->inv[4 * i + j];
+    float mag[3];
+    for (int j = 0; j < 3; ++j) mag[j] = fmaxf(fabsf(r[0][j]), fmaxf(fabsf(r[1][j]), fabsf(r[2][j])));
+    float sc = fmaxf(mag[0], fmaxf(mag[1], mag[2]));
+    float tv[3] = {0.0f, 0.0f, 0.0f};
+    if (sc != 0.0f) {
+        sc = fmaxf(sc * 0.25f, 1.17549435e-38f);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) r[i][j] = r[i][j] / sc;
+        float zero[3] = {0.0f, 0.0f, 0.0f}, u[3];
+        unit_vector_f(cr, zero, u);
+        for (int j = 0; j < 3; ++j) tv[j] = (r[0][j] * u[0] + r[1][j] * u[1]) + r[2][j] * u[2];
+    }
+    float up[3] = {0.0f, 1.0f, 0.0f}, n[3];
+    unit_vector_f(tv, up, n);
+    float sg = (in->flags & 4u) != 0u ? -1.0f : 1.0f;
+    out[0] = sg * n[0]; out[1] = sg * n[1]; out[2] = sg * n[2];
+}
+
+/* ---------------- the REAL-generic parts (float product, double twin) ---------- */
+#define REAL float
+#define SFX(name) name##_f32
+#include "orc_transport_impl.h"
+#undef REAL
+#undef SFX
+#define REAL double
+#define SFX(name) name##_f64
+#include "orc_transport_impl.h"
+#undef REAL
+#undef SFX
+
+void orc_gi_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
+                  const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
+                  float* intervals, int threads) {
+    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, threads);
+}
+void orc_gi_trace_f64(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
+                      const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
+                      double* intervals, int threads) {
+    gi_trace_f64(s, lights, n_lights, mats, n_mats, p, intervals, threads);
+}
+void orc_gi_merge(const orc_gi_params* p, const float* intervals, float* merged) {
+    gi_merge_f32(p, intervals, merged);
+}
+void orc_gi_merge_f64(const orc_gi_params* p, const double* intervals, double* merged) {
+    gi_merge_f64(p, intervals, merged);
+}
+void orc_gi_pack(const orc_gi_params* p, const float* merged, float* probes) {
+    gi_pack_f32(p, merged, probes);
+}
+void orc_gi_pack_f64(const orc_gi_params* p, const double* merged, double* probes) {
+    gi_pack_f64(p, merged, probes);
+}
+void orc_world_resolve(const orc_gi_params* p, const float* probes, const float* receivers,
+                       uint32_t n, float* out) {
+    world_resolve_f32(p, probes, receivers, n, out);
+}
+void orc_world_resolve_f64(const orc_gi_params* p, const double* probes, const float* receivers,
+                           uint32_t n, double* out) {
+    world_resolve_f64(p, probes, receivers, n, out);
+}
