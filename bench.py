@@ -40,7 +40,17 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--waves-per-cu", type=int, default=0)
     p.add_argument("--regroup", type=int, default=0)
+    p.add_argument("--gi-steps", type=int, default=5)
+    p.add_argument("--no-gi", action="store_true")
     return p.parse_args()
+
+
+def colonnade_gi_params(lux):
+    """Full 5-cascade split trace on the 64 x 24 x 32 probe grid at 0.5 m
+    (32 level-0 directions, interval ends 0.5 * 4^L, terminal tail)."""
+    return lux.gi_params((-0.5, -0.5, -0.5), 0.5, (64, 24, 32), 5, base_phi=8, base_y=4,
+                         interval_mode=0, interval_length=0.5, normal_bias=0.005,
+                         emissive_samples=16, seed=0, terminal_tail=1)
 
 
 def main() -> int:
@@ -89,6 +99,24 @@ def main() -> int:
     avg_kernel_ms = float(np.mean(kernel_ms))
     mrays = n / (ms_per_step * 1e-3) / 1e6
     achieved = algo_bytes / (avg_kernel_ms * 1e-3) / 1e9
+
+    gi = None
+    if not args.no_gi:
+        gp = colonnade_gi_params(lux)
+        ctx.gi_configure(gp)
+        n_int, n_probes, n_lights = ctx.gi_sizes()
+        ctx.gi_update()
+        ctx.synchronize()
+        gi_ms = [ctx.gi_update(timed=True) for _ in range(args.gi_steps)]
+        ctx.synchronize()
+        gi = {
+            "ms_per_gi_update": round(float(np.median(gi_ms)), 4),
+            "intervals": n_int,
+            "interval_mrays_s": round(n_int / (float(np.median(gi_ms)) * 1e-3) / 1e6, 3),
+            "probes": n_probes,
+            "emitter_triangles": n_lights,
+            "grid": "64x24x32 at 0.5 m, 5 levels, 32 base directions, NEE 8 samples",
+        }
 
     cpu = None
     if not args.no_cpu:
@@ -143,6 +171,7 @@ def main() -> int:
             "traffic": None,
         },
         "cpu_baseline": cpu,
+        "gi_update": gi,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

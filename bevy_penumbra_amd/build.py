@@ -39,7 +39,7 @@ def hipcc() -> str:
 
 
 def sources() -> list[Path]:
-    return sorted(CSRC.glob("*.hip"))
+    return sorted(CSRC.glob("*.hip")) + sorted((PKG / "host").glob("*.cpp"))
 
 
 def build_hip(verbose: bool = False) -> Path:

@@ -63,6 +63,46 @@ def gi_params(origin, spacing, dims, levels, base_phi=8, base_y=4, interval_mode
     return p
 
 
+class RadianceCascades(C.Structure):
+    """C view of the Bevy RadianceCascades component (include/lux_bevy.h)."""
+    _fields_ = [("method", C.c_uint32), ("history_epoch", C.c_uint32),
+                ("probe_spacing", C.c_float), ("interval_length", C.c_float),
+                ("resolution_divisor", C.c_uint32), ("max_rays", C.c_uint32),
+                ("normal_bias", C.c_float), ("intensity", C.c_float),
+                ("environment", C.c_float * 3), ("temporal_blend", C.c_float),
+                ("split_transmittance_blend", C.c_float), ("history_frames", C.c_uint32),
+                ("spatial_filter_steps", C.c_uint32), ("emissive_samples", C.c_uint32),
+                ("sampling", C.c_uint32 * 6), ("flags", C.c_uint32)]
+
+    @classmethod
+    def default(cls) -> "RadianceCascades":
+        r = cls()
+        load_library().lux_radiance_cascades_default(C.byref(r))
+        return r
+
+    @classmethod
+    def low_end(cls) -> "RadianceCascades":
+        r = cls()
+        load_library().lux_radiance_cascades_low_end(C.byref(r))
+        return r
+
+    def sanitized(self) -> "RadianceCascades":
+        r = RadianceCascades()
+        load_library().lux_radiance_cascades_sanitized(C.byref(self), C.byref(r))
+        return r
+
+    def gi_params(self, origin, dims, cascades, base_phi=8, base_y=4, interval_mode=0,
+                  frame=0) -> "GiParams":
+        p = GiParams()
+        o = (C.c_float * 3)(*origin)
+        d = (C.c_uint32 * 3)(*dims)
+        rc = load_library().lux_radiance_cascades_gi_params(
+            C.byref(self), o, d, cascades, base_phi, base_y, interval_mode, frame, C.byref(p))
+        if rc != LUX_OK:
+            raise LuxError(f"lux_radiance_cascades_gi_params: {rc}")
+        return p
+
+
 class InstanceDesc(C.Structure):
     _fields_ = [
         ("local_from_world", C.c_float * 12),
@@ -117,6 +157,10 @@ def load_library() -> C.CDLL:
         "lux_gi_upload_intervals": (i32, [vp, vp]),
         "lux_gi_device_arrays": (i32, [vp, vp, vp, vp]),
         "lux_world_resolve_device": (i32, [vp, vp, u32, vp]),
+        "lux_radiance_cascades_default": (i32, [vp]),
+        "lux_radiance_cascades_low_end": (i32, [vp]),
+        "lux_radiance_cascades_sanitized": (i32, [vp, vp]),
+        "lux_radiance_cascades_gi_params": (i32, [vp, vp, vp, u32, u32, u32, u32, u32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
