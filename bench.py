@@ -58,8 +58,8 @@ def main() -> int:
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         return 2
-    import torch
     from bevy_penumbra_amd import lux, scenes
+    from bevy_penumbra_amd.lux import DeviceBuffer
 
     scene, _ = scenes.colonnade()
     rays_np = scenes.microbench_rays(scene.bounds, n=args.rays)
@@ -72,28 +72,25 @@ def main() -> int:
     ctx.set_trace_tuning(args.waves_per_cu, args.regroup)
 
     n = rays_np.shape[0]
-    rays = torch.from_numpy(rays_np).to("cuda:0")
-    hits = torch.zeros(n, 4, dtype=torch.int32, device="cuda:0")
-    counters = torch.zeros(3, dtype=torch.int64, device="cuda:0")
-    torch.cuda.synchronize()
+    rays = DeviceBuffer.from_array(ctx, rays_np)
+    hits = DeviceBuffer.empty(ctx, (n, 4), np.uint32)
+    counters = DeviceBuffer.empty(ctx, 3, np.uint64)
 
     # algorithmic work of one launch (instrumented kernel, outside timing)
     ctx.trace_stats_device(rays, n, hits, counters)
     ctx.synchronize()
-    nodes, tris, insts = (int(x) for x in counters.cpu().tolist())
+    nodes, tris, insts = (int(x) for x in counters.numpy().tolist())
     algo_bytes = 32 * n + 16 * n + 32 * nodes + 36 * tris + 48 * insts
 
     for _ in range(args.warmup):
         ctx.trace_device(rays, n, hits)
     ctx.synchronize()
-    torch.cuda.synchronize()
     kernel_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.trace_device(rays, n, hits)
         kernel_ms.append(ctx.last_kernel_ms())
     ctx.synchronize()
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     avg_kernel_ms = float(np.mean(kernel_ms))

@@ -481,3 +481,54 @@ extern "C" int lux_scene_upload_built(lux_ctx* ctx, const lux_bvh_node* blas, ui
     ctx->tlas_built = true;
     return LUX_OK;
 }
+
+namespace {
+__global__ void k_zero_bytes(uint8_t* p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0;
+}
+}  // namespace
+
+extern "C" int lux_device_alloc(lux_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return LUX_ERR_INVALID;
+    *out = nullptr;
+    LUX_HIP(ctx, hipSetDevice(ctx->device));
+    LUX_HIP(ctx, hipMalloc(out, bytes ? bytes : 16));
+    return LUX_OK;
+}
+
+extern "C" int lux_device_free(lux_ctx* ctx, void* ptr) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!ptr) return LUX_OK;
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LUX_HIP(ctx, hipFree(ptr));
+    return LUX_OK;
+}
+
+extern "C" int lux_copy_to_device(lux_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return LUX_ERR_INVALID;
+    if (!bytes) return LUX_OK;
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LUX_HIP(ctx, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return LUX_OK;
+}
+
+extern "C" int lux_copy_to_host(lux_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return LUX_ERR_INVALID;
+    if (!bytes) return LUX_OK;
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LUX_HIP(ctx, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return LUX_OK;
+}
+
+extern "C" int lux_device_zero(lux_ctx* ctx, void* ptr, size_t bytes) {
+    if (!ctx || (bytes && !ptr)) return LUX_ERR_INVALID;
+    if (!bytes) return LUX_OK;
+    size_t blocks = (bytes + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_zero_bytes, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream,
+                       static_cast<uint8_t*>(ptr), bytes);
+    LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}

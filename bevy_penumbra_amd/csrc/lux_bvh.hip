@@ -430,6 +430,11 @@ int lux_lbvh_build_device(lux_ctx* ctx, const float4* lo, const float4* hi, uint
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, 256)), dim3(256), 0, st, lo, hi, n, bounds, codes);
     if (int rc = lux_sort_pairs_impl(ctx, codes, n, sk, perm, tk, tv, hist)) return rc;
     if (n <= leaf_max) {
+        // the root is a leaf; the remaining 2n-2 slots of this range are unused
+        // and kept zero, as in every other unused slot
+        uint32_t words = 8u * (2u * n - 1u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(words, 256)), dim3(256), 0, st,
+                           reinterpret_cast<uint32_t*>(nodes), 0u, words);
         hipLaunchKernelGGL(k_small_root, dim3(1), dim3(64), 0, st, lo, hi, perm, n, nodes,
                            leaf_first_offset);
         LUX_HIP(ctx, hipGetLastError());

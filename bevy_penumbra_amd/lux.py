@@ -157,6 +157,11 @@ def load_library() -> C.CDLL:
         "lux_gi_upload_intervals": (i32, [vp, vp]),
         "lux_gi_device_arrays": (i32, [vp, vp, vp, vp]),
         "lux_world_resolve_device": (i32, [vp, vp, u32, vp]),
+        "lux_device_alloc": (i32, [vp, C.c_size_t, C.POINTER(vp)]),
+        "lux_device_free": (i32, [vp, vp]),
+        "lux_copy_to_device": (i32, [vp, vp, vp, C.c_size_t]),
+        "lux_copy_to_host": (i32, [vp, vp, vp, C.c_size_t]),
+        "lux_device_zero": (i32, [vp, vp, C.c_size_t]),
         "lux_radiance_cascades_default": (i32, [vp]),
         "lux_radiance_cascades_low_end": (i32, [vp]),
         "lux_radiance_cascades_sanitized": (i32, [vp, vp]),
@@ -170,11 +175,67 @@ def load_library() -> C.CDLL:
     return lib
 
 
+class DeviceBuffer:
+    """Device memory owned through the lux ABI (no second HIP runtime)."""
+
+    def __init__(self, ctx: "Context", nbytes: int, dtype=np.uint8, shape=None):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.dtype = np.dtype(dtype)
+        self.shape = shape if shape is not None else (self.nbytes // self.dtype.itemsize,)
+        p = C.c_void_p()
+        ctx._check(ctx.lib.lux_device_alloc(ctx.h, self.nbytes, C.byref(p)), "lux_device_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, ctx: "Context", a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(ctx, a.nbytes, a.dtype, a.shape)
+        b.upload(a)
+        return b
+
+    @classmethod
+    def empty(cls, ctx: "Context", shape, dtype) -> "DeviceBuffer":
+        shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        b = cls(ctx, nbytes, dtype, shape)
+        b.zero()
+        return b
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        self.ctx._check(self.ctx.lib.lux_copy_to_device(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes),
+                        "lux_copy_to_device")
+
+    def zero(self):
+        self.ctx._check(self.ctx.lib.lux_device_zero(self.ctx.h, self.ptr, self.nbytes), "lux_device_zero")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        self.ctx._check(self.ctx.lib.lux_copy_to_host(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes),
+                        "lux_copy_to_host")
+        return out
+
+    def free(self):
+        if self.ptr and getattr(self.ctx, "h", None):
+            self.ctx.lib.lux_device_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def _ptr(a) -> int | None:
     if a is None:
         return None
     if isinstance(a, int):
         return a
+    if isinstance(a, DeviceBuffer):
+        return a.ptr
     if isinstance(a, np.ndarray):
         assert a.flags["C_CONTIGUOUS"], "arrays passed to lux must be C-contiguous"
         return a.ctypes.data

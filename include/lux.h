@@ -109,6 +109,16 @@ const char* lux_last_error(const lux_ctx* ctx);
 void* lux_ctx_stream(lux_ctx* ctx);
 int lux_ctx_synchronize(lux_ctx* ctx);
 
+/* ---- device buffers owned by the caller (rays, hits, receivers) ----
+ * Plain HIP allocations on the context's device; copies are synchronous
+ * with respect to the context stream. */
+int lux_device_alloc(lux_ctx* ctx, size_t bytes, void** out);
+int lux_device_free(lux_ctx* ctx, void* ptr);
+int lux_copy_to_device(lux_ctx* ctx, void* dst, const void* src, size_t bytes);
+int lux_copy_to_host(lux_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Zero device memory with a fill kernel on the context stream. */
+int lux_device_zero(lux_ctx* ctx, void* ptr, size_t bytes);
+
 /* ---- scene (replaces scene.rs triangle/instance packing and uploads,
  *      src/scene.rs:809-823,1025-1122,2096-2117) ---- */
 int lux_scene_upload_soa(lux_ctx* ctx, const lux_mesh_desc* meshes, uint32_t mesh_count,

@@ -12,7 +12,7 @@ import pytest
 
 from bevy_penumbra_amd import lux, scenes
 from oracle import oracle as O
-from tests.gpu_util import dev, host
+from bevy_penumbra_amd.lux import DeviceBuffer
 from tests.transport_cases import (cornell_params, four_cascade_case, four_cascade_expected,
                                    grid_receivers)
 
@@ -35,12 +35,11 @@ def run_gpu(ctx, sc, p, receivers=None):
     ctx.synchronize()
     out = ctx.gi_download()
     if receivers is not None:
-        import torch
-        r = dev(receivers)
-        o = torch.zeros(receivers.shape[0], 4, dtype=torch.float32, device="cuda:0")
+        r = DeviceBuffer.from_array(ctx, np.ascontiguousarray(receivers, np.float32))
+        o = DeviceBuffer.empty(ctx, (receivers.shape[0], 4), np.float32)
         ctx.world_resolve_device(r, receivers.shape[0], o)
         ctx.synchronize()
-        out["resolve"] = o.cpu().numpy()
+        out["resolve"] = o.numpy()
     return out
 
 

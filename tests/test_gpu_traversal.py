@@ -10,7 +10,8 @@ import pytest
 
 from bevy_penumbra_amd import lux, scenes
 from oracle import oracle as O
-from tests.gpu_util import dev, empty_u32, host, trace_gpu
+from bevy_penumbra_amd.lux import DeviceBuffer
+from tests.gpu_util import trace_gpu
 from tests.traversal_check import check_against_brute_force, first_hit_copies, forced_deep
 
 pytestmark = pytest.mark.gpu
@@ -37,20 +38,18 @@ def assert_same_scene(ctx, built):
 
 
 def test_gpu_sort_pairs_is_stable(lux_ctx):
-    import torch
     rng = np.random.default_rng(0)
     for n in [1, 2, 100, 2048, 2049, 100_000, 1 << 20]:
         keys = rng.integers(0, max(2, n // 3), n).astype(np.uint32)
         keys[: n // 4] = rng.integers(0, 2**32, n // 4, dtype=np.uint64).astype(np.uint32)
-        k = dev(keys.view(np.int32))
-        ko = empty_u32(n)
-        io = empty_u32(n)
-        torch.cuda.synchronize()
+        k = DeviceBuffer.from_array(lux_ctx, keys)
+        ko = DeviceBuffer.empty(lux_ctx, n, np.uint32)
+        io = DeviceBuffer.empty(lux_ctx, n, np.uint32)
         lux_ctx.sort_pairs_device(k, n, ko, io)
         lux_ctx.synchronize()
         perm = np.argsort(keys, kind="stable").astype(np.uint32)
-        assert np.array_equal(host(io), perm), n
-        assert np.array_equal(host(ko), keys[perm]), n
+        assert np.array_equal(io.numpy(), perm), n
+        assert np.array_equal(ko.numpy(), keys[perm]), n
 
 
 def test_gpu_morton_codes_match_oracle(lux_ctx):
@@ -62,9 +61,10 @@ def test_gpu_morton_codes_match_oracle(lux_ctx):
     hi[:, 3] = 0
     lo[:10, :3] = 0.0  # ties and signed zeros
     lo[5:10, :3] = -0.0
-    codes = empty_u32(n)
-    lux_ctx.morton_codes_device(dev(lo), dev(hi), n, codes)
-    assert np.array_equal(host(codes), O.morton_codes(lo, hi))
+    codes = DeviceBuffer.empty(lux_ctx, n, np.uint32)
+    lux_ctx.morton_codes_device(DeviceBuffer.from_array(lux_ctx, lo),
+                                DeviceBuffer.from_array(lux_ctx, hi), n, codes)
+    assert np.array_equal(codes.numpy(), O.morton_codes(lo, hi))
 
 
 def test_gpu_lbvh_matches_oracle_colonnade(lux_ctx, colonnade):
