@@ -450,6 +450,8 @@ int orc_scene_build(const float* const* mesh_pos, const uint32_t* const* mesh_ma
                            : orc_lbvh_build(lo, hi, n, leaf_max, codes, perm, blas + blas_base);
         for (uint32_t k = 0; k < cnt; ++k) {
             orc_node* nd = &blas[blas_base + k];
+            /* unused slots stay all-zero (no internal node has first == 0) */
+            if (nd->count == 0 && nd->first == 0) continue;
             if (nd->count == 0) nd->first += blas_base;
             else nd->first += tri_base;
         }
