@@ -149,9 +149,15 @@ extern "C" int lux_ctx_create(int device, lux_ctx** out) {
     ctx->device = device;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
-        delete ctx;
+        lux_ctx_destroy(ctx);
         return LUX_ERR_HIP;
     }
+    for (hipEvent_t& e : ctx->ev_phase)
+        if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+            lux_ctx_destroy(ctx);
+            return LUX_ERR_HIP;
+        }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->cu_count = prop.multiProcessorCount;
@@ -167,8 +173,11 @@ extern "C" void lux_ctx_destroy(lux_ctx* ctx) {
                           &ctx->triA, &ctx->triB, &ctx->triC, &ctx->tri_order, &ctx->blas,
                           &ctx->tlas, &ctx->inst, &ctx->inst_order, &ctx->scratch,
                           &ctx->ray_buf, &ctx->hit_buf, &ctx->gi_intervals, &ctx->gi_merged,
-                          &ctx->gi_probes, &ctx->gi_light_buf};
+                          &ctx->gi_probes, &ctx->gi_light_buf, &ctx->gi_sur_queries,
+                          &ctx->gi_sur_err, &ctx->gi_sur_list};
     for (lux_buffer* b : bufs) lux_release(*b);
+    for (hipEvent_t e : ctx->ev_phase)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -55,6 +55,11 @@ struct lux_ctx {
     uint64_t gi_offset[17] = {};
     uint32_t gi_probes0 = 0, gi_lights = 0;
     lux_buffer gi_intervals, gi_merged, gi_probes, gi_light_buf;
+    // surrogate-assisted update: level queries, per-probe predicted error,
+    // fallback probe list + count
+    lux_buffer gi_sur_queries, gi_sur_err, gi_sur_list;
+    uint32_t gi_sur_probes = 0;
+    hipEvent_t ev_phase[5] = {};
 
     // trace tuning
     uint32_t waves_per_cu = 0, regroup = 0;
@@ -85,3 +90,9 @@ int lux_sort_pairs_impl(lux_ctx* ctx, const uint32_t* keys_in, uint32_t n, uint3
                         uint32_t* vals_out, uint32_t* tmp_keys, uint32_t* tmp_vals,
                         uint32_t* hist);
 size_t lux_sort_hist_words(uint32_t n);
+
+// Surrogate forward pass over device queries (9 floats each); *Y receives a
+// device pointer to n x 16 outputs owned by the surrogate, valid until its
+// next call.
+int lux_surrogate_forward_internal(lux_surrogate* s, const float* queries, uint32_t n, const float** Y);
+lux_ctx* lux_surrogate_context(lux_surrogate* s);

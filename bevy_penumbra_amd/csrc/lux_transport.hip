@@ -326,30 +326,27 @@ __device__ __forceinline__ float interval_end(const lux_gi_params& p, uint32_t l
     return p.interval_length * pow2i((int)(2u * level));
 }
 
-__global__ __launch_bounds__(kWg) void k_gi_trace(GiArgs a) {
-    __shared__ uint2 lds[kLdsDepth][kWg];
-    Stack stk;
-    stk.col = &lds[0][threadIdx.x];
-    uint64_t g = (uint64_t)blockIdx.x * kWg + threadIdx.x;
-    const lux_gi_params& p = a.p;
-    if (g >= a.offset[p.levels]) return;
-    uint32_t L = 0;
-    while (L + 1 < p.levels && g >= a.offset[L + 1]) ++L;
-    uint64_t local = g - a.offset[L];
+// Probe centre of level L for a probe index (x fastest).
+__device__ __forceinline__ float3 probe_centre(const lux_gi_params& p, uint32_t L, uint64_t probe) {
     uint32_t dx = level_dim(p.dims[0], L), dy = level_dim(p.dims[1], L);
-    uint32_t nphi = p.base_phi << L, ny = p.base_y << L;
-    uint32_t D = nphi * ny;
-    uint32_t k = (uint32_t)(local % D);
-    uint64_t probe = local / D;
     uint32_t ix = (uint32_t)(probe % dx);
     uint32_t iy = (uint32_t)((probe / dx) % dy);
     uint32_t iz = (uint32_t)(probe / ((uint64_t)dx * dy));
     float sl = p.spacing * pow2i((int)L);
-    float3 c = make_float3(p.origin[0] + ((float)ix + 0.5f) * sl, p.origin[1] + ((float)iy + 0.5f) * sl,
-                           p.origin[2] + ((float)iz + 0.5f) * sl);
+    return make_float3(p.origin[0] + ((float)ix + 0.5f) * sl, p.origin[1] + ((float)iy + 0.5f) * sl,
+                       p.origin[2] + ((float)iz + 0.5f) * sl);
+}
+
+// One interval: rays_per_interval jittered rays of bin k from the probe
+// centre c over [start, end) of level L (or [start, inf) when terminal):
+// J = mean shaded radiance, beta = mean transmittance (1 on a miss of a
+// bounded interval), beta = -1 when no ray was supported.
+__device__ float4 trace_interval(const GiArgs& a, uint32_t L, float3 c, uint32_t k, bool terminal,
+                                 Stack& stk) {
+    const lux_gi_params& p = a.p;
+    uint32_t nphi = p.base_phi << L, ny = p.base_y << L;
     float end = interval_end(p, L);
     float start = L == 0 ? 0.0f : interval_end(p, L - 1);
-    bool terminal = p.terminal_tail != 0u && L + 1 == p.levels;
     float limit = terminal ? 1e20f : end - start;
     float3 jsum = make_float3(0.0f, 0.0f, 0.0f);
     float bsum = 0.0f;
@@ -375,14 +372,133 @@ __global__ __launch_bounds__(kWg) void k_gi_trace(GiArgs a) {
         bsum = bsum + beta;
         n++;
     }
-    float4 out;
-    if (n == 0) {
-        out = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-    } else {
-        float fn = (float)n;
-        out = make_float4(jsum.x / fn, jsum.y / fn, jsum.z / fn, bsum / fn);
+    if (n == 0) return make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    float fn = (float)n;
+    return make_float4(jsum.x / fn, jsum.y / fn, jsum.z / fn, bsum / fn);
+}
+
+// Intervals with global index in [g_begin, g_end) (levels are contiguous).
+__global__ __launch_bounds__(kWg) void k_gi_trace(GiArgs a, uint64_t g_begin, uint64_t g_end) {
+    __shared__ uint2 lds[kLdsDepth][kWg];
+    Stack stk;
+    stk.col = &lds[0][threadIdx.x];
+    uint64_t g = g_begin + (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    const lux_gi_params& p = a.p;
+    if (g >= g_end) return;
+    uint32_t L = 0;
+    while (L + 1 < p.levels && g >= a.offset[L + 1]) ++L;
+    uint64_t local = g - a.offset[L];
+    uint32_t D = (p.base_phi << L) * (p.base_y << L);
+    uint32_t k = (uint32_t)(local % D);
+    float3 c = probe_centre(p, L, local / D);
+    bool terminal = p.terminal_tail != 0u && L + 1 == p.levels;
+    a.intervals[g] = trace_interval(a, L, c, k, terminal, stk);
+}
+
+// The merge of one interval with no child contribution (its level is the
+// last one traced): Eq. 7 with an empty child sum.
+__device__ __forceinline__ float4 merge_leaf(float4 iv) {
+    float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (iv.w >= 0.0f) {
+        float tr = sat(iv.w);
+        float support = 1.0f - tr;
+        if (support > 0.0f)
+            out = make_float4(iv.x / support, iv.y / support, iv.z / support, sat(support));
     }
-    a.intervals[g] = out;
+    return out;
+}
+
+// Surrogate query of (probe, bin k) at level L: probe centre, the frame's
+// first jittered ray direction of the bin, normal = that direction.
+__device__ __forceinline__ void level_query(const lux_gi_params& p, uint32_t L, uint64_t local,
+                                            float* o) {
+    uint32_t nphi = p.base_phi << L, ny = p.base_y << L;
+    uint32_t D = nphi * ny;
+    float3 c = probe_centre(p, L, local / D);
+    float2 ouv = ray_offset_uv(p.seed, 0u);
+    float3 d = bin_direction((uint32_t)(local % D), nphi, ny, ouv.x, ouv.y);
+    o[0] = c.x; o[1] = c.y; o[2] = c.z;
+    o[3] = d.x; o[4] = d.y; o[5] = d.z;
+    o[6] = d.x; o[7] = d.y; o[8] = d.z;
+}
+
+__global__ __launch_bounds__(kWg) void k_sur_queries(GiArgs a, uint32_t L, uint64_t count, float* out) {
+    uint64_t i = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (i >= count) return;
+    level_query(a.p, L, i, out + 9ull * i);
+}
+
+// Training samples: (probe, bin) drawn by a multiply-shift of a hash of
+// (seed, i); target = terminal trace of the level or the merged value.
+__global__ __launch_bounds__(kWg) void k_sur_samples(GiArgs a, uint32_t L, uint32_t n, uint32_t seed,
+                                                     uint32_t source, float* out) {
+    __shared__ uint2 lds[kLdsDepth][kWg];
+    Stack stk;
+    stk.col = &lds[0][threadIdx.x];
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    const lux_gi_params& p = a.p;
+    uint64_t count = a.offset[L + 1] - a.offset[L];
+    uint32_t h = transport_hash(i + transport_hash(seed ^ 0x3c6ef372u));
+    uint64_t local = ((uint64_t)h * count) >> 32;
+    float* o = out + 12ull * i;
+    level_query(p, L, local, o);
+    uint32_t D = (p.base_phi << L) * (p.base_y << L);
+    float4 m;
+    if (source == 1u) {
+        m = a.merged[a.offset[L] + local];
+    } else {
+        float3 c = probe_centre(p, L, local / D);
+        m = merge_leaf(trace_interval(a, L, c, (uint32_t)(local % D), true, stk));
+    }
+    bool ok = m.w > 0.0f;
+    float nan = __uint_as_float(0x7fc00000u);
+    o[9] = ok ? m.x : nan;
+    o[10] = ok ? m.y : nan;
+    o[11] = ok ? m.z : nan;
+}
+
+// Network outputs -> merged radiance of level L with support 1.
+__global__ __launch_bounds__(kWg) void k_sur_apply(const float* Y, uint64_t count, float4* merged) {
+    uint64_t i = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (i >= count) return;
+    const float* y = Y + 16ull * i;
+    merged[i] = make_float4(smax(y[0], 0.0f), smax(y[1], 0.0f), smax(y[2], 0.0f), 1.0f);
+}
+
+// One wave per probe: predicted RMSE = sqrt(mean_k max(exp(y3) - 1e-4, 0))
+// in a fixed reduction order; probes above the bound (or NaN) are appended
+// to the fallback list.
+__global__ __launch_bounds__(64) void k_sur_probe_error(const float* Y, uint32_t D, float bound,
+                                                        float* err, uint32_t* list, uint32_t* count) {
+    uint32_t probe = blockIdx.x;
+    const float* y = Y + 16ull * D * probe;
+    float sum = 0.0f;
+    for (uint32_t k = threadIdx.x; k < D; k += 64u) sum = sum + smax(expf(y[16ull * k + 3]) - 1e-4f, 0.0f);
+    for (int off = 32; off > 0; off >>= 1) sum = sum + __shfl_xor(sum, off, 64);
+    if (threadIdx.x == 0) {
+        float e = sqrtf(sum / (float)D);
+        err[probe] = e;
+        if (!(e <= bound)) list[atomicAdd(count, 1u)] = probe;
+    }
+}
+
+// Traced fallback: the listed probes' intervals of level L as the unbounded
+// terminal interval, written merged.  Threads past the listed work exit.
+__global__ __launch_bounds__(kWg) void k_sur_fallback(GiArgs a, uint32_t L, const uint32_t* list,
+                                                      const uint32_t* count) {
+    __shared__ uint2 lds[kLdsDepth][kWg];
+    Stack stk;
+    stk.col = &lds[0][threadIdx.x];
+    const lux_gi_params& p = a.p;
+    uint32_t D = (p.base_phi << L) * (p.base_y << L);
+    uint64_t g = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    uint64_t slot = g / D;
+    if (slot >= *count) return;
+    uint32_t probe = list[slot];
+    uint32_t k = (uint32_t)(g % D);
+    float3 c = probe_centre(p, L, probe);
+    a.merged[a.offset[L] + (uint64_t)probe * D + k] = merge_leaf(trace_interval(a, L, c, k, true, stk));
 }
 
 // Support-weighted trilinear child lookup at level Lc.
@@ -779,6 +895,7 @@ extern "C" int lux_gi_configure(lux_ctx* ctx, const lux_gi_params* params) {
                                hipMemcpyHostToDevice));
     ctx->gi = p;
     ctx->gi_ready = true;
+    ctx->gi_sur_probes = 0;
     return LUX_OK;
 }
 
@@ -789,27 +906,42 @@ extern "C" int lux_gi_set_frame(lux_ctx* ctx, uint32_t frame, uint32_t seed) {
     return LUX_OK;
 }
 
-extern "C" int lux_gi_trace(lux_ctx* ctx) {
+extern "C" int lux_gi_trace_levels(lux_ctx* ctx, uint32_t first, uint32_t last) {
     if (!ctx) return LUX_ERR_INVALID;
     if (int rc = gi_check(ctx)) return rc;
+    if (first > last || last >= ctx->gi.levels)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_trace_levels: level range outside the grid");
     GiArgs a = gi_args(ctx);
-    uint32_t blocks = (uint32_t)((ctx->gi_total + kWg - 1) / kWg);
-    hipLaunchKernelGGL(k_gi_trace, dim3(blocks), dim3(kWg), 0, ctx->stream, a);
+    uint64_t g0 = ctx->gi_offset[first], g1 = ctx->gi_offset[last + 1];
+    uint32_t blocks = (uint32_t)((g1 - g0 + kWg - 1) / kWg);
+    if (blocks) hipLaunchKernelGGL(k_gi_trace, dim3(blocks), dim3(kWg), 0, ctx->stream, a, g0, g1);
     LUX_HIP(ctx, hipGetLastError());
     return LUX_OK;
 }
 
-extern "C" int lux_gi_merge(lux_ctx* ctx) {
+extern "C" int lux_gi_trace(lux_ctx* ctx) {
     if (!ctx) return LUX_ERR_INVALID;
-    if (!ctx->gi_ready) return lux_fail(ctx, LUX_ERR_STATE, "GI not configured (lux_gi_configure)");
+    if (int rc = gi_check(ctx)) return rc;
+    return lux_gi_trace_levels(ctx, 0, ctx->gi.levels - 1);
+}
+
+namespace {
+int merge_levels(lux_ctx* ctx, int top) {
     GiArgs a = gi_args(ctx);
-    for (int L = (int)ctx->gi.levels - 1; L >= 0; --L) {
+    for (int L = top; L >= 0; --L) {
         uint64_t count = ctx->gi_offset[L + 1] - ctx->gi_offset[L];
         uint32_t blocks = (uint32_t)((count + kWg - 1) / kWg);
         hipLaunchKernelGGL(k_gi_merge, dim3(blocks), dim3(kWg), 0, ctx->stream, a, (uint32_t)L);
     }
     LUX_HIP(ctx, hipGetLastError());
     return LUX_OK;
+}
+}  // namespace
+
+extern "C" int lux_gi_merge(lux_ctx* ctx) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!ctx->gi_ready) return lux_fail(ctx, LUX_ERR_STATE, "GI not configured (lux_gi_configure)");
+    return merge_levels(ctx, (int)ctx->gi.levels - 1);
 }
 
 extern "C" int lux_gi_pack(lux_ctx* ctx) {
@@ -888,5 +1020,116 @@ extern "C" int lux_world_resolve_device(lux_ctx* ctx, const float* receivers, ui
     hipLaunchKernelGGL(k_world_resolve, dim3((n + kWg - 1) / kWg), dim3(kWg), 0, ctx->stream, a,
                        receivers, n, reinterpret_cast<float4*>(out));
     LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
+// ------------------------------------------------------------------ surrogate
+namespace {
+uint64_t level_count(const lux_ctx* ctx, uint32_t L) { return ctx->gi_offset[L + 1] - ctx->gi_offset[L]; }
+uint32_t level_directions(const lux_gi_params& p, uint32_t L) { return (p.base_phi << L) * (p.base_y << L); }
+}  // namespace
+
+extern "C" int lux_gi_surrogate_queries(lux_ctx* ctx, uint32_t level, float* queries, uint64_t* count) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!ctx->gi_ready) return lux_fail(ctx, LUX_ERR_STATE, "GI not configured (lux_gi_configure)");
+    if (level >= ctx->gi.levels) return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_surrogate_queries: level");
+    uint64_t n = level_count(ctx, level);
+    if (count) *count = n;
+    if (!queries) return LUX_OK;
+    GiArgs a = gi_args(ctx);
+    hipLaunchKernelGGL(k_sur_queries, dim3((uint32_t)((n + kWg - 1) / kWg)), dim3(kWg), 0, ctx->stream, a,
+                       level, n, queries);
+    LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
+extern "C" int lux_gi_surrogate_samples(lux_ctx* ctx, uint32_t level, uint32_t n, uint32_t sample_seed,
+                                        uint32_t source, float* samples) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (source > 1u) return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_surrogate_samples: source");
+    if (source == 0u) {
+        if (int rc = gi_check(ctx)) return rc;
+    } else if (!ctx->gi_ready) {
+        return lux_fail(ctx, LUX_ERR_STATE, "GI not configured (lux_gi_configure)");
+    }
+    if (level >= ctx->gi.levels) return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_surrogate_samples: level");
+    if (n == 0) return LUX_OK;
+    if (!samples) return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_surrogate_samples: null buffer");
+    GiArgs a = gi_args(ctx);
+    hipLaunchKernelGGL(k_sur_samples, dim3((n + kWg - 1) / kWg), dim3(kWg), 0, ctx->stream, a, level, n,
+                       sample_seed, source, samples);
+    LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
+extern "C" int lux_gi_update_surrogate(lux_ctx* ctx, lux_surrogate* s, uint32_t level, float error_bound,
+                                       lux_gi_surrogate_stats* stats) {
+    if (!ctx || !s) return LUX_ERR_INVALID;
+    if (lux_surrogate_context(s) != ctx)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_update_surrogate: surrogate of another context");
+    if (int rc = gi_check(ctx)) return rc;
+    const lux_gi_params& p = ctx->gi;
+    if (level < 1 || level >= p.levels)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_update_surrogate: level must be in 1..levels-1");
+    uint64_t count = level_count(ctx, level);
+    uint32_t D = level_directions(p, level);
+    uint64_t probes = count / D;
+    if (count >= (1ull << 31)) return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_update_surrogate: level too large");
+    if (int rc = lux_ensure(ctx, ctx->gi_sur_queries, 36ull * count)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->gi_sur_err, 4ull * probes)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->gi_sur_list, 4ull * (probes + 1))) return rc;
+    hipStream_t st = ctx->stream;
+    GiArgs a = gi_args(ctx);
+    float4* merged = a.merged + ctx->gi_offset[level];
+    uint32_t* list = static_cast<uint32_t*>(ctx->gi_sur_list.ptr);
+    uint32_t* fcount = list + probes;
+    float* queries = static_cast<float*>(ctx->gi_sur_queries.ptr);
+
+    LUX_HIP(ctx, hipEventRecord(ctx->ev_phase[0], st));
+    if (int rc = lux_gi_trace_levels(ctx, 0, level - 1)) return rc;
+    LUX_HIP(ctx, hipEventRecord(ctx->ev_phase[1], st));
+    uint32_t qblocks = (uint32_t)((count + kWg - 1) / kWg);
+    hipLaunchKernelGGL(k_sur_queries, dim3(qblocks), dim3(kWg), 0, st, a, level, count, queries);
+    const float* Y = nullptr;
+    if (int rc = lux_surrogate_forward_internal(s, queries, (uint32_t)count, &Y)) return rc;
+    hipLaunchKernelGGL(k_sur_apply, dim3(qblocks), dim3(kWg), 0, st, Y, count, merged);
+    if (int rc = lux_device_zero(ctx, fcount, 4)) return rc;
+    hipLaunchKernelGGL(k_sur_probe_error, dim3((uint32_t)probes), dim3(64), 0, st, Y, D, error_bound,
+                       static_cast<float*>(ctx->gi_sur_err.ptr), list, fcount);
+    LUX_HIP(ctx, hipGetLastError());
+    LUX_HIP(ctx, hipEventRecord(ctx->ev_phase[2], st));
+    hipLaunchKernelGGL(k_sur_fallback, dim3(qblocks), dim3(kWg), 0, st, a, level, list,
+                       static_cast<const uint32_t*>(fcount));
+    LUX_HIP(ctx, hipGetLastError());
+    LUX_HIP(ctx, hipEventRecord(ctx->ev_phase[3], st));
+    if (int rc = merge_levels(ctx, (int)level - 1)) return rc;
+    if (int rc = lux_gi_pack(ctx)) return rc;
+    LUX_HIP(ctx, hipEventRecord(ctx->ev_phase[4], st));
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        uint32_t nf = 0;
+        LUX_HIP(ctx, hipMemcpyAsync(&nf, fcount, 4, hipMemcpyDeviceToHost, st));
+        LUX_HIP(ctx, hipStreamSynchronize(st));
+        stats->level = level;
+        stats->probes = (uint32_t)probes;
+        stats->fallback_probes = nf;
+        stats->queries = (uint32_t)count;
+        LUX_HIP(ctx, hipEventElapsedTime(&stats->ms_total, ctx->ev_phase[0], ctx->ev_phase[4]));
+        LUX_HIP(ctx, hipEventElapsedTime(&stats->ms_trace, ctx->ev_phase[0], ctx->ev_phase[1]));
+        LUX_HIP(ctx, hipEventElapsedTime(&stats->ms_infer, ctx->ev_phase[1], ctx->ev_phase[2]));
+        LUX_HIP(ctx, hipEventElapsedTime(&stats->ms_fallback, ctx->ev_phase[2], ctx->ev_phase[3]));
+        LUX_HIP(ctx, hipEventElapsedTime(&stats->ms_merge_pack, ctx->ev_phase[3], ctx->ev_phase[4]));
+    }
+    ctx->gi_sur_probes = (uint32_t)probes;
+    return LUX_OK;
+}
+
+extern "C" int lux_gi_surrogate_probe_error(lux_ctx* ctx, float* out, uint32_t count) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (ctx->gi_sur_probes == 0) return lux_fail(ctx, LUX_ERR_STATE, "no surrogate update has run");
+    if (!out || count != ctx->gi_sur_probes)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_surrogate_probe_error: count must equal the level's probes");
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LUX_HIP(ctx, hipMemcpy(out, ctx->gi_sur_err.ptr, 4ull * count, hipMemcpyDeviceToHost));
     return LUX_OK;
 }

@@ -103,6 +103,60 @@ class RadianceCascades(C.Structure):
         return p
 
 
+class SurrogateParams(C.Structure):
+    """lux_surrogate_params (include/lux.h)."""
+    _fields_ = [
+        ("levels", C.c_uint32),
+        ("features", C.c_uint32),
+        ("log2_table", C.c_uint32),
+        ("base_resolution", C.c_float),
+        ("growth", C.c_float),
+        ("box_min", C.c_float * 3),
+        ("box_max", C.c_float * 3),
+        ("learning_rate", C.c_float),
+        ("error_weight", C.c_float),
+        ("seed", C.c_uint32),
+        ("freeze_updates", C.c_uint32),
+        ("ema_decay", C.c_float),
+        ("table_learning_rate", C.c_float),
+        ("reserved", C.c_uint32 * 3),
+    ]
+
+    @classmethod
+    def default(cls, box_min=(0.0, 0.0, 0.0), box_max=(1.0, 1.0, 1.0), **kw) -> "SurrogateParams":
+        p = cls()
+        load_library().lux_surrogate_default_params(C.byref(p))
+        p.box_min[:] = [float(v) for v in box_min]
+        p.box_max[:] = [float(v) for v in box_max]
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+class SurrogateStats(C.Structure):
+    """lux_gi_surrogate_stats (include/lux.h)."""
+    _fields_ = [
+        ("level", C.c_uint32),
+        ("probes", C.c_uint32),
+        ("fallback_probes", C.c_uint32),
+        ("queries", C.c_uint32),
+        ("ms_total", C.c_float),
+        ("ms_trace", C.c_float),
+        ("ms_infer", C.c_float),
+        ("ms_fallback", C.c_float),
+        ("ms_merge_pack", C.c_float),
+        ("reserved", C.c_float * 3),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: (getattr(self, k) if not isinstance(getattr(self, k), float) else round(getattr(self, k), 4))
+                for k, _ in self._fields_ if k != "reserved"}
+
+
+LUX_SAMPLES_TRACE = 0
+LUX_SAMPLES_MERGED = 1
+
+
 class InstanceDesc(C.Structure):
     _fields_ = [
         ("local_from_world", C.c_float * 12),
@@ -166,6 +220,22 @@ def load_library() -> C.CDLL:
         "lux_radiance_cascades_low_end": (i32, [vp]),
         "lux_radiance_cascades_sanitized": (i32, [vp, vp]),
         "lux_radiance_cascades_gi_params": (i32, [vp, vp, vp, u32, u32, u32, u32, u32, vp]),
+        "lux_gi_trace_levels": (i32, [vp, u32, u32]),
+        "lux_surrogate_default_params": (None, [vp]),
+        "lux_surrogate_create": (i32, [vp, vp, C.POINTER(vp)]),
+        "lux_surrogate_destroy": (None, [vp]),
+        "lux_surrogate_reset": (i32, [vp]),
+        "lux_surrogate_encode": (i32, [vp, vp, u32, vp]),
+        "lux_surrogate_infer": (i32, [vp, vp, u32, vp]),
+        "lux_surrogate_train_step": (i32, [vp, vp, u32, vp]),
+        "lux_surrogate_zero_grad": (i32, [vp]),
+        "lux_surrogate_sizes": (i32, [vp, vp, vp, vp]),
+        "lux_surrogate_download": (i32, [vp, vp, vp, vp, vp]),
+        "lux_surrogate_upload": (i32, [vp, vp, vp]),
+        "lux_gi_surrogate_samples": (i32, [vp, u32, u32, u32, u32, vp]),
+        "lux_gi_surrogate_queries": (i32, [vp, u32, vp, vp]),
+        "lux_gi_update_surrogate": (i32, [vp, vp, u32, C.c_float, vp]),
+        "lux_gi_surrogate_probe_error": (i32, [vp, vp, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -433,3 +503,108 @@ class Context:
     def world_resolve_device(self, receivers, n: int, out):
         self._check(self.lib.lux_world_resolve_device(self.h, _ptr(receivers), n, _ptr(out)),
                     "lux_world_resolve_device")
+
+    # ---- learned far-cascade surrogate ----
+    def gi_trace_levels(self, first: int, last: int):
+        self._check(self.lib.lux_gi_trace_levels(self.h, first, last), "lux_gi_trace_levels")
+
+    def gi_surrogate_samples(self, level: int, n: int, sample_seed: int, source: int, out):
+        self._check(self.lib.lux_gi_surrogate_samples(self.h, level, n, sample_seed, source, _ptr(out)),
+                    "lux_gi_surrogate_samples")
+
+    def gi_surrogate_query_count(self, level: int) -> int:
+        c = C.c_uint64()
+        self._check(self.lib.lux_gi_surrogate_queries(self.h, level, None, C.byref(c)),
+                    "lux_gi_surrogate_queries")
+        return int(c.value)
+
+    def gi_surrogate_queries(self, level: int, out):
+        self._check(self.lib.lux_gi_surrogate_queries(self.h, level, _ptr(out), None),
+                    "lux_gi_surrogate_queries")
+
+    def gi_update_surrogate(self, surrogate: "Surrogate", level: int, error_bound: float,
+                            stats: bool = True):
+        st = SurrogateStats()
+        self._check(self.lib.lux_gi_update_surrogate(self.h, surrogate.h, level, float(error_bound),
+                                                     C.byref(st) if stats else None),
+                    "lux_gi_update_surrogate")
+        return st if stats else None
+
+    def gi_surrogate_probe_error(self, probes: int) -> np.ndarray:
+        out = np.zeros(probes, np.float32)
+        self._check(self.lib.lux_gi_surrogate_probe_error(self.h, out.ctypes.data, probes),
+                    "lux_gi_surrogate_probe_error")
+        return out
+
+
+class Surrogate:
+    """Hash-grid + MLP radiance surrogate owned by a Context (lux_surrogate)."""
+
+    def __init__(self, ctx: Context, params: SurrogateParams):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.params = params
+        h = C.c_void_p()
+        ctx._check(self.lib.lux_surrogate_create(ctx.h, C.byref(params), C.byref(h)), "lux_surrogate_create")
+        self.h = h
+        m, t = C.c_uint64(), C.c_uint64()
+        self.lib.lux_surrogate_sizes(self.h, C.byref(m), C.byref(t), None)
+        self.mlp_params, self.table_params = int(m.value), int(t.value)
+
+    def close(self):
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
+            self.lib.lux_surrogate_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def steps(self) -> int:
+        s = C.c_uint32()
+        self.lib.lux_surrogate_sizes(self.h, None, None, C.byref(s))
+        return int(s.value)
+
+    def reset(self):
+        self.ctx._check(self.lib.lux_surrogate_reset(self.h), "lux_surrogate_reset")
+
+    def encode(self, queries, n: int, out):
+        self.ctx._check(self.lib.lux_surrogate_encode(self.h, _ptr(queries), n, _ptr(out)),
+                        "lux_surrogate_encode")
+
+    def infer(self, queries, n: int, out):
+        self.ctx._check(self.lib.lux_surrogate_infer(self.h, _ptr(queries), n, _ptr(out)),
+                        "lux_surrogate_infer")
+
+    def train_step(self, samples, n: int, want_loss: bool = True):
+        loss = C.c_float()
+        self.ctx._check(self.lib.lux_surrogate_train_step(self.h, _ptr(samples), n,
+                                                          C.byref(loss) if want_loss else None),
+                        "lux_surrogate_train_step")
+        return float(loss.value) if want_loss else None
+
+    def zero_grad(self):
+        self.ctx._check(self.lib.lux_surrogate_zero_grad(self.h), "lux_surrogate_zero_grad")
+
+    def download(self, grads: bool = False, table: bool = True) -> dict:
+        mlp = np.zeros(self.mlp_params, np.float32)
+        mg = np.zeros(self.mlp_params, np.float32) if grads else None
+        tb = np.zeros(self.table_params, np.float32) if table else None
+        tg = np.zeros(self.table_params, np.float32) if (grads and table) else None
+        self.ctx._check(self.lib.lux_surrogate_download(self.h, _ptr(mlp), _ptr(mg), _ptr(tb), _ptr(tg)),
+                        "lux_surrogate_download")
+        out = {"mlp": mlp}
+        if grads:
+            out["mlp_grad"] = mg
+        if table:
+            out["table"] = tb.reshape(16, -1, 2)
+            if grads:
+                out["table_grad"] = tg.reshape(16, -1, 2)
+        return out
+
+    def upload(self, mlp=None, table=None):
+        m = None if mlp is None else np.ascontiguousarray(mlp, np.float32)
+        t = None if table is None else np.ascontiguousarray(table, np.float32)
+        self.ctx._check(self.lib.lux_surrogate_upload(self.h, _ptr(m), _ptr(t)), "lux_surrogate_upload")

@@ -235,6 +235,103 @@ int lux_gi_device_arrays(lux_ctx* ctx, float** intervals, float** merged, float*
  * code (0 not eligible, -1 unsupported, 1 + 0.25 coverage). */
 int lux_world_resolve_device(lux_ctx* ctx, const float* receivers, uint32_t n, float* out);
 
+/* Trace only levels first..last (inclusive) of the configured grid into the
+ * interval array; lux_gi_trace is first = 0, last = levels - 1. */
+int lux_gi_trace_levels(lux_ctx* ctx, uint32_t first, uint32_t last);
+
+/* ---- learned far-cascade surrogate.  No counterpart in the reference: its
+ *      README defers online neural radiance caching (README.md:384-385).
+ *      A multiresolution hash-grid encoder (16 levels x 2 FP16 features,
+ *      2^19 entries per level, base resolution 16, growth 1.5), 8-band real
+ *      SH of the direction and the normal feed an MLP 128-64-64-16 (ReLU,
+ *      BF16 weights, MFMA with FP32 accumulation).  Outputs 0-2 radiance,
+ *      output 3 log of the predicted squared error. ---- */
+typedef struct lux_surrogate lux_surrogate;
+typedef struct {
+    uint32_t levels;          /* hash levels: 16 (the only supported value) */
+    uint32_t features;        /* features per level: 2 */
+    uint32_t log2_table;      /* log2 entries per level: 19 */
+    float base_resolution;    /* level-0 grid resolution, 16 */
+    float growth;             /* per-level resolution factor, 1.5 */
+    float box_min[3];         /* encoder domain (world units); positions are clamped to it */
+    float box_max[3];
+    float learning_rate;      /* Adam step, 1e-2 (beta 0.9 / 0.99, eps 1e-15) */
+    float error_weight;       /* loss weight of the error head, 0.1 */
+    uint32_t seed;            /* initialisation seed */
+    uint32_t freeze_updates;  /* 1: accumulate gradients only, no Adam step */
+    float ema_decay;          /* inference reads an exponential moving average of the
+                                 parameters with this decay (0.95); 0 = the parameters */
+    float table_learning_rate; /* Adam step of the hash table; 0 = learning_rate */
+    uint32_t reserved[3];
+} lux_surrogate_params;
+
+void lux_surrogate_default_params(lux_surrogate_params* params);
+int lux_surrogate_create(lux_ctx* ctx, const lux_surrogate_params* params, lux_surrogate** out);
+void lux_surrogate_destroy(lux_surrogate* s);
+/* Re-initialise parameters and optimiser state from the seed. */
+int lux_surrogate_reset(lux_surrogate* s);
+/* Device buffers.  queries: 9 floats each (position, unit direction, normal).
+ * encode writes the BF16 bit patterns of the 128 network inputs per query;
+ * infer writes 16 floats per query.  Both read the inference parameters
+ * (the moving average when ema_decay > 0). */
+int lux_surrogate_encode(lux_surrogate* s, const float* queries, uint32_t n, uint16_t* inputs);
+int lux_surrogate_infer(lux_surrogate* s, const float* queries, uint32_t n, float* outputs);
+/* One optimisation step over device samples of 12 floats (query + target
+ * radiance; a non-finite target is skipped).  *loss (host, nullable) receives
+ * the step's loss and makes the call synchronous. */
+int lux_surrogate_train_step(lux_surrogate* s, const float* samples, uint32_t n, float* loss);
+int lux_surrogate_zero_grad(lux_surrogate* s);
+int lux_surrogate_sizes(lux_surrogate* s, uint64_t* mlp_params, uint64_t* table_params,
+                        uint32_t* steps);
+/* Host copies (each nullable): MLP weights W1 (128x64) | W2 (64x64) |
+ * W3 (64x16), [in][out] row-major, their gradients, the FP32 table
+ * [16][2^19][2] and its gradient. */
+int lux_surrogate_download(lux_surrogate* s, float* mlp, float* mlp_grad, float* table,
+                           float* table_grad);
+int lux_surrogate_upload(lux_surrogate* s, const float* mlp, const float* table);
+
+/* Training samples from level `level` of the configured grid (device float
+ * [12 * n]): n (probe, direction) pairs drawn by hashing (sample_seed, i);
+ * query = probe centre, the frame's jittered bin direction, normal = that
+ * direction; target = the radiance beyond the level's interval start, either
+ * traced now as the level's unbounded terminal interval (source
+ * LUX_SAMPLES_TRACE) or read from the merged array of the last plain update
+ * (LUX_SAMPLES_MERGED).  Unsupported targets are NaN. */
+#define LUX_SAMPLES_TRACE 0u
+#define LUX_SAMPLES_MERGED 1u
+int lux_gi_surrogate_samples(lux_ctx* ctx, uint32_t level, uint32_t n, uint32_t sample_seed,
+                             uint32_t source, float* samples);
+/* The queries of every (probe, direction) of `level` in merged-array order
+ * (device float[9 * count]); *count (nullable) = intervals of the level. */
+int lux_gi_surrogate_queries(lux_ctx* ctx, uint32_t level, float* queries, uint64_t* count);
+
+typedef struct {
+    uint32_t level;            /* first replaced cascade */
+    uint32_t probes;           /* probes of that level */
+    uint32_t fallback_probes;  /* probes traced because of the error bound */
+    uint32_t queries;          /* network queries */
+    float ms_total;            /* GPU time of the whole update */
+    float ms_trace;            /* levels below `level` */
+    float ms_infer;            /* queries, network, predicted error */
+    float ms_fallback;         /* traced fallback probes */
+    float ms_merge_pack;       /* merge of the traced levels + packing */
+    float reserved[3];
+} lux_gi_surrogate_stats;
+
+/* Surrogate-assisted GI update: trace levels 0..level-1; predict merged
+ * radiance of every (probe, direction) of `level` with support 1; a probe
+ * whose predicted radiance RMSE over its directions exceeds error_bound
+ * (radiance units; negative: every probe, +inf: none) is traced instead as
+ * the unbounded terminal interval of `level`; merge levels level-1..0 and
+ * pack.  Levels above `level` are neither traced nor merged.  With every
+ * probe traced the result equals the plain update of the same grid with
+ * levels = level + 1 and terminal_tail = 1.  stats nullable; when given the
+ * call synchronizes. */
+int lux_gi_update_surrogate(lux_ctx* ctx, lux_surrogate* s, uint32_t level, float error_bound,
+                            lux_gi_surrogate_stats* stats);
+/* Per-probe predicted RMSE of the last surrogate update (host float[probes]). */
+int lux_gi_surrogate_probe_error(lux_ctx* ctx, float* out, uint32_t count);
+
 /* Kernel timing of the last device trace (HIP events on the context stream). */
 int lux_last_kernel_ms(lux_ctx* ctx, float* ms);
 

@@ -192,6 +192,10 @@ uint32_t orc_build_lights(const float* const* mesh_pos, const uint32_t* const* m
                           const uint32_t* mesh_tri_count, uint32_t n_meshes, const float* inst_fwd,
                           const uint32_t* inst_mesh, const uint32_t* inst_flags, uint32_t n_inst,
                           const orc_material* mats, uint32_t n_mats, orc_light* out, uint32_t cap);
+/* Surrogate queries of a level (float[9 * intervals of the level]) and the
+ * level-local index of training sample i. */
+void orc_gi_level_queries(const orc_gi_params* p, uint32_t level, float* out);
+uint64_t orc_gi_sample_index(uint32_t seed, uint32_t i, uint64_t count);
 void orc_bin_direction(uint32_t index, uint32_t nphi, uint32_t ny, float ox, float oy,
                        float out[3]);
 void orc_cos_sin_2pi(float u, float* c, float* s);

@@ -241,6 +241,8 @@ def _tsig():
         ("orc_gi_pack_f64", None, [vp, vp, vp]),
         ("orc_world_resolve", None, [vp, vp, vp, u32, vp]),
         ("orc_world_resolve_f64", None, [vp, vp, vp, u32, vp]),
+        ("orc_gi_level_queries", None, [vp, u32, vp]),
+        ("orc_gi_sample_index", C.c_uint64, [u32, u32, C.c_uint64]),
     ]:
         fn = getattr(lib, name)
         fn.restype = res
@@ -270,6 +272,21 @@ def interval_bounds(p: GiParams, level: int):
 
 def total_intervals(p: GiParams) -> int:
     return level_offset(p, p.levels)
+
+
+def level_queries(p: GiParams, level: int) -> np.ndarray:
+    """Surrogate queries (position, direction, normal) of every (probe, bin)
+    of a level, in merged-array order."""
+    lib = _tsig()
+    n = level_offset(p, level + 1) - level_offset(p, level)
+    out = np.zeros((n, 9), np.float32)
+    lib.orc_gi_level_queries(C.byref(p), level, _p(out))
+    return out
+
+
+def sample_index(seed: int, i, count: int) -> np.ndarray:
+    lib = _tsig()
+    return np.array([lib.orc_gi_sample_index(seed, int(k), count) for k in np.atleast_1d(i)], np.int64)
 
 
 def bin_direction(index, nphi, ny, ox=0.5, oy=0.5):

@@ -369,3 +369,38 @@ void orc_world_resolve_f64(const orc_gi_params* p, const double* probes, const f
                            uint32_t n, double* out) {
     world_resolve_f64(p, probes, receivers, n, out);
 }
+
+/* Surrogate query of every (probe, bin) of a level, in merged-array order:
+ * probe centre, the update's first jittered ray direction of the bin, and
+ * that direction again as the normal channel (9 floats each). */
+void orc_gi_level_queries(const orc_gi_params* p, uint32_t level, float* out) {
+    uint32_t dims[3];
+    uint64_t probes = orc_gi_level_dims(p, level, dims);
+    uint32_t D = orc_gi_level_directions(p, level);
+    uint32_t nphi = p->base_phi << level, ny = p->base_y << level;
+    float sl = p->spacing * ldexpf(1.0f, (int)level);
+    float ouv[2];
+    ray_offset_uv(p->seed, 0u, ouv);
+    for (uint64_t probe = 0; probe < probes; ++probe) {
+        uint32_t ix = (uint32_t)(probe % dims[0]);
+        uint32_t iy = (uint32_t)((probe / dims[0]) % dims[1]);
+        uint32_t iz = (uint32_t)(probe / ((uint64_t)dims[0] * dims[1]));
+        float c[3] = {p->origin[0] + ((float)ix + 0.5f) * sl, p->origin[1] + ((float)iy + 0.5f) * sl,
+                      p->origin[2] + ((float)iz + 0.5f) * sl};
+        for (uint32_t k = 0; k < D; ++k) {
+            float d[3];
+            orc_bin_direction(k, nphi, ny, ouv[0], ouv[1], d);
+            float* o = out + 9 * (probe * D + k);
+            o[0] = c[0]; o[1] = c[1]; o[2] = c[2];
+            o[3] = d[0]; o[4] = d[1]; o[5] = d[2];
+            o[6] = d[0]; o[7] = d[1]; o[8] = d[2];
+        }
+    }
+}
+
+/* Index within its level of training sample i: a multiply-shift of
+ * transportHash(i + transportHash(seed ^ 0x3c6ef372)) onto [0, count). */
+uint64_t orc_gi_sample_index(uint32_t seed, uint32_t i, uint64_t count) {
+    uint32_t h = transport_hash(i + transport_hash(seed ^ 0x3c6ef372u));
+    return ((uint64_t)h * count) >> 32;
+}
