@@ -174,7 +174,8 @@ extern "C" void lux_ctx_destroy(lux_ctx* ctx) {
                           &ctx->tlas, &ctx->inst, &ctx->inst_order, &ctx->scratch,
                           &ctx->ray_buf, &ctx->hit_buf, &ctx->gi_intervals, &ctx->gi_merged,
                           &ctx->gi_probes, &ctx->gi_light_buf, &ctx->gi_sur_queries,
-                          &ctx->gi_sur_err, &ctx->gi_sur_list};
+                          &ctx->gi_sur_err, &ctx->gi_sur_list, &ctx->gi_counters,
+                          &ctx->tlas_tmp, &ctx->mesh_info};
     for (lux_buffer* b : bufs) lux_release(*b);
     for (hipEvent_t e : ctx->ev_phase)
         if (e) (void)hipEventDestroy(e);
@@ -299,6 +300,7 @@ extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_wo
         LUX_HIP(ctx, hipMemcpy(ctx->inst_desc.ptr, idesc.data(), sizeof(InstDesc) * instance_count,
                                hipMemcpyHostToDevice));
     ctx->tlas_built = false;
+    ctx->gi_lights_dirty = true;
     return LUX_OK;
 }
 
@@ -310,12 +312,19 @@ static int build_blas(lux_ctx* ctx) {
     if (int rc = lux_ensure(ctx, ctx->triC, 16ull * nt)) return rc;
     if (int rc = lux_ensure(ctx, ctx->tri_order, 4ull * nt)) return rc;
     if (int rc = lux_ensure(ctx, ctx->blas, 32ull * ctx->n_blas)) return rc;
+    // per-mesh info for the TLAS build (BLAS base | triangle base | count)
+    std::vector<uint32_t> info;
+    info.insert(info.end(), ctx->mesh_blas_base.begin(), ctx->mesh_blas_base.end());
+    info.insert(info.end(), ctx->mesh_tri_base.begin(), ctx->mesh_tri_base.end());
+    info.insert(info.end(), ctx->mesh_tri_count.begin(), ctx->mesh_tri_count.end());
+    if (int rc = lux_ensure(ctx, ctx->mesh_info, 4ull * info.size() + 16)) return rc;
+    LUX_HIP(ctx, hipStreamSynchronize(st));
+    LUX_HIP(ctx, hipMemcpy(ctx->mesh_info.ptr, info.data(), 4ull * info.size(), hipMemcpyHostToDevice));
     // per-mesh temporaries: boxes + perm (largest mesh)
     uint32_t maxn = 1;
     for (uint32_t n : ctx->mesh_tri_count) maxn = n > maxn ? n : maxn;
-    lux_buffer tmp;
-    if (hipMalloc(&tmp.ptr, 36ull * maxn + 64) != hipSuccess)
-        return lux_fail(ctx, LUX_ERR_OOM, "build_blas: temporary allocation failed");
+    lux_buffer& tmp = ctx->tlas_tmp;
+    if (int rc = lux_ensure(ctx, tmp, 36ull * maxn + 64)) return rc;
     float4* lo = static_cast<float4*>(tmp.ptr);
     float4* hi = lo + maxn;
     uint32_t* perm = reinterpret_cast<uint32_t*>(hi + maxn);
@@ -337,7 +346,6 @@ static int build_blas(lux_ctx* ctx) {
         if (e != hipSuccess) rc = lux_hip_check(ctx, e, "build_blas launch");
     }
     (void)hipStreamSynchronize(st);
-    (void)hipFree(tmp.ptr);
     if (rc == LUX_OK) ctx->blas_built = true;
     return rc;
 }
@@ -353,28 +361,18 @@ static int build_tlas(lux_ctx* ctx) {
         ctx->tlas_built = true;
         return LUX_OK;
     }
-    size_t mbytes = 12ull * ctx->n_meshes;
-    lux_buffer tmp;
-    if (hipMalloc(&tmp.ptr, 36ull * ni + mbytes + 64) != hipSuccess)
-        return lux_fail(ctx, LUX_ERR_OOM, "build_tlas: temporary allocation failed");
-    float4* lo = static_cast<float4*>(tmp.ptr);
+    // persistent temporaries (a rigid-motion frame rebuilds the TLAS every
+    // frame: no allocation and no host wait here)
+    if (int rc = lux_ensure(ctx, ctx->tlas_tmp, 36ull * ni + 64)) return rc;
+    float4* lo = static_cast<float4*>(ctx->tlas_tmp.ptr);
     float4* hi = lo + ni;
     uint32_t* perm = reinterpret_cast<uint32_t*>(hi + ni);
-    uint32_t* mb = perm + ni;
-    uint32_t* mt = mb + ctx->n_meshes;
-    uint32_t* mc = mt + ctx->n_meshes;
+    const uint32_t* mb = static_cast<const uint32_t*>(ctx->mesh_info.ptr);
+    const uint32_t* mt = mb + ctx->n_meshes;
+    const uint32_t* mc = mt + ctx->n_meshes;
     int rc = LUX_OK;
     do {
         hipError_t e;
-        if ((e = hipMemcpyAsync(mb, ctx->mesh_blas_base.data(), 4ull * ctx->n_meshes,
-                                hipMemcpyHostToDevice, st)) != hipSuccess ||
-            (e = hipMemcpyAsync(mt, ctx->mesh_tri_base.data(), 4ull * ctx->n_meshes,
-                                hipMemcpyHostToDevice, st)) != hipSuccess ||
-            (e = hipMemcpyAsync(mc, ctx->mesh_tri_count.data(), 4ull * ctx->n_meshes,
-                                hipMemcpyHostToDevice, st)) != hipSuccess) {
-            rc = lux_hip_check(ctx, e, "build_tlas copy");
-            break;
-        }
         const InstDesc* d = static_cast<const InstDesc*>(ctx->inst_desc.ptr);
         hipLaunchKernelGGL(k_inst_boxes, dim3(nblk(ni, 128)), dim3(128), 0, st, d, ni,
                            static_cast<const float4*>(ctx->blas.ptr), mb, lo, hi);
@@ -386,15 +384,15 @@ static int build_tlas(lux_ctx* ctx) {
                            static_cast<uint32_t*>(ctx->inst_order.ptr));
         if ((e = hipGetLastError()) != hipSuccess) rc = lux_hip_check(ctx, e, "build_tlas launch");
     } while (0);
-    (void)hipStreamSynchronize(st);
-    (void)hipFree(tmp.ptr);
     if (rc == LUX_OK) ctx->tlas_built = true;
     return rc;
 }
 
 extern "C" int lux_bvh_build(lux_ctx* ctx, uint32_t mode) {
     if (!ctx) return LUX_ERR_INVALID;
-    if (mode != LUX_BVH_MORTON30) return lux_fail(ctx, LUX_ERR_INVALID, "lux_bvh_build: unknown mode");
+    if ((mode & ~LUX_BVH_FORCE_BLAS) != LUX_BVH_MORTON30)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_bvh_build: unknown mode");
+    if (mode & LUX_BVH_FORCE_BLAS) ctx->blas_built = false;
     if (ctx->n_meshes == 0 && ctx->n_inst == 0)
         return lux_fail(ctx, LUX_ERR_STATE, "lux_bvh_build: no scene uploaded");
     LUX_HIP(ctx, hipSetDevice(ctx->device));

@@ -59,6 +59,10 @@ struct lux_ctx {
     // fallback probe list + count
     lux_buffer gi_sur_queries, gi_sur_err, gi_sur_list;
     uint32_t gi_sur_probes = 0;
+    lux_buffer gi_counters;  // traversal query attempts, admitted (per update)
+    bool gi_lights_dirty = false;  // instance transforms changed since the emitter table
+    lux_buffer tlas_tmp;     // TLAS build temporaries (instance boxes, permutation)
+    lux_buffer mesh_info;    // uint32 per mesh: BLAS base | triangle base | count
     hipEvent_t ev_phase[5] = {};
 
     // trace tuning

@@ -46,7 +46,33 @@ struct GiArgs {
     float4* intervals;
     float4* merged;
     float4* probes;
+    uint32_t* counters;     // [0] attempts, [1] admitted
+    uint32_t admit_limit;   // 0 = unbounded
 };
+
+// Ticketed per-update traversal budget (admitTransportQuery,
+// shaders/traversal.slang:20-38), aggregated per wave: the active lanes take
+// consecutive tickets with one atomic, lanes past the limit are denied and
+// their tickets returned, so the admitted count never exceeds the limit and
+// ends exact.  Attempts are always counted.
+__device__ bool admit_query(const GiArgs& a) {
+    uint64_t mask = __ballot(1);
+    uint32_t lane = __lane_id();
+    uint32_t leader = (uint32_t)(__ffsll((unsigned long long)mask) - 1);
+    uint32_t count = (uint32_t)__popcll(mask);
+    uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (lane == leader) {
+        atomicAdd(&a.counters[0], count);
+        base = atomicAdd(&a.counters[1], count);
+    }
+    if (a.admit_limit == 0u) return true;
+    base = __shfl(base, (int)leader, 64);
+    bool ok = (uint64_t)base + rank < (uint64_t)a.admit_limit;
+    uint64_t denied = __ballot(!ok);
+    if (lane == leader && denied) atomicSub(&a.counters[1], (uint32_t)__popcll(denied));
+    return ok;
+}
 
 __device__ __forceinline__ float smax(float a, float b) { return b > a ? b : a; }
 __device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
@@ -298,6 +324,7 @@ __device__ bool shade(const GiArgs& a, float3 o, float3 d, const lux::HitState& 
                 if (dist <= bias * 0.25f) continue;
                 float3 dir = make_float3(delta.x / dist, delta.y / dist, delta.z / dist);
                 float lim = dist - smin(bias * 0.25f, dist * 0.001f);
+                if (!admit_query(a)) return false;
                 lux::HitState vh;
                 lux::NoStats ns;
                 lux::trace_occluded(s, origin, dir, lim, vh, stk, ns);
@@ -356,6 +383,7 @@ __device__ float4 trace_interval(const GiArgs& a, uint32_t L, float3 c, uint32_t
         float2 ouv = ray_offset_uv(p.seed, r);
         float3 d = bin_direction(k, nphi, ny, ouv.x, ouv.y);
         float3 o = make_float3(c.x + d.x * start, c.y + d.y * start, c.z + d.z * start);
+        if (!admit_query(a)) continue;  // denied: no sample, as an unsupported traversal
         lux::HitState h;
         lux::NoStats ns;
         lux::trace_closest(a.scene, o, d, limit, h, stk, ns, false);
@@ -748,6 +776,49 @@ __global__ __launch_bounds__(kWg) void k_world_resolve(GiArgs a, const float* re
     out[i] = make_float4(res.x, res.y, res.z, supported ? 1.0f + 0.25f * coverage : -1.0f);
 }
 
+struct Camera {
+    float o[3], f[3], r[3], u[3];
+};
+
+// Receivers of a pinhole camera (the depth/normal prepass the reference
+// resolves against): one closest-hit ray per pixel, row-major with y down;
+// receiver = hit position and the triangle's geometric normal turned toward
+// the camera, or a zero normal (not eligible) on a miss or an unsupported
+// traversal.
+__global__ __launch_bounds__(kWg) void k_camera_receivers(lux::DevScene scene, Camera cam, uint32_t w,
+                                                          uint32_t h, float* out) {
+    __shared__ uint2 lds[kLdsDepth][kWg];
+    Stack stk;
+    stk.col = &lds[0][threadIdx.x];
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= w * h) return;
+    uint32_t x = i % w, y = i / w;
+    float u = ((float)x + 0.5f) / (float)w * 2.0f - 1.0f;
+    float v = 1.0f - ((float)y + 0.5f) / (float)h * 2.0f;
+    float3 d = make_float3((cam.f[0] + u * cam.r[0]) + v * cam.u[0], (cam.f[1] + u * cam.r[1]) + v * cam.u[1],
+                           (cam.f[2] + u * cam.r[2]) + v * cam.u[2]);
+    float len = sqrtf(dot3(d, d));
+    d = make_float3(d.x / len, d.y / len, d.z / len);
+    float3 o = make_float3(cam.o[0], cam.o[1], cam.o[2]);
+    lux::HitState hs;
+    lux::NoStats ns;
+    lux::trace_closest(scene, o, d, 1e20f, hs, stk, ns, false);
+    float* r = out + 6ull * i;
+    if (hs.tri == lux::kEmpty || (hs.err & lux::kErrUnsupported)) {
+        r[0] = o.x; r[1] = o.y; r[2] = o.z;
+        r[3] = 0.0f; r[4] = 0.0f; r[5] = 0.0f;
+        return;
+    }
+    const float4* rec = scene.inst + 8u * hs.inst;
+    uint32_t iflags = __float_as_uint(rec[6].x);
+    float4 A = scene.triA[hs.tri], B = scene.triB[hs.tri], C = scene.triC[hs.tri];
+    float3 n = geometric_normal(rec, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
+                                make_float3(C.x, C.y, C.z), iflags);
+    if (dot3(n, d) > 0.0f) n = make_float3(-n.x, -n.y, -n.z);
+    r[0] = o.x + d.x * hs.t; r[1] = o.y + d.y * hs.t; r[2] = o.z + d.z * hs.t;
+    r[3] = n.x; r[4] = n.y; r[5] = n.z;
+}
+
 inline uint32_t level_dim_host(uint32_t d, uint32_t level) {
     for (uint32_t l = 0; l < level; ++l) d = (d + 1u) / 2u;
     return d;
@@ -844,12 +915,30 @@ GiArgs gi_args(lux_ctx* ctx) {
     a.intervals = static_cast<float4*>(ctx->gi_intervals.ptr);
     a.merged = static_cast<float4*>(ctx->gi_merged.ptr);
     a.probes = static_cast<float4*>(ctx->gi_probes.ptr);
+    a.counters = static_cast<uint32_t*>(ctx->gi_counters.ptr);
+    a.admit_limit = ctx->gi.query_limit == 0u ? 0u : (ctx->gi.query_limit / 2u > 1u ? ctx->gi.query_limit / 2u : 1u);
     return a;
+}
+
+// Emitter table of the current instance transforms (build_emissive_lights
+// runs again after the instances moved, as the reference re-packs them).
+int upload_lights(lux_ctx* ctx) {
+    std::vector<Light> lights = build_lights(ctx);
+    ctx->gi_lights = (uint32_t)lights.size();
+    if (int rc = lux_ensure(ctx, ctx->gi_light_buf, sizeof(Light) * (lights.size() + 1))) return rc;
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!lights.empty())
+        LUX_HIP(ctx, hipMemcpy(ctx->gi_light_buf.ptr, lights.data(), sizeof(Light) * lights.size(),
+                               hipMemcpyHostToDevice));
+    ctx->gi_lights_dirty = false;
+    return LUX_OK;
 }
 
 int gi_check(lux_ctx* ctx) {
     if (!ctx->gi_ready) return lux_fail(ctx, LUX_ERR_STATE, "GI not configured (lux_gi_configure)");
     if (!ctx->tlas_built) return lux_fail(ctx, LUX_ERR_STATE, "GI: BVH not built");
+    if (ctx->gi_lights_dirty)
+        if (int rc = upload_lights(ctx)) return rc;
     return LUX_OK;
 }
 
@@ -886,13 +975,9 @@ extern "C" int lux_gi_configure(lux_ctx* ctx, const lux_gi_params* params) {
     if (int rc = lux_ensure(ctx, ctx->gi_intervals, 16ull * off)) return rc;
     if (int rc = lux_ensure(ctx, ctx->gi_merged, 16ull * off)) return rc;
     if (int rc = lux_ensure(ctx, ctx->gi_probes, 16ull * 36 * ctx->gi_probes0)) return rc;
-    std::vector<Light> lights = build_lights(ctx);
-    ctx->gi_lights = (uint32_t)lights.size();
-    if (int rc = lux_ensure(ctx, ctx->gi_light_buf, sizeof(Light) * (lights.size() + 1))) return rc;
-    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (!lights.empty())
-        LUX_HIP(ctx, hipMemcpy(ctx->gi_light_buf.ptr, lights.data(), sizeof(Light) * lights.size(),
-                               hipMemcpyHostToDevice));
+    if (int rc = lux_ensure(ctx, ctx->gi_counters, 16)) return rc;
+    if (int rc = lux_device_zero(ctx, ctx->gi_counters.ptr, 8)) return rc;
+    if (int rc = upload_lights(ctx)) return rc;
     ctx->gi = p;
     ctx->gi_ready = true;
     ctx->gi_sur_probes = 0;
@@ -953,10 +1038,28 @@ extern "C" int lux_gi_pack(lux_ctx* ctx) {
     return LUX_OK;
 }
 
+extern "C" int lux_gi_reset_query_counters(lux_ctx* ctx) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!ctx->gi_ready) return lux_fail(ctx, LUX_ERR_STATE, "GI not configured (lux_gi_configure)");
+    return lux_device_zero(ctx, ctx->gi_counters.ptr, 8);
+}
+
+extern "C" int lux_gi_query_counters(lux_ctx* ctx, uint32_t* attempts, uint32_t* admitted) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!ctx->gi_ready) return lux_fail(ctx, LUX_ERR_STATE, "GI not configured (lux_gi_configure)");
+    uint32_t c[2] = {0, 0};
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LUX_HIP(ctx, hipMemcpy(c, ctx->gi_counters.ptr, 8, hipMemcpyDeviceToHost));
+    if (attempts) *attempts = c[0];
+    if (admitted) *admitted = c[1];
+    return LUX_OK;
+}
+
 extern "C" int lux_gi_update(lux_ctx* ctx, float* ms) {
     if (!ctx) return LUX_ERR_INVALID;
     if (int rc = gi_check(ctx)) return rc;
     LUX_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (int rc = lux_device_zero(ctx, ctx->gi_counters.ptr, 8)) return rc;
     if (int rc = lux_gi_trace(ctx)) return rc;
     if (int rc = lux_gi_merge(ctx)) return rc;
     if (int rc = lux_gi_pack(ctx)) return rc;
@@ -1086,6 +1189,7 @@ extern "C" int lux_gi_update_surrogate(lux_ctx* ctx, lux_surrogate* s, uint32_t 
     float* queries = static_cast<float*>(ctx->gi_sur_queries.ptr);
 
     LUX_HIP(ctx, hipEventRecord(ctx->ev_phase[0], st));
+    if (int rc = lux_device_zero(ctx, ctx->gi_counters.ptr, 8)) return rc;
     if (int rc = lux_gi_trace_levels(ctx, 0, level - 1)) return rc;
     LUX_HIP(ctx, hipEventRecord(ctx->ev_phase[1], st));
     uint32_t qblocks = (uint32_t)((count + kWg - 1) / kWg);
@@ -1131,5 +1235,21 @@ extern "C" int lux_gi_surrogate_probe_error(lux_ctx* ctx, float* out, uint32_t c
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_surrogate_probe_error: count must equal the level's probes");
     LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     LUX_HIP(ctx, hipMemcpy(out, ctx->gi_sur_err.ptr, 4ull * count, hipMemcpyDeviceToHost));
+    return LUX_OK;
+}
+
+extern "C" int lux_camera_receivers(lux_ctx* ctx, const float* camera, uint32_t width, uint32_t height,
+                                    float* receivers) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!camera || !receivers) return lux_fail(ctx, LUX_ERR_INVALID, "lux_camera_receivers: null buffer");
+    if (!ctx->tlas_built) return lux_fail(ctx, LUX_ERR_STATE, "lux_camera_receivers: BVH not built");
+    uint64_t n = (uint64_t)width * height;
+    if (n == 0) return LUX_OK;
+    if (n >= (1ull << 31)) return lux_fail(ctx, LUX_ERR_INVALID, "lux_camera_receivers: image too large");
+    Camera cam;
+    std::memcpy(&cam, camera, sizeof cam);
+    hipLaunchKernelGGL(k_camera_receivers, dim3((uint32_t)((n + kWg - 1) / kWg)), dim3(kWg), 0, ctx->stream,
+                       ctx->dev_scene(), cam, width, height, receivers);
+    LUX_HIP(ctx, hipGetLastError());
     return LUX_OK;
 }

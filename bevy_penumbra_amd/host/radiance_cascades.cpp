@@ -123,6 +123,7 @@ lux_gi_params gi_params_for(const RadianceCascades& camera, const ProbeVolume& v
     p.seed = c.history_epoch;
     p.rays_per_interval = 1;
     p.terminal_tail = 1;
+    p.query_limit = c.sampling.max_traversal_queries;
     return p;
 }
 

@@ -24,6 +24,7 @@ LUX_RAY_ANY_HIT = 1
 LUX_RAY_FIRST_HIT = 2
 LUX_EMPTY = 0xFFFFFFFF
 LUX_BVH_MORTON30 = 0
+LUX_BVH_FORCE_BLAS = 0x100
 
 
 class LuxError(RuntimeError):
@@ -41,12 +42,12 @@ class GiParams(C.Structure):
                 ("interval_mode", C.c_uint32), ("interval_length", C.c_float),
                 ("normal_bias", C.c_float), ("emissive_samples", C.c_uint32),
                 ("frame", C.c_uint32), ("seed", C.c_uint32), ("rays_per_interval", C.c_uint32),
-                ("terminal_tail", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+                ("terminal_tail", C.c_uint32), ("flags", C.c_uint32), ("query_limit", C.c_uint32)]
 
 
 def gi_params(origin, spacing, dims, levels, base_phi=8, base_y=4, interval_mode=0,
               interval_length=0.4, normal_bias=0.005, emissive_samples=16, frame=0, seed=0,
-              rays_per_interval=1, terminal_tail=1) -> GiParams:
+              rays_per_interval=1, terminal_tail=1, query_limit=0) -> GiParams:
     p = GiParams()
     p.origin[:] = [float(x) for x in origin]
     p.spacing = spacing
@@ -60,6 +61,7 @@ def gi_params(origin, spacing, dims, levels, base_phi=8, base_y=4, interval_mode
     p.frame, p.seed = frame, seed
     p.rays_per_interval = rays_per_interval
     p.terminal_tail = terminal_tail
+    p.query_limit = query_limit
     return p
 
 
@@ -221,6 +223,9 @@ def load_library() -> C.CDLL:
         "lux_radiance_cascades_sanitized": (i32, [vp, vp]),
         "lux_radiance_cascades_gi_params": (i32, [vp, vp, vp, u32, u32, u32, u32, u32, vp]),
         "lux_gi_trace_levels": (i32, [vp, u32, u32]),
+        "lux_gi_query_counters": (i32, [vp, vp, vp]),
+        "lux_camera_receivers": (i32, [vp, vp, u32, u32, vp]),
+        "lux_gi_reset_query_counters": (i32, [vp]),
         "lux_surrogate_default_params": (None, [vp]),
         "lux_surrogate_create": (i32, [vp, vp, C.POINTER(vp)]),
         "lux_surrogate_destroy": (None, [vp]),
@@ -503,6 +508,20 @@ class Context:
     def world_resolve_device(self, receivers, n: int, out):
         self._check(self.lib.lux_world_resolve_device(self.h, _ptr(receivers), n, _ptr(out)),
                     "lux_world_resolve_device")
+
+    def camera_receivers(self, camera: np.ndarray, width: int, height: int, out):
+        cam = np.ascontiguousarray(camera, np.float32)
+        assert cam.size == 12
+        self._check(self.lib.lux_camera_receivers(self.h, cam.ctypes.data, width, height, _ptr(out)),
+                    "lux_camera_receivers")
+
+    def gi_query_counters(self):
+        a, b = C.c_uint32(), C.c_uint32()
+        self._check(self.lib.lux_gi_query_counters(self.h, C.byref(a), C.byref(b)), "lux_gi_query_counters")
+        return int(a.value), int(b.value)
+
+    def gi_reset_query_counters(self):
+        self._check(self.lib.lux_gi_reset_query_counters(self.h), "lux_gi_reset_query_counters")
 
     # ---- learned far-cascade surrogate ----
     def gi_trace_levels(self, first: int, last: int):

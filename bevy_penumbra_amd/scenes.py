@@ -340,3 +340,106 @@ def cornell():
     all_mats = np.concatenate(mats)
     return (single_instance(all_tris, all_mats, materials, ((0, 0, 0), (2, 2, 2))),
             np.arange(first, first + 2, dtype=np.uint32))
+
+
+# ---------------------------------------------------------------- dynamic frame
+def _uv_sphere(stacks: int, slices: int) -> np.ndarray:
+    """(2 * stacks * slices, 3, 3) triangles of a unit UV sphere, outward
+    winding, band by band from the +y pole (the pole bands keep one
+    zero-area triangle per quad so every band has 2 * slices triangles)."""
+    th = np.linspace(0.0, np.pi, stacks + 1)
+    ph = np.linspace(0.0, 2 * np.pi, slices + 1)
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    v = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], axis=-1)
+    a, b = v[:-1, :-1], v[1:, :-1]
+    c, d = v[1:, 1:], v[:-1, 1:]
+    t1 = np.stack([a, c, b], axis=2)
+    t2 = np.stack([a, d, c], axis=2)
+    return np.stack([t1, t2], axis=2).reshape(-1, 3, 3)
+
+
+def dynamic(seed: int = 0, meshes: int = 256, instances: int = 256, stacks: int = 32, slices: int = 64):
+    """The dynamic large frame: ``instances`` rigid instances of ``meshes``
+    procedural meshes of 2 * stacks * slices triangles (defaults: one mesh per
+    instance, 256 x 4096 = 1,048,576 triangles) moving inside a 30 x 12 x 15 m
+    volume.  Each mesh is a
+    UV sphere with seeded radial relief, scaled to an ellipsoid of 0.3-1.6 m
+    radii; the band below its top pole (2 * slices triangles) is emissive.  Returns
+    (Scene at frame 0, motion) where ``motion`` feeds ``dynamic_transforms``."""
+    rng = np.random.default_rng(seed)
+    base = _uv_sphere(stacks, slices)
+    mesh_list, mat_list = [], []
+    for m in range(meshes):
+        k = rng.integers(1, 5, size=3)
+        amp = rng.uniform(0.0, 0.15, size=3)
+        ph = rng.uniform(0, 2 * np.pi, size=3)
+        p = base.reshape(-1, 3)
+        r = 1.0 + amp[0] * np.sin(k[0] * np.arctan2(p[:, 2], p[:, 0]) + ph[0]) \
+            + amp[1] * np.sin(k[1] * np.arccos(np.clip(p[:, 1], -1, 1)) + ph[1]) \
+            + amp[2] * np.sin(k[2] * (p[:, 0] + p[:, 1]) + ph[2])
+        radii = rng.uniform(0.3, 1.6, size=3)
+        tri = (p * r[:, None] * radii[None, :]).reshape(-1, 9).astype(np.float32)
+        mesh_list.append(tri)
+        mats = np.full(tri.shape[0], m % 3, np.uint32)
+        mats[2 * slices: 4 * slices] = 3
+        mat_list.append(mats)
+    materials = np.asarray([
+        [0.75, 0.75, 0.75, 1.0, 0, 0, 0, 0],
+        [0.70, 0.55, 0.40, 1.0, 0, 0, 0, 0],
+        [0.30, 0.35, 0.45, 1.0, 0, 0, 0, 0],
+        [0.8, 0.8, 0.8, 1.0, 6.0, 5.0, 3.0, 0],
+    ], np.float32)
+    ext = np.array([30.0, 12.0, 15.0])
+    axis = rng.normal(size=(instances, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    motion = dict(
+        centre=rng.uniform([2.0, 1.5, 2.0], ext - [2.0, 1.5, 2.0], size=(instances, 3)),
+        sway=rng.uniform(0.1, 1.0, size=(instances, 3)),
+        freq=rng.uniform(0.01, 0.05, size=(instances, 3)),
+        phase=rng.uniform(0, 2 * np.pi, size=(instances, 3)),
+        axis=axis,
+        spin=rng.uniform(-0.05, 0.05, size=instances),
+        angle0=rng.uniform(0, 2 * np.pi, size=instances),
+        mesh=(np.arange(instances) % meshes).astype(np.uint32),
+    )
+    inv, fwd = dynamic_transforms(motion, 0)
+    scene = Scene(meshes=mesh_list, mesh_materials=mat_list, inst_inv=inv, inst_fwd=fwd,
+                  inst_mesh=motion["mesh"].copy(), inst_flags=np.zeros(instances, np.uint32),
+                  materials=materials, bounds=((0.0, 0.0, 0.0), tuple(ext)))
+    return scene, motion
+
+
+def dynamic_transforms(motion: dict, frame: int):
+    """Seeded rigid motion of every instance at a frame: rotation about a
+    fixed axis at a fixed rate, translation swaying around a centre.  Returns
+    (local_from_world, world_from_local) as (I, 12) float32 row-major 3x4."""
+    f = float(frame)
+    t = motion["centre"] + motion["sway"] * np.sin(motion["freq"] * f + motion["phase"])
+    ang = motion["angle0"] + motion["spin"] * f
+    a = motion["axis"]
+    c, s = np.cos(ang)[:, None, None], np.sin(ang)[:, None, None]
+    K = np.zeros((a.shape[0], 3, 3))
+    K[:, 0, 1], K[:, 0, 2] = -a[:, 2], a[:, 1]
+    K[:, 1, 0], K[:, 1, 2] = a[:, 2], -a[:, 0]
+    K[:, 2, 0], K[:, 2, 1] = -a[:, 1], a[:, 0]
+    R = np.eye(3)[None] * c + s * K + (1 - c) * a[:, :, None] * a[:, None, :]
+    fwd = np.concatenate([R, t[:, :, None]], axis=2)
+    Rt = np.transpose(R, (0, 2, 1))
+    inv = np.concatenate([Rt, -(Rt @ t[:, :, None])], axis=2)
+    return (np.ascontiguousarray(inv.reshape(-1, 12), np.float32),
+            np.ascontiguousarray(fwd.reshape(-1, 12), np.float32))
+
+
+def camera_rays(origin, target, width: int, height: int, fov_y_deg: float = 60.0) -> np.ndarray:
+    """Pinhole camera basis for ``lux_camera_receivers``: 12 floats = origin,
+    forward (unit), right and up scaled to the half extents of the image
+    plane at unit distance."""
+    o = np.asarray(origin, np.float64)
+    fwd = np.asarray(target, np.float64) - o
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, [0.0, 1.0, 0.0])
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    h = np.tan(np.radians(fov_y_deg) / 2)
+    w = h * width / height
+    return np.concatenate([o, fwd, right * w, up * h]).astype(np.float32)

@@ -131,6 +131,10 @@ int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_world,
 
 /* ---- BVH build (replaces build_bvh/build_tree, src/scene.rs:2374-2554) ---- */
 #define LUX_BVH_MORTON30 0u
+/* or-ed into the mode: rebuild every BLAS too, not only what changed */
+#define LUX_BVH_FORCE_BLAS 0x100u
+/* Builds what is stale: the BLASes after lux_scene_upload_soa, the TLAS after
+ * lux_scene_set_transforms (Morton-30 over instance boxes; no host wait). */
 int lux_bvh_build(lux_ctx* ctx, uint32_t mode);
 /* 128 B instance record of a built scene, in TLAS order (the reference's
  * 9-word record, src/scene.rs:1071-1122, minus layer/material words). */
@@ -207,7 +211,12 @@ typedef struct {
     uint32_t rays_per_interval;
     uint32_t terminal_tail;    /* 1: the last level owns the unbounded tail */
     uint32_t flags;            /* reserved, 0 */
-    uint32_t reserved;
+    uint32_t query_limit;      /* SamplingSettings::max_traversal_queries: per-update
+                                  ticketed budget; interval and NEE traversals are
+                                  admitted up to max(limit / 2, 1), a denied query
+                                  makes its sample unsupported; 0 = unbounded
+                                  (admitTransportQuery, shaders/traversal.slang:20-49;
+                                  traversal_query_caps, src/render.rs:290-300) */
 } lux_gi_params;
 
 /* Allocate the per-level interval/merged arrays (float4 per interval:
@@ -234,6 +243,13 @@ int lux_gi_device_arrays(lux_ctx* ctx, float** intervals, float** merged, float*
  * probes into device float[4*n]: diffuse irradiance/pi rgb and the support
  * code (0 not eligible, -1 unsupported, 1 + 0.25 coverage). */
 int lux_world_resolve_device(lux_ctx* ctx, const float* receivers, uint32_t n, float* out);
+
+/* Traversal query counters of the current update: attempts and admitted
+ * (admitted <= the budget; both count every interval and NEE query).
+ * lux_gi_update and lux_gi_update_surrogate reset them at their start, as
+ * the reference's cache-clear pass does once per frame. */
+int lux_gi_query_counters(lux_ctx* ctx, uint32_t* attempts, uint32_t* admitted);
+int lux_gi_reset_query_counters(lux_ctx* ctx);
 
 /* Trace only levels first..last (inclusive) of the configured grid into the
  * interval array; lux_gi_trace is first = 0, last = levels - 1. */
@@ -331,6 +347,17 @@ int lux_gi_update_surrogate(lux_ctx* ctx, lux_surrogate* s, uint32_t level, floa
                             lux_gi_surrogate_stats* stats);
 /* Per-probe predicted RMSE of the last surrogate update (host float[probes]). */
 int lux_gi_surrogate_probe_error(lux_ctx* ctx, float* out, uint32_t count);
+
+/* Receivers of a pinhole camera for lux_world_resolve_device (device
+ * float[6 * width * height], row-major, y down): per pixel one closest-hit
+ * ray; hit position + geometric normal facing the camera, or a zero normal
+ * (not eligible) on a miss.  camera: origin[3], forward[3] (unit), right[3]
+ * and up[3] scaled to the image plane's half extents at unit distance; the
+ * ray of pixel (x, y) is normalize(forward + s right + t up) with
+ * s = (x + 0.5) / width * 2 - 1, t = 1 - (y + 0.5) / height * 2.  Stands in
+ * for the depth/normal prepass whose receivers the reference resolves. */
+int lux_camera_receivers(lux_ctx* ctx, const float* camera, uint32_t width, uint32_t height,
+                         float* receivers);
 
 /* Kernel timing of the last device trace (HIP events on the context stream). */
 int lux_last_kernel_ms(lux_ctx* ctx, float* ms);

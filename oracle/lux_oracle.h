@@ -162,7 +162,7 @@ typedef struct {
     uint32_t rays_per_interval;
     uint32_t terminal_tail;
     uint32_t flags;
-    uint32_t reserved;
+    uint32_t query_limit; /* unused by the oracle (the GPU's admission budget) */
 } orc_gi_params;
 
 /* Emissive triangle record (GiLight, src/scene.rs:90-108), 96 B. */

@@ -200,12 +200,12 @@ class GiParams(C.Structure):
                 ("interval_mode", C.c_uint32), ("interval_length", C.c_float),
                 ("normal_bias", C.c_float), ("emissive_samples", C.c_uint32),
                 ("frame", C.c_uint32), ("seed", C.c_uint32), ("rays_per_interval", C.c_uint32),
-                ("terminal_tail", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+                ("terminal_tail", C.c_uint32), ("flags", C.c_uint32), ("query_limit", C.c_uint32)]
 
 
 def gi_params(origin, spacing, dims, levels, base_phi=8, base_y=4, interval_mode=0,
               interval_length=0.4, normal_bias=0.005, emissive_samples=16, frame=0, seed=0,
-              rays_per_interval=1, terminal_tail=1) -> GiParams:
+              rays_per_interval=1, terminal_tail=1, query_limit=0) -> GiParams:
     p = GiParams()
     p.origin[:] = [float(x) for x in origin]
     p.spacing = spacing
@@ -219,6 +219,7 @@ def gi_params(origin, spacing, dims, levels, base_phi=8, base_y=4, interval_mode
     p.frame, p.seed = frame, seed
     p.rays_per_interval = rays_per_interval
     p.terminal_tail = terminal_tail
+    p.query_limit = query_limit
     return p
 
 

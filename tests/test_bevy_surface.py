@@ -78,3 +78,14 @@ def test_camera_maps_onto_transport_parameters():
     assert p.spacing == 0.125 and p.interval_length == f32(0.2)
     assert p.emissive_samples == 32 and p.seed == 7 and p.frame == 3
     assert list(p.dims) == [16, 16, 16] and p.levels == 4 and p.terminal_tail == 1
+
+
+def test_traversal_query_budget_reaches_transport_parameters():
+    r = lux.RadianceCascades.default()
+    p = r.gi_params((0, 0, 0), (8, 8, 8), 3)
+    assert p.query_limit == 0                  # unbounded by default
+    low = lux.RadianceCascades.low_end()
+    assert low.sampling[4] == 16384
+    assert low.gi_params((0, 0, 0), (8, 8, 8), 3).query_limit == 16384
+    low.sampling[4] = 10 ** 7                  # sanitized into 1..131072
+    assert low.gi_params((0, 0, 0), (8, 8, 8), 3).query_limit == 131072

@@ -134,3 +134,41 @@ def test_gpu_colonnade_full_grid_properties(lux_ctx):
     iv = a["intervals"]
     assert np.all((iv[:, 3] == -1) | ((iv[:, 3] >= 0) & (iv[:, 3] <= 1)))
     assert (iv[:, 3] >= 0).mean() > 0.999
+
+
+def test_gpu_query_budget_unbounded_counts_every_query(lux_ctx):
+    sc, _ = scenes.cornell()
+    p = cornell_params()
+    g = run_gpu(lux_ctx, sc, p)
+    attempts, admitted = lux_ctx.gi_query_counters()
+    n = O.total_intervals(p)
+    hits = int((g["intervals"][:, 3] == 0).sum())
+    # one primary query per interval plus max(1, 16 / 2) = 8 NEE queries at
+    # most per shaded hit
+    assert attempts == admitted and n <= attempts <= n + 8 * hits
+    o = oracle_pipeline(sc, p)
+    assert_bits(g["intervals"], o["intervals"], "intervals")
+
+
+def test_gpu_query_budget_denies_past_the_limit(lux_ctx):
+    """admitTransportQuery (shaders/traversal.slang:20-38): a denied query is
+    not a miss.  Past the limit every interval is either exactly the oracle's
+    or an unsupported sample (beta = -1, J = 0), the admitted count never
+    exceeds max(limit / 2, 1), and merge and pack treat the unsupported
+    samples exactly as the oracle does."""
+    sc, _ = scenes.cornell()
+    n = O.total_intervals(cornell_params())
+    limit = n                                  # trace/NEE tranche = n / 2
+    p = cornell_params(query_limit=limit)
+    g = run_gpu(lux_ctx, sc, p)
+    attempts, admitted = lux_ctx.gi_query_counters()
+    assert admitted <= limit // 2 < attempts
+    o = oracle_pipeline(sc, cornell_params())
+    iv, ov = g["intervals"], o["intervals"]
+    same = np.all(iv.view(np.uint32) == ov.view(np.uint32), axis=1)
+    denied = (iv[:, 3] == -1.0) & np.all(iv[:, :3] == 0.0, axis=1)
+    assert np.all(same | denied)
+    assert denied.sum() > n // 4 and same.sum() > n // 4
+    m = O.gi_merge(p, iv)
+    assert_bits(g["merged"], m, "merged over unsupported samples")
+    assert_bits(g["probes"], O.gi_pack(p, m), "probes")
