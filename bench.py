@@ -5,13 +5,16 @@ Workload (one "step"): trace 4,194,304 rays -- half closest-hit, half any-hit
 colonnade (30 x 12 x 15 m, 8 emissive quads); rays have origins uniform in
 the scene box, directions uniform on the sphere and tmax log-uniform in
 0.1-20 m.  Rays and hits stay resident in HBM; the timed region is K
-back-to-back trace launches.
+back-to-back trace launches.  Before it, the library's autotuner picks the
+traversal launch configuration on these rays (results do not depend on it).
 
 Prints one JSON line: interval Mrays/s (value), the roofline of the trace
 kernel (HIP-event duration on the library's stream, algorithmic bytes from
 the instrumented kernel's node/triangle/instance counts) and a CPU baseline
 (the C restatement of the reference CPU algorithms -- SAH build_tree and the
 reference traversal -- on a bounded ray sample, timed on this host's cores).
+Further sections: the GI update (plain and surrogate-assisted at equal RMSE)
+and the dynamic 1,048,576-triangle frame (p50/p99).
 """
 from __future__ import annotations
 
@@ -38,8 +41,8 @@ def parse():
     p.add_argument("--rays", type=int, default=4 * 1024 * 1024)
     p.add_argument("--cpu-sample", type=int, default=1 << 20)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--waves-per-cu", type=int, default=0)
-    p.add_argument("--regroup", type=int, default=0)
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep the context's default traversal configuration")
     p.add_argument("--gi-steps", type=int, default=5)
     p.add_argument("--no-gi", action="store_true")
     p.add_argument("--no-surrogate", action="store_true")
@@ -251,12 +254,17 @@ def main() -> int:
     ctx.build_bvh()
     ctx.synchronize()
     build_ms = (time.perf_counter() - t0) * 1e3
-    ctx.set_trace_tuning(args.waves_per_cu, args.regroup)
-
     n = rays_np.shape[0]
     rays = DeviceBuffer.from_array(ctx, rays_np)
     hits = DeviceBuffer.empty(ctx, (n, 4), np.uint32)
     counters = DeviceBuffer.empty(ctx, 3, np.uint64)
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = ctx.autotune_trace(rays, n, hits, repeats=3)
+        tune["tuning_s"] = round(time.perf_counter() - t0, 2)
+        tune["ms"] = round(tune["ms"], 4)
+    trace_cfg = ctx.trace_config()
 
     # algorithmic work of one launch (instrumented kernel, outside timing)
     ctx.trace_stats_device(rays, n, hits, counters)
@@ -342,6 +350,8 @@ def main() -> int:
             "kernel_ms": round(avg_kernel_ms, 4),
             "node_fetches_per_ray": round(nodes / n, 3),
             "tri_tests_per_ray": round(tris / n, 3),
+            "trace_config": trace_cfg,
+            "autotune": tune,
         },
         "roofline": {
             "bound": "hbm",

@@ -610,4 +610,280 @@ __device__ __forceinline__ void trace_occluded(const DevScene& s, float3 o, floa
     if (incomplete) h.err |= kErrUnsupported;
 }
 
+
+// ---------------------------------------------------------------- resumable walk
+// The three traversals above restated as a per-lane state machine: one
+// walk_step performs one iteration of the TLAS loop, starts one instance of
+// a TLAS leaf, or performs one iteration of the BLAS loop (including its
+// leaf's triangles).  Every ray executes exactly the operations, in exactly
+// the order, of trace_closest / trace_instance (modes kWalkClosest and
+// kWalkFirst) or trace_occluded / occluded_instance (kWalkOccluded), so
+// results are bit-identical; only the interleaving across lanes changes,
+// which lets a wave retire finished rays and refill their lanes.
+constexpr uint32_t kWalkClosest = 0u, kWalkFirst = 1u, kWalkOccluded = 2u;
+constexpr uint32_t kPhaseDone = 0u, kPhaseTlas = 1u, kPhaseLeaf = 2u, kPhaseBlas = 3u;
+
+struct Walk {
+    float3 o, d;
+    float limit;             // occlusion limit (closest/first-hit use h.t)
+    HitState h;
+    uint32_t mode, phase;
+    uint32_t tcount, tsteps, ti, tend;
+    bool tready, incomplete;
+    Node node;               // current TLAS node, then the current BLAS node
+    uint32_t ii, flags, gfirst, gcount, bcount, bsteps;
+    bool bready, binc;
+    float3 lo, ld;
+    TriRay ray;
+};
+
+template <class St>
+__device__ __forceinline__ void walk_begin(const DevScene& s, Walk& w, float3 o, float3 d, float tmax,
+                                           uint32_t mode, St& st) {
+    w.o = o;
+    w.d = d;
+    w.limit = tmax;
+    w.mode = mode;
+    w.h.t = tmax;
+    w.h.tri = kEmpty;
+    w.h.inst = kEmpty;
+    w.h.err = 0;
+    w.h.b1 = w.h.b2 = 0.0f;
+    w.phase = kPhaseDone;
+    w.incomplete = false;
+    w.tcount = 0;
+    w.tsteps = 0;
+    if (s.n_inst == 0) return;
+    w.node = load_node(s.tlas, s.n_tlas, 0);
+    st.node(1);
+    float re, rx;
+    bool root = box_interval(o, d, w.node.mn, w.node.mx, tmax, re, rx);
+    if (mode == kWalkOccluded && root) w.h.err |= kHitRoot;
+    if (!root) return;
+    w.tready = re >= 0.0f;
+    if (mode == kWalkOccluded && !w.tready) return;
+    w.phase = kPhaseTlas;
+}
+
+template <class Stack, class St>
+__device__ __forceinline__ void walk_tlas(const DevScene& s, Walk& w, Stack& stk, St& st) {
+    const bool occl = w.mode == kWalkOccluded;
+    if (!(w.tready || w.tcount != 0)) {
+        if (occl && w.incomplete) w.h.err |= kErrUnsupported;
+        w.phase = kPhaseDone;
+        return;
+    }
+    if (++w.tsteps > kMaxSteps) {
+        w.h.err |= kErrUnsupported;
+        w.phase = kPhaseDone;
+        return;
+    }
+    if (!w.tready) {
+        --w.tcount;
+        uint2 ent = stk.get(w.tcount);
+        if (!occl && __uint_as_float(ent.y) > w.h.t) return;
+        w.node = load_node(s.tlas, s.n_tlas, ent.x);
+        st.node(1);
+    }
+    w.tready = false;
+    const uint32_t ninst = s.n_inst;
+    if (w.node.count == 0) {
+        if (w.tcount + 2 > kStackCap) {
+            if (!occl) w.h.err |= kErrTlasOverflow;
+            w.node.first = 0;
+            w.node.count = ninst;
+            w.tcount = 0;
+        } else {
+            uint32_t li = w.node.first, ri = li + 1u;
+            Node ln = load_node(s.tlas, s.n_tlas, li);
+            Node rn = load_node(s.tlas, s.n_tlas, ri);
+            st.node(2);
+            float lim = occl ? w.limit : w.h.t;
+            float le = box_entry(w.o, w.d, ln, lim);
+            float rre = box_entry(w.o, w.d, rn, lim);
+            if (le >= 0.0f && rre >= 0.0f) {
+                if (le <= rre) {
+                    stk.put(w.tcount, make_uint2(ri, occl ? 0u : __float_as_uint(rre)));
+                    w.node = ln;
+                } else {
+                    stk.put(w.tcount, make_uint2(li, occl ? 0u : __float_as_uint(le)));
+                    w.node = rn;
+                }
+                ++w.tcount;
+                w.tready = true;
+            } else if (le >= 0.0f) {
+                w.node = ln;
+                w.tready = true;
+            } else if (rre >= 0.0f) {
+                w.node = rn;
+                w.tready = true;
+            }
+            return;
+        }
+    }
+    uint32_t req_end = w.node.first + w.node.count;
+    bool truncated = req_end < w.node.first || w.node.first > ninst || req_end > ninst ||
+                     w.node.count > kMaxLeafInstances;
+    uint32_t ib = w.node.first < ninst ? w.node.first : ninst;
+    uint32_t ie = req_end < ninst ? req_end : ninst;
+    uint32_t be = ib + kMaxLeafInstances;
+    if (be > ie) be = ie;
+    truncated = truncated || be < ie;
+    if (occl) w.incomplete = w.incomplete || truncated;
+    else if (truncated) w.h.err |= kErrUnsupported;
+    w.ti = ib;
+    w.tend = be;
+    w.phase = kPhaseLeaf;
+}
+
+template <class St>
+__device__ __forceinline__ void walk_instance(const DevScene& s, Walk& w, St& st) {
+    if (w.ti >= w.tend) {
+        w.phase = kPhaseTlas;
+        return;
+    }
+    const uint32_t ii = w.ti;
+    const float4* rec = s.inst + 8u * ii;
+    float4 i0 = rec[0], i1 = rec[1], i2 = rec[2];
+    uint4 g = *reinterpret_cast<const uint4*>(rec + 6);
+    w.ii = ii;
+    w.flags = g.x;
+    w.gfirst = g.z;
+    w.gcount = g.w;
+    w.lo = xform_point(i0, i1, i2, w.o);
+    w.ld = xform_dir(i0, i1, i2, w.d);
+    st.inst();
+    w.bcount = 0;
+    w.node = load_node(s.blas, s.n_blas, g.y);
+    st.node(1);
+    float e, x;
+    float lim = w.mode == kWalkOccluded ? w.limit : w.h.t;
+    w.bready = box_interval(w.lo, w.ld, w.node.mn, w.node.mx, lim, e, x) && e >= 0.0f;
+    w.ray.valid = false;
+    w.ray.kx = w.ray.ky = w.ray.kz = 0;
+    w.ray.sx = w.ray.sy = 0.0f;
+    if (w.bready) w.ray = make_tri_ray(w.ld);
+    w.bsteps = 0;
+    w.binc = false;
+    w.phase = kPhaseBlas;
+}
+
+// End of the current instance's walk: the TLAS leaf loop resumes.
+__device__ __forceinline__ void walk_instance_end(Walk& w, bool supported) {
+    if (w.mode == kWalkOccluded) {
+        w.incomplete = w.incomplete || !supported;
+    } else if (w.mode == kWalkFirst && w.h.tri != kEmpty) {
+        // an accepted hit proves an any-hit query (traversal.slang:786-792)
+        w.h.err &= ~kErrUnsupported;
+        w.tcount = 0;
+        w.phase = kPhaseTlas;
+        return;
+    }
+    ++w.ti;
+    w.phase = kPhaseLeaf;
+}
+
+template <class Stack, class St>
+__device__ __forceinline__ void walk_blas(const DevScene& s, Walk& w, Stack& stk, St& st) {
+    const bool occl = w.mode == kWalkOccluded;
+    if (!(w.bready || w.bcount != 0)) {
+        walk_instance_end(w, !w.binc);
+        return;
+    }
+    if (++w.bsteps > kMaxSteps) {
+        if (!occl) w.h.err |= kErrUnsupported;
+        walk_instance_end(w, false);
+        return;
+    }
+    const uint32_t base = w.tcount;
+    if (!w.bready) {
+        --w.bcount;
+        uint2 ent = stk.get(base + w.bcount);
+        if (!occl && __uint_as_float(ent.y) > w.h.t) return;
+        w.node = load_node(s.blas, s.n_blas, ent.x);
+        st.node(1);
+    }
+    w.bready = false;
+    if (w.node.count == 0) {
+        if (w.bcount + 2 > kStackCap) {
+            if (!occl) w.h.err |= kErrBlasOverflow;
+            w.node.first = w.gfirst;
+            w.node.count = w.gcount;
+            w.bcount = 0;
+        } else {
+            uint32_t li = w.node.first, ri = li + 1u;
+            Node ln = load_node(s.blas, s.n_blas, li);
+            Node rn = load_node(s.blas, s.n_blas, ri);
+            st.node(2);
+            float lim = occl ? w.limit : w.h.t;
+            float le = box_entry(w.lo, w.ld, ln, lim);
+            float re = box_entry(w.lo, w.ld, rn, lim);
+            if (le >= 0.0f && re >= 0.0f) {
+                if (le <= re) {
+                    stk.put(base + w.bcount, make_uint2(ri, occl ? 0u : __float_as_uint(re)));
+                    w.node = ln;
+                } else {
+                    stk.put(base + w.bcount, make_uint2(li, occl ? 0u : __float_as_uint(le)));
+                    w.node = rn;
+                }
+                ++w.bcount;
+                w.bready = true;
+            } else if (le >= 0.0f) {
+                w.node = ln;
+                w.bready = true;
+            } else if (re >= 0.0f) {
+                w.node = rn;
+                w.bready = true;
+            }
+            return;
+        }
+    }
+    uint32_t tbegin = w.node.first > w.gfirst ? w.node.first : w.gfirst;
+    uint32_t tlimit = w.gfirst + w.gcount;
+    uint32_t req_end = w.node.first + w.node.count;
+    uint32_t tend = req_end < tlimit ? req_end : tlimit;
+    bool truncated = w.node.first < w.gfirst || req_end < w.node.first || req_end > tlimit ||
+                     w.node.count > kMaxLeafTriangles;
+    uint32_t bend = tbegin + kMaxLeafTriangles;
+    if (bend > tend) bend = tend;
+    truncated = truncated || bend < tend;
+    if (occl) w.binc = w.binc || truncated;
+    else if (truncated) w.h.err |= kErrUnsupported;
+    for (uint32_t i = tbegin; i < bend; ++i) {
+        uint32_t ti = i < s.n_tri ? i : s.n_tri - 1u;
+        float4 A = s.triA[ti], B = s.triB[ti], C = s.triC[ti];
+        st.tri();
+        float3 a = make_float3(A.x, A.y, A.z), b = make_float3(B.x, B.y, B.z),
+               c = make_float3(C.x, C.y, C.z);
+        if (culled(a, b, c, w.ld, w.flags)) continue;
+        float3 r3 = intersect_tri(w.lo, w.ld, w.ray, a, b, c);
+        float lim = occl ? w.limit : w.h.t;
+        if (r3.x > 0.000025f && r3.x < lim) {
+            w.h.t = r3.x;
+            w.h.tri = i;
+            w.h.inst = w.ii;
+            w.h.b1 = r3.y;
+            w.h.b2 = r3.z;
+            if (occl) {
+                w.h.err |= kHitOccluded;
+                w.phase = kPhaseDone;
+                return;
+            }
+        }
+        if (w.mode == kWalkFirst && w.h.tri != kEmpty) {
+            walk_instance_end(w, true);
+            return;
+        }
+    }
+}
+
+// One iteration; returns true once the ray is finished.
+template <class Stack, class St>
+__device__ __forceinline__ bool walk_step(const DevScene& s, Walk& w, Stack& stk, St& st) {
+    if (w.phase == kPhaseBlas) walk_blas(s, w, stk, st);
+    else if (w.phase == kPhaseTlas) walk_tlas(s, w, stk, st);
+    else if (w.phase == kPhaseLeaf) walk_instance(s, w, st);
+    return w.phase == kPhaseDone;
+}
+
 }  // namespace lux

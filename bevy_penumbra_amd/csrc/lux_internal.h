@@ -66,7 +66,9 @@ struct lux_ctx {
     hipEvent_t ev_phase[5] = {};
 
     // trace tuning
-    uint32_t waves_per_cu = 0, regroup = 0;
+    // traversal launch configuration (defaults: the autotuner's choice on the
+    // 262,144-triangle colonnade microbenchmark)
+    uint32_t waves_per_cu = 20, regroup = 8, trace_depth = 8, trace_batch = 8;
     float last_ms = 0.0f;
 
     lux::DevScene dev_scene() const;

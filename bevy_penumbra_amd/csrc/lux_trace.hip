@@ -1,13 +1,20 @@
 // lux_trace.hip -- closest-hit and any-hit ray traversal kernels for gfx950.
 //
-// Persistent grid: each wave keeps 64 rays in flight and refills idle lanes
-// from a global ray cursor once at least `regroup` lanes are idle (ballot +
-// mbcnt compaction), so rays that finish early do not leave lanes empty
-// while a few long rays keep the wave alive.  Per-lane traversal stacks
-// live in LDS (lane-contiguous columns) with a private spill area.
+// Two persistent kernels over a global ray cursor:
+//   k_trace_persistent  a wave takes 64 rays and runs them to completion
+//                       (the wave lasts as long as its longest ray);
+//   k_trace_regroup     every lane runs a resumable walk (lux::Walk); after
+//                       each batch of steps the wave counts its idle lanes
+//                       (ballot) and, once at least `regroup` are idle,
+//                       refills them with one atomic on the cursor and a
+//                       popcount rank, so finished rays stop wasting lanes.
+// Per-lane traversal stacks live in LDS (lane-contiguous columns, depth a
+// launch parameter) with a private spill area.
 //
 // Replaces the trace/traceVisibility dispatches of the reference
 // (shaders/split.slang:1266-1339 over shaders/traversal.slang:696-1046).
+#include <algorithm>
+#include <cstring>
 #include "lux_internal.h"
 
 namespace {
@@ -22,8 +29,19 @@ struct TraceArgs {
     float* bary;
     uint32_t n;
     uint32_t* cursor;  // global ray cursor (zeroed before launch)
-    uint32_t regroup;
+    uint32_t regroup;  // idle lanes that trigger a refill (1..64)
+    uint32_t batch;    // walk steps between refill checks
 };
+
+__device__ __forceinline__ void write_hit(const TraceArgs& a, uint32_t idx, const lux::HitState& h) {
+    uint4 out;
+    out.x = __float_as_uint(h.t);
+    out.y = h.tri == lux::kEmpty ? lux::kEmpty : __float_as_uint(a.scene.triA[h.tri].w);
+    out.z = h.inst == lux::kEmpty ? lux::kEmpty : __float_as_uint(a.scene.inst[8u * h.inst + 7u].x);
+    out.w = h.err;
+    reinterpret_cast<uint4*>(a.hits)[idx] = out;
+    if (a.bary) reinterpret_cast<float2*>(a.bary)[idx] = make_float2(h.b1, h.b2);
+}
 
 template <class St>
 __device__ __forceinline__ void trace_one(const TraceArgs& a, uint32_t idx,
@@ -59,6 +77,57 @@ __global__ __launch_bounds__(kTraceWg) void k_trace_persistent(TraceArgs a) {
         uint32_t idx = base + lane;
         lux::NoStats st;
         if (idx < a.n) trace_one(a, idx, stk, st);
+    }
+}
+
+// Regrouping persistent kernel (see the file comment).  All lanes run the
+// outer loop together; `active` only predicates the walk steps, so the
+// ballots and shuffles see the whole wave.  A wave exits once the cursor is
+// drained and none of its lanes holds a ray.
+template <int Depth>
+__global__ __launch_bounds__(kTraceWg) void k_trace_regroup(TraceArgs a) {
+    __shared__ uint2 lds[Depth][kTraceWg];
+    lux::LaneStack<Depth, kTraceWg> stk;
+    stk.col = &lds[0][threadIdx.x];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    lux::Walk w;
+    lux::NoStats st;
+    uint32_t idx = 0;
+    bool active = false, drained = false;
+    while (true) {
+        uint64_t idle = __ballot(!active);
+        uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!drained && nidle >= a.regroup) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(a.cursor, nidle);
+            base = __shfl(base, 0, 64);
+            if (base + nidle >= a.n) drained = true;
+            if (!active) {
+                idx = base + (uint32_t)__popcll(idle & below);
+                if (idx < a.n) {
+                    const float4* rp = reinterpret_cast<const float4*>(a.rays) + 2u * idx;
+                    float4 r0 = rp[0], r1 = rp[1];
+                    uint32_t flags = __float_as_uint(r1.w);
+                    uint32_t mode = (flags & lux::kRayAnyHit) ? lux::kWalkOccluded
+                                    : ((flags & lux::kRayFirstHit) ? lux::kWalkFirst : lux::kWalkClosest);
+                    lux::walk_begin(a.scene, w, make_float3(r0.x, r0.y, r0.z), make_float3(r1.x, r1.y, r1.z),
+                                    r0.w, mode, st);
+                    if (w.phase == lux::kPhaseDone) write_hit(a, idx, w.h);
+                    else active = true;
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (drained) break;
+            continue;
+        }
+        for (uint32_t k = 0; k < a.batch; ++k) {
+            if (active && lux::walk_step(a.scene, w, stk, st)) {
+                write_hit(a, idx, w.h);
+                active = false;
+            }
+        }
     }
 }
 
@@ -121,13 +190,26 @@ static int trace_launch(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* 
     a.n = n;
     a.cursor = static_cast<uint32_t*>(ctx->scratch.ptr);
     a.regroup = ctx->regroup;
+    a.batch = ctx->trace_batch ? ctx->trace_batch : 8u;
     hipStream_t st = ctx->stream;
     LUX_HIP(ctx, hipEventRecord(ctx->ev0, st));
-    if (ctx->waves_per_cu == 1000u) {
+    if (ctx->regroup > 0u && ctx->waves_per_cu != 1000u) {
+        uint32_t wpc = ctx->waves_per_cu ? ctx->waves_per_cu : 20u;
+        uint32_t blocks = (uint32_t)ctx->cu_count * wpc / (kTraceWg / 64);
+        uint32_t need = (n + kTraceWg - 1) / kTraceWg;
+        if (blocks > need) blocks = need;
+        if (blocks == 0) blocks = 1;
+        hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, a.cursor);
+        switch (ctx->trace_depth) {
+        case 4: hipLaunchKernelGGL(k_trace_regroup<4>, dim3(blocks), dim3(kTraceWg), 0, st, a); break;
+        case 16: hipLaunchKernelGGL(k_trace_regroup<16>, dim3(blocks), dim3(kTraceWg), 0, st, a); break;
+        default: hipLaunchKernelGGL(k_trace_regroup<8>, dim3(blocks), dim3(kTraceWg), 0, st, a); break;
+        }
+    } else if (ctx->waves_per_cu == 1000u) {
         uint32_t blocks = (n + kTraceWg - 1) / kTraceWg;
         hipLaunchKernelGGL(k_trace_grid, dim3(blocks), dim3(kTraceWg), 0, st, a);
     } else {
-        uint32_t wpc = ctx->waves_per_cu ? ctx->waves_per_cu : 16u;
+        uint32_t wpc = ctx->waves_per_cu ? ctx->waves_per_cu : 20u;
         uint32_t blocks = (uint32_t)ctx->cu_count * wpc / (kTraceWg / 64);
         uint32_t need = (n + kTraceWg - 1) / kTraceWg;
         if (blocks > need) blocks = need;
@@ -177,6 +259,7 @@ extern "C" int lux_trace_stats_device(lux_ctx* ctx, const lux_ray* rays, uint32_
     a.n = n;
     a.cursor = nullptr;
     a.regroup = 0;
+    a.batch = 0;
     hipLaunchKernelGGL(k_trace_stats, dim3((n + kTraceWg - 1) / kTraceWg), dim3(kTraceWg), 0,
                        ctx->stream, a, counters);
     LUX_HIP(ctx, hipGetLastError());
@@ -190,6 +273,35 @@ extern "C" int lux_last_kernel_ms(lux_ctx* ctx, float* ms) {
     return LUX_OK;
 }
 
+extern "C" int lux_set_trace_config(lux_ctx* ctx, const lux_trace_config* cfg) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!cfg) return lux_fail(ctx, LUX_ERR_INVALID, "lux_set_trace_config: null config");
+    if (cfg->waves_per_cu > 32 && cfg->waves_per_cu != 1000u)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_set_trace_config: waves_per_cu > 32");
+    if (cfg->regroup_threshold > 64)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_set_trace_config: regroup threshold > 64");
+    if (cfg->lds_stack_depth != 0 && cfg->lds_stack_depth != 4 && cfg->lds_stack_depth != 8 &&
+        cfg->lds_stack_depth != 16)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_set_trace_config: LDS stack depth must be 4, 8 or 16");
+    if (cfg->steps_per_refill > 256)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_set_trace_config: steps per refill > 256");
+    ctx->waves_per_cu = cfg->waves_per_cu;
+    ctx->regroup = cfg->regroup_threshold;
+    ctx->trace_depth = cfg->lds_stack_depth;
+    ctx->trace_batch = cfg->steps_per_refill;
+    return LUX_OK;
+}
+
+extern "C" int lux_get_trace_config(lux_ctx* ctx, lux_trace_config* cfg) {
+    if (!ctx || !cfg) return LUX_ERR_INVALID;
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->waves_per_cu = ctx->waves_per_cu;
+    cfg->regroup_threshold = ctx->regroup;
+    cfg->lds_stack_depth = ctx->trace_depth;
+    cfg->steps_per_refill = ctx->trace_batch;
+    return LUX_OK;
+}
+
 extern "C" int lux_set_trace_tuning(lux_ctx* ctx, uint32_t waves_per_cu, uint32_t regroup_threshold) {
     if (!ctx) return LUX_ERR_INVALID;
     if (waves_per_cu > 32 && waves_per_cu != 1000u)
@@ -198,5 +310,86 @@ extern "C" int lux_set_trace_tuning(lux_ctx* ctx, uint32_t waves_per_cu, uint32_
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_set_trace_tuning: regroup > 64");
     ctx->waves_per_cu = waves_per_cu;
     ctx->regroup = regroup_threshold;
+    return LUX_OK;
+}
+
+// Autotuner: coordinate descent over the launch knobs (waves per CU, LDS
+// stack depth, refill threshold, steps per refill), each candidate timed with
+// HIP events on the caller's rays (median of `repeats` launches after one
+// warm-up), two passes; the fastest configuration is installed.  Results do
+// not depend on the configuration, so tuning never changes hits.
+extern "C" int lux_autotune_trace(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits,
+                                  uint32_t repeats, lux_trace_config* best, float* best_ms,
+                                  uint32_t* evaluations) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!rays || !hits || n == 0) return lux_fail(ctx, LUX_ERR_INVALID, "lux_autotune_trace: empty ray set");
+    if (!ctx->tlas_built) return lux_fail(ctx, LUX_ERR_STATE, "lux_autotune_trace: BVH not built");
+    if (repeats == 0) repeats = 3;
+    if (repeats > 15) repeats = 15;
+    const uint32_t saved[4] = {ctx->waves_per_cu, ctx->trace_depth, ctx->regroup, ctx->trace_batch};
+    auto measure = [&](const uint32_t c[4], float& ms) -> int {
+        ctx->waves_per_cu = c[0];
+        ctx->trace_depth = c[1];
+        ctx->regroup = c[2];
+        ctx->trace_batch = c[3];
+        float t[16];
+        for (uint32_t r = 0; r <= repeats; ++r) {
+            if (int rc = trace_launch(ctx, rays, n, hits, nullptr)) return rc;
+            LUX_HIP(ctx, hipEventSynchronize(ctx->ev1));
+            float e = 0.0f;
+            LUX_HIP(ctx, hipEventElapsedTime(&e, ctx->ev0, ctx->ev1));
+            if (r > 0) t[r - 1] = e;
+        }
+        std::sort(t, t + repeats);
+        ms = t[repeats / 2];
+        return LUX_OK;
+    };
+    static const uint32_t cand_wpc[] = {12, 16, 20, 24, 32};
+    static const uint32_t cand_depth[] = {4, 8, 16};
+    static const uint32_t cand_regroup[] = {0, 1, 4, 8, 16, 32};
+    static const uint32_t cand_batch[] = {2, 4, 8, 16};
+    const uint32_t* cands[4] = {cand_wpc, cand_depth, cand_regroup, cand_batch};
+    const uint32_t ncand[4] = {5, 3, 6, 4};
+    uint32_t cur[4] = {20, 8, 8, 8};
+    float cur_ms = 0.0f;
+    uint32_t evals = 0;
+    int rc = measure(cur, cur_ms);
+    ++evals;
+    for (int pass = 0; pass < 2 && rc == LUX_OK; ++pass) {
+        for (int k = 0; k < 4 && rc == LUX_OK; ++k) {
+            for (uint32_t j = 0; j < ncand[k] && rc == LUX_OK; ++j) {
+                if (cands[k][j] == cur[k]) continue;
+                uint32_t c[4] = {cur[0], cur[1], cur[2], cur[3]};
+                c[k] = cands[k][j];
+                float ms = 0.0f;
+                rc = measure(c, ms);
+                ++evals;
+                if (rc == LUX_OK && ms < cur_ms) {
+                    cur_ms = ms;
+                    cur[k] = c[k];
+                }
+            }
+        }
+    }
+    if (rc != LUX_OK) {
+        ctx->waves_per_cu = saved[0];
+        ctx->trace_depth = saved[1];
+        ctx->regroup = saved[2];
+        ctx->trace_batch = saved[3];
+        return rc;
+    }
+    ctx->waves_per_cu = cur[0];
+    ctx->trace_depth = cur[1];
+    ctx->regroup = cur[2];
+    ctx->trace_batch = cur[3];
+    if (best) {
+        std::memset(best, 0, sizeof *best);
+        best->waves_per_cu = cur[0];
+        best->lds_stack_depth = cur[1];
+        best->regroup_threshold = cur[2];
+        best->steps_per_refill = cur[3];
+    }
+    if (best_ms) *best_ms = cur_ms;
+    if (evaluations) *evaluations = evals;
     return LUX_OK;
 }

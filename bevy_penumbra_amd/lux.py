@@ -225,6 +225,9 @@ def load_library() -> C.CDLL:
         "lux_gi_trace_levels": (i32, [vp, u32, u32]),
         "lux_gi_query_counters": (i32, [vp, vp, vp]),
         "lux_camera_receivers": (i32, [vp, vp, u32, u32, vp]),
+        "lux_set_trace_config": (i32, [vp, vp]),
+        "lux_get_trace_config": (i32, [vp, vp]),
+        "lux_autotune_trace": (i32, [vp, vp, u32, vp, u32, vp, vp, vp]),
         "lux_gi_reset_query_counters": (i32, [vp]),
         "lux_surrogate_default_params": (None, [vp]),
         "lux_surrogate_create": (i32, [vp, vp, C.POINTER(vp)]),
@@ -455,6 +458,27 @@ class Context:
     def set_trace_tuning(self, waves_per_cu: int = 0, regroup: int = 0):
         self._check(self.lib.lux_set_trace_tuning(self.h, waves_per_cu, regroup),
                     "lux_set_trace_tuning")
+
+    def set_trace_config(self, waves_per_cu: int = 0, lds_stack_depth: int = 0,
+                         regroup_threshold: int = 0, steps_per_refill: int = 0):
+        cfg = (C.c_uint32 * 8)(waves_per_cu, lds_stack_depth, regroup_threshold, steps_per_refill,
+                               0, 0, 0, 0)
+        self._check(self.lib.lux_set_trace_config(self.h, cfg), "lux_set_trace_config")
+
+    def autotune_trace(self, rays, n: int, hits, repeats: int = 3) -> dict:
+        cfg = (C.c_uint32 * 8)()
+        ms = C.c_float()
+        ev = C.c_uint32()
+        self._check(self.lib.lux_autotune_trace(self.h, _ptr(rays), n, _ptr(hits), repeats, cfg,
+                                                C.byref(ms), C.byref(ev)), "lux_autotune_trace")
+        return dict(waves_per_cu=cfg[0], lds_stack_depth=cfg[1], regroup_threshold=cfg[2],
+                    steps_per_refill=cfg[3], ms=float(ms.value), evaluations=int(ev.value))
+
+    def trace_config(self) -> dict:
+        cfg = (C.c_uint32 * 8)()
+        self._check(self.lib.lux_get_trace_config(self.h, cfg), "lux_get_trace_config")
+        return dict(waves_per_cu=cfg[0], lds_stack_depth=cfg[1], regroup_threshold=cfg[2],
+                    steps_per_refill=cfg[3])
 
     # ---- transport (dense-grid GI update) ----
     def gi_configure(self, params: GiParams):

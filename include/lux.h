@@ -362,9 +362,36 @@ int lux_camera_receivers(lux_ctx* ctx, const float* camera, uint32_t width, uint
 /* Kernel timing of the last device trace (HIP events on the context stream). */
 int lux_last_kernel_ms(lux_ctx* ctx, float* ms);
 
-/* Launch tuning (autotuner output): persistent-grid waves per CU, refill
- * threshold (idle lanes before a wave fetches new rays), 0 = defaults. */
+/* Launch tuning: persistent-grid waves per CU (0 = 20) and refill threshold
+ * (idle lanes before a wave fetches new rays; 0 = run each wave's 64 rays to
+ * completion).  Keeps the LDS stack depth and steps per refill. */
 int lux_set_trace_tuning(lux_ctx* ctx, uint32_t waves_per_cu, uint32_t regroup_threshold);
+
+/* Full traversal launch configuration.  regroup_threshold 0 runs each wave's
+ * 64 rays to completion; 1..64 runs resumable walks and refills idle lanes
+ * once that many are idle, checking every steps_per_refill walk steps
+ * (0 = 8).  lds_stack_depth: traversal-stack slots per lane kept in LDS
+ * (4, 8 or 16; 0 = 8), deeper slots spill to private memory.
+ * waves_per_cu: persistent grid size (0 = 20; 1000 = one ray per thread,
+ * no persistence).  A new context starts at {20, 8, 8, 8}.  Results do not
+ * depend on the configuration. */
+typedef struct {
+    uint32_t waves_per_cu;
+    uint32_t lds_stack_depth;
+    uint32_t regroup_threshold;
+    uint32_t steps_per_refill;
+    uint32_t reserved[4];
+} lux_trace_config;
+int lux_set_trace_config(lux_ctx* ctx, const lux_trace_config* cfg);
+int lux_get_trace_config(lux_ctx* ctx, lux_trace_config* cfg);
+/* Autotuner: times candidate configurations on the caller's device rays
+ * (coordinate descent over waves per CU {12..32}, LDS stack depth {4,8,16},
+ * refill threshold {0..32}, steps per refill {2..16}; median of `repeats`
+ * launches each) and installs the fastest.  best, best_ms and evaluations
+ * are nullable.  No counterpart in the reference (its traversal is a fixed
+ * wgpu dispatch); every configuration returns identical hits. */
+int lux_autotune_trace(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits, uint32_t repeats,
+                       lux_trace_config* best, float* best_ms, uint32_t* evaluations);
 
 #ifdef __cplusplus
 }

@@ -140,11 +140,69 @@ def test_gpu_launch_shapes_agree(lux_ctx, colonnade):
     lux_ctx.upload_scene(sc)
     lux_ctx.build_bvh()
     rays = scenes.microbench_rays(sc.bounds, n=200_000, seed=3)
+    saved = lux_ctx.trace_config()
     ref = trace_gpu(lux_ctx, rays)
     for wpc, rg in [(1000, 0), (4, 0), (32, 16)]:
         lux_ctx.set_trace_tuning(wpc, rg)
         assert np.array_equal(trace_gpu(lux_ctx, rays), ref), (wpc, rg)
-    lux_ctx.set_trace_tuning(0, 0)
+    # regrouping walks: every LDS depth, refill threshold and batch
+    for wpc, depth, rg, batch in [(16, 16, 16, 8), (16, 8, 1, 1), (8, 4, 64, 32), (20, 16, 32, 4)]:
+        lux_ctx.set_trace_config(wpc, depth, rg, batch)
+        assert np.array_equal(trace_gpu(lux_ctx, rays), ref), (wpc, depth, rg, batch)
+    lux_ctx.set_trace_config(**saved)
+
+
+REGROUP_CONFIGS = [(0, 0, 0, 0), (16, 16, 16, 8), (16, 4, 1, 1)]
+
+
+@pytest.mark.parametrize("cfg", REGROUP_CONFIGS)
+def test_gpu_regrouped_walk_bit_exact_vs_oracle(lux_ctx, fixture, colonnade, cfg):
+    """The resumable walk with lane refills returns the oracle's hits, error
+    bits and barycentrics: reference ray families on the instanced fixture,
+    forced 64-deep chains (overflow bits) and 1M colonnade rays.  (0, 0, 0, 0)
+    is the run-to-completion kernel."""
+    saved = lux_ctx.trace_config()
+    lux_ctx.set_trace_config(*cfg)
+    try:
+        sc, built, rays = fixture
+        lux_ctx.upload_scene(sc)
+        lux_ctx.build_bvh()
+        hits, bary = trace_gpu(lux_ctx, rays, bary=True)
+        ohits, obary = built.trace(rays, bary=True)
+        assert np.array_equal(hits, ohits)
+        assert np.array_equal(bary.view(np.uint32), obary.view(np.uint32))
+        for blas in (True, False):
+            deep = forced_deep(built, blas)
+            lux_ctx.upload_built(deep.blas, deep.tlas, deep.tri, deep.inst)
+            sel = rays[::3]
+            assert np.array_equal(trace_gpu(lux_ctx, sel), deep.trace(sel)), blas
+        csc, cbuilt = colonnade
+        lux_ctx.upload_scene(csc)
+        lux_ctx.build_bvh()
+        crays = scenes.microbench_rays(csc.bounds, n=1 << 20, seed=11)
+        got = trace_gpu(lux_ctx, crays)
+        want = cbuilt.trace(crays, threads=16)
+        bad = np.nonzero(np.any(got != want, axis=1))[0]
+        assert bad.size == 0, (cfg, bad[:5])
+    finally:
+        lux_ctx.set_trace_config(**saved)
+
+
+def test_gpu_autotuner_installs_a_valid_configuration(lux_ctx, colonnade):
+    sc, _ = colonnade
+    lux_ctx.upload_scene(sc)
+    lux_ctx.build_bvh()
+    saved = lux_ctx.trace_config()
+    rays = scenes.microbench_rays(sc.bounds, n=1 << 18, seed=5)
+    ref = trace_gpu(lux_ctx, rays)
+    r = DeviceBuffer.from_array(lux_ctx, rays)
+    h = DeviceBuffer.empty(lux_ctx, (rays.shape[0], 4), np.uint32)
+    res = lux_ctx.autotune_trace(r, rays.shape[0], h, repeats=1)
+    cfg = lux_ctx.trace_config()
+    assert res["evaluations"] >= 10 and res["ms"] > 0
+    assert {k: cfg[k] for k in cfg} == {k: res[k] for k in cfg}
+    assert np.array_equal(trace_gpu(lux_ctx, rays), ref)
+    lux_ctx.set_trace_config(**saved)
 
 
 def test_gpu_trace_4m_properties(lux_ctx, colonnade):
