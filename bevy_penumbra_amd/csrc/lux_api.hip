@@ -175,7 +175,8 @@ extern "C" void lux_ctx_destroy(lux_ctx* ctx) {
                           &ctx->ray_buf, &ctx->hit_buf, &ctx->gi_intervals, &ctx->gi_merged,
                           &ctx->gi_probes, &ctx->gi_light_buf, &ctx->gi_sur_queries,
                           &ctx->gi_sur_err, &ctx->gi_sur_list, &ctx->gi_counters,
-                          &ctx->tlas_tmp, &ctx->mesh_info};
+                          &ctx->tlas_tmp, &ctx->mesh_info, &ctx->gi_w_prim, &ctx->gi_w_shade,
+                          &ctx->gi_w_slots, &ctx->gi_w_vis, &ctx->gi_w_shadow, &ctx->gi_w_counters};
     for (lux_buffer* b : bufs) lux_release(*b);
     for (hipEvent_t e : ctx->ev_phase)
         if (e) (void)hipEventDestroy(e);

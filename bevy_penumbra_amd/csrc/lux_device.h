@@ -886,4 +886,49 @@ __device__ __forceinline__ bool walk_step(const DevScene& s, Walk& w, Stack& stk
     return w.phase == kPhaseDone;
 }
 
+// Lane-refilling driver for resumable walks.  Src supplies the work items:
+//   uint32_t count()                         number of items
+//   bool begin(uint32_t i, Walk& w, St& st)   start item i; false when it
+//                                            finished without a walk (its
+//                                            result already written)
+//   void finish(uint32_t i, const Walk& w)    store a finished walk
+// All lanes run the loop together (ballots and shuffles see the whole
+// wave); a wave refills its idle lanes with one atomic on *cursor once at
+// least `regroup` lanes are idle, and leaves when the cursor is drained and
+// none of its lanes holds an item.
+template <class Src, class Stack, class St>
+__device__ __forceinline__ void regroup_walks(const DevScene& s, Src& src, uint32_t* cursor,
+                                              uint32_t regroup, uint32_t batch, Stack& stk, St& st) {
+    const uint32_t lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t n = src.count();
+    Walk w;
+    uint32_t idx = 0;
+    bool active = false, drained = false;
+    while (true) {
+        uint64_t idle = __ballot(!active);
+        uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!drained && nidle >= regroup) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(cursor, nidle);
+            base = __shfl(base, 0, 64);
+            if (base >= n || n - base <= nidle) drained = true;
+            if (!active) {
+                idx = base + (uint32_t)__popcll(idle & below);
+                if (idx < n) active = src.begin(idx, w, st);
+            }
+        }
+        if (!__any(active)) {
+            if (drained) break;
+            continue;
+        }
+        for (uint32_t k = 0; k < batch; ++k) {
+            if (active && walk_step(s, w, stk, st)) {
+                src.finish(idx, w);
+                active = false;
+            }
+        }
+    }
+}
+
 }  // namespace lux

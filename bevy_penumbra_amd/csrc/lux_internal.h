@@ -62,6 +62,10 @@ struct lux_ctx {
     lux_buffer gi_counters;  // traversal query attempts, admitted (per update)
     bool gi_lights_dirty = false;  // instance transforms changed since the emitter table
     lux_buffer tlas_tmp;     // TLAS build temporaries (instance boxes, permutation)
+    // wavefront interval trace: mode 0 wavefront, 1 fused; per-item records,
+    // NEE slots, slot visibility, compacted occlusion rays, counters
+    uint32_t gi_trace_mode = 0;
+    lux_buffer gi_w_prim, gi_w_shade, gi_w_slots, gi_w_vis, gi_w_shadow, gi_w_counters;
     lux_buffer mesh_info;    // uint32 per mesh: BLAS base | triangle base | count
     hipEvent_t ev_phase[5] = {};
 

@@ -80,55 +80,35 @@ __global__ __launch_bounds__(kTraceWg) void k_trace_persistent(TraceArgs a) {
     }
 }
 
-// Regrouping persistent kernel (see the file comment).  All lanes run the
-// outer loop together; `active` only predicates the walk steps, so the
-// ballots and shuffles see the whole wave.  A wave exits once the cursor is
-// drained and none of its lanes holds a ray.
+// Regrouping persistent kernel (see the file comment): items are the rays
+// of the caller's list.
+struct RaySource {
+    const TraceArgs* a;
+    __device__ uint32_t count() const { return a->n; }
+    template <class St>
+    __device__ bool begin(uint32_t idx, lux::Walk& w, St& st) {
+        const float4* rp = reinterpret_cast<const float4*>(a->rays) + 2u * idx;
+        float4 r0 = rp[0], r1 = rp[1];
+        uint32_t flags = __float_as_uint(r1.w);
+        uint32_t mode = (flags & lux::kRayAnyHit) ? lux::kWalkOccluded
+                        : ((flags & lux::kRayFirstHit) ? lux::kWalkFirst : lux::kWalkClosest);
+        lux::walk_begin(a->scene, w, make_float3(r0.x, r0.y, r0.z), make_float3(r1.x, r1.y, r1.z), r0.w,
+                        mode, st);
+        if (w.phase != lux::kPhaseDone) return true;
+        write_hit(*a, idx, w.h);
+        return false;
+    }
+    __device__ void finish(uint32_t idx, const lux::Walk& w) { write_hit(*a, idx, w.h); }
+};
+
 template <int Depth>
 __global__ __launch_bounds__(kTraceWg) void k_trace_regroup(TraceArgs a) {
     __shared__ uint2 lds[Depth][kTraceWg];
     lux::LaneStack<Depth, kTraceWg> stk;
     stk.col = &lds[0][threadIdx.x];
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t below = (1ull << lane) - 1ull;
-    lux::Walk w;
     lux::NoStats st;
-    uint32_t idx = 0;
-    bool active = false, drained = false;
-    while (true) {
-        uint64_t idle = __ballot(!active);
-        uint32_t nidle = (uint32_t)__popcll(idle);
-        if (!drained && nidle >= a.regroup) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(a.cursor, nidle);
-            base = __shfl(base, 0, 64);
-            if (base + nidle >= a.n) drained = true;
-            if (!active) {
-                idx = base + (uint32_t)__popcll(idle & below);
-                if (idx < a.n) {
-                    const float4* rp = reinterpret_cast<const float4*>(a.rays) + 2u * idx;
-                    float4 r0 = rp[0], r1 = rp[1];
-                    uint32_t flags = __float_as_uint(r1.w);
-                    uint32_t mode = (flags & lux::kRayAnyHit) ? lux::kWalkOccluded
-                                    : ((flags & lux::kRayFirstHit) ? lux::kWalkFirst : lux::kWalkClosest);
-                    lux::walk_begin(a.scene, w, make_float3(r0.x, r0.y, r0.z), make_float3(r1.x, r1.y, r1.z),
-                                    r0.w, mode, st);
-                    if (w.phase == lux::kPhaseDone) write_hit(a, idx, w.h);
-                    else active = true;
-                }
-            }
-        }
-        if (!__any(active)) {
-            if (drained) break;
-            continue;
-        }
-        for (uint32_t k = 0; k < a.batch; ++k) {
-            if (active && lux::walk_step(a.scene, w, stk, st)) {
-                write_hit(a, idx, w.h);
-                active = false;
-            }
-        }
-    }
+    RaySource src{&a};
+    lux::regroup_walks(a.scene, src, a.cursor, a.regroup, a.batch, stk, st);
 }
 
 // Plain grid: one ray per thread.

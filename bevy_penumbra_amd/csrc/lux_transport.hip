@@ -244,107 +244,147 @@ __device__ float disney(float3 n, float3 l, float3 v, float pr) {
 
 using Stack = lux::LaneStack<kLdsDepth, kWg>;
 
-// Hit shading: emission + emissive NEE (cascades.slang:60-134,196-268) for
-// untextured opaque materials, black environment, no punctual lights.
-// Returns false when a visibility query was unsupported.
-__device__ bool shade(const GiArgs& a, float3 o, float3 d, const lux::HitState& h, Stack& stk,
-                      float3& out) {
+// Hit shading (cascades.slang:60-134,196-268) for untextured opaque
+// materials, black environment, no punctual lights, in three pieces shared by
+// the fused and the wavefront interval traces:
+//   shade_begin   emission of the hit and the NEE context (whether NEE runs)
+//   nee_sample    one emissive sample: false when it is skipped before any
+//                 visibility query, else the occlusion ray and the
+//                 contribution that counts when the ray is unoccluded
+//   shade         the fused loop: queries in sample order, unsupported
+//                 visibility makes the whole sample unsupported
+struct NeeCtx {
+    float3 pos, gn, nrm, view;
+    float rough, bias, phase;
+    float2 rot;
+    uint32_t count, frame;
+    lux_material mat;
+};
+
+// Returns true when NEE runs for this hit; rad receives the emission.
+__device__ bool shade_begin(const GiArgs& a, float3 o, float3 d, const lux::HitState& h, float3& rad,
+                            NeeCtx& c) {
     const lux::DevScene& s = a.scene;
     const float4* rec = s.inst + 8u * h.inst;
     uint32_t iflags = __float_as_uint(rec[6].x);
     float4 A = s.triA[h.tri], B = s.triB[h.tri], C = s.triC[h.tri];
     uint32_t mi = __float_as_uint(B.w);
-    out = make_float3(0.0f, 0.0f, 0.0f);
-    if (a.n_mats == 0) return true;
+    rad = make_float3(0.0f, 0.0f, 0.0f);
+    if (a.n_mats == 0) return false;
     if (mi >= a.n_mats) mi = a.n_mats - 1u;
-    const lux_material mat = a.mats[mi];
-    float3 rad = make_float3(mat.emission[0], mat.emission[1], mat.emission[2]);
-    if ((iflags & 16u) == 0u) {
-        float3 pos = make_float3(o.x + d.x * h.t, o.y + d.y * h.t, o.z + d.z * h.t);
-        float3 gn = geometric_normal(rec, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
-                                     make_float3(C.x, C.y, C.z), iflags);
-        if (dot3(gn, d) > 0.0f) gn = make_float3(-gn.x, -gn.y, -gn.z);
-        float3 nrm = gn;
-        float3 md = make_float3(-d.x, -d.y, -d.z);
-        if (!(dot3(nrm, md) <= 0.0f || dot3(nrm, gn) <= 0.0f) && a.n_lights > 0) {
-            float vl = sqrtf(dot3(md, md));
-            float3 view = make_float3(md.x / vl, md.y / vl, md.z / vl);
-            float rough = lux::isfinite_bits(mat.roughness) ? mat.roughness : 1.0f;
-            uint32_t count = a.p.emissive_samples / 2u;
-            if (count < 1u) count = 1u;
-            float bias = a.p.normal_bias;
-            float3 sum = make_float3(0.0f, 0.0f, 0.0f);
-            float sp = a.p.spacing * 0.05f;
-            uint32_t seed = hash_key((int32_t)floorf(pos.x / sp), (int32_t)floorf(pos.y / sp),
-                                     (int32_t)floorf(pos.z / sp), 0, 16777216u);
-            float2 rot = r2(seed);
-            uint32_t sel_seed = hash_key((int32_t)floorf(pos.x / a.p.spacing),
-                                         (int32_t)floorf(pos.y / a.p.spacing),
-                                         (int32_t)floorf(pos.z / a.p.spacing), 0, 65536u);
-            uint32_t frame = a.p.frame ^ 0xE11E0001u;
-            float phase = fracf(r2(frame).x + r2(sel_seed).x);
-            for (uint32_t smp = 0; smp < count; ++smp) {
-                float2 q = r2(smp + frame * count + 1u);
-                float2 uv = make_float2(fracf(q.x + rot.x), fracf(q.y + rot.y));
-                float selection = fracf(((float)smp + 0.5f) / (float)count + phase);
-                uint32_t lo = 0, hi = a.n_lights - 1u;
-                while (lo < hi) {
-                    uint32_t mid = (lo + hi) / 2u;
-                    if (a.lights[mid].direction.w < selection) lo = mid + 1u;
-                    else hi = mid;
-                }
-                const Light L = a.lights[lo];
-                float3 lp = make_float3(L.position.x, L.position.y, L.position.z);
-                float3 ldir = make_float3(L.direction.x, L.direction.y, L.direction.z);
-                float3 rel = make_float3(pos.x - lp.x, pos.y - lp.y, pos.z - lp.z);
-                if (L.edge1.w <= 0.0f || (L.position.w != -2.0f && dot3(ldir, rel) <= 0.0f)) continue;
-                float root = sqrtf(uv.x);
-                float wa = root * (1.0f - uv.y);
-                float wb = root * uv.y;
-                float3 target = make_float3((lp.x + L.edge1.x * wa) + L.edge2.x * wb,
-                                            (lp.y + L.edge1.y * wa) + L.edge2.y * wb,
-                                            (lp.z + L.edge1.z * wa) + L.edge2.z * wb);
-                float3 tl = make_float3(target.x - pos.x, target.y - pos.y, target.z - pos.z);
-                float ldist = sqrtf(dot3(tl, tl));
-                if (ldist <= bias * 0.25f) continue;
-                float3 Ld = make_float3(tl.x / ldist, tl.y / ldist, tl.z / ldist);
-                float cosine = smax(0.0f, dot3(nrm, Ld));
-                float ec = dot3(ldir, make_float3(-Ld.x, -Ld.y, -Ld.z));
-                if (L.position.w == -2.0f) ec = fabsf(ec);
-                float ecos = smax(0.0f, ec);
-                if (cosine <= 0.0f || ecos <= 0.0f || dot3(gn, Ld) <= 0.0f) continue;
-                float3 em = make_float3(L.color.x, L.color.y, L.color.z);
-                if (!(em.x > 0.0f || em.y > 0.0f || em.z > 0.0f)) continue;
-                float pdf = ((L.edge1.w * ldist) * ldist) / (L.color.w * ecos);
-                float plane = fabsf(dot3(ldir, rel));
-                float ob = smin(bias, plane * 0.25f);
-                float3 origin = ray_offset(pos, gn, ob);
-                float3 delta = make_float3(target.x - origin.x, target.y - origin.y, target.z - origin.z);
-                float dist = sqrtf(dot3(delta, delta));
-                if (dist <= bias * 0.25f) continue;
-                float3 dir = make_float3(delta.x / dist, delta.y / dist, delta.z / dist);
-                float lim = dist - smin(bias * 0.25f, dist * 0.001f);
-                if (!admit_query(a)) return false;
-                lux::HitState vh;
-                lux::NoStats ns;
-                lux::trace_occluded(s, origin, dir, lim, vh, stk, ns);
-                if (vh.err & lux::kHitOccluded) continue;
-                if (vh.err & lux::kErrUnsupported) return false;
-                float dm = disney(nrm, Ld, view, rough) / kPi;
-                float pm = smax(pdf, 1e-20f);
-                sum.x = sum.x + em.x * (((mat.albedo[0] * dm) * cosine) / pm);
-                sum.y = sum.y + em.y * (((mat.albedo[1] * dm) * cosine) / pm);
-                sum.z = sum.z + em.z * (((mat.albedo[2] * dm) * cosine) / pm);
-            }
-            float fc = (float)count;
-            rad = make_float3(rad.x + sum.x / fc, rad.y + sum.y / fc, rad.z + sum.z / fc);
+    c.mat = a.mats[mi];
+    rad = make_float3(c.mat.emission[0], c.mat.emission[1], c.mat.emission[2]);
+    if ((iflags & 16u) != 0u) return false;
+    float3 pos = make_float3(o.x + d.x * h.t, o.y + d.y * h.t, o.z + d.z * h.t);
+    float3 gn = geometric_normal(rec, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
+                                 make_float3(C.x, C.y, C.z), iflags);
+    if (dot3(gn, d) > 0.0f) gn = make_float3(-gn.x, -gn.y, -gn.z);
+    float3 nrm = gn;
+    float3 md = make_float3(-d.x, -d.y, -d.z);
+    if (dot3(nrm, md) <= 0.0f || dot3(nrm, gn) <= 0.0f || a.n_lights == 0) return false;
+    float vl = sqrtf(dot3(md, md));
+    c.view = make_float3(md.x / vl, md.y / vl, md.z / vl);
+    c.rough = lux::isfinite_bits(c.mat.roughness) ? c.mat.roughness : 1.0f;
+    c.count = a.p.emissive_samples / 2u;
+    if (c.count < 1u) c.count = 1u;
+    c.bias = a.p.normal_bias;
+    float sp = a.p.spacing * 0.05f;
+    uint32_t seed = hash_key((int32_t)floorf(pos.x / sp), (int32_t)floorf(pos.y / sp),
+                             (int32_t)floorf(pos.z / sp), 0, 16777216u);
+    c.rot = r2(seed);
+    uint32_t sel_seed = hash_key((int32_t)floorf(pos.x / a.p.spacing), (int32_t)floorf(pos.y / a.p.spacing),
+                                 (int32_t)floorf(pos.z / a.p.spacing), 0, 65536u);
+    c.frame = a.p.frame ^ 0xE11E0001u;
+    c.phase = fracf(r2(c.frame).x + r2(sel_seed).x);
+    c.pos = pos;
+    c.gn = gn;
+    c.nrm = nrm;
+    return true;
+}
+
+__device__ bool nee_sample(const GiArgs& a, const NeeCtx& c, uint32_t smp, float3& origin, float3& dir,
+                           float& lim, float3& term) {
+    float2 q = r2(smp + c.frame * c.count + 1u);
+    float2 uv = make_float2(fracf(q.x + c.rot.x), fracf(q.y + c.rot.y));
+    float selection = fracf(((float)smp + 0.5f) / (float)c.count + c.phase);
+    uint32_t lo = 0, hi = a.n_lights - 1u;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2u;
+        if (a.lights[mid].direction.w < selection) lo = mid + 1u;
+        else hi = mid;
+    }
+    const Light L = a.lights[lo];
+    const float3 pos = c.pos;
+    float3 lp = make_float3(L.position.x, L.position.y, L.position.z);
+    float3 ldir = make_float3(L.direction.x, L.direction.y, L.direction.z);
+    float3 rel = make_float3(pos.x - lp.x, pos.y - lp.y, pos.z - lp.z);
+    if (L.edge1.w <= 0.0f || (L.position.w != -2.0f && dot3(ldir, rel) <= 0.0f)) return false;
+    float root = sqrtf(uv.x);
+    float wa = root * (1.0f - uv.y);
+    float wb = root * uv.y;
+    float3 target = make_float3((lp.x + L.edge1.x * wa) + L.edge2.x * wb,
+                                (lp.y + L.edge1.y * wa) + L.edge2.y * wb,
+                                (lp.z + L.edge1.z * wa) + L.edge2.z * wb);
+    float3 tl = make_float3(target.x - pos.x, target.y - pos.y, target.z - pos.z);
+    float ldist = sqrtf(dot3(tl, tl));
+    if (ldist <= c.bias * 0.25f) return false;
+    float3 Ld = make_float3(tl.x / ldist, tl.y / ldist, tl.z / ldist);
+    float cosine = smax(0.0f, dot3(c.nrm, Ld));
+    float ec = dot3(ldir, make_float3(-Ld.x, -Ld.y, -Ld.z));
+    if (L.position.w == -2.0f) ec = fabsf(ec);
+    float ecos = smax(0.0f, ec);
+    if (cosine <= 0.0f || ecos <= 0.0f || dot3(c.gn, Ld) <= 0.0f) return false;
+    float3 em = make_float3(L.color.x, L.color.y, L.color.z);
+    if (!(em.x > 0.0f || em.y > 0.0f || em.z > 0.0f)) return false;
+    float pdf = ((L.edge1.w * ldist) * ldist) / (L.color.w * ecos);
+    float plane = fabsf(dot3(ldir, rel));
+    float ob = smin(c.bias, plane * 0.25f);
+    origin = ray_offset(pos, c.gn, ob);
+    float3 delta = make_float3(target.x - origin.x, target.y - origin.y, target.z - origin.z);
+    float dist = sqrtf(dot3(delta, delta));
+    if (dist <= c.bias * 0.25f) return false;
+    dir = make_float3(delta.x / dist, delta.y / dist, delta.z / dist);
+    lim = dist - smin(c.bias * 0.25f, dist * 0.001f);
+    float dm = disney(c.nrm, Ld, c.view, c.rough) / kPi;
+    float pm = smax(pdf, 1e-20f);
+    term = make_float3(em.x * (((c.mat.albedo[0] * dm) * cosine) / pm),
+                       em.y * (((c.mat.albedo[1] * dm) * cosine) / pm),
+                       em.z * (((c.mat.albedo[2] * dm) * cosine) / pm));
+    return true;
+}
+
+// Emission + NEE sum -> the sample's radiance (non-finite -> 0, negative
+// clamped), the end of the reference's outgoing-radiance evaluation.
+__device__ __forceinline__ float3 shade_end(float3 rad) {
+    if (!(lux::isfinite_bits(rad.x) && lux::isfinite_bits(rad.y) && lux::isfinite_bits(rad.z)))
+        return make_float3(0.0f, 0.0f, 0.0f);
+    return make_float3(smax(rad.x, 0.0f), smax(rad.y, 0.0f), smax(rad.z, 0.0f));
+}
+
+__device__ bool shade(const GiArgs& a, float3 o, float3 d, const lux::HitState& h, Stack& stk,
+                      float3& out) {
+    float3 rad;
+    NeeCtx c;
+    if (shade_begin(a, o, d, h, rad, c)) {
+        float3 sum = make_float3(0.0f, 0.0f, 0.0f);
+        for (uint32_t smp = 0; smp < c.count; ++smp) {
+            float3 origin, dir, term;
+            float lim;
+            if (!nee_sample(a, c, smp, origin, dir, lim, term)) continue;
+            if (!admit_query(a)) return false;
+            lux::HitState vh;
+            lux::NoStats ns;
+            lux::trace_occluded(a.scene, origin, dir, lim, vh, stk, ns);
+            if (vh.err & lux::kHitOccluded) continue;
+            if (vh.err & lux::kErrUnsupported) return false;
+            sum.x = sum.x + term.x;
+            sum.y = sum.y + term.y;
+            sum.z = sum.z + term.z;
         }
+        float fc = (float)c.count;
+        rad = make_float3(rad.x + sum.x / fc, rad.y + sum.y / fc, rad.z + sum.z / fc);
     }
-    if (!(lux::isfinite_bits(rad.x) && lux::isfinite_bits(rad.y) && lux::isfinite_bits(rad.z))) {
-        out = make_float3(0.0f, 0.0f, 0.0f);
-        return true;
-    }
-    out = make_float3(smax(rad.x, 0.0f), smax(rad.y, 0.0f), smax(rad.z, 0.0f));
+    out = shade_end(rad);
     return true;
 }
 
@@ -421,6 +461,222 @@ __global__ __launch_bounds__(kWg) void k_gi_trace(GiArgs a, uint64_t g_begin, ui
     float3 c = probe_centre(p, L, local / D);
     bool terminal = p.terminal_tail != 0u && L + 1 == p.levels;
     a.intervals[g] = trace_interval(a, L, c, k, terminal, stk);
+}
+
+// ---------------------------------------------------------------- wavefront interval trace
+// k_gi_trace split into four launches so that the primary and the shadow
+// traversals run as regrouped walks (lane refill, lux::regroup_walks):
+//   k_gi_primary  closest-hit walk of every (interval, ray) item
+//   k_gi_setup    per hit: emission and NEE context; each NEE sample's
+//                 contribution goes to a fixed slot, its occlusion ray to a
+//                 compacted list (wave-aggregated append)
+//   k_gi_shadow   occlusion walks of the list -> visibility per slot
+//   k_gi_accum    per interval: the fused kernel's accumulation in sample
+//                 order (same operations, so the same bits)
+struct Wave {
+    uint64_t g0, g1;      // interval range of this chunk
+    uint32_t R, nee;      // rays per interval, NEE slots per ray
+    uint4* prim;          // per item: t bits, tri, inst, err
+    float4* shade;        // per item: emission rgb, w = 1 when NEE runs
+    float4* slots;        // per slot: contribution rgb, w = 1 when a query was issued
+    uint32_t* vis;        // per slot: 0 visible, 1 occluded, 2 unsupported
+    float4* shadow;       // per occlusion ray: (origin, limit), (direction, slot bits)
+    uint32_t* counters;   // [0] primary cursor, [1] shadow rays, [2] shadow cursor
+    uint32_t regroup, batch;
+};
+
+// The ray of item (interval g, ray r): exactly trace_interval's construction.
+__device__ __forceinline__ void item_ray(const GiArgs& a, uint64_t g, uint32_t r, float3& o, float3& d,
+                                         float& limit, bool& terminal) {
+    const lux_gi_params& p = a.p;
+    uint32_t L = 0;
+    while (L + 1 < p.levels && g >= a.offset[L + 1]) ++L;
+    uint64_t local = g - a.offset[L];
+    uint32_t nphi = p.base_phi << L, ny = p.base_y << L;
+    uint32_t D = nphi * ny;
+    uint32_t k = (uint32_t)(local % D);
+    float3 c = probe_centre(p, L, local / D);
+    float end = interval_end(p, L);
+    float start = L == 0 ? 0.0f : interval_end(p, L - 1);
+    terminal = p.terminal_tail != 0u && L + 1 == p.levels;
+    limit = terminal ? 1e20f : end - start;
+    float2 ouv = ray_offset_uv(p.seed, r);
+    d = bin_direction(k, nphi, ny, ouv.x, ouv.y);
+    o = make_float3(c.x + d.x * start, c.y + d.y * start, c.z + d.z * start);
+}
+
+struct PrimarySource {
+    const GiArgs* a;
+    const Wave* w;
+    __device__ uint32_t count() const { return (uint32_t)((w->g1 - w->g0) * w->R); }
+    template <class St>
+    __device__ bool begin(uint32_t i, lux::Walk& wk, St& st) {
+        float3 o, d;
+        float lim;
+        bool term;
+        item_ray(*a, w->g0 + i / w->R, i % w->R, o, d, lim, term);
+        if (!admit_query(*a)) {  // denied: no sample, as an unsupported traversal
+            w->prim[i] = make_uint4(0u, lux::kEmpty, lux::kEmpty, lux::kErrUnsupported);
+            return false;
+        }
+        lux::walk_begin(a->scene, wk, o, d, lim, lux::kWalkClosest, st);
+        if (wk.phase != lux::kPhaseDone) return true;
+        finish(i, wk);
+        return false;
+    }
+    __device__ void finish(uint32_t i, const lux::Walk& wk) {
+        w->prim[i] = make_uint4(__float_as_uint(wk.h.t), wk.h.tri, wk.h.inst, wk.h.err);
+    }
+};
+
+template <int Depth>
+__global__ __launch_bounds__(kWg) void k_gi_primary(GiArgs a, Wave w) {
+    __shared__ uint2 lds[Depth][kWg];
+    lux::LaneStack<Depth, kWg> stk;
+    stk.col = &lds[0][threadIdx.x];
+    lux::NoStats st;
+    PrimarySource src{&a, &w};
+    lux::regroup_walks(a.scene, src, &w.counters[0], w.regroup, w.batch, stk, st);
+}
+
+__global__ __launch_bounds__(kWg) void k_gi_setup(GiArgs a, Wave w) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    uint32_t items = (uint32_t)((w.g1 - w.g0) * w.R);
+    if (i >= items) return;
+    uint4 pr = w.prim[i];
+    float4 sh = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if ((pr.w & lux::kErrUnsupported) == 0u && pr.y != lux::kEmpty) {
+        float3 o, d;
+        float lim;
+        bool term;
+        item_ray(a, w.g0 + i / w.R, i % w.R, o, d, lim, term);
+        lux::HitState h;
+        h.t = __uint_as_float(pr.x);
+        h.tri = pr.y;
+        h.inst = pr.z;
+        h.err = pr.w;
+        h.b1 = h.b2 = 0.0f;
+        float3 rad;
+        NeeCtx c;
+        bool nee = shade_begin(a, o, d, h, rad, c);
+        sh = make_float4(rad.x, rad.y, rad.z, nee ? 1.0f : 0.0f);
+        if (nee) {
+            const uint32_t lane = __lane_id();
+            for (uint32_t smp = 0; smp < c.count; ++smp) {
+                float3 origin = make_float3(0.0f, 0.0f, 0.0f), dir = origin, t3 = origin;
+                float sl = 0.0f;
+                bool q = nee_sample(a, c, smp, origin, dir, sl, t3);
+                uint32_t slot = i * w.nee + smp;
+                w.slots[slot] = make_float4(t3.x, t3.y, t3.z, q ? 1.0f : 0.0f);
+                uint64_t mask = __ballot(q);
+                if (q) {
+                    uint32_t leader = (uint32_t)(__ffsll((unsigned long long)mask) - 1);
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(&w.counters[1], (uint32_t)__popcll(mask));
+                    base = __shfl(base, (int)leader, 64);
+                    uint32_t at = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+                    w.shadow[2ull * at] = make_float4(origin.x, origin.y, origin.z, sl);
+                    w.shadow[2ull * at + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(slot));
+                }
+            }
+        }
+    }
+    w.shade[i] = sh;
+}
+
+struct ShadowSource {
+    const GiArgs* a;
+    const Wave* w;
+    uint32_t n;
+    __device__ uint32_t count() const { return n; }
+    template <class St>
+    __device__ bool begin(uint32_t i, lux::Walk& wk, St& st) {
+        float4 r0 = w->shadow[2ull * i], r1 = w->shadow[2ull * i + 1];
+        if (!admit_query(*a)) {
+            w->vis[__float_as_uint(r1.w)] = 2u;
+            return false;
+        }
+        lux::walk_begin(a->scene, wk, make_float3(r0.x, r0.y, r0.z), make_float3(r1.x, r1.y, r1.z), r0.w,
+                        lux::kWalkOccluded, st);
+        if (wk.phase != lux::kPhaseDone) return true;
+        finish(i, wk);
+        return false;
+    }
+    __device__ void finish(uint32_t i, const lux::Walk& wk) {
+        uint32_t slot = __float_as_uint(w->shadow[2ull * i + 1].w);
+        w->vis[slot] = (wk.h.err & lux::kHitOccluded) ? 1u : ((wk.h.err & lux::kErrUnsupported) ? 2u : 0u);
+    }
+};
+
+template <int Depth>
+__global__ __launch_bounds__(kWg) void k_gi_shadow(GiArgs a, Wave w) {
+    __shared__ uint2 lds[Depth][kWg];
+    lux::LaneStack<Depth, kWg> stk;
+    stk.col = &lds[0][threadIdx.x];
+    lux::NoStats st;
+    ShadowSource src{&a, &w, w.counters[1]};
+    lux::regroup_walks(a.scene, src, &w.counters[2], w.regroup, w.batch, stk, st);
+}
+
+__global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
+    uint64_t g = w.g0 + (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (g >= w.g1) return;
+    float3 o, d;
+    float lim;
+    bool terminal;
+    item_ray(a, g, 0u, o, d, lim, terminal);
+    uint32_t count = a.p.emissive_samples / 2u;
+    if (count < 1u) count = 1u;
+    float3 jsum = make_float3(0.0f, 0.0f, 0.0f);
+    float bsum = 0.0f;
+    uint32_t n = 0;
+    for (uint32_t r = 0; r < w.R; ++r) {
+        uint32_t i = (uint32_t)((g - w.g0) * w.R + r);
+        uint4 pr = w.prim[i];
+        if (pr.w & lux::kErrUnsupported) continue;
+        float3 rad = make_float3(0.0f, 0.0f, 0.0f);
+        float beta;
+        if (pr.y != lux::kEmpty) {
+            float4 sh = w.shade[i];
+            float3 rr = make_float3(sh.x, sh.y, sh.z);
+            if (sh.w != 0.0f) {
+                float3 sum = make_float3(0.0f, 0.0f, 0.0f);
+                bool ok = true;
+                for (uint32_t smp = 0; smp < count; ++smp) {
+                    uint32_t slot = i * w.nee + smp;
+                    float4 s4 = w.slots[slot];
+                    if (s4.w == 0.0f) continue;
+                    uint32_t v = w.vis[slot];
+                    if (v == 1u) continue;
+                    if (v == 2u) {
+                        ok = false;
+                        break;
+                    }
+                    sum.x = sum.x + s4.x;
+                    sum.y = sum.y + s4.y;
+                    sum.z = sum.z + s4.z;
+                }
+                if (!ok) continue;
+                float fc = (float)count;
+                rr = make_float3(rr.x + sum.x / fc, rr.y + sum.y / fc, rr.z + sum.z / fc);
+            }
+            rad = shade_end(rr);
+            beta = 0.0f;
+        } else {
+            beta = terminal ? 0.0f : 1.0f;
+        }
+        jsum = make_float3(jsum.x + rad.x, jsum.y + rad.y, jsum.z + rad.z);
+        bsum = bsum + beta;
+        n++;
+    }
+    float4 out;
+    if (n == 0) {
+        out = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    } else {
+        float fn = (float)n;
+        out = make_float4(jsum.x / fn, jsum.y / fn, jsum.z / fn, bsum / fn);
+    }
+    a.intervals[g] = out;
 }
 
 // The merge of one interval with no child contribution (its level is the
@@ -991,15 +1247,99 @@ extern "C" int lux_gi_set_frame(lux_ctx* ctx, uint32_t frame, uint32_t seed) {
     return LUX_OK;
 }
 
+namespace {
+// Wavefront trace of intervals [g0, g1) in chunks whose scratch stays under
+// about 2 GiB; the four stages per chunk run back to back on the stream.
+int trace_wavefront(lux_ctx* ctx, uint64_t g0, uint64_t g1) {
+    const lux_gi_params& p = ctx->gi;
+    const uint32_t R = p.rays_per_interval ? p.rays_per_interval : 1u;
+    const uint32_t nee = p.emissive_samples / 2u > 1u ? p.emissive_samples / 2u : 1u;
+    const uint64_t per_item = 32ull + (uint64_t)nee * (16ull + 4ull + 32ull);
+    uint64_t chunk_items = (2ull << 30) / per_item;
+    if (chunk_items < 1024) chunk_items = 1024;
+    uint64_t chunk_intervals = chunk_items / R;
+    if (chunk_intervals < 1) chunk_intervals = 1;
+    uint64_t total = g1 - g0;
+    if (chunk_intervals > total) chunk_intervals = total;
+    uint64_t items = chunk_intervals * R;
+    if (items * nee >= (1ull << 31)) return lux_fail(ctx, LUX_ERR_INVALID, "wavefront trace: too many NEE slots");
+    if (int rc = lux_ensure(ctx, ctx->gi_w_prim, 16ull * items)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->gi_w_shade, 16ull * items)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->gi_w_slots, 16ull * items * nee)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->gi_w_vis, 4ull * items * nee)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->gi_w_shadow, 32ull * items * nee)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->gi_w_counters, 16)) return rc;
+    GiArgs a = gi_args(ctx);
+    Wave w;
+    w.R = R;
+    w.nee = nee;
+    w.prim = static_cast<uint4*>(ctx->gi_w_prim.ptr);
+    w.shade = static_cast<float4*>(ctx->gi_w_shade.ptr);
+    w.slots = static_cast<float4*>(ctx->gi_w_slots.ptr);
+    w.vis = static_cast<uint32_t*>(ctx->gi_w_vis.ptr);
+    w.shadow = static_cast<float4*>(ctx->gi_w_shadow.ptr);
+    w.counters = static_cast<uint32_t*>(ctx->gi_w_counters.ptr);
+    w.regroup = ctx->regroup ? ctx->regroup : 1u;
+    w.batch = ctx->trace_batch ? ctx->trace_batch : 8u;
+    const uint32_t wpc = (ctx->waves_per_cu && ctx->waves_per_cu != 1000u) ? ctx->waves_per_cu : 20u;
+    const uint32_t pblocks = (uint32_t)ctx->cu_count * wpc / (kWg / 64);
+    hipStream_t st = ctx->stream;
+    for (uint64_t c0 = g0; c0 < g1; c0 += chunk_intervals) {
+        w.g0 = c0;
+        w.g1 = c0 + chunk_intervals < g1 ? c0 + chunk_intervals : g1;
+        uint32_t n_items = (uint32_t)((w.g1 - w.g0) * R);
+        if (int rc = lux_device_zero(ctx, w.counters, 16)) return rc;
+        uint32_t need = (n_items + kWg - 1) / kWg;
+        uint32_t pb = pblocks < need ? pblocks : need;
+        switch (ctx->trace_depth) {
+        case 4:
+            hipLaunchKernelGGL(k_gi_primary<4>, dim3(pb), dim3(kWg), 0, st, a, w);
+            break;
+        case 16:
+            hipLaunchKernelGGL(k_gi_primary<16>, dim3(pb), dim3(kWg), 0, st, a, w);
+            break;
+        default:
+            hipLaunchKernelGGL(k_gi_primary<8>, dim3(pb), dim3(kWg), 0, st, a, w);
+            break;
+        }
+        hipLaunchKernelGGL(k_gi_setup, dim3(need), dim3(kWg), 0, st, a, w);
+        switch (ctx->trace_depth) {
+        case 4:
+            hipLaunchKernelGGL(k_gi_shadow<4>, dim3(pblocks), dim3(kWg), 0, st, a, w);
+            break;
+        case 16:
+            hipLaunchKernelGGL(k_gi_shadow<16>, dim3(pblocks), dim3(kWg), 0, st, a, w);
+            break;
+        default:
+            hipLaunchKernelGGL(k_gi_shadow<8>, dim3(pblocks), dim3(kWg), 0, st, a, w);
+            break;
+        }
+        uint32_t ab = (uint32_t)((w.g1 - w.g0 + kWg - 1) / kWg);
+        hipLaunchKernelGGL(k_gi_accum, dim3(ab), dim3(kWg), 0, st, a, w);
+        LUX_HIP(ctx, hipGetLastError());
+    }
+    return LUX_OK;
+}
+}  // namespace
+
+extern "C" int lux_gi_set_trace_mode(lux_ctx* ctx, uint32_t mode) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (mode > 1u) return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_set_trace_mode: mode is 0 or 1");
+    ctx->gi_trace_mode = mode;
+    return LUX_OK;
+}
+
 extern "C" int lux_gi_trace_levels(lux_ctx* ctx, uint32_t first, uint32_t last) {
     if (!ctx) return LUX_ERR_INVALID;
     if (int rc = gi_check(ctx)) return rc;
     if (first > last || last >= ctx->gi.levels)
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_trace_levels: level range outside the grid");
-    GiArgs a = gi_args(ctx);
     uint64_t g0 = ctx->gi_offset[first], g1 = ctx->gi_offset[last + 1];
+    if (g1 == g0) return LUX_OK;
+    if (ctx->gi_trace_mode == 0u) return trace_wavefront(ctx, g0, g1);
+    GiArgs a = gi_args(ctx);
     uint32_t blocks = (uint32_t)((g1 - g0 + kWg - 1) / kWg);
-    if (blocks) hipLaunchKernelGGL(k_gi_trace, dim3(blocks), dim3(kWg), 0, ctx->stream, a, g0, g1);
+    hipLaunchKernelGGL(k_gi_trace, dim3(blocks), dim3(kWg), 0, ctx->stream, a, g0, g1);
     LUX_HIP(ctx, hipGetLastError());
     return LUX_OK;
 }

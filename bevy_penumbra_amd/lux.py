@@ -223,6 +223,7 @@ def load_library() -> C.CDLL:
         "lux_radiance_cascades_sanitized": (i32, [vp, vp]),
         "lux_radiance_cascades_gi_params": (i32, [vp, vp, vp, u32, u32, u32, u32, u32, vp]),
         "lux_gi_trace_levels": (i32, [vp, u32, u32]),
+        "lux_gi_set_trace_mode": (i32, [vp, u32]),
         "lux_gi_query_counters": (i32, [vp, vp, vp]),
         "lux_camera_receivers": (i32, [vp, vp, u32, u32, vp]),
         "lux_set_trace_config": (i32, [vp, vp]),
@@ -548,6 +549,10 @@ class Context:
         self._check(self.lib.lux_gi_reset_query_counters(self.h), "lux_gi_reset_query_counters")
 
     # ---- learned far-cascade surrogate ----
+    def gi_set_trace_mode(self, mode: int):
+        """0 wavefront (default), 1 fused; results are identical."""
+        self._check(self.lib.lux_gi_set_trace_mode(self.h, mode), "lux_gi_set_trace_mode")
+
     def gi_trace_levels(self, first: int, last: int):
         self._check(self.lib.lux_gi_trace_levels(self.h, first, last), "lux_gi_trace_levels")
 

@@ -254,6 +254,13 @@ int lux_gi_reset_query_counters(lux_ctx* ctx);
 /* Trace only levels first..last (inclusive) of the configured grid into the
  * interval array; lux_gi_trace is first = 0, last = levels - 1. */
 int lux_gi_trace_levels(lux_ctx* ctx, uint32_t first, uint32_t last);
+/* Interval trace organisation (same results either way): 0 (default)
+ * wavefront -- primary rays, shading setup, compacted NEE occlusion rays and
+ * an ordered accumulation, the traversals as regrouped walks under the
+ * context's trace configuration; 1 fused -- one thread per interval. */
+#define LUX_GI_TRACE_WAVEFRONT 0u
+#define LUX_GI_TRACE_FUSED 1u
+int lux_gi_set_trace_mode(lux_ctx* ctx, uint32_t mode);
 
 /* ---- learned far-cascade surrogate.  No counterpart in the reference: its
  *      README defers online neural radiance caching (README.md:384-385).

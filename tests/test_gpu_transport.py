@@ -150,6 +150,38 @@ def test_gpu_query_budget_unbounded_counts_every_query(lux_ctx):
     assert_bits(g["intervals"], o["intervals"], "intervals")
 
 
+@pytest.mark.parametrize("case", ["cornell_r2", "colonnade_reduced"])
+def test_gpu_wavefront_and_fused_traces_agree(lux_ctx, case):
+    """The wavefront interval trace (regrouped primary and NEE occlusion walks,
+    ordered accumulation) and the fused one-thread-per-interval kernel give
+    the same bits, and both the oracle's."""
+    if case == "cornell_r2":
+        sc, _ = scenes.cornell()
+        p = cornell_params(rays_per_interval=2, seed=5, frame=9)
+    else:
+        sc, _ = scenes.colonnade()
+        p = colonnade_params(dims=(16, 6, 8), spacing=2.0)
+    out = {}
+    for mode in (1, 0):
+        lux_ctx.gi_set_trace_mode(mode)
+        out[mode] = run_gpu(lux_ctx, sc, p)
+    lux_ctx.gi_set_trace_mode(0)
+    for key in ("intervals", "merged", "probes"):
+        assert_bits(out[0][key], out[1][key], key)
+    if case == "cornell_r2":
+        o = oracle_pipeline(sc, p)
+        assert_bits(out[0]["intervals"], o["intervals"], "intervals vs oracle")
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gpu_query_budget_denies_past_the_limit_in_both_modes(lux_ctx, mode):
+    lux_ctx.gi_set_trace_mode(mode)
+    try:
+        test_gpu_query_budget_denies_past_the_limit(lux_ctx)
+    finally:
+        lux_ctx.gi_set_trace_mode(0)
+
+
 def test_gpu_query_budget_denies_past_the_limit(lux_ctx):
     """admitTransportQuery (shaders/traversal.slang:20-38): a denied query is
     not a miss.  Past the limit every interval is either exactly the oracle's
