@@ -463,222 +463,6 @@ __global__ __launch_bounds__(kWg) void k_gi_trace(GiArgs a, uint64_t g_begin, ui
     a.intervals[g] = trace_interval(a, L, c, k, terminal, stk);
 }
 
-// ---------------------------------------------------------------- wavefront interval trace
-// k_gi_trace split into four launches so that the primary and the shadow
-// traversals run as regrouped walks (lane refill, lux::regroup_walks):
-//   k_gi_primary  closest-hit walk of every (interval, ray) item
-//   k_gi_setup    per hit: emission and NEE context; each NEE sample's
-//                 contribution goes to a fixed slot, its occlusion ray to a
-//                 compacted list (wave-aggregated append)
-//   k_gi_shadow   occlusion walks of the list -> visibility per slot
-//   k_gi_accum    per interval: the fused kernel's accumulation in sample
-//                 order (same operations, so the same bits)
-struct Wave {
-    uint64_t g0, g1;      // interval range of this chunk
-    uint32_t R, nee;      // rays per interval, NEE slots per ray
-    uint4* prim;          // per item: t bits, tri, inst, err
-    float4* shade;        // per item: emission rgb, w = 1 when NEE runs
-    float4* slots;        // per slot: contribution rgb, w = 1 when a query was issued
-    uint32_t* vis;        // per slot: 0 visible, 1 occluded, 2 unsupported
-    float4* shadow;       // per occlusion ray: (origin, limit), (direction, slot bits)
-    uint32_t* counters;   // [0] primary cursor, [1] shadow rays, [2] shadow cursor
-    uint32_t regroup, batch;
-};
-
-// The ray of item (interval g, ray r): exactly trace_interval's construction.
-__device__ __forceinline__ void item_ray(const GiArgs& a, uint64_t g, uint32_t r, float3& o, float3& d,
-                                         float& limit, bool& terminal) {
-    const lux_gi_params& p = a.p;
-    uint32_t L = 0;
-    while (L + 1 < p.levels && g >= a.offset[L + 1]) ++L;
-    uint64_t local = g - a.offset[L];
-    uint32_t nphi = p.base_phi << L, ny = p.base_y << L;
-    uint32_t D = nphi * ny;
-    uint32_t k = (uint32_t)(local % D);
-    float3 c = probe_centre(p, L, local / D);
-    float end = interval_end(p, L);
-    float start = L == 0 ? 0.0f : interval_end(p, L - 1);
-    terminal = p.terminal_tail != 0u && L + 1 == p.levels;
-    limit = terminal ? 1e20f : end - start;
-    float2 ouv = ray_offset_uv(p.seed, r);
-    d = bin_direction(k, nphi, ny, ouv.x, ouv.y);
-    o = make_float3(c.x + d.x * start, c.y + d.y * start, c.z + d.z * start);
-}
-
-struct PrimarySource {
-    const GiArgs* a;
-    const Wave* w;
-    __device__ uint32_t count() const { return (uint32_t)((w->g1 - w->g0) * w->R); }
-    template <class St>
-    __device__ bool begin(uint32_t i, lux::Walk& wk, St& st) {
-        float3 o, d;
-        float lim;
-        bool term;
-        item_ray(*a, w->g0 + i / w->R, i % w->R, o, d, lim, term);
-        if (!admit_query(*a)) {  // denied: no sample, as an unsupported traversal
-            w->prim[i] = make_uint4(0u, lux::kEmpty, lux::kEmpty, lux::kErrUnsupported);
-            return false;
-        }
-        lux::walk_begin(a->scene, wk, o, d, lim, lux::kWalkClosest, st);
-        if (wk.phase != lux::kPhaseDone) return true;
-        finish(i, wk);
-        return false;
-    }
-    __device__ void finish(uint32_t i, const lux::Walk& wk) {
-        w->prim[i] = make_uint4(__float_as_uint(wk.h.t), wk.h.tri, wk.h.inst, wk.h.err);
-    }
-};
-
-template <int Depth>
-__global__ __launch_bounds__(kWg) void k_gi_primary(GiArgs a, Wave w) {
-    __shared__ uint2 lds[Depth][kWg];
-    lux::LaneStack<Depth, kWg> stk;
-    stk.col = &lds[0][threadIdx.x];
-    lux::NoStats st;
-    PrimarySource src{&a, &w};
-    lux::regroup_walks(a.scene, src, &w.counters[0], w.regroup, w.batch, stk, st);
-}
-
-__global__ __launch_bounds__(kWg) void k_gi_setup(GiArgs a, Wave w) {
-    uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    uint32_t items = (uint32_t)((w.g1 - w.g0) * w.R);
-    if (i >= items) return;
-    uint4 pr = w.prim[i];
-    float4 sh = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if ((pr.w & lux::kErrUnsupported) == 0u && pr.y != lux::kEmpty) {
-        float3 o, d;
-        float lim;
-        bool term;
-        item_ray(a, w.g0 + i / w.R, i % w.R, o, d, lim, term);
-        lux::HitState h;
-        h.t = __uint_as_float(pr.x);
-        h.tri = pr.y;
-        h.inst = pr.z;
-        h.err = pr.w;
-        h.b1 = h.b2 = 0.0f;
-        float3 rad;
-        NeeCtx c;
-        bool nee = shade_begin(a, o, d, h, rad, c);
-        sh = make_float4(rad.x, rad.y, rad.z, nee ? 1.0f : 0.0f);
-        if (nee) {
-            const uint32_t lane = __lane_id();
-            for (uint32_t smp = 0; smp < c.count; ++smp) {
-                float3 origin = make_float3(0.0f, 0.0f, 0.0f), dir = origin, t3 = origin;
-                float sl = 0.0f;
-                bool q = nee_sample(a, c, smp, origin, dir, sl, t3);
-                uint32_t slot = i * w.nee + smp;
-                w.slots[slot] = make_float4(t3.x, t3.y, t3.z, q ? 1.0f : 0.0f);
-                uint64_t mask = __ballot(q);
-                if (q) {
-                    uint32_t leader = (uint32_t)(__ffsll((unsigned long long)mask) - 1);
-                    uint32_t base = 0;
-                    if (lane == leader) base = atomicAdd(&w.counters[1], (uint32_t)__popcll(mask));
-                    base = __shfl(base, (int)leader, 64);
-                    uint32_t at = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-                    w.shadow[2ull * at] = make_float4(origin.x, origin.y, origin.z, sl);
-                    w.shadow[2ull * at + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(slot));
-                }
-            }
-        }
-    }
-    w.shade[i] = sh;
-}
-
-struct ShadowSource {
-    const GiArgs* a;
-    const Wave* w;
-    uint32_t n;
-    __device__ uint32_t count() const { return n; }
-    template <class St>
-    __device__ bool begin(uint32_t i, lux::Walk& wk, St& st) {
-        float4 r0 = w->shadow[2ull * i], r1 = w->shadow[2ull * i + 1];
-        if (!admit_query(*a)) {
-            w->vis[__float_as_uint(r1.w)] = 2u;
-            return false;
-        }
-        lux::walk_begin(a->scene, wk, make_float3(r0.x, r0.y, r0.z), make_float3(r1.x, r1.y, r1.z), r0.w,
-                        lux::kWalkOccluded, st);
-        if (wk.phase != lux::kPhaseDone) return true;
-        finish(i, wk);
-        return false;
-    }
-    __device__ void finish(uint32_t i, const lux::Walk& wk) {
-        uint32_t slot = __float_as_uint(w->shadow[2ull * i + 1].w);
-        w->vis[slot] = (wk.h.err & lux::kHitOccluded) ? 1u : ((wk.h.err & lux::kErrUnsupported) ? 2u : 0u);
-    }
-};
-
-template <int Depth>
-__global__ __launch_bounds__(kWg) void k_gi_shadow(GiArgs a, Wave w) {
-    __shared__ uint2 lds[Depth][kWg];
-    lux::LaneStack<Depth, kWg> stk;
-    stk.col = &lds[0][threadIdx.x];
-    lux::NoStats st;
-    ShadowSource src{&a, &w, w.counters[1]};
-    lux::regroup_walks(a.scene, src, &w.counters[2], w.regroup, w.batch, stk, st);
-}
-
-__global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
-    uint64_t g = w.g0 + (uint64_t)blockIdx.x * kWg + threadIdx.x;
-    if (g >= w.g1) return;
-    float3 o, d;
-    float lim;
-    bool terminal;
-    item_ray(a, g, 0u, o, d, lim, terminal);
-    uint32_t count = a.p.emissive_samples / 2u;
-    if (count < 1u) count = 1u;
-    float3 jsum = make_float3(0.0f, 0.0f, 0.0f);
-    float bsum = 0.0f;
-    uint32_t n = 0;
-    for (uint32_t r = 0; r < w.R; ++r) {
-        uint32_t i = (uint32_t)((g - w.g0) * w.R + r);
-        uint4 pr = w.prim[i];
-        if (pr.w & lux::kErrUnsupported) continue;
-        float3 rad = make_float3(0.0f, 0.0f, 0.0f);
-        float beta;
-        if (pr.y != lux::kEmpty) {
-            float4 sh = w.shade[i];
-            float3 rr = make_float3(sh.x, sh.y, sh.z);
-            if (sh.w != 0.0f) {
-                float3 sum = make_float3(0.0f, 0.0f, 0.0f);
-                bool ok = true;
-                for (uint32_t smp = 0; smp < count; ++smp) {
-                    uint32_t slot = i * w.nee + smp;
-                    float4 s4 = w.slots[slot];
-                    if (s4.w == 0.0f) continue;
-                    uint32_t v = w.vis[slot];
-                    if (v == 1u) continue;
-                    if (v == 2u) {
-                        ok = false;
-                        break;
-                    }
-                    sum.x = sum.x + s4.x;
-                    sum.y = sum.y + s4.y;
-                    sum.z = sum.z + s4.z;
-                }
-                if (!ok) continue;
-                float fc = (float)count;
-                rr = make_float3(rr.x + sum.x / fc, rr.y + sum.y / fc, rr.z + sum.z / fc);
-            }
-            rad = shade_end(rr);
-            beta = 0.0f;
-        } else {
-            beta = terminal ? 0.0f : 1.0f;
-        }
-        jsum = make_float3(jsum.x + rad.x, jsum.y + rad.y, jsum.z + rad.z);
-        bsum = bsum + beta;
-        n++;
-    }
-    float4 out;
-    if (n == 0) {
-        out = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-    } else {
-        float fn = (float)n;
-        out = make_float4(jsum.x / fn, jsum.y / fn, jsum.z / fn, bsum / fn);
-    }
-    a.intervals[g] = out;
-}
-
 // The merge of one interval with no child contribution (its level is the
 // last one traced): Eq. 7 with an empty child sum.
 __device__ __forceinline__ float4 merge_leaf(float4 iv) {
@@ -783,6 +567,287 @@ __global__ __launch_bounds__(kWg) void k_sur_fallback(GiArgs a, uint32_t L, cons
     uint32_t k = (uint32_t)(g % D);
     float3 c = probe_centre(p, L, probe);
     a.merged[a.offset[L] + (uint64_t)probe * D + k] = merge_leaf(trace_interval(a, L, c, k, true, stk));
+}
+
+// ---------------------------------------------------------------- wavefront interval trace
+// The fused interval trace split into four launches so that the primary and
+// the shadow traversals run as regrouped walks (lux::regroup_walks):
+//   k_gi_primary  closest-hit walk of every (interval, ray) item
+//   k_gi_setup    per hit: emission and NEE context; each NEE sample's
+//                 contribution goes to a fixed slot, its occlusion ray to a
+//                 compacted list (wave-aggregated append)
+//   k_gi_shadow   occlusion walks of the list -> visibility per slot
+//   k_gi_accum    per interval: the fused kernel's accumulation in sample
+//                 order (same operations, so the same bits)
+// Three kinds of interval sets share the stages:
+//   kWaveRange    intervals [g0, g1) -> interval array (J, beta)
+//   kWaveListed   every direction of the listed probes of one level as its
+//                 unbounded terminal interval -> merged array (the
+//                 surrogate's traced fallback)
+//   kWaveSampled  hashed (probe, direction) samples of one level as terminal
+//                 intervals -> 12-float training records
+constexpr uint32_t kWaveRange = 0u, kWaveListed = 1u, kWaveSampled = 2u;
+
+struct Wave {
+    uint32_t kind;
+    uint64_t g0, g1;             // kWaveRange: interval range of this chunk
+    uint32_t level;              // kWaveListed / kWaveSampled
+    const uint32_t* list;        // kWaveListed: probe list and its device-side length
+    const uint32_t* list_count;
+    uint32_t first, n;           // kWaveListed: first listed probe of the chunk, probes in it
+                                 // kWaveSampled: first sample of the chunk, samples in it
+    uint32_t seed;               // kWaveSampled: sample seed
+    float* sample_out;           // kWaveSampled: 12 floats per sample
+    uint32_t R, nee;             // rays per interval, NEE slots per ray
+    uint4* prim;                 // per item: t bits, tri, inst, err
+    float4* shade;               // per item: emission rgb, w = 1 when NEE runs
+    float4* slots;               // per slot: contribution rgb, w = 1 when a query was issued
+    uint32_t* vis;               // per slot: 0 visible, 1 occluded, 2 unsupported
+    float4* shadow;              // per occlusion ray: (origin, limit), (direction, slot bits)
+    uint32_t* counters;          // [0] primary cursor, [1] shadow rays, [2] shadow cursor
+    uint32_t regroup, batch;
+};
+
+// Intervals of the chunk (kWaveListed: listed probes beyond the list's
+// device-side length contribute none).
+__device__ __forceinline__ uint32_t wave_intervals(const GiArgs& a, const Wave& w) {
+    if (w.kind == kWaveRange) return (uint32_t)(w.g1 - w.g0);
+    if (w.kind == kWaveSampled) return w.n;
+    uint32_t total = *w.list_count;
+    uint32_t probes = total > w.first ? total - w.first : 0u;
+    if (probes > w.n) probes = w.n;
+    return probes * ((a.p.base_phi << w.level) * (a.p.base_y << w.level));
+}
+
+// Global interval index of the chunk's j-th interval and whether it is traced
+// as the unbounded terminal interval of its level.
+__device__ __forceinline__ uint64_t wave_interval(const GiArgs& a, const Wave& w, uint32_t j, bool& force) {
+    force = w.kind != kWaveRange;
+    if (w.kind == kWaveRange) return w.g0 + j;
+    uint64_t count = a.offset[w.level + 1] - a.offset[w.level];
+    if (w.kind == kWaveSampled) {
+        uint32_t h = transport_hash((w.first + j) + transport_hash(w.seed ^ 0x3c6ef372u));
+        return a.offset[w.level] + (((uint64_t)h * count) >> 32);
+    }
+    uint32_t D = (a.p.base_phi << w.level) * (a.p.base_y << w.level);
+    uint32_t probe = w.list[w.first + j / D];
+    return a.offset[w.level] + (uint64_t)probe * D + (j % D);
+}
+
+// The ray of item (interval g, ray r): exactly trace_interval's construction.
+__device__ __forceinline__ void item_ray(const GiArgs& a, uint64_t g, uint32_t r, bool force_terminal,
+                                         float3& o, float3& d, float& limit, bool& terminal) {
+    const lux_gi_params& p = a.p;
+    uint32_t L = 0;
+    while (L + 1 < p.levels && g >= a.offset[L + 1]) ++L;
+    uint64_t local = g - a.offset[L];
+    uint32_t nphi = p.base_phi << L, ny = p.base_y << L;
+    uint32_t D = nphi * ny;
+    uint32_t k = (uint32_t)(local % D);
+    float3 c = probe_centre(p, L, local / D);
+    float end = interval_end(p, L);
+    float start = L == 0 ? 0.0f : interval_end(p, L - 1);
+    terminal = force_terminal || (p.terminal_tail != 0u && L + 1 == p.levels);
+    limit = terminal ? 1e20f : end - start;
+    float2 ouv = ray_offset_uv(p.seed, r);
+    d = bin_direction(k, nphi, ny, ouv.x, ouv.y);
+    o = make_float3(c.x + d.x * start, c.y + d.y * start, c.z + d.z * start);
+}
+
+__device__ __forceinline__ void wave_item_ray(const GiArgs& a, const Wave& w, uint32_t i, float3& o, float3& d,
+                                              float& lim, bool& terminal) {
+    bool force;
+    uint64_t g = wave_interval(a, w, i / w.R, force);
+    item_ray(a, g, i % w.R, force, o, d, lim, terminal);
+}
+
+struct PrimarySource {
+    const GiArgs* a;
+    const Wave* w;
+    uint32_t n;
+    __device__ uint32_t count() const { return n; }
+    template <class St>
+    __device__ bool begin(uint32_t i, lux::Walk& wk, St& st) {
+        float3 o, d;
+        float lim;
+        bool term;
+        wave_item_ray(*a, *w, i, o, d, lim, term);
+        if (!admit_query(*a)) {  // denied: no sample, as an unsupported traversal
+            w->prim[i] = make_uint4(0u, lux::kEmpty, lux::kEmpty, lux::kErrUnsupported);
+            return false;
+        }
+        lux::walk_begin(a->scene, wk, o, d, lim, lux::kWalkClosest, st);
+        if (wk.phase != lux::kPhaseDone) return true;
+        finish(i, wk);
+        return false;
+    }
+    __device__ void finish(uint32_t i, const lux::Walk& wk) {
+        w->prim[i] = make_uint4(__float_as_uint(wk.h.t), wk.h.tri, wk.h.inst, wk.h.err);
+    }
+};
+
+template <int Depth>
+__global__ __launch_bounds__(kWg) void k_gi_primary(GiArgs a, Wave w) {
+    __shared__ uint2 lds[Depth][kWg];
+    lux::LaneStack<Depth, kWg> stk;
+    stk.col = &lds[0][threadIdx.x];
+    lux::NoStats st;
+    PrimarySource src{&a, &w, wave_intervals(a, w) * w.R};
+    lux::regroup_walks(a.scene, src, &w.counters[0], w.regroup, w.batch, stk, st);
+}
+
+__global__ __launch_bounds__(kWg) void k_gi_setup(GiArgs a, Wave w) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= wave_intervals(a, w) * w.R) return;
+    uint4 pr = w.prim[i];
+    float4 sh = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if ((pr.w & lux::kErrUnsupported) == 0u && pr.y != lux::kEmpty) {
+        float3 o, d;
+        float lim;
+        bool term;
+        wave_item_ray(a, w, i, o, d, lim, term);
+        lux::HitState h;
+        h.t = __uint_as_float(pr.x);
+        h.tri = pr.y;
+        h.inst = pr.z;
+        h.err = pr.w;
+        h.b1 = h.b2 = 0.0f;
+        float3 rad;
+        NeeCtx c;
+        bool nee = shade_begin(a, o, d, h, rad, c);
+        sh = make_float4(rad.x, rad.y, rad.z, nee ? 1.0f : 0.0f);
+        if (nee) {
+            const uint32_t lane = __lane_id();
+            for (uint32_t smp = 0; smp < c.count; ++smp) {
+                float3 origin = make_float3(0.0f, 0.0f, 0.0f), dir = origin, t3 = origin;
+                float sl = 0.0f;
+                bool q = nee_sample(a, c, smp, origin, dir, sl, t3);
+                uint32_t slot = i * w.nee + smp;
+                w.slots[slot] = make_float4(t3.x, t3.y, t3.z, q ? 1.0f : 0.0f);
+                uint64_t mask = __ballot(q);
+                if (q) {
+                    uint32_t leader = (uint32_t)(__ffsll((unsigned long long)mask) - 1);
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(&w.counters[1], (uint32_t)__popcll(mask));
+                    base = __shfl(base, (int)leader, 64);
+                    uint32_t at = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+                    w.shadow[2ull * at] = make_float4(origin.x, origin.y, origin.z, sl);
+                    w.shadow[2ull * at + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(slot));
+                }
+            }
+        }
+    }
+    w.shade[i] = sh;
+}
+
+struct ShadowSource {
+    const GiArgs* a;
+    const Wave* w;
+    uint32_t n;
+    __device__ uint32_t count() const { return n; }
+    template <class St>
+    __device__ bool begin(uint32_t i, lux::Walk& wk, St& st) {
+        float4 r0 = w->shadow[2ull * i], r1 = w->shadow[2ull * i + 1];
+        if (!admit_query(*a)) {
+            w->vis[__float_as_uint(r1.w)] = 2u;
+            return false;
+        }
+        lux::walk_begin(a->scene, wk, make_float3(r0.x, r0.y, r0.z), make_float3(r1.x, r1.y, r1.z), r0.w,
+                        lux::kWalkOccluded, st);
+        if (wk.phase != lux::kPhaseDone) return true;
+        finish(i, wk);
+        return false;
+    }
+    __device__ void finish(uint32_t i, const lux::Walk& wk) {
+        uint32_t slot = __float_as_uint(w->shadow[2ull * i + 1].w);
+        w->vis[slot] = (wk.h.err & lux::kHitOccluded) ? 1u : ((wk.h.err & lux::kErrUnsupported) ? 2u : 0u);
+    }
+};
+
+template <int Depth>
+__global__ __launch_bounds__(kWg) void k_gi_shadow(GiArgs a, Wave w) {
+    __shared__ uint2 lds[Depth][kWg];
+    lux::LaneStack<Depth, kWg> stk;
+    stk.col = &lds[0][threadIdx.x];
+    lux::NoStats st;
+    ShadowSource src{&a, &w, w.counters[1]};
+    lux::regroup_walks(a.scene, src, &w.counters[2], w.regroup, w.batch, stk, st);
+}
+
+__global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
+    uint32_t j = blockIdx.x * kWg + threadIdx.x;
+    if (j >= wave_intervals(a, w)) return;
+    bool force;
+    uint64_t g = wave_interval(a, w, j, force);
+    float3 o, d;
+    float lim;
+    bool terminal;
+    item_ray(a, g, 0u, force, o, d, lim, terminal);
+    uint32_t count = a.p.emissive_samples / 2u;
+    if (count < 1u) count = 1u;
+    float3 jsum = make_float3(0.0f, 0.0f, 0.0f);
+    float bsum = 0.0f;
+    uint32_t n = 0;
+    for (uint32_t r = 0; r < w.R; ++r) {
+        uint32_t i = j * w.R + r;
+        uint4 pr = w.prim[i];
+        if (pr.w & lux::kErrUnsupported) continue;
+        float3 rad = make_float3(0.0f, 0.0f, 0.0f);
+        float beta;
+        if (pr.y != lux::kEmpty) {
+            float4 sh = w.shade[i];
+            float3 rr = make_float3(sh.x, sh.y, sh.z);
+            if (sh.w != 0.0f) {
+                float3 sum = make_float3(0.0f, 0.0f, 0.0f);
+                bool ok = true;
+                for (uint32_t smp = 0; smp < count; ++smp) {
+                    uint32_t slot = i * w.nee + smp;
+                    float4 s4 = w.slots[slot];
+                    if (s4.w == 0.0f) continue;
+                    uint32_t v = w.vis[slot];
+                    if (v == 1u) continue;
+                    if (v == 2u) {
+                        ok = false;
+                        break;
+                    }
+                    sum.x = sum.x + s4.x;
+                    sum.y = sum.y + s4.y;
+                    sum.z = sum.z + s4.z;
+                }
+                if (!ok) continue;
+                float fc = (float)count;
+                rr = make_float3(rr.x + sum.x / fc, rr.y + sum.y / fc, rr.z + sum.z / fc);
+            }
+            rad = shade_end(rr);
+            beta = 0.0f;
+        } else {
+            beta = terminal ? 0.0f : 1.0f;
+        }
+        jsum = make_float3(jsum.x + rad.x, jsum.y + rad.y, jsum.z + rad.z);
+        bsum = bsum + beta;
+        n++;
+    }
+    float4 out;
+    if (n == 0) {
+        out = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    } else {
+        float fn = (float)n;
+        out = make_float4(jsum.x / fn, jsum.y / fn, jsum.z / fn, bsum / fn);
+    }
+    if (w.kind == kWaveRange) {
+        a.intervals[g] = out;
+    } else if (w.kind == kWaveListed) {
+        a.merged[g] = merge_leaf(out);
+    } else {
+        float* rec = w.sample_out + 12ull * (w.first + j);
+        level_query(a.p, w.level, g - a.offset[w.level], rec);
+        float4 m = merge_leaf(out);
+        bool ok = m.w > 0.0f;
+        float nan = __uint_as_float(0x7fc00000u);
+        rec[9] = ok ? m.x : nan;
+        rec[10] = ok ? m.y : nan;
+        rec[11] = ok ? m.z : nan;
+    }
 }
 
 // Support-weighted trilinear child lookup at level Lc.
@@ -1248,20 +1313,23 @@ extern "C" int lux_gi_set_frame(lux_ctx* ctx, uint32_t frame, uint32_t seed) {
 }
 
 namespace {
-// Wavefront trace of intervals [g0, g1) in chunks whose scratch stays under
-// about 2 GiB; the four stages per chunk run back to back on the stream.
-int trace_wavefront(lux_ctx* ctx, uint64_t g0, uint64_t g1) {
+// Runs the four wavefront stages over a set of intervals in chunks whose
+// scratch stays under about 2 GiB.  `proto` carries the kind and its inputs;
+// `total` is the number of intervals (kWaveRange), listed probes
+// (kWaveListed, an upper bound: the device-side list length decides) or
+// samples (kWaveSampled); `per_unit` intervals per unit (D for listed
+// probes, else 1).
+int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
     const lux_gi_params& p = ctx->gi;
     const uint32_t R = p.rays_per_interval ? p.rays_per_interval : 1u;
     const uint32_t nee = p.emissive_samples / 2u > 1u ? p.emissive_samples / 2u : 1u;
     const uint64_t per_item = 32ull + (uint64_t)nee * (16ull + 4ull + 32ull);
     uint64_t chunk_items = (2ull << 30) / per_item;
-    if (chunk_items < 1024) chunk_items = 1024;
-    uint64_t chunk_intervals = chunk_items / R;
-    if (chunk_intervals < 1) chunk_intervals = 1;
-    uint64_t total = g1 - g0;
-    if (chunk_intervals > total) chunk_intervals = total;
-    uint64_t items = chunk_intervals * R;
+    if (chunk_items < 4096) chunk_items = 4096;
+    uint64_t chunk_units = chunk_items / (R * per_unit);
+    if (chunk_units < 1) chunk_units = 1;
+    if (chunk_units > total) chunk_units = total;
+    uint64_t items = chunk_units * per_unit * R;
     if (items * nee >= (1ull << 31)) return lux_fail(ctx, LUX_ERR_INVALID, "wavefront trace: too many NEE slots");
     if (int rc = lux_ensure(ctx, ctx->gi_w_prim, 16ull * items)) return rc;
     if (int rc = lux_ensure(ctx, ctx->gi_w_shade, 16ull * items)) return rc;
@@ -1270,7 +1338,7 @@ int trace_wavefront(lux_ctx* ctx, uint64_t g0, uint64_t g1) {
     if (int rc = lux_ensure(ctx, ctx->gi_w_shadow, 32ull * items * nee)) return rc;
     if (int rc = lux_ensure(ctx, ctx->gi_w_counters, 16)) return rc;
     GiArgs a = gi_args(ctx);
-    Wave w;
+    Wave w = proto;
     w.R = R;
     w.nee = nee;
     w.prim = static_cast<uint4*>(ctx->gi_w_prim.ptr);
@@ -1284,41 +1352,49 @@ int trace_wavefront(lux_ctx* ctx, uint64_t g0, uint64_t g1) {
     const uint32_t wpc = (ctx->waves_per_cu && ctx->waves_per_cu != 1000u) ? ctx->waves_per_cu : 20u;
     const uint32_t pblocks = (uint32_t)ctx->cu_count * wpc / (kWg / 64);
     hipStream_t st = ctx->stream;
-    for (uint64_t c0 = g0; c0 < g1; c0 += chunk_intervals) {
-        w.g0 = c0;
-        w.g1 = c0 + chunk_intervals < g1 ? c0 + chunk_intervals : g1;
-        uint32_t n_items = (uint32_t)((w.g1 - w.g0) * R);
+    for (uint64_t u0 = 0; u0 < total; u0 += chunk_units) {
+        uint64_t units = total - u0 < chunk_units ? total - u0 : chunk_units;
+        if (w.kind == kWaveRange) {
+            w.g0 = proto.g0 + u0;
+            w.g1 = w.g0 + units;
+        } else {
+            w.first = (uint32_t)u0;
+            w.n = (uint32_t)units;
+        }
+        uint32_t max_intervals = (uint32_t)(units * per_unit);
+        uint32_t max_items = max_intervals * R;
         if (int rc = lux_device_zero(ctx, w.counters, 16)) return rc;
-        uint32_t need = (n_items + kWg - 1) / kWg;
+        uint32_t need = (max_items + kWg - 1) / kWg;
         uint32_t pb = pblocks < need ? pblocks : need;
         switch (ctx->trace_depth) {
-        case 4:
-            hipLaunchKernelGGL(k_gi_primary<4>, dim3(pb), dim3(kWg), 0, st, a, w);
-            break;
-        case 16:
-            hipLaunchKernelGGL(k_gi_primary<16>, dim3(pb), dim3(kWg), 0, st, a, w);
-            break;
-        default:
-            hipLaunchKernelGGL(k_gi_primary<8>, dim3(pb), dim3(kWg), 0, st, a, w);
-            break;
+        case 4: hipLaunchKernelGGL(k_gi_primary<4>, dim3(pb), dim3(kWg), 0, st, a, w); break;
+        case 16: hipLaunchKernelGGL(k_gi_primary<16>, dim3(pb), dim3(kWg), 0, st, a, w); break;
+        default: hipLaunchKernelGGL(k_gi_primary<8>, dim3(pb), dim3(kWg), 0, st, a, w); break;
         }
         hipLaunchKernelGGL(k_gi_setup, dim3(need), dim3(kWg), 0, st, a, w);
         switch (ctx->trace_depth) {
-        case 4:
-            hipLaunchKernelGGL(k_gi_shadow<4>, dim3(pblocks), dim3(kWg), 0, st, a, w);
-            break;
-        case 16:
-            hipLaunchKernelGGL(k_gi_shadow<16>, dim3(pblocks), dim3(kWg), 0, st, a, w);
-            break;
-        default:
-            hipLaunchKernelGGL(k_gi_shadow<8>, dim3(pblocks), dim3(kWg), 0, st, a, w);
-            break;
+        case 4: hipLaunchKernelGGL(k_gi_shadow<4>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
+        case 16: hipLaunchKernelGGL(k_gi_shadow<16>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
+        default: hipLaunchKernelGGL(k_gi_shadow<8>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
         }
-        uint32_t ab = (uint32_t)((w.g1 - w.g0 + kWg - 1) / kWg);
-        hipLaunchKernelGGL(k_gi_accum, dim3(ab), dim3(kWg), 0, st, a, w);
+        hipLaunchKernelGGL(k_gi_accum, dim3((max_intervals + kWg - 1) / kWg), dim3(kWg), 0, st, a, w);
         LUX_HIP(ctx, hipGetLastError());
     }
     return LUX_OK;
+}
+
+Wave wave_proto(uint32_t kind) {
+    Wave w;
+    std::memset(&w, 0, sizeof w);
+    w.kind = kind;
+    return w;
+}
+
+int trace_wavefront(lux_ctx* ctx, uint64_t g0, uint64_t g1) {
+    Wave w = wave_proto(kWaveRange);
+    w.g0 = g0;
+    w.g1 = g1;
+    return run_wave(ctx, w, g1 - g0, 1);
 }
 }  // namespace
 
@@ -1498,6 +1574,13 @@ extern "C" int lux_gi_surrogate_samples(lux_ctx* ctx, uint32_t level, uint32_t n
     if (level >= ctx->gi.levels) return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_surrogate_samples: level");
     if (n == 0) return LUX_OK;
     if (!samples) return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_surrogate_samples: null buffer");
+    if (source == 0u && ctx->gi_trace_mode == 0u) {
+        Wave w = wave_proto(kWaveSampled);
+        w.level = level;
+        w.seed = sample_seed;
+        w.sample_out = samples;
+        return run_wave(ctx, w, n, 1);
+    }
     GiArgs a = gi_args(ctx);
     hipLaunchKernelGGL(k_sur_samples, dim3((n + kWg - 1) / kWg), dim3(kWg), 0, ctx->stream, a, level, n,
                        sample_seed, source, samples);
@@ -1542,9 +1625,17 @@ extern "C" int lux_gi_update_surrogate(lux_ctx* ctx, lux_surrogate* s, uint32_t 
                        static_cast<float*>(ctx->gi_sur_err.ptr), list, fcount);
     LUX_HIP(ctx, hipGetLastError());
     LUX_HIP(ctx, hipEventRecord(ctx->ev_phase[2], st));
-    hipLaunchKernelGGL(k_sur_fallback, dim3(qblocks), dim3(kWg), 0, st, a, level, list,
-                       static_cast<const uint32_t*>(fcount));
-    LUX_HIP(ctx, hipGetLastError());
+    if (ctx->gi_trace_mode == 0u) {
+        Wave w = wave_proto(kWaveListed);
+        w.level = level;
+        w.list = list;
+        w.list_count = fcount;
+        if (int rc = run_wave(ctx, w, probes, D)) return rc;
+    } else {
+        hipLaunchKernelGGL(k_sur_fallback, dim3(qblocks), dim3(kWg), 0, st, a, level, list,
+                           static_cast<const uint32_t*>(fcount));
+        LUX_HIP(ctx, hipGetLastError());
+    }
     LUX_HIP(ctx, hipEventRecord(ctx->ev_phase[3], st));
     if (int rc = merge_levels(ctx, (int)level - 1)) return rc;
     if (int rc = lux_gi_pack(ctx)) return rc;

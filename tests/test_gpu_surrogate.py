@@ -226,7 +226,15 @@ def cornell_surrogate(ctx, seed=5):
     return lux.Surrogate(ctx, p)
 
 
-def test_gpu_surrogate_all_fallback_equals_plain(cornell_ctx):
+@pytest.fixture(params=[0, 1], ids=["wavefront", "fused"])
+def trace_mode(request, cornell_ctx):
+    ctx, _ = cornell_ctx
+    ctx.gi_set_trace_mode(request.param)
+    yield request.param
+    ctx.gi_set_trace_mode(0)
+
+
+def test_gpu_surrogate_all_fallback_equals_plain(cornell_ctx, trace_mode):
     ctx, sc = cornell_ctx
     level = 2
     p4 = cornell_params(frame=3, seed=9)
@@ -284,7 +292,7 @@ def test_gpu_surrogate_without_fallback_merges_predictions(cornell_ctx):
     s.close()
 
 
-def test_gpu_surrogate_samples(cornell_ctx):
+def test_gpu_surrogate_samples(cornell_ctx, trace_mode):
     ctx, sc = cornell_ctx
     level, n, seed = 2, 5000, 11
     p4 = cornell_params(frame=2, seed=6)
