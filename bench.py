@@ -191,6 +191,10 @@ def dynamic_section(lux, scenes, args) -> dict:
         full.append((time.perf_counter() - t0) * 1e3)
     gp = colonnade_gi_params(lux)
     ctx.gi_configure(gp)
+    if not args.no_autotune:
+        res_tune = ctx.autotune_gi(repeats=1)
+    else:
+        res_tune = None
     W, H = 1920, 1080
     recv = DeviceBuffer.empty(ctx, (W * H, 6), np.float32)
     out = DeviceBuffer.empty(ctx, (W * H, 4), np.float32)
@@ -203,6 +207,7 @@ def dynamic_section(lux, scenes, args) -> dict:
     res = {"workload": "1,048,576 triangles (256 meshes x 4,096) in 256 rigid instances, 64x24x32 "
                        "probe grid at 0.5 m, 5 levels, 1920x1080 receivers",
            "first_build_ms": round(first_ms, 3),
+           "gi_trace_config": ctx.gi_trace_config(),
            "full_rebuild_ms": round(float(np.median(full)), 3)}
     for mode in ("plain", "surrogate"):
         times, tlas = [], []
@@ -292,6 +297,12 @@ def main() -> int:
         gp = colonnade_gi_params(lux)
         ctx.gi_configure(gp)
         n_int, n_probes, n_lights = ctx.gi_sizes()
+        gi_tune = None
+        if not args.no_autotune:
+            t0 = time.perf_counter()
+            gi_tune = ctx.autotune_gi(repeats=2)
+            gi_tune["tuning_s"] = round(time.perf_counter() - t0, 2)
+            gi_tune["ms"] = round(gi_tune["ms"], 4)
         ctx.gi_update()
         ctx.synchronize()
         gi_ms = [ctx.gi_update(timed=True) for _ in range(args.gi_steps)]
@@ -303,6 +314,9 @@ def main() -> int:
             "probes": n_probes,
             "emitter_triangles": n_lights,
             "grid": "64x24x32 at 0.5 m, 5 levels, 32 base directions, NEE 8 samples",
+            "trace": "wavefront (regrouped primary and NEE occlusion walks)",
+            "trace_config": ctx.gi_trace_config(),
+            "autotune": gi_tune,
         }
         if not args.no_surrogate:
             gi["surrogate"] = surrogate_section(ctx, lux, gp, args)

@@ -65,6 +65,9 @@ struct lux_ctx {
     // wavefront interval trace: mode 0 wavefront, 1 fused; per-item records,
     // NEE slots, slot visibility, compacted occlusion rays, counters
     uint32_t gi_trace_mode = 0;
+    // launch configuration of the wavefront stages' walks (own tuning: their
+    // item setup is heavier than a ray-list load)
+    uint32_t gi_wpc = 20, gi_depth = 8, gi_regroup = 8, gi_batch = 8;
     lux_buffer gi_w_prim, gi_w_shade, gi_w_slots, gi_w_vis, gi_w_shadow, gi_w_counters;
     lux_buffer mesh_info;    // uint32 per mesh: BLAS base | triangle base | count
     hipEvent_t ev_phase[5] = {};

@@ -19,6 +19,7 @@
 // All arithmetic is IEEE f32 in a fixed operation order; min/max/saturate
 // that feed stored values are compare-selects; cos/sin of 2*pi*u use a fixed
 // f64 Taylor evaluation.  The CPU oracle evaluates the same sequence.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include "lux_internal.h"
@@ -1347,9 +1348,9 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
     w.vis = static_cast<uint32_t*>(ctx->gi_w_vis.ptr);
     w.shadow = static_cast<float4*>(ctx->gi_w_shadow.ptr);
     w.counters = static_cast<uint32_t*>(ctx->gi_w_counters.ptr);
-    w.regroup = ctx->regroup ? ctx->regroup : 1u;
-    w.batch = ctx->trace_batch ? ctx->trace_batch : 8u;
-    const uint32_t wpc = (ctx->waves_per_cu && ctx->waves_per_cu != 1000u) ? ctx->waves_per_cu : 20u;
+    w.regroup = ctx->gi_regroup ? ctx->gi_regroup : 8u;
+    w.batch = ctx->gi_batch ? ctx->gi_batch : 8u;
+    const uint32_t wpc = ctx->gi_wpc ? ctx->gi_wpc : 20u;
     const uint32_t pblocks = (uint32_t)ctx->cu_count * wpc / (kWg / 64);
     hipStream_t st = ctx->stream;
     for (uint64_t u0 = 0; u0 < total; u0 += chunk_units) {
@@ -1366,13 +1367,13 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
         if (int rc = lux_device_zero(ctx, w.counters, 16)) return rc;
         uint32_t need = (max_items + kWg - 1) / kWg;
         uint32_t pb = pblocks < need ? pblocks : need;
-        switch (ctx->trace_depth) {
+        switch (ctx->gi_depth) {
         case 4: hipLaunchKernelGGL(k_gi_primary<4>, dim3(pb), dim3(kWg), 0, st, a, w); break;
         case 16: hipLaunchKernelGGL(k_gi_primary<16>, dim3(pb), dim3(kWg), 0, st, a, w); break;
         default: hipLaunchKernelGGL(k_gi_primary<8>, dim3(pb), dim3(kWg), 0, st, a, w); break;
         }
         hipLaunchKernelGGL(k_gi_setup, dim3(need), dim3(kWg), 0, st, a, w);
-        switch (ctx->trace_depth) {
+        switch (ctx->gi_depth) {
         case 4: hipLaunchKernelGGL(k_gi_shadow<4>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
         case 16: hipLaunchKernelGGL(k_gi_shadow<16>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
         default: hipLaunchKernelGGL(k_gi_shadow<8>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
@@ -1397,6 +1398,110 @@ int trace_wavefront(lux_ctx* ctx, uint64_t g0, uint64_t g1) {
     return run_wave(ctx, w, g1 - g0, 1);
 }
 }  // namespace
+
+extern "C" int lux_gi_set_trace_config(lux_ctx* ctx, const lux_trace_config* cfg) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!cfg) return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_set_trace_config: null config");
+    if (cfg->waves_per_cu > 32)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_set_trace_config: waves_per_cu > 32");
+    if (cfg->regroup_threshold > 64)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_set_trace_config: regroup threshold > 64");
+    if (cfg->lds_stack_depth != 0 && cfg->lds_stack_depth != 4 && cfg->lds_stack_depth != 8 &&
+        cfg->lds_stack_depth != 16)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_set_trace_config: LDS stack depth must be 4, 8 or 16");
+    if (cfg->steps_per_refill > 256)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_set_trace_config: steps per refill > 256");
+    ctx->gi_wpc = cfg->waves_per_cu;
+    ctx->gi_depth = cfg->lds_stack_depth;
+    ctx->gi_regroup = cfg->regroup_threshold;
+    ctx->gi_batch = cfg->steps_per_refill;
+    return LUX_OK;
+}
+
+extern "C" int lux_gi_get_trace_config(lux_ctx* ctx, lux_trace_config* cfg) {
+    if (!ctx || !cfg) return LUX_ERR_INVALID;
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->waves_per_cu = ctx->gi_wpc;
+    cfg->lds_stack_depth = ctx->gi_depth;
+    cfg->regroup_threshold = ctx->gi_regroup;
+    cfg->steps_per_refill = ctx->gi_batch;
+    return LUX_OK;
+}
+
+// Autotuner of the wavefront stages: coordinate descent as for ray lists,
+// each candidate timed over a full interval trace of the configured grid
+// (HIP events, median of `repeats` after one warm-up).  Overwrites the
+// interval array.
+extern "C" int lux_autotune_gi(lux_ctx* ctx, uint32_t repeats, lux_trace_config* best, float* best_ms,
+                               uint32_t* evaluations) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (int rc = gi_check(ctx)) return rc;
+    if (repeats == 0) repeats = 2;
+    if (repeats > 15) repeats = 15;
+    const uint32_t saved[4] = {ctx->gi_wpc, ctx->gi_depth, ctx->gi_regroup, ctx->gi_batch};
+    const uint32_t saved_mode = ctx->gi_trace_mode;
+    ctx->gi_trace_mode = 0u;
+    auto measure = [&](const uint32_t c[4], float& ms) -> int {
+        ctx->gi_wpc = c[0];
+        ctx->gi_depth = c[1];
+        ctx->gi_regroup = c[2];
+        ctx->gi_batch = c[3];
+        float t[16];
+        for (uint32_t r = 0; r <= repeats; ++r) {
+            LUX_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+            if (int rc = lux_gi_trace_levels(ctx, 0, ctx->gi.levels - 1)) return rc;
+            LUX_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            LUX_HIP(ctx, hipEventSynchronize(ctx->ev1));
+            float e = 0.0f;
+            LUX_HIP(ctx, hipEventElapsedTime(&e, ctx->ev0, ctx->ev1));
+            if (r > 0) t[r - 1] = e;
+        }
+        std::sort(t, t + repeats);
+        ms = t[repeats / 2];
+        return LUX_OK;
+    };
+    static const uint32_t cand_wpc[] = {16, 20, 24, 32};
+    static const uint32_t cand_depth[] = {4, 8, 16};
+    static const uint32_t cand_regroup[] = {4, 8, 16, 32, 48};
+    static const uint32_t cand_batch[] = {4, 8, 16};
+    const uint32_t* cands[4] = {cand_wpc, cand_depth, cand_regroup, cand_batch};
+    const uint32_t ncand[4] = {4, 3, 5, 3};
+    uint32_t cur[4] = {20, 8, 8, 8};
+    float cur_ms = 0.0f;
+    uint32_t evals = 1;
+    int rc = measure(cur, cur_ms);
+    for (int pass = 0; pass < 2 && rc == LUX_OK; ++pass)
+        for (int k = 0; k < 4 && rc == LUX_OK; ++k)
+            for (uint32_t j = 0; j < ncand[k] && rc == LUX_OK; ++j) {
+                if (cands[k][j] == cur[k]) continue;
+                uint32_t c[4] = {cur[0], cur[1], cur[2], cur[3]};
+                c[k] = cands[k][j];
+                float ms = 0.0f;
+                rc = measure(c, ms);
+                ++evals;
+                if (rc == LUX_OK && ms < cur_ms) {
+                    cur_ms = ms;
+                    cur[k] = c[k];
+                }
+            }
+    ctx->gi_trace_mode = saved_mode;
+    const uint32_t* fin = rc == LUX_OK ? cur : saved;
+    ctx->gi_wpc = fin[0];
+    ctx->gi_depth = fin[1];
+    ctx->gi_regroup = fin[2];
+    ctx->gi_batch = fin[3];
+    if (rc != LUX_OK) return rc;
+    if (best) {
+        std::memset(best, 0, sizeof *best);
+        best->waves_per_cu = cur[0];
+        best->lds_stack_depth = cur[1];
+        best->regroup_threshold = cur[2];
+        best->steps_per_refill = cur[3];
+    }
+    if (best_ms) *best_ms = cur_ms;
+    if (evaluations) *evaluations = evals;
+    return LUX_OK;
+}
 
 extern "C" int lux_gi_set_trace_mode(lux_ctx* ctx, uint32_t mode) {
     if (!ctx) return LUX_ERR_INVALID;

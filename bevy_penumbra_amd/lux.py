@@ -224,6 +224,9 @@ def load_library() -> C.CDLL:
         "lux_radiance_cascades_gi_params": (i32, [vp, vp, vp, u32, u32, u32, u32, u32, vp]),
         "lux_gi_trace_levels": (i32, [vp, u32, u32]),
         "lux_gi_set_trace_mode": (i32, [vp, u32]),
+        "lux_gi_set_trace_config": (i32, [vp, vp]),
+        "lux_gi_get_trace_config": (i32, [vp, vp]),
+        "lux_autotune_gi": (i32, [vp, u32, vp, vp, vp]),
         "lux_gi_query_counters": (i32, [vp, vp, vp]),
         "lux_camera_receivers": (i32, [vp, vp, u32, u32, vp]),
         "lux_set_trace_config": (i32, [vp, vp]),
@@ -552,6 +555,27 @@ class Context:
     def gi_set_trace_mode(self, mode: int):
         """0 wavefront (default), 1 fused; results are identical."""
         self._check(self.lib.lux_gi_set_trace_mode(self.h, mode), "lux_gi_set_trace_mode")
+
+    def gi_set_trace_config(self, waves_per_cu: int = 20, lds_stack_depth: int = 8,
+                            regroup_threshold: int = 8, steps_per_refill: int = 8):
+        cfg = (C.c_uint32 * 8)(waves_per_cu, lds_stack_depth, regroup_threshold, steps_per_refill,
+                               0, 0, 0, 0)
+        self._check(self.lib.lux_gi_set_trace_config(self.h, cfg), "lux_gi_set_trace_config")
+
+    def gi_trace_config(self) -> dict:
+        cfg = (C.c_uint32 * 8)()
+        self._check(self.lib.lux_gi_get_trace_config(self.h, cfg), "lux_gi_get_trace_config")
+        return dict(waves_per_cu=cfg[0], lds_stack_depth=cfg[1], regroup_threshold=cfg[2],
+                    steps_per_refill=cfg[3])
+
+    def autotune_gi(self, repeats: int = 2) -> dict:
+        cfg = (C.c_uint32 * 8)()
+        ms = C.c_float()
+        ev = C.c_uint32()
+        self._check(self.lib.lux_autotune_gi(self.h, repeats, cfg, C.byref(ms), C.byref(ev)),
+                    "lux_autotune_gi")
+        return dict(waves_per_cu=cfg[0], lds_stack_depth=cfg[1], regroup_threshold=cfg[2],
+                    steps_per_refill=cfg[3], ms=float(ms.value), evaluations=int(ev.value))
 
     def gi_trace_levels(self, first: int, last: int):
         self._check(self.lib.lux_gi_trace_levels(self.h, first, last), "lux_gi_trace_levels")

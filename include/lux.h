@@ -100,6 +100,22 @@ typedef struct {
 
 typedef struct lux_ctx lux_ctx;
 
+/* Full traversal launch configuration.  regroup_threshold 0 runs each wave's
+ * 64 rays to completion; 1..64 runs resumable walks and refills idle lanes
+ * once that many are idle, checking every steps_per_refill walk steps
+ * (0 = 8).  lds_stack_depth: traversal-stack slots per lane kept in LDS
+ * (4, 8 or 16; 0 = 8), deeper slots spill to private memory.
+ * waves_per_cu: persistent grid size (0 = 20; 1000 = one ray per thread,
+ * no persistence).  A new context starts at {20, 8, 8, 8}.  Results do not
+ * depend on the configuration. */
+typedef struct {
+    uint32_t waves_per_cu;
+    uint32_t lds_stack_depth;
+    uint32_t regroup_threshold;
+    uint32_t steps_per_refill;
+    uint32_t reserved[4];
+} lux_trace_config;
+
 /* ---- context (replaces render::install's device/pipeline set-up,
  *      src/render.rs:1641-1760) ---- */
 int lux_ctx_create(int device, lux_ctx** out);
@@ -261,6 +277,15 @@ int lux_gi_trace_levels(lux_ctx* ctx, uint32_t first, uint32_t last);
 #define LUX_GI_TRACE_WAVEFRONT 0u
 #define LUX_GI_TRACE_FUSED 1u
 int lux_gi_set_trace_mode(lux_ctx* ctx, uint32_t mode);
+/* Launch configuration of the wavefront stages' walks (same fields and
+ * limits as lux_trace_config; waves_per_cu 1..32; a new context starts at
+ * {20, 8, 8, 8}) and its autotuner, which times full interval traces of the
+ * configured grid and installs the fastest (overwrites the interval array).
+ * No counterpart in the reference. */
+int lux_gi_set_trace_config(lux_ctx* ctx, const lux_trace_config* cfg);
+int lux_gi_get_trace_config(lux_ctx* ctx, lux_trace_config* cfg);
+int lux_autotune_gi(lux_ctx* ctx, uint32_t repeats, lux_trace_config* best, float* best_ms,
+                    uint32_t* evaluations);
 
 /* ---- learned far-cascade surrogate.  No counterpart in the reference: its
  *      README defers online neural radiance caching (README.md:384-385).
@@ -374,21 +399,6 @@ int lux_last_kernel_ms(lux_ctx* ctx, float* ms);
  * completion).  Keeps the LDS stack depth and steps per refill. */
 int lux_set_trace_tuning(lux_ctx* ctx, uint32_t waves_per_cu, uint32_t regroup_threshold);
 
-/* Full traversal launch configuration.  regroup_threshold 0 runs each wave's
- * 64 rays to completion; 1..64 runs resumable walks and refills idle lanes
- * once that many are idle, checking every steps_per_refill walk steps
- * (0 = 8).  lds_stack_depth: traversal-stack slots per lane kept in LDS
- * (4, 8 or 16; 0 = 8), deeper slots spill to private memory.
- * waves_per_cu: persistent grid size (0 = 20; 1000 = one ray per thread,
- * no persistence).  A new context starts at {20, 8, 8, 8}.  Results do not
- * depend on the configuration. */
-typedef struct {
-    uint32_t waves_per_cu;
-    uint32_t lds_stack_depth;
-    uint32_t regroup_threshold;
-    uint32_t steps_per_refill;
-    uint32_t reserved[4];
-} lux_trace_config;
 int lux_set_trace_config(lux_ctx* ctx, const lux_trace_config* cfg);
 int lux_get_trace_config(lux_ctx* ctx, lux_trace_config* cfg);
 /* Autotuner: times candidate configurations on the caller's device rays

@@ -204,3 +204,20 @@ def test_gpu_query_budget_denies_past_the_limit(lux_ctx):
     m = O.gi_merge(p, iv)
     assert_bits(g["merged"], m, "merged over unsupported samples")
     assert_bits(g["probes"], O.gi_pack(p, m), "probes")
+
+
+def test_gpu_gi_autotuner_keeps_results(lux_ctx):
+    sc, _ = scenes.colonnade()
+    p = colonnade_params(dims=(16, 6, 8), spacing=2.0)
+    ref = run_gpu(lux_ctx, sc, p)
+    saved = lux_ctx.gi_trace_config()
+    res = lux_ctx.autotune_gi(repeats=1)
+    assert res["evaluations"] >= 10 and res["ms"] > 0
+    cfg = lux_ctx.gi_trace_config()
+    assert cfg == {k: res[k] for k in cfg}
+    lux_ctx.gi_update()
+    lux_ctx.synchronize()
+    got = lux_ctx.gi_download()
+    for key in ("intervals", "merged", "probes"):
+        assert_bits(got[key], ref[key], key)
+    lux_ctx.gi_set_trace_config(**saved)
