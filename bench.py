@@ -50,7 +50,60 @@ def parse():
     p.add_argument("--surrogate-levels", default="2,1")
     p.add_argument("--no-dynamic", action="store_true")
     p.add_argument("--dynamic-frames", type=int, default=120)
+    p.add_argument("--pmc-workload", action="store_true",
+                   help="run only the counter-profiling workload (1 GiB device copy x2, three "
+                        "traversal launches, two GI updates; default launch configurations)")
     return p.parse_args()
+
+
+TRAFFIC_PROFILE = ROOT / "profiles" / "pmc_traffic_colonnade_trace_and_gi.md"
+
+
+def measured_traffic(kernel: str, cfg: dict):
+    """HBM-side bytes per launch of `kernel` from the committed counter profile,
+    when it was taken with this traversal configuration; else None."""
+    if not TRAFFIC_PROFILE.exists():
+        return None
+    lines = TRAFFIC_PROFILE.read_text().splitlines()
+    conf = next((l for l in lines if l.startswith("config ")), None)
+    if conf is None:
+        return None
+    want = dict(kv.split("=") for kv in conf.split()[1:])
+    if any(int(want.get(k, -1)) != int(v) for k, v in cfg.items()):
+        return None
+    for l in lines:
+        parts = l.split()
+        if len(parts) == 4 and parts[0] == "traffic" and parts[1] == kernel:
+            vals = dict(kv.split("=") for kv in parts[2:])
+            return int(vals["fetch_bytes"]) + int(vals["write_bytes"])
+    return None
+
+
+def pmc_workload(lux, scenes) -> int:
+    from bevy_penumbra_amd.lux import DeviceBuffer
+    ctx = lux.Context(0)
+    a = DeviceBuffer.empty(ctx, (1 << 28,), np.float32)
+    b = DeviceBuffer.empty(ctx, (1 << 28,), np.float32)
+    for _ in range(2):
+        b.copy_from(a)
+    ctx.synchronize()
+    sc, _ = scenes.colonnade()
+    rays_np = scenes.microbench_rays(sc.bounds)
+    ctx.upload_scene(sc)
+    ctx.build_bvh()
+    n = rays_np.shape[0]
+    rays = DeviceBuffer.from_array(ctx, rays_np)
+    hits = DeviceBuffer.empty(ctx, (n, 4), np.uint32)
+    for _ in range(3):
+        ctx.trace_device(rays, n, hits)
+    ctx.gi_configure(colonnade_gi_params(lux))
+    for _ in range(2):
+        ctx.gi_update()
+    ctx.synchronize()
+    print(json.dumps({"pmc_workload": "done", "trace_config": ctx.trace_config(),
+                      "gi_trace_config": ctx.gi_trace_config()}), flush=True)
+    ctx.close()
+    return 0
 
 
 def colonnade_gi_params(lux):
@@ -146,6 +199,9 @@ def surrogate_section(ctx, lux, gp, args) -> dict:
         if chosen is None:
             chosen = -1.0
         ctx.gi_set_frame(999, 999)
+        nf = [ctx.gi_update_surrogate(s, level, float("inf")) for _ in range(3)]
+        res["no_fallback"] = {"rmse": round(probe_rmse(ctx.gi_download()["probes"], refs[999]), 5),
+                              "ms": round(float(np.median([x.ms_total for x in nf])), 3)}
         sts = [ctx.gi_update_surrogate(s, level, chosen) for _ in range(3)]
         e = probe_rmse(ctx.gi_download()["probes"], refs[999])
         ms = float(np.median([x.ms_total for x in sts]))
@@ -251,6 +307,8 @@ def main() -> int:
     from bevy_penumbra_amd import lux, scenes
     from bevy_penumbra_amd.lux import DeviceBuffer
 
+    if args.pmc_workload:
+        return pmc_workload(lux, scenes)
     scene, _ = scenes.colonnade()
     rays_np = scenes.microbench_rays(scene.bounds, n=args.rays)
     ctx = lux.Context(0)
@@ -373,7 +431,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": measured_traffic("k_trace_regroup<%d>" % trace_cfg["lds_stack_depth"], trace_cfg),
         },
         "cpu_baseline": cpu,
         "gi_update": gi,

@@ -540,3 +540,10 @@ extern "C" int lux_device_zero(lux_ctx* ctx, void* ptr, size_t bytes) {
     LUX_HIP(ctx, hipGetLastError());
     return LUX_OK;
 }
+
+extern "C" int lux_device_copy(lux_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return LUX_ERR_INVALID;
+    if (!bytes) return LUX_OK;
+    LUX_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return LUX_OK;
+}

@@ -218,6 +218,7 @@ def load_library() -> C.CDLL:
         "lux_copy_to_device": (i32, [vp, vp, vp, C.c_size_t]),
         "lux_copy_to_host": (i32, [vp, vp, vp, C.c_size_t]),
         "lux_device_zero": (i32, [vp, vp, C.c_size_t]),
+        "lux_device_copy": (i32, [vp, vp, vp, C.c_size_t]),
         "lux_radiance_cascades_default": (i32, [vp]),
         "lux_radiance_cascades_low_end": (i32, [vp]),
         "lux_radiance_cascades_sanitized": (i32, [vp, vp]),
@@ -289,6 +290,10 @@ class DeviceBuffer:
         assert a.nbytes <= self.nbytes
         self.ctx._check(self.ctx.lib.lux_copy_to_device(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes),
                         "lux_copy_to_device")
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int | None = None):
+        n = min(self.nbytes, other.nbytes) if nbytes is None else nbytes
+        self.ctx._check(self.ctx.lib.lux_device_copy(self.ctx.h, self.ptr, other.ptr, n), "lux_device_copy")
 
     def zero(self):
         self.ctx._check(self.ctx.lib.lux_device_zero(self.ctx.h, self.ptr, self.nbytes), "lux_device_zero")
