@@ -624,6 +624,7 @@ constexpr uint32_t kWalkClosest = 0u, kWalkFirst = 1u, kWalkOccluded = 2u;
 constexpr uint32_t kPhaseDone = 0u, kPhaseTlas = 1u, kPhaseLeaf = 2u, kPhaseBlas = 3u;
 
 struct Walk {
+    const DevScene* sp;      // the lane's scene (multi-scene launches)
     float3 o, d;
     float limit;             // occlusion limit (closest/first-hit use h.t)
     HitState h;
@@ -924,6 +925,44 @@ __device__ __forceinline__ void regroup_walks(const DevScene& s, Src& src, uint3
         }
         for (uint32_t k = 0; k < batch; ++k) {
             if (active && walk_step(s, w, stk, st)) {
+                src.finish(idx, w);
+                active = false;
+            }
+        }
+    }
+}
+
+// regroup_walks for items of several scenes: Src::begin also sets w.sp, and
+// each lane steps through its own scene.
+template <class Src, class Stack, class St>
+__device__ __forceinline__ void regroup_walks_multi(Src& src, uint32_t* cursor, uint32_t regroup,
+                                                    uint32_t batch, Stack& stk, St& st) {
+    const uint32_t lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t n = src.count();
+    Walk w;
+    w.sp = nullptr;
+    uint32_t idx = 0;
+    bool active = false, drained = false;
+    while (true) {
+        uint64_t idle = __ballot(!active);
+        uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!drained && nidle >= regroup) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(cursor, nidle);
+            base = __shfl(base, 0, 64);
+            if (base >= n || n - base <= nidle) drained = true;
+            if (!active) {
+                idx = base + (uint32_t)__popcll(idle & below);
+                if (idx < n) active = src.begin(idx, w, st);
+            }
+        }
+        if (!__any(active)) {
+            if (drained) break;
+            continue;
+        }
+        for (uint32_t k = 0; k < batch; ++k) {
+            if (active && walk_step(*w.sp, w, stk, st)) {
                 src.finish(idx, w);
                 active = false;
             }

@@ -109,3 +109,12 @@ size_t lux_sort_hist_words(uint32_t n);
 // next call.
 int lux_surrogate_forward_internal(lux_surrogate* s, const float* queries, uint32_t n, const float** Y);
 lux_ctx* lux_surrogate_context(lux_surrogate* s);
+// Bytes of caller-owned scratch for an inference pass over n queries, and
+// that pass on a caller's stream into it (*Y points into the scratch).
+// Allocates nothing and keeps no pointer, so the launches can be captured
+// into a graph that outlives later training steps.
+size_t lux_surrogate_scratch_bytes(uint32_t n);
+int lux_surrogate_forward_into(lux_surrogate* s, hipStream_t st, const float* queries, uint32_t n, void* scratch,
+                               const float** Y);
+// GI state check of a context (configured, BVH built, emitter table current).
+int lux_gi_check_internal(lux_ctx* ctx);

@@ -680,6 +680,27 @@ int lux_surrogate_forward_internal(lux_surrogate* s, const float* queries, uint3
 
 lux_ctx* lux_surrogate_context(lux_surrogate* s) { return s->ctx; }
 
+size_t lux_surrogate_scratch_bytes(uint32_t n) {
+    size_t N = ((size_t)n + 15) / 16 * 16;
+    return 2 * kIn * N + 4 * kOut * N;
+}
+
+int lux_surrogate_forward_into(lux_surrogate* s, hipStream_t st, const float* queries, uint32_t n, void* scratch,
+                               const float** Y) {
+    lux_ctx* ctx = s->ctx;
+    size_t N = ((size_t)n + 15) / 16 * 16;
+    short* X = static_cast<short*>(scratch);
+    float* out = reinterpret_cast<float*>(X + kIn * N);
+    const SurDev& dev = s->dev_infer;
+    // inference: no activations are kept (H1 == nullptr leaves H2 unused)
+    hipLaunchKernelGGL(k_sur_encode, dim3(grid_for(n, 256)), dim3(256), 0, st, dev, queries, 9u, n, X);
+    hipLaunchKernelGGL(k_sur_forward, dim3(tile_blocks(ctx, n)), dim3(256), 0, st, dev,
+                       static_cast<const short*>(X), n, (short*)nullptr, (short*)nullptr, out, (n + 15) / 16);
+    LUX_HIP(ctx, hipGetLastError());
+    *Y = out;
+    return LUX_OK;
+}
+
 extern "C" void lux_surrogate_default_params(lux_surrogate_params* p) {
     if (!p) return;
     std::memset(p, 0, sizeof *p);

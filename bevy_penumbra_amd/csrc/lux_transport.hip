@@ -1,13 +1,14 @@
 // lux_transport.hip -- dense-grid interval trace, cascade merge, probe
 // packing and world resolve on gfx950.
 //
-// One GI update = three kernels on the context stream:
-//   k_gi_trace   one thread per interval (probe, direction, level), all levels
-//                in one launch: ray from the probe centre over the interval,
-//                closest hit, hit shading (emission + emissive NEE with
-//                occlusion rays), deposit J/beta -- trace + deposit + reduce of
-//                the reference (shaders/split.slang:1056-1163,1386-1476;
-//                shaders/cascades.slang:60-268)
+// One GI update on the context stream:
+//   interval trace  (probe, direction, level) rays from the probe centre over
+//                the interval, closest hit, hit shading (emission + emissive
+//                NEE with occlusion rays), deposit J/beta -- trace + deposit +
+//                reduce of the reference (shaders/split.slang:1056-1163,
+//                1386-1476; shaders/cascades.slang:60-268); either the four
+//                wavefront stages (k_gi_primary, k_gi_setup, k_gi_shadow,
+//                k_gi_accum) or the fused k_gi_trace, one thread per interval
 //   k_gi_merge   per level, top down: support-aware Eq. 7 merge with 4 child
 //                directions x 8 trilinear corners (split.slang:389-479,1523-1725)
 //   k_gi_pack    one wave per level-0 probe: directions staged in LDS, 36
@@ -22,429 +23,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include "lux_internal.h"
+#include "lux_gi_device.h"
 
 namespace {
-
-constexpr int kWg = 256;
-constexpr int kLdsDepth = 16;
-constexpr float kPi = 3.14159265358979323846f;
-
-struct Light {
-    float4 position, color, direction, edge1, edge2;
-    uint4 source;
-};
-static_assert(sizeof(Light) == 96, "light record");
-
-struct GiArgs {
-    lux::DevScene scene;
-    const Light* lights;
-    uint32_t n_lights;
-    const lux_material* mats;
-    uint32_t n_mats;
-    lux_gi_params p;
-    uint64_t offset[17];
-    float4* intervals;
-    float4* merged;
-    float4* probes;
-    uint32_t* counters;     // [0] attempts, [1] admitted
-    uint32_t admit_limit;   // 0 = unbounded
-};
-
-// Ticketed per-update traversal budget (admitTransportQuery,
-// shaders/traversal.slang:20-38), aggregated per wave: the active lanes take
-// consecutive tickets with one atomic, lanes past the limit are denied and
-// their tickets returned, so the admitted count never exceeds the limit and
-// ends exact.  Attempts are always counted.
-__device__ bool admit_query(const GiArgs& a) {
-    uint64_t mask = __ballot(1);
-    uint32_t lane = __lane_id();
-    uint32_t leader = (uint32_t)(__ffsll((unsigned long long)mask) - 1);
-    uint32_t count = (uint32_t)__popcll(mask);
-    uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    uint32_t base = 0;
-    if (lane == leader) {
-        atomicAdd(&a.counters[0], count);
-        base = atomicAdd(&a.counters[1], count);
-    }
-    if (a.admit_limit == 0u) return true;
-    base = __shfl(base, (int)leader, 64);
-    bool ok = (uint64_t)base + rank < (uint64_t)a.admit_limit;
-    uint64_t denied = __ballot(!ok);
-    if (lane == leader && denied) atomicSub(&a.counters[1], (uint32_t)__popcll(denied));
-    return ok;
-}
-
-__device__ __forceinline__ float smax(float a, float b) { return b > a ? b : a; }
-__device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
-__device__ __forceinline__ float sat(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
-__device__ __forceinline__ float fracf(float x) { return x - floorf(x); }
-__device__ __forceinline__ float pow2i(int e) { return __uint_as_float((uint32_t)(127 + e) << 23); }
-__device__ __forceinline__ float dot3(float3 a, float3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-
-__device__ __forceinline__ uint32_t level_dim(uint32_t d, uint32_t level) {
-    for (uint32_t l = 0; l < level; ++l) d = (d + 1u) / 2u;
-    return d;
-}
-
-__device__ __forceinline__ uint32_t transport_hash(uint32_t v) {
-    v ^= v >> 16;
-    v *= 0x7feb352du;
-    v ^= v >> 15;
-    v *= 0x846ca68bu;
-    v ^= v >> 16;
-    return v;
-}
-
-__device__ __forceinline__ float2 r2(uint32_t index) {
-    uint32_t px = index * 3242174889u, py = index * 2447445414u;
-    return make_float2((float)(px >> 8) / 16777216.0f, (float)(py >> 8) / 16777216.0f);
-}
-
-__device__ __forceinline__ uint32_t hash_key(int32_t x, int32_t y, int32_t z, int32_t w,
-                                             uint32_t buckets) {
-    uint32_t h = (uint32_t)x * 0x8da6b343u ^ (uint32_t)y * 0xd8163841u ^
-                 (uint32_t)z * 0xcb1ab31fu ^ (uint32_t)w * 0x165667b1u;
-    h ^= h >> 16;
-    h *= 0x7feb352du;
-    h ^= h >> 15;
-    return h & (buckets - 1u);
-}
-
-// cos(2 pi u), sin(2 pi u): exact quadrant split, Taylor series in f64.
-__device__ __forceinline__ void cos_sin_2pi(float u, float& c, float& s) {
-    double x = (double)u * 4.0;
-    double q = floor(x);
-    double r = x - q;
-    int qi = ((int)q) & 3;
-    double t = r * 1.5707963267948966;
-    double t2 = t * t;
-    double sn = 1.0 / 51090942171709440000.0;
-    sn = sn * t2 * -1.0 + 1.0 / 121645100408832000.0;
-    sn = sn * t2 * -1.0 + 1.0 / 355687428096000.0;
-    sn = sn * t2 * -1.0 + 1.0 / 1307674368000.0;
-    sn = sn * t2 * -1.0 + 1.0 / 6227020800.0;
-    sn = sn * t2 * -1.0 + 1.0 / 39916800.0;
-    sn = sn * t2 * -1.0 + 1.0 / 362880.0;
-    sn = sn * t2 * -1.0 + 1.0 / 5040.0;
-    sn = sn * t2 * -1.0 + 1.0 / 120.0;
-    sn = sn * t2 * -1.0 + 1.0 / 6.0;
-    sn = sn * t2 * -1.0 + 1.0;
-    sn = sn * t;
-    double cs = 1.0 / 2432902008176640000.0;
-    cs = cs * t2 * -1.0 + 1.0 / 6402373705728000.0;
-    cs = cs * t2 * -1.0 + 1.0 / 20922789888000.0;
-    cs = cs * t2 * -1.0 + 1.0 / 87178291200.0;
-    cs = cs * t2 * -1.0 + 1.0 / 479001600.0;
-    cs = cs * t2 * -1.0 + 1.0 / 3628800.0;
-    cs = cs * t2 * -1.0 + 1.0 / 40320.0;
-    cs = cs * t2 * -1.0 + 1.0 / 720.0;
-    cs = cs * t2 * -1.0 + 1.0 / 24.0;
-    cs = cs * t2 * -1.0 + 1.0 / 2.0;
-    cs = cs * t2 * -1.0 + 1.0;
-    double oc, os;
-    if (qi == 0) { oc = cs; os = sn; }
-    else if (qi == 1) { oc = -sn; os = cs; }
-    else if (qi == 2) { oc = -cs; os = -sn; }
-    else { oc = sn; os = -cs; }
-    c = (float)oc;
-    s = (float)os;
-}
-
-// Equal-area bin direction with an in-bin offset (binDirection generalised).
-__device__ __forceinline__ float3 bin_direction(uint32_t index, uint32_t nphi, uint32_t ny,
-                                                float ox, float oy) {
-    float ux = ((float)(index % nphi) + ox) / (float)nphi;
-    float uy = ((float)(index / nphi) + oy) / (float)ny;
-    float c, s;
-    cos_sin_2pi(ux, c, s);
-    float y = uy * 2.0f - 1.0f;
-    float q = 1.0f - y * y;
-    float r = sqrtf(q > 0.0f ? q : 0.0f);
-    return make_float3(r * c, y, r * s);
-}
-
-__device__ __forceinline__ float2 ray_offset_uv(uint32_t seed, uint32_t r) {
-    float2 j = r2(transport_hash(seed ^ 0x5a17c9e3u));
-    float2 q = r2(r);
-    return make_float2(fracf(0.5f + j.x + q.x), fracf(0.5f + j.y + q.y));
-}
-
-__device__ __forceinline__ float offset1(float pos, float n, float bias) {
-    float origin = pos + n * bias;
-    int32_t step = (int32_t)(n * 256.0f);
-    uint32_t moved = __float_as_uint(origin) + (uint32_t)(origin < 0.0f ? -step : step);
-    float integer_offset = __uint_as_float(moved);
-    float shifted = fabsf(origin) < 0.03125f ? origin + n / 65536.0f : integer_offset;
-    return lux::isfinite_bits(shifted) ? shifted : pos;
-}
-
-// rayOffset (common.slang:345-359).
-__device__ __forceinline__ float3 ray_offset(float3 p, float3 n, float bias) {
-    return make_float3(offset1(p.x, n.x, bias), offset1(p.y, n.y, bias), offset1(p.z, n.z, bias));
-}
-
-__device__ __forceinline__ float3 unit_vector(float3 v, float3 fb) {
-    float sc = fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z)));
-    if (sc == 0.0f || !lux::isfinite_bits(sc)) return fb;
-    float div = fmaxf(sc * 0.25f, 1.17549435e-38f);
-    float3 w = make_float3(v.x / div, v.y / div, v.z / div);
-    float len = sqrtf(dot3(w, w));
-    return make_float3(w.x / len, w.y / len, w.z / len);
-}
-
-// triangleGeometricNormal through the inverse rows (traversal.slang:248-251).
-__device__ float3 geometric_normal(const float4* rec, float3 a, float3 b, float3 c, uint32_t flags) {
-    float3 e1 = make_float3(b.x - a.x, b.y - a.y, b.z - a.z);
-    float3 e2 = make_float3(c.x - a.x, c.y - a.y, c.z - a.z);
-    float3 cr = make_float3(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z,
-                            e1.x * e2.y - e1.y * e2.x);
-    float4 r0 = rec[0], r1 = rec[1], r2v = rec[2];
-    float m0 = fmaxf(fabsf(r0.x), fmaxf(fabsf(r1.x), fabsf(r2v.x)));
-    float m1 = fmaxf(fabsf(r0.y), fmaxf(fabsf(r1.y), fabsf(r2v.y)));
-    float m2 = fmaxf(fabsf(r0.z), fmaxf(fabsf(r1.z), fabsf(r2v.z)));
-    float sc = fmaxf(m0, fmaxf(m1, m2));
-    float3 tv = make_float3(0.0f, 0.0f, 0.0f);
-    if (sc != 0.0f) {
-        sc = fmaxf(sc * 0.25f, 1.17549435e-38f);
-        float3 a0 = make_float3(r0.x / sc, r0.y / sc, r0.z / sc);
-        float3 a1 = make_float3(r1.x / sc, r1.y / sc, r1.z / sc);
-        float3 a2 = make_float3(r2v.x / sc, r2v.y / sc, r2v.z / sc);
-        float3 u = unit_vector(cr, make_float3(0.0f, 0.0f, 0.0f));
-        tv = make_float3((a0.x * u.x + a1.x * u.y) + a2.x * u.z, (a0.y * u.x + a1.y * u.y) + a2.y * u.z,
-                         (a0.z * u.x + a1.z * u.y) + a2.z * u.z);
-    }
-    float3 n = unit_vector(tv, make_float3(0.0f, 1.0f, 0.0f));
-    float sg = (flags & 4u) != 0u ? -1.0f : 1.0f;
-    return make_float3(sg * n.x, sg * n.y, sg * n.z);
-}
-
-__device__ __forceinline__ float pow5(float x) {
-    float x2 = x * x;
-    float x4 = x2 * x2;
-    return x4 * x;
-}
-
-// disneyDiffuseMultiplier (common.slang:299-314).
-__device__ float disney(float3 n, float3 l, float3 v, float pr) {
-    float ndl = sat(dot3(n, l));
-    float ndv = sat(dot3(n, v));
-    float3 hs = make_float3(l.x + v.x, l.y + v.y, l.z + v.z);
-    float hl2 = dot3(hs, hs);
-    if (ndl <= 0.0f || ndv <= 0.0f || hl2 <= 1e-20f) return 0.0f;
-    float rs = 1.0f / sqrtf(hl2);
-    float3 h = make_float3(hs.x * rs, hs.y * rs, hs.z * rs);
-    float rough = smax(sat(pr), 0.089f);
-    rough = rough * rough;
-    float ldh = sat(dot3(l, h));
-    float f90 = 0.5f + ((2.0f * rough) * ldh) * ldh;
-    float ls = 1.0f + (f90 - 1.0f) * pow5(1.0f - ndl);
-    float vs = 1.0f + (f90 - 1.0f) * pow5(1.0f - ndv);
-    return ls * vs;
-}
-
-using Stack = lux::LaneStack<kLdsDepth, kWg>;
-
-// Hit shading (cascades.slang:60-134,196-268) for untextured opaque
-// materials, black environment, no punctual lights, in three pieces shared by
-// the fused and the wavefront interval traces:
-//   shade_begin   emission of the hit and the NEE context (whether NEE runs)
-//   nee_sample    one emissive sample: false when it is skipped before any
-//                 visibility query, else the occlusion ray and the
-//                 contribution that counts when the ray is unoccluded
-//   shade         the fused loop: queries in sample order, unsupported
-//                 visibility makes the whole sample unsupported
-struct NeeCtx {
-    float3 pos, gn, nrm, view;
-    float rough, bias, phase;
-    float2 rot;
-    uint32_t count, frame;
-    lux_material mat;
-};
-
-// Returns true when NEE runs for this hit; rad receives the emission.
-__device__ bool shade_begin(const GiArgs& a, float3 o, float3 d, const lux::HitState& h, float3& rad,
-                            NeeCtx& c) {
-    const lux::DevScene& s = a.scene;
-    const float4* rec = s.inst + 8u * h.inst;
-    uint32_t iflags = __float_as_uint(rec[6].x);
-    float4 A = s.triA[h.tri], B = s.triB[h.tri], C = s.triC[h.tri];
-    uint32_t mi = __float_as_uint(B.w);
-    rad = make_float3(0.0f, 0.0f, 0.0f);
-    if (a.n_mats == 0) return false;
-    if (mi >= a.n_mats) mi = a.n_mats - 1u;
-    c.mat = a.mats[mi];
-    rad = make_float3(c.mat.emission[0], c.mat.emission[1], c.mat.emission[2]);
-    if ((iflags & 16u) != 0u) return false;
-    float3 pos = make_float3(o.x + d.x * h.t, o.y + d.y * h.t, o.z + d.z * h.t);
-    float3 gn = geometric_normal(rec, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
-                                 make_float3(C.x, C.y, C.z), iflags);
-    if (dot3(gn, d) > 0.0f) gn = make_float3(-gn.x, -gn.y, -gn.z);
-    float3 nrm = gn;
-    float3 md = make_float3(-d.x, -d.y, -d.z);
-    if (dot3(nrm, md) <= 0.0f || dot3(nrm, gn) <= 0.0f || a.n_lights == 0) return false;
-    float vl = sqrtf(dot3(md, md));
-    c.view = make_float3(md.x / vl, md.y / vl, md.z / vl);
-    c.rough = lux::isfinite_bits(c.mat.roughness) ? c.mat.roughness : 1.0f;
-    c.count = a.p.emissive_samples / 2u;
-    if (c.count < 1u) c.count = 1u;
-    c.bias = a.p.normal_bias;
-    float sp = a.p.spacing * 0.05f;
-    uint32_t seed = hash_key((int32_t)floorf(pos.x / sp), (int32_t)floorf(pos.y / sp),
-                             (int32_t)floorf(pos.z / sp), 0, 16777216u);
-    c.rot = r2(seed);
-    uint32_t sel_seed = hash_key((int32_t)floorf(pos.x / a.p.spacing), (int32_t)floorf(pos.y / a.p.spacing),
-                                 (int32_t)floorf(pos.z / a.p.spacing), 0, 65536u);
-    c.frame = a.p.frame ^ 0xE11E0001u;
-    c.phase = fracf(r2(c.frame).x + r2(sel_seed).x);
-    c.pos = pos;
-    c.gn = gn;
-    c.nrm = nrm;
-    return true;
-}
-
-__device__ bool nee_sample(const GiArgs& a, const NeeCtx& c, uint32_t smp, float3& origin, float3& dir,
-                           float& lim, float3& term) {
-    float2 q = r2(smp + c.frame * c.count + 1u);
-    float2 uv = make_float2(fracf(q.x + c.rot.x), fracf(q.y + c.rot.y));
-    float selection = fracf(((float)smp + 0.5f) / (float)c.count + c.phase);
-    uint32_t lo = 0, hi = a.n_lights - 1u;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) / 2u;
-        if (a.lights[mid].direction.w < selection) lo = mid + 1u;
-        else hi = mid;
-    }
-    const Light L = a.lights[lo];
-    const float3 pos = c.pos;
-    float3 lp = make_float3(L.position.x, L.position.y, L.position.z);
-    float3 ldir = make_float3(L.direction.x, L.direction.y, L.direction.z);
-    float3 rel = make_float3(pos.x - lp.x, pos.y - lp.y, pos.z - lp.z);
-    if (L.edge1.w <= 0.0f || (L.position.w != -2.0f && dot3(ldir, rel) <= 0.0f)) return false;
-    float root = sqrtf(uv.x);
-    float wa = root * (1.0f - uv.y);
-    float wb = root * uv.y;
-    float3 target = make_float3((lp.x + L.edge1.x * wa) + L.edge2.x * wb,
-                                (lp.y + L.edge1.y * wa) + L.edge2.y * wb,
-                                (lp.z + L.edge1.z * wa) + L.edge2.z * wb);
-    float3 tl = make_float3(target.x - pos.x, target.y - pos.y, target.z - pos.z);
-    float ldist = sqrtf(dot3(tl, tl));
-    if (ldist <= c.bias * 0.25f) return false;
-    float3 Ld = make_float3(tl.x / ldist, tl.y / ldist, tl.z / ldist);
-    float cosine = smax(0.0f, dot3(c.nrm, Ld));
-    float ec = dot3(ldir, make_float3(-Ld.x, -Ld.y, -Ld.z));
-    if (L.position.w == -2.0f) ec = fabsf(ec);
-    float ecos = smax(0.0f, ec);
-    if (cosine <= 0.0f || ecos <= 0.0f || dot3(c.gn, Ld) <= 0.0f) return false;
-    float3 em = make_float3(L.color.x, L.color.y, L.color.z);
-    if (!(em.x > 0.0f || em.y > 0.0f || em.z > 0.0f)) return false;
-    float pdf = ((L.edge1.w * ldist) * ldist) / (L.color.w * ecos);
-    float plane = fabsf(dot3(ldir, rel));
-    float ob = smin(c.bias, plane * 0.25f);
-    origin = ray_offset(pos, c.gn, ob);
-    float3 delta = make_float3(target.x - origin.x, target.y - origin.y, target.z - origin.z);
-    float dist = sqrtf(dot3(delta, delta));
-    if (dist <= c.bias * 0.25f) return false;
-    dir = make_float3(delta.x / dist, delta.y / dist, delta.z / dist);
-    lim = dist - smin(c.bias * 0.25f, dist * 0.001f);
-    float dm = disney(c.nrm, Ld, c.view, c.rough) / kPi;
-    float pm = smax(pdf, 1e-20f);
-    term = make_float3(em.x * (((c.mat.albedo[0] * dm) * cosine) / pm),
-                       em.y * (((c.mat.albedo[1] * dm) * cosine) / pm),
-                       em.z * (((c.mat.albedo[2] * dm) * cosine) / pm));
-    return true;
-}
-
-// Emission + NEE sum -> the sample's radiance (non-finite -> 0, negative
-// clamped), the end of the reference's outgoing-radiance evaluation.
-__device__ __forceinline__ float3 shade_end(float3 rad) {
-    if (!(lux::isfinite_bits(rad.x) && lux::isfinite_bits(rad.y) && lux::isfinite_bits(rad.z)))
-        return make_float3(0.0f, 0.0f, 0.0f);
-    return make_float3(smax(rad.x, 0.0f), smax(rad.y, 0.0f), smax(rad.z, 0.0f));
-}
-
-__device__ bool shade(const GiArgs& a, float3 o, float3 d, const lux::HitState& h, Stack& stk,
-                      float3& out) {
-    float3 rad;
-    NeeCtx c;
-    if (shade_begin(a, o, d, h, rad, c)) {
-        float3 sum = make_float3(0.0f, 0.0f, 0.0f);
-        for (uint32_t smp = 0; smp < c.count; ++smp) {
-            float3 origin, dir, term;
-            float lim;
-            if (!nee_sample(a, c, smp, origin, dir, lim, term)) continue;
-            if (!admit_query(a)) return false;
-            lux::HitState vh;
-            lux::NoStats ns;
-            lux::trace_occluded(a.scene, origin, dir, lim, vh, stk, ns);
-            if (vh.err & lux::kHitOccluded) continue;
-            if (vh.err & lux::kErrUnsupported) return false;
-            sum.x = sum.x + term.x;
-            sum.y = sum.y + term.y;
-            sum.z = sum.z + term.z;
-        }
-        float fc = (float)c.count;
-        rad = make_float3(rad.x + sum.x / fc, rad.y + sum.y / fc, rad.z + sum.z / fc);
-    }
-    out = shade_end(rad);
-    return true;
-}
-
-__device__ __forceinline__ float interval_end(const lux_gi_params& p, uint32_t level) {
-    if (p.interval_mode == 1u) return p.interval_length * (float)((2u << level) - 1u);
-    return p.interval_length * pow2i((int)(2u * level));
-}
-
-// Probe centre of level L for a probe index (x fastest).
-__device__ __forceinline__ float3 probe_centre(const lux_gi_params& p, uint32_t L, uint64_t probe) {
-    uint32_t dx = level_dim(p.dims[0], L), dy = level_dim(p.dims[1], L);
-    uint32_t ix = (uint32_t)(probe % dx);
-    uint32_t iy = (uint32_t)((probe / dx) % dy);
-    uint32_t iz = (uint32_t)(probe / ((uint64_t)dx * dy));
-    float sl = p.spacing * pow2i((int)L);
-    return make_float3(p.origin[0] + ((float)ix + 0.5f) * sl, p.origin[1] + ((float)iy + 0.5f) * sl,
-                       p.origin[2] + ((float)iz + 0.5f) * sl);
-}
-
-// One interval: rays_per_interval jittered rays of bin k from the probe
-// centre c over [start, end) of level L (or [start, inf) when terminal):
-// J = mean shaded radiance, beta = mean transmittance (1 on a miss of a
-// bounded interval), beta = -1 when no ray was supported.
-__device__ float4 trace_interval(const GiArgs& a, uint32_t L, float3 c, uint32_t k, bool terminal,
-                                 Stack& stk) {
-    const lux_gi_params& p = a.p;
-    uint32_t nphi = p.base_phi << L, ny = p.base_y << L;
-    float end = interval_end(p, L);
-    float start = L == 0 ? 0.0f : interval_end(p, L - 1);
-    float limit = terminal ? 1e20f : end - start;
-    float3 jsum = make_float3(0.0f, 0.0f, 0.0f);
-    float bsum = 0.0f;
-    uint32_t n = 0;
-    uint32_t R = p.rays_per_interval ? p.rays_per_interval : 1u;
-    for (uint32_t r = 0; r < R; ++r) {
-        float2 ouv = ray_offset_uv(p.seed, r);
-        float3 d = bin_direction(k, nphi, ny, ouv.x, ouv.y);
-        float3 o = make_float3(c.x + d.x * start, c.y + d.y * start, c.z + d.z * start);
-        if (!admit_query(a)) continue;  // denied: no sample, as an unsupported traversal
-        lux::HitState h;
-        lux::NoStats ns;
-        lux::trace_closest(a.scene, o, d, limit, h, stk, ns, false);
-        if (h.err & lux::kErrUnsupported) continue;
-        float3 rad = make_float3(0.0f, 0.0f, 0.0f);
-        float beta;
-        if (h.tri != lux::kEmpty) {
-            if (!shade(a, o, d, h, stk, rad)) continue;
-            beta = 0.0f;
-        } else {
-            beta = terminal ? 0.0f : 1.0f;
-        }
-        jsum = make_float3(jsum.x + rad.x, jsum.y + rad.y, jsum.z + rad.z);
-        bsum = bsum + beta;
-        n++;
-    }
-    if (n == 0) return make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-    float fn = (float)n;
-    return make_float4(jsum.x / fn, jsum.y / fn, jsum.z / fn, bsum / fn);
-}
 
 // Intervals with global index in [g_begin, g_end) (levels are contiguous).
 __global__ __launch_bounds__(kWg) void k_gi_trace(GiArgs a, uint64_t g_begin, uint64_t g_end) {
@@ -462,33 +43,6 @@ __global__ __launch_bounds__(kWg) void k_gi_trace(GiArgs a, uint64_t g_begin, ui
     float3 c = probe_centre(p, L, local / D);
     bool terminal = p.terminal_tail != 0u && L + 1 == p.levels;
     a.intervals[g] = trace_interval(a, L, c, k, terminal, stk);
-}
-
-// The merge of one interval with no child contribution (its level is the
-// last one traced): Eq. 7 with an empty child sum.
-__device__ __forceinline__ float4 merge_leaf(float4 iv) {
-    float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (iv.w >= 0.0f) {
-        float tr = sat(iv.w);
-        float support = 1.0f - tr;
-        if (support > 0.0f)
-            out = make_float4(iv.x / support, iv.y / support, iv.z / support, sat(support));
-    }
-    return out;
-}
-
-// Surrogate query of (probe, bin k) at level L: probe centre, the frame's
-// first jittered ray direction of the bin, normal = that direction.
-__device__ __forceinline__ void level_query(const lux_gi_params& p, uint32_t L, uint64_t local,
-                                            float* o) {
-    uint32_t nphi = p.base_phi << L, ny = p.base_y << L;
-    uint32_t D = nphi * ny;
-    float3 c = probe_centre(p, L, local / D);
-    float2 ouv = ray_offset_uv(p.seed, 0u);
-    float3 d = bin_direction((uint32_t)(local % D), nphi, ny, ouv.x, ouv.y);
-    o[0] = c.x; o[1] = c.y; o[2] = c.z;
-    o[3] = d.x; o[4] = d.y; o[5] = d.z;
-    o[6] = d.x; o[7] = d.y; o[8] = d.z;
 }
 
 __global__ __launch_bounds__(kWg) void k_sur_queries(GiArgs a, uint32_t L, uint64_t count, float* out) {
@@ -633,26 +187,6 @@ __device__ __forceinline__ uint64_t wave_interval(const GiArgs& a, const Wave& w
     uint32_t D = (a.p.base_phi << w.level) * (a.p.base_y << w.level);
     uint32_t probe = w.list[w.first + j / D];
     return a.offset[w.level] + (uint64_t)probe * D + (j % D);
-}
-
-// The ray of item (interval g, ray r): exactly trace_interval's construction.
-__device__ __forceinline__ void item_ray(const GiArgs& a, uint64_t g, uint32_t r, bool force_terminal,
-                                         float3& o, float3& d, float& limit, bool& terminal) {
-    const lux_gi_params& p = a.p;
-    uint32_t L = 0;
-    while (L + 1 < p.levels && g >= a.offset[L + 1]) ++L;
-    uint64_t local = g - a.offset[L];
-    uint32_t nphi = p.base_phi << L, ny = p.base_y << L;
-    uint32_t D = nphi * ny;
-    uint32_t k = (uint32_t)(local % D);
-    float3 c = probe_centre(p, L, local / D);
-    float end = interval_end(p, L);
-    float start = L == 0 ? 0.0f : interval_end(p, L - 1);
-    terminal = force_terminal || (p.terminal_tail != 0u && L + 1 == p.levels);
-    limit = terminal ? 1e20f : end - start;
-    float2 ouv = ray_offset_uv(p.seed, r);
-    d = bin_direction(k, nphi, ny, ouv.x, ouv.y);
-    o = make_float3(c.x + d.x * start, c.y + d.y * start, c.z + d.z * start);
 }
 
 __device__ __forceinline__ void wave_item_ray(const GiArgs& a, const Wave& w, uint32_t i, float3& o, float3& d,
@@ -851,141 +385,17 @@ __global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
     }
 }
 
-// Support-weighted trilinear child lookup at level Lc.
-__device__ float4 interp_child(const GiArgs& a, uint32_t Lc, uint32_t dir, float3 rel) {
-    const lux_gi_params& p = a.p;
-    uint32_t dx = level_dim(p.dims[0], Lc), dy = level_dim(p.dims[1], Lc), dz = level_dim(p.dims[2], Lc);
-    uint32_t D = (p.base_phi << Lc) * (p.base_y << Lc);
-    float sc = p.spacing * pow2i((int)Lc);
-    float gx = rel.x / sc - 0.5f, gy = rel.y / sc - 0.5f, gz = rel.z / sc - 0.5f;
-    float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
-    int32_t bx0 = (int32_t)flx, by0 = (int32_t)fly, bz0 = (int32_t)flz;
-    float fx = gx - flx, fy = gy - fly, fz = gz - flz;
-    const float4* mc = a.merged + a.offset[Lc];
-    float3 sum = make_float3(0.0f, 0.0f, 0.0f);
-    float total = 0.0f, potential = 0.0f;
-    for (uint32_t corner = 0; corner < 8u; ++corner) {
-        uint32_t bx = corner & 1u, by = (corner >> 1) & 1u, bz = corner >> 2;
-        float ax = bx == 0u ? 1.0f - fx : fx;
-        float ay = by == 0u ? 1.0f - fy : fy;
-        float az = bz == 0u ? 1.0f - fz : fz;
-        float w = (ax * ay) * az;
-        if (!(w > 0.0f)) continue;
-        potential = potential + w;
-        int64_t cx = (int64_t)bx0 + bx, cy = (int64_t)by0 + by, cz = (int64_t)bz0 + bz;
-        if (cx < 0 || cy < 0 || cz < 0 || cx >= dx || cy >= dy || cz >= dz) continue;
-        uint64_t probe = ((uint64_t)cz * dy + (uint64_t)cy) * dx + (uint64_t)cx;
-        float4 smp = mc[probe * D + dir];
-        if (!(smp.w > 0.0f) || !(lux::isfinite_bits(smp.x) && lux::isfinite_bits(smp.y) &&
-                                 lux::isfinite_bits(smp.z) && lux::isfinite_bits(smp.w)))
-            continue;
-        float sw = w * sat(smp.w);
-        sum = make_float3(sum.x + smp.x * sw, sum.y + smp.y * sw, sum.z + smp.z * sw);
-        total = total + sw;
-    }
-    if (total > 0.0f)
-        return make_float4(sum.x / total, sum.y / total, sum.z / total,
-                           sat(total / smax(potential, 1e-8f)));
-    return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-}
-
 __global__ __launch_bounds__(kWg) void k_gi_merge(GiArgs a, uint32_t L) {
-    const lux_gi_params& p = a.p;
     uint64_t local = (uint64_t)blockIdx.x * kWg + threadIdx.x;
-    uint64_t count = a.offset[L + 1] - a.offset[L];
-    if (local >= count) return;
-    uint32_t dx = level_dim(p.dims[0], L), dy = level_dim(p.dims[1], L);
-    uint32_t nphi = p.base_phi << L;
-    uint32_t D = nphi * (p.base_y << L);
-    uint32_t k = (uint32_t)(local % D);
-    uint64_t probe = local / D;
-    uint32_t ix = (uint32_t)(probe % dx), iy = (uint32_t)((probe / dx) % dy);
-    uint32_t iz = (uint32_t)(probe / ((uint64_t)dx * dy));
-    float sl = p.spacing * pow2i((int)L);
-    float3 rel = make_float3(((float)ix + 0.5f) * sl, ((float)iy + 0.5f) * sl, ((float)iz + 0.5f) * sl);
-    float4 iv = a.intervals[a.offset[L] + local];
-    float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (iv.w >= 0.0f) {
-        float tr = sat(iv.w);
-        float support = 1.0f - tr;
-        float3 rad = make_float3(0.0f, 0.0f, 0.0f);
-        if (support > 0.0f) rad = make_float3(iv.x, iv.y, iv.z);
-        float3 csum = make_float3(0.0f, 0.0f, 0.0f);
-        float csup = 0.0f;
-        if (tr > 0.0f && L + 1 < p.levels) {
-            uint32_t x = k % nphi, y = k / nphi;
-            for (uint32_t child = 0; child < 4u; ++child) {
-                uint32_t fx = x * 2u + (child & 1u), fy = y * 2u + (child >> 1);
-                uint32_t fd = fx + fy * (2u * nphi);
-                float4 v = interp_child(a, L + 1, fd, rel);
-                if (!(v.w > 0.0f)) continue;
-                csum = make_float3(csum.x + v.x * v.w, csum.y + v.y * v.w, csum.z + v.z * v.w);
-                csup = csup + v.w;
-            }
-        }
-        if (csup > 0.0f) {
-            rad = make_float3(rad.x + (tr * csum.x) * 0.25f, rad.y + (tr * csum.y) * 0.25f,
-                              rad.z + (tr * csum.z) * 0.25f);
-            support = support + (tr * csup) * 0.25f;
-        }
-        if (support > 0.0f)
-            out = make_float4(rad.x / support, rad.y / support, rad.z / support, sat(support));
-    }
-    a.merged[a.offset[L] + local] = out;
+    if (local >= a.offset[L + 1] - a.offset[L]) return;
+    merge_interval(a, L, local);
 }
-
-// decodeNormal (common.slang:370-375).
-__device__ __forceinline__ float3 decode_normal(float px, float py) {
-    float x = px * 2.0f - 1.0f, y = py * 2.0f - 1.0f;
-    float z = 1.0f - fabsf(x) - fabsf(y);
-    float nz = -z;
-    float t = nz < 0.0f ? 0.0f : (nz > 1.0f ? 1.0f : nz);
-    x = x + (x >= 0.0f ? -t : t);
-    y = y + (y >= 0.0f ? -t : t);
-    float l = sqrtf((x * x + y * y) + z * z);
-    return make_float3(x / l, y / l, z / l);
-}
-
-constexpr uint32_t kMaxPackDirections = 256;
 
 // One 64-lane wave per level-0 probe (packProbes, split.slang:1788-1918).
 __global__ __launch_bounds__(64) void k_gi_pack(GiArgs a) {
     __shared__ float4 smp[kMaxPackDirections];
     __shared__ float3 dirs[kMaxPackDirections];
-    const lux_gi_params& p = a.p;
-    uint32_t probe = blockIdx.x;
-    uint32_t D = p.base_phi * p.base_y;
-    const float4* m = a.merged + (uint64_t)probe * D;
-    for (uint32_t k = threadIdx.x; k < D; k += 64u) {
-        float4 v = m[k];
-        bool ok = v.w > 0.0f && lux::isfinite_bits(v.x) && lux::isfinite_bits(v.y) &&
-                  lux::isfinite_bits(v.z) && lux::isfinite_bits(v.w);
-        smp[k] = ok ? v : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        dirs[k] = bin_direction(k, p.base_phi, p.base_y, 0.5f, 0.5f);
-    }
-    __syncthreads();
-    uint32_t t = threadIdx.x;
-    if (t >= 36u) return;
-    float3 n = decode_normal(((float)(t % 6u) + 0.5f) / 6.0f, ((float)(t / 6u) + 0.5f) / 6.0f);
-    float3 sum = make_float3(0.0f, 0.0f, 0.0f);
-    float nominal = 0.0f, observed = 0.0f;
-    for (uint32_t k = 0; k < D; ++k) {
-        float c = smax(dot3(n, dirs[k]), 0.0f);
-        nominal = nominal + c;
-        if (c <= 0.0f) continue;
-        float4 v = smp[k];
-        if (!(v.w > 0.0f)) continue;
-        float w = c * sat(v.w);
-        sum = make_float3(sum.x + v.x * w, sum.y + v.y * w, sum.z + v.z * w);
-        observed = observed + w;
-    }
-    float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (nominal > 0.0f && observed > 0.0f) {
-        float sc = nominal * (4.0f / (float)D);
-        out = make_float4((sum.x / observed) * sc, (sum.y / observed) * sc, (sum.z / observed) * sc,
-                          sat(observed / nominal));
-    }
-    a.probes[36ull * probe + t] = out;
+    pack_probe(a, blockIdx.x, smp, dirs);
 }
 
 __device__ __forceinline__ void wrap_texel(int32_t& x, int32_t& y) {
@@ -1225,22 +635,6 @@ std::vector<Light> build_lights(const lux_ctx* ctx) {
     return out;
 }
 
-GiArgs gi_args(lux_ctx* ctx) {
-    GiArgs a;
-    a.scene = ctx->dev_scene();
-    a.lights = static_cast<const Light*>(ctx->gi_light_buf.ptr);
-    a.n_lights = ctx->gi_lights;
-    a.mats = static_cast<const lux_material*>(ctx->materials.ptr);
-    a.n_mats = ctx->n_materials;
-    a.p = ctx->gi;
-    for (int i = 0; i < 17; ++i) a.offset[i] = ctx->gi_offset[i];
-    a.intervals = static_cast<float4*>(ctx->gi_intervals.ptr);
-    a.merged = static_cast<float4*>(ctx->gi_merged.ptr);
-    a.probes = static_cast<float4*>(ctx->gi_probes.ptr);
-    a.counters = static_cast<uint32_t*>(ctx->gi_counters.ptr);
-    a.admit_limit = ctx->gi.query_limit == 0u ? 0u : (ctx->gi.query_limit / 2u > 1u ? ctx->gi.query_limit / 2u : 1u);
-    return a;
-}
 
 // Emitter table of the current instance transforms (build_emissive_lights
 // runs again after the instances moved, as the reference re-packs them).
@@ -1265,6 +659,8 @@ int gi_check(lux_ctx* ctx) {
 }
 
 }  // namespace
+
+int lux_gi_check_internal(lux_ctx* ctx) { return gi_check(ctx); }
 
 extern "C" int lux_gi_configure(lux_ctx* ctx, const lux_gi_params* params) {
     if (!ctx) return LUX_ERR_INVALID;

@@ -249,6 +249,20 @@ def load_library() -> C.CDLL:
         "lux_gi_surrogate_queries": (i32, [vp, u32, vp, vp]),
         "lux_gi_update_surrogate": (i32, [vp, vp, u32, C.c_float, vp]),
         "lux_gi_surrogate_probe_error": (i32, [vp, vp, u32]),
+        "lux_batch_create": (i32, [vp, u32, C.POINTER(vp)]),
+        "lux_batch_destroy": (None, [vp]),
+        "lux_batch_last_error": (C.c_char_p, [vp]),
+        "lux_batch_set_surrogate": (i32, [vp, u32, vp, u32]),
+        "lux_batch_refresh": (i32, [vp]),
+        "lux_batch_set_frame": (i32, [vp, u32, u32]),
+        "lux_batch_update": (i32, [vp, vp]),
+        "lux_batch_capture": (i32, [vp]),
+        "lux_batch_launch": (i32, [vp, vp]),
+        "lux_batch_synchronize": (i32, [vp]),
+        "lux_batch_sizes": (i32, [vp, vp, vp, vp, vp]),
+        "lux_batch_set_config": (i32, [vp, vp]),
+        "lux_batch_get_config": (i32, [vp, vp]),
+        "lux_batch_autotune": (i32, [vp, u32, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -685,3 +699,97 @@ class Surrogate:
         m = None if mlp is None else np.ascontiguousarray(mlp, np.float32)
         t = None if table is None else np.ascontiguousarray(table, np.float32)
         self.ctx._check(self.lib.lux_surrogate_upload(self.h, _ptr(m), _ptr(t)), "lux_surrogate_upload")
+
+
+class Batch:
+    """One GI update of many configured scenes at once (lux_batch).
+
+    Holds references to its contexts (and surrogates) so they outlive it."""
+
+    def __init__(self, contexts, surrogates=None):
+        self.contexts = list(contexts)
+        if not self.contexts:
+            raise ValueError("Batch needs at least one context")
+        self.lib = self.contexts[0].lib
+        self.surrogates = {}
+        arr = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = C.c_void_p()
+        rc = self.lib.lux_batch_create(arr, len(self.contexts), C.byref(h))
+        if rc != LUX_OK:
+            msg = self.lib.lux_last_error(self.contexts[0].h)
+            raise LuxError(f"lux_batch_create: {rc} ({msg.decode() if msg else ''})")
+        self.h = h
+        for i, (s, level) in (surrogates or {}).items():
+            self.set_surrogate(i, s, level)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lux_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != LUX_OK:
+            msg = self.lib.lux_batch_last_error(self.h)
+            raise LuxError(f"{what}: {rc} ({msg.decode() if msg else ''})")
+
+    def set_surrogate(self, scene: int, surrogate, level: int = 0):
+        self._check(self.lib.lux_batch_set_surrogate(self.h, scene, surrogate.h if surrogate else None, level),
+                    "lux_batch_set_surrogate")
+        if surrogate is None:
+            self.surrogates.pop(scene, None)
+        else:
+            self.surrogates[scene] = (surrogate, level)
+
+    def refresh(self):
+        self._check(self.lib.lux_batch_refresh(self.h), "lux_batch_refresh")
+
+    def set_frame(self, frame: int, seed: int):
+        self._check(self.lib.lux_batch_set_frame(self.h, frame, seed), "lux_batch_set_frame")
+
+    def update(self, timed: bool = False):
+        ms = C.c_float()
+        self._check(self.lib.lux_batch_update(self.h, C.byref(ms) if timed else None), "lux_batch_update")
+        return float(ms.value) if timed else None
+
+    def capture(self):
+        self._check(self.lib.lux_batch_capture(self.h), "lux_batch_capture")
+
+    def launch(self, timed: bool = False):
+        ms = C.c_float()
+        self._check(self.lib.lux_batch_launch(self.h, C.byref(ms) if timed else None), "lux_batch_launch")
+        return float(ms.value) if timed else None
+
+    def synchronize(self):
+        self._check(self.lib.lux_batch_synchronize(self.h), "lux_batch_synchronize")
+
+    def sizes(self) -> dict:
+        it, iv, pr, sc = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint32()
+        self._check(self.lib.lux_batch_sizes(self.h, C.byref(it), C.byref(iv), C.byref(pr), C.byref(sc)),
+                    "lux_batch_sizes")
+        return dict(items=int(it.value), intervals=int(iv.value), probes=int(pr.value), scenes=int(sc.value))
+
+    def set_config(self, waves_per_cu: int = 16, lds_stack_depth: int = 16, regroup_threshold: int = 16,
+                   steps_per_refill: int = 8):
+        cfg = (C.c_uint32 * 8)(waves_per_cu, lds_stack_depth, regroup_threshold, steps_per_refill, 0, 0, 0, 0)
+        self._check(self.lib.lux_batch_set_config(self.h, cfg), "lux_batch_set_config")
+
+    def config(self) -> dict:
+        cfg = (C.c_uint32 * 8)()
+        self._check(self.lib.lux_batch_get_config(self.h, cfg), "lux_batch_get_config")
+        return dict(waves_per_cu=cfg[0], lds_stack_depth=cfg[1], regroup_threshold=cfg[2],
+                    steps_per_refill=cfg[3])
+
+    def autotune(self, repeats: int = 2) -> dict:
+        cfg = (C.c_uint32 * 8)()
+        ms = C.c_float()
+        ev = C.c_uint32()
+        self._check(self.lib.lux_batch_autotune(self.h, repeats, cfg, C.byref(ms), C.byref(ev)),
+                    "lux_batch_autotune")
+        return dict(waves_per_cu=cfg[0], lds_stack_depth=cfg[1], regroup_threshold=cfg[2],
+                    steps_per_refill=cfg[3], ms=float(ms.value), evaluations=int(ev.value))

@@ -98,6 +98,35 @@ def quad(p0, e1, e2) -> np.ndarray:
     return np.asarray([np.concatenate([a, b, c]), np.concatenate([a, c, d])])
 
 
+def _clutter_boxes(rng, nbox: int, ext, smallest: float, largest: float) -> np.ndarray:
+    """(12 * nbox, 9) triangles of boxes whose longest side is log-uniform in
+    [smallest, largest] (the other two 10-100 % of it), random yaw, centres
+    uniform where the box stays 0.05 m inside [0, ext]."""
+    ext = np.asarray(ext, np.float64)
+    longest = np.exp(rng.uniform(np.log(smallest), np.log(largest), size=nbox))
+    aspect = rng.uniform(0.1, 1.0, size=(nbox, 3))
+    aspect[np.arange(nbox), rng.integers(0, 3, nbox)] = 1.0
+    size = longest[:, None] * aspect
+    half = size / 2
+    lo = half + 0.05
+    hi = ext - half - 0.05
+    hi = np.maximum(hi, lo)
+    center = lo + rng.random((nbox, 3)) * (hi - lo)
+    yaw = rng.uniform(0, 2 * np.pi, nbox)
+    k = np.arange(8)
+    sgn = np.stack([np.where(k & 1, 1.0, -1.0), np.where(k & 2, 1.0, -1.0),
+                    np.where(k & 4, 1.0, -1.0)], axis=1)        # (8,3)
+    local = sgn[None, :, :] * half[:, None, :]                  # (n,8,3)
+    c, s = np.cos(yaw)[:, None], np.sin(yaw)[:, None]
+    x = local[..., 0] * c + local[..., 2] * s
+    z = -local[..., 0] * s + local[..., 2] * c
+    corners = np.stack([x, local[..., 1], z], axis=-1) + center[:, None, :]
+    faces = corners[:, _BOX_FACES]                              # (n,6,4,3)
+    t1 = np.concatenate([faces[:, :, 0], faces[:, :, 1], faces[:, :, 2]], axis=-1)
+    t2 = np.concatenate([faces[:, :, 0], faces[:, :, 2], faces[:, :, 3]], axis=-1)
+    return np.stack([t1, t2], axis=2).reshape(nbox * 12, 9)
+
+
 # ---------------------------------------------------------------- colonnade
 def colonnade(seed: int = 0, triangles: int = 262_144):
     """The procedural colonnade scene.  Returns (Scene, emissive triangle ids)."""
@@ -154,28 +183,7 @@ def colonnade(seed: int = 0, triangles: int = 262_144):
     nbox = remaining // 12
     # clutter: extruded boxes whose longest side is log-uniform in 0.05-4 m
     # (the other two sides 10-100 % of it), random yaw, kept inside the box
-    longest = np.exp(rng.uniform(np.log(0.05), np.log(4.0), size=nbox))
-    aspect = rng.uniform(0.1, 1.0, size=(nbox, 3))
-    aspect[np.arange(nbox), rng.integers(0, 3, nbox)] = 1.0
-    size = longest[:, None] * aspect
-    half = size / 2
-    lo = half + 0.05
-    hi = ext - half - 0.05
-    hi = np.maximum(hi, lo)
-    center = lo + rng.random((nbox, 3)) * (hi - lo)
-    yaw = rng.uniform(0, 2 * np.pi, nbox)
-    k = np.arange(8)
-    sgn = np.stack([np.where(k & 1, 1.0, -1.0), np.where(k & 2, 1.0, -1.0),
-                    np.where(k & 4, 1.0, -1.0)], axis=1)        # (8,3)
-    local = sgn[None, :, :] * half[:, None, :]                  # (n,8,3)
-    c, s = np.cos(yaw)[:, None], np.sin(yaw)[:, None]
-    x = local[..., 0] * c + local[..., 2] * s
-    z = -local[..., 0] * s + local[..., 2] * c
-    corners = np.stack([x, local[..., 1], z], axis=-1) + center[:, None, :]
-    faces = corners[:, _BOX_FACES]                              # (n,6,4,3)
-    t1 = np.concatenate([faces[:, :, 0], faces[:, :, 1], faces[:, :, 2]], axis=-1)
-    t2 = np.concatenate([faces[:, :, 0], faces[:, :, 2], faces[:, :, 3]], axis=-1)
-    clutter = np.stack([t1, t2], axis=2).reshape(nbox * 12, 9)
+    clutter = _clutter_boxes(rng, nbox, ext, 0.05, 4.0)
     tris.append(clutter)
     mats.append(np.repeat(rng.integers(0, 3, nbox).astype(np.uint32), 12))
     all_tris = np.concatenate(tris).astype(np.float32)
@@ -443,3 +451,59 @@ def camera_rays(origin, target, width: int, height: int, fov_y_deg: float = 60.0
     h = np.tan(np.radians(fov_y_deg) / 2)
     w = h * width / height
     return np.concatenate([o, fwd, right * w, up * h]).astype(np.float32)
+
+
+# ---------------------------------------------------------------- batched rooms
+def clutter_room(seed: int, triangles: int, size: float = 8.0):
+    """A closed cubic room of side `size` m (floor, ceiling and four walls as
+    0.1 m slabs), 2-4 emissive ceiling quads and boxes of log-uniform size
+    0.05-1.5 m filling it to `triangles` (rounded down to whole boxes, at
+    least 84 triangles).  Returns (Scene, emissive triangle ids)."""
+    rng = np.random.default_rng(seed)
+    L = float(size)
+    materials = [
+        [0.75, 0.75, 0.75, 1.0, 0, 0, 0, 0],
+        [0.63, 0.20, 0.15, 1.0, 0, 0, 0, 0],
+        [0.20, 0.45, 0.30, 1.0, 0, 0, 0, 0],
+        [0.8, 0.8, 0.8, 1.0, 17.0, 12.0, 4.0, 0],
+    ]
+    tris, mats = [], []
+    t = 0.05
+    for c, h in (((L / 2, t, L / 2), (L / 2, t, L / 2)), ((L / 2, L - t, L / 2), (L / 2, t, L / 2)),
+                 ((t, L / 2, L / 2), (t, L / 2, L / 2)), ((L - t, L / 2, L / 2), (t, L / 2, L / 2)),
+                 ((L / 2, L / 2, t), (L / 2, L / 2, t)), ((L / 2, L / 2, L - t), (L / 2, L / 2, t))):
+        tris.append(hexahedron(box_corners(c, h)))
+        mats.append(np.full(12, len(mats) % 3, np.uint32))
+    emit_first = sum(x.shape[0] for x in tris)
+    n_emit = int(rng.integers(2, 5))
+    for _ in range(n_emit):
+        sx, sz = rng.uniform(0.1, 0.25, 2) * L
+        x0, z0 = rng.uniform(0.1 * L, 0.9 * L - sx), rng.uniform(0.1 * L, 0.9 * L - sz)
+        tris.append(quad((x0, L - 2 * t - 1e-3, z0), (sx, 0, 0), (0, 0, sz)))  # normal -y
+        mats.append(np.full(2, 3, np.uint32))
+    emit_ids = np.arange(emit_first, emit_first + 2 * n_emit, dtype=np.uint32)
+    used = sum(x.shape[0] for x in tris)
+    nbox = max(1, (int(triangles) - used) // 12)
+    tris.append(_clutter_boxes(rng, nbox, (L, 0.85 * L, L), 0.05, 1.5))
+    mats.append(np.repeat(rng.integers(0, 3, nbox).astype(np.uint32), 12))
+    all_tris = np.concatenate(tris).astype(np.float32)
+    all_mats = np.concatenate(mats).astype(np.uint32)
+    return (single_instance(all_tris, all_mats, np.asarray(materials, np.float32), ((0.0, 0.0, 0.0), (L, L, L))),
+            emit_ids)
+
+
+def batch_set(seed: int = 0, count: int = 64, triangles=(1_000, 200_000), grid=(8, 48), levels=(3, 5)):
+    """Seeded specifications of `count` uneven scenes: triangle counts and
+    cubic probe-grid sides log-uniform in the given ranges, cascade counts
+    uniform in `levels`, room sides 4-12 m.  Each entry: seed, triangles,
+    size (m), dims, spacing (m), levels."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        tri = int(np.exp(rng.uniform(np.log(triangles[0]), np.log(triangles[1]))))
+        d = int(round(np.exp(rng.uniform(np.log(grid[0]), np.log(grid[1])))))
+        lv = int(rng.integers(levels[0], levels[1] + 1))
+        size = float(rng.uniform(4.0, 12.0))
+        out.append(dict(seed=int(seed) * 1000 + i, triangles=tri, size=size, dims=(d, d, d),
+                        spacing=size / d, levels=lv))
+    return out

@@ -412,6 +412,54 @@ int lux_get_trace_config(lux_ctx* ctx, lux_trace_config* cfg);
 int lux_autotune_trace(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits, uint32_t repeats,
                        lux_trace_config* best, float* best_ms, uint32_t* evaluations);
 
+/* ------------------------------------------------------------------ batch
+ * One GI update of many configured scenes of uneven size at once.  The
+ * reference runs the Split dispatch sequence once per view and scene
+ * (src/render.rs:9553-9801); a batch runs every scene's traced levels in ONE
+ * persistent closest-hit launch and ONE persistent occlusion launch whose
+ * lanes are refilled across scenes, then every scene's merge and pack, and
+ * can be captured once as a HIP graph and replayed per frame.  Each scene's
+ * intervals, merged radiance, probes and query counters are exactly what its
+ * own lux_gi_update (or lux_gi_update_surrogate with error_bound = +inf)
+ * writes.  The batch works on the scenes' own device arrays through a table
+ * built at create / refresh: after lux_gi_configure, lux_scene_upload_* or a
+ * BVH build of a member scene, call lux_batch_refresh (which drops a captured
+ * graph).  Batch work runs on the batch's own stream: call
+ * lux_batch_synchronize before reading a scene's results with per-scene calls,
+ * and do not update a member scene concurrently.  lux_batch_last_error gives
+ * the message of a failed batch call.  No counterpart in the reference. */
+typedef struct lux_batch lux_batch;
+int lux_batch_create(lux_ctx* const* scenes, uint32_t n, lux_batch** out);
+void lux_batch_destroy(lux_batch* b);
+const char* lux_batch_last_error(const lux_batch* b);
+/* Predict `level` of scene `scene` with surrogate s (of that scene's context;
+ * NULL: trace every level), as lux_gi_update_surrogate with no fallback. */
+int lux_batch_set_surrogate(lux_batch* b, uint32_t scene, lux_surrogate* s, uint32_t level);
+int lux_batch_refresh(lux_batch* b);
+/* Frame index and seed of every scene (kept in device memory: a captured
+ * graph picks them up); the member contexts' frame and seed follow. */
+int lux_batch_set_frame(lux_batch* b, uint32_t frame, uint32_t seed);
+/* Enqueue one update of every scene; ms nullable (when given, synchronizes
+ * and returns the update's device time). */
+int lux_batch_update(lux_batch* b, float* ms);
+/* Capture the update's launch sequence as a HIP graph / replay it. */
+int lux_batch_capture(lux_batch* b);
+int lux_batch_launch(lux_batch* b, float* ms);
+int lux_batch_synchronize(lux_batch* b);
+/* Totals over the batch: closest-hit items (interval x ray), traced
+ * intervals, level-0 probes, scenes.  Each pointer nullable. */
+int lux_batch_sizes(lux_batch* b, uint64_t* items, uint64_t* intervals, uint64_t* probes, uint32_t* scenes);
+/* Launch configuration of the two persistent walks (lux_trace_config
+ * meaning); setting it drops a captured graph. */
+int lux_batch_set_config(lux_batch* b, const lux_trace_config* cfg);
+int lux_batch_get_config(lux_batch* b, lux_trace_config* cfg);
+/* Autotuner over whole batch updates (coordinate descent over waves per CU,
+ * LDS stack depth, refill threshold, steps per refill; median of `repeats`);
+ * installs the fastest and drops a captured graph.  Every configuration
+ * writes identical results. */
+int lux_batch_autotune(lux_batch* b, uint32_t repeats, lux_trace_config* best, float* best_ms,
+                       uint32_t* evaluations);
+
 #ifdef __cplusplus
 }
 #endif
