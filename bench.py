@@ -13,8 +13,9 @@ kernel (HIP-event duration on the library's stream, algorithmic bytes from
 the instrumented kernel's node/triangle/instance counts) and a CPU baseline
 (the C restatement of the reference CPU algorithms -- SAH build_tree and the
 reference traversal -- on a bounded ray sample, timed on this host's cores).
-Further sections: the GI update (plain and surrogate-assisted at equal RMSE)
-and the dynamic 1,048,576-triangle frame (p50/p99).
+Further sections: the GI update (plain and surrogate-assisted at equal RMSE),
+the dynamic 1,048,576-triangle frame (p50/p99) and 64 uneven scenes updated
+as one batch captured in a HIP graph.
 """
 from __future__ import annotations
 
@@ -50,6 +51,10 @@ def parse():
     p.add_argument("--surrogate-levels", default="2,1")
     p.add_argument("--no-dynamic", action="store_true")
     p.add_argument("--dynamic-frames", type=int, default=120)
+    p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--batch-scenes", type=int, default=64)
+    p.add_argument("--batch-train", type=int, default=100,
+                   help="online training steps of each batch scene's surrogate")
     p.add_argument("--pmc-workload", action="store_true",
                    help="run only the counter-profiling workload (1 GiB device copy x2, three "
                         "traversal launches, two GI updates; default launch configurations)")
@@ -299,6 +304,150 @@ def dynamic_section(lux, scenes, args) -> dict:
     return res
 
 
+def batch_section(lux, scenes, args) -> dict:
+    """Batched uneven scenes: --batch-scenes clutter rooms (triangles
+    log-uniform 1k-200k, cubic probe grids log-uniform 8^3-48^3, 3-5 levels,
+    16 base directions, 4 emissive samples, seeded), updated as ONE batch: one
+    persistent closest-hit launch and one persistent occlusion launch over
+    every scene's intervals, then every scene's merge and pack, captured as a
+    HIP graph and replayed.  Reported: graph replay and eager ms of the plain
+    batch; the same scenes updated by lux_gi_update each, issued back to back
+    (every context has its own stream, so these run concurrently); the batch
+    with every scene's levels >= 2 predicted by its own surrogate (trained
+    online for --batch-train steps of 32,768 traced samples, no fallback),
+    each scene's probe RMSE against its plain update of the same frame, and
+    for context the cascades truncated at level 2 and the noise floor."""
+    import ctypes
+    import time
+    from bevy_penumbra_amd.lux import DeviceBuffer
+    specs = scenes.batch_set(seed=0, count=args.batch_scenes)
+    ctxs = []
+    t0 = time.perf_counter()
+    for s in specs:
+        sc, _ = scenes.clutter_room(s["seed"], s["triangles"], s["size"])
+        c = lux.Context(0)
+        c.upload_scene(sc)
+        c.build_bvh()
+        sp = s["spacing"]
+        c.gi_configure(lux.gi_params((0.5 * sp, 0.5 * sp, 0.5 * sp), sp, s["dims"], s["levels"], base_phi=4,
+                                     base_y=4, interval_length=sp, emissive_samples=4, terminal_tail=1))
+        ctxs.append(c)
+    for c in ctxs:
+        c.synchronize()
+    setup_s = time.perf_counter() - t0
+    b = lux.Batch(ctxs)
+    sizes = b.sizes()
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = b.autotune(repeats=1)
+        tune["tuning_s"] = round(time.perf_counter() - t0, 2)
+        tune["ms"] = round(tune["ms"], 3)
+    frame = 1
+    b.set_frame(frame, frame)
+    b.update()
+    eager = float(np.median([b.update(timed=True) for _ in range(3)]))
+    b.capture()
+    b.launch()
+    graph = float(np.median([b.launch(timed=True) for _ in range(5)]))
+    b.synchronize()
+    plain = [c.gi_download()["probes"] for c in ctxs]
+    seq = []
+    for _ in range(3):
+        for c in ctxs:
+            c.gi_set_frame(frame, frame)
+        t0 = time.perf_counter()
+        for c in ctxs:
+            c.gi_update()
+        for c in ctxs:
+            c.synchronize()
+        seq.append((time.perf_counter() - t0) * 1e3)
+    res = {
+        "workload": f"{len(specs)} clutter rooms, triangles log-uniform 1k-200k "
+                    f"({sum(s['triangles'] for s in specs)} total), cubic probe grids log-uniform 8^3-48^3, "
+                    "3-5 levels, 16 base directions, 4 emissive samples",
+        "scenes": sizes["scenes"], "intervals": sizes["intervals"], "items": sizes["items"],
+        "probes": sizes["probes"], "setup_s": round(setup_s, 2),
+        "plain": {"graph_ms": round(graph, 3), "eager_ms": round(eager, 3),
+                  "per_scene_updates_ms": round(float(np.median(seq)), 3),
+                  "interval_mrays_s": round(sizes["items"] / (graph * 1e-3) / 1e6, 2)},
+        "config": b.config(), "autotune": tune,
+    }
+    b.close()
+    level = 2
+    # context for the surrogate's error: levels >= 2 dropped (truncated) and
+    # levels >= 2 taken from another frame's trace (Monte Carlo noise floor)
+    floor = []
+    for c, p0 in zip(ctxs, plain):
+        c.gi_set_frame(frame + 1, frame + 1)
+        c.gi_trace_levels(level, c._gi.levels - 1)
+        c.gi_set_frame(frame, frame)
+        c.gi_trace_levels(0, level - 1)
+        c.gi_merge()
+        c.gi_pack()
+        floor.append(probe_rmse(c.gi_download()["probes"], p0))
+    saved = []
+    for c in ctxs:
+        tp = lux.GiParams()
+        ctypes.memmove(ctypes.byref(tp), ctypes.byref(c._gi), ctypes.sizeof(tp))
+        saved.append(c._gi)
+        tp.levels = level
+        tp.terminal_tail = 0
+        c.gi_configure(tp)
+    bt = lux.Batch(ctxs)
+    bt.set_config(**res["config"])
+    bt.set_frame(frame, frame)
+    bt.update()
+    t_ms = float(np.median([bt.update(timed=True) for _ in range(3)]))
+    bt.synchronize()
+    trunc = [probe_rmse(c.gi_download()["probes"], p0) for c, p0 in zip(ctxs, plain)]
+    bt.close()
+    for c, gp in zip(ctxs, saved):
+        c.gi_configure(gp)
+    surs, bufs = [], []
+    ns = 32768
+    for s, c in zip(specs, ctxs):
+        L = s["size"]
+        surs.append(lux.Surrogate(c, lux.SurrogateParams.default((0.0, 0.0, 0.0), (L, L, L))))
+        bufs.append(DeviceBuffer.empty(c, (ns, 12), np.float32))
+    t0 = time.perf_counter()
+    for f in range(args.batch_train):
+        for c, sur, buf in zip(ctxs, surs, bufs):
+            c.gi_set_frame(100 + f, 100 + f)
+            c.gi_surrogate_samples(level, ns, f, lux.LUX_SAMPLES_TRACE, buf)
+            sur.train_step(buf, ns, want_loss=False)
+    for c in ctxs:
+        c.synchronize()
+    train_s = time.perf_counter() - t0
+    bs = lux.Batch(ctxs, surrogates={i: (surs[i], level) for i in range(len(ctxs))})
+    bs.set_config(**res["config"])
+    bs.set_frame(frame, frame)
+    bs.update()
+    bs.capture()
+    bs.launch()
+    sgraph = float(np.median([bs.launch(timed=True) for _ in range(5)]))
+    sz2 = bs.sizes()
+    bs.synchronize()
+    rmse = np.array([probe_rmse(c.gi_download()["probes"], p0) for c, p0 in zip(ctxs, plain)])
+    res["surrogate"] = {
+        "level": level, "train_steps": args.batch_train, "samples_per_step": ns, "train_s": round(train_s, 2),
+        "graph_ms": round(sgraph, 3), "traced_items": sz2["items"],
+        "speedup_vs_plain_graph": round(graph / sgraph, 3),
+        "probe_rmse_median": round(float(np.median(rmse)), 5),
+        "probe_rmse_max": round(float(np.max(rmse)), 5),
+        "scenes_within_2pct": int((rmse <= 0.02).sum()),
+        "truncated": {"ms": round(t_ms, 3), "probe_rmse_median": round(float(np.median(trunc)), 5),
+                      "probe_rmse_max": round(float(np.max(trunc)), 5)},
+        "noise_floor_probe_rmse_median": round(float(np.median(floor)), 5),
+    }
+    bs.close()
+    for sur in surs:
+        sur.close()
+    for c in ctxs:
+        c.close()
+    return res
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -439,6 +588,8 @@ def main() -> int:
     ctx.close()
     if not args.no_dynamic:
         out["dynamic_frame"] = dynamic_section(lux, scenes, args)
+    if not args.no_batch:
+        out["batched_scenes"] = batch_section(lux, scenes, args)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -221,3 +221,25 @@ def test_gpu_gi_autotuner_keeps_results(lux_ctx):
     for key in ("intervals", "merged", "probes"):
         assert_bits(got[key], ref[key], key)
     lux_ctx.gi_set_trace_config(**saved)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gpu_level_ranges_compose_to_the_full_trace(lux_ctx, mode):
+    """Tracing levels [2, 4] and then [0, 1] (chunks starting past interval 0)
+    writes exactly the intervals of one full trace, in both trace modes."""
+    sc, _ = scenes.colonnade()
+    p = colonnade_params(dims=(16, 6, 8), spacing=2.0)
+    lux_ctx.gi_set_trace_mode(mode)
+    try:
+        full = run_gpu(lux_ctx, sc, p)
+        lux_ctx.gi_upload_intervals(np.full_like(full["intervals"], np.nan))
+        lux_ctx.gi_trace_levels(2, 4)
+        lux_ctx.gi_trace_levels(0, 1)
+        lux_ctx.gi_merge()
+        lux_ctx.gi_pack()
+        lux_ctx.synchronize()
+        parts = lux_ctx.gi_download()
+    finally:
+        lux_ctx.gi_set_trace_mode(0)
+    for key in ("intervals", "merged", "probes"):
+        assert_bits(parts[key], full[key], key)
