@@ -408,23 +408,21 @@ __device__ __forceinline__ void wrap_texel(int32_t& x, int32_t& y) {
     y = py;
 }
 
-// worldResolve at LOD 0 (world_resolve.slang:1-196,281-313).
-__global__ __launch_bounds__(kWg) void k_world_resolve(GiArgs a, const float* recv, uint32_t n,
-                                                       float4* out) {
-    uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    if (i >= n) return;
-    const lux_gi_params& p = a.p;
-    const float* r = recv + 6ull * i;
-    float nx = r[3], ny = r[4], nz = r[5];
-    float nn = (nx * nx + ny * ny) + nz * nz;
-    if (!(lux::isfinite_bits(nx) && lux::isfinite_bits(ny) && lux::isfinite_bits(nz)) || !(nn > 0.01f)) {
-        out[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        return;
-    }
+// Receiver eligibility (finite normal with |n|^2 > 0.01) and its unit normal.
+__device__ __forceinline__ bool resolve_normal(const float* r, float& nx, float& ny, float& nz) {
+    float x = r[3], y = r[4], z = r[5];
+    float nn = (x * x + y * y) + z * z;
+    if (!(lux::isfinite_bits(x) && lux::isfinite_bits(y) && lux::isfinite_bits(z)) || !(nn > 0.01f)) return false;
     float nl = sqrtf(nn);
-    nx = nx / nl;
-    ny = ny / nl;
-    nz = nz / nl;
+    nx = x / nl;
+    ny = y / nl;
+    nz = z / nl;
+    return true;
+}
+
+// irradianceLookup (world_resolve.slang:25-40): the octahedral footprint of
+// the unit normal -- 4 texels and their bilinear weights.
+__device__ __forceinline__ void resolve_lookup(float nx, float ny, float nz, uint32_t tex[4], float bw[4]) {
     float sabs = (fabsf(nx) + fabsf(ny)) + fabsf(nz);
     float ex = nx / sabs, ey = ny / sabs, ez = nz / sabs;
     if (ez < 0.0f) {
@@ -437,26 +435,43 @@ __global__ __launch_bounds__(kWg) void k_world_resolve(GiArgs a, const float* re
     float flx = floorf(gx), fly = floorf(gy);
     float fx = gx - flx, fy = gy - fly;
     float cx = 1.0f - fx, cy = 1.0f - fy;
-    float bw[4] = {cx * cy, fx * cy, cx * fy, fx * fy};
+    bw[0] = cx * cy;
+    bw[1] = fx * cy;
+    bw[2] = cx * fy;
+    bw[3] = fx * fy;
     int32_t bxi = (int32_t)flx, byi = (int32_t)fly;
-    uint32_t tex[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         int32_t tx = bxi + (c & 1), ty = byi + (c >> 1);
         wrap_texel(tx, ty);
         tex[c] = (uint32_t)(ty * 6 + tx);
     }
-    uint32_t dx = p.dims[0], dy = p.dims[1], dz = p.dims[2];
-    float rel[3] = {r[0] - p.origin[0], r[1] - p.origin[1], r[2] - p.origin[2]};
+}
+
+// One dense probe grid of the resolve (a LOD of the pyramid).
+struct ResolveGrid {
+    const float4* probes;  // 36 packed texels per probe
+    float origin[3];
+    float spacing;
+    uint32_t dims[3];
+};
+
+// diffuseAtLod + diffuseProbePage (world_resolve.slang:52-170): 8 probes
+// trilinear x 4 texels bilinear, support-normalised; probes outside the grid
+// are missing (they still count in the potential weight).
+__device__ void diffuse_at(const ResolveGrid& g, const float* pos, const uint32_t tex[4], const float bw[4],
+                           float3& val, float& cov) {
+    float rel[3] = {pos[0] - g.origin[0], pos[1] - g.origin[1], pos[2] - g.origin[2]};
     float fr[3];
     int32_t base[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        float g = rel[q] / p.spacing - 0.5f;
-        float fl = floorf(g);
+        float gq = rel[q] / g.spacing - 0.5f;
+        float fl = floorf(gq);
         base[q] = (int32_t)fl;
-        fr[q] = g - fl;
+        fr[q] = gq - fl;
     }
+    const uint32_t dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
     float3 sum = make_float3(0.0f, 0.0f, 0.0f);
     float observed = 0.0f, potential = 0.0f;
     for (uint32_t corner = 0; corner < 8u; ++corner) {
@@ -475,7 +490,7 @@ __global__ __launch_bounds__(kWg) void k_world_resolve(GiArgs a, const float* re
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             if (!(bw[c] > 0.0f)) continue;
-            float4 smp = a.probes[36ull * probe + tex[c]];
+            float4 smp = g.probes[36ull * probe + tex[c]];
             if (!(smp.w > 0.0f) || !(lux::isfinite_bits(smp.x) && lux::isfinite_bits(smp.y) &&
                                      lux::isfinite_bits(smp.z) && lux::isfinite_bits(smp.w)))
                 continue;
@@ -491,21 +506,137 @@ __global__ __launch_bounds__(kWg) void k_world_resolve(GiArgs a, const float* re
         sum = make_float3(sum.x + pv.x * sw, sum.y + pv.y * sw, sum.z + pv.z * sw);
         observed = observed + sw;
     }
-    float cov = 0.0f;
-    float3 val = make_float3(0.0f, 0.0f, 0.0f);
+    cov = 0.0f;
+    val = make_float3(0.0f, 0.0f, 0.0f);
     if (potential > 0.0f) {
         cov = sat(observed / potential);
         if (observed > 0.0f) val = make_float3(sum.x / observed, sum.y / observed, sum.z / observed);
     }
-    float lw = cov * 1.0f, hw = 1.0f * 0.0f, tw = lw + hw;
+}
+
+// resolveLodSource (world_resolve.slang:172-182) and the raw alpha code
+// (src/lib.rs:320-335): 0 not eligible, -1 unsupported, 1 + coverage / 4.
+__device__ __forceinline__ float4 resolve_combine(float3 lv, float lc, float3 hv, float hc, float blend) {
+    float lw = lc * (1.0f - blend), hw = hc * blend, tw = lw + hw;
     float coverage = sat(tw);
     float3 res = make_float3(0.0f, 0.0f, 0.0f);
     if (tw > 0.0f)
-        res = make_float3((val.x * lw + 0.0f * hw) / tw, (val.y * lw + 0.0f * hw) / tw,
-                          (val.z * lw + 0.0f * hw) / tw);
+        res = make_float3((lv.x * lw + hv.x * hw) / tw, (lv.y * lw + hv.y * hw) / tw, (lv.z * lw + hv.z * hw) / tw);
     bool supported = coverage > 0.0f && lux::isfinite_bits(res.x) && lux::isfinite_bits(res.y) &&
                      lux::isfinite_bits(res.z);
-    out[i] = make_float4(res.x, res.y, res.z, supported ? 1.0f + 0.25f * coverage : -1.0f);
+    return make_float4(res.x, res.y, res.z, supported ? 1.0f + 0.25f * coverage : -1.0f);
+}
+
+// splitLodBlendForParameters (shaders/common.slang:431-461).  floor(log2(x))
+// of x >= 1 comes from the exponent field (exact); exp2 of an integer is a
+// power of two; max(lodScale, 1) maps NaN to 1 (IEEE maxNum).
+__device__ void lod_select(const lux_lod_view& v, const float* pos, uint32_t& low, uint32_t& high,
+                           float& blend) {
+    const float* pp = v.clip_row3;
+    bool perspective = lux::isfinite_bits(pp[0]) && lux::isfinite_bits(pp[1]) && lux::isfinite_bits(pp[2]) &&
+                       ((pp[0] * pp[0] + pp[1] * pp[1]) + pp[2] * pp[2]) > 1e-12f;
+    const float* vp = v.clip_row1;
+    float focal = 0.5f * (float)v.viewport_height * sqrtf((vp[0] * vp[0] + vp[1] * vp[1]) + vp[2] * vp[2]);
+    uint32_t target = v.target_pixels > 1u ? v.target_pixels : 1u;
+    float ts = v.probe_spacing;
+    float scale;
+    if (perspective) {
+        float ox = fabsf(pos[0] - v.camera_position[0]);
+        float oy = fabsf(pos[1] - v.camera_position[1]);
+        float oz = fabsf(pos[2] - v.camera_position[2]);
+        float m = oz > oy ? oz : oy;
+        float radius = m > ox ? m : ox;
+        float bd = focal * ts / (float)target;
+        float base = ts > bd ? ts : bd;
+        scale = radius / base;
+    } else {
+        float d = focal * ts;
+        float den = 1e-20f > d ? 1e-20f : d;
+        scale = (float)target / den;
+    }
+    float m = scale >= 1.0f ? scale : 1.0f;
+    uint32_t e = ((__float_as_uint(m) >> 23) & 0xffu) - 127u;
+    uint32_t lo = e < 24u ? e : 24u;
+    low = lo;
+    high = lo;
+    blend = 0.0f;
+    if (lo < 24u) {
+        float next = pow2i((int)lo + 1);
+        float start = 0.9f * next;
+        if (scale >= start && scale < next) {
+            high = lo + 1u;
+            blend = sat((scale - start) / (next - start));
+        }
+    }
+}
+
+__device__ __forceinline__ ResolveGrid grid_of(const GiArgs& a) {
+    ResolveGrid g;
+    g.probes = a.probes;
+    g.origin[0] = a.p.origin[0];
+    g.origin[1] = a.p.origin[1];
+    g.origin[2] = a.p.origin[2];
+    g.spacing = a.p.spacing;
+    g.dims[0] = a.p.dims[0];
+    g.dims[1] = a.p.dims[1];
+    g.dims[2] = a.p.dims[2];
+    return g;
+}
+
+// worldResolve at LOD 0 (world_resolve.slang:1-196,281-313): the LOD pair
+// with blend 0 (no high LOD, its coverage 1).
+__global__ __launch_bounds__(kWg) void k_world_resolve(GiArgs a, const float* recv, uint32_t n,
+                                                       float4* out) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    const float* r = recv + 6ull * i;
+    float nx, ny, nz;
+    if (!resolve_normal(r, nx, ny, nz)) {
+        out[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    uint32_t tex[4];
+    float bw[4];
+    resolve_lookup(nx, ny, nz, tex, bw);
+    ResolveGrid g = grid_of(a);
+    float3 val;
+    float cov;
+    diffuse_at(g, r, tex, bw, val, cov);
+    out[i] = resolve_combine(val, cov, make_float3(0.0f, 0.0f, 0.0f), 1.0f, 0.0f);
+}
+
+constexpr uint32_t kMaxResolveLods = 8;
+struct ResolveLods {
+    ResolveGrid g[kMaxResolveLods];
+    uint32_t count;
+};
+
+// resolveSplitDiffuse (world_resolve.slang:184-219) over a pyramid of dense
+// grids, grid k at LOD k; LODs at or past `count` have no probes.
+__global__ __launch_bounds__(kWg) void k_world_resolve_lods(ResolveLods lods, lux_lod_view view, const float* recv,
+                                                            uint32_t n, float4* out) {
+    uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    const float* r = recv + 6ull * i;
+    float nx, ny, nz;
+    if (!resolve_normal(r, nx, ny, nz)) {
+        out[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    uint32_t tex[4];
+    float bw[4];
+    resolve_lookup(nx, ny, nz, tex, bw);
+    uint32_t low, high;
+    float blend;
+    lod_select(view, r, low, high, blend);
+    float3 lv = make_float3(0.0f, 0.0f, 0.0f), hv = lv;
+    float lc = 0.0f, hc = 1.0f;
+    if (low < lods.count) diffuse_at(lods.g[low], r, tex, bw, lv, lc);
+    if (high != low) {
+        hc = 0.0f;
+        if (high < lods.count) diffuse_at(lods.g[high], r, tex, bw, hv, hc);
+    }
+    out[i] = resolve_combine(lv, lc, hv, hc, blend);
 }
 
 struct Camera {
@@ -1038,6 +1169,38 @@ extern "C" int lux_world_resolve_device(lux_ctx* ctx, const float* receivers, ui
     if (!receivers || !out) return lux_fail(ctx, LUX_ERR_INVALID, "lux_world_resolve_device: null buffer");
     GiArgs a = gi_args(ctx);
     hipLaunchKernelGGL(k_world_resolve, dim3((n + kWg - 1) / kWg), dim3(kWg), 0, ctx->stream, a,
+                       receivers, n, reinterpret_cast<float4*>(out));
+    LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
+extern "C" int lux_world_resolve_lods(lux_ctx* const* lods, uint32_t lod_count, const lux_lod_view* view,
+                                      const float* receivers, uint32_t n, float* out) {
+    if (!lods || lod_count == 0 || !lods[0]) return LUX_ERR_INVALID;
+    lux_ctx* ctx = lods[0];
+    if (lod_count > kMaxResolveLods)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_world_resolve_lods: at most 8 LOD grids");
+    if (!view) return lux_fail(ctx, LUX_ERR_INVALID, "lux_world_resolve_lods: null view");
+    ResolveLods g{};
+    g.count = lod_count;
+    for (uint32_t k = 0; k < lod_count; ++k) {
+        lux_ctx* c = lods[k];
+        if (!c || c->device != ctx->device)
+            return lux_fail(ctx, LUX_ERR_INVALID, "lux_world_resolve_lods: LOD grid of another device");
+        if (!c->gi_ready)
+            return lux_fail(ctx, LUX_ERR_STATE, "lux_world_resolve_lods: LOD grid not configured");
+        g.g[k].probes = static_cast<const float4*>(c->gi_probes.ptr);
+        for (int q = 0; q < 3; ++q) {
+            g.g[k].origin[q] = c->gi.origin[q];
+            g.g[k].dims[q] = c->gi.dims[q];
+        }
+        g.g[k].spacing = c->gi.spacing;
+        // the other grids' packs run on their own streams
+        if (k > 0) LUX_HIP(ctx, hipStreamSynchronize(c->stream));
+    }
+    if (n == 0) return LUX_OK;
+    if (!receivers || !out) return lux_fail(ctx, LUX_ERR_INVALID, "lux_world_resolve_lods: null buffer");
+    hipLaunchKernelGGL(k_world_resolve_lods, dim3((n + kWg - 1) / kWg), dim3(kWg), 0, ctx->stream, g, *view,
                        receivers, n, reinterpret_cast<float4*>(out));
     LUX_HIP(ctx, hipGetLastError());
     return LUX_OK;

@@ -155,6 +155,45 @@ class SurrogateStats(C.Structure):
                 for k, _ in self._fields_ if k != "reserved"}
 
 
+class LodView(C.Structure):
+    """lux_lod_view (include/lux.h): camera terms of the split LOD choice."""
+    _fields_ = [("camera_position", C.c_float * 3), ("probe_spacing", C.c_float),
+                ("clip_row1", C.c_float * 3), ("viewport_height", C.c_uint32),
+                ("clip_row3", C.c_float * 3), ("target_pixels", C.c_uint32)]
+
+    @classmethod
+    def make(cls, camera=(0.0, 0.0, 0.0), probe_spacing=1.0, clip_row1=(0.0, 1.0, 0.0), viewport_height=2,
+             clip_row3=(0.0, 0.0, -1.0), target_pixels=1) -> "LodView":
+        v = cls()
+        v.camera_position[:] = [float(x) for x in camera]
+        v.probe_spacing = probe_spacing
+        v.clip_row1[:] = [float(x) for x in clip_row1]
+        v.viewport_height = viewport_height
+        v.clip_row3[:] = [float(x) for x in clip_row3]
+        v.target_pixels = target_pixels
+        return v
+
+
+def lod_gi_params(p: "GiParams", lod: int) -> "GiParams":
+    """The grid of LOD `lod` of a LOD-0 grid: spacing and interval length
+    times 2^lod (shaders/common.slang:414-416, shaders/split.slang:224-227),
+    dims halved (rounded up) per LOD, same origin."""
+    q = GiParams()
+    C.memmove(C.byref(q), C.byref(p), C.sizeof(p))
+    q.spacing = p.spacing * 2.0 ** lod
+    q.interval_length = p.interval_length * 2.0 ** lod
+    q.dims[:] = [max(1, -(-int(d) // (1 << lod))) for d in p.dims]
+    return q
+
+
+def world_resolve_lods(contexts, view: LodView, receivers, n: int, out):
+    """lux_world_resolve_lods: resolve against the LOD pyramid contexts[k] = LOD k."""
+    lib = contexts[0].lib
+    arr = (C.c_void_p * len(contexts))(*[c.h for c in contexts])
+    contexts[0]._check(lib.lux_world_resolve_lods(arr, len(contexts), C.byref(view), _ptr(receivers), n, _ptr(out)),
+                       "lux_world_resolve_lods")
+
+
 LUX_SAMPLES_TRACE = 0
 LUX_SAMPLES_MERGED = 1
 
@@ -249,6 +288,7 @@ def load_library() -> C.CDLL:
         "lux_gi_surrogate_queries": (i32, [vp, u32, vp, vp]),
         "lux_gi_update_surrogate": (i32, [vp, vp, u32, C.c_float, vp]),
         "lux_gi_surrogate_probe_error": (i32, [vp, vp, u32]),
+        "lux_world_resolve_lods": (i32, [vp, u32, vp, vp, u32, vp]),
         "lux_batch_create": (i32, [vp, u32, C.POINTER(vp)]),
         "lux_batch_destroy": (None, [vp]),
         "lux_batch_last_error": (C.c_char_p, [vp]),

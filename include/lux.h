@@ -262,6 +262,30 @@ int lux_gi_device_arrays(lux_ctx* ctx, float** intervals, float** merged, float*
  * code (0 not eligible, -1 unsupported, 1 + 0.25 coverage). */
 int lux_world_resolve_device(lux_ctx* ctx, const float* receivers, uint32_t n, float* out);
 
+/* Camera terms of the split LOD choice (splitLodBlendForParameters,
+ * shaders/common.slang:431-461).  The clip rows are rows 1 (clip y) and 3
+ * (clip w) of clip_from_world in column-vector convention, xyz only; row 3 is
+ * zero for an orthographic camera.  48 bytes. */
+typedef struct {
+    float camera_position[3];
+    float probe_spacing;        /* RadianceCascades::probe_spacing (tuning.x) */
+    float clip_row1[3];
+    uint32_t viewport_height;   /* pixels (viewport.w) */
+    float clip_row3[3];
+    uint32_t target_pixels;     /* split_lod_target_pixels (src/render.rs:181-183), >= 1 */
+} lux_lod_view;
+
+/* resolveSplitDiffuse (shaders/world_resolve.slang:184-219, worldResolve
+ * :281-313) over a pyramid of dense probe grids: lods[k] is a configured
+ * context whose grid is LOD k (spacing and interval length scaled by 2^k,
+ * shaders/common.slang:414-416, shaders/split.slang:224-227).  Per receiver:
+ * the LOD pair and blend from `view`, each LOD's diffuse lookup (LODs at or
+ * beyond lod_count have no probes), resolveLodSource, the support code of
+ * lux_world_resolve_device.  Runs on lods[0]'s stream after waiting for the
+ * other contexts' streams; lod_count <= 8. */
+int lux_world_resolve_lods(lux_ctx* const* lods, uint32_t lod_count, const lux_lod_view* view,
+                           const float* receivers, uint32_t n, float* out);
+
 /* Traversal query counters of the current update: attempts and admitted
  * (admitted <= the budget; both count every interval and NEE query).
  * lux_gi_update and lux_gi_update_surrogate reset them at their start, as

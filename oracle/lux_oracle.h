@@ -211,6 +211,24 @@ void orc_gi_pack_f64(const orc_gi_params* p, const double* merged, double* probe
 /* receivers: 6 floats (position, normal); out: rgba (alpha = support code). */
 void orc_world_resolve(const orc_gi_params* p, const float* probes, const float* receivers,
                        uint32_t n, float* out);
+/* Camera terms of the split LOD choice (splitLodBlendForParameters,
+ * shaders/common.slang:431-461); the layout of lux_lod_view. */
+typedef struct {
+    float camera_position[3];
+    float probe_spacing;       /* tuning.x */
+    float clip_row1[3];        /* clip_from_world row 1 (clip y), xyz */
+    uint32_t viewport_height;  /* viewport.w */
+    float clip_row3[3];        /* clip_from_world row 3 (clip w), xyz; zero: orthographic */
+    uint32_t target_pixels;    /* ownerSequence.x (split_lod_target_pixels) */
+} orc_lod_view;
+void orc_lod_select(const orc_lod_view* v, const float* position, uint32_t* low, uint32_t* high,
+                    float* blend);
+/* resolveSplitDiffuse over a pyramid of dense grids (grid k = LOD k). */
+void orc_world_resolve_lods(const orc_gi_params* const* grids, const float* const* probes, uint32_t count,
+                            const orc_lod_view* view, const float* receivers, uint32_t n, float* out);
+void orc_world_resolve_lods_f64(const orc_gi_params* const* grids, const double* const* probes,
+                                uint32_t count, const orc_lod_view* view, const float* receivers, uint32_t n,
+                                double* out);
 void orc_world_resolve_f64(const orc_gi_params* p, const double* probes, const float* receivers,
                            uint32_t n, double* out);
 /* Shading in f64 (FP64 twin): same hits and visibility, f64 radiance. */

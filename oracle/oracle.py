@@ -242,6 +242,9 @@ def _tsig():
         ("orc_gi_pack_f64", None, [vp, vp, vp]),
         ("orc_world_resolve", None, [vp, vp, vp, u32, vp]),
         ("orc_world_resolve_f64", None, [vp, vp, vp, u32, vp]),
+        ("orc_lod_select", None, [vp, vp, vp, vp, vp]),
+        ("orc_world_resolve_lods", None, [vp, vp, u32, vp, vp, u32, vp]),
+        ("orc_world_resolve_lods_f64", None, [vp, vp, u32, vp, vp, u32, vp]),
         ("orc_gi_level_queries", None, [vp, u32, vp]),
         ("orc_gi_sample_index", C.c_uint64, [u32, u32, C.c_uint64]),
     ]:
@@ -351,6 +354,47 @@ def gi_pack(p: GiParams, merged: np.ndarray) -> np.ndarray:
     out = np.zeros((n0, 36, 4), merged.dtype)
     m = np.ascontiguousarray(merged)
     (lib.orc_gi_pack_f64 if f64 else lib.orc_gi_pack)(C.byref(p), m.ctypes.data, out.ctypes.data)
+    return out
+
+
+class LodView(C.Structure):
+    """Mirror of orc_lod_view / lux_lod_view: camera terms of the LOD choice."""
+    _fields_ = [("camera_position", C.c_float * 3), ("probe_spacing", C.c_float),
+                ("clip_row1", C.c_float * 3), ("viewport_height", C.c_uint32),
+                ("clip_row3", C.c_float * 3), ("target_pixels", C.c_uint32)]
+
+
+def lod_view(camera=(0.0, 0.0, 0.0), probe_spacing=1.0, clip_row1=(0.0, 1.0, 0.0), viewport_height=2,
+             clip_row3=(0.0, 0.0, -1.0), target_pixels=1) -> LodView:
+    v = LodView()
+    v.camera_position[:] = [float(x) for x in camera]
+    v.probe_spacing = probe_spacing
+    v.clip_row1[:] = [float(x) for x in clip_row1]
+    v.viewport_height = viewport_height
+    v.clip_row3[:] = [float(x) for x in clip_row3]
+    v.target_pixels = target_pixels
+    return v
+
+
+def lod_select(view: LodView, position) -> tuple:
+    lo, hi, b = C.c_uint32(), C.c_uint32(), C.c_float()
+    pos = np.ascontiguousarray(position, np.float32)
+    _tsig().orc_lod_select(C.byref(view), pos.ctypes.data, C.byref(lo), C.byref(hi), C.byref(b))
+    return int(lo.value), int(hi.value), float(b.value)
+
+
+def world_resolve_lods(grids, probes, view: LodView, receivers: np.ndarray) -> np.ndarray:
+    """resolveSplitDiffuse over dense grids: grids[k] (GiParams) with packed
+    probes[k] at LOD k."""
+    lib = _tsig()
+    f64 = probes[0].dtype == np.float64
+    arrs = [np.ascontiguousarray(a) for a in probes]
+    gp = (C.c_void_p * len(grids))(*[C.cast(C.byref(g), C.c_void_p) for g in grids])
+    pp = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    rc = np.ascontiguousarray(receivers, np.float32)
+    out = np.zeros((rc.shape[0], 4), arrs[0].dtype)
+    (lib.orc_world_resolve_lods_f64 if f64 else lib.orc_world_resolve_lods)(
+        gp, pp, len(grids), C.byref(view), rc.ctypes.data, rc.shape[0], out.ctypes.data)
     return out
 
 

@@ -328,6 +328,51 @@ static void geometric_normal(const orc_instance* in, const float* tri, float out
 }
 
 /* ---------------- the REAL-generic parts (float product, double twin) ---------- */
+/* splitLodBlendForParameters (shaders/common.slang:431-461).  floor(log2(x))
+ * of x >= 1 is read from the exponent field (exact, where a log2
+ * approximation may round across a power of two); exp2 of an integer is a
+ * power of two.  max(lodScale, 1) maps NaN to 1 as IEEE maxNum does. */
+void orc_lod_select(const orc_lod_view* v, const float* position, uint32_t* low, uint32_t* high,
+                    float* blend) {
+    const float* pp = v->clip_row3;
+    int perspective = finitef_(pp[0]) && finitef_(pp[1]) && finitef_(pp[2]) &&
+                      ((pp[0] * pp[0] + pp[1] * pp[1]) + pp[2] * pp[2]) > 1e-12f;
+    const float* vp = v->clip_row1;
+    float focal = 0.5f * (float)v->viewport_height * sqrtf((vp[0] * vp[0] + vp[1] * vp[1]) + vp[2] * vp[2]);
+    uint32_t target = v->target_pixels > 1u ? v->target_pixels : 1u;
+    float ts = v->probe_spacing;
+    float scale;
+    if (perspective) {
+        float ox = fabsf(position[0] - v->camera_position[0]);
+        float oy = fabsf(position[1] - v->camera_position[1]);
+        float oz = fabsf(position[2] - v->camera_position[2]);
+        float m = oz > oy ? oz : oy;
+        float radius = m > ox ? m : ox;
+        float bd = focal * ts / (float)target;
+        float base = ts > bd ? ts : bd;
+        scale = radius / base;
+    } else {
+        float d = focal * ts;
+        float den = 1e-20f > d ? 1e-20f : d;
+        scale = (float)target / den;
+    }
+    float m = scale >= 1.0f ? scale : 1.0f;
+    uint32_t e = ((fbits(m) >> 23) & 0xffu) - 127u; /* m >= 1: floor(log2 m); inf: 128 */
+    uint32_t lo = e < 24u ? e : 24u;
+    *low = lo;
+    *high = lo;
+    *blend = 0.0f;
+    if (lo < 24u) {
+        float next = ldexpf(1.0f, (int)lo + 1);
+        float start = 0.9f * next;
+        if (scale >= start && scale < next) {
+            *high = lo + 1u;
+            float b = (scale - start) / (next - start);
+            *blend = b < 0.0f ? 0.0f : (b > 1.0f ? 1.0f : b);
+        }
+    }
+}
+
 #define REAL float
 #define SFX(name) name##_f32
 #include "orc_transport_impl.h"
@@ -368,6 +413,15 @@ void orc_world_resolve(const orc_gi_params* p, const float* probes, const float*
 void orc_world_resolve_f64(const orc_gi_params* p, const double* probes, const float* receivers,
                            uint32_t n, double* out) {
     world_resolve_f64(p, probes, receivers, n, out);
+}
+void orc_world_resolve_lods(const orc_gi_params* const* grids, const float* const* probes, uint32_t count,
+                            const orc_lod_view* view, const float* receivers, uint32_t n, float* out) {
+    world_resolve_lods_f32(grids, probes, count, view, receivers, n, out);
+}
+void orc_world_resolve_lods_f64(const orc_gi_params* const* grids, const double* const* probes,
+                                uint32_t count, const orc_lod_view* view, const float* receivers, uint32_t n,
+                                double* out) {
+    world_resolve_lods_f64(grids, probes, count, view, receivers, n, out);
 }
 
 /* Surrogate query of every (probe, bin) of a level, in merged-array order:

@@ -441,7 +441,114 @@ static void SFX(wrap_texel)(int32_t* x, int32_t* y) {
     *x = px; *y = py;
 }
 
-/* worldResolve at LOD 0 (world_resolve.slang:1-196,281-313). */
+/* irradianceLookup (world_resolve.slang:25-40): the octahedral footprint of
+ * the unit normal -- 4 texels and bilinear weights. */
+static void SFX(resolve_lookup)(float nx, float ny, float nz, uint32_t tex[4], REAL bw[4]) {
+    float sabs = (fabsf(nx) + fabsf(ny)) + fabsf(nz);
+    float ex = nx / sabs, ey = ny / sabs, ez = nz / sabs;
+    if (ez < 0.0f) {
+        float tx = (1.0f - fabsf(ey)) * (ex >= 0.0f ? 1.0f : -1.0f);
+        float ty = (1.0f - fabsf(ex)) * (ey >= 0.0f ? 1.0f : -1.0f);
+        ex = tx; ey = ty;
+    }
+    float gx = (ex * 0.5f + 0.5f) * 6.0f - 0.5f, gy = (ey * 0.5f + 0.5f) * 6.0f - 0.5f;
+    float flx = floorf(gx), fly = floorf(gy);
+    float fx = gx - flx, fy = gy - fly;
+    float cx = 1.0f - fx, cy = 1.0f - fy;
+    bw[0] = (REAL)(cx * cy); bw[1] = (REAL)(fx * cy); bw[2] = (REAL)(cx * fy); bw[3] = (REAL)(fx * fy);
+    int32_t bxi = (int32_t)flx, byi = (int32_t)fly;
+    for (int c = 0; c < 4; ++c) {
+        int32_t tx = bxi + (c & 1), ty = byi + (c >> 1);
+        SFX(wrap_texel)(&tx, &ty);
+        tex[c] = (uint32_t)(ty * 6 + tx);
+    }
+}
+
+/* diffuseAtLod + diffuseProbePage (world_resolve.slang:52-170) over one dense
+ * probe grid: 8 probes trilinear x 4 texels bilinear, support-normalised.
+ * Probes outside the grid are missing (they count in the potential weight).
+ * probes == NULL: every probe missing. */
+static void SFX(diffuse_at)(const float origin[3], float spacing, const uint32_t dims[3], const REAL* probes,
+                            const float* pos, const uint32_t tex[4], const REAL bw[4], REAL val[3],
+                            REAL* cov) {
+    float rel[3] = {pos[0] - origin[0], pos[1] - origin[1], pos[2] - origin[2]};
+    float fr[3];
+    int32_t base[3];
+    for (int a = 0; a < 3; ++a) {
+        float g = rel[a] / spacing - 0.5f;
+        float fl = floorf(g);
+        base[a] = (int32_t)fl;
+        fr[a] = g - fl;
+    }
+    REAL sum[3] = {K(0.0), K(0.0), K(0.0)}, observed = K(0.0), potential = K(0.0);
+    for (uint32_t corner = 0; corner < 8u; ++corner) {
+        uint32_t bx = corner & 1u, by = (corner >> 1) & 1u, bz = corner >> 2;
+        REAL ax = bx == 0u ? K(1.0) - (REAL)fr[0] : (REAL)fr[0];
+        REAL ay = by == 0u ? K(1.0) - (REAL)fr[1] : (REAL)fr[1];
+        REAL az = bz == 0u ? K(1.0) - (REAL)fr[2] : (REAL)fr[2];
+        REAL w = (ax * ay) * az;
+        if (!(w > K(0.0))) continue;
+        potential = potential + w;
+        if (!probes) continue;
+        int64_t qx = (int64_t)base[0] + bx, qy = (int64_t)base[1] + by, qz = (int64_t)base[2] + bz;
+        if (qx < 0 || qy < 0 || qz < 0 || qx >= dims[0] || qy >= dims[1] || qz >= dims[2]) continue;
+        uint64_t probe = ((uint64_t)qz * dims[1] + (uint64_t)qy) * dims[0] + (uint64_t)qx;
+        /* diffuseProbePage */
+        REAL psum[3] = {K(0.0), K(0.0), K(0.0)}, pobs = K(0.0);
+        for (int c = 0; c < 4; ++c) {
+            if (!(bw[c] > K(0.0))) continue;
+            const REAL* smp = probes + 4 * (36ull * probe + tex[c]);
+            if (!(smp[3] > K(0.0)) || !(isfinite(smp[0]) && isfinite(smp[1]) &&
+                                        isfinite(smp[2]) && isfinite(smp[3])))
+                continue;
+            REAL sw = bw[c] * SFX(sat)(smp[3]);
+            for (int q = 0; q < 3; ++q) psum[q] = psum[q] + smp[q] * sw;
+            pobs = pobs + sw;
+        }
+        REAL pv[3] = {K(0.0), K(0.0), K(0.0)};
+        if (pobs > K(0.0))
+            for (int q = 0; q < 3; ++q) pv[q] = psum[q] / pobs;
+        REAL support = SFX(sat)(pobs);
+        if (!(support > K(0.0))) continue;
+        REAL sw = w * support;
+        for (int q = 0; q < 3; ++q) sum[q] = sum[q] + pv[q] * sw;
+        observed = observed + sw;
+    }
+    val[0] = val[1] = val[2] = K(0.0);
+    *cov = K(0.0);
+    if (potential > K(0.0)) {
+        *cov = SFX(sat)(observed / potential);
+        if (observed > K(0.0))
+            for (int q = 0; q < 3; ++q) val[q] = sum[q] / observed;
+    }
+}
+
+/* resolveLodSource (world_resolve.slang:172-182) and the raw alpha code
+ * (src/lib.rs:320-335): 0 not eligible, -1 unsupported, 1 + coverage / 4. */
+static void SFX(resolve_combine)(const REAL lval[3], REAL lcov, const REAL hval[3], REAL hcov, float blend,
+                                 REAL* o) {
+    REAL lw = lcov * (K(1.0) - (REAL)blend), hw = hcov * (REAL)blend, tw = lw + hw;
+    REAL coverage = SFX(sat)(tw);
+    REAL res[3] = {K(0.0), K(0.0), K(0.0)};
+    if (tw > K(0.0))
+        for (int q = 0; q < 3; ++q) res[q] = (lval[q] * lw + hval[q] * hw) / tw;
+    int supported = coverage > K(0.0) && isfinite(res[0]) && isfinite(res[1]) && isfinite(res[2]);
+    o[0] = res[0]; o[1] = res[1]; o[2] = res[2];
+    o[3] = supported ? K(1.0) + K(0.25) * coverage : K(-1.0);
+}
+
+/* Receiver eligibility (finite normal, |n|^2 > 0.01) and its unit normal. */
+static int SFX(resolve_normal)(const float* r, float* nx, float* ny, float* nz) {
+    float x = r[3], y = r[4], z = r[5];
+    float nn = (x * x + y * y) + z * z;
+    if (!(finitef_(x) && finitef_(y) && finitef_(z)) || !(nn > 0.01f)) return 0;
+    float nl = sqrtf(nn);
+    *nx = x / nl; *ny = y / nl; *nz = z / nl;
+    return 1;
+}
+
+/* worldResolve at LOD 0 (world_resolve.slang:1-196,281-313): the LOD pair
+ * with blend 0 (no high LOD, its coverage 1). */
 static void SFX(world_resolve)(const orc_gi_params* p, const REAL* probes, const float* recv,
                                uint32_t n, REAL* out) {
     uint32_t dims[3];
@@ -450,89 +557,52 @@ static void SFX(world_resolve)(const orc_gi_params* p, const REAL* probes, const
         const float* r = recv + 6ull * i;
         REAL* o = out + 4ull * i;
         o[0] = o[1] = o[2] = o[3] = K(0.0);
-        float nx = r[3], ny = r[4], nz = r[5];
-        float nn = (nx * nx + ny * ny) + nz * nz;
-        if (!(finitef_(nx) && finitef_(ny) && finitef_(nz)) || !(nn > 0.01f)) continue; /* not eligible */
-        float nl = sqrtf(nn);
-        nx = nx / nl; ny = ny / nl; nz = nz / nl;
-        /* irradianceLookup: encodeNormal */
-        float sabs = (fabsf(nx) + fabsf(ny)) + fabsf(nz);
-        float ex = nx / sabs, ey = ny / sabs, ez = nz / sabs;
-        if (ez < 0.0f) {
-            float tx = (1.0f - fabsf(ey)) * (ex >= 0.0f ? 1.0f : -1.0f);
-            float ty = (1.0f - fabsf(ex)) * (ey >= 0.0f ? 1.0f : -1.0f);
-            ex = tx; ey = ty;
-        }
-        float gx = (ex * 0.5f + 0.5f) * 6.0f - 0.5f, gy = (ey * 0.5f + 0.5f) * 6.0f - 0.5f;
-        float flx = floorf(gx), fly = floorf(gy);
-        float fx = gx - flx, fy = gy - fly;
-        float cx = 1.0f - fx, cy = 1.0f - fy;
-        REAL bw[4] = {(REAL)(cx * cy), (REAL)(fx * cy), (REAL)(cx * fy), (REAL)(fx * fy)};
-        int32_t bxi = (int32_t)flx, byi = (int32_t)fly;
+        float nx, ny, nz;
+        if (!SFX(resolve_normal)(r, &nx, &ny, &nz)) continue; /* not eligible */
         uint32_t tex[4];
-        for (int c = 0; c < 4; ++c) {
-            int32_t tx = bxi + (c & 1), ty = byi + (c >> 1);
-            SFX(wrap_texel)(&tx, &ty);
-            tex[c] = (uint32_t)(ty * 6 + tx);
+        REAL bw[4];
+        SFX(resolve_lookup)(nx, ny, nz, tex, bw);
+        REAL val[3], cov;
+        SFX(diffuse_at)(p->origin, p->spacing, dims, probes, r, tex, bw, val, &cov);
+        const REAL zero[3] = {K(0.0), K(0.0), K(0.0)};
+        SFX(resolve_combine)(val, cov, zero, K(1.0), 0.0f, o);
+    }
+}
+
+/* resolveSplitDiffuse (world_resolve.slang:184-219) over a pyramid of dense
+ * probe grids, grid k at LOD k: the LOD pair and blend of each receiver
+ * (orc_lod_select), each LOD's diffuse lookup (LODs >= count have no probes),
+ * resolveLodSource. */
+static void SFX(world_resolve_lods)(const orc_gi_params* const* grids, const REAL* const* probes, uint32_t count,
+                                    const orc_lod_view* view, const float* recv, uint32_t n, REAL* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = recv + 6ull * i;
+        REAL* o = out + 4ull * i;
+        o[0] = o[1] = o[2] = o[3] = K(0.0);
+        float nx, ny, nz;
+        if (!SFX(resolve_normal)(r, &nx, &ny, &nz)) continue;
+        uint32_t tex[4];
+        REAL bw[4];
+        SFX(resolve_lookup)(nx, ny, nz, tex, bw);
+        uint32_t low, high;
+        float blend;
+        orc_lod_select(view, r, &low, &high, &blend);
+        REAL lval[3] = {K(0.0), K(0.0), K(0.0)}, lcov = K(0.0);
+        REAL hval[3] = {K(0.0), K(0.0), K(0.0)}, hcov = K(1.0);
+        uint32_t dims[3];
+        if (low < count) {
+            orc_gi_level_dims(grids[low], 0, dims);
+            SFX(diffuse_at)(grids[low]->origin, grids[low]->spacing, dims, probes[low], r, tex, bw, lval, &lcov);
         }
-        /* diffuseAtLod(position, lookup, 0) */
-        float rel[3] = {r[0] - p->origin[0], r[1] - p->origin[1], r[2] - p->origin[2]};
-        float grid[3], fr[3];
-        int32_t base[3];
-        for (int a = 0; a < 3; ++a) {
-            grid[a] = rel[a] / p->spacing - 0.5f;
-            float fl = floorf(grid[a]);
-            base[a] = (int32_t)fl;
-            fr[a] = grid[a] - fl;
-        }
-        REAL sum[3] = {K(0.0), K(0.0), K(0.0)}, observed = K(0.0), potential = K(0.0);
-        for (uint32_t corner = 0; corner < 8u; ++corner) {
-            uint32_t bx = corner & 1u, by = (corner >> 1) & 1u, bz = corner >> 2;
-            REAL ax = bx == 0u ? K(1.0) - (REAL)fr[0] : (REAL)fr[0];
-            REAL ay = by == 0u ? K(1.0) - (REAL)fr[1] : (REAL)fr[1];
-            REAL az = bz == 0u ? K(1.0) - (REAL)fr[2] : (REAL)fr[2];
-            REAL w = (ax * ay) * az;
-            if (!(w > K(0.0))) continue;
-            potential = potential + w;
-            int64_t qx = (int64_t)base[0] + bx, qy = (int64_t)base[1] + by, qz = (int64_t)base[2] + bz;
-            if (qx < 0 || qy < 0 || qz < 0 || qx >= dims[0] || qy >= dims[1] || qz >= dims[2]) continue;
-            uint64_t probe = ((uint64_t)qz * dims[1] + (uint64_t)qy) * dims[0] + (uint64_t)qx;
-            /* diffuseProbePage */
-            REAL psum[3] = {K(0.0), K(0.0), K(0.0)}, pobs = K(0.0);
-            for (int c = 0; c < 4; ++c) {
-                if (!(bw[c] > K(0.0))) continue;
-                const REAL* smp = probes + 4 * (36ull * probe + tex[c]);
-                if (!(smp[3] > K(0.0)) || !(isfinite(smp[0]) && isfinite(smp[1]) &&
-                                            isfinite(smp[2]) && isfinite(smp[3])))
-                    continue;
-                REAL sw = bw[c] * SFX(sat)(smp[3]);
-                for (int q = 0; q < 3; ++q) psum[q] = psum[q] + smp[q] * sw;
-                pobs = pobs + sw;
+        if (high != low) {
+            hcov = K(0.0);
+            if (high < count) {
+                orc_gi_level_dims(grids[high], 0, dims);
+                SFX(diffuse_at)(grids[high]->origin, grids[high]->spacing, dims, probes[high], r, tex, bw, hval,
+                                &hcov);
             }
-            REAL pv[3] = {K(0.0), K(0.0), K(0.0)};
-            if (pobs > K(0.0))
-                for (int q = 0; q < 3; ++q) pv[q] = psum[q] / pobs;
-            REAL support = SFX(sat)(pobs);
-            if (!(support > K(0.0))) continue;
-            REAL sw = w * support;
-            for (int q = 0; q < 3; ++q) sum[q] = sum[q] + pv[q] * sw;
-            observed = observed + sw;
         }
-        REAL cov = K(0.0), val[3] = {K(0.0), K(0.0), K(0.0)};
-        if (potential > K(0.0)) {
-            cov = SFX(sat)(observed / potential);
-            if (observed > K(0.0))
-                for (int q = 0; q < 3; ++q) val[q] = sum[q] / observed;
-        }
-        /* resolveLodSource with a single LOD (blend 0, high coverage 1) */
-        REAL lw = cov * K(1.0), hw = K(1.0) * K(0.0), tw = lw + hw;
-        REAL coverage = SFX(sat)(tw);
-        REAL res[3] = {K(0.0), K(0.0), K(0.0)};
-        if (tw > K(0.0))
-            for (int q = 0; q < 3; ++q) res[q] = (val[q] * lw + K(0.0) * hw) / tw;
-        int supported = coverage > K(0.0) && isfinite(res[0]) && isfinite(res[1]) && isfinite(res[2]);
-        o[0] = res[0]; o[1] = res[1]; o[2] = res[2];
-        o[3] = supported ? K(1.0) + K(0.25) * coverage : K(-1.0);
+        SFX(resolve_combine)(lval, lcov, hval, hcov, blend, o);
     }
 }
 #undef K

@@ -52,10 +52,10 @@ def test_headers_compile_as_c_and_struct_sizes_match_the_bindings(tmp_path):
     src.write_text(
         '#include <stdio.h>\n#include "lux.h"\n#include "lux_bevy.h"\n'
         "int main(void) {\n"
-        '  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(lux_ray), sizeof(lux_hit),'
+        '  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(lux_ray), sizeof(lux_hit),'
         " sizeof(lux_gi_params), sizeof(lux_surrogate_params), sizeof(lux_gi_surrogate_stats),"
         " sizeof(lux_trace_config), sizeof(lux_instance_desc), sizeof(lux_material),"
-        " sizeof(lux_radiance_cascades));\n"
+        " sizeof(lux_radiance_cascades), sizeof(lux_lod_view));\n"
         "  return 0;\n}\n")
     exe = tmp_path / "sizes"
     subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", "-I", str(ROOT / "include"), str(src), "-o", str(exe)],
@@ -63,5 +63,5 @@ def test_headers_compile_as_c_and_struct_sizes_match_the_bindings(tmp_path):
     got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     want = [32, 16, ctypes.sizeof(lux.GiParams), ctypes.sizeof(lux.SurrogateParams),
             ctypes.sizeof(lux.SurrogateStats), 32, ctypes.sizeof(lux.InstanceDesc), 32,
-            ctypes.sizeof(lux.RadianceCascades)]
+            ctypes.sizeof(lux.RadianceCascades), ctypes.sizeof(lux.LodView)]
     assert got == want
