@@ -5,6 +5,7 @@
 // copied on call.  Errors never abort: they return a negative status and the
 // text is kept for lux_last_error (forwarded to
 // RadianceCascadesStatus::record_failure, src/lib.rs:225-230).
+#include <algorithm>
 #include <cstring>
 #include <cstdio>
 #include "lux_internal.h"
@@ -287,6 +288,45 @@ extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_wo
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_set_transforms: instance count changed");
     if (instance_count && (!local_from_world || !world_from_local))
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_set_transforms: null array");
+    // dirty regions for the GI history: the world boxes of every moved
+    // instance before and after the move (a whole-history reset past the
+    // dirty-box capacity)
+    std::vector<float> dirty;
+    bool reset_history = false;
+    if (ctx->gi_hist_on) {
+        uint32_t moved = 0;
+        for (uint32_t i = 0; i < instance_count; ++i)
+            if (memcmp(ctx->instances[i].world_from_local, world_from_local + 12ull * i, 48) != 0) ++moved;
+        if (2 * moved > 16) {
+            reset_history = true;
+        } else {
+            for (uint32_t i = 0; i < instance_count; ++i) {
+                const float* nf = world_from_local + 12ull * i;
+                if (memcmp(ctx->instances[i].world_from_local, nf, 48) == 0) continue;
+                uint32_t m = ctx->instances[i].mesh;
+                if (m >= ctx->host_pos.size() || ctx->host_pos[m].empty()) continue;
+                const std::vector<float>& pos = ctx->host_pos[m];
+                float lo[3] = {pos[0], pos[1], pos[2]}, hi[3] = {pos[0], pos[1], pos[2]};
+                for (size_t v = 0; v < pos.size(); v += 3)
+                    for (int q = 0; q < 3; ++q) {
+                        lo[q] = std::min(lo[q], pos[v + q]);
+                        hi[q] = std::max(hi[q], pos[v + q]);
+                    }
+                for (const float* f : {static_cast<const float*>(ctx->instances[i].world_from_local), nf}) {
+                    float wlo[3] = {INFINITY, INFINITY, INFINITY}, whi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                    for (int c = 0; c < 8; ++c) {
+                        float pl[3] = {(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+                        for (int r = 0; r < 3; ++r) {
+                            float w = f[4 * r] * pl[0] + f[4 * r + 1] * pl[1] + f[4 * r + 2] * pl[2] + f[4 * r + 3];
+                            wlo[r] = std::min(wlo[r], w);
+                            whi[r] = std::max(whi[r], w);
+                        }
+                    }
+                    dirty.insert(dirty.end(), {wlo[0], wlo[1], wlo[2], whi[0], whi[1], whi[2]});
+                }
+            }
+        }
+    }
     std::vector<InstDesc> idesc(instance_count);
     for (uint32_t i = 0; i < instance_count; ++i) {
         memcpy(idesc[i].inv, local_from_world + 12ull * i, 48);
@@ -302,6 +342,9 @@ extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_wo
                                hipMemcpyHostToDevice));
     ctx->tlas_built = false;
     ctx->gi_lights_dirty = true;
+    if (reset_history) return lux_gi_reset_history(ctx);
+    for (size_t b = 0; b + 6 <= dirty.size(); b += 6)
+        if (int rc = lux_gi_invalidate_history(ctx, &dirty[b], &dirty[b + 3])) return rc;
     return LUX_OK;
 }
 

@@ -282,6 +282,28 @@ __global__ __launch_bounds__(kWg) void k_b_accum(BatchWave w) {
         out = make_float4(jsum.x / fn, jsum.y / fn, jsum.z / fn, bsum / fn);
     }
     a.intervals[g] = out;
+    if (a.samples) a.samples[g] = n;
+}
+
+// Temporal history of every traced interval of the scenes that keep one.
+__global__ __launch_bounds__(kWg) void k_b_history(BatchWave w) {
+    uint32_t j = blockIdx.x * kWg + threadIdx.x;
+    if (j >= w.interval_base[w.n_scenes]) return;
+    uint32_t s = find_scene(w.interval_base, w.n_scenes, j);
+    const GiArgs& a = w.sc[s].a;
+    if (!a.history) return;
+    uint64_t g = j - w.interval_base[s];
+    uint32_t L = 0;
+    while (L + 1 < a.p.levels && g >= a.offset[L + 1]) ++L;
+    history_blend(a, g, L, g - a.offset[L]);
+}
+
+__global__ void k_b_history_done(const BatchScene* sc, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && sc[i].a.hist_state) {
+        sc[i].a.hist_state[0] = 1u;
+        sc[i].a.hist_state[1] = 0u;
+    }
 }
 
 __global__ __launch_bounds__(kWg) void k_b_merge(BatchWave w, uint32_t L) {
@@ -344,6 +366,7 @@ struct lux_batch {
     std::vector<BatchScene> host_table;
     std::vector<uint64_t> item_base, interval_base, probe_base, merge_base;
     uint32_t nee_max = 1, max_levels = 1;
+    bool any_history = false;
     uint64_t query_max = 0;
     lux_ctx* owner = nullptr;  // allocations go through the first scene's context
     lux_buffer table, bases, prim, shade, slots, vis, shadow, counters, queries, sur_scratch;
@@ -394,11 +417,13 @@ int prepare(lux_batch* b) {
     b->nee_max = 1;
     b->max_levels = 1;
     b->query_max = 0;
+    b->any_history = false;
     for (uint32_t s = 0; s < n; ++s) {
         lux_ctx* c = b->scenes[s];
         if (int rc = lux_gi_check_internal(c)) return bfail(b, rc, std::string("scene ") + std::to_string(s) + ": " + lux_last_error(c));
         BatchScene& bs = b->host_table[s];
         bs.a = gi_args(c);
+        if (bs.a.history) b->any_history = true;
         bs.R = c->gi.rays_per_interval ? c->gi.rays_per_interval : 1u;
         uint32_t levels = c->gi.levels;
         uint32_t S = b->sur[s] ? b->sur_level[s] : levels;
@@ -493,6 +518,11 @@ int enqueue(lux_batch* b) {
         default: hipLaunchKernelGGL(k_b_shadow<16>, dim3(pblocks), dim3(kWg), 0, st, w); break;
         }
         hipLaunchKernelGGL(k_b_accum, dim3((uint32_t)((intervals + kWg - 1) / kWg)), dim3(kWg), 0, st, w);
+        if (b->any_history) {
+            hipLaunchKernelGGL(k_b_history, dim3((uint32_t)((intervals + kWg - 1) / kWg)), dim3(kWg), 0, st, w);
+            hipLaunchKernelGGL(k_b_history_done, dim3((n + 255) / 256), dim3(256), 0, st,
+                               static_cast<const BatchScene*>(b->table.ptr), n);
+        }
     }
     for (uint32_t s = 0; s < n; ++s) {
         if (!b->sur[s]) continue;
@@ -610,6 +640,7 @@ extern "C" int lux_batch_update(lux_batch* b, float* ms) {
     if (!b->ready) return bfail(b, LUX_ERR_STATE, "lux_batch_update: batch not prepared");
     LUX_BHIP(b, hipEventRecord(b->ev0, b->stream));
     if (int rc = enqueue(b)) return rc;
+    for (lux_ctx* c : b->scenes) c->gi_dirty_n = 0;
     LUX_BHIP(b, hipEventRecord(b->ev1, b->stream));
     if (ms) {
         LUX_BHIP(b, hipEventSynchronize(b->ev1));
@@ -642,6 +673,7 @@ extern "C" int lux_batch_launch(lux_batch* b, float* ms) {
     if (!b->exec) return bfail(b, LUX_ERR_STATE, "lux_batch_launch: no captured graph (lux_batch_capture)");
     LUX_BHIP(b, hipEventRecord(b->ev0, b->stream));
     LUX_BHIP(b, hipGraphLaunch(b->exec, b->stream));
+    for (lux_ctx* c : b->scenes) c->gi_dirty_n = 0;
     LUX_BHIP(b, hipEventRecord(b->ev1, b->stream));
     if (ms) {
         LUX_BHIP(b, hipEventSynchronize(b->ev1));

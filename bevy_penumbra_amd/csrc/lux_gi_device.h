@@ -33,7 +33,19 @@ struct GiArgs {
     float4* probes;
     uint32_t* counters;     // [0] attempts, [1] admitted
     uint32_t admit_limit;   // 0 = unbounded
+    // temporal history of raw intervals (null when off): supported rays per
+    // interval of this update; per interval (J.rgb, radiance mass) and
+    // (beta, transmittance mass, last sampled frame bits, 0); state words
+    // [0] history valid, [1] dirty boxes, [2..7] emitter bounds (lo, hi),
+    // then 8 floats per dirty box (lo, pad, hi, pad) from word 8
+    uint32_t* samples;
+    float4* history;
+    uint32_t* hist_state;
+    float hist_rb, hist_tb;
+    uint32_t hist_frames, hist_persistent;
 };
+
+constexpr uint32_t kMaxDirtyBoxes = 16;
 
 // Ticketed per-update traversal budget (admitTransportQuery,
 // shaders/traversal.slang:20-38), aggregated per wave: the active lanes take
@@ -394,7 +406,7 @@ __device__ __forceinline__ float3 probe_centre(const lux_gi_params& p, uint32_t 
 // J = mean shaded radiance, beta = mean transmittance (1 on a miss of a
 // bounded interval), beta = -1 when no ray was supported.
 __device__ float4 trace_interval(const GiArgs& a, uint32_t L, float3 c, uint32_t k, bool terminal,
-                                 Stack& stk) {
+                                 Stack& stk, uint32_t* n_out = nullptr) {
     const lux_gi_params& p = a.p;
     uint32_t nphi = p.base_phi << L, ny = p.base_y << L;
     float end = interval_end(p, L);
@@ -425,6 +437,7 @@ __device__ float4 trace_interval(const GiArgs& a, uint32_t L, float3 c, uint32_t
         bsum = bsum + beta;
         n++;
     }
+    if (n_out) *n_out = n;
     if (n == 0) return make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     float fn = (float)n;
     return make_float4(jsum.x / fn, jsum.y / fn, jsum.z / fn, bsum / fn);
@@ -527,11 +540,24 @@ __device__ void merge_interval(const GiArgs& a, uint32_t L, uint64_t local) {
     uint32_t iz = (uint32_t)(probe / ((uint64_t)dx * dy));
     float sl = p.spacing * pow2i((int)L);
     float3 rel = make_float3(((float)ix + 0.5f) * sl, ((float)iy + 0.5f) * sl, ((float)iz + 0.5f) * sl);
-    float4 iv = a.intervals[a.offset[L] + local];
+    const uint64_t g = a.offset[L] + local;
+    float4 iv = a.intervals[g];
     float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (iv.w >= 0.0f) {
-        float tr = sat(iv.w);
-        float support = 1.0f - tr;
+    // own radiance needs both masses; children need transmittance only
+    // (beginMergeGroup, split.slang:1533-1545); without history both masses
+    // are 1 for a sampled interval (beta >= 0) and 0 otherwise
+    bool has;
+    float tr;
+    if (a.history) {
+        float rm = a.history[2 * g].w, tm = a.history[2 * g + 1].y;
+        tr = tm > 0.0f ? sat(iv.w) : 0.0f;
+        has = rm > 0.0f && tm > 0.0f;
+    } else {
+        has = iv.w >= 0.0f;
+        tr = has ? sat(iv.w) : 0.0f;
+    }
+    if (has || tr > 0.0f) {
+        float support = has ? 1.0f - tr : 0.0f;
         float3 rad = make_float3(0.0f, 0.0f, 0.0f);
         if (support > 0.0f) rad = make_float3(iv.x, iv.y, iv.z);
         float3 csum = make_float3(0.0f, 0.0f, 0.0f);
@@ -555,7 +581,97 @@ __device__ void merge_interval(const GiArgs& a, uint32_t L, uint64_t local) {
         if (support > 0.0f)
             out = make_float4(rad.x / support, rad.y / support, rad.z / support, sat(support));
     }
-    a.merged[a.offset[L] + local] = out;
+    a.merged[g] = out;
+}
+
+// x^n for a small integer n by repeated multiplication (pow(x, n) of the
+// reference for integer sample counts; exact for n = 1).
+__device__ __forceinline__ float powi(float x, uint32_t n) {
+    float r = 1.0f;
+    for (uint32_t i = 0; i < n; ++i) r = r * x;
+    return r;
+}
+
+__device__ __forceinline__ float valid_mass(float m) { return (lux::isfinite_bits(m) && m >= 0.0f) ? m : 0.0f; }
+
+// Temporal blend of interval g (level L, index `local` in the level) with
+// its history: mergeEntry's history part (split.slang:1624-1705) and
+// mergeRawRadianceGroup (:1478-1521) for the one radiance source.  History
+// is reused when valid, alive (persistent world intervals, or sampled within
+// hist_frames frames; intervalHistoryAlive, common.slang:85-89) and the
+// interval's dependency box meets no dirty box.  The dependency box bounds
+// every segment the interval's rays and their NEE occlusion rays traverse:
+// the box of probe centre +- interval end joined with the emitter bounds
+// (unbounded for a terminal interval).  Writes the blended (J, beta) in
+// place and the new history.
+__device__ void history_blend(const GiArgs& a, uint64_t g, uint32_t L, uint64_t local) {
+    const lux_gi_params& p = a.p;
+    const uint32_t* st = a.hist_state;
+    float4 iv = a.intervals[g];
+    uint32_t n = a.samples[g];
+    float4 h0 = a.history[2 * g], h1 = a.history[2 * g + 1];
+    uint32_t last = __float_as_uint(h1.z);
+    bool reuse = st[0] != 0u && a.hist_frames > 0u;
+    if (reuse) reuse = a.hist_persistent != 0u || p.frame - last <= a.hist_frames;
+    uint32_t nd = st[1] < kMaxDirtyBoxes ? st[1] : kMaxDirtyBoxes;
+    if (reuse && nd > 0u) {
+        bool terminal = p.terminal_tail != 0u && L + 1 == p.levels;
+        uint32_t D = (p.base_phi << L) * (p.base_y << L);
+        float3 c = probe_centre(p, L, local / D);
+        float e = interval_end(p, L);
+        const float* em = reinterpret_cast<const float*>(st + 2);
+        float dlo[3] = {smin(c.x - e, em[0]), smin(c.y - e, em[1]), smin(c.z - e, em[2])};
+        float dhi[3] = {smax(c.x + e, em[3]), smax(c.y + e, em[4]), smax(c.z + e, em[5])};
+        for (uint32_t b = 0; b < nd && reuse; ++b) {
+            const float* box = reinterpret_cast<const float*>(st + 8) + 8 * b;
+            bool hit = terminal || (dlo[0] <= box[4] && dhi[0] >= box[0] && dlo[1] <= box[5] &&
+                                    dhi[1] >= box[1] && dlo[2] <= box[6] && dhi[2] >= box[2]);
+            if (hit) reuse = false;
+        }
+    }
+    const float rbr = a.hist_rb, tbr = a.hist_tb;
+    const float rb = smin(sat(rbr), 0.999f), tb = smin(sat(tbr), 0.999f);
+    const bool reuse_r = reuse && rbr > 0.0f, reuse_t = reuse && tbr > 0.0f;
+    const float rlim = 1.0f / (1.0f - rb), tlim = 1.0f / (1.0f - tb);
+    const float om = valid_mass(h0.w), otm = valid_mass(h1.y);
+    float3 J = make_float3(iv.x, iv.y, iv.z);
+    float beta = iv.w, mass, ts;
+    if (n > 0u) {
+        float ret = powi(rb, n);
+        mass = (1.0f - ret) * rlim;
+        float o = smin(om, rlim) * ret;
+        if (reuse_r && o > 0.0f) {
+            float t = mass + o;
+            J = make_float3((J.x * mass + h0.x * o) / t, (J.y * mass + h0.y * o) / t, (J.z * mass + h0.z * o) / t);
+            mass = mass + o;
+        }
+        float tret = powi(tb, n);
+        ts = (1.0f - tret) * tlim;
+        float ot = smin(otm, tlim) * tret;
+        if (reuse_t && ot > 0.0f) {
+            beta = (beta * ts + h1.x * ot) / (ts + ot);
+            ts = ts + ot;
+        }
+        last = p.frame;
+    } else {
+        if (reuse_r && om > 0.0f) {
+            J = make_float3(h0.x, h0.y, h0.z);
+            mass = om;
+        } else {
+            J = make_float3(0.0f, 0.0f, 0.0f);
+            mass = 0.0f;
+        }
+        if (reuse_t && otm > 0.0f) {
+            beta = h1.x;
+            ts = otm;
+        } else {
+            beta = 1.0f;
+            ts = 0.0f;
+        }
+    }
+    a.intervals[g] = make_float4(J.x, J.y, J.z, beta);
+    a.history[2 * g] = make_float4(J.x, J.y, J.z, mass);
+    a.history[2 * g + 1] = make_float4(beta, ts, __uint_as_float(last), 0.0f);
 }
 
 // decodeNormal (common.slang:370-375).
@@ -625,6 +741,14 @@ inline GiArgs gi_args(lux_ctx* ctx) {
     a.probes = static_cast<float4*>(ctx->gi_probes.ptr);
     a.counters = static_cast<uint32_t*>(ctx->gi_counters.ptr);
     a.admit_limit = ctx->gi.query_limit == 0u ? 0u : (ctx->gi.query_limit / 2u > 1u ? ctx->gi.query_limit / 2u : 1u);
+    const bool hist = ctx->gi_hist_on;
+    a.samples = hist ? static_cast<uint32_t*>(ctx->gi_samples.ptr) : nullptr;
+    a.history = hist ? static_cast<float4*>(ctx->gi_history.ptr) : nullptr;
+    a.hist_state = hist ? static_cast<uint32_t*>(ctx->gi_hist_state.ptr) : nullptr;
+    a.hist_rb = ctx->gi_hist.radiance_blend;
+    a.hist_tb = ctx->gi_hist.transmittance_blend;
+    a.hist_frames = ctx->gi_hist.history_frames;
+    a.hist_persistent = ctx->gi_hist.persistent;
     return a;
 }
 

@@ -71,6 +71,13 @@ struct lux_ctx {
     lux_buffer gi_w_prim, gi_w_shade, gi_w_slots, gi_w_vis, gi_w_shadow, gi_w_counters;
     lux_buffer mesh_info;    // uint32 per mesh: BLAS base | triangle base | count
     hipEvent_t ev_phase[5] = {};
+    // temporal history of raw intervals: on when a blend is positive; sample
+    // counts, two float4 per interval, state words (valid, dirty boxes)
+    bool gi_hist_on = false;
+    lux_gi_history_params gi_hist{};
+    lux_buffer gi_samples, gi_history, gi_hist_state;
+    uint32_t gi_dirty_n = 0;                 // dirty boxes pending on the device
+    float gi_emit_lo[3] = {}, gi_emit_hi[3] = {};  // emitter bounds (world)
 
     // trace tuning
     // traversal launch configuration (defaults: the autotuner's choice on the

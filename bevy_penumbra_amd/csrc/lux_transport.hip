@@ -42,7 +42,9 @@ __global__ __launch_bounds__(kWg) void k_gi_trace(GiArgs a, uint64_t g_begin, ui
     uint32_t k = (uint32_t)(local % D);
     float3 c = probe_centre(p, L, local / D);
     bool terminal = p.terminal_tail != 0u && L + 1 == p.levels;
-    a.intervals[g] = trace_interval(a, L, c, k, terminal, stk);
+    uint32_t ns = 0;
+    a.intervals[g] = trace_interval(a, L, c, k, terminal, stk, &ns);
+    if (a.samples) a.samples[g] = ns;
 }
 
 __global__ __launch_bounds__(kWg) void k_sur_queries(GiArgs a, uint32_t L, uint64_t count, float* out) {
@@ -371,6 +373,7 @@ __global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
     }
     if (w.kind == kWaveRange) {
         a.intervals[g] = out;
+        if (a.samples) a.samples[g] = n;
     } else if (w.kind == kWaveListed) {
         a.merged[g] = merge_leaf(out);
     } else {
@@ -382,6 +385,23 @@ __global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
         rec[9] = ok ? m.x : nan;
         rec[10] = ok ? m.y : nan;
         rec[11] = ok ? m.z : nan;
+    }
+}
+
+// Temporal blend of intervals [0, count) (levels are contiguous).
+__global__ __launch_bounds__(kWg) void k_gi_history(GiArgs a, uint64_t count) {
+    uint64_t g = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (g >= count) return;
+    uint32_t L = 0;
+    while (L + 1 < a.p.levels && g >= a.offset[L + 1]) ++L;
+    history_blend(a, g, L, g - a.offset[L]);
+}
+
+// After a history pass: the history is valid and the dirty boxes consumed.
+__global__ void k_gi_history_done(uint32_t* state) {
+    if (threadIdx.x == 0) {
+        state[0] = 1u;
+        state[1] = 0u;
     }
 }
 
@@ -772,8 +792,28 @@ std::vector<Light> build_lights(const lux_ctx* ctx) {
 int upload_lights(lux_ctx* ctx) {
     std::vector<Light> lights = build_lights(ctx);
     ctx->gi_lights = (uint32_t)lights.size();
+    for (int q = 0; q < 3; ++q) {
+        ctx->gi_emit_lo[q] = INFINITY;
+        ctx->gi_emit_hi[q] = -INFINITY;
+    }
+    for (const Light& l : lights) {
+        const float a[3] = {l.position.x, l.position.y, l.position.z};
+        const float e1[3] = {l.edge1.x, l.edge1.y, l.edge1.z}, e2[3] = {l.edge2.x, l.edge2.y, l.edge2.z};
+        for (int q = 0; q < 3; ++q) {
+            const float v[3] = {a[q], a[q] + e1[q], a[q] + e2[q]};
+            for (float x : v) {
+                ctx->gi_emit_lo[q] = std::min(ctx->gi_emit_lo[q], x);
+                ctx->gi_emit_hi[q] = std::max(ctx->gi_emit_hi[q], x);
+            }
+        }
+    }
     if (int rc = lux_ensure(ctx, ctx->gi_light_buf, sizeof(Light) * (lights.size() + 1))) return rc;
     LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->gi_hist_state.ptr) {
+        float em[6] = {ctx->gi_emit_lo[0], ctx->gi_emit_lo[1], ctx->gi_emit_lo[2],
+                       ctx->gi_emit_hi[0], ctx->gi_emit_hi[1], ctx->gi_emit_hi[2]};
+        LUX_HIP(ctx, hipMemcpy(static_cast<char*>(ctx->gi_hist_state.ptr) + 8, em, 24, hipMemcpyHostToDevice));
+    }
     if (!lights.empty())
         LUX_HIP(ctx, hipMemcpy(ctx->gi_light_buf.ptr, lights.data(), sizeof(Light) * lights.size(),
                                hipMemcpyHostToDevice));
@@ -792,6 +832,10 @@ int gi_check(lux_ctx* ctx) {
 }  // namespace
 
 int lux_gi_check_internal(lux_ctx* ctx) { return gi_check(ctx); }
+
+namespace {
+int history_alloc(lux_ctx* ctx);
+}
 
 extern "C" int lux_gi_configure(lux_ctx* ctx, const lux_gi_params* params) {
     if (!ctx) return LUX_ERR_INVALID;
@@ -830,7 +874,8 @@ extern "C" int lux_gi_configure(lux_ctx* ctx, const lux_gi_params* params) {
     ctx->gi = p;
     ctx->gi_ready = true;
     ctx->gi_sur_probes = 0;
-    return LUX_OK;
+    // a new grid starts without history
+    return history_alloc(ctx);
 }
 
 extern "C" int lux_gi_set_frame(lux_ctx* ctx, uint32_t frame, uint32_t seed) {
@@ -1103,12 +1148,87 @@ extern "C" int lux_gi_query_counters(lux_ctx* ctx, uint32_t* attempts, uint32_t*
     return LUX_OK;
 }
 
+namespace {
+// Temporal history pass over the first `count` intervals (the traced ones).
+int history_pass(lux_ctx* ctx, uint64_t count) {
+    if (!ctx->gi_hist_on || count == 0) return LUX_OK;
+    GiArgs a = gi_args(ctx);
+    hipLaunchKernelGGL(k_gi_history, dim3((uint32_t)((count + kWg - 1) / kWg)), dim3(kWg), 0, ctx->stream, a, count);
+    hipLaunchKernelGGL(k_gi_history_done, dim3(1), dim3(64), 0, ctx->stream, a.hist_state);
+    LUX_HIP(ctx, hipGetLastError());
+    ctx->gi_dirty_n = 0;
+    return LUX_OK;
+}
+
+int history_alloc(lux_ctx* ctx) {
+    if (!ctx->gi_hist_on || !ctx->gi_ready) return LUX_OK;
+    if (int rc = lux_ensure(ctx, ctx->gi_samples, 4ull * ctx->gi_total)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->gi_history, 32ull * ctx->gi_total)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->gi_hist_state, 32ull + 32ull * kMaxDirtyBoxes)) return rc;
+    ctx->gi_dirty_n = 0;
+    uint32_t st[8] = {0u, 0u};
+    std::memcpy(st + 2, ctx->gi_emit_lo, 12);
+    std::memcpy(st + 5, ctx->gi_emit_hi, 12);
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LUX_HIP(ctx, hipMemcpy(ctx->gi_hist_state.ptr, st, 32, hipMemcpyHostToDevice));
+    return LUX_OK;
+}
+}  // namespace
+
+extern "C" int lux_gi_set_history(lux_ctx* ctx, const lux_gi_history_params* hp) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (hp && !(std::isfinite(hp->radiance_blend) && std::isfinite(hp->transmittance_blend)))
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_set_history: blends must be finite");
+    ctx->gi_hist_on = hp && (hp->radiance_blend > 0.0f || hp->transmittance_blend > 0.0f);
+    if (hp) ctx->gi_hist = *hp;
+    else std::memset(&ctx->gi_hist, 0, sizeof ctx->gi_hist);
+    return history_alloc(ctx);
+}
+
+extern "C" int lux_gi_reset_history(lux_ctx* ctx) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!ctx->gi_hist_on || !ctx->gi_hist_state.ptr) return LUX_OK;
+    ctx->gi_dirty_n = 0;
+    return lux_device_zero(ctx, ctx->gi_hist_state.ptr, 8);
+}
+
+extern "C" int lux_gi_invalidate_history(lux_ctx* ctx, const float* lo, const float* hi) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!lo || !hi) return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_invalidate_history: null box");
+    if (!ctx->gi_hist_on || !ctx->gi_hist_state.ptr) return LUX_OK;
+    bool empty = false, emitters = ctx->gi_lights > 0;
+    for (int q = 0; q < 3; ++q) {
+        if (!(lo[q] <= hi[q])) empty = true;
+        if (!(lo[q] <= ctx->gi_emit_hi[q] && hi[q] >= ctx->gi_emit_lo[q])) emitters = false;
+    }
+    if (empty) return LUX_OK;
+    // a change that may touch an emitter reaches every interval through its
+    // NEE occlusion rays; past the box capacity the whole history goes too
+    if (emitters || ctx->gi_dirty_n >= kMaxDirtyBoxes) return lux_gi_reset_history(ctx);
+    float box[8] = {lo[0], lo[1], lo[2], 0.0f, hi[0], hi[1], hi[2], 0.0f};
+    uint32_t n = ctx->gi_dirty_n + 1;
+    char* st = static_cast<char*>(ctx->gi_hist_state.ptr);
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LUX_HIP(ctx, hipMemcpy(st + 32 + 32ull * ctx->gi_dirty_n, box, 32, hipMemcpyHostToDevice));
+    LUX_HIP(ctx, hipMemcpy(st + 4, &n, 4, hipMemcpyHostToDevice));
+    ctx->gi_dirty_n = n;
+    return LUX_OK;
+}
+
+extern "C" int lux_gi_history_arrays(lux_ctx* ctx, float** history, uint32_t** samples) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (history) *history = ctx->gi_hist_on ? static_cast<float*>(ctx->gi_history.ptr) : nullptr;
+    if (samples) *samples = ctx->gi_hist_on ? static_cast<uint32_t*>(ctx->gi_samples.ptr) : nullptr;
+    return LUX_OK;
+}
+
 extern "C" int lux_gi_update(lux_ctx* ctx, float* ms) {
     if (!ctx) return LUX_ERR_INVALID;
     if (int rc = gi_check(ctx)) return rc;
     LUX_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (int rc = lux_device_zero(ctx, ctx->gi_counters.ptr, 8)) return rc;
     if (int rc = lux_gi_trace(ctx)) return rc;
+    if (int rc = history_pass(ctx, ctx->gi_total)) return rc;
     if (int rc = lux_gi_merge(ctx)) return rc;
     if (int rc = lux_gi_pack(ctx)) return rc;
     LUX_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -1300,6 +1420,7 @@ extern "C" int lux_gi_update_surrogate(lux_ctx* ctx, lux_surrogate* s, uint32_t 
                            static_cast<const uint32_t*>(fcount));
         LUX_HIP(ctx, hipGetLastError());
     }
+    if (int rc = history_pass(ctx, ctx->gi_offset[level])) return rc;
     LUX_HIP(ctx, hipEventRecord(ctx->ev_phase[3], st));
     if (int rc = merge_levels(ctx, (int)level - 1)) return rc;
     if (int rc = lux_gi_pack(ctx)) return rc;

@@ -280,4 +280,14 @@ int lux_radiance_cascades_gi_params(const lux_radiance_cascades* in, const float
     return LUX_OK;
 }
 
+int lux_radiance_cascades_history_params(const lux_radiance_cascades* in, lux_gi_history_params* out) {
+    if (!in || !out) return LUX_ERR_INVALID;
+    bevy_lux::RadianceCascades r = from_c(in).sanitized();
+    out->radiance_blend = r.temporal_blend;
+    out->transmittance_blend = r.split_transmittance_blend;
+    out->history_frames = r.history_frames;
+    out->persistent = 1u;
+    return LUX_OK;
+}
+
 }  // extern "C"

@@ -155,6 +155,12 @@ class SurrogateStats(C.Structure):
                 for k, _ in self._fields_ if k != "reserved"}
 
 
+class HistoryParams(C.Structure):
+    """lux_gi_history_params (include/lux.h)."""
+    _fields_ = [("radiance_blend", C.c_float), ("transmittance_blend", C.c_float),
+                ("history_frames", C.c_uint32), ("persistent", C.c_uint32)]
+
+
 class LodView(C.Structure):
     """lux_lod_view (include/lux.h): camera terms of the split LOD choice."""
     _fields_ = [("camera_position", C.c_float * 3), ("probe_spacing", C.c_float),
@@ -289,6 +295,11 @@ def load_library() -> C.CDLL:
         "lux_gi_update_surrogate": (i32, [vp, vp, u32, C.c_float, vp]),
         "lux_gi_surrogate_probe_error": (i32, [vp, vp, u32]),
         "lux_world_resolve_lods": (i32, [vp, u32, vp, vp, u32, vp]),
+        "lux_gi_set_history": (i32, [vp, vp]),
+        "lux_gi_reset_history": (i32, [vp]),
+        "lux_gi_invalidate_history": (i32, [vp, vp, vp]),
+        "lux_gi_history_arrays": (i32, [vp, vp, vp]),
+        "lux_radiance_cascades_history_params": (i32, [vp, vp]),
         "lux_batch_create": (i32, [vp, u32, C.POINTER(vp)]),
         "lux_batch_destroy": (None, [vp]),
         "lux_batch_last_error": (C.c_char_p, [vp]),
@@ -660,6 +671,36 @@ class Context:
                                                      C.byref(st) if stats else None),
                     "lux_gi_update_surrogate")
         return st if stats else None
+
+    def gi_set_history(self, radiance_blend: float = 0.88, transmittance_blend: float = 0.98,
+                       history_frames: int = 24, persistent: int = 1):
+        hp = HistoryParams(radiance_blend, transmittance_blend, history_frames, persistent)
+        self._check(self.lib.lux_gi_set_history(self.h, C.byref(hp)), "lux_gi_set_history")
+
+    def gi_history_off(self):
+        self._check(self.lib.lux_gi_set_history(self.h, None), "lux_gi_set_history")
+
+    def gi_reset_history(self):
+        self._check(self.lib.lux_gi_reset_history(self.h), "lux_gi_reset_history")
+
+    def gi_invalidate_history(self, lo, hi):
+        a = (C.c_float * 3)(*[float(x) for x in lo])
+        b = (C.c_float * 3)(*[float(x) for x in hi])
+        self._check(self.lib.lux_gi_invalidate_history(self.h, a, b), "lux_gi_invalidate_history")
+
+    def gi_history_download(self):
+        """(history (n, 8) float32, samples (n,) uint32) of the last update."""
+        hp, sp = C.c_void_p(), C.c_void_p()
+        self._check(self.lib.lux_gi_history_arrays(self.h, C.byref(hp), C.byref(sp)), "lux_gi_history_arrays")
+        if not hp.value:
+            return None, None
+        n = self.gi_sizes()[0]
+        self.synchronize()
+        hist = np.zeros((n, 8), np.float32)
+        ns = np.zeros(n, np.uint32)
+        self._check(self.lib.lux_copy_to_host(self.h, hist.ctypes.data, hp.value, hist.nbytes), "lux_copy_to_host")
+        self._check(self.lib.lux_copy_to_host(self.h, ns.ctypes.data, sp.value, ns.nbytes), "lux_copy_to_host")
+        return hist, ns
 
     def gi_surrogate_probe_error(self, probes: int) -> np.ndarray:
         out = np.zeros(probes, np.float32)

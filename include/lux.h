@@ -286,6 +286,38 @@ typedef struct {
 int lux_world_resolve_lods(lux_ctx* const* lods, uint32_t lod_count, const lux_lod_view* view,
                            const float* receivers, uint32_t n, float* out);
 
+/* Temporal history of the raw intervals (mergeEntry's history part,
+ * shaders/split.slang:1624-1705, mergeRawRadianceGroup :1478-1521): each
+ * update blends every traced interval's (J, beta) with its history by
+ * sample masses, mass limits 1 / (1 - blend), and merges with the blended
+ * values.  radiance_blend = RadianceCascades::temporal_blend (history.x),
+ * transmittance_blend = split_transmittance_blend (intervalHistory.x),
+ * history_frames (history.w; 0: nothing is reused), persistent (world
+ * intervals never expire; intervalHistory.y, set by the reference's renderer;
+ * otherwise an interval last sampled more than history_frames frames ago is
+ * not reused).  16 bytes. */
+typedef struct {
+    float radiance_blend;
+    float transmittance_blend;
+    uint32_t history_frames;
+    uint32_t persistent;
+} lux_gi_history_params;
+/* NULL or both blends 0: history off (the default).  Turning it on, and
+ * every lux_gi_configure, starts from an empty history. */
+int lux_gi_set_history(lux_ctx* ctx, const lux_gi_history_params* params);
+int lux_gi_reset_history(lux_ctx* ctx);
+/* Dirty region (world AABB): the next update reuses no history of an
+ * interval whose dependency box meets it -- the box of probe centre +-
+ * interval end joined with the emitter bounds (every terminal interval).  A
+ * region touching an emitter, or more than 16 pending regions, drops the
+ * whole history.  lux_scene_set_transforms marks the old and new boxes of
+ * every moved instance (more than 8 moved instances: the whole history). */
+int lux_gi_invalidate_history(lux_ctx* ctx, const float* lo, const float* hi);
+/* Device addresses of the history (8 floats per interval: J.rgb, radiance
+ * mass, beta, transmittance mass, last sampled frame bits, 0) and of the
+ * supported-ray counts of the last update; NULL when history is off. */
+int lux_gi_history_arrays(lux_ctx* ctx, float** history, uint32_t** samples);
+
 /* Traversal query counters of the current update: attempts and admitted
  * (admitted <= the budget; both count every interval and NEE query).
  * lux_gi_update and lux_gi_update_surrogate reset them at their start, as

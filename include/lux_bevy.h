@@ -54,6 +54,11 @@ int lux_radiance_cascades_gi_params(const lux_radiance_cascades* in, const float
                                     const uint32_t dims[3], uint32_t cascades, uint32_t base_phi,
                                     uint32_t base_y, uint32_t interval_mode, uint32_t frame,
                                     lux_gi_params* out);
+/* The interval history of a (sanitized) camera component:
+ * temporal_blend -> radiance_blend, split_transmittance_blend ->
+ * transmittance_blend, history_frames, persistent world intervals (the
+ * renderer's intervalHistory.y = 1, src/render.rs:8455-8462). */
+int lux_radiance_cascades_history_params(const lux_radiance_cascades* in, lux_gi_history_params* out);
 
 #ifdef __cplusplus
 }

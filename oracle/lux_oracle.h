@@ -204,6 +204,18 @@ void orc_gi_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights
                   const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                   float* intervals, int threads);
 void orc_gi_merge(const orc_gi_params* p, const float* intervals, float* merged);
+/* Temporal history (mergeEntry's history part): settings and state. */
+typedef struct {
+    float radiance_blend, transmittance_blend;
+    uint32_t history_frames, persistent;
+    uint32_t valid;  /* a previous update wrote the history */
+} orc_history_params;
+void orc_gi_trace_samples(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
+                          const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
+                          float* intervals, uint32_t* samples, int threads);
+void orc_gi_history(const orc_gi_params* p, const orc_history_params* h, const float* emit, const float* boxes,
+                    uint32_t n_boxes, uint64_t count, float* intervals, const uint32_t* samples, float* history);
+void orc_gi_merge_history(const orc_gi_params* p, const float* intervals, const float* history, float* merged);
 void orc_gi_merge_f64(const orc_gi_params* p, const double* intervals, double* merged);
 /* 36 octahedral texels (irradiance/pi rgb, coverage) per level-0 probe. */
 void orc_gi_pack(const orc_gi_params* p, const float* merged, float* probes);

@@ -387,18 +387,131 @@ void orc_lod_select(const orc_lod_view* v, const float* position, uint32_t* low,
 void orc_gi_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                   const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                   float* intervals, int threads) {
-    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, threads);
+    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads);
 }
 void orc_gi_trace_f64(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                       const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                       double* intervals, int threads) {
-    gi_trace_f64(s, lights, n_lights, mats, n_mats, p, intervals, threads);
+    gi_trace_f64(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads);
 }
+void orc_gi_trace_samples(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
+                          const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
+                          float* intervals, uint32_t* samples, int threads) {
+    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, samples, threads);
+}
+void orc_gi_merge_history(const orc_gi_params* p, const float* intervals, const float* history, float* merged) {
+    gi_merge_f32(p, intervals, history, merged);
+}
+
+static inline float hsat(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
+static inline float hmin(float a, float b) { return b < a ? b : a; }
+static inline float hmax(float a, float b) { return b > a ? b : a; }
+static inline float hmass(float m) { return (finitef_(m) && m >= 0.0f) ? m : 0.0f; }
+
+/* Temporal history of intervals [0, count) (mergeEntry's history part,
+ * split.slang:1624-1705; mergeRawRadianceGroup :1478-1521): reuse when the
+ * history is valid, alive (persistent, or sampled within history_frames
+ * frames, common.slang:85-89) and the dependency box (probe centre +-
+ * interval end joined with the emitter bounds; unbounded when terminal)
+ * meets no dirty box; per channel only when that channel's blend is
+ * positive.  Masses: retention = blend^n, new mass (1 - retention) /
+ * (1 - blend), old mass min(old, limit) * retention, sample-mass weighted
+ * means; an update with no sample keeps the old value and mass.  pow(blend,
+ * n) for an integer n is a product of n factors.  history: 8 floats per
+ * interval (J.rgb, radiance mass, beta, transmittance mass, last sampled
+ * frame bits, 0).  emit: lo.xyz, hi.xyz; boxes: lo.xyz, hi.xyz each. */
+void orc_gi_history(const orc_gi_params* p, const orc_history_params* h, const float* emit, const float* boxes,
+                    uint32_t n_boxes, uint64_t count, float* intervals, const uint32_t* samples, float* history) {
+    uint64_t off[17];
+    for (uint32_t l = 0; l <= p->levels; ++l) off[l] = orc_gi_level_offset(p, l);
+    const float rb = hmin(hsat(h->radiance_blend), 0.999f), tb = hmin(hsat(h->transmittance_blend), 0.999f);
+    const float rlim = 1.0f / (1.0f - rb), tlim = 1.0f / (1.0f - tb);
+    for (uint64_t g = 0; g < count; ++g) {
+        uint32_t L = 0;
+        while (L + 1 < p->levels && g >= off[L + 1]) ++L;
+        float* iv = intervals + 4 * g;
+        float* hs = history + 8 * g;
+        uint32_t n = samples[g];
+        uint32_t last;
+        memcpy(&last, hs + 6, 4);
+        int reuse = h->valid && h->history_frames > 0;
+        if (reuse) reuse = h->persistent || p->frame - last <= h->history_frames;
+        if (reuse && n_boxes) {
+            uint32_t dims[3];
+            orc_gi_level_dims(p, L, dims);
+            uint32_t D = orc_gi_level_directions(p, L);
+            uint64_t probe = (g - off[L]) / D;
+            uint32_t ix = (uint32_t)(probe % dims[0]), iy = (uint32_t)((probe / dims[0]) % dims[1]);
+            uint32_t iz = (uint32_t)(probe / ((uint64_t)dims[0] * dims[1]));
+            float sl = p->spacing * ldexpf(1.0f, (int)L);
+            float c[3] = {p->origin[0] + ((float)ix + 0.5f) * sl, p->origin[1] + ((float)iy + 0.5f) * sl,
+                          p->origin[2] + ((float)iz + 0.5f) * sl};
+            float start, e;
+            orc_gi_interval_bounds(p, L, &start, &e);
+            int terminal = p->terminal_tail && L + 1 == p->levels;
+            float dlo[3], dhi[3];
+            for (int q = 0; q < 3; ++q) {
+                dlo[q] = hmin(c[q] - e, emit[q]);
+                dhi[q] = hmax(c[q] + e, emit[3 + q]);
+            }
+            for (uint32_t b = 0; b < n_boxes && reuse; ++b) {
+                const float* bx = boxes + 6 * b;
+                int hit = terminal || (dlo[0] <= bx[3] && dhi[0] >= bx[0] && dlo[1] <= bx[4] && dhi[1] >= bx[1] &&
+                                       dlo[2] <= bx[5] && dhi[2] >= bx[2]);
+                if (hit) reuse = 0;
+            }
+        }
+        int reuse_r = reuse && h->radiance_blend > 0.0f, reuse_t = reuse && h->transmittance_blend > 0.0f;
+        float om = hmass(hs[3]), otm = hmass(hs[5]);
+        float J[3] = {iv[0], iv[1], iv[2]}, beta = iv[3], mass, ts;
+        if (n > 0) {
+            float ret = 1.0f;
+            for (uint32_t i = 0; i < n; ++i) ret = ret * rb;
+            mass = (1.0f - ret) * rlim;
+            float o = hmin(om, rlim) * ret;
+            if (reuse_r && o > 0.0f) {
+                float t = mass + o;
+                for (int q = 0; q < 3; ++q) J[q] = (J[q] * mass + hs[q] * o) / t;
+                mass = mass + o;
+            }
+            float tret = 1.0f;
+            for (uint32_t i = 0; i < n; ++i) tret = tret * tb;
+            ts = (1.0f - tret) * tlim;
+            float ot = hmin(otm, tlim) * tret;
+            if (reuse_t && ot > 0.0f) {
+                beta = (beta * ts + hs[4] * ot) / (ts + ot);
+                ts = ts + ot;
+            }
+            last = p->frame;
+        } else {
+            if (reuse_r && om > 0.0f) {
+                J[0] = hs[0]; J[1] = hs[1]; J[2] = hs[2];
+                mass = om;
+            } else {
+                J[0] = J[1] = J[2] = 0.0f;
+                mass = 0.0f;
+            }
+            if (reuse_t && otm > 0.0f) {
+                beta = hs[4];
+                ts = otm;
+            } else {
+                beta = 1.0f;
+                ts = 0.0f;
+            }
+        }
+        iv[0] = J[0]; iv[1] = J[1]; iv[2] = J[2]; iv[3] = beta;
+        hs[0] = J[0]; hs[1] = J[1]; hs[2] = J[2]; hs[3] = mass;
+        hs[4] = beta; hs[5] = ts;
+        memcpy(hs + 6, &last, 4);
+        hs[7] = 0.0f;
+    }
+}
+
 void orc_gi_merge(const orc_gi_params* p, const float* intervals, float* merged) {
-    gi_merge_f32(p, intervals, merged);
+    gi_merge_f32(p, intervals, NULL, merged);
 }
 void orc_gi_merge_f64(const orc_gi_params* p, const double* intervals, double* merged) {
-    gi_merge_f64(p, intervals, merged);
+    gi_merge_f64(p, intervals, NULL, merged);
 }
 void orc_gi_pack(const orc_gi_params* p, const float* merged, float* probes) {
     gi_pack_f32(p, merged, probes);

@@ -185,6 +185,7 @@ typedef struct {
     uint32_t nm;
     const orc_gi_params* p;
     REAL* intervals;
+    uint32_t* samples; /* nullable: supported rays per interval */
     uint64_t begin, end;
 } SFX(trace_job);
 
@@ -246,6 +247,7 @@ static void* SFX(trace_worker)(void* arg) {
             bsum = bsum + beta;
             n++;
         }
+        if (j->samples) j->samples[g] = n;
         REAL* out = j->intervals + 4 * g;
         if (n == 0) {
             out[0] = out[1] = out[2] = K(0.0);
@@ -261,7 +263,7 @@ static void* SFX(trace_worker)(void* arg) {
 
 static void SFX(gi_trace)(const orc_scene* s, const orc_light* lights, uint32_t nl,
                           const orc_material* mats, uint32_t nm, const orc_gi_params* p,
-                          REAL* intervals, int threads) {
+                          REAL* intervals, uint32_t* samples, int threads) {
     uint64_t total = orc_gi_level_offset(p, p->levels);
     if (threads < 1) threads = 1;
     if (threads > 64) threads = 64;
@@ -270,7 +272,7 @@ static void SFX(gi_trace)(const orc_scene* s, const orc_light* lights, uint32_t 
     uint64_t per = (total + (uint64_t)threads - 1) / (uint64_t)threads;
     for (int t = 0; t < threads; ++t) {
         jobs[t].s = s; jobs[t].lights = lights; jobs[t].nl = nl; jobs[t].mats = mats;
-        jobs[t].nm = nm; jobs[t].p = p; jobs[t].intervals = intervals;
+        jobs[t].nm = nm; jobs[t].p = p; jobs[t].intervals = intervals; jobs[t].samples = samples;
         jobs[t].begin = per * (uint64_t)t < total ? per * (uint64_t)t : total;
         jobs[t].end = jobs[t].begin + per < total ? jobs[t].begin + per : total;
     }
@@ -325,7 +327,7 @@ static void SFX(interp_child)(const orc_gi_params* p, const REAL* merged_c, uint
 }
 
 /* mergeEntry for one history-free update, top level down. */
-static void SFX(gi_merge)(const orc_gi_params* p, const REAL* intervals, REAL* merged) {
+static void SFX(gi_merge)(const orc_gi_params* p, const REAL* intervals, const float* history, REAL* merged) {
     for (int Li = (int)p->levels - 1; Li >= 0; --Li) {
         uint32_t L = (uint32_t)Li;
         uint32_t dims[3];
@@ -339,14 +341,27 @@ static void SFX(gi_merge)(const orc_gi_params* p, const REAL* intervals, REAL* m
             uint32_t ix = probe % dims[0], iy = (probe / dims[0]) % dims[1], iz = probe / (dims[0] * dims[1]);
             float rel[3] = {((float)ix + 0.5f) * sl, ((float)iy + 0.5f) * sl, ((float)iz + 0.5f) * sl};
             for (uint32_t k = 0; k < D; ++k) {
-                const REAL* iv = intervals + 4 * (off + (uint64_t)probe * D + k);
-                REAL* out = merged + 4 * (off + (uint64_t)probe * D + k);
-                if (!(iv[3] >= K(0.0))) { /* no supported sample */
+                uint64_t g = off + (uint64_t)probe * D + k;
+                const REAL* iv = intervals + 4 * g;
+                REAL* out = merged + 4 * g;
+                /* own radiance needs both masses, children only transmittance
+                 * (beginMergeGroup, split.slang:1533-1545); with no history
+                 * both masses are 1 for a sampled interval, 0 otherwise */
+                int has;
+                REAL tr;
+                if (history) {
+                    float rm = history[8 * g + 3], tm = history[8 * g + 5];
+                    tr = tm > 0.0f ? SFX(sat)(iv[3]) : K(0.0);
+                    has = rm > 0.0f && tm > 0.0f;
+                } else {
+                    has = iv[3] >= K(0.0);
+                    tr = has ? SFX(sat)(iv[3]) : K(0.0);
+                }
+                if (!has && !(tr > K(0.0))) {
                     out[0] = out[1] = out[2] = out[3] = K(0.0);
                     continue;
                 }
-                REAL tr = SFX(sat)(iv[3]);
-                REAL support = K(1.0) - tr;
+                REAL support = has ? K(1.0) - tr : K(0.0);
                 REAL rad[3] = {K(0.0), K(0.0), K(0.0)};
                 if (support > K(0.0)) { rad[0] = iv[0]; rad[1] = iv[1]; rad[2] = iv[2]; }
                 REAL csum[3] = {K(0.0), K(0.0), K(0.0)}, csup = K(0.0);
