@@ -65,14 +65,15 @@ __global__ void k_tri_boxes(const float* pos, uint32_t n, float4* lo, float4* hi
 
 // Gather triangles into BLAS leaf order.
 __global__ void k_tri_gather(const float* pos, const uint32_t* mat, const uint32_t* perm, uint32_t n,
-                             float4* A, float4* B, float4* C, uint32_t* order, uint32_t tri_base) {
+                             float4* tri, uint32_t* order, uint32_t tri_base) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     uint32_t s = perm[j];
     const float* p = pos + 9ull * s;
-    A[j] = make_float4(p[0], p[1], p[2], __uint_as_float(s));
-    B[j] = make_float4(p[3], p[4], p[5], __uint_as_float(mat ? mat[s] : 0u));
-    C[j] = make_float4(p[6], p[7], p[8], 0.0f);
+    float4* t = tri + 3ull * j;
+    t[0] = make_float4(p[0], p[1], p[2], __uint_as_float(s));
+    t[1] = make_float4(p[3], p[4], p[5], __uint_as_float(mat ? mat[s] : 0u));
+    t[2] = make_float4(p[6], p[7], p[8], 0.0f);
     order[j] = tri_base + s;
 }
 
@@ -171,7 +172,7 @@ extern "C" void lux_ctx_destroy(lux_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     lux_buffer* bufs[] = {&ctx->raw_pos, &ctx->raw_mat, &ctx->inst_desc, &ctx->materials,
-                          &ctx->triA, &ctx->triB, &ctx->triC, &ctx->tri_order, &ctx->blas,
+                          &ctx->tris, &ctx->tri_order, &ctx->blas,
                           &ctx->tlas, &ctx->inst, &ctx->inst_order, &ctx->scratch,
                           &ctx->ray_buf, &ctx->hit_buf, &ctx->gi_intervals, &ctx->gi_merged,
                           &ctx->gi_probes, &ctx->gi_light_buf, &ctx->gi_sur_queries,
@@ -351,9 +352,7 @@ extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_wo
 static int build_blas(lux_ctx* ctx) {
     hipStream_t st = ctx->stream;
     uint32_t nt = ctx->n_tri;
-    if (int rc = lux_ensure(ctx, ctx->triA, 16ull * nt)) return rc;
-    if (int rc = lux_ensure(ctx, ctx->triB, 16ull * nt)) return rc;
-    if (int rc = lux_ensure(ctx, ctx->triC, 16ull * nt)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->tris, 48ull * nt)) return rc;
     if (int rc = lux_ensure(ctx, ctx->tri_order, 4ull * nt)) return rc;
     if (int rc = lux_ensure(ctx, ctx->blas, 32ull * ctx->n_blas)) return rc;
     // per-mesh info for the TLAS build (BLAS base | triangle base | count)
@@ -382,9 +381,7 @@ static int build_blas(lux_ctx* ctx) {
         rc = lux_lbvh_build_device(ctx, lo, hi, n, 4u, nodes, perm, bb, tb);
         if (rc) break;
         hipLaunchKernelGGL(k_tri_gather, dim3(nblk(n, 256)), dim3(256), 0, st, pos, mat, perm, n,
-                           static_cast<float4*>(ctx->triA.ptr) + tb,
-                           static_cast<float4*>(ctx->triB.ptr) + tb,
-                           static_cast<float4*>(ctx->triC.ptr) + tb,
+                           static_cast<float4*>(ctx->tris.ptr) + 3ull * tb,
                            static_cast<uint32_t*>(ctx->tri_order.ptr) + tb, tb);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) rc = lux_hip_check(ctx, e, "build_blas launch");
@@ -509,16 +506,12 @@ extern "C" int lux_scene_upload_built(lux_ctx* ctx, const lux_bvh_node* blas, ui
     if (int rc = lux_ensure(ctx, ctx->blas, 32ull * blas_count)) return rc;
     if (int rc = lux_ensure(ctx, ctx->tlas, 32ull * tlas_count)) return rc;
     if (int rc = lux_ensure(ctx, ctx->inst, 128ull * instance_count)) return rc;
-    if (int rc = lux_ensure(ctx, ctx->triA, 16ull * triangle_count)) return rc;
-    if (int rc = lux_ensure(ctx, ctx->triB, 16ull * triangle_count)) return rc;
-    if (int rc = lux_ensure(ctx, ctx->triC, 16ull * triangle_count)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->tris, 48ull * triangle_count)) return rc;
     LUX_HIP(ctx, hipMemcpy(ctx->blas.ptr, blas, 32ull * blas_count, hipMemcpyHostToDevice));
     LUX_HIP(ctx, hipMemcpy(ctx->tlas.ptr, tlas, 32ull * tlas_count, hipMemcpyHostToDevice));
     LUX_HIP(ctx, hipMemcpy(ctx->inst.ptr, instances, 128ull * instance_count, hipMemcpyHostToDevice));
-    // 12 floats per triangle -> three float4 arrays
-    LUX_HIP(ctx, hipMemcpy2D(ctx->triA.ptr, 16, triangles, 48, 16, triangle_count, hipMemcpyHostToDevice));
-    LUX_HIP(ctx, hipMemcpy2D(ctx->triB.ptr, 16, triangles + 4, 48, 16, triangle_count, hipMemcpyHostToDevice));
-    LUX_HIP(ctx, hipMemcpy2D(ctx->triC.ptr, 16, triangles + 8, 48, 16, triangle_count, hipMemcpyHostToDevice));
+    // 12 floats per triangle: the device layout as given
+    LUX_HIP(ctx, hipMemcpy(ctx->tris.ptr, triangles, 48ull * triangle_count, hipMemcpyHostToDevice));
     ctx->n_blas = blas_count;
     ctx->n_tlas = tlas_count;
     ctx->n_inst = instance_count;

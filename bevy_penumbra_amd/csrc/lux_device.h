@@ -36,9 +36,7 @@ struct DevScene {
     const float4* tlas;
     const float4* inst;
     const float4* blas;
-    const float4* triA;
-    const float4* triB;
-    const float4* triC;
+    const float4* tri;  // 3 per triangle, interleaved: A (w = index in mesh), B (w = material), C
     uint32_t n_tlas, n_inst, n_blas, n_tri;
 };
 
@@ -322,7 +320,8 @@ __device__ __forceinline__ void trace_instance(const DevScene& s, float3 o, floa
         if (truncated) h.err |= kErrUnsupported;
         for (uint32_t i = tbegin; i < bend; ++i) {
             uint32_t ti = i < s.n_tri ? i : s.n_tri - 1u;
-            float4 A = s.triA[ti], B = s.triB[ti], C = s.triC[ti];
+            const float4* tp = s.tri + 3ull * ti;
+            float4 A = tp[0], B = tp[1], C = tp[2];
             st.tri();
             float3 a = make_float3(A.x, A.y, A.z), b = make_float3(B.x, B.y, B.z),
                    c = make_float3(C.x, C.y, C.z);
@@ -504,7 +503,8 @@ __device__ __forceinline__ bool occluded_instance(const DevScene& s, float3 o, f
         incomplete = incomplete || truncated;
         for (uint32_t i = tbegin; i < bend; ++i) {
             uint32_t ti = i < s.n_tri ? i : s.n_tri - 1u;
-            float4 A = s.triA[ti], B = s.triB[ti], C = s.triC[ti];
+            const float4* tp = s.tri + 3ull * ti;
+            float4 A = tp[0], B = tp[1], C = tp[2];
             st.tri();
             float3 a = make_float3(A.x, A.y, A.z), b = make_float3(B.x, B.y, B.z),
                    c = make_float3(C.x, C.y, C.z);
@@ -621,7 +621,7 @@ __device__ __forceinline__ void trace_occluded(const DevScene& s, float3 o, floa
 // results are bit-identical; only the interleaving across lanes changes,
 // which lets a wave retire finished rays and refill their lanes.
 constexpr uint32_t kWalkClosest = 0u, kWalkFirst = 1u, kWalkOccluded = 2u;
-constexpr uint32_t kPhaseDone = 0u, kPhaseTlas = 1u, kPhaseLeaf = 2u, kPhaseBlas = 3u;
+constexpr uint32_t kPhaseDone = 0u, kPhaseTlas = 1u, kPhaseLeaf = 2u, kPhaseBlas = 3u, kPhaseTri = 4u;
 
 struct Walk {
     const DevScene* sp;      // the lane's scene (multi-scene launches)
@@ -633,6 +633,7 @@ struct Walk {
     bool tready, incomplete;
     Node node;               // current TLAS node, then the current BLAS node
     uint32_t ii, flags, gfirst, gcount, bcount, bsteps;
+    uint32_t ti_next, ti_end;  // BLAS leaf: next triangle, end (one triangle per step)
     bool bready, binc;
     float3 lo, ld;
     TriRay ray;
@@ -850,38 +851,49 @@ __device__ __forceinline__ void walk_blas(const DevScene& s, Walk& w, Stack& stk
     truncated = truncated || bend < tend;
     if (occl) w.binc = w.binc || truncated;
     else if (truncated) w.h.err |= kErrUnsupported;
-    for (uint32_t i = tbegin; i < bend; ++i) {
-        uint32_t ti = i < s.n_tri ? i : s.n_tri - 1u;
-        float4 A = s.triA[ti], B = s.triB[ti], C = s.triC[ti];
-        st.tri();
-        float3 a = make_float3(A.x, A.y, A.z), b = make_float3(B.x, B.y, B.z),
-               c = make_float3(C.x, C.y, C.z);
-        if (culled(a, b, c, w.ld, w.flags)) continue;
-        float3 r3 = intersect_tri(w.lo, w.ld, w.ray, a, b, c);
-        float lim = occl ? w.limit : w.h.t;
-        if (r3.x > 0.000025f && r3.x < lim) {
-            w.h.t = r3.x;
-            w.h.tri = i;
-            w.h.inst = w.ii;
-            w.h.b1 = r3.y;
-            w.h.b2 = r3.z;
-            if (occl) {
-                w.h.err |= kHitOccluded;
-                w.phase = kPhaseDone;
-                return;
-            }
-        }
-        if (w.mode == kWalkFirst && w.h.tri != kEmpty) {
-            walk_instance_end(w, true);
+    // the leaf's triangles follow, one per step, before the BLAS loop resumes
+    if (tbegin < bend) {
+        w.ti_next = tbegin;
+        w.ti_end = bend;
+        w.phase = kPhaseTri;
+    }
+}
+
+// One triangle of the current BLAS leaf (the leaf loop of trace_instance /
+// occluded_instance, in order).
+template <class St>
+__device__ __forceinline__ void walk_tri(const DevScene& s, Walk& w, St& st) {
+    const bool occl = w.mode == kWalkOccluded;
+    const uint32_t i = w.ti_next++;
+    if (w.ti_next >= w.ti_end) w.phase = kPhaseBlas;
+    uint32_t ti = i < s.n_tri ? i : s.n_tri - 1u;
+    const float4* tp = s.tri + 3ull * ti;
+            float4 A = tp[0], B = tp[1], C = tp[2];
+    st.tri();
+    float3 a = make_float3(A.x, A.y, A.z), b = make_float3(B.x, B.y, B.z), c = make_float3(C.x, C.y, C.z);
+    if (culled(a, b, c, w.ld, w.flags)) return;
+    float3 r3 = intersect_tri(w.lo, w.ld, w.ray, a, b, c);
+    float lim = occl ? w.limit : w.h.t;
+    if (r3.x > 0.000025f && r3.x < lim) {
+        w.h.t = r3.x;
+        w.h.tri = i;
+        w.h.inst = w.ii;
+        w.h.b1 = r3.y;
+        w.h.b2 = r3.z;
+        if (occl) {
+            w.h.err |= kHitOccluded;
+            w.phase = kPhaseDone;
             return;
         }
     }
+    if (w.mode == kWalkFirst && w.h.tri != kEmpty) walk_instance_end(w, true);
 }
 
 // One iteration; returns true once the ray is finished.
 template <class Stack, class St>
 __device__ __forceinline__ bool walk_step(const DevScene& s, Walk& w, Stack& stk, St& st) {
-    if (w.phase == kPhaseBlas) walk_blas(s, w, stk, st);
+    if (w.phase == kPhaseTri) walk_tri(s, w, st);
+    else if (w.phase == kPhaseBlas) walk_blas(s, w, stk, st);
     else if (w.phase == kPhaseTlas) walk_tlas(s, w, stk, st);
     else if (w.phase == kPhaseLeaf) walk_instance(s, w, st);
     return w.phase == kPhaseDone;

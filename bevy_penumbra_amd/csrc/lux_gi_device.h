@@ -264,7 +264,8 @@ __device__ bool shade_begin(const GiArgs& a, float3 o, float3 d, const lux::HitS
     const lux::DevScene& s = a.scene;
     const float4* rec = s.inst + 8u * h.inst;
     uint32_t iflags = __float_as_uint(rec[6].x);
-    float4 A = s.triA[h.tri], B = s.triB[h.tri], C = s.triC[h.tri];
+    const float4* tp = s.tri + 3ull * h.tri;
+    float4 A = tp[0], B = tp[1], C = tp[2];
     uint32_t mi = __float_as_uint(B.w);
     rad = make_float3(0.0f, 0.0f, 0.0f);
     if (a.n_mats == 0) return false;

@@ -31,7 +31,7 @@ struct lux_ctx {
     // built scene
     bool blas_built = false, tlas_built = false;
     uint32_t n_blas = 0, n_tlas = 0;
-    lux_buffer triA, triB, triC;  // float4[n_tri]
+    lux_buffer tris;              // float4[3 * n_tri]: A, B, C of each sorted triangle
     lux_buffer tri_order;         // uint32[n_tri] sorted -> input triangle (global)
     lux_buffer blas;              // float4[2*n_blas]
     lux_buffer tlas;              // float4[2*n_tlas]
@@ -82,7 +82,7 @@ struct lux_ctx {
     // trace tuning
     // traversal launch configuration (defaults: the autotuner's choice on the
     // 262,144-triangle colonnade microbenchmark)
-    uint32_t waves_per_cu = 20, regroup = 4, trace_depth = 8, trace_batch = 8;
+    uint32_t waves_per_cu = 20, regroup = 8, trace_depth = 8, trace_batch = 8;
     float last_ms = 0.0f;
 
     lux::DevScene dev_scene() const;

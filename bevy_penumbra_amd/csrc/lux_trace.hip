@@ -36,7 +36,7 @@ struct TraceArgs {
 __device__ __forceinline__ void write_hit(const TraceArgs& a, uint32_t idx, const lux::HitState& h) {
     uint4 out;
     out.x = __float_as_uint(h.t);
-    out.y = h.tri == lux::kEmpty ? lux::kEmpty : __float_as_uint(a.scene.triA[h.tri].w);
+    out.y = h.tri == lux::kEmpty ? lux::kEmpty : __float_as_uint(a.scene.tri[3ull * h.tri].w);
     out.z = h.inst == lux::kEmpty ? lux::kEmpty : __float_as_uint(a.scene.inst[8u * h.inst + 7u].x);
     out.w = h.err;
     reinterpret_cast<uint4*>(a.hits)[idx] = out;
@@ -56,7 +56,7 @@ __device__ __forceinline__ void trace_one(const TraceArgs& a, uint32_t idx,
     else lux::trace_closest(a.scene, o, d, r0.w, h, stk, st, (flags & lux::kRayFirstHit) != 0u);
     uint4 out;
     out.x = __float_as_uint(h.t);
-    out.y = h.tri == lux::kEmpty ? lux::kEmpty : __float_as_uint(a.scene.triA[h.tri].w);
+    out.y = h.tri == lux::kEmpty ? lux::kEmpty : __float_as_uint(a.scene.tri[3ull * h.tri].w);
     out.z = h.inst == lux::kEmpty ? lux::kEmpty : __float_as_uint(a.scene.inst[8u * h.inst + 7u].x);
     out.w = h.err;
     reinterpret_cast<uint4*>(a.hits)[idx] = out;
@@ -147,9 +147,7 @@ lux::DevScene lux_ctx::dev_scene() const {
     s.tlas = static_cast<const float4*>(tlas.ptr);
     s.inst = static_cast<const float4*>(inst.ptr);
     s.blas = static_cast<const float4*>(blas.ptr);
-    s.triA = static_cast<const float4*>(triA.ptr);
-    s.triB = static_cast<const float4*>(triB.ptr);
-    s.triC = static_cast<const float4*>(triC.ptr);
+    s.tri = static_cast<const float4*>(tris.ptr);
     s.n_tlas = n_tlas;
     s.n_inst = n_inst;
     s.n_blas = n_blas;

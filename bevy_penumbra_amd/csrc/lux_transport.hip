@@ -694,7 +694,8 @@ __global__ __launch_bounds__(kWg) void k_camera_receivers(lux::DevScene scene, C
     }
     const float4* rec = scene.inst + 8u * hs.inst;
     uint32_t iflags = __float_as_uint(rec[6].x);
-    float4 A = scene.triA[hs.tri], B = scene.triB[hs.tri], C = scene.triC[hs.tri];
+    const float4* tp = scene.tri + 3ull * hs.tri;
+    float4 A = tp[0], B = tp[1], C = tp[2];
     float3 n = geometric_normal(rec, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
                                 make_float3(C.x, C.y, C.z), iflags);
     if (dot3(n, d) > 0.0f) n = make_float3(-n.x, -n.y, -n.z);

@@ -106,7 +106,7 @@ typedef struct lux_ctx lux_ctx;
  * (0 = 8).  lds_stack_depth: traversal-stack slots per lane kept in LDS
  * (4, 8 or 16; 0 = 8), deeper slots spill to private memory.
  * waves_per_cu: persistent grid size (0 = 20; 1000 = one ray per thread,
- * no persistence).  A new context starts at {20, 8, 4, 8}.  Results do not
+ * no persistence).  A new context starts at {20, 8, 8, 8}.  Results do not
  * depend on the configuration. */
 typedef struct {
     uint32_t waves_per_cu;

@@ -232,3 +232,28 @@ def test_gpu_errors_are_reported(lux_ctx):
     with pytest.raises(lux.LuxError):
         ctx2.trace(np.zeros((4, 8), np.float32))  # no scene built
     ctx2.close()
+
+
+@pytest.mark.parametrize("case", ["colonnade", "fixture"])
+def test_gpu_traversal_work_equals_oracle(lux_ctx, colonnade, fixture, case):
+    """Node fetches, triangle tests and instance visits summed over every ray
+    equal the oracle's: each box test (the slab divisions included) took the
+    same branch as the reference traversal's on every node."""
+    if case == "colonnade":
+        sc, built = colonnade
+        rays = scenes.microbench_rays(sc.bounds, n=1 << 20, seed=3)
+    else:
+        sc, built, rays = fixture
+    lux_ctx.upload_scene(sc)
+    lux_ctx.build_bvh()
+    n = rays.shape[0]
+    r = DeviceBuffer.from_array(lux_ctx, np.ascontiguousarray(rays, np.float32))
+    h = DeviceBuffer.empty(lux_ctx, (n, 4), np.uint32)
+    c = DeviceBuffer.empty(lux_ctx, 3, np.uint64)
+    c.zero()
+    lux_ctx.trace_stats_device(r, n, h, c)
+    lux_ctx.synchronize()
+    got = c.numpy().tolist()
+    ohits, st = built.trace(rays, threads=16, stats=True)
+    assert np.array_equal(h.numpy(), ohits)
+    assert got == [st["node_fetches"], st["tri_tests"], st["inst_visits"]], (got, st)
