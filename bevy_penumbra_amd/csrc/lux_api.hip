@@ -136,6 +136,114 @@ __global__ void k_inst_records(const InstDesc* d, const uint32_t* perm, uint32_t
     order[k] = s;
 }
 
+// Refit of a built TLAS to new instance boxes (the reference's refit,
+// src/scene.rs:1040-1058): topology and instance order stay, every node
+// reachable from the root gets the union of its children's boxes, a leaf the
+// union of its instances' (TLAS slot j holds caller instance order[j]).
+// One thread, post-order with an explicit stack (children may sit at lower
+// slots than their parent in the LBVH layout).
+// The same refit with nodes and instance boxes staged in LDS (dynamic
+// shared memory: 32 bytes per node, then 32 per instance): the block loads
+// them, one thread refits, the block writes the nodes back.
+__global__ __launch_bounds__(256) void k_tlas_refit_lds(float4* nodes, uint32_t n_nodes, const float4* lo,
+                                                        const float4* hi, const uint32_t* order, uint32_t n_inst) {
+    extern __shared__ float4 sm[];
+    float4* sn = sm;                  // 2 per node
+    float4* sb = sm + 2u * n_nodes;   // lo, hi per TLAS slot (instance order[j])
+    for (uint32_t i = threadIdx.x; i < 2u * n_nodes; i += blockDim.x) sn[i] = nodes[i];
+    for (uint32_t j = threadIdx.x; j < n_inst; j += blockDim.x) {
+        uint32_t c = order[j];
+        sb[2u * j] = lo[c];
+        sb[2u * j + 1u] = hi[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t stack[128];
+        uint32_t sp = 0;
+        stack[sp++] = 0u;
+        while (sp > 0) {
+            uint32_t top = stack[sp - 1];
+            uint32_t slot = top & 0x7fffffffu;
+            if (slot >= n_nodes) {
+                --sp;
+                continue;
+            }
+            float4 a = sn[2u * slot], b = sn[2u * slot + 1u];
+            uint32_t first = __float_as_uint(a.w), count = __float_as_uint(b.w);
+            float4 mn = make_float4(INFINITY, INFINITY, INFINITY, a.w);
+            float4 mx = make_float4(-INFINITY, -INFINITY, -INFINITY, b.w);
+            if (count > 0u) {
+                for (uint32_t j = first; j < first + count && j < n_inst; ++j) {
+                    float4 l = sb[2u * j], h = sb[2u * j + 1u];
+                    mn = make_float4(sel_min(mn.x, l.x), sel_min(mn.y, l.y), sel_min(mn.z, l.z), a.w);
+                    mx = make_float4(sel_max(mx.x, h.x), sel_max(mx.y, h.y), sel_max(mx.z, h.z), b.w);
+                }
+            } else if (!(top & 0x80000000u) && sp + 2 <= 128u && first + 1u < n_nodes) {
+                stack[sp - 1] = top | 0x80000000u;
+                stack[sp++] = first;
+                stack[sp++] = first + 1u;
+                continue;
+            } else if (first + 1u < n_nodes) {
+                float4 la = sn[2u * first], lb = sn[2u * first + 1u];
+                float4 ra = sn[2u * first + 2u], rb = sn[2u * first + 3u];
+                mn = make_float4(sel_min(la.x, ra.x), sel_min(la.y, ra.y), sel_min(la.z, ra.z), a.w);
+                mx = make_float4(sel_max(lb.x, rb.x), sel_max(lb.y, rb.y), sel_max(lb.z, rb.z), b.w);
+            } else {
+                --sp;
+                continue;
+            }
+            sn[2u * slot] = mn;
+            sn[2u * slot + 1u] = mx;
+            --sp;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 2u * n_nodes; i += blockDim.x) nodes[i] = sn[i];
+}
+
+__global__ void k_tlas_refit(float4* nodes, uint32_t n_nodes, const float4* lo, const float4* hi,
+                             const uint32_t* order, uint32_t n_inst) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t stack[128];
+    uint32_t sp = 0;
+    stack[sp++] = 0u;  // bit 31: children done
+    while (sp > 0) {
+        uint32_t top = stack[sp - 1];
+        uint32_t slot = top & 0x7fffffffu;
+        if (slot >= n_nodes) {
+            --sp;
+            continue;
+        }
+        float4 a = nodes[2u * slot], b = nodes[2u * slot + 1u];
+        uint32_t first = __float_as_uint(a.w), count = __float_as_uint(b.w);
+        float4 mn = make_float4(INFINITY, INFINITY, INFINITY, a.w), mx = make_float4(-INFINITY, -INFINITY, -INFINITY, b.w);
+        if (count > 0u) {
+            for (uint32_t j = first; j < first + count && j < n_inst; ++j) {
+                uint32_t s = order[j];
+                float4 l = lo[s], h = hi[s];
+                mn = make_float4(sel_min(mn.x, l.x), sel_min(mn.y, l.y), sel_min(mn.z, l.z), a.w);
+                mx = make_float4(sel_max(mx.x, h.x), sel_max(mx.y, h.y), sel_max(mx.z, h.z), b.w);
+            }
+        } else if (!(top & 0x80000000u) && sp + 2 <= 128u && first + 1u < n_nodes) {
+            stack[sp - 1] = top | 0x80000000u;
+            stack[sp++] = first;
+            stack[sp++] = first + 1u;
+            continue;
+        } else if (first + 1u < n_nodes) {
+            float4 la = nodes[2u * first], lb = nodes[2u * first + 1u];
+            float4 ra = nodes[2u * first + 2u], rb = nodes[2u * first + 3u];
+            mn = make_float4(sel_min(la.x, ra.x), sel_min(la.y, ra.y), sel_min(la.z, ra.z), a.w);
+            mx = make_float4(sel_max(lb.x, rb.x), sel_max(lb.y, rb.y), sel_max(lb.z, rb.z), b.w);
+        } else {
+            --sp;
+            continue;
+        }
+        nodes[2u * slot] = mn;
+        nodes[2u * slot + 1u] = mx;
+        --sp;
+    }
+}
+
 inline uint32_t nblk(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
 
 }  // namespace
@@ -429,9 +537,38 @@ static int build_tlas(lux_ctx* ctx) {
     return rc;
 }
 
+// Refit path of lux_bvh_build: new instance boxes and records in the
+// existing TLAS order, boxes refitted (no allocation, no host wait).
+static int refit_tlas(lux_ctx* ctx) {
+    hipStream_t st = ctx->stream;
+    const uint32_t ni = ctx->n_inst;
+    float4* lo = static_cast<float4*>(ctx->tlas_tmp.ptr);
+    float4* hi = lo + ni;
+    const uint32_t* mb = static_cast<const uint32_t*>(ctx->mesh_info.ptr);
+    const uint32_t* mt = mb + ctx->n_meshes;
+    const uint32_t* mc = mt + ctx->n_meshes;
+    const InstDesc* d = static_cast<const InstDesc*>(ctx->inst_desc.ptr);
+    const uint32_t* order = static_cast<const uint32_t*>(ctx->inst_order.ptr);
+    hipLaunchKernelGGL(k_inst_boxes, dim3(nblk(ni, 128)), dim3(128), 0, st, d, ni,
+                       static_cast<const float4*>(ctx->blas.ptr), mb, lo, hi);
+    const size_t lds = 32ull * ctx->n_tlas + 32ull * ni;
+    if (lds <= 60ull * 1024)
+        hipLaunchKernelGGL(k_tlas_refit_lds, dim3(1), dim3(256), lds, st, static_cast<float4*>(ctx->tlas.ptr),
+                           ctx->n_tlas, static_cast<const float4*>(lo), static_cast<const float4*>(hi), order, ni);
+    else
+        hipLaunchKernelGGL(k_tlas_refit, dim3(1), dim3(64), 0, st, static_cast<float4*>(ctx->tlas.ptr), ctx->n_tlas,
+                           static_cast<const float4*>(lo), static_cast<const float4*>(hi), order, ni);
+    // the records take the new transforms in the same order (order[k] stays)
+    hipLaunchKernelGGL(k_inst_records, dim3(nblk(ni, 128)), dim3(128), 0, st, d, order, ni, mb, mt, mc,
+                       static_cast<float4*>(ctx->inst.ptr), static_cast<uint32_t*>(ctx->tlas_tmp.ptr) + 8ull * ni);
+    LUX_HIP(ctx, hipGetLastError());
+    ctx->tlas_built = true;
+    return LUX_OK;
+}
+
 extern "C" int lux_bvh_build(lux_ctx* ctx, uint32_t mode) {
     if (!ctx) return LUX_ERR_INVALID;
-    if ((mode & ~LUX_BVH_FORCE_BLAS) != LUX_BVH_MORTON30)
+    if ((mode & ~(LUX_BVH_FORCE_BLAS | LUX_BVH_REFIT)) != LUX_BVH_MORTON30)
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_bvh_build: unknown mode");
     if (mode & LUX_BVH_FORCE_BLAS) ctx->blas_built = false;
     if (ctx->n_meshes == 0 && ctx->n_inst == 0)
@@ -440,9 +577,14 @@ extern "C" int lux_bvh_build(lux_ctx* ctx, uint32_t mode) {
     if (!ctx->blas_built) {
         if (int rc = build_blas(ctx)) return rc;
         ctx->tlas_built = false;
+        ctx->tlas_topology = false;
     }
-    if (!ctx->tlas_built)
+    if (!ctx->tlas_built) {
+        // refit when only transforms changed since a TLAS build of this scene
+        if ((mode & LUX_BVH_REFIT) && ctx->tlas_topology && ctx->n_inst > 1) return refit_tlas(ctx);
         if (int rc = build_tlas(ctx)) return rc;
+        ctx->tlas_topology = true;
+    }
     return LUX_OK;
 }
 

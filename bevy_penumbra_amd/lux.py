@@ -25,6 +25,7 @@ LUX_RAY_FIRST_HIT = 2
 LUX_EMPTY = 0xFFFFFFFF
 LUX_BVH_MORTON30 = 0
 LUX_BVH_FORCE_BLAS = 0x100
+LUX_BVH_REFIT = 0x200
 
 
 class LuxError(RuntimeError):

@@ -151,6 +151,11 @@ int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_world,
 #define LUX_BVH_MORTON30 0u
 /* or-ed into the mode: rebuild every BLAS too, not only what changed */
 #define LUX_BVH_FORCE_BLAS 0x100u
+/* or-ed into the mode: when only instance transforms changed since the last
+ * TLAS build, keep its topology and instance order and refit its boxes to
+ * the new instance bounds (the reference's per-frame TLAS refit,
+ * src/scene.rs:1040-1058,2033-2047); otherwise build as usual. */
+#define LUX_BVH_REFIT 0x200u
 /* Builds what is stale: the BLASes after lux_scene_upload_soa, the TLAS after
  * lux_scene_set_transforms (Morton-30 over instance boxes; no host wait). */
 int lux_bvh_build(lux_ctx* ctx, uint32_t mode);

@@ -118,6 +118,11 @@ typedef struct {
     uint32_t n_tlas;
     uint32_t n_inst;
 } orc_scene_sizes;
+/* World bounds of an instance from its forward rows and BLAS root (the
+ * boxes the TLAS is built from); TLAS refit to new instance boxes. */
+void orc_instance_bounds(const float* fwd, const orc_node* root, float* lo, float* hi);
+void orc_tlas_refit(orc_node* nodes, uint32_t n_nodes, const float* lo, const float* hi, const uint32_t* order,
+                    uint32_t n_inst);
 int orc_scene_build(const float* const* mesh_pos, const uint32_t* const* mesh_mat,
                     const uint32_t* mesh_tri_count, uint32_t n_meshes, const float* inst_inv,
                     const float* inst_fwd, const uint32_t* inst_mesh, const uint32_t* inst_flags,

@@ -32,6 +32,8 @@ def load() -> C.CDLL:
         "orc_morton_codes": (None, [vp, vp, u32, vp]),
         "orc_stable_sort_pairs": (None, [vp, u32, vp]),
         "orc_scene_build": (i32, [vp, vp, vp, u32, vp, vp, vp, vp, u32, u32, i32, vp, vp, vp, vp, vp]),
+        "orc_instance_bounds": (None, [vp, vp, vp, vp]),
+        "orc_tlas_refit": (None, [vp, u32, vp, vp, vp, u32]),
         "orc_scene_capacity": (None, [vp, u32, u32, vp, vp, vp]),
         "orc_sah_build": (u32, [vp, vp, u32, vp, vp]),
         "orc_tree_cost": (C.c_double, [vp, u32]),
@@ -124,6 +126,27 @@ def build_scene(scene, sah: bool = False, leaf_max: int = 4) -> BuiltScene:
                         inst.ctypes.data, sizes.ctypes.data)
     return BuiltScene(blas[: sizes[1]].copy(), tlas[: sizes[2]].copy(), tri[: sizes[0]].copy(),
                       inst[:ni].copy())
+
+
+def instance_bounds(fwd_rows: np.ndarray, blas_root: np.ndarray):
+    """(lo, hi) float32[4] world bounds of an instance (12 forward-row floats,
+    its BLAS root node as 8 uint32 words)."""
+    f = np.ascontiguousarray(fwd_rows, np.float32)
+    r = np.ascontiguousarray(blas_root, np.uint32)
+    lo, hi = np.zeros(4, np.float32), np.zeros(4, np.float32)
+    load().orc_instance_bounds(f.ctypes.data, r.ctypes.data, lo.ctypes.data, hi.ctypes.data)
+    return lo, hi
+
+
+def tlas_refit(tlas: np.ndarray, lo: np.ndarray, hi: np.ndarray, order: np.ndarray) -> np.ndarray:
+    """Refitted copy of TLAS nodes (n, 8) uint32 for instance boxes lo/hi
+    (caller order, float32[n_inst, 4]) and the TLAS instance order."""
+    t = np.array(tlas, np.uint32, copy=True, order="C")
+    l = np.ascontiguousarray(lo, np.float32)
+    h = np.ascontiguousarray(hi, np.float32)
+    o = np.ascontiguousarray(order, np.uint32)
+    load().orc_tlas_refit(t.ctypes.data, t.shape[0], l.ctypes.data, h.ctypes.data, o.ctypes.data, o.shape[0])
+    return t
 
 
 def morton_codes(lo4: np.ndarray, hi4: np.ndarray) -> np.ndarray:

@@ -428,6 +428,51 @@ static void instance_bounds(const float* fwd, const orc_node* root, float* lo, f
     lo[3] = hi[3] = 0.0f;
 }
 
+void orc_instance_bounds(const float* fwd, const orc_node* root, float* lo, float* hi) {
+    instance_bounds(fwd, root, lo, hi);
+}
+
+/* Post-order refit of the subtree at `slot` (see orc_tlas_refit). */
+static void refit_node(orc_node* nodes, uint32_t n_nodes, uint32_t slot, const float* lo, const float* hi,
+                       const uint32_t* order, uint32_t n_inst) {
+    if (slot >= n_nodes) return;
+    orc_node* nd = &nodes[slot];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (nd->count > 0) {
+        for (uint32_t j = nd->first; j < nd->first + nd->count && j < n_inst; ++j) {
+            const float* l = lo + 4ull * order[j];
+            const float* h = hi + 4ull * order[j];
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = sel_min(mn[a], l[a]);
+                mx[a] = sel_max(mx[a], h[a]);
+            }
+        }
+    } else {
+        if (nd->first + 1u >= n_nodes) return;
+        refit_node(nodes, n_nodes, nd->first, lo, hi, order, n_inst);
+        refit_node(nodes, n_nodes, nd->first + 1u, lo, hi, order, n_inst);
+        const orc_node* a = &nodes[nd->first];
+        const orc_node* b = &nodes[nd->first + 1u];
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = sel_min(a->mn[k], b->mn[k]);
+            mx[k] = sel_max(a->mx[k], b->mx[k]);
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        nd->mn[a] = mn[a];
+        nd->mx[a] = mx[a];
+    }
+}
+
+/* refit (src/scene.rs:1040-1058): every node reachable from the root gets
+ * the union of its instances' boxes (leaf; TLAS slot j holds instance
+ * order[j]) or of its children's boxes, children first.  lo/hi: 4 floats
+ * per instance.  Topology and unreachable slots are unchanged. */
+void orc_tlas_refit(orc_node* nodes, uint32_t n_nodes, const float* lo, const float* hi, const uint32_t* order,
+                    uint32_t n_inst) {
+    if (n_nodes > 0) refit_node(nodes, n_nodes, 0u, lo, hi, order, n_inst);
+}
+
 int orc_scene_build(const float* const* mesh_pos, const uint32_t* const* mesh_mat,
                     const uint32_t* mesh_tri_count, uint32_t n_meshes, const float* inst_inv,
                     const float* inst_fwd, const uint32_t* inst_mesh, const uint32_t* inst_flags,

@@ -132,3 +132,46 @@ def test_gpu_camera_receivers_are_oracle_hits(lux_ctx, small_dynamic):
     for e in (e1, e2):
         cos = np.abs(np.sum(n * e, axis=1)) / np.linalg.norm(e, axis=1)
         assert cos.max() < 1e-4
+
+
+def test_gpu_tlas_refit_matches_oracle_refit(lux_ctx, small_dynamic):
+    """LUX_BVH_REFIT after motion keeps the TLAS topology and instance order
+    and refits every box (src/scene.rs:1040-1058): bit for bit the oracle's
+    refit of the pre-motion TLAS with the moved instances' bounds, and the
+    oracle's traversal of that refitted tree gives the GPU's hits.  A refit
+    without motion reproduces the built TLAS."""
+    sc, motion = small_dynamic
+    sc0, inv0, fwd0 = moved(sc, motion, 0)
+    lux_ctx.upload_scene(sc0)
+    lux_ctx.build_bvh()
+    g0 = lux_ctx.download_bvh()
+    lux_ctx.set_transforms(inv0, fwd0)
+    lux_ctx.build_bvh(lux.LUX_BVH_MORTON30 | lux.LUX_BVH_REFIT)
+    same = lux_ctx.download_bvh()
+    assert np.array_equal(same["tlas"], g0["tlas"]) and np.array_equal(same["inst_order"], g0["inst_order"])
+    order = g0["inst_order"]
+    tlas = g0["tlas"]
+    for frame in (5, 40):
+        sc2, inv, fwd = moved(sc, motion, frame)
+        lux_ctx.set_transforms(inv, fwd)
+        lux_ctx.build_bvh(lux.LUX_BVH_MORTON30 | lux.LUX_BVH_REFIT)
+        g = lux_ctx.download_bvh()
+        built = O.build_scene(sc2)
+        roots = {int(r[28]): built.blas[int(r[25])] for r in built.inst}
+        lo = np.zeros((len(order), 4), np.float32)
+        hi = np.zeros((len(order), 4), np.float32)
+        for c in range(len(order)):
+            lo[c], hi[c] = O.instance_bounds(fwd[c], roots[c])
+        want = O.tlas_refit(tlas, lo, hi, order)
+        assert np.array_equal(g["inst_order"], order), frame
+        assert np.array_equal(g["tlas"], want), frame
+        assert not np.array_equal(want, tlas)
+        by_caller = {int(r[28]): r for r in built.inst}
+        inst = np.stack([by_caller[int(c)] for c in order])
+        refit_scene = O.BuiltScene(built.blas, want, built.tri, inst)
+        rays = scenes.microbench_rays(sc.bounds, n=65536, seed=frame + 11)
+        hits = trace_gpu(lux_ctx, rays)
+        expect = refit_scene.trace(rays)
+        bad = np.nonzero(np.any(hits != expect, axis=1))[0]
+        assert bad.size == 0, (frame, bad[:5])
+        tlas = want

@@ -207,3 +207,45 @@ def test_lbvh_on_sah_fixture_is_valid():
     lo, hi = sah_fixture_250k()
     codes, perm, nodes = O.lbvh(lo, hi, leaf_max=4)
     check_tree(nodes, lo, hi, perm, 4)
+
+
+def test_oracle_tlas_refit_is_a_tight_union():
+    """refit (src/scene.rs:1040-1058): with the build's own instance boxes the
+    TLAS is reproduced; after motion every reachable node is exactly the
+    union of its children (internal) or instances (leaf), topology kept."""
+    import dataclasses
+    from bevy_penumbra_amd import scenes
+    sc, motion = scenes.dynamic(seed=5, meshes=6, instances=40, stacks=6, slices=12)
+    built = O.build_scene(sc)
+    order = built.inst[:, 28].astype(np.uint32)
+    roots = {int(r[28]): built.blas[int(r[25])] for r in built.inst}
+
+    def boxes(fwd):
+        lo = np.zeros((len(order), 4), np.float32)
+        hi = np.zeros((len(order), 4), np.float32)
+        for c in range(len(order)):
+            lo[c], hi[c] = O.instance_bounds(fwd[c], roots[c])
+        return lo, hi
+
+    lo, hi = boxes(sc.inst_fwd)
+    assert np.array_equal(O.tlas_refit(built.tlas, lo, hi, order), built.tlas)
+    inv, fwd = scenes.dynamic_transforms(motion, 17)
+    lo, hi = boxes(fwd)
+    t = O.tlas_refit(built.tlas, lo, hi, order)
+    f = t.view(np.float32)
+    assert np.array_equal(t[:, [3, 7]], built.tlas[:, [3, 7]])       # first / count unchanged
+    seen = 0
+    stack = [0]
+    while stack:
+        k = stack.pop()
+        seen += 1
+        first, count = int(t[k, 3]), int(t[k, 7])
+        if count:
+            idx = order[first:first + count]
+            assert np.array_equal(f[k, :3], lo[idx, :3].min(axis=0))
+            assert np.array_equal(f[k, 4:7], hi[idx, :3].max(axis=0))
+        else:
+            assert np.array_equal(f[k, :3], np.minimum(f[first, :3], f[first + 1, :3]))
+            assert np.array_equal(f[k, 4:7], np.maximum(f[first, 4:7], f[first + 1, 4:7]))
+            stack += [first, first + 1]
+    assert seen > 1
