@@ -369,6 +369,8 @@ extern "C" int lux_scene_upload_soa(lux_ctx* ctx, const lux_mesh_desc* meshes, u
     }
     ctx->host_materials.assign(materials, materials + material_count);
     ctx->gi_ready = false;
+    ctx->prebuilt = false;
+    ctx->inst_material.clear();
     ctx->instances.assign(instances, instances + instance_count);
     std::vector<InstDesc> idesc(instance_count);
     for (uint32_t i = 0; i < instance_count; ++i) {
@@ -393,6 +395,8 @@ extern "C" int lux_scene_upload_soa(lux_ctx* ctx, const lux_mesh_desc* meshes, u
 extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_world,
                                         const float* world_from_local, uint32_t instance_count) {
     if (!ctx) return LUX_ERR_INVALID;
+    if (ctx->prebuilt)
+        return lux_fail(ctx, LUX_ERR_STATE, "lux_scene_set_transforms: scene installed built (upload new words)");
     if (instance_count != ctx->n_inst)
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_set_transforms: instance count changed");
     if (instance_count && (!local_from_world || !world_from_local))
@@ -573,6 +577,7 @@ extern "C" int lux_bvh_build(lux_ctx* ctx, uint32_t mode) {
     if (mode & LUX_BVH_FORCE_BLAS) ctx->blas_built = false;
     if (ctx->n_meshes == 0 && ctx->n_inst == 0)
         return lux_fail(ctx, LUX_ERR_STATE, "lux_bvh_build: no scene uploaded");
+    if (ctx->prebuilt) return LUX_OK;  // installed built: nothing to build
     LUX_HIP(ctx, hipSetDevice(ctx->device));
     if (!ctx->blas_built) {
         if (int rc = build_blas(ctx)) return rc;
@@ -643,6 +648,13 @@ extern "C" int lux_scene_upload_built(lux_ctx* ctx, const lux_bvh_node* blas, ui
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_built: empty or null array");
     static_assert(sizeof(lux_instance_record) == 128, "instance record layout");
     static_assert(sizeof(lux_bvh_node) == 32, "node layout");
+    // shading reads the hit triangle unclamped: every instance's range must lie inside the table
+    for (uint32_t k = 0; k < instance_count; ++k) {
+        uint32_t g[4];
+        std::memcpy(g, reinterpret_cast<const uint8_t*>(instances + k) + 96, 16);
+        if ((uint64_t)g[2] + g[3] > triangle_count || g[1] >= blas_count)
+            return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_built: instance triangles or root out of range");
+    }
     LUX_HIP(ctx, hipSetDevice(ctx->device));
     LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (int rc = lux_ensure(ctx, ctx->blas, 32ull * blas_count)) return rc;
@@ -663,18 +675,167 @@ extern "C" int lux_scene_upload_built(lux_ctx* ctx, const lux_bvh_node* blas, ui
     ctx->mesh_tri_base.clear();
     ctx->mesh_blas_base.clear();
     ctx->instances.clear();
+    ctx->inst_material.clear();
     ctx->blas_built = true;
     ctx->tlas_built = true;
+    ctx->tlas_topology = false;
+    ctx->prebuilt = true;
     return LUX_OK;
 }
 
 namespace {
+// Triangle records from the reference's geometry words: triangle t's vertex
+// indices at word 1 + t, vertex v's position at word vertex_base + stride v
+// (clamped reads, as traversal.slang:51-62 clamps every word address).
+__global__ void k_words_triangles(const uint4* words, uint32_t n_words, uint32_t n_tri, uint32_t vertex_base,
+                                  uint32_t stride, float4* tri) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tri) return;
+    const uint32_t last = n_words - 1u;
+    uint32_t iw = 1u + t;
+    uint4 idx = words[iw < last ? iw : last];
+    uint32_t v[3] = {idx.x, idx.y, idx.z};
+    float4* out = tri + 3ull * t;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        uint64_t w = (uint64_t)vertex_base + (uint64_t)stride * v[c];
+        uint4 p = words[w < last ? w : last];
+        out[c] = make_float4(__uint_as_float(p.x), __uint_as_float(p.y), __uint_as_float(p.z),
+                             c == 0 ? __uint_as_float(t) : 0.0f);
+    }
+}
+
 __global__ void k_zero_bytes(uint8_t* p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = 0;
 }
 }  // namespace
+
+// Packed-scene interchange: the reference's inline scene and geometry words
+// (src/scene.rs:809-823 geometry_words, :1025-1038 node_words, :1071-1122
+// pack_instances, :2102-2116 the prefix header) installed as a built scene.
+extern "C" int lux_scene_upload_words(lux_ctx* ctx, const uint32_t* scene_words, uint32_t n_scene_words,
+                                      const uint32_t* geometry_words, uint32_t n_geometry_words,
+                                      const lux_material* materials, uint32_t material_count) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!scene_words || !geometry_words || n_scene_words < 4 || n_geometry_words < 2)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_words: empty or null words");
+    if (material_count && !materials) return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_words: null materials");
+    auto word = [&](const uint32_t* w, uint64_t i, int c) { return w[4ull * i + c]; };
+    const uint32_t inst_base = word(scene_words, 0, 1), n_inst = word(scene_words, 0, 2),
+                   stride_i = word(scene_words, 0, 3);
+    if (stride_i != 9u) return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_words: instance stride is not 9 words");
+    if (n_inst == 0 || inst_base < 3 || (uint64_t)inst_base + 9ull * n_inst > n_scene_words)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_words: instance block outside the scene words");
+    const uint32_t n_nodes = (inst_base - 1u) / 2u;
+    const uint32_t tri_base = word(geometry_words, 0, 0), vertex_base = word(geometry_words, 0, 1),
+                   vstride = word(geometry_words, 0, 2), n_vert = word(geometry_words, 0, 3);
+    if (tri_base != 1u || vstride == 0 || vertex_base < 1 ||
+        (uint64_t)vertex_base + (uint64_t)vstride * n_vert > n_geometry_words)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_words: malformed geometry header");
+    const uint64_t material_base = (uint64_t)inst_base + 9ull * n_inst;
+    // instances: rows, flags, material, BVH root and triangle range
+    std::vector<float> rec(32ull * n_inst);
+    std::vector<uint32_t> inst_mat(n_inst);
+    std::vector<std::pair<uint32_t, uint32_t>> ranges;
+    std::vector<uint32_t> inst_mesh(n_inst);
+    uint32_t n_tri = 0;
+    for (uint32_t k = 0; k < n_inst; ++k) {
+        const uint64_t b = (uint64_t)inst_base + 9ull * k;
+        float* r = rec.data() + 32ull * k;
+        for (int row = 0; row < 6; ++row)
+            for (int c = 0; c < 4; ++c) {
+                uint32_t u = word(scene_words, b + row, c);
+                std::memcpy(&r[4 * row + c], &u, 4);
+            }
+        uint32_t flags = word(scene_words, b + 6, 3), mat_addr = word(scene_words, b + 7, 3);
+        uint32_t root = word(scene_words, b + 8, 0), first = word(scene_words, b + 8, 1),
+                 count = word(scene_words, b + 8, 2);
+        if (root >= n_nodes) return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_words: instance root out of range");
+        if ((uint64_t)first + count > (uint64_t)vertex_base - 1u)
+            return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_words: instance triangles outside the geometry");
+        n_tri = std::max(n_tri, first + count);
+        uint64_t mi = mat_addr >= material_base ? (mat_addr - material_base) / 8u : 0u;
+        inst_mat[k] = material_count ? (uint32_t)std::min<uint64_t>(mi, material_count - 1u) : 0u;
+        uint32_t m = 0;
+        while (m < ranges.size() && ranges[m] != std::make_pair(first, count)) ++m;
+        if (m == ranges.size()) ranges.emplace_back(first, count);
+        inst_mesh[k] = m;
+        uint32_t w6[4] = {flags, root, first, count}, w7[4] = {k, m, inst_mat[k] + 1u, 0u};
+        std::memcpy(&r[24], w6, 16);
+        std::memcpy(&r[28], w7, 16);
+    }
+    if (n_tri == 0) return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_upload_words: no triangles");
+    LUX_HIP(ctx, hipSetDevice(ctx->device));
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // nodes: one array serves as TLAS (root 0) and BLAS (absolute roots)
+    if (int rc = lux_ensure(ctx, ctx->blas, 32ull * n_nodes)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->tlas, 32ull * n_nodes)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->inst, 128ull * n_inst)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->tris, 48ull * n_tri)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->materials, sizeof(lux_material) * (material_count ? material_count : 1))) return rc;
+    LUX_HIP(ctx, hipMemcpy(ctx->blas.ptr, scene_words + 4, 32ull * n_nodes, hipMemcpyHostToDevice));
+    LUX_HIP(ctx, hipMemcpy(ctx->tlas.ptr, scene_words + 4, 32ull * n_nodes, hipMemcpyHostToDevice));
+    LUX_HIP(ctx, hipMemcpy(ctx->inst.ptr, rec.data(), 128ull * n_inst, hipMemcpyHostToDevice));
+    if (material_count)
+        LUX_HIP(ctx, hipMemcpy(ctx->materials.ptr, materials, sizeof(lux_material) * material_count,
+                               hipMemcpyHostToDevice));
+    // triangles: the geometry words go up once and are repacked on the device
+    if (int rc = lux_ensure(ctx, ctx->scratch, 16ull * n_geometry_words)) return rc;
+    LUX_HIP(ctx, hipMemcpy(ctx->scratch.ptr, geometry_words, 16ull * n_geometry_words, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_words_triangles, dim3((n_tri + 255) / 256), dim3(256), 0, ctx->stream,
+                       static_cast<const uint4*>(ctx->scratch.ptr), n_geometry_words, n_tri, vertex_base, vstride,
+                       static_cast<float4*>(ctx->tris.ptr));
+    LUX_HIP(ctx, hipGetLastError());
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // host copies for the emitter table: one mesh per distinct triangle range
+    ctx->host_pos.assign(ranges.size(), {});
+    ctx->host_mat.assign(ranges.size(), {});
+    for (size_t m = 0; m < ranges.size(); ++m) {
+        std::vector<float>& hp = ctx->host_pos[m];
+        hp.resize(9ull * ranges[m].second);
+        for (uint32_t t = 0; t < ranges[m].second; ++t) {
+            uint32_t idx[3] = {word(geometry_words, 1ull + ranges[m].first + t, 0),
+                               word(geometry_words, 1ull + ranges[m].first + t, 1),
+                               word(geometry_words, 1ull + ranges[m].first + t, 2)};
+            for (int c = 0; c < 3; ++c) {
+                uint64_t w = std::min<uint64_t>((uint64_t)vertex_base + (uint64_t)vstride * idx[c], n_geometry_words - 1u);
+                for (int a = 0; a < 3; ++a) {
+                    uint32_t u = word(geometry_words, w, a);
+                    std::memcpy(&hp[9ull * t + 3 * c + a], &u, 4);
+                }
+            }
+        }
+        ctx->host_mat[m].assign(ranges[m].second, 0u);
+    }
+    ctx->host_materials.assign(materials, materials + material_count);
+    ctx->instances.assign(n_inst, lux_instance_desc{});
+    for (uint32_t k = 0; k < n_inst; ++k) {
+        lux_instance_desc& d = ctx->instances[k];
+        std::memcpy(d.local_from_world, rec.data() + 32ull * k, 48);
+        std::memcpy(d.world_from_local, rec.data() + 32ull * k + 12, 48);
+        d.mesh = inst_mesh[k];
+        d.flags = word(scene_words, (uint64_t)inst_base + 9ull * k + 6, 3);
+    }
+    ctx->inst_material = inst_mat;
+    ctx->n_blas = n_nodes;
+    ctx->n_tlas = n_nodes;
+    ctx->n_inst = n_inst;
+    ctx->n_tri = n_tri;
+    ctx->n_materials = material_count;
+    ctx->n_meshes = 0;
+    ctx->mesh_tri_count.clear();
+    ctx->mesh_tri_base.clear();
+    ctx->mesh_blas_base.clear();
+    ctx->blas_built = true;
+    ctx->tlas_built = true;
+    ctx->tlas_topology = false;
+    ctx->prebuilt = true;
+    ctx->gi_ready = false;
+    ctx->gi_lights_dirty = true;
+    return LUX_OK;
+}
 
 extern "C" int lux_device_alloc(lux_ctx* ctx, size_t bytes, void** out) {
     if (!ctx || !out) return LUX_ERR_INVALID;

@@ -266,7 +266,9 @@ __device__ bool shade_begin(const GiArgs& a, float3 o, float3 d, const lux::HitS
     uint32_t iflags = __float_as_uint(rec[6].x);
     const float4* tp = s.tri + 3ull * h.tri;
     float4 A = tp[0], B = tp[1], C = tp[2];
-    uint32_t mi = __float_as_uint(B.w);
+    // a nonzero instance material (scenes installed from scene words) wins
+    const uint32_t mo = __float_as_uint(rec[7].z);
+    uint32_t mi = mo ? mo - 1u : __float_as_uint(B.w);
     rad = make_float3(0.0f, 0.0f, 0.0f);
     if (a.n_mats == 0) return false;
     if (mi >= a.n_mats) mi = a.n_mats - 1u;

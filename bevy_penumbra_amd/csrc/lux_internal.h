@@ -31,6 +31,8 @@ struct lux_ctx {
     // built scene
     bool blas_built = false, tlas_built = false;
     bool tlas_topology = false;  // a TLAS built by lux_bvh_build for the current instance set
+    bool prebuilt = false;       // scene installed with its BVH (upload_built / upload_words)
+    std::vector<uint32_t> inst_material;  // per-instance material (upload_words), else empty
     uint32_t n_blas = 0, n_tlas = 0;
     lux_buffer tris;              // float4[3 * n_tri]: A, B, C of each sorted triangle
     lux_buffer tri_order;         // uint32[n_tri] sorted -> input triangle (global)

@@ -723,8 +723,10 @@ std::vector<Light> build_lights(const lux_ctx* ctx) {
         const std::vector<float>& pos = ctx->host_pos[in.mesh];
         const std::vector<uint32_t>& mat = ctx->host_mat[in.mesh];
         uint32_t nt = (uint32_t)(pos.size() / 9);
+        const bool inst_mat = k < ctx->inst_material.size();
         for (uint32_t t = 0; t < nt; ++t) {
-            uint32_t mi = mat[t] < nm ? mat[t] : nm - 1u;
+            uint32_t mt = inst_mat ? ctx->inst_material[k] : mat[t];
+            uint32_t mi = mt < nm ? mt : nm - 1u;
             const lux_material& m = ctx->host_materials[mi];
             float em[3];
             for (int c = 0; c < 3; ++c) em[c] = m.emission[c] > 0.0f ? m.emission[c] : 0.0f;

@@ -296,6 +296,7 @@ def load_library() -> C.CDLL:
         "lux_gi_update_surrogate": (i32, [vp, vp, u32, C.c_float, vp]),
         "lux_gi_surrogate_probe_error": (i32, [vp, vp, u32]),
         "lux_world_resolve_lods": (i32, [vp, u32, vp, vp, u32, vp]),
+        "lux_scene_upload_words": (i32, [vp, vp, u32, vp, u32, vp, u32]),
         "lux_gi_set_history": (i32, [vp, vp]),
         "lux_gi_reset_history": (i32, [vp]),
         "lux_gi_invalidate_history": (i32, [vp, vp, vp]),
@@ -476,6 +477,15 @@ class Context:
                                              t.shape[0], tr.ctypes.data, tr.shape[0],
                                              ins.ctypes.data, ins.shape[0])
         self._check(rc, "lux_scene_upload_built")
+
+    def upload_words(self, scene_words: np.ndarray, geometry_words: np.ndarray, materials: np.ndarray):
+        """Install a scene from the reference's scene and geometry words ((n, 4) uint32 each) and an
+        (M, 8) float32 material table (albedo.rgb, roughness, emission.rgb, pad)."""
+        sw = np.ascontiguousarray(scene_words, np.uint32)
+        gw = np.ascontiguousarray(geometry_words, np.uint32)
+        mt = np.ascontiguousarray(materials, np.float32)
+        self._check(self.lib.lux_scene_upload_words(self.h, sw.ctypes.data, sw.shape[0], gw.ctypes.data, gw.shape[0],
+                                                    mt.ctypes.data, mt.shape[0]), "lux_scene_upload_words")
 
     def build_bvh(self, mode: int = LUX_BVH_MORTON30) -> None:
         self._check(self.lib.lux_bvh_build(self.h, mode), "lux_bvh_build")

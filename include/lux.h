@@ -165,7 +165,9 @@ typedef struct {
     float local_from_world[12];
     float world_from_local[12];
     uint32_t flags, blas_root, tri_first, tri_count;
-    uint32_t orig_index, mesh, reserved0, reserved1;
+    uint32_t orig_index, mesh;
+    uint32_t material; /* 0: the triangles' materials; m + 1: material m for every triangle */
+    uint32_t reserved1;
 } lux_instance_record;
 
 /* Install an already-built scene (packed-scene interchange): BLAS/TLAS nodes
@@ -176,6 +178,24 @@ int lux_scene_upload_built(lux_ctx* ctx, const lux_bvh_node* blas, uint32_t blas
                            const lux_bvh_node* tlas, uint32_t tlas_count, const float* triangles,
                            uint32_t triangle_count, const lux_instance_record* instances,
                            uint32_t instance_count);
+/* Install a scene from the reference's packed inline scene and geometry
+ * words (16-byte words; src/scene.rs:809-823 geometry_words, :1025-1038
+ * node_words, :1071-1122 pack_instances, :2102-2116 the prefix header
+ * (materials end, instance base, instance count, 9)).  The node array
+ * (words 1 .. instance base) is installed as it is -- TLAS root 0, BLAS
+ * roots and child indices absolute; each instance's transforms, flags
+ * (word 6.w), material (word 7.w, an address into the material block that
+ * follows the instances: material index = (address - block) / 8, taken from
+ * `materials`), BLAS root and triangle range (word 8); triangle t's vertex
+ * indices at geometry word 1 + t and vertex positions at vertex_base +
+ * stride * index.  The geometry words are repacked on the device into the
+ * triangle records; a hit reports t as its triangle.  Layer masks, textures
+ * and paged words are not read.  The scene counts as built: transforms
+ * cannot change (upload new words); the GI emitter table follows the
+ * instance materials. */
+int lux_scene_upload_words(lux_ctx* ctx, const uint32_t* scene_words, uint32_t n_scene_words,
+                           const uint32_t* geometry_words, uint32_t n_geometry_words,
+                           const lux_material* materials, uint32_t material_count);
 
 /* Node/instance counts of the built scene. */
 int lux_bvh_info(lux_ctx* ctx, uint32_t* blas_nodes, uint32_t* tlas_nodes,

@@ -54,7 +54,9 @@ typedef struct {
     float inv[12];
     float fwd[12];
     uint32_t flags, root, first, count;
-    uint32_t orig, mesh, pad0, pad1;
+    uint32_t orig, mesh;
+    uint32_t material; /* 0: the triangles' materials; m + 1: material m for the whole instance */
+    uint32_t pad1;
 } orc_instance;
 
 typedef struct {

@@ -56,7 +56,8 @@ static int SFX(shade)(const orc_scene* s, const orc_light* lights, uint32_t nl,
                       uint32_t inst_slot, REAL out[3]) {
     const orc_instance* in = &s->inst[inst_slot];
     const float* tri = s->tri + 12ull * tri_slot;
-    uint32_t mi = fbits(tri[7]);
+    /* a nonzero instance material (scenes given as scene words) wins */
+    uint32_t mi = in->material ? in->material - 1u : fbits(tri[7]);
     if (nm == 0) { out[0] = out[1] = out[2] = K(0.0); return 1; }
     if (mi >= nm) mi = nm - 1;
     const orc_material* mat = &mats[mi];
