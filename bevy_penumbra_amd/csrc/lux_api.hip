@@ -286,7 +286,11 @@ extern "C" void lux_ctx_destroy(lux_ctx* ctx) {
                           &ctx->gi_probes, &ctx->gi_light_buf, &ctx->gi_sur_queries,
                           &ctx->gi_sur_err, &ctx->gi_sur_list, &ctx->gi_counters,
                           &ctx->tlas_tmp, &ctx->mesh_info, &ctx->gi_w_prim, &ctx->gi_w_shade,
-                          &ctx->gi_w_slots, &ctx->gi_w_vis, &ctx->gi_w_shadow, &ctx->gi_w_counters};
+                          &ctx->gi_w_slots, &ctx->gi_w_vis, &ctx->gi_w_shadow, &ctx->gi_w_counters,
+                          &ctx->gi_samples, &ctx->gi_history, &ctx->gi_hist_state,
+                          &ctx->rx_keys, &ctx->rx_flags, &ctx->rx_slot, &ctx->rx_lod, &ctx->rx_ord,
+                          &ctx->rx_seq, &ctx->rx_dir, &ctx->rx_tickets, &ctx->rx_dep_val,
+                          &ctx->rx_sort, &ctx->rx_scan, &ctx->rx_counters};
     for (lux_buffer* b : bufs) lux_release(*b);
     for (hipEvent_t e : ctx->ev_phase)
         if (e) (void)hipEventDestroy(e);

@@ -729,6 +729,49 @@ __device__ void pack_probe(const GiArgs& a, uint32_t probe, float4* smp, float3*
     a.probes[36ull * probe + t] = out;
 }
 
+// splitLodBlendForParameters (shaders/common.slang:431-461).  floor(log2(x))
+// of x >= 1 comes from the exponent field (exact); exp2 of an integer is a
+// power of two; max(lodScale, 1) maps NaN to 1 (IEEE maxNum).
+__device__ void lod_select(const lux_lod_view& v, const float* pos, uint32_t& low, uint32_t& high,
+                           float& blend) {
+    const float* pp = v.clip_row3;
+    bool perspective = lux::isfinite_bits(pp[0]) && lux::isfinite_bits(pp[1]) && lux::isfinite_bits(pp[2]) &&
+                       ((pp[0] * pp[0] + pp[1] * pp[1]) + pp[2] * pp[2]) > 1e-12f;
+    const float* vp = v.clip_row1;
+    float focal = 0.5f * (float)v.viewport_height * sqrtf((vp[0] * vp[0] + vp[1] * vp[1]) + vp[2] * vp[2]);
+    uint32_t target = v.target_pixels > 1u ? v.target_pixels : 1u;
+    float ts = v.probe_spacing;
+    float scale;
+    if (perspective) {
+        float ox = fabsf(pos[0] - v.camera_position[0]);
+        float oy = fabsf(pos[1] - v.camera_position[1]);
+        float oz = fabsf(pos[2] - v.camera_position[2]);
+        float m = oz > oy ? oz : oy;
+        float radius = m > ox ? m : ox;
+        float bd = focal * ts / (float)target;
+        float base = ts > bd ? ts : bd;
+        scale = radius / base;
+    } else {
+        float d = focal * ts;
+        float den = 1e-20f > d ? 1e-20f : d;
+        scale = (float)target / den;
+    }
+    float m = scale >= 1.0f ? scale : 1.0f;
+    uint32_t e = ((__float_as_uint(m) >> 23) & 0xffu) - 127u;
+    uint32_t lo = e < 24u ? e : 24u;
+    low = lo;
+    high = lo;
+    blend = 0.0f;
+    if (lo < 24u) {
+        float next = pow2i((int)lo + 1);
+        float start = 0.9f * next;
+        if (scale >= start && scale < next) {
+            high = lo + 1u;
+            blend = sat((scale - start) / (next - start));
+        }
+    }
+}
+
 // Kernel arguments of a configured context (host).
 inline GiArgs gi_args(lux_ctx* ctx) {
     GiArgs a;

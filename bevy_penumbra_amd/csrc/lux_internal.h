@@ -81,6 +81,14 @@ struct lux_ctx {
     lux_buffer gi_samples, gi_history, gi_hist_state;
     uint32_t gi_dirty_n = 0;                 // dirty boxes pending on the device
     float gi_emit_lo[3] = {}, gi_emit_hi[3] = {};  // emitter bounds (world)
+    // receiver-driven updates (lux_gi_update_receivers, state on the first
+    // LOD's context): descriptor keys / flags / directory slots, per
+    // (variant, receiver) LOD, ordinal and sequence, the probe directory
+    // (owner, canonical, count, offset per level), receiver tickets, deposit
+    // values (their keys live in the sort scratch), scan scratch, counters
+    lux_buffer rx_keys, rx_flags, rx_slot, rx_lod, rx_ord, rx_seq, rx_dir, rx_tickets;
+    lux_buffer rx_dep_val, rx_sort, rx_scan, rx_counters;
+    uint32_t rx_count = 0, rx_levels = 0, rx_cap = 0, rx_tickets_n = 0;
 
     // trace tuning
     // traversal launch configuration (defaults: the autotuner's choice on the
@@ -128,3 +136,5 @@ int lux_surrogate_forward_into(lux_surrogate* s, hipStream_t st, const float* qu
                                const float** Y);
 // GI state check of a context (configured, BVH built, emitter table current).
 int lux_gi_check_internal(lux_ctx* ctx);
+// History blend (when on), merge and pack of a context's intervals on stream st.
+int lux_gi_finish_internal(lux_ctx* ctx, hipStream_t st);

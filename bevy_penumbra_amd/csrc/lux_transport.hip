@@ -547,49 +547,6 @@ __device__ __forceinline__ float4 resolve_combine(float3 lv, float lc, float3 hv
     return make_float4(res.x, res.y, res.z, supported ? 1.0f + 0.25f * coverage : -1.0f);
 }
 
-// splitLodBlendForParameters (shaders/common.slang:431-461).  floor(log2(x))
-// of x >= 1 comes from the exponent field (exact); exp2 of an integer is a
-// power of two; max(lodScale, 1) maps NaN to 1 (IEEE maxNum).
-__device__ void lod_select(const lux_lod_view& v, const float* pos, uint32_t& low, uint32_t& high,
-                           float& blend) {
-    const float* pp = v.clip_row3;
-    bool perspective = lux::isfinite_bits(pp[0]) && lux::isfinite_bits(pp[1]) && lux::isfinite_bits(pp[2]) &&
-                       ((pp[0] * pp[0] + pp[1] * pp[1]) + pp[2] * pp[2]) > 1e-12f;
-    const float* vp = v.clip_row1;
-    float focal = 0.5f * (float)v.viewport_height * sqrtf((vp[0] * vp[0] + vp[1] * vp[1]) + vp[2] * vp[2]);
-    uint32_t target = v.target_pixels > 1u ? v.target_pixels : 1u;
-    float ts = v.probe_spacing;
-    float scale;
-    if (perspective) {
-        float ox = fabsf(pos[0] - v.camera_position[0]);
-        float oy = fabsf(pos[1] - v.camera_position[1]);
-        float oz = fabsf(pos[2] - v.camera_position[2]);
-        float m = oz > oy ? oz : oy;
-        float radius = m > ox ? m : ox;
-        float bd = focal * ts / (float)target;
-        float base = ts > bd ? ts : bd;
-        scale = radius / base;
-    } else {
-        float d = focal * ts;
-        float den = 1e-20f > d ? 1e-20f : d;
-        scale = (float)target / den;
-    }
-    float m = scale >= 1.0f ? scale : 1.0f;
-    uint32_t e = ((__float_as_uint(m) >> 23) & 0xffu) - 127u;
-    uint32_t lo = e < 24u ? e : 24u;
-    low = lo;
-    high = lo;
-    blend = 0.0f;
-    if (lo < 24u) {
-        float next = pow2i((int)lo + 1);
-        float start = 0.9f * next;
-        if (scale >= start && scale < next) {
-            high = lo + 1u;
-            blend = sat((scale - start) / (next - start));
-        }
-    }
-}
-
 __device__ __forceinline__ ResolveGrid grid_of(const GiArgs& a) {
     ResolveGrid g;
     g.probes = a.probes;
@@ -1107,12 +1064,13 @@ extern "C" int lux_gi_trace(lux_ctx* ctx) {
 }
 
 namespace {
-int merge_levels(lux_ctx* ctx, int top) {
+int merge_levels(lux_ctx* ctx, int top, hipStream_t st = nullptr) {
+    if (!st) st = ctx->stream;
     GiArgs a = gi_args(ctx);
     for (int L = top; L >= 0; --L) {
         uint64_t count = ctx->gi_offset[L + 1] - ctx->gi_offset[L];
         uint32_t blocks = (uint32_t)((count + kWg - 1) / kWg);
-        hipLaunchKernelGGL(k_gi_merge, dim3(blocks), dim3(kWg), 0, ctx->stream, a, (uint32_t)L);
+        hipLaunchKernelGGL(k_gi_merge, dim3(blocks), dim3(kWg), 0, st, a, (uint32_t)L);
     }
     LUX_HIP(ctx, hipGetLastError());
     return LUX_OK;
@@ -1153,11 +1111,12 @@ extern "C" int lux_gi_query_counters(lux_ctx* ctx, uint32_t* attempts, uint32_t*
 
 namespace {
 // Temporal history pass over the first `count` intervals (the traced ones).
-int history_pass(lux_ctx* ctx, uint64_t count) {
+int history_pass(lux_ctx* ctx, uint64_t count, hipStream_t st = nullptr) {
     if (!ctx->gi_hist_on || count == 0) return LUX_OK;
+    if (!st) st = ctx->stream;
     GiArgs a = gi_args(ctx);
-    hipLaunchKernelGGL(k_gi_history, dim3((uint32_t)((count + kWg - 1) / kWg)), dim3(kWg), 0, ctx->stream, a, count);
-    hipLaunchKernelGGL(k_gi_history_done, dim3(1), dim3(64), 0, ctx->stream, a.hist_state);
+    hipLaunchKernelGGL(k_gi_history, dim3((uint32_t)((count + kWg - 1) / kWg)), dim3(kWg), 0, st, a, count);
+    hipLaunchKernelGGL(k_gi_history_done, dim3(1), dim3(64), 0, st, a.hist_state);
     LUX_HIP(ctx, hipGetLastError());
     ctx->gi_dirty_n = 0;
     return LUX_OK;
@@ -1239,6 +1198,17 @@ extern "C" int lux_gi_update(lux_ctx* ctx, float* ms) {
         LUX_HIP(ctx, hipEventSynchronize(ctx->ev1));
         LUX_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
     }
+    return LUX_OK;
+}
+
+// History blend (when on), merge and pack of a context's intervals on a
+// given stream: the end of an update whose intervals another pass wrote.
+int lux_gi_finish_internal(lux_ctx* ctx, hipStream_t st) {
+    if (int rc = history_pass(ctx, ctx->gi_total, st)) return rc;
+    if (int rc = merge_levels(ctx, (int)ctx->gi.levels - 1, st)) return rc;
+    GiArgs a = gi_args(ctx);
+    hipLaunchKernelGGL(k_gi_pack, dim3(ctx->gi_probes0), dim3(64), 0, st, a);
+    LUX_HIP(ctx, hipGetLastError());
     return LUX_OK;
 }
 

@@ -290,4 +290,67 @@ int lux_radiance_cascades_history_params(const lux_radiance_cascades* in, lux_gi
     return LUX_OK;
 }
 
+// resolution() (src/render.rs:5350-5380): the receiver (candidate) grid of
+// a viewport -- the smallest divisor >= the configured one whose grid holds at
+// most max_receiver_cells receivers (131072 when 0).
+int lux_radiance_cascades_receiver_grid(const lux_radiance_cascades* in, uint32_t viewport_width,
+                                        uint32_t viewport_height, uint32_t* width, uint32_t* height) {
+    if (!in || !width || !height) return LUX_ERR_INVALID;
+    auto size_at = [&](uint32_t d, uint32_t& w, uint32_t& h) {
+        w = std::max<uint32_t>(1u, viewport_width / d + (viewport_width % d != 0));
+        h = std::max<uint32_t>(1u, viewport_height / d + (viewport_height % d != 0));
+    };
+    const uint64_t max_receivers = in->sampling[1] == 0 ? 131072u : in->sampling[1];
+    const uint32_t divisor = in->sampling[0] != 0 ? in->sampling[0] : in->resolution_divisor;
+    uint32_t low = std::max<uint32_t>(divisor, 1u);
+    uint32_t high = std::max(std::max(viewport_width, viewport_height), low);
+    while (low < high) {
+        uint32_t mid = low + (high - low) / 2u, w, h;
+        size_at(mid, w, h);
+        if ((uint64_t)w * h > max_receivers) low = mid + 1u;
+        else high = mid;
+    }
+    size_at(low, *width, *height);
+    return LUX_OK;
+}
+
+// selected_update_budget / runtime_update_budget (src/render.rs:243-255):
+// max_updates_per_frame (max_rays when 0) capped by max_rays, at least 1,
+// then by the runtime cap when has_cap (a cap of 0 counts as 1).
+int lux_radiance_cascades_update_budget(const lux_radiance_cascades* in, uint32_t has_cap, uint32_t runtime_cap,
+                                        uint32_t* budget) {
+    if (!in || !budget) return LUX_ERR_INVALID;
+    const uint32_t ceiling = std::max<uint32_t>(in->max_rays, 1u);
+    const uint32_t requested = in->sampling[2] == 0 ? ceiling : in->sampling[2];
+    uint32_t b = std::max<uint32_t>(std::min(requested, ceiling), 1u);
+    if (has_cap) b = std::min(b, std::max<uint32_t>(runtime_cap, 1u));
+    *budget = b;
+    return LUX_OK;
+}
+
+// CandidateSchedule (src/render.rs:5172-5211): the fair window's cursor over
+// the stable rank permutation, advanced only after a submitted update.
+int lux_candidate_schedule_phase(lux_candidate_schedule* s, uint32_t width, uint32_t height, uint32_t count,
+                                 uint32_t budget, uint32_t* cursor, uint32_t* quota) {
+    if (!s || !cursor || !quota) return LUX_ERR_INVALID;
+    if (s->width != width || s->height != height || s->count != count) {
+        s->width = width;
+        s->height = height;
+        s->count = count;
+        s->cursor = 0;
+    }
+    s->cursor = count > 0 ? s->cursor % count : 0u;
+    s->pending = std::min(budget, count);
+    *cursor = s->cursor;
+    *quota = s->pending;
+    return LUX_OK;
+}
+
+int lux_candidate_schedule_advance(lux_candidate_schedule* s, uint32_t consume_fair_cursor) {
+    if (!s) return LUX_ERR_INVALID;
+    if (consume_fair_cursor && s->count > 0)
+        s->cursor = (uint32_t)(((uint64_t)s->cursor + s->pending) % s->count);
+    return LUX_OK;
+}
+
 }  // extern "C"

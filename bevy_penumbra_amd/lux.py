@@ -105,6 +105,33 @@ class RadianceCascades(C.Structure):
             raise LuxError(f"lux_radiance_cascades_gi_params: {rc}")
         return p
 
+    def receiver_grid(self, viewport_width: int, viewport_height: int):
+        w, h = C.c_uint32(), C.c_uint32()
+        load_library().lux_radiance_cascades_receiver_grid(C.byref(self), viewport_width, viewport_height,
+                                                           C.byref(w), C.byref(h))
+        return int(w.value), int(h.value)
+
+    def update_budget(self, runtime_cap=None) -> int:
+        b = C.c_uint32()
+        load_library().lux_radiance_cascades_update_budget(C.byref(self), runtime_cap is not None,
+                                                           runtime_cap or 0, C.byref(b))
+        return int(b.value)
+
+
+class CandidateSchedule(C.Structure):
+    """lux_candidate_schedule: the fair window's cursor (CandidateSchedule)."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("count", C.c_uint32),
+                ("cursor", C.c_uint32), ("pending", C.c_uint32)]
+
+    def phase_for(self, width: int, height: int, count: int, budget: int):
+        c, q = C.c_uint32(), C.c_uint32()
+        load_library().lux_candidate_schedule_phase(C.byref(self), width, height, count, budget, C.byref(c),
+                                                    C.byref(q))
+        return int(c.value), int(q.value)
+
+    def advance_after_successful_update(self, consume_fair_cursor: bool = True):
+        load_library().lux_candidate_schedule_advance(C.byref(self), int(consume_fair_cursor))
+
 
 class SurrogateParams(C.Structure):
     """lux_surrogate_params (include/lux.h)."""
@@ -199,6 +226,64 @@ def world_resolve_lods(contexts, view: LodView, receivers, n: int, out):
     arr = (C.c_void_p * len(contexts))(*[c.h for c in contexts])
     contexts[0]._check(lib.lux_world_resolve_lods(arr, len(contexts), C.byref(view), _ptr(receivers), n, _ptr(out)),
                        "lux_world_resolve_lods")
+
+
+class ReceiverParams(C.Structure):
+    """lux_receiver_params (include/lux.h)."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("budget", C.c_uint32),
+                ("fair_cursor", C.c_uint32), ("fair_quota", C.c_uint32), ("frame", C.c_uint32),
+                ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class ReceiverStats(C.Structure):
+    """lux_receiver_stats (include/lux.h)."""
+    _fields_ = [("receivers", C.c_uint32), ("tickets", C.c_uint32), ("rays", C.c_uint32),
+                ("unsupported", C.c_uint32), ("deposits", C.c_uint32), ("outside", C.c_uint32),
+                ("root_rays", C.c_uint32), ("overflow", C.c_uint32), ("ms", C.c_float),
+                ("reserved", C.c_uint32 * 3)]
+
+
+def receiver_params(width: int, height: int, budget: int = 0, fair_cursor: int = 0, fair_quota=None,
+                    frame: int = 0, disabled: bool = False) -> ReceiverParams:
+    rp = ReceiverParams()
+    rp.width, rp.height, rp.budget = width, height, budget
+    rp.fair_cursor = fair_cursor
+    rp.fair_quota = min(budget, width * height) if fair_quota is None else fair_quota
+    rp.frame = frame
+    rp.flags = 1 if disabled else 0
+    return rp
+
+
+def update_receivers(contexts, view, receivers, params: ReceiverParams) -> ReceiverStats:
+    """lux_gi_update_receivers over the LOD pyramid contexts[k] = LOD k; view
+    None: LOD 0 for every receiver."""
+    lib = contexts[0].lib
+    arr = (C.c_void_p * len(contexts))(*[c.h for c in contexts])
+    st = ReceiverStats()
+    contexts[0]._check(lib.lux_gi_update_receivers(arr, len(contexts), None if view is None else C.byref(view),
+                                                   _ptr(receivers), C.byref(params), C.byref(st)),
+                       "lux_gi_update_receivers")
+    return st
+
+
+def receivers_download(ctx, count: int, levels: int) -> dict:
+    """The last receiver update's selection on contexts[0] (see
+    lux_receivers_download)."""
+    nd = 2 * count * levels
+    keys = np.zeros((nd, 4), np.int32)
+    flags = np.zeros(nd, np.uint32)
+    canon = np.zeros(nd, np.uint32)
+    cnt = np.zeros(nd, np.uint32)
+    off = np.zeros(nd, np.uint32)
+    lods = np.zeros((2, count), np.uint32)
+    seq = np.zeros((2, count), np.uint32)
+    tickets = np.zeros(max(count, 1), np.uint32)
+    nt = C.c_uint32()
+    ctx._check(ctx.lib.lux_receivers_download(ctx.h, keys.ctypes.data, flags.ctypes.data, canon.ctypes.data,
+                                              cnt.ctypes.data, off.ctypes.data, lods.ctypes.data, seq.ctypes.data,
+                                              tickets.ctypes.data, C.byref(nt)), "lux_receivers_download")
+    return dict(keys=keys, flags=flags, canonical=canon, count=cnt, offset=off, lods=lods, sequence=seq,
+                tickets=tickets[:nt.value])
 
 
 LUX_SAMPLES_TRACE = 0
@@ -302,6 +387,13 @@ def load_library() -> C.CDLL:
         "lux_gi_invalidate_history": (i32, [vp, vp, vp]),
         "lux_gi_history_arrays": (i32, [vp, vp, vp]),
         "lux_radiance_cascades_history_params": (i32, [vp, vp]),
+        "lux_radiance_cascades_receiver_grid": (i32, [vp, u32, u32, vp, vp]),
+        "lux_radiance_cascades_update_budget": (i32, [vp, u32, u32, vp]),
+        "lux_candidate_schedule_phase": (i32, [vp, u32, u32, u32, u32, vp, vp]),
+        "lux_candidate_schedule_advance": (i32, [vp, u32]),
+        "lux_gi_update_receivers": (i32, [vp, u32, vp, vp, vp, vp]),
+        "lux_receivers_download": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "lux_gi_samples_download": (i32, [vp, vp]),
         "lux_batch_create": (i32, [vp, u32, C.POINTER(vp)]),
         "lux_batch_destroy": (None, [vp]),
         "lux_batch_last_error": (C.c_char_p, [vp]),
@@ -593,6 +685,12 @@ class Context:
         l = C.c_uint32()
         self._check(self.lib.lux_gi_sizes(self.h, C.byref(n), C.byref(p), C.byref(l)), "lux_gi_sizes")
         return int(n.value), int(p.value), int(l.value)
+
+    def gi_samples_download(self) -> np.ndarray:
+        n, _, _ = self.gi_sizes()
+        out = np.zeros(n, np.uint32)
+        self._check(self.lib.lux_gi_samples_download(self.h, out.ctypes.data), "lux_gi_samples_download")
+        return out
 
     def gi_download(self):
         n, p, l = self.gi_sizes()

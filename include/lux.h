@@ -343,6 +343,56 @@ int lux_gi_invalidate_history(lux_ctx* ctx, const float* lo, const float* hi);
  * supported-ray counts of the last update; NULL when history is off. */
 int lux_gi_history_arrays(lux_ctx* ctx, float** history, uint32_t** samples);
 
+/* ---- receiver-driven updates (the sparse Split path, shaders/split.slang:
+ *      567-1476, src/render.rs:9640-9794 for a frame's dispatches) ----
+ * Receivers (device float[6 * width * height], row-major: position, normal;
+ * a zero normal marks no surface) choose the probes and rays of an update:
+ * seed / rank / parents / prefix / generateDirections, then one jittered ray
+ * per selected receiver, deposits per level up to the terminating one, the
+ * mean per interval, and each LOD grid's history blend, merge and pack.
+ * lods[k] is the dense grid of LOD k (spacing and interval length x 2^k,
+ * origins on the coarsest level's cells, same levels and directions); a
+ * deposit outside the grids is dropped.  Orders the reference leaves to its
+ * atomics are the sequential ones (lowest descriptor index canonical,
+ * ordinals and tickets in receiver order, roots by canonical index). */
+typedef struct {
+    uint32_t width, height; /* receiver grid (params.size.xy) */
+    uint32_t budget;        /* sampling.x: receivers traced per update, 0 = all */
+    uint32_t fair_cursor;   /* ownerSequence.w: fair window start (lux_candidate_schedule_phase) */
+    uint32_t fair_quota;    /* ownerSequence.z: window length */
+    uint32_t frame;         /* params.frame.x: LOD phase, direction jitter, NEE sequence */
+    uint32_t flags;         /* 1: updates disabled (no rays) */
+    uint32_t reserved;
+} lux_receiver_params;
+
+typedef struct {
+    uint32_t receivers;   /* eligible receivers */
+    uint32_t tickets;     /* receivers tracing this update */
+    uint32_t rays;        /* rays traced */
+    uint32_t unsupported; /* rays without a sample (traversal or visibility unsupported) */
+    uint32_t deposits;    /* deposits kept in a grid */
+    uint32_t outside;     /* deposits outside every grid */
+    uint32_t root_rays;   /* ray total of the prefix roots (= rays selected) */
+    uint32_t overflow;    /* descriptors the probe directory could not hold */
+    float ms;             /* device time of the update */
+    uint32_t reserved[3];
+} lux_receiver_stats;
+
+int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count, const lux_lod_view* view,
+                            const float* receivers, const lux_receiver_params* params,
+                            lux_receiver_stats* stats /* nullable */);
+/* The last receiver update's selection (on lods[0]'s context; any pointer may
+ * be NULL): per descriptor (2 * count * levels, (variant * count + receiver)
+ * * levels + level) its key (int32[4]), flags (1 seeded, 2 ray), canonical
+ * descriptor, probe ray count and offset; per (variant, receiver) its LOD and
+ * sequence (0xffffffff: no ray); the tickets and their number. */
+int lux_receivers_download(lux_ctx* ctx, int32_t* keys, uint32_t* flags, uint32_t* canonical,
+                           uint32_t* probe_count, uint32_t* probe_offset, uint32_t* lods, uint32_t* sequence,
+                           uint32_t* tickets, uint32_t* n_tickets);
+/* Samples per interval of the last update that counted them (receiver
+ * updates always; plain updates with history on). */
+int lux_gi_samples_download(lux_ctx* ctx, uint32_t* samples);
+
 /* Traversal query counters of the current update: attempts and admitted
  * (admitted <= the budget; both count every interval and NEE query).
  * lux_gi_update and lux_gi_update_surrogate reset them at their start, as

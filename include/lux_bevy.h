@@ -60,6 +60,25 @@ int lux_radiance_cascades_gi_params(const lux_radiance_cascades* in, const float
  * renderer's intervalHistory.y = 1, src/render.rs:8455-8462). */
 int lux_radiance_cascades_history_params(const lux_radiance_cascades* in, lux_gi_history_params* out);
 
+/* resolution() (src/render.rs:5350-5380): the receiver grid (candidate
+ * count = width * height) of a viewport. */
+int lux_radiance_cascades_receiver_grid(const lux_radiance_cascades* in, uint32_t viewport_width,
+                                        uint32_t viewport_height, uint32_t* width, uint32_t* height);
+/* runtime_update_budget (src/render.rs:243-255): receivers traced per
+ * update; has_cap = 0 is the reference's None. */
+int lux_radiance_cascades_update_budget(const lux_radiance_cascades* in, uint32_t has_cap, uint32_t runtime_cap,
+                                        uint32_t* budget);
+
+/* CandidateSchedule (src/render.rs:5172-5211): zero-initialise, then per
+ * update phase() -> lux_receiver_params.fair_cursor / fair_quota and, after
+ * a submitted update, advance(1). */
+typedef struct lux_candidate_schedule {
+    uint32_t width, height, count, cursor, pending;
+} lux_candidate_schedule;
+int lux_candidate_schedule_phase(lux_candidate_schedule* s, uint32_t width, uint32_t height, uint32_t count,
+                                 uint32_t budget, uint32_t* cursor, uint32_t* quota);
+int lux_candidate_schedule_advance(lux_candidate_schedule* s, uint32_t consume_fair_cursor);
+
 #ifdef __cplusplus
 }
 #endif

@@ -250,6 +250,48 @@ void orc_world_resolve_lods_f64(const orc_gi_params* const* grids, const double*
                                 double* out);
 void orc_world_resolve_f64(const orc_gi_params* p, const double* probes, const float* receivers,
                            uint32_t n, double* out);
+/* ---- receiver-driven probe selection (orc_receivers.c) and the receiver
+ * trace (orc_transport.c) ---- */
+#define ORC_RX_DISABLED 1u /* ownerSequence.w bit 30: updates disabled */
+#define ORC_RX_VALID 1u    /* descriptor flags: seeded */
+#define ORC_RX_RAY 2u      /* descriptor flags: padding.x ray marker */
+typedef struct {
+    uint32_t width, height; /* receiver grid (params.size.xy); count = width * height */
+    uint32_t budget;        /* sampling.x: 0 = every receiver */
+    uint32_t fair_cursor;   /* ownerSequence.w low 26 bits: fair window start (rank) */
+    uint32_t fair_quota;    /* ownerSequence.z */
+    uint32_t frame;         /* params.frame.x */
+    uint32_t flags;         /* ORC_RX_DISABLED */
+    uint32_t reserved;
+} orc_receiver_params;
+uint32_t orc_candidate_rank(uint32_t sample, uint32_t width, uint32_t height, uint32_t count);
+uint32_t orc_candidate_from_rank(uint32_t rank, uint32_t width, uint32_t height, uint32_t count);
+int orc_candidate_selected(uint32_t sample, const orc_receiver_params* rp);
+int orc_rx_surface(const float* receiver, float pos[3], float n[3]);
+void orc_rx_cell(const float pos[3], float base_spacing, uint32_t level, uint32_t lod, int32_t key[4]);
+uint32_t orc_rx_directory_capacity(uint32_t count);
+/* Descriptors (variant, receiver, level) -> keys int32[4], flags, and the
+ * LOD of each (variant, receiver). view may be NULL (LOD 0 everywhere). */
+void orc_rx_seed(const float* receivers, const orc_receiver_params* rp, const orc_lod_view* view,
+                 float base_spacing, uint32_t levels, int32_t* keys, uint32_t* flags, uint32_t* lods);
+/* rank/parents/prefix/generateDirections over 2 * count * levels descriptors.
+ * Per descriptor: canonical index (EMPTY if not seeded), its probe's ray
+ * count and offset; per (variant, receiver): sequence (EMPTY: no ray); the
+ * ticket list.  Returns the root total. */
+uint32_t orc_rx_select(uint32_t count, uint32_t levels, const int32_t* keys, const uint32_t* flags,
+                       uint32_t* canonical, uint32_t* probe_count, uint32_t* probe_offset, uint32_t* sequence,
+                       uint32_t* tickets, uint32_t* n_tickets);
+/* directionIndex (common.slang:384-394) over nphi x ny bins. */
+uint32_t orc_direction_index(const float d[3], uint32_t nphi, uint32_t ny);
+/* Receiver trace + deposit + reduce into the dense grids of a LOD pyramid
+ * (grid k = LOD k).  stats: [0] rays traced, [1] unsupported, [2] deposits
+ * kept, [3] deposits outside the grids. */
+void orc_rx_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights, const orc_material* mats,
+                  uint32_t n_mats, const orc_gi_params* const* grids, uint32_t lod_count, const float* receivers,
+                  uint32_t count, const uint32_t* lods, const uint32_t* sequence, const uint32_t* tickets,
+                  uint32_t n_tickets, uint32_t frame, float* const* intervals, uint32_t* const* samples,
+                  uint32_t stats[4]);
+
 /* Shading in f64 (FP64 twin): same hits and visibility, f64 radiance. */
 void orc_gi_trace_f64(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                       const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,

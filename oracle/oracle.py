@@ -273,6 +273,15 @@ def _tsig():
         ("orc_world_resolve_lods_f64", None, [vp, vp, u32, vp, vp, u32, vp]),
         ("orc_gi_level_queries", None, [vp, u32, vp]),
         ("orc_gi_sample_index", C.c_uint64, [u32, u32, C.c_uint64]),
+        ("orc_candidate_rank", u32, [u32, u32, u32, u32]),
+        ("orc_candidate_from_rank", u32, [u32, u32, u32, u32]),
+        ("orc_candidate_selected", i32, [u32, vp]),
+        ("orc_rx_cell", None, [vp, C.c_float, u32, u32, vp]),
+        ("orc_rx_directory_capacity", u32, [u32]),
+        ("orc_rx_seed", None, [vp, vp, vp, C.c_float, u32, vp, vp, vp]),
+        ("orc_rx_select", u32, [u32, u32, vp, vp, vp, vp, vp, vp, vp, vp]),
+        ("orc_direction_index", u32, [vp, u32, u32]),
+        ("orc_rx_trace", None, [vp, vp, u32, vp, u32, vp, u32, vp, u32, vp, vp, vp, u32, u32, vp, vp, vp]),
     ]:
         fn = getattr(lib, name)
         fn.restype = res
@@ -472,3 +481,106 @@ def world_resolve(p: GiParams, probes: np.ndarray, receivers: np.ndarray) -> np.
     (lib.orc_world_resolve_f64 if f64 else lib.orc_world_resolve)(
         C.byref(p), pr.ctypes.data, rc.ctypes.data, rc.shape[0], out.ctypes.data)
     return out
+
+
+# ---------------- receiver-driven probe selection ----------------
+EMPTY = 0xFFFFFFFF
+RX_VALID, RX_RAY = 1, 2
+
+
+class ReceiverParams(C.Structure):
+    """Mirror of orc_receiver_params / lux_receiver_params."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("budget", C.c_uint32),
+                ("fair_cursor", C.c_uint32), ("fair_quota", C.c_uint32), ("frame", C.c_uint32),
+                ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+def receiver_params(width, height, budget=0, fair_cursor=0, fair_quota=None, frame=0, disabled=False):
+    rp = ReceiverParams()
+    rp.width, rp.height, rp.budget = width, height, budget
+    rp.fair_cursor = fair_cursor
+    rp.fair_quota = min(budget, width * height) if fair_quota is None else fair_quota
+    rp.frame = frame
+    rp.flags = 1 if disabled else 0
+    return rp
+
+
+def candidate_rank(sample: int, width: int, height: int, count: int) -> int:
+    return int(_tsig().orc_candidate_rank(sample, width, height, count))
+
+
+def candidate_from_rank(rank: int, width: int, height: int, count: int) -> int:
+    return int(_tsig().orc_candidate_from_rank(rank, width, height, count))
+
+
+def candidate_selected(sample: int, rp: ReceiverParams) -> bool:
+    return bool(_tsig().orc_candidate_selected(sample, C.byref(rp)))
+
+
+def rx_cell(pos, base_spacing: float, level: int, lod: int) -> np.ndarray:
+    k = np.zeros(4, np.int32)
+    p = np.ascontiguousarray(pos, np.float32)
+    _tsig().orc_rx_cell(p.ctypes.data, base_spacing, level, lod, k.ctypes.data)
+    return k
+
+
+def direction_index(d, nphi: int, ny: int) -> int:
+    v = np.ascontiguousarray(d, np.float32)
+    return int(_tsig().orc_direction_index(v.ctypes.data, nphi, ny))
+
+
+def rx_seed(receivers: np.ndarray, rp: ReceiverParams, view, base_spacing: float, levels: int):
+    """Descriptors of every (variant, receiver, level): keys (2*count*levels, 4)
+    int32, flags uint32, and the LOD of each (variant, receiver) (2, count)."""
+    count = rp.width * rp.height
+    rc = np.ascontiguousarray(receivers, np.float32)
+    assert rc.shape == (count, 6)
+    nd = 2 * count * levels
+    keys = np.zeros((nd, 4), np.int32)
+    flags = np.zeros(nd, np.uint32)
+    lods = np.zeros((2, count), np.uint32)
+    _tsig().orc_rx_seed(rc.ctypes.data, C.byref(rp), None if view is None else C.byref(view), base_spacing,
+                        levels, keys.ctypes.data, flags.ctypes.data, lods.ctypes.data)
+    return keys, flags, lods
+
+
+def rx_select(count: int, levels: int, keys: np.ndarray, flags: np.ndarray) -> dict:
+    keys = np.ascontiguousarray(keys, np.int32)
+    flags = np.ascontiguousarray(flags, np.uint32)
+    nd = 2 * count * levels
+    assert keys.shape == (nd, 4) and flags.shape == (nd,)
+    canonical = np.zeros(nd, np.uint32)
+    pcount = np.zeros(nd, np.uint32)
+    poffset = np.zeros(nd, np.uint32)
+    seq = np.zeros((2, count), np.uint32)
+    tickets = np.zeros(max(count, 1), np.uint32)
+    nt = C.c_uint32()
+    root = _tsig().orc_rx_select(count, levels, keys.ctypes.data, flags.ctypes.data, canonical.ctypes.data,
+                                 pcount.ctypes.data, poffset.ctypes.data, seq.ctypes.data, tickets.ctypes.data,
+                                 C.byref(nt))
+    return dict(canonical=canonical, count=pcount, offset=poffset, sequence=seq, tickets=tickets[:nt.value],
+                root=int(root))
+
+
+def rx_trace(built: BuiltScene, lights: np.ndarray, materials: np.ndarray, grids, receivers: np.ndarray,
+             lods: np.ndarray, sequence: np.ndarray, tickets: np.ndarray, frame: int):
+    """Receiver trace + deposit + reduce into the grids of a LOD pyramid:
+    (intervals per LOD, samples per LOD, stats[rays, unsupported, kept, outside])."""
+    lib = _tsig()
+    lights = np.ascontiguousarray(lights, np.float32)
+    mats = np.ascontiguousarray(materials, np.float32)
+    rc = np.ascontiguousarray(receivers, np.float32)
+    count = rc.shape[0]
+    lods = np.ascontiguousarray(lods, np.uint32)
+    seq = np.ascontiguousarray(sequence, np.uint32)
+    tk = np.ascontiguousarray(tickets, np.uint32)
+    ivs = [np.zeros((total_intervals(g), 4), np.float32) for g in grids]
+    sms = [np.zeros(total_intervals(g), np.uint32) for g in grids]
+    gp = (C.c_void_p * len(grids))(*[C.cast(C.byref(g), C.c_void_p) for g in grids])
+    ip = (C.c_void_p * len(grids))(*[a.ctypes.data for a in ivs])
+    sp = (C.c_void_p * len(grids))(*[a.ctypes.data for a in sms])
+    st = np.zeros(4, np.uint32)
+    lib.orc_rx_trace(C.byref(built._s), lights.ctypes.data, lights.shape[0], mats.ctypes.data, mats.shape[0], gp,
+                     len(grids), rc.ctypes.data, count, lods.ctypes.data, seq.ctypes.data, tk.ctypes.data,
+                     tk.shape[0], frame, ip, sp, st.ctypes.data)
+    return ivs, sms, st

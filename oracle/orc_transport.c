@@ -571,3 +571,144 @@ uint64_t orc_gi_sample_index(uint32_t seed, uint32_t i, uint64_t count) {
     uint32_t h = transport_hash(i + transport_hash(seed ^ 0x3c6ef372u));
     return ((uint64_t)h * count) >> 32;
 }
+
+/* ---------------- receiver trace (traceTransport / deposit / reduce) ---------------- */
+/* atan of |t| <= tan(pi/8): Taylor series to t^47 in f64 (+ and * only). */
+static double atan_small(double t) {
+    double t2 = t * t;
+    double s = 1.0 / 47.0;
+    for (int k = 22; k >= 0; --k) s = s * t2 + ((k & 1) ? -1.0 : 1.0) / (double)(2 * k + 1);
+    return s * t;
+}
+
+/* atan2 from a fixed f64 series (the same arithmetic on CPU and GPU, so the
+ * azimuth bin is deterministic); IEEE atan2's signed-zero conventions. */
+static double atan2_fixed(double y, double x) {
+    double ax = fabs(x), ay = fabs(y);
+    double mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    double a = mx > 0.0 ? mn / mx : 0.0;
+    double r = a > 0.41421356237309503 ? 0.7853981633974483 + atan_small((a - 1.0) / (a + 1.0)) : atan_small(a);
+    if (ay > ax) r = 1.5707963267948966 - r;
+    if (signbit(x)) r = 3.141592653589793 - r;
+    if (signbit(y)) r = -r;
+    return r;
+}
+
+/* directionIndex (common.slang:384-394): encodeDirection's azimuth
+ * frac(atan2(z, x) / 2pi + 1) and saturate(y / 2 + 1/2), binned over
+ * nphi x ny (2 theta x theta in the reference). */
+uint32_t orc_direction_index(const float d[3], uint32_t nphi, uint32_t ny) {
+    float u = (float)(atan2_fixed((double)d[2], (double)d[0]) * 0.15915494309189535);
+    u = fracf_(u + 1.0f);
+    float v = d[1] * 0.5f + 0.5f;
+    v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+    uint32_t x = (uint32_t)(u * (float)nphi), y = (uint32_t)(v * (float)ny);
+    if (x > nphi - 1u) x = nphi - 1u;
+    if (y > ny - 1u) y = ny - 1u;
+    return x + y * nphi;
+}
+
+/* traceTransport (split.slang:1123-1163) for each ticket's selected
+ * variants, finishTransportVariant (:1189-1218) with this path's shading,
+ * deposit (:1056-1118: J / beta per level up to the terminating one) and
+ * reduce (:1386-1476: mean over the update's samples of a key), the keys
+ * (cell, direction bin, level, LOD) addressed in the dense grid of their
+ * LOD.  Sums run in ticket order. */
+void orc_rx_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights, const orc_material* mats,
+                  uint32_t n_mats, const orc_gi_params* const* grids, uint32_t lod_count, const float* receivers,
+                  uint32_t count, const uint32_t* lods, const uint32_t* sequence, const uint32_t* tickets,
+                  uint32_t n_tickets, uint32_t frame, float* const* intervals, uint32_t* const* samples,
+                  uint32_t stats[4]) {
+    orc_gi_params p0c = *grids[0];
+    p0c.frame = frame; /* params.frame.x drives the NEE sequence too */
+    const orc_gi_params* p0 = &p0c;
+    const uint32_t levels = p0->levels;
+    stats[0] = stats[1] = stats[2] = stats[3] = 0;
+    for (uint32_t k = 0; k < lod_count; ++k) {
+        uint64_t n = orc_gi_level_offset(grids[k], grids[k]->levels);
+        memset(intervals[k], 0, sizeof(float) * 4 * n);
+        memset(samples[k], 0, sizeof(uint32_t) * n);
+    }
+    float jit[2];
+    r2(transport_hash(frame ^ 0x5a17c9e3u), jit);
+    for (uint32_t t = 0; t < n_tickets; ++t) {
+        const uint32_t smp = tickets[t];
+        float pos[3], n[3];
+        if (!orc_rx_surface(receivers + 6ull * smp, pos, n)) continue;
+        for (uint32_t v = 0; v < 2; ++v) {
+            const uint32_t seq = sequence[(uint64_t)v * count + smp];
+            if (seq == ORC_EMPTY) continue;
+            const uint32_t lod = lods[(uint64_t)v * count + smp];
+            stats[0]++;
+            float q[2], d[3], o[3];
+            r2(seq + 1u, q);
+            decode_direction(fracf_((0.5f + q[0]) + jit[0]), fracf_((0.5f + q[1]) + jit[1]), d);
+            if (dot3f(d, n) < 0.0f) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; }
+            ray_offset(pos, n, p0->normal_bias, o);
+            orc_ray ray;
+            ray.ox = o[0]; ray.oy = o[1]; ray.oz = o[2];
+            ray.tmax = 1e20f;
+            ray.dx = d[0]; ray.dy = d[1]; ray.dz = d[2];
+            ray.flags = 0;
+            orc_hit h;
+            uint32_t ts = 0, is = 0;
+            orc_trace_ray(s, &ray, &h, NULL, &ts, &is);
+            if (h.flags & ORC_ERR_UNSUPPORTED) { stats[1]++; continue; }
+            float rad[3] = {0.0f, 0.0f, 0.0f};
+            const int hit = h.triangle != ORC_EMPTY;
+            if (hit && !shade_f32(s, lights, n_lights, mats, n_mats, p0, o, d, h.t, ts, is, rad)) {
+                stats[1]++;
+                continue;
+            }
+            const orc_gi_params* g = lod < lod_count ? grids[lod] : NULL;
+            const float lod_scale = ldexpf(1.0f, (int)(lod < 24u ? lod : 24u));
+            for (uint32_t L = 0; L < levels; ++L) {
+                float start = L == 0 ? 0.0f : interval_end(p0, L - 1u) * lod_scale;
+                float end = interval_end(p0, L) * lod_scale;
+                int terminal = p0->terminal_tail && L + 1u == levels;
+                int terminates = terminal || (hit && h.t > start && h.t <= end);
+                int32_t key[4];
+                orc_rx_cell(pos, p0->spacing, L, lod, key);
+                int kept = 0;
+                if (g) {
+                    uint32_t dims[3];
+                    orc_gi_level_dims(g, L, dims);
+                    float sl = g->spacing * ldexpf(1.0f, (int)L);
+                    int64_t idx[3];
+                    int inside = 1;
+                    for (int a = 0; a < 3; ++a) {
+                        idx[a] = (int64_t)key[a] - (int64_t)floorf(g->origin[a] / sl);
+                        inside = inside && idx[a] >= 0 && idx[a] < (int64_t)dims[a];
+                    }
+                    if (inside) {
+                        uint32_t D = orc_gi_level_directions(g, L);
+                        uint32_t dir = orc_direction_index(d, g->base_phi << L, g->base_y << L);
+                        uint64_t probe = (uint64_t)idx[0] + (uint64_t)dims[0] * ((uint64_t)idx[1] + (uint64_t)dims[1] * (uint64_t)idx[2]);
+                        uint64_t slot = orc_gi_level_offset(g, L) + probe * D + dir;
+                        float* iv = intervals[lod] + 4 * slot;
+                        for (int c = 0; c < 3; ++c) iv[c] = iv[c] + (terminates ? rad[c] : 0.0f);
+                        iv[3] = iv[3] + (terminates ? 0.0f : 1.0f);
+                        samples[lod][slot]++;
+                        kept = 1;
+                    }
+                }
+                stats[kept ? 2 : 3]++;
+                if (terminates) break;
+            }
+        }
+    }
+    for (uint32_t k = 0; k < lod_count; ++k) {
+        uint64_t n = orc_gi_level_offset(grids[k], grids[k]->levels);
+        for (uint64_t g = 0; g < n; ++g) {
+            float* iv = intervals[k] + 4 * g;
+            uint32_t c = samples[k][g];
+            if (c == 0) {
+                iv[0] = iv[1] = iv[2] = 0.0f;
+                iv[3] = -1.0f;
+            } else {
+                float fc = (float)c;
+                iv[0] = iv[0] / fc; iv[1] = iv[1] / fc; iv[2] = iv[2] / fc; iv[3] = iv[3] / fc;
+            }
+        }
+    }
+}

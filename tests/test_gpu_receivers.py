@@ -1,0 +1,199 @@
+"""HIP receiver-driven updates (lux_gi_update_receivers) against the oracle.
+
+Camera receivers over the Cornell box drive the update of a LOD pyramid of
+dense grids: the seed (keys, flags, LOD pairs), the probe directory
+(canonical descriptors, ray counts and offsets), the sequences and tickets
+equal the oracle's selection; the deposited intervals and their sample
+counts, the merged intervals and the packed probes of every LOD grid equal
+the oracle's receiver trace, merge and pack bit for bit -- with every
+receiver tracing, with a fair window (budget and cursor), with the LOD
+overlap of a perspective camera (deposits of LODs without a grid dropped),
+and with the temporal history over consecutive updates.
+"""
+import numpy as np
+import pytest
+
+from bevy_penumbra_amd import lux, scenes
+from bevy_penumbra_amd.lux import DeviceBuffer
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+W, H = 96, 64
+
+
+def to_oracle(p):
+    q = O.GiParams()
+    for name, _ in O.GiParams._fields_:
+        setattr(q, name, getattr(p, name))
+    return q
+
+
+def bits_equal(a, b, what):
+    a = np.ascontiguousarray(a).view(np.uint32)
+    b = np.ascontiguousarray(b).view(np.uint32)
+    assert a.shape == b.shape, what
+    bad = np.nonzero((a != b).reshape(a.shape[0], -1).any(axis=1))[0]
+    assert bad.size == 0, (what, bad[:5], a.reshape(a.shape[0], -1)[bad[:3]], b.reshape(b.shape[0], -1)[bad[:3]])
+
+
+def base_params():
+    return lux.gi_params((0.0, 0.0, 0.0), 0.125, (16, 16, 16), 4, base_phi=8, base_y=4, interval_length=0.1,
+                         emissive_samples=8, terminal_tail=1)
+
+
+def pyramid(sc, p0, n):
+    ctxs = []
+    for k in range(n):
+        c = lux.Context(0)
+        c.upload_scene(sc)
+        c.build_bvh()
+        c.gi_configure(lux.lod_gi_params(p0, k))
+        ctxs.append(c)
+    return ctxs
+
+
+def camera_receivers(ctx):
+    cam = scenes.camera_rays((1.0, 1.0, 1.95), (1.0, 0.9, 0.0), W, H, 60.0)
+    recv = DeviceBuffer.empty(ctx, (W * H, 6), np.float32)
+    ctx.camera_receivers(cam, W, H, recv)
+    ctx.synchronize()
+    f = 1.0 / np.tan(np.radians(30.0))
+    up = cam[9:12] / np.linalg.norm(cam[9:12])
+    terms = dict(camera=tuple(cam[:3]), probe_spacing=0.125, clip_row1=tuple(up * f), viewport_height=H,
+                 clip_row3=tuple(cam[3:6]), target_pixels=8)
+    return recv, recv.numpy(), terms
+
+
+def oracle_update(sc, p0, n_lods, rec, rp, oview):
+    levels = p0.levels
+    keys, flags, lods = O.rx_seed(rec, rp, oview, p0.spacing, levels)
+    sel = O.rx_select(W * H, levels, keys, flags)
+    grids = []
+    for k in range(n_lods):
+        g = to_oracle(lux.lod_gi_params(p0, k))
+        g.frame = rp.frame
+        grids.append(g)
+    built = O.build_scene(sc)
+    lights = O.build_lights(sc)
+    ivs, sms, st = O.rx_trace(built, lights, sc.materials, grids, rec, lods, sel["sequence"], sel["tickets"],
+                              rp.frame)
+    return dict(keys=keys, flags=flags, lods=lods, sel=sel, grids=grids, ivs=ivs, sms=sms, stats=st)
+
+
+@pytest.mark.parametrize("case", ["all_receivers", "fair_window_lod_pair"])
+def test_gpu_receiver_update_matches_oracle(case):
+    sc, _ = scenes.cornell()
+    p0 = base_params()
+    n_lods = 1 if case == "all_receivers" else 2
+    ctxs = pyramid(sc, p0, n_lods)
+    try:
+        recv, rec, terms = camera_receivers(ctxs[0])
+        if case == "all_receivers":
+            view, oview = None, None
+            rp_args = dict(budget=0, frame=3)
+        else:
+            view, oview = lux.LodView.make(**terms), O.lod_view(**terms)
+            rp_args = dict(budget=1500, fair_cursor=700, frame=5)
+        st = lux.update_receivers(ctxs, view, recv, lux.receiver_params(W, H, **rp_args))
+        o = oracle_update(sc, p0, n_lods, rec, O.receiver_params(W, H, **rp_args), oview)
+        got = lux.receivers_download(ctxs[0], W * H, p0.levels)
+        assert np.array_equal(got["keys"], o["keys"])
+        assert np.array_equal(got["flags"], o["flags"])
+        assert np.array_equal(got["lods"], o["lods"])
+        for key in ("canonical", "count", "offset", "sequence", "tickets"):
+            assert np.array_equal(got[key], o["sel"][key]), key
+        assert st.root_rays == o["sel"]["root"] and st.tickets == len(o["sel"]["tickets"])
+        ost = o["stats"]
+        assert (st.rays, st.unsupported, st.deposits, st.outside) == tuple(int(x) for x in ost)
+        assert st.receivers == int((np.isfinite(rec).all(axis=1) & ((rec[:, 3:] ** 2).sum(axis=1) > 0.01)).sum())
+        assert st.overflow == 0
+        if case == "all_receivers":
+            # every receiver traces; those on the far walls (x, y or z = 2) sit in cell 16, past the grid
+            assert st.tickets == st.receivers and 0 < st.outside < st.deposits
+        else:
+            assert st.tickets < st.receivers and len(set(o["lods"][0])) > 1
+            assert (o["sel"]["sequence"][1] != O.EMPTY).any()       # the overlap traced some high LODs
+        for k, c in enumerate(ctxs):
+            g = c.gi_download()
+            bits_equal(g["intervals"], o["ivs"][k], f"intervals LOD {k}")
+            assert np.array_equal(c.gi_samples_download(), o["sms"][k]), k
+            merged = O.gi_merge(o["grids"][k], o["ivs"][k])
+            bits_equal(g["merged"], merged, f"merged LOD {k}")
+            bits_equal(g["probes"], O.gi_pack(o["grids"][k], merged), f"probes LOD {k}")
+        assert o["sms"][0].sum() > 0
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+def test_gpu_receiver_updates_with_history_match_oracle():
+    """Three updates of a moving fair window with the temporal history on:
+    the blended intervals, history and probes follow the oracle's chain."""
+    sc, _ = scenes.cornell()
+    p0 = base_params()
+    RB, TB, FRAMES = 0.88, 0.98, 24
+    ctxs = pyramid(sc, p0, 1)
+    try:
+        c = ctxs[0]
+        c.gi_set_history(RB, TB, FRAMES, 1)
+        recv, rec, _ = camera_receivers(c)
+        g0 = to_oracle(p0)
+        built = O.build_scene(sc)
+        lights = O.build_lights(sc)
+        pts = np.concatenate([np.stack([l[0:3], l[0:3] + l[12:15], l[0:3] + l[16:19]]) for l in lights])
+        emit = (pts.min(axis=0), pts.max(axis=0))
+        hist = np.zeros((O.total_intervals(g0), 8), np.float32)
+        sched = lux.CandidateSchedule()
+        for f in range(3):
+            cursor, quota = sched.phase_for(W, H, W * H, 2000)
+            rp_args = dict(budget=2000, fair_cursor=cursor, fair_quota=quota, frame=10 + f)
+            lux.update_receivers(ctxs, None, recv, lux.receiver_params(W, H, **rp_args))
+            sched.advance_after_successful_update(True)
+            rp = O.receiver_params(W, H, **rp_args)
+            keys, flags, lods = O.rx_seed(rec, rp, None, p0.spacing, p0.levels)
+            sel = O.rx_select(W * H, p0.levels, keys, flags)
+            g0.frame = rp.frame
+            ivs, sms, _ = O.rx_trace(built, lights, sc.materials, [g0], rec, lods, sel["sequence"], sel["tickets"],
+                                     rp.frame)
+            blended, hist = O.gi_history(g0, RB, TB, FRAMES, 1, f > 0, ivs[0], sms[0], hist, emit=emit)
+            merged = O.gi_merge(g0, blended, hist)
+            g = c.gi_download()
+            gh, gs = c.gi_history_download()
+            assert np.array_equal(gs, sms[0]), f
+            bits_equal(g["intervals"], blended, f"intervals, update {f}")
+            bits_equal(gh, hist, f"history, update {f}")
+            bits_equal(g["merged"], merged, f"merged, update {f}")
+            bits_equal(g["probes"], O.gi_pack(g0, merged), f"probes, update {f}")
+        assert (hist[:, 3] > 1.0).any()                   # some intervals saw more than one update
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+def test_gpu_receiver_update_rejects_misaligned_pyramids():
+    sc, _ = scenes.cornell()
+    p0 = base_params()
+    ctxs = pyramid(sc, p0, 2)
+    try:
+        recv, _, _ = camera_receivers(ctxs[0])
+        bad = lux.lod_gi_params(p0, 1)
+        bad.spacing = 0.3                                 # not twice the LOD-0 spacing
+        ctxs[1].gi_configure(bad)
+        with pytest.raises(lux.LuxError):
+            lux.update_receivers(ctxs, None, recv, lux.receiver_params(W, H))
+        moved = lux.lod_gi_params(p0, 1)
+        moved.origin[0] = 0.25                            # not on the coarsest level's cells (2 m)
+        ctxs[1].gi_configure(moved)
+        with pytest.raises(lux.LuxError):
+            lux.update_receivers(ctxs, None, recv, lux.receiver_params(W, H))
+        with pytest.raises(lux.LuxError):
+            lux.update_receivers(ctxs[:1], None, recv, lux.receiver_params(0, H))
+        # disabled updates trace nothing: every interval unsupported
+        st = lux.update_receivers(ctxs[:1], None, recv, lux.receiver_params(W, H, disabled=True))
+        assert st.rays == 0 and st.tickets == 0
+        g = ctxs[0].gi_download()
+        assert (g["intervals"][:, 3] == -1.0).all()
+    finally:
+        for c in ctxs:
+            c.close()
