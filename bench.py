@@ -55,6 +55,8 @@ def parse():
     p.add_argument("--batch-scenes", type=int, default=64)
     p.add_argument("--batch-train", type=int, default=100,
                    help="online training steps of each batch scene's surrogate")
+    p.add_argument("--no-receivers", action="store_true")
+    p.add_argument("--receiver-steps", type=int, default=10)
     p.add_argument("--pmc-workload", action="store_true",
                    help="run only the counter-profiling workload (1 GiB device copy x2, three "
                         "traversal launches, two GI updates; default launch configurations)")
@@ -449,6 +451,123 @@ def batch_section(lux, scenes, args) -> dict:
     return res
 
 
+def receivers_section(lux, scenes, args) -> dict:
+    """Receiver-driven updates (seed / rank / parents / prefix /
+    generateDirections, receiver trace, reduce, then history-free merge and
+    pack of each LOD grid) on the colonnade: 1920 x 1080 camera receivers
+    (2,073,600) over a three-LOD pyramid at 0.5 / 1 / 2 m covering the scene,
+    every receiver tracing, and the default component's update budget with
+    the fair window advancing."""
+    from bevy_penumbra_amd.lux import DeviceBuffer
+    scene, _ = scenes.colonnade()
+    # grid k (LOD k): spacing and interval length 0.5 * 2^k m, its origin on its
+    # coarsest level's cells (8 * 2^k m, as the world-anchored keys require),
+    # covering the scene's triangles (x -1.1 .. 31.1, z -1.3 .. 16.2)
+    lo_t, hi_t = np.array([-1.2, 0.0, -1.4]), np.array([31.2, 12.0, 16.3])
+    grids = []
+    for k in range(3):
+        sk, top = 0.5 * 2 ** k, 0.5 * 2 ** k * 16
+        org = np.floor(lo_t / top) * top
+        org[1] = 0.0
+        dims = [int(np.ceil((hi_t[a] - org[a]) / sk)) for a in range(3)]
+        grids.append(lux.gi_params(tuple(org), sk, tuple(dims), 5, base_phi=8, base_y=4, interval_mode=0,
+                                   interval_length=0.5 * 2 ** k, normal_bias=0.005, emissive_samples=16, seed=0,
+                                   terminal_tail=1))
+    p0 = grids[0]
+    ctxs = []
+    for k in range(3):
+        c = lux.Context(0)
+        c.upload_scene(scene)
+        c.build_bvh()
+        c.gi_configure(grids[k])
+        ctxs.append(c)
+    # the clutter fills the volume (every interior eye sits inside a box): the
+    # camera looks along the rows from 12 m outside the x = 0 end; 92 target
+    # pixels put the visible receivers in LOD 0, the overlap and LOD 1
+    W, H = 1920, 1080
+    cam = scenes.camera_rays((-12.0, 6.0, 7.5), (15.0, 6.0, 7.5), W, H, 50.0)
+    recv = DeviceBuffer.empty(ctxs[0], (W * H, 6), np.float32)
+    ctxs[0].camera_receivers(cam, W, H, recv)
+    ctxs[0].synchronize()
+    f = 1.0 / np.tan(np.radians(25.0))
+    up = cam[9:12] / np.linalg.norm(cam[9:12])
+    terms = dict(camera=tuple(cam[:3]), probe_spacing=0.5, clip_row1=tuple(up * f), viewport_height=H,
+                 clip_row3=tuple(cam[3:6]), target_pixels=92)
+    view = lux.LodView.make(**terms)
+    rc = lux.RadianceCascades.default()
+    budget = rc.update_budget()
+    out = {"receivers": W * H,
+           "grids": ["%dx%dx%d at %.1f m from (%g, %g, %g)" % (tuple(g.dims) + (g.spacing,) + tuple(g.origin))
+                     for g in grids],
+           "levels": 5, "base_directions": 32, "intervals": [int(c.gi_sizes()[0]) for c in ctxs]}
+    for name, b in (("all_receivers", 0), ("default_budget", budget)):
+        sched = lux.CandidateSchedule()
+        ms, wall = [], []
+        st = None
+        for i in range(args.receiver_steps + 2):
+            cursor, quota = sched.phase_for(W, H, W * H, b)
+            rp = lux.receiver_params(W, H, b, cursor, quota, frame=i)
+            t0 = time.perf_counter()
+            st = lux.update_receivers(ctxs, view, recv, rp)
+            for c in ctxs:
+                c.synchronize()
+            if i >= 2:
+                wall.append((time.perf_counter() - t0) * 1e3)
+                ms.append(st.ms)
+            sched.advance_after_successful_update(True)
+        med = float(np.median(ms))
+        sel_view = lux.receivers_download(ctxs[0], W * H, p0.levels, directory=False)
+        seeded = (sel_view["flags"][: W * H * p0.levels: p0.levels] & 1) == 1    # variant 0, level 0
+        lod_hist = np.bincount(sel_view["lods"][0][seeded], minlength=3)[:4]
+        out[name] = {
+            "budget": b,
+            "ms_per_update": round(med, 4),
+            "wall_ms_per_update": round(float(np.median(wall)), 4),
+            "rays": st.rays,
+            "high_lod_rays": int((sel_view["sequence"][1] != 0xFFFFFFFF).sum()),
+            "receiver_mrays_s": round(st.rays / (med * 1e-3) / 1e6, 3),
+            "eligible_receivers": st.receivers,
+            "receivers_by_low_lod": lod_hist.tolist(),
+            "tickets": st.tickets,
+            "deposits": st.deposits,
+            "outside": st.outside,
+            "unsupported": st.unsupported,
+        }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        rec = recv.numpy()
+        cpu_budget = 16384
+        rp = O.receiver_params(W, H, cpu_budget, 0, cpu_budget, frame=0)
+        oview = O.lod_view(**terms)
+        built = O.build_scene(scene)
+        lights = O.build_lights(scene)
+        ogrids = []
+        for q in grids:
+            g = O.GiParams()
+            for fname, _ in O.GiParams._fields_:
+                setattr(g, fname, getattr(q, fname))
+            ogrids.append(g)
+        t0 = time.perf_counter()
+        keys, flags, lods = O.rx_seed(rec, rp, oview, p0.spacing, p0.levels)
+        sel = O.rx_select(W * H, p0.levels, keys, flags)
+        t1 = time.perf_counter()
+        _, _, ost = O.rx_trace(built, lights, scene.materials, ogrids, rec, lods, sel["sequence"], sel["tickets"], 0)
+        t2 = time.perf_counter()
+        out["cpu_baseline"] = {
+            "select_ms": round((t1 - t0) * 1e3, 1),
+            "trace_ms": round((t2 - t1) * 1e3, 1),
+            "receiver_mrays_s": round(int(ost[0]) / (t2 - t1) / 1e6, 4),
+            "cores": 1,
+            "kind": "port",
+            "sample": f"one update of budget {cpu_budget} over the same 2,073,600 receivers: C restatement "
+                      "of seed/rank/parents/prefix/generateDirections and of the receiver trace, deposit "
+                      "and reduce (merge and pack not included), one thread",
+        }
+    for c in ctxs:
+        c.close()
+    return out
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -591,6 +710,8 @@ def main() -> int:
         out["dynamic_frame"] = dynamic_section(lux, scenes, args)
     if not args.no_batch:
         out["batched_scenes"] = batch_section(lux, scenes, args)
+    if not args.no_receivers:
+        out["receiver_updates"] = receivers_section(lux, scenes, args)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -289,7 +289,7 @@ extern "C" void lux_ctx_destroy(lux_ctx* ctx) {
                           &ctx->gi_w_slots, &ctx->gi_w_vis, &ctx->gi_w_shadow, &ctx->gi_w_counters,
                           &ctx->gi_samples, &ctx->gi_history, &ctx->gi_hist_state,
                           &ctx->rx_keys, &ctx->rx_flags, &ctx->rx_slot, &ctx->rx_lod, &ctx->rx_ord,
-                          &ctx->rx_seq, &ctx->rx_dir, &ctx->rx_tickets, &ctx->rx_dep_val,
+                          &ctx->rx_seq, &ctx->rx_dir, &ctx->rx_tickets, &ctx->rx_dep_val, &ctx->rx_pack, &ctx->rx_traced,
                           &ctx->rx_sort, &ctx->rx_scan, &ctx->rx_counters};
     for (lux_buffer* b : bufs) lux_release(*b);
     for (hipEvent_t e : ctx->ev_phase)
@@ -635,7 +635,7 @@ extern "C" int lux_sort_pairs_device(lux_ctx* ctx, const uint32_t* keys_in, uint
     uint32_t* tk = static_cast<uint32_t*>(tmp.ptr);
     uint32_t* tv = tk + n;
     uint32_t* hist = tv + n;
-    int rc = lux_sort_pairs_impl(ctx, keys_in, n, keys_out, index_out, tk, tv, hist);
+    int rc = lux_sort_pairs_impl(ctx, keys_in, n, keys_out, index_out, tk, tv, hist, 32);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(tmp.ptr);
     return rc;

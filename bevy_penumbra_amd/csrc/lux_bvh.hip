@@ -6,7 +6,8 @@
 // shaders/split.slang:3-10).  Pipeline per primitive set:
 //   1. centre bounds       atomic min/max on order-preserving integer images
 //   2. Morton codes        10 bits/axis of c = lo*0.5 + hi*0.5 (scene.rs:2461)
-//   3. stable LSD radix    4 passes x 8 bits, wave-ballot ranking (wave64)
+//   3. stable LSD radix    8-bit digits (4 passes for 30-bit codes), wave-ballot
+//                          ranking (wave64), multi-block histogram scan
 //   4. Karras radix tree   one thread per internal node
 //   5. bottom-up boxes     one thread per leaf, atomic arrival counters
 //   6. node emission       reference BvhNode layout; internal node k keeps its
@@ -132,42 +133,64 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* keys, ui
     hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of `total` words in one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* data, uint32_t total) {
-    __shared__ uint32_t warp_sums[16];
+// ---------------- exclusive scan (multi-block) ----------------
+constexpr uint32_t kScanBlock = 1024;
+
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wsum[kScanBlock / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63u) wsum[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t w = lane < kScanBlock / 64 ? wsum[lane] : 0u;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            uint32_t y = __shfl_up(w, off, 64);
+            if (lane >= (uint32_t)off) w += y;
+        }
+        if (lane < kScanBlock / 64) wsum[lane] = w;
+    }
+    __syncthreads();
+    const uint32_t before = wave ? wsum[wave - 1] : 0u;
+    *total = wsum[kScanBlock / 64 - 1];
+    return before + x - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_blocks(uint32_t* data, uint32_t n, uint32_t* sums) {
+    const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+    uint32_t total;
+    uint32_t e = block_scan_excl(i < n ? data[i] : 0u, &total);
+    if (i < n) data[i] = e;
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Scan of the block sums in one block (looping); the grand total goes to *out.
+__global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t* sums, uint32_t nb, uint32_t* out) {
     __shared__ uint32_t carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    for (uint32_t base = 0; base < total; base += 1024) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t v = i < total ? data[i] : 0u;
-        // inclusive wave scan
-        uint32_t x = v;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            uint32_t y = __shfl_up(x, off, 64);
-            if (lane >= (uint32_t)off) x += y;
-        }
-        if (lane == 63) warp_sums[wave] = x;
+    for (uint32_t base = 0; base < nb; base += kScanBlock) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t total;
+        const uint32_t c = carry;
+        uint32_t e = block_scan_excl(i < nb ? sums[i] : 0u, &total);
+        if (i < nb) sums[i] = c + e;
         __syncthreads();
-        if (wave == 0) {
-            uint32_t s = lane < 16 ? warp_sums[lane] : 0u;
-#pragma unroll
-            for (int off = 1; off < 16; off <<= 1) {
-                uint32_t y = __shfl_up(s, off, 64);
-                if (lane >= (uint32_t)off) s += y;
-            }
-            if (lane < 16) warp_sums[lane] = s;
-        }
-        __syncthreads();
-        uint32_t wave_prefix = wave ? warp_sums[wave - 1] : 0u;
-        uint32_t c = carry;
-        if (i < total) data[i] = c + wave_prefix + x - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry = c + wave_prefix + x;
+        if (threadIdx.x == 0) carry = c + total;
         __syncthreads();
     }
+    if (threadIdx.x == 0) *out = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_add(uint32_t* data, uint32_t n, const uint32_t* sums) {
+    const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+    if (i < n) data[i] += sums[blockIdx.x];
 }
 
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* keys_in,
@@ -362,27 +385,46 @@ inline uint32_t blocks_for(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
 
 }  // namespace
 
+size_t lux_scan_sums_words(uint32_t n) { return (size_t)(n + kScanBlock - 1) / kScanBlock + 2; }
+
+int lux_scan_exclusive_impl(lux_ctx* ctx, uint32_t* data, uint32_t n, uint32_t* sums, uint32_t* total) {
+    hipStream_t st = ctx->stream;
+    const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+    if (nb) hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanBlock), 0, st, data, n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, st, sums, nb, total);
+    if (nb) hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanBlock), 0, st, data, n, sums);
+    LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
+// Histogram words (256 digits x tiles) followed by the histogram scan's
+// block sums and total.
 size_t lux_sort_hist_words(uint32_t n) {
     uint32_t nb = (n + kRsTile - 1) / kRsTile;
     if (nb == 0) nb = 1;
-    return (size_t)kRsBins * nb;
+    return (size_t)kRsBins * nb + lux_scan_sums_words(kRsBins * nb);
 }
 
 int lux_sort_pairs_impl(lux_ctx* ctx, const uint32_t* keys_in, uint32_t n, uint32_t* keys_out,
                         uint32_t* vals_out, uint32_t* tmp_keys, uint32_t* tmp_vals,
-                        uint32_t* hist) {
+                        uint32_t* hist, uint32_t bits) {
     if (n == 0) return LUX_OK;
     hipStream_t st = ctx->stream;
     uint32_t nb = (n + kRsTile - 1) / kRsTile;
-    // passes: in -> tmp -> out -> tmp -> out (4 passes end in out)
+    const uint32_t words = (uint32_t)kRsBins * nb;
+    uint32_t* sums = hist + words;
+    const int passes = bits == 0 ? 1 : (int)((bits + 7u) / 8u);
+    // ping-pong so that the last pass writes keys_out / vals_out
     const uint32_t* kin = keys_in;
     const uint32_t* vin = nullptr;
-    for (int pass = 0; pass < 4; ++pass) {
+    for (int pass = 0; pass < passes; ++pass) {
         uint32_t shift = 8u * (uint32_t)pass;
-        uint32_t* kout = (pass & 1) ? keys_out : tmp_keys;
-        uint32_t* vout = (pass & 1) ? vals_out : tmp_vals;
+        const bool last_parity = ((passes - 1 - pass) & 1) == 0;
+        uint32_t* kout = last_parity ? keys_out : tmp_keys;
+        uint32_t* vout = last_parity ? vals_out : tmp_vals;
         hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(kRsThreads), 0, st, kin, n, shift, nb, hist);
-        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, hist, (uint32_t)(kRsBins * nb));
+        if (int rc = lux_scan_exclusive_impl(ctx, hist, words, sums, sums + lux_scan_sums_words(words) - 1))
+            return rc;
         hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(kRsThreads), 0, st, kin, vin, kout, vout, n,
                            shift, nb, hist);
         kin = kout;
@@ -428,7 +470,7 @@ int lux_lbvh_build_device(lux_ctx* ctx, const float4* lo, const float4* hi, uint
     if (cb > 2048) cb = 2048;
     hipLaunchKernelGGL(k_centre_bounds, dim3(cb), dim3(256), 0, st, lo, hi, n, bounds);
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, 256)), dim3(256), 0, st, lo, hi, n, bounds, codes);
-    if (int rc = lux_sort_pairs_impl(ctx, codes, n, sk, perm, tk, tv, hist)) return rc;
+    if (int rc = lux_sort_pairs_impl(ctx, codes, n, sk, perm, tk, tv, hist, 32)) return rc;
     if (n <= leaf_max) {
         // the root is a leaf; the remaining 2n-2 slots of this range are unused
         // and kept zero, as in every other unused slot

@@ -772,6 +772,42 @@ __device__ void lod_select(const lux_lod_view& v, const float* pos, uint32_t& lo
     }
 }
 
+// Eligible receiver (finite position and normal, |n|^2 > 0.01), unit normal.
+__device__ __forceinline__ bool rx_surface(const float* r, float3& pos, float3& n) {
+    float x = r[3], y = r[4], z = r[5];
+    float nn = (x * x + y * y) + z * z;
+    if (!(lux::isfinite_bits(r[0]) && lux::isfinite_bits(r[1]) && lux::isfinite_bits(r[2]))) return false;
+    if (!(lux::isfinite_bits(x) && lux::isfinite_bits(y) && lux::isfinite_bits(z)) || !(nn > 0.01f)) return false;
+    float nl = sqrtf(nn);
+    pos = make_float3(r[0], r[1], r[2]);
+    n = make_float3(x / nl, y / nl, z / nl);
+    return true;
+}
+
+// The ray of a selected (variant, receiver) j = variant * count + receiver
+// (traceTransport, split.slang:1135-1150): uv = frac(0.5 + r2(sequence + 1) +
+// jitter), jitter = r2(transportHash(frame ^ 0x5a17c9e3)), the decoded
+// direction turned into the receiver's hemisphere, origin rayOffset from the
+// surface.  False when the receiver has no surface.
+__device__ __forceinline__ bool rx_ray(const float* recv, uint32_t count, uint32_t j, uint32_t sq, uint32_t frame,
+                                       float bias, float3& pos, float3& o, float3& d) {
+    const uint32_t smp = j < count ? j : j - count;
+    float3 n;
+    if (!rx_surface(recv + 6ull * smp, pos, n)) return false;
+    const float2 jit = r2(transport_hash(frame ^ 0x5a17c9e3u));
+    const float2 q = r2(sq + 1u);
+    const float ux = fracf((0.5f + q.x) + jit.x), uy = fracf((0.5f + q.y) + jit.y);
+    float c, s;
+    cos_sin_2pi(ux, c, s);
+    const float y = uy * 2.0f - 1.0f;
+    const float qq = 1.0f - y * y;
+    const float r = sqrtf(qq > 0.0f ? qq : 0.0f);
+    d = make_float3(r * c, y, r * s);
+    if (dot3(d, n) < 0.0f) d = make_float3(-d.x, -d.y, -d.z);
+    o = ray_offset(pos, n, bias);
+    return true;
+}
+
 // Kernel arguments of a configured context (host).
 inline GiArgs gi_args(lux_ctx* ctx) {
     GiArgs a;

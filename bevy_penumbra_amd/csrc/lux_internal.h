@@ -84,10 +84,11 @@ struct lux_ctx {
     // receiver-driven updates (lux_gi_update_receivers, state on the first
     // LOD's context): descriptor keys / flags / directory slots, per
     // (variant, receiver) LOD, ordinal and sequence, the probe directory
-    // (owner, canonical, count, offset per level), receiver tickets, deposit
-    // values (their keys live in the sort scratch), scan scratch, counters
+    // (owner, canonical, count, offset per level), receiver tickets and ray
+    // list, per-ray deposit slots, compacted deposit values and long-run list,
+    // sort and scan scratch, counters
     lux_buffer rx_keys, rx_flags, rx_slot, rx_lod, rx_ord, rx_seq, rx_dir, rx_tickets;
-    lux_buffer rx_dep_val, rx_sort, rx_scan, rx_counters;
+    lux_buffer rx_dep_val, rx_pack, rx_traced, rx_sort, rx_scan, rx_counters;
     uint32_t rx_count = 0, rx_levels = 0, rx_cap = 0, rx_tickets_n = 0;
 
     // trace tuning
@@ -117,10 +118,16 @@ void lux_release(lux_buffer& b);
 int lux_lbvh_build_device(lux_ctx* ctx, const float4* lo, const float4* hi, uint32_t n,
                           uint32_t leaf_max, float4* nodes, uint32_t* perm,
                           uint32_t node_first_offset, uint32_t leaf_first_offset);
+// Stable LSD radix sort of n keys on their low `bits` bits (values = input
+// positions); hist holds lux_sort_hist_words(n) words.
 int lux_sort_pairs_impl(lux_ctx* ctx, const uint32_t* keys_in, uint32_t n, uint32_t* keys_out,
                         uint32_t* vals_out, uint32_t* tmp_keys, uint32_t* tmp_vals,
-                        uint32_t* hist);
+                        uint32_t* hist, uint32_t bits);
 size_t lux_sort_hist_words(uint32_t n);
+// Exclusive scan of n words in place on the context stream; sums holds
+// lux_scan_sums_words(n) words, *total (device) receives the sum.
+int lux_scan_exclusive_impl(lux_ctx* ctx, uint32_t* data, uint32_t n, uint32_t* sums, uint32_t* total);
+size_t lux_scan_sums_words(uint32_t n);
 
 // Surrogate forward pass over device queries (9 floats each); *Y receives a
 // device pointer to n x 16 outputs owned by the surrogate, valid until its
@@ -138,3 +145,7 @@ int lux_surrogate_forward_into(lux_surrogate* s, hipStream_t st, const float* qu
 int lux_gi_check_internal(lux_ctx* ctx);
 // History blend (when on), merge and pack of a context's intervals on stream st.
 int lux_gi_finish_internal(lux_ctx* ctx, hipStream_t st);
+// Receiver rays of a ray list through the wavefront stages on the context
+// stream: per ray (rad.rgb, hit t) and status (1 hit, 2 unsupported).
+int lux_rx_wave_trace(lux_ctx* ctx, const float* recv, uint32_t count, const uint32_t* rays, uint32_t n_rays,
+                      const uint32_t* seq, uint32_t frame, float4* out, uint32_t* status);

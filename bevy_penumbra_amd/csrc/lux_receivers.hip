@@ -85,18 +85,6 @@ __device__ bool candidate_selected(const RxSeed& s, uint32_t sample) {
     return rank >= start ? rank - start < s.quota : s.count - start + rank < s.quota;
 }
 
-// Eligible receiver (finite position and normal, |n|^2 > 0.01), unit normal.
-__device__ __forceinline__ bool rx_surface(const float* r, float3& pos, float3& n) {
-    float x = r[3], y = r[4], z = r[5];
-    float nn = (x * x + y * y) + z * z;
-    if (!(lux::isfinite_bits(r[0]) && lux::isfinite_bits(r[1]) && lux::isfinite_bits(r[2]))) return false;
-    if (!(lux::isfinite_bits(x) && lux::isfinite_bits(y) && lux::isfinite_bits(z)) || !(nn > 0.01f)) return false;
-    float nl = sqrtf(nn);
-    pos = make_float3(r[0], r[1], r[2]);
-    n = make_float3(x / nl, y / nl, z / nl);
-    return true;
-}
-
 // cacheKey (common.slang:418-423), direction 0.
 __device__ __forceinline__ int4 rx_cell(float3 p, float base, uint32_t level, uint32_t lod) {
     uint32_t l = lod < 24u ? lod : 24u;
@@ -188,54 +176,89 @@ __global__ __launch_bounds__(kWg) void k_rx_dir_clear(RxDir D) {
     }
 }
 
-__global__ __launch_bounds__(kWg) void k_rx_insert(RxDir D, const int4* keys, const uint32_t* flags, uint64_t nd,
-                                                   uint32_t* slot_out, uint32_t* counters) {
-    const uint64_t d = (uint64_t)blockIdx.x * kWg + threadIdx.x;
-    if (d >= nd) return;
-    if (!(flags[d] & kRxValid)) {
-        slot_out[d] = kRxEmpty;
-        return;
-    }
-    const uint32_t L = (uint32_t)(d % D.levels);
-    const int4 k = keys[d];
+// Insert one key at level L (the owner word goes from empty to a descriptor
+// index once; the canonical word only decreases, so stale plain loads are
+// safe: they cost at most an extra atomic).
+__device__ uint32_t dir_insert(const RxDir& D, const int4* keys, uint32_t L, int4 k, uint32_t d) {
     uint32_t* ow = D.owner + (uint64_t)L * D.cap;
+    uint32_t* cn = D.canon + (uint64_t)L * D.cap;
     uint32_t b = dir_bucket(k, D.cap);
     for (uint32_t a = 0; a < D.cap; ++a) {
-        // keys are all published by the seed launch; the CAS publishes an index
-        uint32_t cur = atomicCAS(&ow[b], kRxEmpty, (uint32_t)d);
-        if (cur == kRxEmpty || same_probe(keys[cur], k)) {
-            atomicMin(&D.canon[(uint64_t)L * D.cap + b], (uint32_t)d);
-            slot_out[d] = b;
-            return;
+        uint32_t cur = ow[b];
+        if (cur == kRxEmpty) {
+            // keys are all published by the seed launch; the CAS publishes an index
+            cur = atomicCAS(&ow[b], kRxEmpty, d);
+            if (cur == kRxEmpty) cur = d;
+        }
+        if (cur == d || same_probe(keys[cur], k)) {
+            if (d < cn[b]) atomicMin(&cn[b], d);
+            return b;
         }
         b = (b + 1u) & (D.cap - 1u);
     }
-    slot_out[d] = kRxEmpty;
-    atomicAdd(&counters[7], 1u);
+    return kRxEmpty;
 }
 
-// Sort keys of the (variant, receiver) ray markers: level-0 slot, or empty.
+// Level-major over the descriptors (thread t: level t / m, (variant,
+// receiver) t % m, m = 2 * count), so a wave holds 64 neighbouring receivers
+// of one level; lanes with the same probe key elect their lowest lane, which
+// inserts once for all of them.
+__global__ __launch_bounds__(kWg) void k_rx_insert(RxDir D, const int4* keys, const uint32_t* flags, uint32_t m,
+                                                   uint32_t* slot_out, uint32_t* counters) {
+    const uint64_t t = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    uint32_t L = 0, d = 0;
+    bool valid = false;
+    int4 k = make_int4(0, 0, 0, 0);
+    if (t < (uint64_t)m * D.levels) {
+        L = (uint32_t)(t / m);
+        d = (uint32_t)((t % m) * D.levels + L);
+        valid = (flags[d] & kRxValid) != 0u;
+        if (valid) k = keys[d];
+        else slot_out[d] = kRxEmpty;
+    }
+    uint64_t pending = __ballot(valid);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int4 lk = make_int4(__shfl(k.x, leader, 64), __shfl(k.y, leader, 64), __shfl(k.z, leader, 64),
+                                  __shfl(k.w, leader, 64));
+        const uint32_t lL = __shfl(L, leader, 64);
+        const bool peer = valid && ((pending >> lane) & 1ull) && L == lL && same_probe(k, lk);
+        const uint64_t peers = __ballot(peer);
+        uint32_t slot = kRxEmpty;
+        if ((int)lane == leader) slot = dir_insert(D, keys, L, k, d);
+        slot = __shfl(slot, leader, 64);
+        if (peer) slot_out[d] = slot;
+        if ((int)lane == leader && slot == kRxEmpty) atomicAdd(&counters[7], (uint32_t)__popcll(peers));
+        pending &= ~peers;
+    }
+}
+
+// Sort keys of the (variant, receiver) ray markers: level-0 slot, or `none`
+// (the directory capacity, above every slot).
 __global__ __launch_bounds__(kWg) void k_rx_ray_keys(const uint32_t* flags, const uint32_t* slot, uint32_t m,
-                                                     uint32_t levels, uint32_t* out) {
+                                                     uint32_t levels, uint32_t none, uint32_t* out) {
     const uint32_t j = blockIdx.x * kWg + threadIdx.x;
     if (j >= m) return;
     const uint64_t d0 = (uint64_t)j * levels;
-    out[j] = (flags[d0] & kRxRay) && slot[d0] != kRxEmpty ? slot[d0] : kRxEmpty;
+    out[j] = (flags[d0] & kRxRay) && slot[d0] != kRxEmpty ? slot[d0] : none;
 }
 
-__global__ __launch_bounds__(kWg) void k_rx_run_start(const uint32_t* sk, uint32_t m, uint32_t* start) {
+__global__ __launch_bounds__(kWg) void k_rx_run_start(const uint32_t* sk, uint32_t m, uint32_t none,
+                                                      uint32_t* start) {
     const uint32_t i = blockIdx.x * kWg + threadIdx.x;
     if (i >= m) return;
     const uint32_t k = sk[i];
-    if (k != kRxEmpty && (i == 0 || sk[i - 1] != k)) start[k] = i;
+    if (k != none && (i == 0 || sk[i - 1] != k)) start[k] = i;
 }
 
 __global__ __launch_bounds__(kWg) void k_rx_ordinals(const uint32_t* sk, const uint32_t* sv, uint32_t m,
-                                                     const uint32_t* start, uint32_t* count0, uint32_t* ord) {
+                                                     uint32_t none, const uint32_t* start, uint32_t* count0,
+                                                     uint32_t* ord) {
     const uint32_t i = blockIdx.x * kWg + threadIdx.x;
     if (i >= m) return;
     const uint32_t k = sk[i];
-    if (k == kRxEmpty) return;
+    if (k == none) return;
     const uint32_t s = start[k];
     ord[sv[i]] = i - s;
     if (i + 1 == m || sk[i + 1] != k) count0[k] = i - s + 1u;
@@ -300,89 +323,39 @@ __global__ __launch_bounds__(kWg) void k_rx_prefix(RxDir D, const int4* keys, ui
     D.offset[(uint64_t)L * D.cap + b] = off;
 }
 
-// generateDirections (split.slang:963-983): sequences and ticket flags.
+// generateDirections (split.slang:963-983): sequences, ticket flags and the
+// number of rays of each receiver.
 __global__ __launch_bounds__(kWg) void k_rx_generate(const uint32_t* flags, const uint32_t* slot, const uint32_t* ord,
                                                      const uint32_t* offset0, uint32_t count, uint32_t levels,
-                                                     uint32_t* seq, uint32_t* sel) {
+                                                     uint32_t* seq, uint32_t* sel, uint32_t* nray) {
     const uint32_t s = blockIdx.x * kWg + threadIdx.x;
     if (s >= count) return;
-    uint32_t any = 0;
+    uint32_t rays = 0;
     for (uint32_t v = 0; v < 2; ++v) {
         const uint64_t j = (uint64_t)v * count + s;
         const uint64_t d0 = j * levels;
         uint32_t q = kRxEmpty;
         if ((flags[d0] & kRxRay) && slot[d0] != kRxEmpty) {
             q = ord[j] + offset0[slot[d0]];
-            any = 1u;
+            ++rays;
         }
         seq[j] = q;
     }
-    sel[s] = any;
+    sel[s] = rays ? 1u : 0u;
+    nray[s] = rays;
 }
 
-__global__ __launch_bounds__(kWg) void k_rx_compact(const uint32_t* seq, const uint32_t* pos, uint32_t count,
-                                                    uint32_t* tickets) {
+// Tickets (receivers with rays) and the ray list ((variant, receiver) index
+// j = variant * count + receiver), both in receiver order.
+__global__ __launch_bounds__(kWg) void k_rx_compact(const uint32_t* seq, const uint32_t* tpos, const uint32_t* rpos,
+                                                    uint32_t count, uint32_t* tickets, uint32_t* rays) {
     const uint32_t s = blockIdx.x * kWg + threadIdx.x;
     if (s >= count) return;
-    if (seq[s] != kRxEmpty || seq[(uint64_t)count + s] != kRxEmpty) tickets[pos[s]] = s;
-}
-
-// ---------------- exclusive scan (multi-block) ----------------
-__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* total) {
-    __shared__ uint32_t wsum[kScanBlock / 64];
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= (uint32_t)off) x += y;
-    }
-    if (lane == 63u) wsum[wave] = x;
-    __syncthreads();
-    if (wave == 0) {
-        uint32_t w = lane < kScanBlock / 64 ? wsum[lane] : 0u;
-#pragma unroll
-        for (int off = 1; off < 16; off <<= 1) {
-            uint32_t y = __shfl_up(w, off, 64);
-            if (lane >= (uint32_t)off) w += y;
-        }
-        if (lane < kScanBlock / 64) wsum[lane] = w;
-    }
-    __syncthreads();
-    const uint32_t before = wave ? wsum[wave - 1] : 0u;
-    *total = wsum[kScanBlock / 64 - 1];
-    return before + x - v;
-}
-
-__global__ __launch_bounds__(kScanBlock) void k_scan_blocks(uint32_t* data, uint32_t n, uint32_t* sums) {
-    const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
-    uint32_t total;
-    uint32_t e = block_scan_excl(i < n ? data[i] : 0u, &total);
-    if (i < n) data[i] = e;
-    if (threadIdx.x == 0) sums[blockIdx.x] = total;
-}
-
-// Scan of the block sums in one block (looping); the grand total goes to *out.
-__global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t* sums, uint32_t nb, uint32_t* out) {
-    __shared__ uint32_t carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < nb; base += kScanBlock) {
-        const uint32_t i = base + threadIdx.x;
-        uint32_t total;
-        const uint32_t c = carry;
-        uint32_t e = block_scan_excl(i < nb ? sums[i] : 0u, &total);
-        if (i < nb) sums[i] = c + e;
-        __syncthreads();
-        if (threadIdx.x == 0) carry = c + total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = carry;
-}
-
-__global__ __launch_bounds__(kScanBlock) void k_scan_add(uint32_t* data, uint32_t n, const uint32_t* sums) {
-    const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
-    if (i < n) data[i] += sums[blockIdx.x];
+    const bool a = seq[s] != kRxEmpty, b = seq[(uint64_t)count + s] != kRxEmpty;
+    if (a || b) tickets[tpos[s]] = s;
+    uint32_t p = rpos[s];
+    if (a) rays[p++] = s;
+    if (b) rays[p] = count + s;
 }
 
 // ---------------- receiver trace ----------------
@@ -433,83 +406,119 @@ struct RxGrids {
     RxGrid g[kRxMaxLods];
 };
 
+// deposit (split.slang:1056-1118) of one ray: the levels up to the
+// terminating one, those inside a grid written contiguously to kout / vout.
+__device__ void rx_deposits(const lux_gi_params& p, const RxGrids& grids, float3 pos, float3 d, uint32_t lod,
+                            bool hit, float t, float3 rad, uint32_t* kout, float4* vout, uint32_t& kept,
+                            uint32_t& outside) {
+    const uint32_t levels = p.levels;
+    const RxGrid* g = lod < grids.count ? &grids.g[lod] : nullptr;
+    const float lod_scale = pow2i((int)(lod < 24u ? lod : 24u));
+    for (uint32_t L = 0; L < levels; ++L) {
+        const float start = L == 0 ? 0.0f : interval_end(p, L - 1u) * lod_scale;
+        const float end = interval_end(p, L) * lod_scale;
+        const bool terminal = p.terminal_tail != 0u && L + 1u == levels;
+        const bool term = terminal || (hit && t > start && t <= end);
+        const int4 key = rx_cell(pos, p.spacing, L, lod);
+        bool in = false;
+        if (g) {
+            const float sl = g->spacing * pow2i((int)L);
+            const uint32_t dx = level_dim(g->dims[0], L), dy = level_dim(g->dims[1], L), dz = level_dim(g->dims[2], L);
+            const int64_t ix = (int64_t)key.x - (int64_t)floorf(g->origin[0] / sl);
+            const int64_t iy = (int64_t)key.y - (int64_t)floorf(g->origin[1] / sl);
+            const int64_t iz = (int64_t)key.z - (int64_t)floorf(g->origin[2] / sl);
+            in = ix >= 0 && iy >= 0 && iz >= 0 && ix < dx && iy < dy && iz < dz;
+            if (in) {
+                const uint32_t nphi = g->base_phi << L, ny = g->base_y << L;
+                const uint64_t probe = (uint64_t)ix + (uint64_t)dx * ((uint64_t)iy + (uint64_t)dy * (uint64_t)iz);
+                kout[kept] = g->base + (uint32_t)(g->offset[L] + probe * nphi * ny + direction_index(d, nphi, ny));
+                vout[kept] = term ? make_float4(rad.x, rad.y, rad.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            }
+        }
+        if (in) ++kept;
+        else ++outside;
+        if (term) break;
+    }
+}
+
+// Fused form: one thread per ray of the ray list traces, shades (emissive
+// NEE with its occlusion rays) and deposits; dcount[ray] = deposits kept.
 __global__ __launch_bounds__(kWg) void k_rx_trace(GiArgs a, RxGrids grids, const float* recv, uint32_t count,
-                                                  const uint32_t* tickets, uint32_t n_items, const uint32_t* seq,
-                                                  const uint32_t* lods, uint32_t frame, uint32_t* dep_key,
-                                                  float4* dep_val, uint32_t* counters) {
+                                                  const uint32_t* rays, uint32_t n_rays, const uint32_t* seq,
+                                                  const uint32_t* lods, uint32_t frame, uint32_t* dkey,
+                                                  float4* dval, uint32_t* dcount, uint32_t* counters) {
     __shared__ uint2 lds[kLdsDepth][kWg];
     Stack stk;
     stk.col = &lds[0][threadIdx.x];
     const uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    const lux_gi_params& p = a.p;
-    const uint32_t levels = p.levels;
     bool traced = false, unsupported = false;
     uint32_t kept = 0, outside = 0;
-    if (i < n_items) {
-        uint32_t* kout = dep_key + (uint64_t)i * levels;
-        float4* vout = dep_val + (uint64_t)i * levels;
-        for (uint32_t L = 0; L < levels; ++L) kout[L] = kRxEmpty;
-        const uint32_t smp = tickets[i >> 1];
-        const uint64_t j = (uint64_t)(i & 1u) * count + smp;
-        const uint32_t sq = seq[j];
-        float3 pos, n;
-        if (sq != kRxEmpty && rx_surface(recv + 6ull * smp, pos, n)) {
+    if (i < n_rays) {
+        const uint32_t j = rays[i];
+        float3 pos, o, d;
+        if (rx_ray(recv, count, j, seq[j], frame, a.p.normal_bias, pos, o, d)) {
             traced = true;
-            const uint32_t lod = lods[j];
-            const float2 jit = r2(transport_hash(frame ^ 0x5a17c9e3u));
-            const float2 q = r2(sq + 1u);
-            const float ux = fracf((0.5f + q.x) + jit.x), uy = fracf((0.5f + q.y) + jit.y);
-            float c, s;
-            cos_sin_2pi(ux, c, s);
-            const float y = uy * 2.0f - 1.0f;
-            const float qq = 1.0f - y * y;
-            const float r = sqrtf(qq > 0.0f ? qq : 0.0f);
-            float3 d = make_float3(r * c, y, r * s);
-            if (dot3(d, n) < 0.0f) d = make_float3(-d.x, -d.y, -d.z);
-            const float3 o = ray_offset(pos, n, p.normal_bias);
             lux::HitState h;
             lux::NoStats ns;
             lux::trace_closest(a.scene, o, d, 1e20f, h, stk, ns, false);
             float3 rad = make_float3(0.0f, 0.0f, 0.0f);
             const bool hit = h.tri != lux::kEmpty;
-            if ((h.err & lux::kErrUnsupported) || (hit && !shade(a, o, d, h, stk, rad))) {
+            if ((h.err & lux::kErrUnsupported) || (hit && !shade(a, o, d, h, stk, rad)))
                 unsupported = true;
-            } else {
-                const RxGrid* g = lod < grids.count ? &grids.g[lod] : nullptr;
-                const float lod_scale = pow2i((int)(lod < 24u ? lod : 24u));
-                for (uint32_t L = 0; L < levels; ++L) {
-                    const float start = L == 0 ? 0.0f : interval_end(p, L - 1u) * lod_scale;
-                    const float end = interval_end(p, L) * lod_scale;
-                    const bool terminal = p.terminal_tail != 0u && L + 1u == levels;
-                    const bool term = terminal || (hit && h.t > start && h.t <= end);
-                    const int4 key = rx_cell(pos, p.spacing, L, lod);
-                    bool in = false;
-                    if (g) {
-                        const float sl = g->spacing * pow2i((int)L);
-                        const uint32_t dx = level_dim(g->dims[0], L), dy = level_dim(g->dims[1], L),
-                                       dz = level_dim(g->dims[2], L);
-                        const int64_t ix = (int64_t)key.x - (int64_t)floorf(g->origin[0] / sl);
-                        const int64_t iy = (int64_t)key.y - (int64_t)floorf(g->origin[1] / sl);
-                        const int64_t iz = (int64_t)key.z - (int64_t)floorf(g->origin[2] / sl);
-                        in = ix >= 0 && iy >= 0 && iz >= 0 && ix < dx && iy < dy && iz < dz;
-                        if (in) {
-                            const uint32_t nphi = g->base_phi << L, ny = g->base_y << L;
-                            const uint64_t probe = (uint64_t)ix + (uint64_t)dx * ((uint64_t)iy + (uint64_t)dy * (uint64_t)iz);
-                            kout[L] = g->base + (uint32_t)(g->offset[L] + probe * nphi * ny + direction_index(d, nphi, ny));
-                            vout[L] = term ? make_float4(rad.x, rad.y, rad.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-                        }
-                    }
-                    if (in) ++kept;
-                    else ++outside;
-                    if (term) break;
-                }
-            }
+            else
+                rx_deposits(a.p, grids, pos, d, lods[j], hit, h.t, rad, dkey + (uint64_t)i * a.p.levels,
+                            dval + (uint64_t)i * a.p.levels, kept, outside);
         }
+        dcount[i] = kept;
     }
     wave_count(&counters[2], traced);
     wave_count(&counters[3], unsupported);
     if (kept) atomicAdd(&counters[4], kept);
     if (outside) atomicAdd(&counters[5], outside);
+}
+
+// Wavefront form: deposits of each ray from its traced (rad, t) and status.
+__global__ __launch_bounds__(kWg) void k_rx_deposit(lux_gi_params p, RxGrids grids, const float* recv,
+                                                    uint32_t count, const uint32_t* rays, uint32_t n_rays,
+                                                    const uint32_t* seq, const uint32_t* lods, uint32_t frame,
+                                                    const float4* traced_rt, const uint32_t* status,
+                                                    uint32_t* dkey, float4* dval, uint32_t* dcount,
+                                                    uint32_t* counters) {
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    bool traced = false, unsupported = false;
+    uint32_t kept = 0, outside = 0;
+    if (i < n_rays) {
+        const uint32_t j = rays[i];
+        float3 pos, o, d;
+        if (rx_ray(recv, count, j, seq[j], frame, p.normal_bias, pos, o, d)) {
+            traced = true;
+            const uint32_t stt = status[i];
+            const float4 rt = traced_rt[i];
+            if (stt == 2u)
+                unsupported = true;
+            else
+                rx_deposits(p, grids, pos, d, lods[j], stt == 1u, rt.w, make_float3(rt.x, rt.y, rt.z),
+                            dkey + (uint64_t)i * p.levels, dval + (uint64_t)i * p.levels, kept, outside);
+        }
+        dcount[i] = kept;
+    }
+    wave_count(&counters[2], traced);
+    wave_count(&counters[3], unsupported);
+    if (kept) atomicAdd(&counters[4], kept);
+    if (outside) atomicAdd(&counters[5], outside);
+}
+
+// Deposits of ray i from its fixed slots to [dpos[i], dpos[i + 1]).
+__global__ __launch_bounds__(kWg) void k_rx_pack(const uint32_t* dkey, const float4* dval, const uint32_t* dpos,
+                                                 uint32_t n_rays, uint32_t levels, uint32_t* key_out,
+                                                 float4* val_out) {
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n_rays) return;
+    const uint32_t p = dpos[i], n = dpos[i + 1] - p;
+    for (uint32_t k = 0; k < n; ++k) {
+        key_out[p + k] = dkey[(uint64_t)i * levels + k];
+        val_out[p + k] = dval[(uint64_t)i * levels + k];
+    }
 }
 
 struct RxOut {
@@ -526,30 +535,77 @@ __global__ __launch_bounds__(kWg) void k_rx_clear(float4* iv, uint32_t* sm, uint
     }
 }
 
-// reduce (split.slang:1386-1476): per slot, the mean of its deposits in
-// ticket order.
-__global__ __launch_bounds__(kWg) void k_rx_reduce(const uint32_t* sk, const uint32_t* sv, const float4* val,
-                                                   uint32_t n, RxOut out) {
-    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t k = sk[i];
-    if (k == kRxEmpty || (i > 0 && sk[i - 1] == k)) return;
-    float jx = 0.0f, jy = 0.0f, jz = 0.0f, b = 0.0f;
-    uint32_t c = 0;
-    for (uint32_t t = i; t < n && sk[t] == k; ++t) {
-        const float4 v = val[sv[t]];
-        jx = jx + v.x;
-        jy = jy + v.y;
-        jz = jz + v.z;
-        b = b + v.w;
-        ++c;
-    }
+// reduce (split.slang:1386-1476): per slot, the mean of its deposits.  The
+// sum of a run of deposits (in ray order) is fixed as: chunks of 64
+// consecutive deposits, each summed by a pairwise tree (x[i] += x[i + s] for
+// i a multiple of 2s, s = 1, 2, 4, ..., 32), the chunk sums added in order.
+// Runs of at most 8 deposits are summed by one thread, longer ones by a wave.
+__device__ __forceinline__ float4 add4(float4 a, float4 b) {
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+__device__ __forceinline__ void rx_write(const RxOut& out, uint32_t k, float4 sum, uint32_t c) {
     uint32_t g = 0;
     while (g + 1 < out.count && k >= out.base[g + 1]) ++g;
     const uint32_t local = k - out.base[g];
     const float fc = (float)c;
-    out.iv[g][local] = make_float4(jx / fc, jy / fc, jz / fc, b / fc);
+    out.iv[g][local] = make_float4(sum.x / fc, sum.y / fc, sum.z / fc, sum.w / fc);
     out.sm[g][local] = c;
+}
+
+__global__ __launch_bounds__(kWg) void k_rx_reduce_short(const uint32_t* sk, const uint32_t* sv, const float4* val,
+                                                         uint32_t n, RxOut out, uint32_t* long_runs,
+                                                         uint32_t* n_long) {
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = sk[i];
+    if (i > 0 && sk[i - 1] == k) return;
+    uint32_t c = 1;
+    while (c <= 8u && i + c < n && sk[i + c] == k) ++c;
+    if (c > 8u) {
+        long_runs[atomicAdd(n_long, 1u)] = i;
+        return;
+    }
+    float4 x[8];
+#pragma unroll
+    for (uint32_t t = 0; t < 8u; ++t) x[t] = t < c ? val[sv[i + t]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (uint32_t st = 1; st < 8u; st <<= 1)
+#pragma unroll
+        for (uint32_t t = 0; t + st < 8u; t += 2u * st)
+            if (t + st < c) x[t] = add4(x[t], x[t + st]);
+    rx_write(out, k, x[0], c);
+}
+
+__global__ __launch_bounds__(kWg) void k_rx_reduce_long(const uint32_t* sk, const uint32_t* sv, const float4* val,
+                                                        uint32_t n, RxOut out, const uint32_t* long_runs,
+                                                        const uint32_t* n_long) {
+    const uint32_t lane = __lane_id();
+    const uint32_t waves = gridDim.x * (kWg / 64u);
+    const uint32_t total = *n_long;
+    for (uint32_t r = blockIdx.x * (kWg / 64u) + (threadIdx.x >> 6); r < total; r += waves) {
+        const uint32_t start = long_runs[r];
+        const uint32_t k = sk[start];
+        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t c = 0;
+        for (uint32_t base = start;; base += 64u) {
+            const uint32_t pos = base + lane;
+            const bool in = pos < n && sk[pos] == k;
+            const uint32_t nc = (uint32_t)__popcll(__ballot(in));
+            if (nc == 0) break;
+            float4 x = in ? val[sv[pos]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (uint32_t st = 1; st < 64u; st <<= 1) {
+                float4 y = make_float4(__shfl_down(x.x, st, 64), __shfl_down(x.y, st, 64), __shfl_down(x.z, st, 64),
+                                       __shfl_down(x.w, st, 64));
+                if ((lane & (2u * st - 1u)) == 0u && lane + st < nc) x = add4(x, y);
+            }
+            acc = c == 0 ? x : add4(acc, x);
+            c += nc;
+            if (nc < 64u) break;
+        }
+        if (lane == 0) rx_write(out, k, acc, c);
+    }
 }
 
 // Descriptor view for downloads: canonical, count and offset of each
@@ -588,18 +644,8 @@ RxDir dir_of(lux_ctx* c) {
 
 // Exclusive scan of n words in place; the total is written to *total (device).
 int scan_exclusive(lux_ctx* c, uint32_t* data, uint32_t n, uint32_t* total) {
-    const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
-    if (int rc = lux_ensure(c, c->rx_scan, 4ull * (nb ? nb : 1))) return rc;
-    uint32_t* sums = static_cast<uint32_t*>(c->rx_scan.ptr);
-    if (nb == 0) {
-        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, c->stream, sums, 0u, total);
-    } else {
-        hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanBlock), 0, c->stream, data, n, sums);
-        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, c->stream, sums, nb, total);
-        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanBlock), 0, c->stream, data, n, sums);
-    }
-    LUX_HIP(c, hipGetLastError());
-    return LUX_OK;
+    if (int rc = lux_ensure(c, c->rx_scan, 4ull * lux_scan_sums_words(n))) return rc;
+    return lux_scan_exclusive_impl(c, data, n, static_cast<uint32_t*>(c->rx_scan.ptr), total);
 }
 
 // Stable sort of n u32 keys (values = input positions) in the sort scratch:
@@ -614,9 +660,16 @@ int sort_keys(lux_ctx* c, uint32_t n, uint32_t** keys_in, uint32_t** keys_out, u
     return LUX_OK;
 }
 
-int run_sort(lux_ctx* c, uint32_t n) {
+// Sort of keys below 2^bits.
+int run_sort(lux_ctx* c, uint32_t n, uint32_t bits) {
     uint32_t* w = static_cast<uint32_t*>(c->rx_sort.ptr);
-    return lux_sort_pairs_impl(c, w, n, w + n, w + 2ull * n, w + 3ull * n, w + 4ull * n, w + 5ull * n);
+    return lux_sort_pairs_impl(c, w, n, w + n, w + 2ull * n, w + 3ull * n, w + 4ull * n, w + 5ull * n, bits);
+}
+
+uint32_t bits_for(uint64_t max_key) {
+    uint32_t b = 1;
+    while (b < 32u && (max_key >> b) != 0u) ++b;
+    return b;
 }
 
 int check_pyramid(lux_ctx* const* lods, uint32_t lod_count) {
@@ -682,7 +735,7 @@ extern "C" int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count,
     if (int rc = lux_ensure(c, c->rx_ord, 4ull * m)) return rc;
     if (int rc = lux_ensure(c, c->rx_seq, 4ull * m)) return rc;
     if (int rc = lux_ensure(c, c->rx_dir, 16ull * levels * cap)) return rc;
-    if (int rc = lux_ensure(c, c->rx_tickets, 8ull * count)) return rc;
+    if (int rc = lux_ensure(c, c->rx_tickets, 20ull * count + 16)) return rc;
     if (int rc = lux_ensure(c, c->rx_counters, 64)) return rc;
     for (uint32_t k = 0; k < lod_count; ++k)
         if (int rc = lux_ensure(lods[k], lods[k]->gi_samples, 4ull * lods[k]->gi_total)) return rc;
@@ -700,8 +753,12 @@ extern "C" int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count,
     uint32_t* lodv = static_cast<uint32_t*>(c->rx_lod.ptr);
     uint32_t* ord = static_cast<uint32_t*>(c->rx_ord.ptr);
     uint32_t* seq = static_cast<uint32_t*>(c->rx_seq.ptr);
+    // receiver words: tickets, ticket flags (scanned into positions), ray
+    // counts (scanned into positions), the ray list (up to 2 per receiver)
     uint32_t* tickets = static_cast<uint32_t*>(c->rx_tickets.ptr);
-    uint32_t* sel = tickets + count;  // ticket flags, scanned into positions
+    uint32_t* tsel = tickets + count;
+    uint32_t* rpos = tsel + count;
+    uint32_t* rays = rpos + count;
     // seed
     RxSeed sd;
     std::memset(&sd, 0, sizeof sd);
@@ -723,13 +780,13 @@ extern "C" int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count,
     RxDir D = dir_of(c);
     const uint32_t clear_blocks = std::min<uint32_t>(blocks_for((uint64_t)levels * cap), 4096u);
     hipLaunchKernelGGL(k_rx_dir_clear, dim3(clear_blocks), dim3(kWg), 0, st, D);
-    hipLaunchKernelGGL(k_rx_insert, dim3(blocks_for(nd)), dim3(kWg), 0, st, D, keys, flags, nd, slot, counters);
+    hipLaunchKernelGGL(k_rx_insert, dim3(blocks_for(nd)), dim3(kWg), 0, st, D, keys, flags, m, slot, counters);
     uint32_t *sk_in, *sk, *sv;
     if (int rc = sort_keys(c, std::max<uint32_t>(m, 1u), &sk_in, &sk, &sv)) return rc;
-    hipLaunchKernelGGL(k_rx_ray_keys, dim3(blocks_for(m)), dim3(kWg), 0, st, flags, slot, m, levels, sk_in);
-    if (int rc = run_sort(c, m)) return rc;
-    hipLaunchKernelGGL(k_rx_run_start, dim3(blocks_for(m)), dim3(kWg), 0, st, sk, m, D.offset);
-    hipLaunchKernelGGL(k_rx_ordinals, dim3(blocks_for(m)), dim3(kWg), 0, st, sk, sv, m, D.offset, D.count, ord);
+    hipLaunchKernelGGL(k_rx_ray_keys, dim3(blocks_for(m)), dim3(kWg), 0, st, flags, slot, m, levels, cap, sk_in);
+    if (int rc = run_sort(c, m, bits_for(cap))) return rc;
+    hipLaunchKernelGGL(k_rx_run_start, dim3(blocks_for(m)), dim3(kWg), 0, st, sk, m, cap, D.offset);
+    hipLaunchKernelGGL(k_rx_ordinals, dim3(blocks_for(m)), dim3(kWg), 0, st, sk, sv, m, cap, D.offset, D.count, ord);
     // parents, roots, prefix
     for (uint32_t L = 1; L < levels; ++L)
         hipLaunchKernelGGL(k_rx_parents, dim3(blocks_for(cap)), dim3(kWg), 0, st, D, keys, L);
@@ -740,20 +797,23 @@ extern "C" int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count,
     hipLaunchKernelGGL(k_rx_root_offsets, dim3(blocks_for(cap)), dim3(kWg), 0, st, D, root_arr);
     for (uint32_t L = levels - 1u; L-- > 0;)
         hipLaunchKernelGGL(k_rx_prefix, dim3(blocks_for(cap)), dim3(kWg), 0, st, D, keys, L);
-    // generateDirections: sequences, tickets in receiver order
+    // generateDirections: sequences, tickets and the ray list in receiver order
     hipLaunchKernelGGL(k_rx_generate, dim3(blocks_for(count)), dim3(kWg), 0, st, flags, slot, ord, D.offset, count,
-                       levels, seq, sel);
-    if (int rc = scan_exclusive(c, sel, count, counters + 1)) return rc;
-    hipLaunchKernelGGL(k_rx_compact, dim3(blocks_for(count)), dim3(kWg), 0, st, seq, sel, count, tickets);
+                       levels, seq, tsel, rpos);
+    if (int rc = scan_exclusive(c, tsel, count, counters + 1)) return rc;
+    if (int rc = scan_exclusive(c, rpos, count, counters + 8)) return rc;
+    hipLaunchKernelGGL(k_rx_compact, dim3(blocks_for(count)), dim3(kWg), 0, st, seq, tsel, rpos, count, tickets,
+                       rays);
     LUX_HIP(c, hipGetLastError());
-    uint32_t nt = 0;
-    LUX_HIP(c, hipMemcpyAsync(&nt, counters + 1, 4, hipMemcpyDeviceToHost, st));
+    uint32_t nts[2] = {0, 0};
+    LUX_HIP(c, hipMemcpyAsync(&nts[0], counters + 1, 4, hipMemcpyDeviceToHost, st));
+    LUX_HIP(c, hipMemcpyAsync(&nts[1], counters + 8, 4, hipMemcpyDeviceToHost, st));
     LUX_HIP(c, hipStreamSynchronize(st));
-    c->rx_tickets_n = nt;
+    c->rx_tickets_n = nts[0];
+    const uint32_t n_rays = nts[1];
     // trace and deposit
-    const uint32_t items = 2u * nt;
-    const uint64_t n_dep = (uint64_t)items * levels;
-    if (n_dep >= 0xffffffffull) return lux_fail(c, LUX_ERR_INVALID, "lux_gi_update_receivers: too many deposits");
+    const uint64_t n_fixed = (uint64_t)n_rays * levels;
+    if (n_fixed >= 0xffffffffull) return lux_fail(c, LUX_ERR_INVALID, "lux_gi_update_receivers: too many deposits");
     RxGrids grids;
     std::memset(&grids, 0, sizeof grids);
     RxOut out;
@@ -781,16 +841,46 @@ extern "C" int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count,
                            out.iv[k], out.sm[k], (uint64_t)g->gi_total);
     }
     out.base[lod_count] = base;
-    if (items > 0) {
-        if (int rc = lux_ensure(c, c->rx_dep_val, 16ull * n_dep)) return rc;
-        if (int rc = sort_keys(c, (uint32_t)n_dep, &sk_in, &sk, &sv)) return rc;
-        GiArgs a = gi_args(c);
-        a.admit_limit = 0;  // the fair window bounds the work
-        hipLaunchKernelGGL(k_rx_trace, dim3(blocks_for(items)), dim3(kWg), 0, st, a, grids, receivers, count, tickets,
-                           items, seq, lodv, rp->frame, sk_in, static_cast<float4*>(c->rx_dep_val.ptr), counters);
-        if (int rc = run_sort(c, (uint32_t)n_dep)) return rc;
-        hipLaunchKernelGGL(k_rx_reduce, dim3(blocks_for(n_dep)), dim3(kWg), 0, st, sk, sv,
-                           static_cast<const float4*>(c->rx_dep_val.ptr), (uint32_t)n_dep, out);
+    if (n_rays > 0) {
+        // fixed deposit slots (key, value) per ray, then the compacted copies
+        if (int rc = lux_ensure(c, c->rx_dep_val, 20ull * n_fixed + 4ull * (n_rays + 1u))) return rc;
+        float4* dval = static_cast<float4*>(c->rx_dep_val.ptr);
+        uint32_t* dkey = reinterpret_cast<uint32_t*>(dval + n_fixed);
+        uint32_t* dpos = dkey + n_fixed;
+        if (int rc = lux_device_zero(c, dpos + n_rays, 4)) return rc;
+        if (c->gi_trace_mode == 1u) {
+            GiArgs a = gi_args(c);
+            a.admit_limit = 0;  // the fair window bounds the work
+            hipLaunchKernelGGL(k_rx_trace, dim3(blocks_for(n_rays)), dim3(kWg), 0, st, a, grids, receivers, count,
+                               rays, n_rays, seq, lodv, rp->frame, dkey, dval, dpos, counters);
+        } else {
+            // regrouped primary and occlusion walks; (rad, t) and status per ray
+            if (int rc = lux_ensure(c, c->rx_traced, 20ull * n_rays)) return rc;
+            float4* rt = static_cast<float4*>(c->rx_traced.ptr);
+            uint32_t* stt = reinterpret_cast<uint32_t*>(rt + n_rays);
+            if (int rc = lux_rx_wave_trace(c, receivers, count, rays, n_rays, seq, rp->frame, rt, stt)) return rc;
+            hipLaunchKernelGGL(k_rx_deposit, dim3(blocks_for(n_rays)), dim3(kWg), 0, st, c->gi, grids, receivers,
+                               count, rays, n_rays, seq, lodv, rp->frame, rt, stt, dkey, dval, dpos, counters);
+        }
+        if (int rc = scan_exclusive(c, dpos, n_rays + 1u, counters + 9)) return rc;
+        uint32_t n_dep = 0;
+        LUX_HIP(c, hipMemcpyAsync(&n_dep, counters + 9, 4, hipMemcpyDeviceToHost, st));
+        LUX_HIP(c, hipStreamSynchronize(st));
+        if (n_dep > 0) {
+            if (int rc = lux_ensure(c, c->rx_pack, 16ull * n_dep + 4ull * (n_dep + 2u))) return rc;
+            float4* cval = static_cast<float4*>(c->rx_pack.ptr);
+            uint32_t* long_runs = reinterpret_cast<uint32_t*>(cval + n_dep);
+            uint32_t* n_long = long_runs + n_dep;
+            if (int rc = sort_keys(c, n_dep, &sk_in, &sk, &sv)) return rc;
+            hipLaunchKernelGGL(k_rx_pack, dim3(blocks_for(n_rays)), dim3(kWg), 0, st, dkey, dval, dpos, n_rays, levels,
+                               sk_in, cval);
+            if (int rc = run_sort(c, n_dep, bits_for(base ? base - 1u : 0u))) return rc;
+            if (int rc = lux_device_zero(c, n_long, 4)) return rc;
+            hipLaunchKernelGGL(k_rx_reduce_short, dim3(blocks_for(n_dep)), dim3(kWg), 0, st, sk, sv, cval, n_dep, out,
+                               long_runs, n_long);
+            hipLaunchKernelGGL(k_rx_reduce_long, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(n_dep), 4096u))),
+                               dim3(kWg), 0, st, sk, sv, cval, n_dep, out, long_runs, n_long);
+        }
         LUX_HIP(c, hipGetLastError());
     }
     // history, merge and pack of every grid

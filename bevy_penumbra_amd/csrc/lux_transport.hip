@@ -143,7 +143,9 @@ __global__ __launch_bounds__(kWg) void k_sur_fallback(GiArgs a, uint32_t L, cons
 //                 surrogate's traced fallback)
 //   kWaveSampled  hashed (probe, direction) samples of one level as terminal
 //                 intervals -> 12-float training records
-constexpr uint32_t kWaveRange = 0u, kWaveListed = 1u, kWaveSampled = 2u;
+//   kWaveReceiver the rays of a receiver update's ray list (one per item) ->
+//                 per ray its shaded radiance, hit distance and status
+constexpr uint32_t kWaveRange = 0u, kWaveListed = 1u, kWaveSampled = 2u, kWaveReceiver = 3u;
 
 struct Wave {
     uint32_t kind;
@@ -163,13 +165,21 @@ struct Wave {
     float4* shadow;              // per occlusion ray: (origin, limit), (direction, slot bits)
     uint32_t* counters;          // [0] primary cursor, [1] shadow rays, [2] shadow cursor
     uint32_t regroup, batch;
+    // kWaveReceiver: receivers, ray list ((variant, receiver) indices),
+    // sequences, receiver count, frame; per ray (rad.rgb, t) and status bits
+    const float* recv;
+    const uint32_t* rays;
+    const uint32_t* seq;
+    uint32_t count, frame;
+    float4* rx_out;
+    uint32_t* rx_status;
 };
 
 // Intervals of the chunk (kWaveListed: listed probes beyond the list's
 // device-side length contribute none).
 __device__ __forceinline__ uint32_t wave_intervals(const GiArgs& a, const Wave& w) {
     if (w.kind == kWaveRange) return (uint32_t)(w.g1 - w.g0);
-    if (w.kind == kWaveSampled) return w.n;
+    if (w.kind == kWaveSampled || w.kind == kWaveReceiver) return w.n;
     uint32_t total = *w.list_count;
     uint32_t probes = total > w.first ? total - w.first : 0u;
     if (probes > w.n) probes = w.n;
@@ -193,6 +203,15 @@ __device__ __forceinline__ uint64_t wave_interval(const GiArgs& a, const Wave& w
 
 __device__ __forceinline__ void wave_item_ray(const GiArgs& a, const Wave& w, uint32_t i, float3& o, float3& d,
                                               float& lim, bool& terminal) {
+    if (w.kind == kWaveReceiver) {
+        const uint32_t j = w.rays[w.first + i];
+        float3 pos;
+        // a listed ray always has a surface (its receiver was seeded)
+        rx_ray(w.recv, w.count, j, w.seq[j], w.frame, a.p.normal_bias, pos, o, d);
+        lim = 1e20f;
+        terminal = true;
+        return;
+    }
     bool force;
     uint64_t g = wave_interval(a, w, i / w.R, force);
     item_ray(a, g, i % w.R, force, o, d, lim, terminal);
@@ -386,6 +405,47 @@ __global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
         rec[10] = ok ? m.y : nan;
         rec[11] = ok ? m.z : nan;
     }
+}
+
+// Receiver rays: the fused shade()'s sum of visible NEE contributions in
+// sample order, per ray (rad.rgb, t) and status (1 hit, 2 unsupported).
+__global__ __launch_bounds__(kWg) void k_rx_wave_finish(GiArgs a, Wave w) {
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    if (i >= w.n) return;
+    uint32_t count = a.p.emissive_samples / 2u;
+    if (count < 1u) count = 1u;
+    const uint4 pr = w.prim[i];
+    float3 rad = make_float3(0.0f, 0.0f, 0.0f);
+    uint32_t status = 0;
+    if (pr.w & lux::kErrUnsupported) {
+        status = 2u;
+    } else if (pr.y != lux::kEmpty) {
+        status = 1u;
+        const float4 sh = w.shade[i];
+        float3 rr = make_float3(sh.x, sh.y, sh.z);
+        if (sh.w != 0.0f) {
+            float3 sum = make_float3(0.0f, 0.0f, 0.0f);
+            for (uint32_t smp = 0; smp < count; ++smp) {
+                const uint32_t slot = i * w.nee + smp;
+                const float4 s4 = w.slots[slot];
+                if (s4.w == 0.0f) continue;
+                const uint32_t v = w.vis[slot];
+                if (v == 1u) continue;
+                if (v == 2u) {
+                    status = 2u;
+                    break;
+                }
+                sum.x = sum.x + s4.x;
+                sum.y = sum.y + s4.y;
+                sum.z = sum.z + s4.z;
+            }
+            const float fc = (float)count;
+            rr = make_float3(rr.x + sum.x / fc, rr.y + sum.y / fc, rr.z + sum.z / fc);
+        }
+        if (status == 1u) rad = shade_end(rr);
+    }
+    w.rx_out[w.first + i] = make_float4(rad.x, rad.y, rad.z, __uint_as_float(pr.x));
+    w.rx_status[w.first + i] = status;
 }
 
 // Temporal blend of intervals [0, count) (levels are contiguous).
@@ -854,7 +914,8 @@ namespace {
 // probes, else 1).
 int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
     const lux_gi_params& p = ctx->gi;
-    const uint32_t R = p.rays_per_interval ? p.rays_per_interval : 1u;
+    const bool rx = proto.kind == kWaveReceiver;
+    const uint32_t R = rx ? 1u : (p.rays_per_interval ? p.rays_per_interval : 1u);
     const uint32_t nee = p.emissive_samples / 2u > 1u ? p.emissive_samples / 2u : 1u;
     const uint64_t per_item = 32ull + (uint64_t)nee * (16ull + 4ull + 32ull);
     uint64_t chunk_items = (2ull << 30) / per_item;
@@ -871,6 +932,7 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
     if (int rc = lux_ensure(ctx, ctx->gi_w_shadow, 32ull * items * nee)) return rc;
     if (int rc = lux_ensure(ctx, ctx->gi_w_counters, 16)) return rc;
     GiArgs a = gi_args(ctx);
+    if (rx) a.admit_limit = 0;  // a receiver update is bounded by its fair window
     Wave w = proto;
     w.R = R;
     w.nee = nee;
@@ -910,7 +972,8 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
         case 16: hipLaunchKernelGGL(k_gi_shadow<16>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
         default: hipLaunchKernelGGL(k_gi_shadow<8>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
         }
-        hipLaunchKernelGGL(k_gi_accum, dim3((max_intervals + kWg - 1) / kWg), dim3(kWg), 0, st, a, w);
+        if (rx) hipLaunchKernelGGL(k_rx_wave_finish, dim3((max_intervals + kWg - 1) / kWg), dim3(kWg), 0, st, a, w);
+        else hipLaunchKernelGGL(k_gi_accum, dim3((max_intervals + kWg - 1) / kWg), dim3(kWg), 0, st, a, w);
         LUX_HIP(ctx, hipGetLastError());
     }
     return LUX_OK;
@@ -923,6 +986,25 @@ Wave wave_proto(uint32_t kind) {
     return w;
 }
 
+}  // namespace
+
+// Receiver rays through the wavefront stages (regrouped primary and
+// occlusion walks): per ray of the list (rad.rgb, t) and status.
+int lux_rx_wave_trace(lux_ctx* ctx, const float* recv, uint32_t count, const uint32_t* rays, uint32_t n_rays,
+                      const uint32_t* seq, uint32_t frame, float4* out, uint32_t* status) {
+    if (n_rays == 0) return LUX_OK;
+    Wave w = wave_proto(kWaveReceiver);
+    w.recv = recv;
+    w.rays = rays;
+    w.seq = seq;
+    w.count = count;
+    w.frame = frame;
+    w.rx_out = out;
+    w.rx_status = status;
+    return run_wave(ctx, w, n_rays, 1);
+}
+
+namespace {
 int trace_wavefront(lux_ctx* ctx, uint64_t g0, uint64_t g1) {
     Wave w = wave_proto(kWaveRange);
     w.g0 = g0;

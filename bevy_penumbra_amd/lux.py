@@ -266,22 +266,23 @@ def update_receivers(contexts, view, receivers, params: ReceiverParams) -> Recei
     return st
 
 
-def receivers_download(ctx, count: int, levels: int) -> dict:
+def receivers_download(ctx, count: int, levels: int, directory: bool = True) -> dict:
     """The last receiver update's selection on contexts[0] (see
-    lux_receivers_download)."""
+    lux_receivers_download); directory=False skips the per-descriptor keys,
+    canonical indices, counts and offsets."""
     nd = 2 * count * levels
-    keys = np.zeros((nd, 4), np.int32)
+    keys = np.zeros((nd, 4), np.int32) if directory else None
     flags = np.zeros(nd, np.uint32)
-    canon = np.zeros(nd, np.uint32)
-    cnt = np.zeros(nd, np.uint32)
-    off = np.zeros(nd, np.uint32)
+    canon = np.zeros(nd, np.uint32) if directory else None
+    cnt = np.zeros(nd, np.uint32) if directory else None
+    off = np.zeros(nd, np.uint32) if directory else None
     lods = np.zeros((2, count), np.uint32)
     seq = np.zeros((2, count), np.uint32)
     tickets = np.zeros(max(count, 1), np.uint32)
     nt = C.c_uint32()
-    ctx._check(ctx.lib.lux_receivers_download(ctx.h, keys.ctypes.data, flags.ctypes.data, canon.ctypes.data,
-                                              cnt.ctypes.data, off.ctypes.data, lods.ctypes.data, seq.ctypes.data,
-                                              tickets.ctypes.data, C.byref(nt)), "lux_receivers_download")
+    ctx._check(ctx.lib.lux_receivers_download(ctx.h, _ptr(keys), _ptr(flags), _ptr(canon), _ptr(cnt), _ptr(off),
+                                              _ptr(lods), _ptr(seq), _ptr(tickets), C.byref(nt)),
+               "lux_receivers_download")
     return dict(keys=keys, flags=flags, canonical=canon, count=cnt, offset=off, lods=lods, sequence=seq,
                 tickets=tickets[:nt.value])
 

@@ -613,7 +613,10 @@ uint32_t orc_direction_index(const float d[3], uint32_t nphi, uint32_t ny) {
  * deposit (:1056-1118: J / beta per level up to the terminating one) and
  * reduce (:1386-1476: mean over the update's samples of a key), the keys
  * (cell, direction bin, level, LOD) addressed in the dense grid of their
- * LOD.  Sums run in ticket order. */
+ * LOD.  The deposits of a key are in ray order (tickets, then variants,
+ * then levels); their sum is fixed as chunks of 64 summed by a pairwise tree,
+ * the chunk sums added in order (the reference's order is that of its
+ * hash-bucket lists, left open by its atomics). */
 void orc_rx_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights, const orc_material* mats,
                   uint32_t n_mats, const orc_gi_params* const* grids, uint32_t lod_count, const float* receivers,
                   uint32_t count, const uint32_t* lods, const uint32_t* sequence, const uint32_t* tickets,
@@ -624,11 +627,11 @@ void orc_rx_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights
     const orc_gi_params* p0 = &p0c;
     const uint32_t levels = p0->levels;
     stats[0] = stats[1] = stats[2] = stats[3] = 0;
-    for (uint32_t k = 0; k < lod_count; ++k) {
-        uint64_t n = orc_gi_level_offset(grids[k], grids[k]->levels);
-        memset(intervals[k], 0, sizeof(float) * 4 * n);
-        memset(samples[k], 0, sizeof(uint32_t) * n);
-    }
+    /* deposits in ray order: (grid, slot) and J.rgb / beta */
+    size_t cap = 1024, nd = 0;
+    uint32_t* dg = (uint32_t*)malloc(sizeof(uint32_t) * cap);
+    uint64_t* ds = (uint64_t*)malloc(sizeof(uint64_t) * cap);
+    float* dv = (float*)malloc(sizeof(float) * 4 * cap);
     float jit[2];
     r2(transport_hash(frame ^ 0x5a17c9e3u), jit);
     for (uint32_t t = 0; t < n_tickets; ++t) {
@@ -684,11 +687,17 @@ void orc_rx_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights
                         uint32_t D = orc_gi_level_directions(g, L);
                         uint32_t dir = orc_direction_index(d, g->base_phi << L, g->base_y << L);
                         uint64_t probe = (uint64_t)idx[0] + (uint64_t)dims[0] * ((uint64_t)idx[1] + (uint64_t)dims[1] * (uint64_t)idx[2]);
-                        uint64_t slot = orc_gi_level_offset(g, L) + probe * D + dir;
-                        float* iv = intervals[lod] + 4 * slot;
-                        for (int c = 0; c < 3; ++c) iv[c] = iv[c] + (terminates ? rad[c] : 0.0f);
-                        iv[3] = iv[3] + (terminates ? 0.0f : 1.0f);
-                        samples[lod][slot]++;
+                        if (nd == cap) {
+                            cap *= 2;
+                            dg = (uint32_t*)realloc(dg, sizeof(uint32_t) * cap);
+                            ds = (uint64_t*)realloc(ds, sizeof(uint64_t) * cap);
+                            dv = (float*)realloc(dv, sizeof(float) * 4 * cap);
+                        }
+                        dg[nd] = lod;
+                        ds[nd] = orc_gi_level_offset(g, L) + probe * D + dir;
+                        for (int c = 0; c < 3; ++c) dv[4 * nd + c] = terminates ? rad[c] : 0.0f;
+                        dv[4 * nd + 3] = terminates ? 0.0f : 1.0f;
+                        nd++;
                         kept = 1;
                     }
                 }
@@ -697,18 +706,49 @@ void orc_rx_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights
             }
         }
     }
+    /* reduce: per slot its deposits in ray order (a stable counting sort),
+     * summed in chunks of 64 by a pairwise tree (x[i] += x[i + s] for i a
+     * multiple of 2s, s = 1, 2, ..., 32), chunk sums added in order */
     for (uint32_t k = 0; k < lod_count; ++k) {
         uint64_t n = orc_gi_level_offset(grids[k], grids[k]->levels);
+        uint32_t* start = (uint32_t*)calloc((size_t)n + 1, sizeof(uint32_t));
+        for (size_t e = 0; e < nd; ++e)
+            if (dg[e] == k) start[ds[e] + 1]++;
+        for (uint64_t g = 0; g < n; ++g) start[g + 1] += start[g];
+        uint32_t* fill = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n + 1));
+        memcpy(fill, start, sizeof(uint32_t) * ((size_t)n + 1));
+        uint32_t total = start[n];
+        uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * (total ? total : 1));
+        for (size_t e = 0; e < nd; ++e)
+            if (dg[e] == k) order[fill[ds[e]]++] = (uint32_t)e;
         for (uint64_t g = 0; g < n; ++g) {
             float* iv = intervals[k] + 4 * g;
-            uint32_t c = samples[k][g];
+            uint32_t c = start[g + 1] - start[g];
+            samples[k][g] = c;
             if (c == 0) {
                 iv[0] = iv[1] = iv[2] = 0.0f;
                 iv[3] = -1.0f;
-            } else {
-                float fc = (float)c;
-                iv[0] = iv[0] / fc; iv[1] = iv[1] / fc; iv[2] = iv[2] / fc; iv[3] = iv[3] / fc;
+                continue;
             }
+            float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (uint32_t b = 0; b < c; b += 64u) {
+                uint32_t m = c - b < 64u ? c - b : 64u;
+                float x[64][4];
+                for (uint32_t t = 0; t < m; ++t)
+                    for (int q = 0; q < 4; ++q) x[t][q] = dv[4 * (size_t)order[start[g] + b + t] + q];
+                for (uint32_t st = 1; st < 64u; st <<= 1)
+                    for (uint32_t t = 0; t + st < m; t += 2u * st)
+                        for (int q = 0; q < 4; ++q) x[t][q] = x[t][q] + x[t + st][q];
+                for (int q = 0; q < 4; ++q) acc[q] = b == 0 ? x[0][q] : acc[q] + x[0][q];
+            }
+            float fc = (float)c;
+            for (int q = 0; q < 4; ++q) iv[q] = acc[q] / fc;
         }
+        free(start);
+        free(fill);
+        free(order);
     }
+    free(dg);
+    free(ds);
+    free(dv);
 }

@@ -8,7 +8,9 @@ counts, the merged intervals and the packed probes of every LOD grid equal
 the oracle's receiver trace, merge and pack bit for bit -- with every
 receiver tracing, with a fair window (budget and cursor), with the LOD
 overlap of a perspective camera (deposits of LODs without a grid dropped),
-and with the temporal history over consecutive updates.
+with long runs of deposits per interval, and with the temporal history over
+consecutive updates; the receiver rays traced as regrouped wavefront stages
+and as the fused kernel.
 """
 import numpy as np
 import pytest
@@ -68,7 +70,7 @@ def camera_receivers(ctx):
 def oracle_update(sc, p0, n_lods, rec, rp, oview):
     levels = p0.levels
     keys, flags, lods = O.rx_seed(rec, rp, oview, p0.spacing, levels)
-    sel = O.rx_select(W * H, levels, keys, flags)
+    sel = O.rx_select(rp.width * rp.height, levels, keys, flags)
     grids = []
     for k in range(n_lods):
         g = to_oracle(lux.lod_gi_params(p0, k))
@@ -81,12 +83,14 @@ def oracle_update(sc, p0, n_lods, rec, rp, oview):
     return dict(keys=keys, flags=flags, lods=lods, sel=sel, grids=grids, ivs=ivs, sms=sms, stats=st)
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["wavefront", "fused"])
 @pytest.mark.parametrize("case", ["all_receivers", "fair_window_lod_pair"])
-def test_gpu_receiver_update_matches_oracle(case):
+def test_gpu_receiver_update_matches_oracle(case, mode):
     sc, _ = scenes.cornell()
     p0 = base_params()
     n_lods = 1 if case == "all_receivers" else 2
     ctxs = pyramid(sc, p0, n_lods)
+    ctxs[0].gi_set_trace_mode(mode)
     try:
         recv, rec, terms = camera_receivers(ctxs[0])
         if case == "all_receivers":
@@ -122,6 +126,37 @@ def test_gpu_receiver_update_matches_oracle(case):
             bits_equal(g["merged"], merged, f"merged LOD {k}")
             bits_equal(g["probes"], O.gi_pack(o["grids"][k], merged), f"probes LOD {k}")
         assert o["sms"][0].sum() > 0
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+def test_gpu_receiver_long_runs_match_oracle():
+    """4096 receivers inside one level-0 cell of the floor: level-0 slots
+    gather about 128 deposits each (two 64-deposit chunks of the fixed sum
+    order, the wave path of the reduce), coarse slots thousands."""
+    sc, _ = scenes.cornell()
+    p0 = base_params()
+    ctxs = pyramid(sc, p0, 1)
+    try:
+        n = 64 * 64
+        rng = np.random.default_rng(11)
+        rec = np.zeros((n, 6), np.float32)
+        rec[:, 0] = rng.uniform(1.01, 1.11, n)
+        rec[:, 1] = 0.0
+        rec[:, 2] = rng.uniform(0.51, 0.61, n)
+        rec[:, 4] = 1.0
+        recv = DeviceBuffer.from_array(ctxs[0], rec)
+        st = lux.update_receivers(ctxs, None, recv, lux.receiver_params(64, 64, 0, frame=2))
+        o = oracle_update(sc, p0, 1, rec, O.receiver_params(64, 64, 0, frame=2), None)
+        sm = ctxs[0].gi_samples_download()
+        assert np.array_equal(sm, o["sms"][0])
+        assert sm.max() > 128 and (sm[:O.level_offset(o["grids"][0], 1)] > 64).sum() >= 8
+        g = ctxs[0].gi_download()
+        bits_equal(g["intervals"], o["ivs"][0], "intervals")
+        merged = O.gi_merge(o["grids"][0], o["ivs"][0])
+        bits_equal(g["merged"], merged, "merged")
+        assert st.deposits == int(o["stats"][2])
     finally:
         for c in ctxs:
             c.close()
