@@ -100,6 +100,10 @@ struct lux_ctx {
     lux::DevScene dev_scene() const;
 };
 
+// Autotuners keep the incumbent launch configuration unless a candidate's
+// median time is more than 1.5 % lower (run-to-run noise of a timed launch).
+constexpr float kTuneMargin = 0.985f;
+
 int lux_fail(lux_ctx* ctx, int code, const std::string& msg);
 int lux_hip_check(lux_ctx* ctx, hipError_t e, const char* what);
 int lux_ensure(lux_ctx* ctx, lux_buffer& b, size_t bytes);

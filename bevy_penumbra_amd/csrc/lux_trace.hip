@@ -342,7 +342,8 @@ extern "C" int lux_autotune_trace(lux_ctx* ctx, const lux_ray* rays, uint32_t n,
                 float ms = 0.0f;
                 rc = measure(c, ms);
                 ++evals;
-                if (rc == LUX_OK && ms < cur_ms) {
+                // a candidate must beat the incumbent by more than the timing noise
+                if (rc == LUX_OK && ms < cur_ms * kTuneMargin) {
                     cur_ms = ms;
                     cur[k] = c[k];
                 }

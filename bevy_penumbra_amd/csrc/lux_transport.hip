@@ -1093,7 +1093,8 @@ extern "C" int lux_autotune_gi(lux_ctx* ctx, uint32_t repeats, lux_trace_config*
                 float ms = 0.0f;
                 rc = measure(c, ms);
                 ++evals;
-                if (rc == LUX_OK && ms < cur_ms) {
+                // a candidate must beat the incumbent by more than the timing noise
+                if (rc == LUX_OK && ms < cur_ms * kTuneMargin) {
                     cur_ms = ms;
                     cur[k] = c[k];
                 }
