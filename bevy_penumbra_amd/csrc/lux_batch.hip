@@ -27,6 +27,7 @@ namespace {
 
 constexpr uint32_t kMaxBatchLevels = 8;
 
+
 struct BatchScene {
     GiArgs a;
     uint32_t R;             // rays per interval
@@ -47,6 +48,8 @@ struct BatchWave {
     uint32_t* vis;
     float4* shadow;
     uint32_t* counters;  // [0] primary cursor, [1] occlusion rays, [2] occlusion cursor
+    float4* slab;        // per setup wave: its occlusion rays (64 * nee_max)
+    uint32_t* wave_cnt;  // per setup wave: its ray count, then scanned into offsets
     uint32_t regroup, batch;
 };
 
@@ -136,10 +139,12 @@ __global__ __launch_bounds__(kWg) void k_b_primary(BatchWave w) {
 }
 
 __global__ __launch_bounds__(kWg) void k_b_setup(BatchWave w) {
-    uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    if (i >= w.item_base[w.n_scenes]) return;
-    uint4 pr = w.prim[i];
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    const uint32_t gw = blockIdx.x * (kWg / 64u) + (threadIdx.x >> 6);
+    const bool live = i < w.item_base[w.n_scenes];
+    uint4 pr = live ? w.prim[i] : make_uint4(0u, lux::kEmpty, lux::kEmpty, lux::kErrUnsupported);
     float4 sh = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t running = 0;  // rays of this wave so far (the same in every lane of a ballot)
     if ((pr.w & lux::kErrUnsupported) == 0u && pr.y != lux::kEmpty) {
         uint32_t s = find_scene(w.item_base, w.n_scenes, i);
         const BatchScene& S = w.sc[s];
@@ -168,18 +173,18 @@ __global__ __launch_bounds__(kWg) void k_b_setup(BatchWave w) {
                 w.slots[slot] = make_float4(t3.x, t3.y, t3.z, q ? 1.0f : 0.0f);
                 uint64_t mask = __ballot(q);
                 if (q) {
-                    uint32_t leader = (uint32_t)(__ffsll((unsigned long long)mask) - 1);
-                    uint32_t base = 0;
-                    if (lane == leader) base = atomicAdd(&w.counters[1], (uint32_t)__popcll(mask));
-                    base = __shfl(base, (int)leader, 64);
-                    uint32_t at = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-                    w.shadow[2ull * at] = make_float4(origin.x, origin.y, origin.z, sl);
-                    w.shadow[2ull * at + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(slot));
+                    const uint64_t at = (uint64_t)gw * 64u * w.nee_max + running +
+                                        (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+                    w.slab[2ull * at] = make_float4(origin.x, origin.y, origin.z, sl);
+                    w.slab[2ull * at + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(slot));
                 }
+                running += (uint32_t)__popcll(mask);
             }
         }
     }
-    w.shade[i] = sh;
+    if (live) w.shade[i] = sh;
+    const uint32_t total = wave_max_u32(running);
+    if (__lane_id() == 0u) w.wave_cnt[gw] = total;
 }
 
 struct BatchShadow {
@@ -368,6 +373,7 @@ struct lux_batch {
     uint32_t nee_max = 1, max_levels = 1;
     bool any_history = false;
     uint64_t query_max = 0;
+    uint64_t setup_waves = 0;  // waves of the NEE setup launch (occlusion slabs)
     lux_ctx* owner = nullptr;  // allocations go through the first scene's context
     lux_buffer table, bases, prim, shade, slots, vis, shadow, counters, queries, sur_scratch;
     uint32_t wpc = 16, depth = 16, regroup = 16, batch = 8;
@@ -456,8 +462,13 @@ int prepare(lux_batch* b) {
     if (int rc = ensure(b, b->shade, 16 * items + 16)) return rc;
     if (int rc = ensure(b, b->slots, 16 * items * b->nee_max + 16)) return rc;
     if (int rc = ensure(b, b->vis, 4 * items * b->nee_max + 16)) return rc;
-    if (int rc = ensure(b, b->shadow, 32 * items * b->nee_max + 32)) return rc;
-    if (int rc = ensure(b, b->counters, 16)) return rc;
+    // occlusion rays: per setup wave a slab of 64 * nee_max, then the packed list
+    b->setup_waves = ((items + kWg - 1) / kWg) * (kWg / 64u);
+    if (b->setup_waves * 64u * b->nee_max >= (1ull << 31))
+        return bfail(b, LUX_ERR_INVALID, "batch too large for one launch (occlusion list)");
+    if (int rc = ensure(b, b->shadow, 64ull * b->setup_waves * 64u * b->nee_max + 32)) return rc;
+    if (int rc = ensure(b, b->counters, 4ull * (4u + b->setup_waves + 1u + lux_scan_sums_words((uint32_t)b->setup_waves + 1u))))
+        return rc;
     if (int rc = ensure(b, b->queries, 36 * b->query_max + 36)) return rc;
     if (int rc = ensure(b, b->sur_scratch, lux_surrogate_scratch_bytes((uint32_t)b->query_max) + 256)) return rc;
     std::vector<uint64_t> all;
@@ -488,6 +499,8 @@ BatchWave wave_of(lux_batch* b) {
     w.vis = static_cast<uint32_t*>(b->vis.ptr);
     w.shadow = static_cast<float4*>(b->shadow.ptr);
     w.counters = static_cast<uint32_t*>(b->counters.ptr);
+    w.wave_cnt = w.counters + 4;
+    w.slab = w.shadow + 2ull * b->setup_waves * 64u * b->nee_max;
     w.regroup = b->regroup ? b->regroup : 1u;
     w.batch = b->batch ? b->batch : 8u;
     return w;
@@ -512,6 +525,13 @@ int enqueue(lux_batch* b) {
         default: hipLaunchKernelGGL(k_b_primary<16>, dim3(pb), dim3(kWg), 0, st, w); break;
         }
         hipLaunchKernelGGL(k_b_setup, dim3(need), dim3(kWg), 0, st, w);
+        const uint32_t nw = need * (kWg / 64u);
+        hipLaunchKernelGGL(k_fill_u32_one, dim3(1), dim3(1), 0, st, w.wave_cnt + nw, 0u);
+        uint32_t* scan_sums = w.wave_cnt + b->setup_waves + 1u;
+        if (int rc = lux_scan_exclusive_on(b->owner, st, w.wave_cnt, nw + 1u, scan_sums, &w.counters[1]))
+            return bfail(b, rc, lux_last_error(b->owner));
+        hipLaunchKernelGGL(k_shadow_compact, dim3((nw + 3u) / 4u), dim3(256), 0, st, w.slab, w.wave_cnt, nw,
+                           64u * w.nee_max, w.shadow);
         switch (b->depth) {
         case 4: hipLaunchKernelGGL(k_b_shadow<4>, dim3(pblocks), dim3(kWg), 0, st, w); break;
         case 8: hipLaunchKernelGGL(k_b_shadow<8>, dim3(pblocks), dim3(kWg), 0, st, w); break;

@@ -388,7 +388,11 @@ inline uint32_t blocks_for(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
 size_t lux_scan_sums_words(uint32_t n) { return (size_t)(n + kScanBlock - 1) / kScanBlock + 2; }
 
 int lux_scan_exclusive_impl(lux_ctx* ctx, uint32_t* data, uint32_t n, uint32_t* sums, uint32_t* total) {
-    hipStream_t st = ctx->stream;
+    return lux_scan_exclusive_on(ctx, ctx->stream, data, n, sums, total);
+}
+
+int lux_scan_exclusive_on(lux_ctx* ctx, hipStream_t st, uint32_t* data, uint32_t n, uint32_t* sums,
+                          uint32_t* total) {
     const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
     if (nb) hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanBlock), 0, st, data, n, sums);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, st, sums, nb, total);

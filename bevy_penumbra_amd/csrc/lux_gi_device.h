@@ -808,6 +808,34 @@ __device__ __forceinline__ bool rx_ray(const float* recv, uint32_t count, uint32
     return true;
 }
 
+// Occlusion rays of the NEE setup: each setup wave writes its rays to its
+// own slab (wave_id * 64 * nee rays, ranked by ballots, no atomics) and its
+// count; an exclusive scan over the counts and k_shadow_compact pack them
+// into one list in item order for the occlusion walks.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__global__ void k_fill_u32_one(uint32_t* p, uint32_t v) { *p = v; }
+
+// One wave per setup wave: its rays [0, count) from the slab to the list.
+__global__ __launch_bounds__(256) void k_shadow_compact(const float4* slab, const uint32_t* off, uint32_t n_waves,
+                                                        uint32_t slab_rays, float4* list) {
+    const uint32_t gw = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (gw >= n_waves) return;
+    const uint32_t base = off[gw], cnt = off[gw + 1] - base;
+    const float4* src = slab + 2ull * gw * slab_rays;
+    for (uint32_t j = __lane_id(); j < cnt; j += 64u) {
+        list[2ull * (base + j)] = src[2ull * j];
+        list[2ull * (base + j) + 1] = src[2ull * j + 1];
+    }
+}
+
 // Kernel arguments of a configured context (host).
 inline GiArgs gi_args(lux_ctx* ctx) {
     GiArgs a;

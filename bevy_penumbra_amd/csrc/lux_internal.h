@@ -131,6 +131,9 @@ size_t lux_sort_hist_words(uint32_t n);
 // Exclusive scan of n words in place on the context stream; sums holds
 // lux_scan_sums_words(n) words, *total (device) receives the sum.
 int lux_scan_exclusive_impl(lux_ctx* ctx, uint32_t* data, uint32_t n, uint32_t* sums, uint32_t* total);
+// The same on stream st (no host synchronisation; capturable in a graph).
+int lux_scan_exclusive_on(lux_ctx* ctx, hipStream_t st, uint32_t* data, uint32_t n, uint32_t* sums,
+                          uint32_t* total);
 size_t lux_scan_sums_words(uint32_t n);
 
 // Surrogate forward pass over device queries (9 floats each); *Y receives a

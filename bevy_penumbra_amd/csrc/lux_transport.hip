@@ -131,8 +131,8 @@ __global__ __launch_bounds__(kWg) void k_sur_fallback(GiArgs a, uint32_t L, cons
 // the shadow traversals run as regrouped walks (lux::regroup_walks):
 //   k_gi_primary  closest-hit walk of every (interval, ray) item
 //   k_gi_setup    per hit: emission and NEE context; each NEE sample's
-//                 contribution goes to a fixed slot, its occlusion ray to a
-//                 compacted list (wave-aggregated append)
+//                 contribution goes to a fixed slot, its occlusion ray to the
+//                 wave's slab (then packed into one list in item order)
 //   k_gi_shadow   occlusion walks of the list -> visibility per slot
 //   k_gi_accum    per interval: the fused kernel's accumulation in sample
 //                 order (same operations, so the same bits)
@@ -146,6 +146,8 @@ __global__ __launch_bounds__(kWg) void k_sur_fallback(GiArgs a, uint32_t L, cons
 //   kWaveReceiver the rays of a receiver update's ray list (one per item) ->
 //                 per ray its shaded radiance, hit distance and status
 constexpr uint32_t kWaveRange = 0u, kWaveListed = 1u, kWaveSampled = 2u, kWaveReceiver = 3u;
+
+
 
 struct Wave {
     uint32_t kind;
@@ -164,6 +166,8 @@ struct Wave {
     uint32_t* vis;               // per slot: 0 visible, 1 occluded, 2 unsupported
     float4* shadow;              // per occlusion ray: (origin, limit), (direction, slot bits)
     uint32_t* counters;          // [0] primary cursor, [1] shadow rays, [2] shadow cursor
+    float4* slab;                // per setup wave: its occlusion rays (64 * nee)
+    uint32_t* wave_cnt;          // per setup wave: its ray count, then scanned into offsets
     uint32_t regroup, batch;
     // kWaveReceiver: receivers, ray list ((variant, receiver) indices),
     // sequences, receiver count, frame; per ray (rad.rgb, t) and status bits
@@ -253,10 +257,12 @@ __global__ __launch_bounds__(kWg) void k_gi_primary(GiArgs a, Wave w) {
 }
 
 __global__ __launch_bounds__(kWg) void k_gi_setup(GiArgs a, Wave w) {
-    uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    if (i >= wave_intervals(a, w) * w.R) return;
-    uint4 pr = w.prim[i];
+    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
+    const uint32_t gw = blockIdx.x * (kWg / 64u) + (threadIdx.x >> 6);
+    const bool live = i < wave_intervals(a, w) * w.R;
+    uint4 pr = live ? w.prim[i] : make_uint4(0u, lux::kEmpty, lux::kEmpty, lux::kErrUnsupported);
     float4 sh = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t running = 0;  // rays of this wave so far (the same in every lane of a ballot)
     if ((pr.w & lux::kErrUnsupported) == 0u && pr.y != lux::kEmpty) {
         float3 o, d;
         float lim;
@@ -282,18 +288,18 @@ __global__ __launch_bounds__(kWg) void k_gi_setup(GiArgs a, Wave w) {
                 w.slots[slot] = make_float4(t3.x, t3.y, t3.z, q ? 1.0f : 0.0f);
                 uint64_t mask = __ballot(q);
                 if (q) {
-                    uint32_t leader = (uint32_t)(__ffsll((unsigned long long)mask) - 1);
-                    uint32_t base = 0;
-                    if (lane == leader) base = atomicAdd(&w.counters[1], (uint32_t)__popcll(mask));
-                    base = __shfl(base, (int)leader, 64);
-                    uint32_t at = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-                    w.shadow[2ull * at] = make_float4(origin.x, origin.y, origin.z, sl);
-                    w.shadow[2ull * at + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(slot));
+                    const uint64_t at = (uint64_t)gw * 64u * w.nee + running +
+                                        (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+                    w.slab[2ull * at] = make_float4(origin.x, origin.y, origin.z, sl);
+                    w.slab[2ull * at + 1] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(slot));
                 }
+                running += (uint32_t)__popcll(mask);
             }
         }
     }
-    w.shade[i] = sh;
+    if (live) w.shade[i] = sh;
+    const uint32_t total = wave_max_u32(running);
+    if (__lane_id() == 0u) w.wave_cnt[gw] = total;
 }
 
 struct ShadowSource {
@@ -917,7 +923,7 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
     const bool rx = proto.kind == kWaveReceiver;
     const uint32_t R = rx ? 1u : (p.rays_per_interval ? p.rays_per_interval : 1u);
     const uint32_t nee = p.emissive_samples / 2u > 1u ? p.emissive_samples / 2u : 1u;
-    const uint64_t per_item = 32ull + (uint64_t)nee * (16ull + 4ull + 32ull);
+    const uint64_t per_item = 32ull + (uint64_t)nee * (16ull + 4ull + 64ull);
     uint64_t chunk_items = (2ull << 30) / per_item;
     if (chunk_items < 4096) chunk_items = 4096;
     uint64_t chunk_units = chunk_items / (R * per_unit);
@@ -929,8 +935,11 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
     if (int rc = lux_ensure(ctx, ctx->gi_w_shade, 16ull * items)) return rc;
     if (int rc = lux_ensure(ctx, ctx->gi_w_slots, 16ull * items * nee)) return rc;
     if (int rc = lux_ensure(ctx, ctx->gi_w_vis, 4ull * items * nee)) return rc;
-    if (int rc = lux_ensure(ctx, ctx->gi_w_shadow, 32ull * items * nee)) return rc;
-    if (int rc = lux_ensure(ctx, ctx->gi_w_counters, 16)) return rc;
+    // occlusion rays: per setup wave a slab of 64 * nee, then the packed list
+    const uint64_t setup_waves = ((items + kWg - 1) / kWg) * (kWg / 64u);
+    if (int rc = lux_ensure(ctx, ctx->gi_w_shadow, 32ull * setup_waves * 64u * nee * 2u)) return rc;
+    const size_t counter_words = 4u + (setup_waves + 1u) + lux_scan_sums_words((uint32_t)setup_waves + 1u);
+    if (int rc = lux_ensure(ctx, ctx->gi_w_counters, 4ull * counter_words)) return rc;
     GiArgs a = gi_args(ctx);
     if (rx) a.admit_limit = 0;  // a receiver update is bounded by its fair window
     Wave w = proto;
@@ -942,6 +951,9 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
     w.vis = static_cast<uint32_t*>(ctx->gi_w_vis.ptr);
     w.shadow = static_cast<float4*>(ctx->gi_w_shadow.ptr);
     w.counters = static_cast<uint32_t*>(ctx->gi_w_counters.ptr);
+    w.wave_cnt = w.counters + 4;
+    uint32_t* scan_sums = w.wave_cnt + setup_waves + 1u;
+    w.slab = w.shadow + 2ull * setup_waves * 64u * nee;
     w.regroup = ctx->gi_regroup ? ctx->gi_regroup : 8u;
     w.batch = ctx->gi_batch ? ctx->gi_batch : 8u;
     const uint32_t wpc = ctx->gi_wpc ? ctx->gi_wpc : 20u;
@@ -967,6 +979,11 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
         default: hipLaunchKernelGGL(k_gi_primary<8>, dim3(pb), dim3(kWg), 0, st, a, w); break;
         }
         hipLaunchKernelGGL(k_gi_setup, dim3(need), dim3(kWg), 0, st, a, w);
+        const uint32_t nw = need * (kWg / 64u);
+        hipLaunchKernelGGL(k_fill_u32_one, dim3(1), dim3(1), 0, st, w.wave_cnt + nw, 0u);
+        if (int rc = lux_scan_exclusive_on(ctx, st, w.wave_cnt, nw + 1u, scan_sums, &w.counters[1])) return rc;
+        hipLaunchKernelGGL(k_shadow_compact, dim3((nw + 3u) / 4u), dim3(256), 0, st, w.slab, w.wave_cnt, nw,
+                           64u * w.nee, w.shadow);
         switch (ctx->gi_depth) {
         case 4: hipLaunchKernelGGL(k_gi_shadow<4>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
         case 16: hipLaunchKernelGGL(k_gi_shadow<16>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
