@@ -47,6 +47,29 @@ struct GiArgs {
 
 constexpr uint32_t kMaxDirtyBoxes = 16;
 
+__device__ bool admit_query(const GiArgs& a);
+
+// With an unbounded budget every query is admitted: count it in a per-lane
+// tally instead of the shared counters, and add each wave's tally once
+// (query_tally_flush, called by the whole wave) -- the same totals.
+__device__ __forceinline__ bool admit_query_tallied(const GiArgs& a, uint32_t& tally) {
+    if (a.admit_limit == 0u) {
+        ++tally;
+        return true;
+    }
+    return admit_query(a);
+}
+
+__device__ __forceinline__ void query_tally_flush(const GiArgs& a, uint32_t tally) {
+    uint32_t s = tally;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (__lane_id() == 0u && s) {
+        atomicAdd(&a.counters[0], s);
+        atomicAdd(&a.counters[1], s);
+    }
+}
+
 // Ticketed per-update traversal budget (admitTransportQuery,
 // shaders/traversal.slang:20-38), aggregated per wave: the active lanes take
 // consecutive tickets with one atomic, lanes past the limit are denied and

@@ -225,6 +225,7 @@ struct PrimarySource {
     const GiArgs* a;
     const Wave* w;
     uint32_t n;
+    uint32_t tally;  // queries admitted under an unbounded budget
     __device__ uint32_t count() const { return n; }
     template <class St>
     __device__ bool begin(uint32_t i, lux::Walk& wk, St& st) {
@@ -232,7 +233,7 @@ struct PrimarySource {
         float lim;
         bool term;
         wave_item_ray(*a, *w, i, o, d, lim, term);
-        if (!admit_query(*a)) {  // denied: no sample, as an unsupported traversal
+        if (!admit_query_tallied(*a, tally)) {  // denied: no sample, as an unsupported traversal
             w->prim[i] = make_uint4(0u, lux::kEmpty, lux::kEmpty, lux::kErrUnsupported);
             return false;
         }
@@ -252,8 +253,9 @@ __global__ __launch_bounds__(kWg) void k_gi_primary(GiArgs a, Wave w) {
     lux::LaneStack<Depth, kWg> stk;
     stk.col = &lds[0][threadIdx.x];
     lux::NoStats st;
-    PrimarySource src{&a, &w, wave_intervals(a, w) * w.R};
+    PrimarySource src{&a, &w, wave_intervals(a, w) * w.R, 0u};
     lux::regroup_walks(a.scene, src, &w.counters[0], w.regroup, w.batch, stk, st);
+    query_tally_flush(a, src.tally);
 }
 
 __global__ __launch_bounds__(kWg) void k_gi_setup(GiArgs a, Wave w) {
@@ -306,11 +308,12 @@ struct ShadowSource {
     const GiArgs* a;
     const Wave* w;
     uint32_t n;
+    uint32_t tally;  // queries admitted under an unbounded budget
     __device__ uint32_t count() const { return n; }
     template <class St>
     __device__ bool begin(uint32_t i, lux::Walk& wk, St& st) {
         float4 r0 = w->shadow[2ull * i], r1 = w->shadow[2ull * i + 1];
-        if (!admit_query(*a)) {
+        if (!admit_query_tallied(*a, tally)) {
             w->vis[__float_as_uint(r1.w)] = 2u;
             return false;
         }
@@ -332,8 +335,9 @@ __global__ __launch_bounds__(kWg) void k_gi_shadow(GiArgs a, Wave w) {
     lux::LaneStack<Depth, kWg> stk;
     stk.col = &lds[0][threadIdx.x];
     lux::NoStats st;
-    ShadowSource src{&a, &w, w.counters[1]};
+    ShadowSource src{&a, &w, w.counters[1], 0u};
     lux::regroup_walks(a.scene, src, &w.counters[2], w.regroup, w.batch, stk, st);
+    query_tally_flush(a, src.tally);
 }
 
 __global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
