@@ -917,7 +917,7 @@ extern "C" int lux_gi_set_frame(lux_ctx* ctx, uint32_t frame, uint32_t seed) {
 
 namespace {
 // Runs the four wavefront stages over a set of intervals in chunks whose
-// scratch stays under about 2 GiB.  `proto` carries the kind and its inputs;
+// scratch stays under about 6 GiB.  `proto` carries the kind and its inputs;
 // `total` is the number of intervals (kWaveRange), listed probes
 // (kWaveListed, an upper bound: the device-side list length decides) or
 // samples (kWaveSampled); `per_unit` intervals per unit (D for listed
@@ -928,7 +928,7 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
     const uint32_t R = rx ? 1u : (p.rays_per_interval ? p.rays_per_interval : 1u);
     const uint32_t nee = p.emissive_samples / 2u > 1u ? p.emissive_samples / 2u : 1u;
     const uint64_t per_item = 32ull + (uint64_t)nee * (16ull + 4ull + 64ull);
-    uint64_t chunk_items = (2ull << 30) / per_item;
+    uint64_t chunk_items = (6ull << 30) / per_item;
     if (chunk_items < 4096) chunk_items = 4096;
     uint64_t chunk_units = chunk_items / (R * per_unit);
     if (chunk_units < 1) chunk_units = 1;
