@@ -542,6 +542,8 @@ int enqueue(lux_batch* b) {
             hipLaunchKernelGGL(k_b_history, dim3((uint32_t)((intervals + kWg - 1) / kWg)), dim3(kWg), 0, st, w);
             hipLaunchKernelGGL(k_b_history_done, dim3((n + 255) / 256), dim3(256), 0, st,
                                static_cast<const BatchScene*>(b->table.ptr), n);
+            for (uint32_t s = 0; s < n; ++s)
+                if (b->scenes[s]->gi_hist_on) b->scenes[s]->gi_hist_valid = true;
         }
     }
     for (uint32_t s = 0; s < n; ++s) {

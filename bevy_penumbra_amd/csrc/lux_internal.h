@@ -77,6 +77,7 @@ struct lux_ctx {
     // temporal history of raw intervals: on when a blend is positive; sample
     // counts, two float4 per interval, state words (valid, dirty boxes)
     bool gi_hist_on = false;
+    bool gi_hist_valid = false;  // a history pass ran since the last reset
     lux_gi_history_params gi_hist{};
     lux_buffer gi_samples, gi_history, gi_hist_state;
     uint32_t gi_dirty_n = 0;                 // dirty boxes pending on the device
@@ -88,7 +89,7 @@ struct lux_ctx {
     // list, per-ray deposit slots, compacted deposit values and long-run list,
     // sort and scan scratch, counters
     lux_buffer rx_keys, rx_flags, rx_slot, rx_lod, rx_ord, rx_seq, rx_dir, rx_tickets;
-    lux_buffer rx_dep_val, rx_pack, rx_traced, rx_sort, rx_scan, rx_counters;
+    lux_buffer rx_dep_val, rx_pack, rx_traced, rx_sort, rx_scan, rx_counters, rx_cold;
     uint32_t rx_count = 0, rx_levels = 0, rx_cap = 0, rx_tickets_n = 0;
 
     // trace tuning

@@ -11,9 +11,14 @@
 //   k_rx_insert    one thread per descriptor: open-addressing probe directory
 //                  per level (CAS on the owner word); canonical = the lowest
 //                  descriptor index of the key (atomicMin)
+//   cold priority  (first update of an invalid persistent history) every
+//                  receiver is eligible; per probe a nominee (lowest
+//                  frame-seeded rank), populations binned by log2, nominees
+//                  admitted by bin within the budget (scan for the boundary
+//                  bin), fair-window receivers fill the rest (scan)
 //   ordinals       stable radix sort of the (variant, receiver) ray markers by
-//                  level-0 directory slot: ordinal = rank in the run, level-0
-//                  ray count = run length
+//                  level-0 directory slot (a cold nominee first): ordinal =
+//                  rank in the run, level-0 ray count = run length
 //   k_rx_parents   levels 1.. in order: ray count from the eight children
 //   roots          root counts scattered by canonical index, exclusive scan
 //   k_rx_prefix    levels top-1 .. 0: parent offset + earlier siblings' counts
@@ -29,6 +34,8 @@
 // ordinals, root ranges and tickets); here each is the order of a
 // sequential pass over ascending descriptor / receiver index, so the
 // results are a deterministic function of the input, equal to the oracle's.
+// Cold admission and fill tickets go in ascending fair rank (variant 0
+// first), the order of the reference's rank-mapped dispatch.
 #include <algorithm>
 #include <cstring>
 #include "lux_gi_device.h"
@@ -36,7 +43,7 @@
 namespace {
 
 constexpr uint32_t kRxEmpty = 0xffffffffu;
-constexpr uint32_t kRxValid = 1u, kRxRay = 2u;
+constexpr uint32_t kRxValid = 1u, kRxRay = 2u, kRxNominee = 4u;
 constexpr uint32_t kRxLodMask = 31u << 18;
 constexpr uint32_t kRxMaxLods = 8;
 constexpr uint32_t kScanBlock = 1024;
@@ -59,30 +66,70 @@ __device__ __forceinline__ uint32_t perm(uint32_t v, uint32_t mask, uint32_t see
     return v;
 }
 
-__device__ uint32_t candidate_rank(uint32_t sample, uint32_t w, uint32_t h, uint32_t count) {
-    if (count <= 1u) return 0u;
+__device__ __forceinline__ uint32_t inv_xorshift(uint32_t v, uint32_t shift) {
+    uint32_t r = v;
+    for (uint32_t off = shift; off < 32u; off += shift) r ^= v >> off;
+    return r;
+}
+
+__device__ __forceinline__ uint32_t inv_perm(uint32_t v, uint32_t mask, uint32_t seed) {
+    v = inv_xorshift(v, 16u) & mask;
+    v = (v * 0x43021123u) & mask;
+    v = inv_xorshift(v, 15u) & mask;
+    v = (v * 0x1d69e2a5u) & mask;
+    v = inv_xorshift(v, 16u) & mask;
+    return (v ^ seed) & mask;
+}
+
+__device__ __forceinline__ uint32_t perm_mask(uint32_t count) {
     uint32_t mask = 0;
     while (mask < count - 1u) mask = (mask << 1) | 1u;
-    uint32_t seed = ((w * 0x9e3779b9u) ^ (h * 0x85ebca6bu) ^ count) & mask;
+    return mask;
+}
+
+// Rank of a receiver in the cycle-walked permutation whose seed is the grid
+// term xor `extra` (0: transportCandidateRank; frame * phi:
+// coldCandidateRank, split.slang:607-645).
+__device__ uint32_t rank_of(uint32_t sample, uint32_t w, uint32_t h, uint32_t count, uint32_t extra) {
+    if (count <= 1u) return 0u;
+    const uint32_t mask = perm_mask(count);
+    const uint32_t seed = ((w * 0x9e3779b9u) ^ (h * 0x85ebca6bu) ^ count ^ extra) & mask;
     uint32_t r = perm(sample, mask, seed);
     while (r >= count) r = perm(r, mask, seed);
     return r;
 }
 
+// Its inverse (transportCandidateFromRank / coldCandidateFromRank, :622-657).
+__device__ uint32_t sample_of(uint32_t rank, uint32_t w, uint32_t h, uint32_t count, uint32_t extra) {
+    if (count <= 1u) return 0u;
+    const uint32_t mask = perm_mask(count);
+    const uint32_t seed = ((w * 0x9e3779b9u) ^ (h * 0x85ebca6bu) ^ count ^ extra) & mask;
+    uint32_t s = inv_perm(rank, mask, seed);
+    while (s >= count) s = inv_perm(s, mask, seed);
+    return s;
+}
+
 struct RxSeed {
     uint32_t count, width, height, budget, cursor, quota, frame, flags;
-    uint32_t levels, has_view;
+    uint32_t levels, has_view, cold;
     float spacing;
     lux_lod_view view;
 };
 
-__device__ bool candidate_selected(const RxSeed& s, uint32_t sample) {
+// transportCandidateSelected (split.slang:659-683) of the receiver with a
+// given fair rank.
+__device__ bool rank_selected(const RxSeed& s, uint32_t rank) {
     if (s.flags & 1u) return false;
     if (s.count == 0u || s.budget == 0u || s.budget >= s.count) return true;
     uint32_t start = (s.cursor & 0x03ffffffu) % s.count;
-    uint32_t rank = candidate_rank(sample, s.width, s.height, s.count);
     if (s.quota == 0u) return false;
     return rank >= start ? rank - start < s.quota : s.count - start + rank < s.quota;
+}
+
+__device__ bool candidate_selected(const RxSeed& s, uint32_t sample) {
+    if (s.flags & 1u) return false;
+    if (s.count == 0u || s.budget == 0u || s.budget >= s.count) return true;
+    return rank_selected(s, rank_of(sample, s.width, s.height, s.count, 0u));
 }
 
 // cacheKey (common.slang:418-423), direction 0.
@@ -110,7 +157,8 @@ __global__ __launch_bounds__(kWg) void k_rx_seed(RxSeed s, const float* recv, in
             float phase = (float)(bits >> 8) * (1.0f / 16777216.0f);
             sel_lod = phase < blend ? 1u : 0u;
         }
-        const bool ray = ok && candidate_selected(s, i);
+        // cold priority: every receiver's selected LOD is eligible (eligibleRay)
+        const bool ray = ok && (s.cold || candidate_selected(s, i));
         for (uint32_t v = 0; v < 2; ++v) {
             lods[(uint64_t)v * s.count + i] = v ? high : low;
             for (uint32_t L = 0; L < s.levels; ++L) {
@@ -235,33 +283,189 @@ __global__ __launch_bounds__(kWg) void k_rx_insert(RxDir D, const int4* keys, co
 }
 
 // Sort keys of the (variant, receiver) ray markers: level-0 slot, or `none`
-// (the directory capacity, above every slot).
+// (above every key).  With `cold` the key is slot * 2 + (not the probe's
+// nominee), so a probe's nominee sorts first.
 __global__ __launch_bounds__(kWg) void k_rx_ray_keys(const uint32_t* flags, const uint32_t* slot, uint32_t m,
-                                                     uint32_t levels, uint32_t none, uint32_t* out) {
+                                                     uint32_t levels, uint32_t none, uint32_t cold, uint32_t* out) {
     const uint32_t j = blockIdx.x * kWg + threadIdx.x;
     if (j >= m) return;
     const uint64_t d0 = (uint64_t)j * levels;
-    out[j] = (flags[d0] & kRxRay) && slot[d0] != kRxEmpty ? slot[d0] : none;
+    const uint32_t f = flags[d0], b = slot[d0];
+    uint32_t k = none;
+    if ((f & kRxRay) && b != kRxEmpty) k = cold ? (b << 1) | ((f & kRxNominee) ? 0u : 1u) : b;
+    out[j] = k;
 }
 
-__global__ __launch_bounds__(kWg) void k_rx_run_start(const uint32_t* sk, uint32_t m, uint32_t none,
+// Runs of equal slots (key >> shift) in the sorted keys.
+__global__ __launch_bounds__(kWg) void k_rx_run_start(const uint32_t* sk, uint32_t m, uint32_t none, uint32_t shift,
                                                       uint32_t* start) {
     const uint32_t i = blockIdx.x * kWg + threadIdx.x;
     if (i >= m) return;
     const uint32_t k = sk[i];
-    if (k != none && (i == 0 || sk[i - 1] != k)) start[k] = i;
+    if (k != none && (i == 0 || (sk[i - 1] >> shift) != (k >> shift))) start[k >> shift] = i;
 }
 
 __global__ __launch_bounds__(kWg) void k_rx_ordinals(const uint32_t* sk, const uint32_t* sv, uint32_t m,
-                                                     uint32_t none, const uint32_t* start, uint32_t* count0,
-                                                     uint32_t* ord) {
+                                                     uint32_t none, uint32_t shift, const uint32_t* start,
+                                                     uint32_t* count0, uint32_t* ord) {
     const uint32_t i = blockIdx.x * kWg + threadIdx.x;
     if (i >= m) return;
     const uint32_t k = sk[i];
     if (k == none) return;
-    const uint32_t s = start[k];
+    const uint32_t b = k >> shift;
+    const uint32_t s = start[b];
     ord[sv[i]] = i - s;
-    if (i + 1 == m || sk[i + 1] != k) count0[k] = i - s + 1u;
+    if (i + 1 == m || (sk[i + 1] >> shift) != b) count0[b] = i - s + 1u;
+}
+
+// ---------------- cold priority (split.slang:685-705, rank :836-888) ----------------
+// Words: per level-0 slot the nominee priority (0xffffffff - lowest cold
+// rank) and the population (eligible receivers); a table of 32 population
+// bins, 32 admission quotas, the boundary bin, admitted nominees and the fill
+// quota; m words of flags scanned into positions.
+struct RxCold {
+    uint32_t* prio;
+    uint32_t* pop;
+    uint32_t* table;
+    uint32_t* pos;
+};
+constexpr uint32_t kColdTable = 80;
+
+__device__ __forceinline__ uint32_t cold_bin(uint32_t population) {
+    return population > 1u ? 31u - (uint32_t)__clz(population) : 0u;
+}
+
+// Phase 0: per eligible (variant, receiver) its probe's nominee priority and
+// population; lanes of one probe reduce to one atomic pair.
+__global__ __launch_bounds__(kWg) void k_rx_cold_nominate(RxSeed sd, const uint32_t* flags, const uint32_t* slot,
+                                                          uint32_t m, RxCold cw) {
+    const uint32_t j = blockIdx.x * kWg + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    bool e = false;
+    uint32_t b = kRxEmpty, pr = 0u;
+    if (j < m) {
+        const uint64_t d0 = (uint64_t)j * sd.levels;
+        b = slot[d0];
+        e = (flags[d0] & kRxRay) && b != kRxEmpty;
+        if (e) pr = 0xffffffffu - rank_of(j % sd.count, sd.width, sd.height, sd.count, sd.frame * 0x9e3779b9u);
+    }
+    uint64_t pending = __ballot(e);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lb = __shfl(b, leader, 64);
+        const bool peer = e && b == lb;
+        const uint64_t peers = __ballot(peer);
+        const uint32_t mx = wave_max_u32(peer ? pr : 0u);
+        if ((int)lane == leader) {
+            atomicMax(&cw.prio[lb], mx);
+            atomicAdd(&cw.pop[lb], (uint32_t)__popcll(peers));
+        }
+        pending &= ~peers;
+    }
+}
+
+// Phase 1: probes per population bin.
+__global__ __launch_bounds__(kWg) void k_rx_cold_hist(uint32_t cap, RxCold cw) {
+    __shared__ uint32_t h[32];
+    if (threadIdx.x < 32) h[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t b = blockIdx.x * kWg + threadIdx.x;
+    if (b < cap && cw.pop[b] != 0u) atomicAdd(&h[cold_bin(cw.pop[b])], 1u);
+    __syncthreads();
+    if (threadIdx.x < 32 && h[threadIdx.x]) atomicAdd(&cw.table[threadIdx.x], h[threadIdx.x]);
+}
+
+// Admission per bin: a probe of bin b is admitted while fewer than
+// budget - (probes in higher bins) of its bin came before it; at most one bin
+// is admitted in part.
+__global__ void k_rx_cold_bounds(RxCold cw, uint32_t budget, uint32_t* counters) {
+    if (threadIdx.x != 0) return;
+    uint32_t higher = 0, admitted = 0, boundary = 32u;
+    for (int b = 31; b >= 0; --b) {
+        const uint32_t hb = cw.table[b];
+        const uint32_t q = higher < budget ? budget - higher : 0u;
+        cw.table[32 + b] = q;
+        if (hb > q && q > 0u) boundary = (uint32_t)b;
+        admitted += hb < q ? hb : q;
+        higher += hb;
+    }
+    cw.table[64] = boundary;
+    cw.table[65] = admitted;
+    cw.table[66] = budget - admitted;
+    counters[10] = admitted;
+}
+
+// Position p = fair rank * 2 + variant (the rank-mapped dispatch order) to
+// its (variant, receiver) index.
+__device__ __forceinline__ uint32_t rank_mapped(const RxSeed& sd, uint32_t p) {
+    return (p & 1u) * sd.count + sample_of(p >> 1, sd.width, sd.height, sd.count, 0u);
+}
+
+// Canonical level-0 descriptors of the boundary bin, flagged in rank-mapped
+// order (scanned: their admission tickets).
+__global__ __launch_bounds__(kWg) void k_rx_cold_order(RxSeed sd, RxDir D, const uint32_t* slot, uint32_t m,
+                                                       RxCold cw) {
+    const uint32_t p = blockIdx.x * kWg + threadIdx.x;
+    if (p >= m) return;
+    const uint32_t boundary = cw.table[64];
+    uint32_t f = 0u;
+    if (boundary < 32u) {
+        const uint64_t d0 = (uint64_t)rank_mapped(sd, p) * sd.levels;
+        const uint32_t b = slot[d0];
+        if (b != kRxEmpty && D.canon[b] == (uint32_t)d0 && cw.pop[b] != 0u && cold_bin(cw.pop[b]) == boundary) f = 1u;
+    }
+    cw.pos[p] = f;
+}
+
+// Phase 2: each probe's nominee (the receiver of its lowest cold rank, on
+// the variant that holds the probe) is admitted or loses its ray.
+__global__ __launch_bounds__(kWg) void k_rx_cold_admit(RxSeed sd, RxDir D, const uint32_t* slot, uint32_t* flags,
+                                                       RxCold cw) {
+    const uint32_t b = blockIdx.x * kWg + threadIdx.x;
+    if (b >= D.cap || cw.prio[b] == 0u) return;
+    const uint32_t ns = sample_of(0xffffffffu - cw.prio[b], sd.width, sd.height, sd.count, sd.frame * 0x9e3779b9u);
+    uint64_t nom = (uint64_t)ns * sd.levels;
+    if (slot[nom] != b) nom = ((uint64_t)sd.count + ns) * sd.levels;
+    const uint32_t bin = cold_bin(cw.pop[b]);
+    const uint32_t q = cw.table[32 + bin];
+    bool admitted = q > 0u;
+    if (bin == cw.table[64]) {
+        const uint32_t jc = D.canon[b] / sd.levels;
+        const uint32_t sc = jc % sd.count, vc = jc / sd.count;
+        admitted = cw.pos[rank_of(sc, sd.width, sd.height, sd.count, 0u) * 2u + vc] < q;
+    }
+    const uint32_t f = flags[nom];
+    flags[nom] = admitted ? (f | kRxNominee) : (f & ~kRxRay);
+}
+
+// Phase 3: eligible receivers other than nominees, fair-selected, flagged in
+// rank-mapped order (scanned: their fill tickets).
+__device__ __forceinline__ bool cold_fill_candidate(const RxSeed& sd, const uint32_t* flags, const uint32_t* slot,
+                                                    uint32_t p, uint64_t* d0_out) {
+    const uint64_t d0 = (uint64_t)rank_mapped(sd, p) * sd.levels;
+    *d0_out = d0;
+    const uint32_t f = flags[d0];
+    return (f & kRxRay) && !(f & kRxNominee) && slot[d0] != kRxEmpty && rank_selected(sd, p >> 1);
+}
+
+__global__ __launch_bounds__(kWg) void k_rx_cold_fill_flags(RxSeed sd, const uint32_t* flags, const uint32_t* slot,
+                                                            uint32_t m, RxCold cw) {
+    const uint32_t p = blockIdx.x * kWg + threadIdx.x;
+    if (p >= m) return;
+    uint64_t d0;
+    cw.pos[p] = cold_fill_candidate(sd, flags, slot, p, &d0) ? 1u : 0u;
+}
+
+// The first fill-quota candidates keep their rays; every other non-nominee
+// loses its.
+__global__ __launch_bounds__(kWg) void k_rx_cold_fill(RxSeed sd, uint32_t* flags, const uint32_t* slot, uint32_t m,
+                                                      RxCold cw) {
+    const uint32_t p = blockIdx.x * kWg + threadIdx.x;
+    if (p >= m) return;
+    uint64_t d0;
+    const bool cand = cold_fill_candidate(sd, flags, slot, p, &d0);
+    const uint32_t f = flags[d0];
+    if ((f & kRxRay) && !(f & kRxNominee) && !(cand && cw.pos[p] < cw.table[66])) flags[d0] = f & ~kRxRay;
 }
 
 // parents (split.slang:898-922): level L's ray count from its eight children.
@@ -642,6 +846,16 @@ RxDir dir_of(lux_ctx* c) {
     return D;
 }
 
+RxCold cold_of(lux_ctx* c, uint32_t cap) {
+    uint32_t* w = static_cast<uint32_t*>(c->rx_cold.ptr);
+    RxCold cw;
+    cw.prio = w;
+    cw.pop = w + cap;
+    cw.table = w + 2ull * cap;
+    cw.pos = cw.table + kColdTable;
+    return cw;
+}
+
 // Exclusive scan of n words in place; the total is written to *total (device).
 int scan_exclusive(lux_ctx* c, uint32_t* data, uint32_t n, uint32_t* total) {
     if (int rc = lux_ensure(c, c->rx_scan, 4ull * lux_scan_sums_words(n))) return rc;
@@ -739,6 +953,10 @@ extern "C" int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count,
     if (int rc = lux_ensure(c, c->rx_counters, 64)) return rc;
     for (uint32_t k = 0; k < lod_count; ++k)
         if (int rc = lux_ensure(lods[k], lods[k]->gi_samples, 4ull * lods[k]->gi_total)) return rc;
+    // cold priority (coldPriorityEnabled, split.slang:685-689; src/render.rs:
+    // 128-131): the first update of a persistent history that is not valid
+    const bool cold = !(rp->flags & 1u) && rp->budget > 0u && c->gi_hist_on && c->gi_hist.history_frames > 0u &&
+                      c->gi_hist.persistent != 0u && !c->gi_hist_valid;
     c->rx_count = count;
     c->rx_levels = levels;
     c->rx_cap = cap;
@@ -772,6 +990,7 @@ extern "C" int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count,
     sd.flags = rp->flags;
     sd.levels = levels;
     sd.has_view = view != nullptr;
+    sd.cold = cold ? 1u : 0u;
     sd.spacing = c->gi.spacing;
     if (view) sd.view = *view;
     hipLaunchKernelGGL(k_rx_seed, dim3(blocks_for(count)), dim3(kWg), 0, st, sd, receivers, keys, flags, lodv,
@@ -781,12 +1000,31 @@ extern "C" int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count,
     const uint32_t clear_blocks = std::min<uint32_t>(blocks_for((uint64_t)levels * cap), 4096u);
     hipLaunchKernelGGL(k_rx_dir_clear, dim3(clear_blocks), dim3(kWg), 0, st, D);
     hipLaunchKernelGGL(k_rx_insert, dim3(blocks_for(nd)), dim3(kWg), 0, st, D, keys, flags, m, slot, counters);
+    if (cold) {
+        // nominate, histogram the populations, admit nominees by bin, fill
+        // from the fair window up to the budget
+        if (int rc = lux_ensure(c, c->rx_cold, 4ull * (2ull * cap + kColdTable + m))) return rc;
+        const RxCold cw = cold_of(c, cap);
+        if (int rc = lux_device_zero(c, cw.prio, 4ull * (2ull * cap + kColdTable))) return rc;
+        hipLaunchKernelGGL(k_rx_cold_nominate, dim3(blocks_for(m)), dim3(kWg), 0, st, sd, flags, slot, m, cw);
+        hipLaunchKernelGGL(k_rx_cold_hist, dim3(blocks_for(cap)), dim3(kWg), 0, st, cap, cw);
+        hipLaunchKernelGGL(k_rx_cold_bounds, dim3(1), dim3(64), 0, st, cw, rp->budget, counters);
+        hipLaunchKernelGGL(k_rx_cold_order, dim3(blocks_for(m)), dim3(kWg), 0, st, sd, D, slot, m, cw);
+        if (int rc = scan_exclusive(c, cw.pos, m, counters + 11)) return rc;
+        hipLaunchKernelGGL(k_rx_cold_admit, dim3(blocks_for(cap)), dim3(kWg), 0, st, sd, D, slot, flags, cw);
+        hipLaunchKernelGGL(k_rx_cold_fill_flags, dim3(blocks_for(m)), dim3(kWg), 0, st, sd, flags, slot, m, cw);
+        if (int rc = scan_exclusive(c, cw.pos, m, counters + 12)) return rc;
+        hipLaunchKernelGGL(k_rx_cold_fill, dim3(blocks_for(m)), dim3(kWg), 0, st, sd, flags, slot, m, cw);
+    }
+    const uint32_t none = cold ? cap << 1 : cap, shift = cold ? 1u : 0u;
     uint32_t *sk_in, *sk, *sv;
     if (int rc = sort_keys(c, std::max<uint32_t>(m, 1u), &sk_in, &sk, &sv)) return rc;
-    hipLaunchKernelGGL(k_rx_ray_keys, dim3(blocks_for(m)), dim3(kWg), 0, st, flags, slot, m, levels, cap, sk_in);
-    if (int rc = run_sort(c, m, bits_for(cap))) return rc;
-    hipLaunchKernelGGL(k_rx_run_start, dim3(blocks_for(m)), dim3(kWg), 0, st, sk, m, cap, D.offset);
-    hipLaunchKernelGGL(k_rx_ordinals, dim3(blocks_for(m)), dim3(kWg), 0, st, sk, sv, m, cap, D.offset, D.count, ord);
+    hipLaunchKernelGGL(k_rx_ray_keys, dim3(blocks_for(m)), dim3(kWg), 0, st, flags, slot, m, levels, none,
+                       cold ? 1u : 0u, sk_in);
+    if (int rc = run_sort(c, m, bits_for(none))) return rc;
+    hipLaunchKernelGGL(k_rx_run_start, dim3(blocks_for(m)), dim3(kWg), 0, st, sk, m, none, shift, D.offset);
+    hipLaunchKernelGGL(k_rx_ordinals, dim3(blocks_for(m)), dim3(kWg), 0, st, sk, sv, m, none, shift, D.offset,
+                       D.count, ord);
     // parents, roots, prefix
     for (uint32_t L = 1; L < levels; ++L)
         hipLaunchKernelGGL(k_rx_parents, dim3(blocks_for(cap)), dim3(kWg), 0, st, D, keys, L);
@@ -888,8 +1126,8 @@ extern "C" int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count,
         if (int rc = lux_gi_finish_internal(lods[k], st)) return rc;
     LUX_HIP(c, hipEventRecord(c->ev1, st));
     if (stats) {
-        uint32_t cv[8];
-        LUX_HIP(c, hipMemcpyAsync(cv, counters, 32, hipMemcpyDeviceToHost, st));
+        uint32_t cv[11];
+        LUX_HIP(c, hipMemcpyAsync(cv, counters, 44, hipMemcpyDeviceToHost, st));
         LUX_HIP(c, hipStreamSynchronize(st));
         std::memset(stats, 0, sizeof *stats);
         stats->receivers = cv[0];
@@ -900,6 +1138,8 @@ extern "C" int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count,
         stats->outside = cv[5];
         stats->root_rays = cv[6];
         stats->overflow = cv[7];
+        stats->cold = cold ? 1u : 0u;
+        stats->nominees = cold ? cv[10] : 0u;
         LUX_HIP(c, hipEventElapsedTime(&stats->ms, c->ev0, c->ev1));
     }
     return LUX_OK;

@@ -1223,10 +1223,12 @@ int history_pass(lux_ctx* ctx, uint64_t count, hipStream_t st = nullptr) {
     hipLaunchKernelGGL(k_gi_history_done, dim3(1), dim3(64), 0, st, a.hist_state);
     LUX_HIP(ctx, hipGetLastError());
     ctx->gi_dirty_n = 0;
+    ctx->gi_hist_valid = true;
     return LUX_OK;
 }
 
 int history_alloc(lux_ctx* ctx) {
+    ctx->gi_hist_valid = false;
     if (!ctx->gi_hist_on || !ctx->gi_ready) return LUX_OK;
     if (int rc = lux_ensure(ctx, ctx->gi_samples, 4ull * ctx->gi_total)) return rc;
     if (int rc = lux_ensure(ctx, ctx->gi_history, 32ull * ctx->gi_total)) return rc;
@@ -1253,6 +1255,7 @@ extern "C" int lux_gi_set_history(lux_ctx* ctx, const lux_gi_history_params* hp)
 
 extern "C" int lux_gi_reset_history(lux_ctx* ctx) {
     if (!ctx) return LUX_ERR_INVALID;
+    ctx->gi_hist_valid = false;
     if (!ctx->gi_hist_on || !ctx->gi_hist_state.ptr) return LUX_OK;
     ctx->gi_dirty_n = 0;
     return lux_device_zero(ctx, ctx->gi_hist_state.ptr, 8);

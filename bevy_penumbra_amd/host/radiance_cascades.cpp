@@ -353,4 +353,8 @@ int lux_candidate_schedule_advance(lux_candidate_schedule* s, uint32_t consume_f
     return LUX_OK;
 }
 
+int lux_cold_priority_enabled(uint32_t history_frames, uint32_t history_was_valid) {
+    return history_frames > 0 && !history_was_valid ? 1 : 0;
+}
+
 }  // extern "C"

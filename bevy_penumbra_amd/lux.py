@@ -133,6 +133,11 @@ class CandidateSchedule(C.Structure):
         load_library().lux_candidate_schedule_advance(C.byref(self), int(consume_fair_cursor))
 
 
+def cold_priority_enabled(history_frames: int, history_was_valid: bool) -> bool:
+    """cold_priority_enabled (src/render.rs:128-131)."""
+    return bool(load_library().lux_cold_priority_enabled(history_frames, int(history_was_valid)))
+
+
 class SurrogateParams(C.Structure):
     """lux_surrogate_params (include/lux.h)."""
     _fields_ = [
@@ -240,7 +245,7 @@ class ReceiverStats(C.Structure):
     _fields_ = [("receivers", C.c_uint32), ("tickets", C.c_uint32), ("rays", C.c_uint32),
                 ("unsupported", C.c_uint32), ("deposits", C.c_uint32), ("outside", C.c_uint32),
                 ("root_rays", C.c_uint32), ("overflow", C.c_uint32), ("ms", C.c_float),
-                ("reserved", C.c_uint32 * 3)]
+                ("cold", C.c_uint32), ("nominees", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 def receiver_params(width: int, height: int, budget: int = 0, fair_cursor: int = 0, fair_quota=None,
@@ -392,6 +397,7 @@ def load_library() -> C.CDLL:
         "lux_radiance_cascades_update_budget": (i32, [vp, u32, u32, vp]),
         "lux_candidate_schedule_phase": (i32, [vp, u32, u32, u32, u32, vp, vp]),
         "lux_candidate_schedule_advance": (i32, [vp, u32]),
+        "lux_cold_priority_enabled": (i32, [u32, u32]),
         "lux_gi_update_receivers": (i32, [vp, u32, vp, vp, vp, vp]),
         "lux_receivers_download": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "lux_gi_samples_download": (i32, [vp, vp]),

@@ -354,7 +354,16 @@ int lux_gi_history_arrays(lux_ctx* ctx, float** history, uint32_t** samples);
  * origins on the coarsest level's cells, same levels and directions); a
  * deposit outside the grids is dropped.  Orders the reference leaves to its
  * atomics are the sequential ones (lowest descriptor index canonical,
- * ordinals and tickets in receiver order, roots by canonical index). */
+ * ordinals and tickets in receiver order, roots by canonical index).
+ * Cold priority (coldPriorityEnabled, split.slang:685-705, rank :836-888):
+ * when lods[0] has a persistent history with history_frames > 0 that is not
+ * valid (never blended since lux_gi_set_history / lux_gi_configure /
+ * lux_gi_reset_history or a whole-history invalidation), a budgeted update
+ * first admits one nominee per probe (its receiver of lowest frame-seeded
+ * rank), probes of larger receiver populations first, then fills the budget
+ * from the fair window; admission and fill go in ascending fair rank, a
+ * nominee's ordinal is 0.  The caller then keeps its fair cursor
+ * (stats->cold). */
 typedef struct {
     uint32_t width, height; /* receiver grid (params.size.xy) */
     uint32_t budget;        /* sampling.x: receivers traced per update, 0 = all */
@@ -375,7 +384,9 @@ typedef struct {
     uint32_t root_rays;   /* ray total of the prefix roots (= rays selected) */
     uint32_t overflow;    /* descriptors the probe directory could not hold */
     float ms;             /* device time of the update */
-    uint32_t reserved[3];
+    uint32_t cold;        /* 1: cold-priority selection (do not advance the fair cursor) */
+    uint32_t nominees;    /* cold nominees admitted */
+    uint32_t reserved;
 } lux_receiver_stats;
 
 int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count, const lux_lod_view* view,
@@ -383,7 +394,8 @@ int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count, const lux_
                             lux_receiver_stats* stats /* nullable */);
 /* The last receiver update's selection (on lods[0]'s context; any pointer may
  * be NULL): per descriptor (2 * count * levels, (variant * count + receiver)
- * * levels + level) its key (int32[4]), flags (1 seeded, 2 ray), canonical
+ * * levels + level) its key (int32[4]), flags (1 seeded, 2 ray, 4 admitted
+ * cold nominee), canonical
  * descriptor, probe ray count and offset; per (variant, receiver) its LOD and
  * sequence (0xffffffff: no ray); the tickets and their number. */
 int lux_receivers_download(lux_ctx* ctx, int32_t* keys, uint32_t* flags, uint32_t* canonical,

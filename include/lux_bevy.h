@@ -71,13 +71,16 @@ int lux_radiance_cascades_update_budget(const lux_radiance_cascades* in, uint32_
 
 /* CandidateSchedule (src/render.rs:5172-5211): zero-initialise, then per
  * update phase() -> lux_receiver_params.fair_cursor / fair_quota and, after
- * a submitted update, advance(1). */
+ * a submitted update, advance(!lux_receiver_stats.cold). */
 typedef struct lux_candidate_schedule {
     uint32_t width, height, count, cursor, pending;
 } lux_candidate_schedule;
 int lux_candidate_schedule_phase(lux_candidate_schedule* s, uint32_t width, uint32_t height, uint32_t count,
                                  uint32_t budget, uint32_t* cursor, uint32_t* quota);
 int lux_candidate_schedule_advance(lux_candidate_schedule* s, uint32_t consume_fair_cursor);
+/* cold_priority_enabled (src/render.rs:128-131): history frames > 0 and the
+ * history not valid (the renderer's world intervals are persistent). */
+int lux_cold_priority_enabled(uint32_t history_frames, uint32_t history_was_valid);
 
 #ifdef __cplusplus
 }

@@ -273,14 +273,19 @@ uint32_t orc_rx_directory_capacity(uint32_t count);
 /* Descriptors (variant, receiver, level) -> keys int32[4], flags, and the
  * LOD of each (variant, receiver). view may be NULL (LOD 0 everywhere). */
 void orc_rx_seed(const float* receivers, const orc_receiver_params* rp, const orc_lod_view* view,
-                 float base_spacing, uint32_t levels, int32_t* keys, uint32_t* flags, uint32_t* lods);
-/* rank/parents/prefix/generateDirections over 2 * count * levels descriptors.
+                 float base_spacing, uint32_t levels, int cold, int32_t* keys, uint32_t* flags, uint32_t* lods);
+uint32_t orc_cold_candidate_rank(uint32_t sample, uint32_t width, uint32_t height, uint32_t count, uint32_t frame);
+uint32_t orc_cold_candidate_from_rank(uint32_t rank, uint32_t width, uint32_t height, uint32_t count,
+                                      uint32_t frame);
+/* rank (with its cold-priority phases when cold)/parents/prefix/
+ * generateDirections over 2 * count * levels descriptors.
  * Per descriptor: canonical index (EMPTY if not seeded), its probe's ray
  * count and offset; per (variant, receiver): sequence (EMPTY: no ray); the
  * ticket list.  Returns the root total. */
 uint32_t orc_rx_select(uint32_t count, uint32_t levels, const int32_t* keys, const uint32_t* flags,
-                       uint32_t* canonical, uint32_t* probe_count, uint32_t* probe_offset, uint32_t* sequence,
-                       uint32_t* tickets, uint32_t* n_tickets);
+                       const orc_receiver_params* rp, int cold, uint32_t* canonical, uint32_t* probe_count,
+                       uint32_t* probe_offset, uint32_t* sequence, uint32_t* tickets, uint32_t* n_tickets,
+                       uint32_t* flags_out /* nullable: the flags after rank */);
 /* directionIndex (common.slang:384-394) over nphi x ny bins. */
 uint32_t orc_direction_index(const float d[3], uint32_t nphi, uint32_t ny);
 /* Receiver trace + deposit + reduce into the dense grids of a LOD pyramid

@@ -278,8 +278,10 @@ def _tsig():
         ("orc_candidate_selected", i32, [u32, vp]),
         ("orc_rx_cell", None, [vp, C.c_float, u32, u32, vp]),
         ("orc_rx_directory_capacity", u32, [u32]),
-        ("orc_rx_seed", None, [vp, vp, vp, C.c_float, u32, vp, vp, vp]),
-        ("orc_rx_select", u32, [u32, u32, vp, vp, vp, vp, vp, vp, vp, vp]),
+        ("orc_rx_seed", None, [vp, vp, vp, C.c_float, u32, i32, vp, vp, vp]),
+        ("orc_rx_select", u32, [u32, u32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp]),
+        ("orc_cold_candidate_rank", u32, [u32, u32, u32, u32, u32]),
+        ("orc_cold_candidate_from_rank", u32, [u32, u32, u32, u32, u32]),
         ("orc_direction_index", u32, [vp, u32, u32]),
         ("orc_rx_trace", None, [vp, vp, u32, vp, u32, vp, u32, vp, u32, vp, vp, vp, u32, u32, vp, vp, vp]),
     ]:
@@ -529,9 +531,18 @@ def direction_index(d, nphi: int, ny: int) -> int:
     return int(_tsig().orc_direction_index(v.ctypes.data, nphi, ny))
 
 
-def rx_seed(receivers: np.ndarray, rp: ReceiverParams, view, base_spacing: float, levels: int):
+def cold_candidate_rank(sample: int, width: int, height: int, count: int, frame: int) -> int:
+    return int(_tsig().orc_cold_candidate_rank(sample, width, height, count, frame))
+
+
+def cold_candidate_from_rank(rank: int, width: int, height: int, count: int, frame: int) -> int:
+    return int(_tsig().orc_cold_candidate_from_rank(rank, width, height, count, frame))
+
+
+def rx_seed(receivers: np.ndarray, rp: ReceiverParams, view, base_spacing: float, levels: int, cold: bool = False):
     """Descriptors of every (variant, receiver, level): keys (2*count*levels, 4)
-    int32, flags uint32, and the LOD of each (variant, receiver) (2, count)."""
+    int32, flags uint32, and the LOD of each (variant, receiver) (2, count);
+    cold: every receiver's selected LOD is eligible (cold priority)."""
     count = rp.width * rp.height
     rc = np.ascontiguousarray(receivers, np.float32)
     assert rc.shape == (count, 6)
@@ -540,11 +551,16 @@ def rx_seed(receivers: np.ndarray, rp: ReceiverParams, view, base_spacing: float
     flags = np.zeros(nd, np.uint32)
     lods = np.zeros((2, count), np.uint32)
     _tsig().orc_rx_seed(rc.ctypes.data, C.byref(rp), None if view is None else C.byref(view), base_spacing,
-                        levels, keys.ctypes.data, flags.ctypes.data, lods.ctypes.data)
+                        levels, int(cold), keys.ctypes.data, flags.ctypes.data, lods.ctypes.data)
     return keys, flags, lods
 
 
-def rx_select(count: int, levels: int, keys: np.ndarray, flags: np.ndarray) -> dict:
+def rx_select(count: int, levels: int, keys: np.ndarray, flags: np.ndarray, rp: ReceiverParams | None = None,
+              cold: bool = False) -> dict:
+    """rank / parents / prefix / generateDirections; cold runs rank's
+    cold-priority phases with the receiver parameters rp.  flags: the
+    descriptor flags after rank (cold: level-0 ray = traced, 4 = admitted
+    nominee)."""
     keys = np.ascontiguousarray(keys, np.int32)
     flags = np.ascontiguousarray(flags, np.uint32)
     nd = 2 * count * levels
@@ -555,11 +571,14 @@ def rx_select(count: int, levels: int, keys: np.ndarray, flags: np.ndarray) -> d
     seq = np.zeros((2, count), np.uint32)
     tickets = np.zeros(max(count, 1), np.uint32)
     nt = C.c_uint32()
-    root = _tsig().orc_rx_select(count, levels, keys.ctypes.data, flags.ctypes.data, canonical.ctypes.data,
+    fout = np.zeros(nd, np.uint32)
+    assert not cold or rp is not None
+    root = _tsig().orc_rx_select(count, levels, keys.ctypes.data, flags.ctypes.data,
+                                 None if rp is None else C.byref(rp), int(cold), canonical.ctypes.data,
                                  pcount.ctypes.data, poffset.ctypes.data, seq.ctypes.data, tickets.ctypes.data,
-                                 C.byref(nt))
+                                 C.byref(nt), fout.ctypes.data)
     return dict(canonical=canonical, count=pcount, offset=poffset, sequence=seq, tickets=tickets[:nt.value],
-                root=int(root))
+                root=int(root), flags=fout)
 
 
 def rx_trace(built: BuiltScene, lights: np.ndarray, materials: np.ndarray, grids, receivers: np.ndarray,

@@ -7,9 +7,10 @@
  *                             transportCandidateFromRank / transportCandidateSelected
  *                             (shaders/split.slang:567-663; the reference's own restatement
  *                             src/transport_tests.rs:1127-1231, src/render.rs:12954-13001)
- *   seed                      shaders/split.slang:734-793 (primary receivers; no cold priority,
- *                             no secondary seeds)
- *   rank                      shaders/split.slang:799-893 (the non-cold branch)
+ *   seed                      shaders/split.slang:734-793 (primary receivers; no secondary
+ *                             seeds)
+ *   rank                      shaders/split.slang:799-893, with its cold-priority phases
+ *                             (coldPriorityEnabled / reserveColdUpdate, :685-705)
  *   parents / prefix          shaders/split.slang:898-961
  *   generateDirections        shaders/split.slang:963-983
  *
@@ -101,6 +102,39 @@ uint32_t orc_candidate_from_rank(uint32_t rank, uint32_t width, uint32_t height,
     return s;
 }
 
+/* coldCandidateRank / coldCandidateFromRank (split.slang:631-657): the same
+ * permutation with a frame-seeded seed. */
+static uint32_t cold_seed(uint32_t w, uint32_t h, uint32_t count, uint32_t frame, uint32_t mask) {
+    return (perm_seed(w, h, count, mask) ^ (frame * 0x9e3779b9u)) & mask;
+}
+
+uint32_t orc_cold_candidate_rank(uint32_t sample, uint32_t width, uint32_t height, uint32_t count, uint32_t frame) {
+    if (count <= 1u) return 0u;
+    uint32_t mask = perm_mask(count), seed = cold_seed(width, height, count, frame, mask);
+    uint32_t r = permute(sample, mask, seed);
+    while (r >= count) r = permute(r, mask, seed);
+    return r;
+}
+
+uint32_t orc_cold_candidate_from_rank(uint32_t rank, uint32_t width, uint32_t height, uint32_t count,
+                                      uint32_t frame) {
+    if (count <= 1u) return 0u;
+    uint32_t mask = perm_mask(count), seed = cold_seed(width, height, count, frame, mask);
+    uint32_t s = inverse_permute(rank, mask, seed);
+    while (s >= count) s = inverse_permute(s, mask, seed);
+    return s;
+}
+
+/* coldPopulationBin (split.slang:680-687). */
+static uint32_t cold_bin(uint32_t population) {
+    uint32_t bin = 0;
+    while (population > 1u && bin < 31u) {
+        population >>= 1;
+        ++bin;
+    }
+    return bin;
+}
+
 int orc_candidate_selected(uint32_t sample, const orc_receiver_params* rp) {
     uint32_t count = rp->width * rp->height;
     if (rp->flags & ORC_RX_DISABLED) return 0;
@@ -134,7 +168,7 @@ void orc_rx_cell(const float pos[3], float base_spacing, uint32_t level, uint32_
 }
 
 void orc_rx_seed(const float* receivers, const orc_receiver_params* rp, const orc_lod_view* view,
-                 float base_spacing, uint32_t levels, int32_t* keys, uint32_t* flags, uint32_t* lods) {
+                 float base_spacing, uint32_t levels, int cold, int32_t* keys, uint32_t* flags, uint32_t* lods) {
     const uint32_t count = rp->width * rp->height;
     for (uint32_t s = 0; s < count; ++s) {
         float pos[3], n[3];
@@ -148,7 +182,8 @@ void orc_rx_seed(const float* receivers, const orc_receiver_params* rp, const or
             float phase = (float)(bits >> 8) * (1.0f / 16777216.0f);
             selected_lod = phase < blend ? 1u : 0u;
         }
-        int ray = ok && orc_candidate_selected(s, rp);
+        /* cold priority: every receiver's selected LOD is eligible (eligibleRay) */
+        int ray = ok && (cold || orc_candidate_selected(s, rp));
         for (uint32_t v = 0; v < 2; ++v) {
             lods[v * count + s] = v ? high : low;
             for (uint32_t L = 0; L < levels; ++L) {
@@ -219,9 +254,81 @@ uint32_t orc_rx_directory_capacity(uint32_t count) {
     return v + 1u;
 }
 
+/* The cold-priority phases of rank (split.slang:836-888) after the
+ * descriptors are ranked: one nominee per probe (lowest frame-seeded rank
+ * among its eligible descriptors), probes admitted by population bin within
+ * the budget, then fair-window receivers fill what the budget leaves.  The
+ * orders the reference leaves to its atomics are fixed: admission tickets
+ * inside a bin and the fair fill go in the order of its rank-mapped dispatch
+ * (ascending candidate rank, variant 0 first); local ordinals are 0 for the
+ * nominee, then the fill receivers in ascending descriptor index.  mark: 2
+ * admitted nominee, 1 other traced; ordinals and level-0 counts are
+ * written. */
+static void cold_phases(uint32_t count, uint32_t levels, const int32_t* keys, const uint32_t* flags,
+                        const uint32_t* canonical, const orc_receiver_params* rp, uint8_t* mark, uint32_t* cnt,
+                        uint32_t* ordinal) {
+    const uint64_t nd = 2ull * count * levels;
+    const uint32_t budget = rp->budget;
+    uint32_t* prio = (uint32_t*)calloc((size_t)nd, sizeof(uint32_t));
+    uint32_t* pop = (uint32_t*)calloc((size_t)nd, sizeof(uint32_t));
+    /* phase 0: nominee priority and population per level-0 probe */
+    for (uint64_t d = 0; d < nd; d += levels) {
+        if (!mark[d]) continue;
+        uint32_t P = canonical[d], smp = (uint32_t)((d / levels) % count);
+        uint32_t pr = 0xffffffffu - orc_cold_candidate_rank(smp, rp->width, rp->height, count, rp->frame);
+        if (pr > prio[P]) prio[P] = pr;
+        pop[P]++;
+    }
+    /* phase 1: probes per population bin */
+    uint32_t hist[32] = {0};
+    for (uint64_t d = 0; d < nd; d += levels)
+        if (canonical[d] == (uint32_t)d && pop[d]) hist[cold_bin(pop[d])]++;
+    const uint32_t maxbin = cold_bin(count);
+    /* phase 2 in rank-mapped dispatch order: admit nominees */
+    uint32_t tickets[32] = {0}, counter = 0;
+    for (uint32_t r = 0; r < count; ++r) {
+        uint32_t smp = orc_candidate_from_rank(r, rp->width, rp->height, count);
+        for (uint32_t v = 0; v < 2; ++v) {
+            uint64_t P = ((uint64_t)v * count + smp) * levels;
+            if (canonical[P] != (uint32_t)P || prio[P] == 0u) continue;
+            uint32_t nsmp = orc_cold_candidate_from_rank(0xffffffffu - prio[P], rp->width, rp->height, count,
+                                                         rp->frame);
+            uint64_t nom = (uint64_t)nsmp * levels;
+            const int32_t* a = keys + 4 * nom;
+            const int32_t* b = keys + 4 * P;
+            if (canonical[nom] == EMPTY || a[0] != b[0] || a[1] != b[1] || a[2] != b[2] || a[3] != b[3])
+                nom = ((uint64_t)count + nsmp) * levels;
+            uint32_t bin = cold_bin(pop[P]), higher = 0;
+            for (uint32_t hb = bin + 1u; hb <= maxbin; ++hb) higher += hist[hb];
+            int admitted = 0;
+            if (pop[P] != 0u && higher < budget) admitted = tickets[bin]++ < budget - higher;
+            mark[nom] = admitted ? 2u : 0u;
+            ordinal[nom] = 0;
+            cnt[P] = admitted ? 1u : 0u;
+            if (admitted) counter++;
+        }
+    }
+    /* phase 3 in the same order: admitted nominees stay, fair-window
+     * receivers take what the budget leaves */
+    for (uint32_t r = 0; r < count; ++r) {
+        uint32_t smp = orc_candidate_from_rank(r, rp->width, rp->height, count);
+        for (uint32_t v = 0; v < 2; ++v) {
+            uint64_t d = ((uint64_t)v * count + smp) * levels;
+            if (mark[d] != 1u) continue;
+            if (orc_candidate_selected(smp, rp) && counter < budget) counter++;
+            else mark[d] = 0u;
+        }
+    }
+    for (uint64_t d = 0; d < nd; d += levels)
+        if (mark[d] == 1u) ordinal[d] = cnt[canonical[d]]++;
+    free(prio);
+    free(pop);
+}
+
 uint32_t orc_rx_select(uint32_t count, uint32_t levels, const int32_t* keys, const uint32_t* flags,
-                       uint32_t* canonical, uint32_t* probe_count, uint32_t* probe_offset, uint32_t* sequence,
-                       uint32_t* tickets, uint32_t* n_tickets) {
+                       const orc_receiver_params* rp, int cold, uint32_t* canonical, uint32_t* probe_count,
+                       uint32_t* probe_offset, uint32_t* sequence, uint32_t* tickets, uint32_t* n_tickets,
+                       uint32_t* flags_out) {
     const uint64_t nd = 2ull * count * levels;
     directory D;
     D.cap = orc_rx_directory_capacity(count);
@@ -240,9 +347,20 @@ uint32_t orc_rx_select(uint32_t count, uint32_t levels, const int32_t* keys, con
         uint32_t L = (uint32_t)(d % levels);
         canonical[d] = dir_insert(&D, keys + 4 * d, L, (uint32_t)d);
     }
-    for (uint64_t d = 0; d < nd; d += levels) {
-        if (canonical[d] == EMPTY || !(flags[d] & ORC_RX_RAY)) continue;
-        ordinal[d] = cnt[canonical[d]]++;
+    uint8_t* mark = (uint8_t*)calloc((size_t)nd, 1);
+    for (uint64_t d = 0; d < nd; d += levels) mark[d] = canonical[d] != EMPTY && (flags[d] & ORC_RX_RAY);
+    if (flags_out) memcpy(flags_out, flags, sizeof(uint32_t) * (size_t)nd);
+    if (cold) {
+        cold_phases(count, levels, keys, flags, canonical, rp, mark, cnt, ordinal);
+        /* level-0 flags after the cold phases: ray = traced, 4 = admitted nominee */
+        for (uint64_t d = 0; d < nd; d += levels) {
+            if (flags_out)
+                flags_out[d] = (flags[d] & ORC_RX_VALID) | (mark[d] ? ORC_RX_RAY : 0u) | (mark[d] == 2u ? 4u : 0u);
+            if (mark[d]) mark[d] = 1u;
+        }
+    } else {
+        for (uint64_t d = 0; d < nd; d += levels)
+            if (mark[d]) ordinal[d] = cnt[canonical[d]]++;
     }
     /* parents: level L counts from the eight children at L - 1 */
     for (uint32_t L = 1; L < levels; ++L)
@@ -289,7 +407,7 @@ uint32_t orc_rx_select(uint32_t count, uint32_t levels, const int32_t* keys, con
         for (uint32_t v = 0; v < 2; ++v) {
             uint64_t d = ((uint64_t)v * count + s) * levels;
             sequence[(uint64_t)v * count + s] = EMPTY;
-            if (canonical[d] == EMPTY || !(flags[d] & ORC_RX_RAY)) continue;
+            if (!mark[d]) continue;
             sequence[(uint64_t)v * count + s] = ordinal[d] + off[canonical[d]];
             sel = 1;
         }
@@ -305,5 +423,6 @@ uint32_t orc_rx_select(uint32_t count, uint32_t levels, const int32_t* keys, con
     free(cnt);
     free(off);
     free(ordinal);
+    free(mark);
     return root;
 }

@@ -10,7 +10,9 @@ receiver tracing, with a fair window (budget and cursor), with the LOD
 overlap of a perspective camera (deposits of LODs without a grid dropped),
 with long runs of deposits per interval, and with the temporal history over
 consecutive updates; the receiver rays traced as regrouped wavefront stages
-and as the fused kernel.
+and as the fused kernel.  Cold-priority selection (a persistent history not
+yet valid) equals the oracle's at budgets below the probe count, at a
+partly admitted population bin and with a fair fill.
 """
 import numpy as np
 import pytest
@@ -163,8 +165,9 @@ def test_gpu_receiver_long_runs_match_oracle():
 
 
 def test_gpu_receiver_updates_with_history_match_oracle():
-    """Three updates of a moving fair window with the temporal history on:
-    the blended intervals, history and probes follow the oracle's chain."""
+    """Three updates of a moving fair window with the temporal history on
+    (the first one cold-priority): the selections, blended intervals,
+    history and probes follow the oracle's chain."""
     sc, _ = scenes.cornell()
     p0 = base_params()
     RB, TB, FRAMES = 0.88, 0.98, 24
@@ -183,11 +186,15 @@ def test_gpu_receiver_updates_with_history_match_oracle():
         for f in range(3):
             cursor, quota = sched.phase_for(W, H, W * H, 2000)
             rp_args = dict(budget=2000, fair_cursor=cursor, fair_quota=quota, frame=10 + f)
-            lux.update_receivers(ctxs, None, recv, lux.receiver_params(W, H, **rp_args))
-            sched.advance_after_successful_update(True)
+            st = lux.update_receivers(ctxs, None, recv, lux.receiver_params(W, H, **rp_args))
+            cold = f == 0                                 # the first update finds no valid history
+            assert st.cold == int(cold)
+            sched.advance_after_successful_update(not st.cold)
             rp = O.receiver_params(W, H, **rp_args)
-            keys, flags, lods = O.rx_seed(rec, rp, None, p0.spacing, p0.levels)
-            sel = O.rx_select(W * H, p0.levels, keys, flags)
+            keys, flags, lods = O.rx_seed(rec, rp, None, p0.spacing, p0.levels, cold=cold)
+            sel = O.rx_select(W * H, p0.levels, keys, flags, rp, cold=cold)
+            got = lux.receivers_download(c, W * H, p0.levels, directory=False)
+            assert np.array_equal(got["sequence"], sel["sequence"]) and np.array_equal(got["tickets"], sel["tickets"])
             g0.frame = rp.frame
             ivs, sms, _ = O.rx_trace(built, lights, sc.materials, [g0], rec, lods, sel["sequence"], sel["tickets"],
                                      rp.frame)
@@ -201,6 +208,67 @@ def test_gpu_receiver_updates_with_history_match_oracle():
             bits_equal(g["merged"], merged, f"merged, update {f}")
             bits_equal(g["probes"], O.gi_pack(g0, merged), f"probes, update {f}")
         assert (hist[:, 3] > 1.0).any()                   # some intervals saw more than one update
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+def check_selection(ctx, rec, rp, oview, levels, cold, st):
+    keys, flags, lods = O.rx_seed(rec, rp, oview, 0.125, levels, cold=cold)
+    sel = O.rx_select(rp.width * rp.height, levels, keys, flags, rp, cold=cold)
+    got = lux.receivers_download(ctx, rp.width * rp.height, levels)
+    assert np.array_equal(got["keys"], keys)
+    assert np.array_equal(got["flags"], sel["flags"])
+    assert np.array_equal(got["lods"], lods)
+    for key in ("canonical", "count", "offset", "sequence", "tickets"):
+        assert np.array_equal(got[key], sel[key]), key
+    assert st.cold == int(cold) and st.root_rays == sel["root"] and st.tickets == len(sel["tickets"])
+    assert st.nominees == int(((sel["flags"][::levels] & 4) != 0).sum())
+    # probes with an eligible receiver (cold: every receiver's selected LOD)
+    sel["probes"] = len(set(sel["canonical"][::levels][(flags[::levels] & O.RX_RAY) != 0].tolist()))
+    return sel
+
+
+@pytest.mark.parametrize("budget", [64, 700, 3000])
+def test_gpu_cold_priority_selection_matches_oracle(budget):
+    """A persistent history that is not valid makes a budgeted update
+    cold-priority: nominees per probe admitted by population bin (in part
+    at the boundary bin), the fair fill, ordinals with the nominee first --
+    equal to the oracle.  The next update, with the history valid, is the
+    plain fair window; a reset makes the following one cold again;
+    history_frames 0, a non-persistent history and budget 0 never are."""
+    sc, _ = scenes.cornell()
+    p0 = base_params()
+    ctxs = pyramid(sc, p0, 2)
+    try:
+        c = ctxs[0]
+        recv, rec, terms = camera_receivers(c)
+        view, oview = lux.LodView.make(**terms), O.lod_view(**terms)
+        c.gi_set_history(0.88, 0.98, 24, 1)
+        args = dict(budget=budget, fair_cursor=900, frame=128)
+        rp, orp = lux.receiver_params(W, H, **args), O.receiver_params(W, H, **args)
+        st = lux.update_receivers(ctxs, view, recv, rp)
+        sel = check_selection(c, rec, orp, oview, p0.levels, True, st)
+        assert 0 < st.nominees <= st.root_rays <= budget
+        if sel["probes"] > budget:
+            assert st.nominees == budget == st.root_rays  # more probes than the budget: no fill
+        else:
+            assert st.nominees == sel["probes"] < st.root_rays  # every probe admitted, then the fair fill
+        if budget == 64:
+            assert sel["probes"] > 64
+        if budget == 3000:
+            assert sel["probes"] < 3000
+        st = lux.update_receivers(ctxs, view, recv, rp)   # the history is valid now
+        warm = check_selection(c, rec, orp, oview, p0.levels, False, st)
+        assert not np.array_equal(warm["sequence"], sel["sequence"])
+        c.gi_reset_history()
+        st = lux.update_receivers(ctxs, view, recv, rp)
+        check_selection(c, rec, orp, oview, p0.levels, True, st)
+        for frames, persistent, b in [(0, 1, budget), (24, 0, budget), (24, 1, 0)]:
+            c.gi_set_history(0.88, 0.98, frames, persistent)
+            bargs = dict(args, budget=b)
+            st = lux.update_receivers(ctxs, view, recv, lux.receiver_params(W, H, **bargs))
+            check_selection(c, rec, O.receiver_params(W, H, **bargs), oview, p0.levels, False, st)
     finally:
         for c in ctxs:
             c.close()

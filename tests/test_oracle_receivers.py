@@ -16,7 +16,14 @@ Pinned on the reference's own tests:
     count, children's ray ranges are contiguous under their parent, roots
     tile [0, selected);
   * receiver-grid and update-budget values of the low-end preset
-    (src/render.rs:13316-13336).
+    (src/render.rs:13316-13336);
+  * the cold-priority fixture (src/transport_tests.rs:3015-3620): each
+    probe's nominee is its lowest frame-seeded rank, one nominee admitted
+    per probe, the fair fill reaching the budget with fair-selected
+    receivers only, one LOD per receiver, ordinals 0..count-1 per probe,
+    populations 4 and 2 admitted before 1, a budget below the probe count
+    admitting exactly that many nominees, warm frames keeping the plain fair
+    window; the cold rank's seed through frame 0 and frame = width.
 Beyond those: seed keys against floor(p / spacing) per level; the direction
 bin against numpy's arctan2 and round trips of the dense bin centres; the
 receiver trace's deposits against an independent numpy restatement of the
@@ -271,6 +278,162 @@ def test_prefix_fixture_orders_are_the_sequential_ones():
         assert canon[d] == same.min()
     s47 = r["sequence"][0][1:48]
     assert (np.diff(s47.astype(np.int64)) == 1).all()
+
+
+# ---------------- cold priority ----------------
+def test_cold_rank_folds_the_frame_into_the_permutation_seed():
+    """coldCandidateRank (split.slang:631-657, restated by the reference at
+    src/transport_tests.rs:1170-1194): the fair permutation with seed
+    (w*phi ^ h*c ^ count ^ frame*phi) & mask.  Frame 0 gives the fair rank;
+    frame = width cancels the width term, which is the fair rank of width 0;
+    every frame is a bijection with the inverse as its inverse."""
+    for w, h in [(8, 1), (13, 7), (160, 90), (31, 3)]:
+        n = w * h
+        for frame in [0, 1, 128, 977, w]:
+            ranks = [O.cold_candidate_rank(s, w, h, n, frame) for s in range(n)]
+            assert sorted(ranks) == list(range(n))
+            for s in range(0, n, max(1, n // 50)):
+                assert O.cold_candidate_from_rank(ranks[s], w, h, n, frame) == s
+        fair = [O.candidate_rank(s, w, h, n) for s in range(n)]
+        assert [O.cold_candidate_rank(s, w, h, n, 0) for s in range(n)] == fair
+        assert [O.cold_candidate_rank(s, w, h, n, w) for s in range(n)] == \
+            [O.candidate_rank(s, 0, h, n) for s in range(n)]
+    # frame * phi reaches the 3-bit seed of 8 receivers only through odd frames
+    fair8 = [O.candidate_rank(s, 8, 1, 8) for s in range(8)]
+    assert [O.cold_candidate_rank(s, 8, 1, 8, 128) for s in range(8)] == fair8
+    assert [O.cold_candidate_rank(s, 8, 1, 8, 1) for s in range(8)] != fair8
+
+
+COLD_W, COLD_FRAME = 8, 128
+
+
+def cold_fixture(weighted):
+    """The entries of src/transport_tests.rs:3015-3620: 8 receivers on one
+    row, 4 levels, keys (group >> level, 0, 0, variant << 18 | level << 12),
+    the selected LOD = receiver % 2.  Unweighted: group = receiver / 4 (four
+    probes of population 2).  Weighted: variant 0 is group 0; variant 1 is
+    group 0 for receivers 1 and 3, 1 for 5, 2 for 7, 3 otherwise."""
+    count, levels = COLD_W, 4
+    keys = np.zeros((2 * count * levels, 4), np.int32)
+    flags = np.zeros(2 * count * levels, np.uint32)
+    for s in range(count):
+        for v in range(2):
+            if not weighted:
+                group = s // 4
+            else:
+                group = 0 if v == 0 else {1: 0, 3: 0, 5: 1, 7: 2}.get(s, 3)
+            for L in range(levels):
+                d = (v * count + s) * levels + L
+                keys[d] = (group >> L, 0, 0, (v << 18) | (L << 12))
+                flags[d] = O.RX_VALID | (O.RX_RAY if L == 0 and v == s % 2 else 0)
+    return count, levels, keys, flags
+
+
+def cold_select(weighted, budget):
+    count, levels, keys, flags = cold_fixture(weighted)
+    rp = O.receiver_params(COLD_W, 1, budget, 2, budget, frame=COLD_FRAME)
+    return count, levels, keys, rp, O.rx_select(count, levels, keys, flags, rp, cold=True)
+
+
+def traced(r, count, levels):
+    """(receiver, variant) -> local ordinal of every traced ray."""
+    out = {}
+    for v in range(2):
+        for s in range(count):
+            q = int(r["sequence"][v, s])
+            if q != O.EMPTY:
+                out[(s, v)] = q - int(r["offset"][(v * count + s) * levels])
+    return out
+
+
+def probe_of(r, count, levels, s, v):
+    return int(r["canonical"][(v * count + s) * levels])
+
+
+def test_cold_priority_nominates_each_probe_then_fills_the_fair_budget():
+    count, levels, keys, rp, r = cold_select(False, 6)
+    fair = [O.candidate_selected(s, rp) for s in range(count)]
+    rays = traced(r, count, levels)
+    assert len(rays) == 6 and r["root"] == 6                       # fair fill reaches the budget
+    assert len({s for s, _ in rays}) == 6                          # one LOD per receiver
+    per_probe = {}
+    for (s, v), ordinal in rays.items():
+        per_probe.setdefault(probe_of(r, count, levels, s, v), []).append((s, ordinal))
+    for g in range(2):
+        for v in range(2):
+            first, second = g * 4 + v, g * 4 + v + 2
+            nominee = first if O.cold_candidate_rank(first, COLD_W, 1, count, COLD_FRAME) < \
+                O.cold_candidate_rank(second, COLD_W, 1, count, COLD_FRAME) else second
+            probe = probe_of(r, count, levels, first, v)
+            assert probe == (v * count + g * 4) * levels           # canonical: lowest descriptor
+            assert rays[(nominee, v)] == 0                         # every probe's nominee, ordinal 0
+            got = per_probe[probe]
+            assert sorted(o for _, o in got) == list(range(len(got))) == list(range(int(r["count"][probe])))
+            for s, o in got:
+                if s != nominee:
+                    assert fair[s] and o > 0                       # fill receivers are fair-selected
+    assert ((r["flags"][::levels] & 4) != 0).sum() == 4                # one admitted nominee per probe
+    # the fill takes the fair-selected non-nominees in ascending candidate rank
+    nominees = {s for (s, v), o in rays.items() if o == 0}
+    fill = sorted((O.candidate_rank(s, COLD_W, 1, count), s) for s in range(count) if fair[s] and s not in nominees)
+    assert {s for _, s in fill[:2]} == {s for (s, v) in rays if s not in nominees}
+
+
+def test_cold_budget_admits_dense_probes_before_sparse_ones():
+    count, levels, keys, rp, r = cold_select(True, 2)
+    # projected populations of the selected-LOD receivers' probes
+    pop = {}
+    for s in range(count):
+        pop[tuple(keys[((s % 2) * count + s) * levels])] = pop.get(tuple(keys[((s % 2) * count + s) * levels]), 0) + 1
+    assert [pop[tuple(keys[((s % 2) * count + s) * levels])] for s in range(count)] == [4, 2, 4, 2, 4, 1, 4, 1]
+    counts = [int(r["count"][((s % 2) * count + s) * levels]) for s in [0, 1, 5, 7]]
+    assert counts == [1, 1, 0, 0]                                  # populations 4 and 2 before 1
+    assert r["root"] == 2 and len(traced(r, count, levels)) == 2
+
+
+def test_cold_admission_stays_within_a_budget_below_the_probe_count():
+    count, levels, keys, rp, r = cold_select(False, 2)
+    rays = traced(r, count, levels)
+    assert len(rays) == 2 and r["root"] == 2
+    assert sorted(int(r["count"][(v * count + g * 4 + v) * levels]) for g in range(2) for v in range(2)) == [0, 0, 1, 1]
+    assert all(o == 0 for o in rays.values())                      # each ray is its probe's nominee
+    assert ((r["flags"][::levels] & 4) != 0).sum() == 2
+    # admission tickets inside one population bin go in the rank-mapped
+    # dispatch order of the canonical descriptors
+    order = sorted((O.candidate_rank(g * 4, COLD_W, 1, count) * 2 + v, (g, v)) for g in range(2) for v in range(2))
+    admitted = {(s // 4, v) for (s, v) in rays}
+    assert admitted == {gv for _, gv in order[:2]}
+
+
+def test_warm_selection_is_the_plain_fair_window():
+    count, levels, keys, flags = cold_fixture(False)
+    rp = O.receiver_params(COLD_W, 1, 6, 2, 6, frame=COLD_FRAME)
+    fair = [O.candidate_selected(s, rp) for s in range(count)]
+    warm_flags = flags.copy()
+    for s in range(count):
+        if not fair[s]:
+            warm_flags[((s % 2) * count + s) * levels] &= ~np.uint32(O.RX_RAY)
+    r = O.rx_select(count, levels, keys, warm_flags)
+    rays = traced(r, count, levels)
+    assert sorted(rays) == sorted((s, s % 2) for s in range(count) if fair[s])
+    assert len(rays) == 6 and r["root"] == 6
+
+
+def test_cold_seed_makes_every_receiver_eligible():
+    rng = np.random.default_rng(5)
+    w, h, levels = 6, 5, 3
+    n = w * h
+    rc = np.zeros((n, 6), np.float32)
+    rc[:, :3] = rng.uniform(-2, 2, (n, 3))
+    rc[:, 4] = 1.0
+    rc[7, 3:] = 0.0                                                # no surface
+    rp = O.receiver_params(w, h, 4, 3, 4, frame=9)
+    keys, flags, _ = O.rx_seed(rc, rp, None, 0.5, levels)
+    ckeys, cflags, _ = O.rx_seed(rc, rp, None, 0.5, levels, cold=True)
+    assert (keys == ckeys).all()
+    ray = (flags[: n * levels: levels] & O.RX_RAY) != 0
+    cray = (cflags[: n * levels: levels] & O.RX_RAY) != 0
+    assert ray.sum() == 4 and cray.sum() == n - 1 and not cray[7] and (cray | ~ray).all()
 
 
 # ---------------- seed ----------------
