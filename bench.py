@@ -453,11 +453,12 @@ def batch_section(lux, scenes, args) -> dict:
 
 def receivers_section(lux, scenes, args) -> dict:
     """Receiver-driven updates (seed / rank / parents / prefix /
-    generateDirections, receiver trace, reduce, then history-free merge and
-    pack of each LOD grid) on the colonnade: 1920 x 1080 camera receivers
-    (2,073,600) over a three-LOD pyramid at 0.5 / 1 / 2 m covering the scene,
-    every receiver tracing, and the default component's update budget with
-    the fair window advancing."""
+    generateDirections, receiver trace, reduce, then merge and pack of each
+    LOD grid) on the colonnade: 1920 x 1080 camera receivers (2,073,600) over
+    a three-LOD pyramid at 0.5 / 1 / 2 m covering the scene, every receiver
+    tracing, and the default component's update budget with the fair window
+    advancing -- without history, with the temporal history (warm), and
+    cold-priority (the history reset before each update)."""
     from bevy_penumbra_amd.lux import DeviceBuffer
     scene, _ = scenes.colonnade()
     # grid k (LOD k): spacing and interval length 0.5 * 2^k m, its origin on its
@@ -500,13 +501,24 @@ def receivers_section(lux, scenes, args) -> dict:
            "grids": ["%dx%dx%d at %.1f m from (%g, %g, %g)" % (tuple(g.dims) + (g.spacing,) + tuple(g.origin))
                      for g in grids],
            "levels": 5, "base_directions": 32, "intervals": [int(c.gi_sizes()[0]) for c in ctxs]}
-    for name, b in (("all_receivers", 0), ("default_budget", budget)):
+    # with the temporal history on (persistent, 24 frames): warm updates, and
+    # cold-priority ones (the history reset before each)
+    for name, b, hist in (("all_receivers", 0, None), ("default_budget", budget, None),
+                          ("default_budget_history", budget, "warm"), ("cold_default_budget", budget, "cold")):
+        for c in ctxs:
+            if hist:
+                c.gi_set_history(0.88, 0.98, 24, 1)
+            else:
+                c.gi_history_off()
         sched = lux.CandidateSchedule()
         ms, wall = [], []
         st = None
         for i in range(args.receiver_steps + 2):
             cursor, quota = sched.phase_for(W, H, W * H, b)
             rp = lux.receiver_params(W, H, b, cursor, quota, frame=i)
+            if hist == "cold":
+                for c in ctxs:
+                    c.gi_reset_history()
             t0 = time.perf_counter()
             st = lux.update_receivers(ctxs, view, recv, rp)
             for c in ctxs:
@@ -514,7 +526,8 @@ def receivers_section(lux, scenes, args) -> dict:
             if i >= 2:
                 wall.append((time.perf_counter() - t0) * 1e3)
                 ms.append(st.ms)
-            sched.advance_after_successful_update(True)
+            assert st.cold == (hist == "cold" or (hist == "warm" and i == 0))
+            sched.advance_after_successful_update(not st.cold)
         med = float(np.median(ms))
         sel_view = lux.receivers_download(ctxs[0], W * H, p0.levels, directory=False)
         seeded = (sel_view["flags"][: W * H * p0.levels: p0.levels] & 1) == 1    # variant 0, level 0
@@ -532,7 +545,11 @@ def receivers_section(lux, scenes, args) -> dict:
             "deposits": st.deposits,
             "outside": st.outside,
             "unsupported": st.unsupported,
+            "cold": st.cold,
+            "cold_nominees": st.nominees,
         }
+    for c in ctxs:
+        c.gi_history_off()
     if not args.no_cpu:
         from oracle import oracle as O
         rec = recv.numpy()

@@ -43,7 +43,7 @@
 namespace {
 
 constexpr uint32_t kRxEmpty = 0xffffffffu;
-constexpr uint32_t kRxValid = 1u, kRxRay = 2u, kRxNominee = 4u;
+constexpr uint32_t kRxValid = 1u, kRxRay = 2u, kRxNominee = 4u, kRxFill = 8u;
 constexpr uint32_t kRxLodMask = 31u << 18;
 constexpr uint32_t kRxMaxLods = 8;
 constexpr uint32_t kScanBlock = 1024;
@@ -283,14 +283,21 @@ __global__ __launch_bounds__(kWg) void k_rx_insert(RxDir D, const int4* keys, co
 }
 
 // Sort keys of the (variant, receiver) ray markers: level-0 slot, or `none`
-// (above every key).  With `cold` the key is slot * 2 + (not the probe's
-// nominee), so a probe's nominee sorts first.
-__global__ __launch_bounds__(kWg) void k_rx_ray_keys(const uint32_t* flags, const uint32_t* slot, uint32_t m,
+// (above every key).  With `cold` a ray is traced when it is an admitted
+// nominee or a kept fill (every other loses its ray flag here), and the key
+// is slot * 2 + (not the probe's nominee), so a probe's nominee sorts first.
+__global__ __launch_bounds__(kWg) void k_rx_ray_keys(uint32_t* flags, const uint32_t* slot, uint32_t m,
                                                      uint32_t levels, uint32_t none, uint32_t cold, uint32_t* out) {
     const uint32_t j = blockIdx.x * kWg + threadIdx.x;
     if (j >= m) return;
     const uint64_t d0 = (uint64_t)j * levels;
-    const uint32_t f = flags[d0], b = slot[d0];
+    uint32_t f = flags[d0];
+    const uint32_t b = slot[d0];
+    if (cold && (f & kRxRay)) {
+        const uint32_t g = (f & (kRxNominee | kRxFill)) ? f & ~kRxFill : f & ~kRxRay;
+        if (g != f) flags[d0] = g;
+        f = g;
+    }
     uint32_t k = none;
     if ((f & kRxRay) && b != kRxEmpty) k = cold ? (b << 1) | ((f & kRxNominee) ? 0u : 1u) : b;
     out[j] = k;
@@ -438,34 +445,40 @@ __global__ __launch_bounds__(kWg) void k_rx_cold_admit(RxSeed sd, RxDir D, const
     flags[nom] = admitted ? (f | kRxNominee) : (f & ~kRxRay);
 }
 
-// Phase 3: eligible receivers other than nominees, fair-selected, flagged in
-// rank-mapped order (scanned: their fill tickets).
-__device__ __forceinline__ bool cold_fill_candidate(const RxSeed& sd, const uint32_t* flags, const uint32_t* slot,
-                                                    uint32_t p, uint64_t* d0_out) {
-    const uint64_t d0 = (uint64_t)rank_mapped(sd, p) * sd.levels;
-    *d0_out = d0;
-    const uint32_t f = flags[d0];
-    return (f & kRxRay) && !(f & kRxNominee) && slot[d0] != kRxEmpty && rank_selected(sd, p >> 1);
+// Phase 3 over the fair window only (its ranks [start, start + len) mod
+// count; every fill candidate is in it): position t enumerates the window's
+// (rank, variant) pairs in ascending rank -- the wrapped part [0, start + len
+// - count) first -- so a scan of the candidate flags gives the fill tickets.
+struct RxWindow {
+    uint32_t start, len, wrapped;
+};
+
+__device__ __forceinline__ uint64_t window_descriptor(const RxSeed& sd, const RxWindow& w, uint32_t t) {
+    const uint32_t i = t >> 1;
+    const uint32_t r = i < w.wrapped ? i : w.start + (i - w.wrapped);
+    return ((uint64_t)(t & 1u) * sd.count + sample_of(r, sd.width, sd.height, sd.count, 0u)) * sd.levels;
 }
 
-__global__ __launch_bounds__(kWg) void k_rx_cold_fill_flags(RxSeed sd, const uint32_t* flags, const uint32_t* slot,
-                                                            uint32_t m, RxCold cw) {
-    const uint32_t p = blockIdx.x * kWg + threadIdx.x;
-    if (p >= m) return;
-    uint64_t d0;
-    cw.pos[p] = cold_fill_candidate(sd, flags, slot, p, &d0) ? 1u : 0u;
+__device__ __forceinline__ bool cold_fill_candidate(uint32_t f, uint32_t b) {
+    return (f & kRxRay) && !(f & kRxNominee) && b != kRxEmpty;
 }
 
-// The first fill-quota candidates keep their rays; every other non-nominee
-// loses its.
-__global__ __launch_bounds__(kWg) void k_rx_cold_fill(RxSeed sd, uint32_t* flags, const uint32_t* slot, uint32_t m,
+__global__ __launch_bounds__(kWg) void k_rx_cold_fill_flags(RxSeed sd, RxWindow w, const uint32_t* flags,
+                                                            const uint32_t* slot, RxCold cw) {
+    const uint32_t t = blockIdx.x * kWg + threadIdx.x;
+    if (t >= 2u * w.len) return;
+    const uint64_t d0 = window_descriptor(sd, w, t);
+    cw.pos[t] = cold_fill_candidate(flags[d0], slot[d0]) ? 1u : 0u;
+}
+
+// The first fill-quota candidates are marked kept (kRxFill).
+__global__ __launch_bounds__(kWg) void k_rx_cold_fill(RxSeed sd, RxWindow w, uint32_t* flags, const uint32_t* slot,
                                                       RxCold cw) {
-    const uint32_t p = blockIdx.x * kWg + threadIdx.x;
-    if (p >= m) return;
-    uint64_t d0;
-    const bool cand = cold_fill_candidate(sd, flags, slot, p, &d0);
+    const uint32_t t = blockIdx.x * kWg + threadIdx.x;
+    if (t >= 2u * w.len) return;
+    const uint64_t d0 = window_descriptor(sd, w, t);
     const uint32_t f = flags[d0];
-    if ((f & kRxRay) && !(f & kRxNominee) && !(cand && cw.pos[p] < cw.table[66])) flags[d0] = f & ~kRxRay;
+    if (cold_fill_candidate(f, slot[d0]) && cw.pos[t] < cw.table[66]) flags[d0] = f | kRxFill;
 }
 
 // parents (split.slang:898-922): level L's ray count from its eight children.
@@ -1012,9 +1025,23 @@ extern "C" int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count,
         hipLaunchKernelGGL(k_rx_cold_order, dim3(blocks_for(m)), dim3(kWg), 0, st, sd, D, slot, m, cw);
         if (int rc = scan_exclusive(c, cw.pos, m, counters + 11)) return rc;
         hipLaunchKernelGGL(k_rx_cold_admit, dim3(blocks_for(cap)), dim3(kWg), 0, st, sd, D, slot, flags, cw);
-        hipLaunchKernelGGL(k_rx_cold_fill_flags, dim3(blocks_for(m)), dim3(kWg), 0, st, sd, flags, slot, m, cw);
-        if (int rc = scan_exclusive(c, cw.pos, m, counters + 12)) return rc;
-        hipLaunchKernelGGL(k_rx_cold_fill, dim3(blocks_for(m)), dim3(kWg), 0, st, sd, flags, slot, m, cw);
+        // the fair window (transportCandidateSelected's ranks)
+        RxWindow win;
+        if (rp->budget >= count || rp->fair_quota >= count) {
+            win.start = 0;
+            win.len = count;
+        } else {
+            win.start = (rp->fair_cursor & 0x03ffffffu) % count;
+            win.len = rp->fair_quota;
+        }
+        win.wrapped = win.start + win.len > count ? win.start + win.len - count : 0u;
+        if (win.len > 0) {
+            hipLaunchKernelGGL(k_rx_cold_fill_flags, dim3(blocks_for(2ull * win.len)), dim3(kWg), 0, st, sd, win,
+                               flags, slot, cw);
+            if (int rc = scan_exclusive(c, cw.pos, 2u * win.len, counters + 12)) return rc;
+            hipLaunchKernelGGL(k_rx_cold_fill, dim3(blocks_for(2ull * win.len)), dim3(kWg), 0, st, sd, win, flags,
+                               slot, cw);
+        }
     }
     const uint32_t none = cold ? cap << 1 : cap, shift = cold ? 1u : 0u;
     uint32_t *sk_in, *sk, *sv;
