@@ -259,6 +259,53 @@ def receiver_params(width: int, height: int, budget: int = 0, fair_cursor: int =
     return rp
 
 
+class ScreenParams(C.Structure):
+    """lux_screen_params (include/lux.h): the temporal / denoise parameters."""
+    _fields_ = [
+        ("world_from_clip", C.c_float * 16),
+        ("previous_clip_from_world", C.c_float * 16),
+        ("previous_world_from_clip", C.c_float * 16),
+        ("viewport", C.c_uint32 * 4),
+        ("target", C.c_uint32 * 2),
+        ("size", C.c_uint32 * 2),
+        ("resolve_scale", C.c_uint32),
+        ("persistent", C.c_uint32),
+        ("probe_spacing", C.c_float),
+        ("normal_bias", C.c_float),
+        ("history_blend", C.c_float),
+        ("history_frames", C.c_float),
+        ("stability", C.c_float * 4),
+        ("source_changes", C.c_uint32),
+        ("source_flags", C.c_uint32),
+        ("motion_epoch", C.c_uint32),
+        ("frame", C.c_uint32),
+        ("dirty_min", C.c_float * 4),
+        ("dirty_max", C.c_float * 4),
+    ]
+
+    def extent(self):
+        w, h = C.c_uint32(), C.c_uint32()
+        load_library().lux_screen_extent(C.byref(self), C.byref(w), C.byref(h))
+        return int(w.value), int(h.value)
+
+
+def screen_temporal(ctx, params: ScreenParams, current, current_background, depth, gbuffer, raster, motion,
+                    previous, previous_metadata, previous_receiver, previous_background, irradiance, metadata,
+                    receiver, background):
+    """lux_screen_temporal on device buffers (None for the nullable ones)."""
+    ctx._check(ctx.lib.lux_screen_temporal(ctx.h, C.byref(params), *[_ptr(b) for b in (
+        current, current_background, depth, gbuffer, raster, motion, previous, previous_metadata,
+        previous_receiver, previous_background, irradiance, metadata, receiver, background)]),
+        "lux_screen_temporal")
+
+
+def screen_denoise(ctx, params: ScreenParams, pass_index: int, irradiance, moments, receiver, depth, out,
+                   out_receiver):
+    """lux_screen_denoise: one pass on device buffers."""
+    ctx._check(ctx.lib.lux_screen_denoise(ctx.h, C.byref(params), pass_index, *[_ptr(b) for b in (
+        irradiance, moments, receiver, depth, out, out_receiver)]), "lux_screen_denoise")
+
+
 def update_receivers(contexts, view, receivers, params: ReceiverParams) -> ReceiverStats:
     """lux_gi_update_receivers over the LOD pyramid contexts[k] = LOD k; view
     None: LOD 0 for every receiver."""
@@ -398,6 +445,9 @@ def load_library() -> C.CDLL:
         "lux_candidate_schedule_phase": (i32, [vp, u32, u32, u32, u32, vp, vp]),
         "lux_candidate_schedule_advance": (i32, [vp, u32]),
         "lux_cold_priority_enabled": (i32, [u32, u32]),
+        "lux_screen_extent": (i32, [vp, vp, vp]),
+        "lux_screen_temporal": (i32, [vp, vp] + [vp] * 14),
+        "lux_screen_denoise": (i32, [vp, vp, u32] + [vp] * 6),
         "lux_gi_update_receivers": (i32, [vp, u32, vp, vp, vp, vp]),
         "lux_receivers_download": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "lux_gi_samples_download": (i32, [vp, vp]),

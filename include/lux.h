@@ -603,6 +603,59 @@ int lux_batch_get_config(lux_batch* b, lux_trace_config* cfg);
 int lux_batch_autotune(lux_batch* b, uint32_t repeats, lux_trace_config* best, float* best_ms,
                        uint32_t* evaluations);
 
+/* ---- screen-space temporal and denoise passes (f4's screen filters) ----
+ * temporal (shaders/cascades.slang:310-622) and denoise (:624-711) over the
+ * resolve extent ((viewport w + s - 1) / s) x ((viewport h + s - 1) / s), s =
+ * max(resolve_scale, 1) (giResolveScale / giResolveExtent, common.slang:95-100).
+ * Resolve-extent images are float4 row-major; full-resolution images (the
+ * render target, target[0] x target[1], viewport inside it) are depth
+ * (float, reverse-Z, > 0 on surfaces), the G-buffer (uint4; .w holds the
+ * packed shading normal), the raster receiver (uint4: identity .xy, packed
+ * geometric normal .z) and motion (float2, NDC units per viewport; read only
+ * when stability[2] > 0 and, for persistent fields, stability[1] > 0.5).
+ * All pointers are device pointers; outputs must not alias inputs.  The
+ * passes run on ctx's stream.  Matrices are row-major (h = M (x, y, z, 1)).
+ * Field mapping to the reference's Parameters: history_blend = history.x,
+ * history_frames = history.w, persistent = intervalHistory.y > 0.5,
+ * stability = stability.xyz(w), source_changes / source_flags /
+ * motion_epoch = sourceHistory.yzw, frame = frame.x, probe_spacing /
+ * normal_bias = tuning.x / tuning.z, size = size.xy, resolve_scale =
+ * sampling.w, dirty_min / dirty_max = dirtyMin / dirtyMax (w > 0: on).
+ * The debug-only temporal diagnostic counters are not produced.  312 bytes. */
+typedef struct {
+    float world_from_clip[16];
+    float previous_clip_from_world[16];
+    float previous_world_from_clip[16];
+    uint32_t viewport[4];
+    uint32_t target[2];
+    uint32_t size[2];
+    uint32_t resolve_scale;
+    uint32_t persistent;
+    float probe_spacing, normal_bias;
+    float history_blend, history_frames;
+    float stability[4];
+    uint32_t source_changes, source_flags, motion_epoch, frame;
+    float dirty_min[4], dirty_max[4];
+} lux_screen_params;
+int lux_screen_extent(const lux_screen_params* params, uint32_t* width, uint32_t* height);
+/* temporal: current raw resolve (alpha: 0 not eligible, -1 unsupported, 1 +
+ * 0.25 coverage), its background (nullable: none), the previous frame's
+ * temporal irradiance / metadata / receiver words / background (the last one
+ * nullable) -> this frame's four. */
+int lux_screen_temporal(lux_ctx* ctx, const lux_screen_params* params, const float* current,
+                        const float* current_background, const float* depth, const uint32_t* gbuffer,
+                        const uint32_t* raster, const float* motion, const float* previous,
+                        const float* previous_metadata, const float* previous_receiver,
+                        const float* previous_background, float* irradiance, float* metadata, float* receiver,
+                        float* background);
+/* One denoise pass (pass index = the reference's frame.y: step 1, then
+ * max(round(cell stride / 4), 1) * 2^pass): the previous pass's (or
+ * temporal's) irradiance and receiver words, temporal's metadata as moments
+ * -> filtered irradiance and receiver words.  Ping-pong between passes. */
+int lux_screen_denoise(lux_ctx* ctx, const lux_screen_params* params, uint32_t pass, const float* irradiance,
+                       const float* moments, const float* receiver, const float* depth, float* out,
+                       float* out_receiver);
+
 #ifdef __cplusplus
 }
 #endif

@@ -302,6 +302,31 @@ void orc_gi_trace_f64(const orc_scene* s, const orc_light* lights, uint32_t n_li
                       const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                       double* intervals, int threads);
 
+/* ---- screen-space temporal and denoise passes (orc_screen.c) ---- */
+typedef struct {
+    float world_from_clip[16];          /* row-major */
+    float previous_clip_from_world[16];
+    float previous_world_from_clip[16];
+    uint32_t viewport[4];               /* x, y, width, height */
+    uint32_t target[2];                 /* full-resolution image width, height */
+    uint32_t size[2];                   /* receiver grid */
+    uint32_t resolve_scale;
+    uint32_t persistent;
+    float probe_spacing, normal_bias;
+    float history_blend, history_frames;
+    float stability[4];
+    uint32_t source_changes, source_flags, motion_epoch, frame;
+    float dirty_min[4], dirty_max[4];
+} orc_screen_params;
+void orc_screen_extent(const orc_screen_params* P, uint32_t* w, uint32_t* h);
+void orc_screen_temporal(const orc_screen_params* P, const float* current, const float* current_bg, const float* depth,
+                         const uint32_t* gbuffer, const uint32_t* raster, const float* motion, const float* prev,
+                         const float* prev_meta, const float* prev_recv, const float* prev_bg, float* out_irr,
+                         float* out_meta, float* out_recv, float* out_bg);
+void orc_screen_denoise(const orc_screen_params* P, uint32_t pass, const float* in_irr, const float* moments,
+                        const float* in_recv, const float* depth, float* out, float* out_recv);
+float orc_screen_packed_normal(const float* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -603,3 +603,87 @@ def rx_trace(built: BuiltScene, lights: np.ndarray, materials: np.ndarray, grids
                      len(grids), rc.ctypes.data, count, lods.ctypes.data, seq.ctypes.data, tk.ctypes.data,
                      tk.shape[0], frame, ip, sp, st.ctypes.data)
     return ivs, sms, st
+
+
+# ---------------- screen-space temporal and denoise (orc_screen.c) ----------------
+class ScreenParams(C.Structure):
+    """orc_screen_params (oracle/lux_oracle.h) = lux_screen_params (include/lux.h)."""
+    _fields_ = [
+        ("world_from_clip", C.c_float * 16),
+        ("previous_clip_from_world", C.c_float * 16),
+        ("previous_world_from_clip", C.c_float * 16),
+        ("viewport", C.c_uint32 * 4),
+        ("target", C.c_uint32 * 2),
+        ("size", C.c_uint32 * 2),
+        ("resolve_scale", C.c_uint32),
+        ("persistent", C.c_uint32),
+        ("probe_spacing", C.c_float),
+        ("normal_bias", C.c_float),
+        ("history_blend", C.c_float),
+        ("history_frames", C.c_float),
+        ("stability", C.c_float * 4),
+        ("source_changes", C.c_uint32),
+        ("source_flags", C.c_uint32),
+        ("motion_epoch", C.c_uint32),
+        ("frame", C.c_uint32),
+        ("dirty_min", C.c_float * 4),
+        ("dirty_max", C.c_float * 4),
+    ]
+
+
+def _screen_sig():
+    lib = load()
+    vp, u32 = C.c_void_p, C.c_uint32
+    lib.orc_screen_extent.argtypes = [vp, vp, vp]
+    lib.orc_screen_temporal.argtypes = [vp] + [vp] * 14
+    lib.orc_screen_denoise.argtypes = [vp, u32, vp, vp, vp, vp, vp, vp]
+    lib.orc_screen_packed_normal.restype = C.c_float
+    lib.orc_screen_packed_normal.argtypes = [vp]
+    return lib
+
+
+def screen_extent(P: ScreenParams):
+    w, h = C.c_uint32(), C.c_uint32()
+    _screen_sig().orc_screen_extent(C.byref(P), C.byref(w), C.byref(h))
+    return int(w.value), int(h.value)
+
+
+def screen_packed_normal(n) -> float:
+    v = np.ascontiguousarray(n, np.float32)
+    return float(_screen_sig().orc_screen_packed_normal(v.ctypes.data))
+
+
+def _f(a, shape, dtype=np.float32):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype)
+    assert a.size == int(np.prod(shape)), (a.shape, shape)
+    return a
+
+
+def screen_temporal(P: ScreenParams, current, current_background, depth, gbuffer, raster, motion, previous,
+                    previous_metadata, previous_receiver, previous_background):
+    """temporal over the resolve extent: (irradiance, metadata, receiver,
+    background), each (RH, RW, 4) float32.  Full-resolution inputs are
+    (target h, target w[, c]); background inputs and motion may be None."""
+    RW, RH = screen_extent(P)
+    TW, TH = P.target[0], P.target[1]
+    r = (RH, RW, 4)
+    ins = [_f(current, r), _f(current_background, r), _f(depth, (TH, TW)), _f(gbuffer, (TH, TW, 4), np.uint32),
+           _f(raster, (TH, TW, 4), np.uint32), _f(motion, (TH, TW, 2)), _f(previous, r), _f(previous_metadata, r),
+           _f(previous_receiver, r), _f(previous_background, r)]
+    outs = [np.zeros(r, np.float32) for _ in range(4)]
+    _screen_sig().orc_screen_temporal(C.byref(P), *[None if a is None else a.ctypes.data for a in ins],
+                                      *[o.ctypes.data for o in outs])
+    return tuple(outs)
+
+
+def screen_denoise(P: ScreenParams, pass_index: int, irradiance, moments, receiver, depth):
+    """One denoise pass: (irradiance, receiver), each (RH, RW, 4)."""
+    RW, RH = screen_extent(P)
+    r = (RH, RW, 4)
+    ins = [_f(irradiance, r), _f(moments, r), _f(receiver, r), _f(depth, (P.target[1], P.target[0]))]
+    out, out_r = np.zeros(r, np.float32), np.zeros(r, np.float32)
+    _screen_sig().orc_screen_denoise(C.byref(P), pass_index, *[a.ctypes.data for a in ins], out.ctypes.data,
+                                     out_r.ctypes.data)
+    return out, out_r
