@@ -56,6 +56,7 @@ def parse():
     p.add_argument("--batch-train", type=int, default=100,
                    help="online training steps of each batch scene's surrogate")
     p.add_argument("--no-receivers", action="store_true")
+    p.add_argument("--no-screen", action="store_true")
     p.add_argument("--receiver-steps", type=int, default=10)
     p.add_argument("--pmc-workload", action="store_true",
                    help="run only the counter-profiling workload (1 GiB device copy x2, three "
@@ -585,6 +586,83 @@ def receivers_section(lux, scenes, args) -> dict:
     return out
 
 
+def screen_section(lux, args) -> dict:
+    """The screen passes after the resolve at 1920 x 1080 (resolve scale 1:
+    2,073,600 texels): one temporal pass and three ping-ponged denoise
+    passes over a synthetic frame -- two depth regions, two raster
+    identities, a few tilted normals, 80 % supported samples, previous
+    history with a small camera shift (fractional reprojection).  Bytes per
+    texel are the algorithmic reads and writes: temporal 196 (current,
+    background, depth, G-buffer and raster words, previous irradiance,
+    metadata, receiver and background; four float4 written), a denoise pass
+    84 (irradiance, moments, receiver words, depth; two float4 written)."""
+    from bevy_penumbra_amd.lux import DeviceBuffer
+    ctx = lux.Context(0)
+    rng = np.random.default_rng(1)
+    W, H = 1920, 1080
+    P = lux.ScreenParams()
+    for m in ("world_from_clip", "previous_clip_from_world", "previous_world_from_clip"):
+        getattr(P, m)[:] = np.eye(4, dtype=np.float32).ravel()
+    P.previous_clip_from_world[3], P.previous_world_from_clip[3] = 0.0013, -0.0013
+    P.viewport[:] = [0, 0, W, H]
+    P.target[:] = [W, H]
+    P.size[:] = [W // 8, H // 8]
+    P.resolve_scale = 1
+    P.persistent = 1
+    P.probe_spacing, P.normal_bias = 0.25, 0.005
+    P.history_blend, P.history_frames = 0.9, 24.0
+    P.stability[:] = [1.0, 1.0, 0.0, 0.0]
+    RW, RH = P.extent()
+    yy, xx = np.mgrid[0:H, 0:W]
+    depth = np.where(xx + yy < 1500, 0.4, 0.7).astype(np.float32)
+    raster = np.zeros((H, W, 4), np.uint32)
+    raster[..., 0] = np.where(xx < 960, 0x3F800000, 0x40400000)
+    raster[..., 1] = 0x40000000
+    raster[..., 2] = np.where(rng.random((H, W)) < 0.05, 0x80000000 | 19384 | (18384 << 15), 0)
+    gbuffer = np.zeros((H, W, 4), np.uint32)
+    gbuffer[..., 3] = 0x0080_0800
+    f4 = lambda: np.zeros((RH, RW, 4), np.float32)
+    cur, cbg, prev, pmeta, precv, pbg = f4(), f4(), f4(), f4(), f4(), f4()
+    cur[..., :3] = rng.uniform(0, 4, (RH, RW, 3))
+    cur[..., 3] = np.where(rng.random((RH, RW)) < 0.8, 1.25, -1.0)
+    cbg[..., :3], cbg[..., 3] = 0.5, 1.25
+    prev[..., :3], prev[..., 3] = rng.uniform(0, 4, (RH, RW, 3)), depth[:RH, :RW]
+    pmeta[..., :2], pmeta[..., 2], pmeta[..., 3] = 0.5, 8.0, 0.05
+    precv[..., 0] = np.where(xx[:RH, :RW] < 960, 1.0, 3.0)
+    precv[..., 1], precv[..., 2], precv[..., 3] = 2.0, 8_390_656.0, 65_537.0
+    pbg[..., :3], pbg[..., 3] = 0.5, 8.0
+    up = lambda a: DeviceBuffer.from_array(ctx, np.ascontiguousarray(a))
+    d_in = [up(cur), up(cbg), up(depth), up(gbuffer), up(raster), None, up(prev), up(pmeta), up(precv), up(pbg)]
+    d_out = [DeviceBuffer.empty(ctx, (RH, RW, 4), np.float32) for _ in range(4)]
+    ping = [DeviceBuffer.empty(ctx, (RH, RW, 4), np.float32) for _ in range(4)]
+    steps = max(args.receiver_steps, 3)
+
+    def temporal():
+        lux.screen_temporal(ctx, P, *d_in, *d_out)
+
+    def denoise3():
+        src_i, src_r = d_out[0], d_out[2]
+        for k in range(3):
+            dst_i, dst_r = ping[2 * (k % 2)], ping[2 * (k % 2) + 1]
+            lux.screen_denoise(ctx, P, k, src_i, d_out[1], src_r, d_in[2], dst_i, dst_r)
+            src_i, src_r = dst_i, dst_r
+
+    res = {"texels": RW * RH, "resolve_scale": 1, "denoise_passes": 3}
+    for name, fn, per_texel in (("temporal", temporal, 196), ("denoise_3_passes", denoise3, 3 * 84)):
+        fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        res[name] = {"ms": round(ms, 4), "algorithmic_gb_s": round(per_texel * RW * RH / (ms * 1e-3) / 1e9, 1)}
+    lit = d_out[0].numpy()[..., 3]
+    res["temporal_lit_fraction"] = round(float((lit > 0).mean()), 4)
+    ctx.close()
+    return res
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -729,6 +807,8 @@ def main() -> int:
         out["batched_scenes"] = batch_section(lux, scenes, args)
     if not args.no_receivers:
         out["receiver_updates"] = receivers_section(lux, scenes, args)
+    if not args.no_screen:
+        out["screen_filters"] = screen_section(lux, args)
     print(json.dumps(out), flush=True)
     return 0
 

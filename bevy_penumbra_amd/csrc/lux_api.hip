@@ -290,7 +290,8 @@ extern "C" void lux_ctx_destroy(lux_ctx* ctx) {
                           &ctx->gi_samples, &ctx->gi_history, &ctx->gi_hist_state,
                           &ctx->rx_keys, &ctx->rx_flags, &ctx->rx_slot, &ctx->rx_lod, &ctx->rx_ord,
                           &ctx->rx_seq, &ctx->rx_dir, &ctx->rx_tickets, &ctx->rx_dep_val, &ctx->rx_pack, &ctx->rx_traced,
-                          &ctx->rx_sort, &ctx->rx_scan, &ctx->rx_counters, &ctx->rx_cold};
+                          &ctx->rx_sort, &ctx->rx_scan, &ctx->rx_counters, &ctx->rx_cold,
+                          &ctx->screen_cache};
     for (lux_buffer* b : bufs) lux_release(*b);
     for (hipEvent_t e : ctx->ev_phase)
         if (e) (void)hipEventDestroy(e);

@@ -90,6 +90,9 @@ struct lux_ctx {
     // sort and scan scratch, counters
     lux_buffer rx_keys, rx_flags, rx_slot, rx_lod, rx_ord, rx_seq, rx_dir, rx_tickets;
     lux_buffer rx_dep_val, rx_pack, rx_traced, rx_sort, rx_scan, rx_counters, rx_cold;
+    // screen passes: per resolve texel its decoded receiver words, normal and
+    // world position (temporal: 3 float4; denoise: 2 float4)
+    lux_buffer screen_cache;
     uint32_t rx_count = 0, rx_levels = 0, rx_cap = 0, rx_tickets_n = 0;
 
     // trace tuning

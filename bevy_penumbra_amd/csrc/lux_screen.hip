@@ -2,14 +2,15 @@
 // temporal (shaders/cascades.slang:310-622) and one denoise pass
 // (:624-711), one thread per resolve texel in 8 x 8 tiles (one wave).
 //
-// Both are gather-bound image filters: temporal reads its texel's current
-// sample and background, the 3 x 3 neighbourhood (current value, depth,
-// G-buffer normal word and raster identity at each neighbour's
-// full-resolution representative) and up to four previous texels with their
-// metadata, receiver words and background; denoise reads a 5 x 5 footprint
-// without its four far corners.  Neighbouring threads share most taps, so
-// they hit in L2; HBM traffic stays near one read of each image and one
-// write of each output.
+// Both are gather filters whose taps were dominated by decoding (IEEE
+// divides and square roots of the packed normals and the world positions):
+// for temporal a cache pass decodes each resolve texel once (the raster
+// receiver words, normal, depth and world position of its representative
+// pixel) and the 3 x 3 neighbourhood reads those (L2-resident); it then reads
+// up to four previous texels with their metadata, receiver words and
+// background.  Denoise (a 5 x 5 footprint without its four far corners)
+// decodes in its taps: a decode cache there, or straight-line taps with all
+// loads in flight, both measured slower than early rejection.
 //
 // Restatement choices shared with the oracle: row-major matrices with
 // h_i = ((m_i0 x + m_i1 y) + m_i2 z) + m_i3; dot products summed x, y, z;
@@ -83,10 +84,12 @@ __device__ __forceinline__ float4 with_support(float4 m, uint32_t s) {
     m.w = (float)packed;
     return m;
 }
-__device__ __forceinline__ bool recv_compatible(float4 a, float4 b) {
+__device__ __forceinline__ bool recv_same(float4 a, float4 b) {
     return hist_len(a) > 0u && hist_len(b) > 0u && __float_as_uint(a.x) == __float_as_uint(b.x) &&
-           __float_as_uint(a.y) == __float_as_uint(b.y) && dot3(recv_normal(a), recv_normal(b)) > 0.97f;
+           __float_as_uint(a.y) == __float_as_uint(b.y);
 }
+// splitReceiverCompatible = recv_same && dot(normals) > 0.97; the callers
+// decode each packed normal once and test the two parts in that order.
 
 __device__ float3 geometric_normal(uint32_t packed, float3 fallback) {
     fallback = norm3(fallback);
@@ -157,12 +160,37 @@ __device__ __forceinline__ uint64_t full_of(const ScreenArgs& a, int tx, int ty,
 
 __device__ __forceinline__ void store(float4* img, uint64_t i, float4 v) { img[i] = v; }
 
+// Per resolve texel, once: the raster receiver words of its representative
+// pixel, their decoded normal with the depth, and the world position (taps of
+// the 3 x 3 neighbourhood read these instead of decoding again).
+__global__ __launch_bounds__(64) void k_screen_cache_raster(ScreenArgs a, const float* __restrict__ depth,
+                                                            const uint4* __restrict__ gbuffer,
+                                                            const uint4* __restrict__ raster,
+                                                            float4* __restrict__ cache) {
+    const int tx = (int)(blockIdx.x * kTile + threadIdx.x % kTile);
+    const int ty = (int)(blockIdx.y * kTile + threadIdx.x / kTile);
+    if (tx >= (int)a.rw || ty >= (int)a.rh) return;
+    const uint64_t pix = (uint64_t)ty * a.rw + tx;
+    uint32_t fx, fy;
+    const uint64_t full = full_of(a, tx, ty, fx, fy);
+    const float d = depth[full];
+    const float4 m = raster_meta(gbuffer, raster, full);
+    const float3 n = recv_normal(m);
+    const float3 p = d > 0.0f ? world_pos(a.p.world_from_clip, a.p, (float)fx + 0.5f, (float)fy + 0.5f, d)
+                              : make_float3(0.0f, 0.0f, 0.0f);
+    cache[3 * pix] = m;
+    cache[3 * pix + 1] = make_float4(n.x, n.y, n.z, d);
+    cache[3 * pix + 2] = make_float4(p.x, p.y, p.z, 0.0f);
+}
+
+
 __global__ __launch_bounds__(64) void k_screen_temporal(
     ScreenArgs a, const float4* __restrict__ current, const float4* __restrict__ current_bg,
     const float* __restrict__ depth, const uint4* __restrict__ gbuffer, const uint4* __restrict__ raster,
     const float2* __restrict__ motion, const float4* __restrict__ prev, const float4* __restrict__ prev_meta,
-    const float4* __restrict__ prev_recv, const float4* __restrict__ prev_bg, float4* __restrict__ out_irr,
-    float4* __restrict__ out_meta, float4* __restrict__ out_recv, float4* __restrict__ out_bg) {
+    const float4* __restrict__ prev_recv, const float4* __restrict__ prev_bg, const float4* __restrict__ cache,
+    float4* __restrict__ out_irr, float4* __restrict__ out_meta, float4* __restrict__ out_recv,
+    float4* __restrict__ out_bg) {
     const int tx = (int)(blockIdx.x * kTile + threadIdx.x % kTile);
     const int ty = (int)(blockIdx.y * kTile + threadIdx.x / kTile);
     if (tx >= (int)a.rw || ty >= (int)a.rh) return;
@@ -173,8 +201,9 @@ __global__ __launch_bounds__(64) void k_screen_temporal(
     float4 center = current[pix];
     uint32_t fx, fy;
     const uint64_t full = full_of(a, tx, ty, fx, fy);
-    const float4 receiver = raster_meta(gbuffer, raster, full);
-    const float d = depth[full];
+    const float4 receiver = cache[3 * pix];
+    const float4 nd_c = cache[3 * pix + 1];
+    const float d = nd_c.w;
     if (d <= 0.0f || hist_len(receiver) == 0u) {
         store(out_bg, pix, zero);
         store(out_irr, pix, zero);
@@ -197,9 +226,10 @@ __global__ __launch_bounds__(64) void k_screen_temporal(
     if (!cur_sup) center.x = center.y = center.z = 0.0f;
     center.w = d;
     uint32_t out_mask = cur_mask;
-    const float3 gn = recv_normal(receiver);
+    const float3 gn = make_float3(nd_c.x, nd_c.y, nd_c.z);
     const float3 normal = gn;
-    const float3 position = world_pos(P.world_from_clip, P, (float)fx + 0.5f, (float)fy + 0.5f, d);
+    const float4 pos_c = cache[3 * pix + 2];
+    const float3 position = make_float3(pos_c.x, pos_c.y, pos_c.z);
     const float tol = surface_tol(P);
     float3 mean = make_float3(0.0f, 0.0f, 0.0f), square = mean;
     float count = 0.0f;
@@ -211,14 +241,15 @@ __global__ __launch_bounds__(64) void k_screen_temporal(
             if (value.w <= 0.0f) continue;
             const uint32_t nmask = ((uint32_t)floorf(value.w)) & 1u;
             if (cur_mask != 0u && nmask != cur_mask) continue;
-            uint32_t ofx, ofy;
-            const uint64_t of = full_of(a, nx, ny, ofx, ofy);
-            const float od = depth[of];
-            if (od <= 0.0f) continue;
-            const float4 orc = raster_meta(gbuffer, raster, of);
-            if (!recv_compatible(receiver, orc)) continue;
-            const float3 on = recv_normal(orc);
-            const float3 op = world_pos(P.world_from_clip, P, (float)ofx + 0.5f, (float)ofy + 0.5f, od);
+            const uint64_t nt = (uint64_t)ny * RW + nx;
+            const float4 ond = cache[3 * nt + 1];
+            if (ond.w <= 0.0f) continue;
+            const float4 orc = cache[3 * nt];
+            if (!recv_same(receiver, orc)) continue;
+            const float3 on = make_float3(ond.x, ond.y, ond.z);
+            if (!(dot3(gn, on) > 0.97f)) continue;
+            const float4 opc = cache[3 * nt + 2];
+            const float3 op = make_float3(opc.x, opc.y, opc.z);
             if (dot3(gn, on) < 0.95f || plane_dist(sub3(op, position), gn, on) > tol) continue;
             mean.x += value.x; mean.y += value.y; mean.z += value.z;
             square.x += value.x * value.x; square.y += value.y * value.y; square.z += value.z * value.z;
@@ -272,8 +303,10 @@ __global__ __launch_bounds__(64) void k_screen_temporal(
             const uint64_t op = (uint64_t)oy * RW + ox;
             const float4 old = prev[op], meta = prev_meta[op], old_r = prev_recv[op];
             if (meta.z <= 0.0f || old.w <= 0.0f || !(fin(old.x) && fin(old.y) && fin(old.z)) ||
-                dot3(normal, dec_normal(meta.x, meta.y)) < 0.95f || !recv_compatible(receiver, old_r))
+                dot3(normal, dec_normal(meta.x, meta.y)) < 0.95f || !recv_same(receiver, old_r))
                 continue;
+            const float3 old_n = recv_normal(old_r);
+            if (!(dot3(gn, old_n) > 0.97f)) continue;
             uint32_t ofx, ofy;
             full_of(a, ox, oy, ofx, ofy);
             const float3 old_pos = world_pos(P.previous_world_from_clip, P, (float)ofx + 0.5f, (float)ofy + 0.5f, old.w);
@@ -290,11 +323,12 @@ __global__ __launch_bounds__(64) void k_screen_temporal(
                     const uint64_t nb = (uint64_t)ny * RW + nx;
                     const float4 nr = prev_recv[nb];
                     const float nd = prev[nb].w;
-                    if (recv_compatible(old_r, nr) && nd > 0.0f) t += 2.0f * fabsf(nd - old.w);
+                    if (recv_same(old_r, nr) && dot3(old_n, recv_normal(nr)) > 0.97f && nd > 0.0f)
+                        t += 2.0f * fabsf(nd - old.w);
                 }
                 if (fabsf(expected - old.w) > t) continue;
             } else {
-                if (plane_dist(sub3(old_pos, position), gn, recv_normal(old_r)) > tol) continue;
+                if (plane_dist(sub3(old_pos, position), gn, old_n) > tol) continue;
             }
             const float wx = bitx ? frx : 1.0f - frx, wy = bity ? fry : 1.0f - fry;
             const float weight = wx * wy;
@@ -466,11 +500,12 @@ __global__ __launch_bounds__(64) void k_screen_denoise(ScreenArgs a, int step_x,
             const float4 om = moments[op];
             if (om.z <= 0.0f) continue;
             const float4 orr = in_recv[op];
-            if (!recv_compatible(receiver, orr)) continue;
+            if (!recv_same(receiver, orr)) continue;
+            const float3 on = recv_normal(orr);
+            if (!(dot3(gn, on) > 0.97f)) continue;
             const uint32_t omask = support_mask(orr);
             if (cmask != 0u && omask != cmask) continue;
             if (cmask == 0u && omask == 0u) continue;
-            const float3 on = recv_normal(orr);
             uint32_t ofx, ofy;
             full_of(a, ox, oy, ofx, ofy);
             const float3 opos = world_pos(P.world_from_clip, P, (float)ofx + 0.5f, (float)ofy + 0.5f, value.w);
@@ -553,14 +588,19 @@ extern "C" int lux_screen_temporal(lux_ctx* ctx, const lux_screen_params* p, con
         for (const float* i : ins)
             if (o == i) return lux_fail(ctx, LUX_ERR_INVALID, "lux_screen_temporal: outputs must not alias inputs");
     LUX_HIP(ctx, hipSetDevice(ctx->device));
-    hipLaunchKernelGGL(k_screen_temporal, dim3((a.rw + kTile - 1) / kTile, (a.rh + kTile - 1) / kTile), dim3(64), 0,
-                       ctx->stream, a, reinterpret_cast<const float4*>(current),
+    if (int rc = lux_ensure(ctx, ctx->screen_cache, 48ull * a.rw * a.rh)) return rc;
+    float4* cache = static_cast<float4*>(ctx->screen_cache.ptr);
+    const dim3 grid((a.rw + kTile - 1) / kTile, (a.rh + kTile - 1) / kTile);
+    hipLaunchKernelGGL(k_screen_cache_raster, grid, dim3(64), 0, ctx->stream, a, depth,
+                       reinterpret_cast<const uint4*>(gbuffer), reinterpret_cast<const uint4*>(raster), cache);
+    hipLaunchKernelGGL(k_screen_temporal, grid, dim3(64), 0, ctx->stream, a, reinterpret_cast<const float4*>(current),
                        reinterpret_cast<const float4*>(current_background), depth,
                        reinterpret_cast<const uint4*>(gbuffer), reinterpret_cast<const uint4*>(raster),
                        reinterpret_cast<const float2*>(motion), reinterpret_cast<const float4*>(previous),
                        reinterpret_cast<const float4*>(previous_metadata),
                        reinterpret_cast<const float4*>(previous_receiver),
-                       reinterpret_cast<const float4*>(previous_background), reinterpret_cast<float4*>(irradiance),
+                       reinterpret_cast<const float4*>(previous_background), cache,
+                       reinterpret_cast<float4*>(irradiance),
                        reinterpret_cast<float4*>(metadata), reinterpret_cast<float4*>(receiver),
                        reinterpret_cast<float4*>(background));
     LUX_HIP(ctx, hipGetLastError());
@@ -589,10 +629,11 @@ extern "C" int lux_screen_denoise(lux_ctx* ctx, const lux_screen_params* p, uint
         step[q] = pass == 0 ? 1 : (int)(b * (1u << pass));
     }
     LUX_HIP(ctx, hipSetDevice(ctx->device));
-    hipLaunchKernelGGL(k_screen_denoise, dim3((a.rw + kTile - 1) / kTile, (a.rh + kTile - 1) / kTile), dim3(64), 0,
-                       ctx->stream, a, step[0], step[1], reinterpret_cast<const float4*>(irradiance),
-                       reinterpret_cast<const float4*>(moments), reinterpret_cast<const float4*>(receiver), depth,
-                       reinterpret_cast<float4*>(out), reinterpret_cast<float4*>(out_receiver));
+    const dim3 grid((a.rw + kTile - 1) / kTile, (a.rh + kTile - 1) / kTile);
+    hipLaunchKernelGGL(k_screen_denoise, grid, dim3(64), 0, ctx->stream, a, step[0], step[1],
+                       reinterpret_cast<const float4*>(irradiance), reinterpret_cast<const float4*>(moments),
+                       reinterpret_cast<const float4*>(receiver), depth, reinterpret_cast<float4*>(out),
+                       reinterpret_cast<float4*>(out_receiver));
     LUX_HIP(ctx, hipGetLastError());
     return LUX_OK;
 }
