@@ -57,6 +57,9 @@ def parse():
                    help="online training steps of each batch scene's surrogate")
     p.add_argument("--no-receivers", action="store_true")
     p.add_argument("--no-screen", action="store_true")
+    p.add_argument("--blas", choices=("sah", "morton"), default="sah",
+                   help="BLAS builder of the traversal scene: the reference's binned SAH on the host "
+                        "(LUX_BVH_SAH) or Morton-30 on the GPU")
     p.add_argument("--receiver-steps", type=int, default=10)
     p.add_argument("--pmc-workload", action="store_true",
                    help="run only the counter-profiling workload (1 GiB device copy x2, three "
@@ -69,7 +72,8 @@ TRAFFIC_PROFILE = ROOT / "profiles" / "pmc_traffic_colonnade_trace_and_gi.md"
 
 def measured_traffic(kernel: str, cfg: dict):
     """HBM-side bytes per launch of `kernel` from the committed counter profile,
-    when it was taken with this traversal configuration; else None."""
+    when it was taken with this traversal configuration and BLAS builder (the
+    profile's `config` line; `blas` absent means Morton-30); else None."""
     if not TRAFFIC_PROFILE.exists():
         return None
     lines = TRAFFIC_PROFILE.read_text().splitlines()
@@ -77,7 +81,7 @@ def measured_traffic(kernel: str, cfg: dict):
     if conf is None:
         return None
     want = dict(kv.split("=") for kv in conf.split()[1:])
-    if any(int(want.get(k, -1)) != int(v) for k, v in cfg.items()):
+    if any(str(want.get(k, "morton" if k == "blas" else -1)) != str(v) for k, v in cfg.items()):
         return None
     for l in lines:
         parts = l.split()
@@ -87,7 +91,11 @@ def measured_traffic(kernel: str, cfg: dict):
     return None
 
 
-def pmc_workload(lux, scenes) -> int:
+def blas_mode(lux, args) -> int:
+    return lux.LUX_BVH_SAH if args.blas == "sah" else lux.LUX_BVH_MORTON30
+
+
+def pmc_workload(lux, scenes, args) -> int:
     from bevy_penumbra_amd.lux import DeviceBuffer
     ctx = lux.Context(0)
     a = DeviceBuffer.empty(ctx, (1 << 28,), np.float32)
@@ -98,7 +106,7 @@ def pmc_workload(lux, scenes) -> int:
     sc, _ = scenes.colonnade()
     rays_np = scenes.microbench_rays(sc.bounds)
     ctx.upload_scene(sc)
-    ctx.build_bvh()
+    ctx.build_bvh(blas_mode(lux, args))
     n = rays_np.shape[0]
     rays = DeviceBuffer.from_array(ctx, rays_np)
     hits = DeviceBuffer.empty(ctx, (n, 4), np.uint32)
@@ -331,7 +339,7 @@ def batch_section(lux, scenes, args) -> dict:
         sc, _ = scenes.clutter_room(s["seed"], s["triangles"], s["size"])
         c = lux.Context(0)
         c.upload_scene(sc)
-        c.build_bvh()
+        c.build_bvh(blas_mode(lux, args))
         sp = s["spacing"]
         c.gi_configure(lux.gi_params((0.5 * sp, 0.5 * sp, 0.5 * sp), sp, s["dims"], s["levels"], base_phi=4,
                                      base_y=4, interval_length=sp, emissive_samples=4, terminal_tail=1))
@@ -480,7 +488,7 @@ def receivers_section(lux, scenes, args) -> dict:
     for k in range(3):
         c = lux.Context(0)
         c.upload_scene(scene)
-        c.build_bvh()
+        c.build_bvh(blas_mode(lux, args))
         c.gi_configure(grids[k])
         ctxs.append(c)
     # the clutter fills the volume (every interior eye sits inside a box): the
@@ -672,13 +680,13 @@ def main() -> int:
     from bevy_penumbra_amd.lux import DeviceBuffer
 
     if args.pmc_workload:
-        return pmc_workload(lux, scenes)
+        return pmc_workload(lux, scenes, args)
     scene, _ = scenes.colonnade()
     rays_np = scenes.microbench_rays(scene.bounds, n=args.rays)
     ctx = lux.Context(0)
     ctx.upload_scene(scene)
     t0 = time.perf_counter()
-    ctx.build_bvh()
+    ctx.build_bvh(blas_mode(lux, args))
     ctx.synchronize()
     build_ms = (time.perf_counter() - t0) * 1e3
     n = rays_np.shape[0]
@@ -779,7 +787,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": "traversal microbench: 262,144-triangle colonnade, 4,194,304 rays "
-                        "(half closest-hit, half any-hit), Morton-30 LBVH",
+                        "(half closest-hit, half any-hit), "
+                        + ("binned-SAH BLAS (host), Morton-30 TLAS" if args.blas == "sah" else "Morton-30 LBVH"),
             "rays": n,
             "triangles": scene.triangle_count,
             "bvh_build_ms": round(build_ms, 3),
@@ -795,7 +804,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic("k_trace_regroup<%d>" % trace_cfg["lds_stack_depth"], trace_cfg),
+            "traffic": measured_traffic("k_trace_regroup<%d>" % trace_cfg["lds_stack_depth"],
+                                        dict(trace_cfg, blas=args.blas)),
         },
         "cpu_baseline": cpu,
         "gi_update": gi,

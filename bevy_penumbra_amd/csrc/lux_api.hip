@@ -466,7 +466,167 @@ extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_wo
     return LUX_OK;
 }
 
-static int build_blas(lux_ctx* ctx) {
+namespace {
+// Binned SAH over one mesh's triangle boxes on the host: build_tree
+// (src/scene.rs:2398-2554) -- 12 bins on the centroid span per axis with an
+// f64 bin index, bin costs area(f64) x count, the cheapest split with both
+// sides non-empty (axis order x, y, z, first strictly cheaper wins), Hoare
+// partition by bin, leaves of at most 4 and depth 48, and for coincident
+// centroids a median split on the widest centroid axis.  The reference leaves
+// that median's element order to select_nth_unstable; here the elements are
+// fully sorted by (centre, index).  Nodes are written depth-first as the
+// reference appends them (children pairs after their parent).
+struct SahBuilder {
+    const float* lo;  // 4 floats per element
+    const float* hi;
+    uint32_t* perm;
+    std::vector<lux_bvh_node> nodes;
+
+    float centre(uint32_t e, int a) const { return lo[4 * e + a] * 0.5f + hi[4 * e + a] * 0.5f; }
+
+    static double area(const float* mn, const float* mx) {
+        const double sx = (double)mx[0] - (double)mn[0], sy = (double)mx[1] - (double)mn[1];
+        const double sz = (double)mx[2] - (double)mn[2];
+        return 2.0 * (sx * sy + sy * sz + sz * sx);
+    }
+
+    struct Bin {
+        float mn[3], mx[3];
+        size_t count;
+        void clear() {
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = INFINITY;
+                mx[a] = -INFINITY;
+            }
+            count = 0;
+        }
+        void add(const Bin& o) {
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = std::fmin(mn[a], o.mn[a]);
+                mx[a] = std::fmax(mx[a], o.mx[a]);
+            }
+            count += o.count;
+        }
+        double cost() const { return count == 0 ? 0.0 : area(mn, mx) * (double)count; }
+    };
+
+    static size_t bin_of(float c, float cmin, double span) {
+        const size_t b = (size_t)(((double)c - (double)cmin) / span * 12.0);
+        return b < 11 ? b : 11;
+    }
+
+    void build(uint32_t index, uint32_t first, uint32_t count, uint32_t depth) {
+        uint32_t* el = perm + first;
+        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = 0; i < count; ++i)
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = std::fmin(mn[a], lo[4 * el[i] + a]);
+                mx[a] = std::fmax(mx[a], hi[4 * el[i] + a]);
+            }
+        lux_bvh_node& nd = nodes[index];
+        std::memcpy(nd.mn, mn, 12);
+        std::memcpy(nd.mx, mx, 12);
+        nd.first = first;
+        nd.count = count;
+        if (count <= 4 || depth == 48) return;
+        float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = 0; i < count; ++i)
+            for (int a = 0; a < 3; ++a) {
+                const float c = centre(el[i], a);
+                cmin[a] = std::fmin(cmin[a], c);
+                cmax[a] = std::fmax(cmax[a], c);
+            }
+        double best = INFINITY, best_span = 0.0;
+        int best_axis = -1;
+        size_t best_bin = 0;
+        for (int axis = 0; axis < 3; ++axis) {
+            const double span = (double)cmax[axis] - (double)cmin[axis];
+            if (span <= 0.0) continue;
+            Bin bins[12], right[12], sum, left;
+            for (Bin& b : bins) b.clear();
+            for (uint32_t i = 0; i < count; ++i) {
+                Bin one;
+                for (int a = 0; a < 3; ++a) {
+                    one.mn[a] = lo[4 * el[i] + a];
+                    one.mx[a] = hi[4 * el[i] + a];
+                }
+                one.count = 1;
+                bins[bin_of(centre(el[i], axis), cmin[axis], span)].add(one);
+            }
+            sum.clear();
+            for (int b = 11; b >= 0; --b) {
+                sum.add(bins[b]);
+                right[b] = sum;
+            }
+            left.clear();
+            for (size_t b = 0; b < 11; ++b) {
+                left.add(bins[b]);
+                const double cost = left.cost() + right[b + 1].cost();
+                if (left.count > 0 && right[b + 1].count > 0 && cost < best) {
+                    best = cost;
+                    best_axis = axis;
+                    best_bin = b;
+                    best_span = span;
+                }
+            }
+        }
+        uint32_t mid;
+        if (best_axis >= 0) {
+            uint32_t l = 0, r = count;
+            while (l < r) {
+                if (bin_of(centre(el[l], best_axis), cmin[best_axis], best_span) <= best_bin) {
+                    ++l;
+                } else {
+                    --r;
+                    std::swap(el[l], el[r]);
+                }
+            }
+            mid = l;
+        } else {
+            const float sx = cmax[0] - cmin[0], sy = cmax[1] - cmin[1], sz = cmax[2] - cmin[2];
+            const int axis = (sx >= sy && sx >= sz) ? 0 : (sy >= sz ? 1 : 2);
+            std::sort(el, el + count, [&](uint32_t a, uint32_t b) {
+                const float ca = centre(a, axis), cb = centre(b, axis);
+                return ca < cb || (!(cb < ca) && a < b);
+            });
+            mid = count / 2;
+        }
+        const uint32_t children = (uint32_t)nodes.size();
+        nodes.resize(nodes.size() + 2);
+        nodes[index].first = children;
+        nodes[index].count = 0;
+        build(children, first, mid, depth + 1);
+        build(children + 1, first + mid, count - mid, depth + 1);
+    }
+};
+
+// One mesh's SAH BLAS: nodes with absolute child (node_base) and triangle
+// (tri_base) indices, and the triangle order.
+void sah_mesh(const float* pos, uint32_t n, uint32_t node_base, uint32_t tri_base, std::vector<lux_bvh_node>& nodes,
+              std::vector<uint32_t>& perm) {
+    std::vector<float> lo(4ull * n), hi(4ull * n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* p = pos + 9ull * i;
+        for (int a = 0; a < 3; ++a) {
+            // Triangle::bounds: a.min(b).min(c) as compare-selects
+            const float m01 = p[3 + a] < p[a] ? p[3 + a] : p[a];
+            lo[4 * i + a] = p[6 + a] < m01 ? p[6 + a] : m01;
+            const float x01 = p[3 + a] > p[a] ? p[3 + a] : p[a];
+            hi[4 * i + a] = p[6 + a] > x01 ? p[6 + a] : x01;
+        }
+        lo[4 * i + 3] = hi[4 * i + 3] = 0.0f;
+    }
+    perm.resize(n);
+    for (uint32_t i = 0; i < n; ++i) perm[i] = i;
+    SahBuilder b{lo.data(), hi.data(), perm.data(), {}};
+    b.nodes.resize(1);
+    b.build(0, 0, n, 0);
+    for (lux_bvh_node& nd : b.nodes) nd.first += nd.count == 0 ? node_base : tri_base;
+    nodes.swap(b.nodes);
+}
+}  // namespace
+
+static int build_blas(lux_ctx* ctx, bool sah) {
     hipStream_t st = ctx->stream;
     uint32_t nt = ctx->n_tri;
     if (int rc = lux_ensure(ctx, ctx->tris, 48ull * nt)) return rc;
@@ -493,10 +653,23 @@ static int build_blas(lux_ctx* ctx) {
         uint32_t n = ctx->mesh_tri_count[m], tb = ctx->mesh_tri_base[m], bb = ctx->mesh_blas_base[m];
         const float* pos = static_cast<const float*>(ctx->raw_pos.ptr) + 9ull * tb;
         const uint32_t* mat = static_cast<const uint32_t*>(ctx->raw_mat.ptr) + tb;
-        hipLaunchKernelGGL(k_tri_boxes, dim3(nblk(n, 256)), dim3(256), 0, st, pos, n, lo, hi);
         float4* nodes = static_cast<float4*>(ctx->blas.ptr) + 2ull * bb;
-        rc = lux_lbvh_build_device(ctx, lo, hi, n, 4u, nodes, perm, bb, tb);
-        if (rc) break;
+        if (sah) {
+            // host build from the uploaded positions; unused slots of the
+            // mesh's 2n - 1 region stay zero
+            std::vector<lux_bvh_node> hn;
+            std::vector<uint32_t> hp;
+            sah_mesh(ctx->host_pos[m].data(), n, bb, tb, hn, hp);
+            const uint32_t region = n > 1 ? 2 * n - 1 : 1;
+            LUX_HIP(ctx, hipStreamSynchronize(st));
+            LUX_HIP(ctx, hipMemset(nodes, 0, 32ull * region));
+            LUX_HIP(ctx, hipMemcpy(nodes, hn.data(), 32ull * hn.size(), hipMemcpyHostToDevice));
+            LUX_HIP(ctx, hipMemcpy(perm, hp.data(), 4ull * n, hipMemcpyHostToDevice));
+        } else {
+            hipLaunchKernelGGL(k_tri_boxes, dim3(nblk(n, 256)), dim3(256), 0, st, pos, n, lo, hi);
+            rc = lux_lbvh_build_device(ctx, lo, hi, n, 4u, nodes, perm, bb, tb);
+            if (rc) break;
+        }
         hipLaunchKernelGGL(k_tri_gather, dim3(nblk(n, 256)), dim3(256), 0, st, pos, mat, perm, n,
                            static_cast<float4*>(ctx->tris.ptr) + 3ull * tb,
                            static_cast<uint32_t*>(ctx->tri_order.ptr) + tb, tb);
@@ -577,15 +750,20 @@ static int refit_tlas(lux_ctx* ctx) {
 
 extern "C" int lux_bvh_build(lux_ctx* ctx, uint32_t mode) {
     if (!ctx) return LUX_ERR_INVALID;
-    if ((mode & ~(LUX_BVH_FORCE_BLAS | LUX_BVH_REFIT)) != LUX_BVH_MORTON30)
+    if ((mode & ~(LUX_BVH_FORCE_BLAS | LUX_BVH_REFIT | LUX_BVH_SAH)) != LUX_BVH_MORTON30)
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_bvh_build: unknown mode");
+    // a change of BLAS builder rebuilds the BLASes
+    if (ctx->blas_built && ctx->blas_sah != ((mode & LUX_BVH_SAH) != 0)) ctx->blas_built = false;
     if (mode & LUX_BVH_FORCE_BLAS) ctx->blas_built = false;
     if (ctx->n_meshes == 0 && ctx->n_inst == 0)
         return lux_fail(ctx, LUX_ERR_STATE, "lux_bvh_build: no scene uploaded");
     if (ctx->prebuilt) return LUX_OK;  // installed built: nothing to build
     LUX_HIP(ctx, hipSetDevice(ctx->device));
     if (!ctx->blas_built) {
-        if (int rc = build_blas(ctx)) return rc;
+        if ((mode & LUX_BVH_SAH) && ctx->host_pos.size() != ctx->n_meshes)
+            return lux_fail(ctx, LUX_ERR_STATE, "lux_bvh_build: SAH needs the scene's host positions");
+        if (int rc = build_blas(ctx, (mode & LUX_BVH_SAH) != 0)) return rc;
+        ctx->blas_sah = (mode & LUX_BVH_SAH) != 0;
         ctx->tlas_built = false;
         ctx->tlas_topology = false;
     }

@@ -76,6 +76,7 @@ struct lux_ctx {
     hipEvent_t ev_phase[5] = {};
     // temporal history of raw intervals: on when a blend is positive; sample
     // counts, two float4 per interval, state words (valid, dirty boxes)
+    bool blas_sah = false;   // the built BLASes came from the host SAH build
     bool gi_hist_on = false;
     bool gi_hist_valid = false;  // a history pass ran since the last reset
     lux_gi_history_params gi_hist{};

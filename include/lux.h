@@ -156,6 +156,13 @@ int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_world,
  * the new instance bounds (the reference's per-frame TLAS refit,
  * src/scene.rs:1040-1058,2033-2047); otherwise build as usual. */
 #define LUX_BVH_REFIT 0x200u
+/* or-ed into the mode: build each BLAS with the reference's binned SAH
+ * (build_tree, src/scene.rs:2398-2554) on the host from the uploaded
+ * positions -- the reference's own tree, fewer node visits than Morton-30,
+ * about a second per million triangles -- instead of Morton-30 on the GPU;
+ * the TLAS stays Morton-30 on the GPU.  A change of builder rebuilds the
+ * BLASes. */
+#define LUX_BVH_SAH 0x1u
 /* Builds what is stale: the BLASes after lux_scene_upload_soa, the TLAS after
  * lux_scene_set_transforms (Morton-30 over instance boxes; no host wait). */
 int lux_bvh_build(lux_ctx* ctx, uint32_t mode);

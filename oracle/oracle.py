@@ -99,7 +99,10 @@ class BuiltScene:
         return ts[:k], tris[:k], insts[:k]
 
 
-def build_scene(scene, sah: bool = False, leaf_max: int = 4) -> BuiltScene:
+def build_scene(scene, sah: int = 0, leaf_max: int = 4) -> BuiltScene:
+    """sah 0: LBVH BLASes and TLAS; 1: binned-SAH BLASes (packed) and TLAS;
+    2: binned-SAH BLASes in LBVH-sized regions with an LBVH TLAS (the
+    library's LUX_BVH_SAH layout)."""
     lib = load()
     nm = len(scene.meshes)
     pos = [np.ascontiguousarray(m, np.float32) for m in scene.meshes]
@@ -122,7 +125,7 @@ def build_scene(scene, sah: bool = False, leaf_max: int = 4) -> BuiltScene:
     fl = np.ascontiguousarray(scene.inst_flags, np.uint32)
     lib.orc_scene_build(pos_ptrs, mat_ptrs, counts.ctypes.data, nm, inv.ctypes.data,
                         fwd.ctypes.data, im.ctypes.data, fl.ctypes.data, ni, leaf_max,
-                        1 if sah else 0, tri.ctypes.data, blas.ctypes.data, tlas.ctypes.data,
+                        int(sah), tri.ctypes.data, blas.ctypes.data, tlas.ctypes.data,
                         inst.ctypes.data, sizes.ctypes.data)
     return BuiltScene(blas[: sizes[1]].copy(), tlas[: sizes[2]].copy(), tri[: sizes[0]].copy(),
                       inst[:ni].copy())

@@ -510,7 +510,8 @@ int orc_scene_build(const float* const* mesh_pos, const uint32_t* const* mesh_ma
         }
         free(lo); free(hi); free(codes); free(perm);
         tri_base += n;
-        blas_base += sah ? cnt : (n > 1 ? 2 * n - 1 : 1);
+        /* sah 1: BLASes packed; sah 2: each in its LBVH-sized region (2n - 1) */
+        blas_base += sah == 1 ? cnt : (n > 1 ? 2 * n - 1 : 1);
     }
     mesh_tri_base[n_meshes] = tri_base;
     mesh_blas_base[n_meshes] = blas_base;
@@ -531,9 +532,9 @@ int orc_scene_build(const float* const* mesh_pos, const uint32_t* const* mesh_ma
         }
         uint32_t* codes = (uint32_t*)malloc(sizeof(uint32_t) * n_inst);
         uint32_t* perm = (uint32_t*)malloc(sizeof(uint32_t) * n_inst);
-        n_tlas = sah ? orc_sah_build(lo, hi, n_inst, perm, tlas)
+        n_tlas = sah == 1 ? orc_sah_build(lo, hi, n_inst, perm, tlas)
                      : orc_lbvh_build(lo, hi, n_inst, leaf_max, codes, perm, tlas);
-        if (!sah) n_tlas = n_inst > 1 ? 2 * n_inst - 1 : 1;
+        if (sah != 1) n_tlas = n_inst > 1 ? 2 * n_inst - 1 : 1;
         for (uint32_t k = 0; k < n_inst; ++k) {
             uint32_t src = perm[k];
             orc_instance* in = &inst[k];

@@ -124,6 +124,26 @@ def test_gpu_traverses_sah_built_scene(lux_ctx, colonnade):
     assert np.array_equal(trace_gpu(lux_ctx, rays), sah.trace(rays, threads=16))
 
 
+def test_gpu_sah_blas_build_matches_oracle(lux_ctx, colonnade, fixture):
+    """LUX_BVH_SAH: the reference's binned-SAH BLASes built on the host,
+    Morton-30 TLAS; node arrays, triangle order and traversal equal the
+    oracle's SAH-BLAS scene; switching back rebuilds the LBVH BLASes."""
+    for sc in (colonnade[0], fixture[0]):
+        lux_ctx.upload_scene(sc)
+        lux_ctx.build_bvh(lux.LUX_BVH_SAH)
+        built = O.build_scene(sc, sah=2)
+        assert_same_scene(lux_ctx, built)
+        if len(sc.meshes) == 1:                           # one mesh: the order is the permutation itself
+            assert np.array_equal(lux_ctx.download_bvh()["tri_order"], built.tri[:, 3].view(np.uint32))
+        rays = scenes.microbench_rays(sc.bounds, n=200_000, seed=5)
+        assert np.array_equal(trace_gpu(lux_ctx, rays), built.trace(rays, threads=16))
+        lux_ctx.build_bvh()
+        assert_same_scene(lux_ctx, O.build_scene(sc))
+    # the SAH tree is the cheaper one (the reference's SAH cost)
+    sc = colonnade[0]
+    assert O.tree_cost(O.build_scene(sc, sah=2).blas) < O.tree_cost(O.build_scene(sc).blas)
+
+
 def test_gpu_trace_colonnade_1m_bit_exact(lux_ctx, colonnade):
     sc, built = colonnade
     lux_ctx.upload_scene(sc)
