@@ -21,6 +21,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <cmath>
 #include "lux_gi_device.h"
 
 namespace {
@@ -368,6 +369,7 @@ struct lux_batch {
     std::vector<lux_ctx*> scenes;
     std::vector<lux_surrogate*> sur;
     std::vector<uint32_t> sur_level;
+    std::vector<float> sur_bound;  // per scene: the fallback's error bound (+inf: none)
     std::vector<BatchScene> host_table;
     std::vector<uint64_t> item_base, interval_base, probe_base, merge_base;
     uint32_t nee_max = 1, max_levels = 1;
@@ -559,6 +561,8 @@ int enqueue(lux_batch* b) {
             return bfail(b, rc, lux_last_error(c));
         hipLaunchKernelGGL(k_b_sur_apply, dim3(qb), dim3(kWg), 0, st, Y, q,
                            static_cast<float4*>(c->gi_merged.ptr) + c->gi_offset[S]);
+        if (b->sur_bound[s] < INFINITY)
+            if (int rc = lux_sur_fallback_on(c, st, S, Y, b->sur_bound[s])) return bfail(b, rc, lux_last_error(c));
     }
     for (int L = (int)b->max_levels - 1; L >= 0; --L) {
         uint64_t cnt = b->merge_base[(size_t)L * (n + 1) + n];
@@ -584,6 +588,7 @@ extern "C" int lux_batch_create(lux_ctx* const* scenes, uint32_t n, lux_batch** 
     b->scenes.assign(scenes, scenes + n);
     b->sur.assign(n, nullptr);
     b->sur_level.assign(n, 0);
+    b->sur_bound.assign(n, INFINITY);
     if (hipSetDevice(b->device) != hipSuccess ||
         hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
@@ -629,8 +634,22 @@ extern "C" int lux_batch_set_surrogate(lux_batch* b, uint32_t scene, lux_surroga
             return bfail(b, LUX_ERR_INVALID, "lux_batch_set_surrogate: level must be in 1..levels-1");
     }
     b->sur[scene] = s;
+    b->sur_bound[scene] = INFINITY;  // a new network or level starts without fallback
     b->sur_level[scene] = s ? level : 0u;
     return prepare(b);
+}
+
+extern "C" int lux_batch_set_surrogate_bound(lux_batch* b, uint32_t scene, float error_bound) {
+    if (!b) return LUX_ERR_INVALID;
+    if (scene >= b->scenes.size()) return bfail(b, LUX_ERR_INVALID, "lux_batch_set_surrogate_bound: scene index");
+    if (!b->sur[scene])
+        return bfail(b, LUX_ERR_STATE, "lux_batch_set_surrogate_bound: the scene has no surrogate");
+    if (std::isnan(error_bound)) return bfail(b, LUX_ERR_INVALID, "lux_batch_set_surrogate_bound: NaN bound");
+    if (error_bound < INFINITY)
+        if (int rc = lux_sur_fallback_prepare(b->scenes[scene], b->sur_level[scene]))
+            return bfail(b, rc, lux_last_error(b->scenes[scene]));
+    b->sur_bound[scene] = error_bound;
+    return LUX_OK;
 }
 
 extern "C" int lux_batch_refresh(lux_batch* b) {

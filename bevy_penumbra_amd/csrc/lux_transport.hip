@@ -1452,6 +1452,40 @@ extern "C" int lux_gi_surrogate_samples(lux_ctx* ctx, uint32_t level, uint32_t n
     return LUX_OK;
 }
 
+namespace {
+__global__ void k_sur_count_zero(uint32_t* c) {
+    if (threadIdx.x == 0) *c = 0u;
+}
+}  // namespace
+
+// Buffers of the traced per-probe fallback of level `level` (list and errors).
+int lux_sur_fallback_prepare(lux_ctx* ctx, uint32_t level) {
+    const uint64_t probes = level_count(ctx, level) / level_directions(ctx->gi, level);
+    if (int rc = lux_ensure(ctx, ctx->gi_sur_err, 4ull * probes)) return rc;
+    return lux_ensure(ctx, ctx->gi_sur_list, 4ull * (probes + 1));
+}
+
+// The traced per-probe fallback on a stream (no allocation, no host wait, so
+// a captured graph may hold it): per-probe predicted RMSE from the network
+// outputs Y of the level's queries, then the probes above the bound traced as
+// the level's unbounded terminal intervals and written merged (the fused form
+// of lux_gi_update_surrogate's fallback; the same bits).
+int lux_sur_fallback_on(lux_ctx* ctx, hipStream_t st, uint32_t level, const float* Y, float bound) {
+    const uint64_t count = level_count(ctx, level);
+    const uint32_t D = level_directions(ctx->gi, level);
+    const uint64_t probes = count / D;
+    uint32_t* list = static_cast<uint32_t*>(ctx->gi_sur_list.ptr);
+    uint32_t* fcount = list + probes;
+    hipLaunchKernelGGL(k_sur_count_zero, dim3(1), dim3(64), 0, st, fcount);
+    hipLaunchKernelGGL(k_sur_probe_error, dim3((uint32_t)probes), dim3(64), 0, st, Y, D, bound,
+                       static_cast<float*>(ctx->gi_sur_err.ptr), list, fcount);
+    GiArgs a = gi_args(ctx);
+    hipLaunchKernelGGL(k_sur_fallback, dim3((uint32_t)((count + kWg - 1) / kWg)), dim3(kWg), 0, st, a, level, list,
+                       static_cast<const uint32_t*>(fcount));
+    LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
 extern "C" int lux_gi_update_surrogate(lux_ctx* ctx, lux_surrogate* s, uint32_t level, float error_bound,
                                        lux_gi_surrogate_stats* stats) {
     if (!ctx || !s) return LUX_ERR_INVALID;

@@ -456,6 +456,7 @@ def load_library() -> C.CDLL:
         "lux_batch_destroy": (None, [vp]),
         "lux_batch_last_error": (C.c_char_p, [vp]),
         "lux_batch_set_surrogate": (i32, [vp, u32, vp, u32]),
+        "lux_batch_set_surrogate_bound": (i32, [vp, u32, C.c_float]),
         "lux_batch_refresh": (i32, [vp]),
         "lux_batch_set_frame": (i32, [vp, u32, u32]),
         "lux_batch_update": (i32, [vp, vp]),
@@ -994,6 +995,10 @@ class Batch:
         else:
             self.surrogates[scene] = (surrogate, level)
 
+    def set_surrogate_bound(self, scene: int, error_bound: float):
+        """lux_batch_set_surrogate_bound: the scene's traced per-probe fallback."""
+        self._check(self.lib.lux_batch_set_surrogate_bound(self.h, scene, error_bound),
+                    "lux_batch_set_surrogate_bound")
     def refresh(self):
         self._check(self.lib.lux_batch_refresh(self.h), "lux_batch_refresh")
 

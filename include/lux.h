@@ -585,6 +585,12 @@ const char* lux_batch_last_error(const lux_batch* b);
 /* Predict `level` of scene `scene` with surrogate s (of that scene's context;
  * NULL: trace every level), as lux_gi_update_surrogate with no fallback. */
 int lux_batch_set_surrogate(lux_batch* b, uint32_t scene, lux_surrogate* s, uint32_t level);
+/* The traced per-probe fallback of that scene's surrogate level, as
+ * lux_gi_update_surrogate(..., error_bound): probes whose predicted RMSE is
+ * above the bound (or NaN) are traced inside the batch launch (and a
+ * captured graph).  +inf (the default, and after lux_batch_set_surrogate):
+ * none.  Needs a surrogate on the scene. */
+int lux_batch_set_surrogate_bound(lux_batch* b, uint32_t scene, float error_bound);
 int lux_batch_refresh(lux_batch* b);
 /* Frame index and seed of every scene (kept in device memory: a captured
  * graph picks them up); the member contexts' frame and seed follow. */

@@ -5,7 +5,9 @@ rays per interval and NEE sample counts run as one batch: every scene's
 intervals, merged radiance, packed probes and query counters must equal, bit
 for bit, what the scene's own lux_gi_update writes (which the transport tests
 hold to the oracle).  A scene with a surrogate must equal its own
-lux_gi_update_surrogate with no fallback.  The captured graph must give the
+lux_gi_update_surrogate with no fallback, and with a finite error bound (its
+traced per-probe fallback inside the batch launch, eager and replayed) its
+own update with that bound.  The captured graph must give the
 eager result on every replay, with eager work between replays, and pick up a
 new frame set after capture.
 """
@@ -189,6 +191,63 @@ def test_gpu_batch_surrogate_scene_equals_its_own_update(rooms):
     b.close()
     for s, _ in sur.values():
         s.close()
+
+
+def test_gpu_batch_surrogate_fallback_equals_its_own_update(rooms):
+    """A finite error bound: scene 4's probes above it are traced inside the
+    batch launch (eager and graph replay), equal to lux_gi_update_surrogate
+    with the same bound; a bound of 0 traces every probe of the level."""
+    i, level = 4, 1
+    c = rooms[i]
+    tri, size, d, levels, R, ns = SPECS[i]
+    s = lux.Surrogate(c, lux.SurrogateParams.default((0, 0, 0), (size, size, size), seed=7))
+    refs = per_scene(rooms, 4, 4)
+    # a few training steps make the predicted errors differ between probes
+    samples = lux.DeviceBuffer.empty(c, (4096, 12), np.float32)
+    for k in range(3):
+        c.gi_surrogate_samples(level, 4096, k + 1, lux.LUX_SAMPLES_TRACE, samples)
+        s.train_step(samples, 4096, want_loss=False)
+    c.synchronize()
+    st = c.gi_update_surrogate(s, level, 0.0)
+    assert st.fallback_probes == st.probes                          # bound 0: every probe traced
+    lo, hi = 0.0, 1e6                                               # bisect to a partial fallback
+    chosen = None
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        nf = c.gi_update_surrogate(s, level, mid).fallback_probes
+        if 0 < nf < st.probes:
+            chosen = mid
+            break
+        if nf == 0:
+            hi = mid
+        else:
+            lo = mid
+    assert chosen is not None
+    offs = level_offsets(c)
+    upto = dict(intervals=offs[level], merged=offs[level + 1])
+    b = lux.Batch(rooms, surrogates={i: (s, level)})
+    for bound in (0.0, chosen):
+        c.gi_update_surrogate(s, level, bound, stats=False)
+        c.synchronize()
+        ref = state(c)
+        for k in range(len(rooms)):
+            poison(rooms[k])
+        b.set_surrogate_bound(i, bound)
+        b.set_frame(4, 4)
+        b.update()
+        got = batch_states(rooms, b)
+        assert_same(ref, got[i], f"scene {i}, bound {bound}", upto=upto)
+        for k in range(len(rooms)):
+            if k != i:
+                assert_same(refs[k], got[k], f"scene {k}")
+        b.capture()
+        poison(c)
+        b.launch()
+        assert_same(ref, batch_states(rooms, b)[i], f"scene {i}, bound {bound}, graph", upto=upto)
+    with pytest.raises(lux.LuxError):
+        b.set_surrogate_bound(0, 1.0)                               # no surrogate on scene 0
+    b.close()
+    s.close()
 
 
 def level_offsets(ctx):
