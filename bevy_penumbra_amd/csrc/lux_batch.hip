@@ -370,6 +370,7 @@ struct lux_batch {
     std::vector<lux_surrogate*> sur;
     std::vector<uint32_t> sur_level;
     std::vector<float> sur_bound;  // per scene: the fallback's error bound (+inf: none)
+    std::vector<lux_buffer> sur_fb;  // per scene: the batch's own fallback words
     std::vector<BatchScene> host_table;
     std::vector<uint64_t> item_base, interval_base, probe_base, merge_base;
     uint32_t nee_max = 1, max_levels = 1;
@@ -562,7 +563,8 @@ int enqueue(lux_batch* b) {
         hipLaunchKernelGGL(k_b_sur_apply, dim3(qb), dim3(kWg), 0, st, Y, q,
                            static_cast<float4*>(c->gi_merged.ptr) + c->gi_offset[S]);
         if (b->sur_bound[s] < INFINITY)
-            if (int rc = lux_sur_fallback_on(c, st, S, Y, b->sur_bound[s])) return bfail(b, rc, lux_last_error(c));
+            if (int rc = lux_sur_fallback_on(c, st, S, Y, b->sur_bound[s], static_cast<uint32_t*>(b->sur_fb[s].ptr)))
+                return bfail(b, rc, lux_last_error(c));
     }
     for (int L = (int)b->max_levels - 1; L >= 0; --L) {
         uint64_t cnt = b->merge_base[(size_t)L * (n + 1) + n];
@@ -589,6 +591,7 @@ extern "C" int lux_batch_create(lux_ctx* const* scenes, uint32_t n, lux_batch** 
     b->sur.assign(n, nullptr);
     b->sur_level.assign(n, 0);
     b->sur_bound.assign(n, INFINITY);
+    b->sur_fb.resize(n);
     if (hipSetDevice(b->device) != hipSuccess ||
         hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
@@ -613,6 +616,7 @@ extern "C" void lux_batch_destroy(lux_batch* b) {
     lux_buffer* bufs[] = {&b->table, &b->bases, &b->prim, &b->shade, &b->slots,
                           &b->vis, &b->shadow, &b->counters, &b->queries, &b->sur_scratch};
     for (lux_buffer* x : bufs) lux_release(*x);
+    for (lux_buffer& x : b->sur_fb) lux_release(x);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -633,6 +637,7 @@ extern "C" int lux_batch_set_surrogate(lux_batch* b, uint32_t scene, lux_surroga
         if (level < 1 || level >= c->gi.levels)
             return bfail(b, LUX_ERR_INVALID, "lux_batch_set_surrogate: level must be in 1..levels-1");
     }
+    drop_graph(b);  // a captured graph would keep the old prediction set
     b->sur[scene] = s;
     b->sur_bound[scene] = INFINITY;  // a new network or level starts without fallback
     b->sur_level[scene] = s ? level : 0u;
@@ -645,9 +650,14 @@ extern "C" int lux_batch_set_surrogate_bound(lux_batch* b, uint32_t scene, float
     if (!b->sur[scene])
         return bfail(b, LUX_ERR_STATE, "lux_batch_set_surrogate_bound: the scene has no surrogate");
     if (std::isnan(error_bound)) return bfail(b, LUX_ERR_INVALID, "lux_batch_set_surrogate_bound: NaN bound");
-    if (error_bound < INFINITY)
-        if (int rc = lux_sur_fallback_prepare(b->scenes[scene], b->sur_level[scene]))
-            return bfail(b, rc, lux_last_error(b->scenes[scene]));
+    // a captured graph holds the bound and the words' address: drop it
+    drop_graph(b);
+    LUX_BHIP(b, hipStreamSynchronize(b->stream));
+    if (error_bound < INFINITY) {
+        lux_ctx* c = b->scenes[scene];
+        if (int rc = lux_ensure(c, b->sur_fb[scene], 4ull * lux_sur_fallback_words(c, b->sur_level[scene])))
+            return bfail(b, rc, lux_last_error(c));
+    }
     b->sur_bound[scene] = error_bound;
     return LUX_OK;
 }

@@ -157,8 +157,9 @@ int lux_surrogate_forward_into(lux_surrogate* s, hipStream_t st, const float* qu
 int lux_gi_check_internal(lux_ctx* ctx);
 // History blend (when on), merge and pack of a context's intervals on stream st.
 int lux_gi_finish_internal(lux_ctx* ctx, hipStream_t st);
-int lux_sur_fallback_prepare(lux_ctx* ctx, uint32_t level);
-int lux_sur_fallback_on(lux_ctx* ctx, hipStream_t st, uint32_t level, const float* Y, float bound);
+uint64_t lux_sur_fallback_words(const lux_ctx* ctx, uint32_t level);
+int lux_sur_fallback_on(lux_ctx* ctx, hipStream_t st, uint32_t level, const float* Y, float bound,
+                        uint32_t* words);
 // Receiver rays of a ray list through the wavefront stages on the context
 // stream: per ray (rad.rgb, hit t) and status (1 hit, 2 unsupported).
 int lux_rx_wave_trace(lux_ctx* ctx, const float* recv, uint32_t count, const uint32_t* rays, uint32_t n_rays,

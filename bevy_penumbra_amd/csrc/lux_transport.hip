@@ -1458,27 +1458,28 @@ __global__ void k_sur_count_zero(uint32_t* c) {
 }
 }  // namespace
 
-// Buffers of the traced per-probe fallback of level `level` (list and errors).
-int lux_sur_fallback_prepare(lux_ctx* ctx, uint32_t level) {
-    const uint64_t probes = level_count(ctx, level) / level_directions(ctx->gi, level);
-    if (int rc = lux_ensure(ctx, ctx->gi_sur_err, 4ull * probes)) return rc;
-    return lux_ensure(ctx, ctx->gi_sur_list, 4ull * (probes + 1));
+// Words of the traced per-probe fallback of level `level`: the probe list,
+// its count and the per-probe errors (2 * probes + 1).
+uint64_t lux_sur_fallback_words(const lux_ctx* ctx, uint32_t level) {
+    return 2ull * (level_count(ctx, level) / level_directions(ctx->gi, level)) + 1ull;
 }
 
 // The traced per-probe fallback on a stream (no allocation, no host wait, so
-// a captured graph may hold it): per-probe predicted RMSE from the network
+// a captured graph may hold it; `words` of lux_sur_fallback_words are the
+// caller's): per-probe predicted RMSE from the network
 // outputs Y of the level's queries, then the probes above the bound traced as
 // the level's unbounded terminal intervals and written merged (the fused form
 // of lux_gi_update_surrogate's fallback; the same bits).
-int lux_sur_fallback_on(lux_ctx* ctx, hipStream_t st, uint32_t level, const float* Y, float bound) {
+int lux_sur_fallback_on(lux_ctx* ctx, hipStream_t st, uint32_t level, const float* Y, float bound,
+                        uint32_t* words) {
     const uint64_t count = level_count(ctx, level);
     const uint32_t D = level_directions(ctx->gi, level);
     const uint64_t probes = count / D;
-    uint32_t* list = static_cast<uint32_t*>(ctx->gi_sur_list.ptr);
+    uint32_t* list = words;
     uint32_t* fcount = list + probes;
+    float* err = reinterpret_cast<float*>(fcount + 1);
     hipLaunchKernelGGL(k_sur_count_zero, dim3(1), dim3(64), 0, st, fcount);
-    hipLaunchKernelGGL(k_sur_probe_error, dim3((uint32_t)probes), dim3(64), 0, st, Y, D, bound,
-                       static_cast<float*>(ctx->gi_sur_err.ptr), list, fcount);
+    hipLaunchKernelGGL(k_sur_probe_error, dim3((uint32_t)probes), dim3(64), 0, st, Y, D, bound, err, list, fcount);
     GiArgs a = gi_args(ctx);
     hipLaunchKernelGGL(k_sur_fallback, dim3((uint32_t)((count + kWg - 1) / kWg)), dim3(kWg), 0, st, a, level, list,
                        static_cast<const uint32_t*>(fcount));

@@ -589,7 +589,8 @@ int lux_batch_set_surrogate(lux_batch* b, uint32_t scene, lux_surrogate* s, uint
  * lux_gi_update_surrogate(..., error_bound): probes whose predicted RMSE is
  * above the bound (or NaN) are traced inside the batch launch (and a
  * captured graph).  +inf (the default, and after lux_batch_set_surrogate):
- * none.  Needs a surrogate on the scene. */
+ * none.  Needs a surrogate on the scene.  Both calls drop a captured graph
+ * (lux_batch_capture again). */
 int lux_batch_set_surrogate_bound(lux_batch* b, uint32_t scene, float error_bound);
 int lux_batch_refresh(lux_batch* b);
 /* Frame index and seed of every scene (kept in device memory: a captured
