@@ -473,8 +473,8 @@ namespace {
 // sides non-empty (axis order x, y, z, first strictly cheaper wins), Hoare
 // partition by bin, leaves of at most 4 and depth 48, and for coincident
 // centroids a median split on the widest centroid axis.  The reference leaves
-// that median's element order to select_nth_unstable; here the elements are
-// fully sorted by (centre, index).  Nodes are written depth-first as the
+// that median's element order to select_nth_unstable (by f32::total_cmp);
+// here the elements are fully sorted by (total-order centre, index).  Nodes are written depth-first as the
 // reference appends them (children pairs after their parent).
 struct SahBuilder {
     const float* lo;  // 4 floats per element
@@ -585,9 +585,16 @@ struct SahBuilder {
         } else {
             const float sx = cmax[0] - cmin[0], sy = cmax[1] - cmin[1], sz = cmax[2] - cmin[2];
             const int axis = (sx >= sy && sx >= sz) ? 0 : (sy >= sz ? 1 : 2);
+            // select_nth_unstable_by's f32::total_cmp, as an unsigned key
+            auto key = [&](uint32_t e) {
+                uint32_t u;
+                const float c = centre(e, axis);
+                std::memcpy(&u, &c, 4);
+                return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+            };
             std::sort(el, el + count, [&](uint32_t a, uint32_t b) {
-                const float ca = centre(a, axis), cb = centre(b, axis);
-                return ca < cb || (!(cb < ca) && a < b);
+                const uint32_t ka = key(a), kb = key(b);
+                return ka < kb || (ka == kb && a < b);
             });
             mid = count / 2;
         }

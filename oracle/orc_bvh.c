@@ -243,12 +243,17 @@ static float centre(const sah_ctx* c, uint32_t e, int axis) {
 
 static int g_med_axis;
 static const float *g_med_lo, *g_med_hi;
+/* f32::total_cmp as an unsigned key: -NaN < -inf < ... < -0 < +0 < ... < NaN */
+static uint32_t total_key(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
 static int med_cmp(const void* a, const void* b) {
     uint32_t i = *(const uint32_t*)a, j = *(const uint32_t*)b;
-    float ci = g_med_lo[4 * i + g_med_axis] * 0.5f + g_med_hi[4 * i + g_med_axis] * 0.5f;
-    float cj = g_med_lo[4 * j + g_med_axis] * 0.5f + g_med_hi[4 * j + g_med_axis] * 0.5f;
-    if (ci < cj) return -1;
-    if (ci > cj) return 1;
+    uint32_t ci = total_key(g_med_lo[4 * i + g_med_axis] * 0.5f + g_med_hi[4 * i + g_med_axis] * 0.5f);
+    uint32_t cj = total_key(g_med_lo[4 * j + g_med_axis] * 0.5f + g_med_hi[4 * j + g_med_axis] * 0.5f);
+    if (ci != cj) return ci < cj ? -1 : 1;
     return i < j ? -1 : (i > j ? 1 : 0);
 }
 

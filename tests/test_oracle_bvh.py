@@ -249,3 +249,17 @@ def test_oracle_tlas_refit_is_a_tight_union():
             assert np.array_equal(f[k, 4:7], np.maximum(f[first, 4:7], f[first + 1, 4:7]))
             stack += [first, first + 1]
     assert seen > 1
+
+
+def test_sah_median_fallback_orders_by_total_cmp():
+    """Coincident centroids take build_tree's median split, ordered by
+    f32::total_cmp (src/scene.rs:2523-2533): -0 sorts before +0 (which `<`
+    calls equal), ties keep the index order."""
+    n = 8
+    lo = np.zeros((n, 4), np.float32)
+    hi = np.zeros((n, 4), np.float32)
+    neg = [1, 4, 6]                                        # boxes at x = -0
+    lo[neg, 0] = hi[neg, 0] = np.float32(-0.0)
+    perm, nodes = O.sah(lo, hi)
+    assert nodes.shape[0] == 3 and nodes[0, 7] == 0 and nodes[1, 7] == 4 and nodes[2, 7] == 4
+    assert perm.tolist() == [1, 4, 6, 0, 2, 3, 5, 7]
