@@ -13,9 +13,10 @@
  *    (every primitive covered once, children contained, ranges contiguous).
  * 2. Binned SAH build_tree (src/scene.rs:2398-2554): 12 bins, f64 bin index,
  *    in-place swap partition, leaves <= 4, depth <= 48.  The median fallback
- *    for coincident centroids uses Rust's select_nth_unstable_by, whose
- *    element order is implementation-defined; here it is a stable sort by
- *    centre (parity unpinned for that branch only; the fixtures avoid it).
+ *    for coincident centroids uses Rust's select_nth_unstable_by (by
+ *    f32::total_cmp), whose element order within each side is
+ *    implementation-defined; here it is a full sort by (total-order centre,
+ *    index): the same split sets, the order inside them unpinned.
  */
 #include "lux_oracle.h"
 #include <math.h>
