@@ -22,6 +22,7 @@ HIP_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-shared",
+    "-pthread",  # the host SAH build spreads meshes over threads
     # IEEE f32 with no contraction: the traversal and merge arithmetic must
     # match a scalar CPU evaluation bit for bit.
     "-ffp-contract=off",

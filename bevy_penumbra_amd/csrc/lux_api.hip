@@ -6,6 +6,8 @@
 // text is kept for lux_last_error (forwarded to
 // RadianceCascadesStatus::record_failure, src/lib.rs:225-230).
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cstring>
 #include <cstdio>
 #include "lux_internal.h"
@@ -656,6 +658,25 @@ static int build_blas(lux_ctx* ctx, bool sah) {
     float4* hi = lo + maxn;
     uint32_t* perm = reinterpret_cast<uint32_t*>(hi + maxn);
     int rc = LUX_OK;
+    // SAH: every mesh's tree on the host first, meshes spread over threads
+    std::vector<std::vector<lux_bvh_node>> sah_nodes;
+    std::vector<std::vector<uint32_t>> sah_perm;
+    if (sah) {
+        sah_nodes.resize(ctx->n_meshes);
+        sah_perm.resize(ctx->n_meshes);
+        const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        const uint32_t nth = std::min<uint32_t>(hw, ctx->n_meshes);
+        std::atomic<uint32_t> next{0};
+        auto work = [&]() {
+            for (uint32_t m = next++; m < ctx->n_meshes; m = next++)
+                sah_mesh(ctx->host_pos[m].data(), ctx->mesh_tri_count[m], ctx->mesh_blas_base[m],
+                         ctx->mesh_tri_base[m], sah_nodes[m], sah_perm[m]);
+        };
+        std::vector<std::thread> pool;
+        for (uint32_t t = 1; t < nth; ++t) pool.emplace_back(work);
+        work();
+        for (std::thread& t : pool) t.join();
+    }
     for (uint32_t m = 0; m < ctx->n_meshes && rc == LUX_OK; ++m) {
         uint32_t n = ctx->mesh_tri_count[m], tb = ctx->mesh_tri_base[m], bb = ctx->mesh_blas_base[m];
         const float* pos = static_cast<const float*>(ctx->raw_pos.ptr) + 9ull * tb;
@@ -664,9 +685,8 @@ static int build_blas(lux_ctx* ctx, bool sah) {
         if (sah) {
             // host build from the uploaded positions; unused slots of the
             // mesh's 2n - 1 region stay zero
-            std::vector<lux_bvh_node> hn;
-            std::vector<uint32_t> hp;
-            sah_mesh(ctx->host_pos[m].data(), n, bb, tb, hn, hp);
+            const std::vector<lux_bvh_node>& hn = sah_nodes[m];
+            const std::vector<uint32_t>& hp = sah_perm[m];
             const uint32_t region = n > 1 ? 2 * n - 1 : 1;
             LUX_HIP(ctx, hipStreamSynchronize(st));
             LUX_HIP(ctx, hipMemset(nodes, 0, 32ull * region));

@@ -158,8 +158,9 @@ int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_world,
 #define LUX_BVH_REFIT 0x200u
 /* or-ed into the mode: build each BLAS with the reference's binned SAH
  * (build_tree, src/scene.rs:2398-2554) on the host from the uploaded
- * positions -- the reference's own tree, fewer node visits than Morton-30,
- * about a second per million triangles -- instead of Morton-30 on the GPU;
+ * positions -- the reference's own tree, fewer node visits than Morton-30;
+ * meshes build in parallel on up to 16 host threads, one 262k-triangle mesh
+ * in about 150 ms -- instead of Morton-30 on the GPU;
  * the TLAS stays Morton-30 on the GPU.  A change of builder rebuilds the
  * BLASes. */
 #define LUX_BVH_SAH 0x1u
