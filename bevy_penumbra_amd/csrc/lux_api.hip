@@ -673,7 +673,11 @@ static int build_blas(lux_ctx* ctx, bool sah) {
                          ctx->mesh_tri_base[m], sah_nodes[m], sah_perm[m]);
         };
         std::vector<std::thread> pool;
-        for (uint32_t t = 1; t < nth; ++t) pool.emplace_back(work);
+        try {
+            for (uint32_t t = 1; t < nth; ++t) pool.emplace_back(work);
+        } catch (...) {
+            // no more threads: the ones started and this one finish the work
+        }
         work();
         for (std::thread& t : pool) t.join();
     }
