@@ -604,6 +604,34 @@ struct SahBuilder {
         nodes.resize(nodes.size() + 2);
         nodes[index].first = children;
         nodes[index].count = 0;
+        if (depth < 3 && count >= 65536) {
+            // large top-level subtrees in parallel, each into its own array,
+            // then spliced in the sequential pre-order: the left subtree's
+            // nodes after the pair, then the right's (local k -> base + k - 1)
+            SahBuilder left{lo, hi, perm, {}}, right{lo, hi, perm, {}};
+            left.nodes.resize(1);
+            right.nodes.resize(1);
+            std::thread t;
+            try {
+                t = std::thread([&] { left.build(0, first, mid, depth + 1); });
+            } catch (...) {
+                left.build(0, first, mid, depth + 1);
+            }
+            right.build(0, first + mid, count - mid, depth + 1);
+            if (t.joinable()) t.join();
+            const uint32_t base_l = (uint32_t)nodes.size();
+            const uint32_t base_r = base_l + (uint32_t)left.nodes.size() - 1u;
+            auto moved = [](lux_bvh_node nd, uint32_t base) {
+                if (nd.count == 0) nd.first = base + nd.first - 1u;
+                return nd;
+            };
+            nodes[children] = moved(left.nodes[0], base_l);
+            nodes[children + 1] = moved(right.nodes[0], base_r);
+            nodes.reserve(nodes.size() + left.nodes.size() + right.nodes.size());
+            for (size_t k = 1; k < left.nodes.size(); ++k) nodes.push_back(moved(left.nodes[k], base_l));
+            for (size_t k = 1; k < right.nodes.size(); ++k) nodes.push_back(moved(right.nodes[k], base_r));
+            return;
+        }
         build(children, first, mid, depth + 1);
         build(children + 1, first + mid, count - mid, depth + 1);
     }

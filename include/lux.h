@@ -159,8 +159,9 @@ int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_world,
 /* or-ed into the mode: build each BLAS with the reference's binned SAH
  * (build_tree, src/scene.rs:2398-2554) on the host from the uploaded
  * positions -- the reference's own tree, fewer node visits than Morton-30;
- * meshes build in parallel on up to 16 host threads, one 262k-triangle mesh
- * in about 150 ms -- instead of Morton-30 on the GPU;
+ * meshes build in parallel on up to 16 host threads and a large mesh's top
+ * subtrees in parallel too (one 262k-triangle mesh in about 55 ms), with the
+ * sequential node order -- instead of Morton-30 on the GPU;
  * the TLAS stays Morton-30 on the GPU.  A change of builder rebuilds the
  * BLASes. */
 #define LUX_BVH_SAH 0x1u
