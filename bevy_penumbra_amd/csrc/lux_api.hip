@@ -474,15 +474,20 @@ namespace {
 // f64 bin index, bin costs area(f64) x count, the cheapest split with both
 // sides non-empty (axis order x, y, z, first strictly cheaper wins), Hoare
 // partition by bin, leaves of at most 4 and depth 48, and for coincident
-// centroids a median split on the widest centroid axis.  The reference leaves
-// that median's element order to select_nth_unstable (by f32::total_cmp);
-// here the elements are fully sorted by (total-order centre, index).  Nodes are written depth-first as the
-// reference appends them (children pairs after their parent).
+// centroids a median split on the widest centroid axis.  The reference takes
+// that median with select_nth_unstable_by (by f32::total_cmp), which sorts a
+// slice of up to 10 elements by stable insertion; here the elements are
+// stably sorted by the total-order centre alone, the same order for those
+// slices (elements of equal key keep their current order).  Nodes are
+// written depth-first as the reference appends them (children pairs after
+// their parent).  A failed allocation in a worker thread is recorded in
+// `failed` and reported by the caller.
 struct SahBuilder {
     const float* lo;  // 4 floats per element
     const float* hi;
     uint32_t* perm;
     std::vector<lux_bvh_node> nodes;
+    std::atomic<bool>* failed = nullptr;
 
     float centre(uint32_t e, int a) const { return lo[4 * e + a] * 0.5f + hi[4 * e + a] * 0.5f; }
 
@@ -594,10 +599,7 @@ struct SahBuilder {
                 std::memcpy(&u, &c, 4);
                 return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
             };
-            std::sort(el, el + count, [&](uint32_t a, uint32_t b) {
-                const uint32_t ka = key(a), kb = key(b);
-                return ka < kb || (ka == kb && a < b);
-            });
+            std::stable_sort(el, el + count, [&](uint32_t a, uint32_t b) { return key(a) < key(b); });
             mid = count / 2;
         }
         const uint32_t children = (uint32_t)nodes.size();
@@ -608,17 +610,25 @@ struct SahBuilder {
             // large top-level subtrees in parallel, each into its own array,
             // then spliced in the sequential pre-order: the left subtree's
             // nodes after the pair, then the right's (local k -> base + k - 1)
-            SahBuilder left{lo, hi, perm, {}}, right{lo, hi, perm, {}};
+            SahBuilder left{lo, hi, perm, {}, failed}, right{lo, hi, perm, {}, failed};
             left.nodes.resize(1);
             right.nodes.resize(1);
+            auto guarded = [this](SahBuilder& sub, uint32_t f, uint32_t c, uint32_t d) {
+                try {
+                    sub.build(0, f, c, d);
+                } catch (...) {
+                    failed->store(true);
+                }
+            };
             std::thread t;
             try {
-                t = std::thread([&] { left.build(0, first, mid, depth + 1); });
+                t = std::thread([&] { guarded(left, first, mid, depth + 1); });
             } catch (...) {
-                left.build(0, first, mid, depth + 1);
+                guarded(left, first, mid, depth + 1);  // no thread: build it here
             }
-            right.build(0, first + mid, count - mid, depth + 1);
+            guarded(right, first + mid, count - mid, depth + 1);
             if (t.joinable()) t.join();
+            if (failed->load()) return;
             const uint32_t base_l = (uint32_t)nodes.size();
             const uint32_t base_r = base_l + (uint32_t)left.nodes.size() - 1u;
             auto moved = [](lux_bvh_node nd, uint32_t base) {
@@ -640,7 +650,7 @@ struct SahBuilder {
 // One mesh's SAH BLAS: nodes with absolute child (node_base) and triangle
 // (tri_base) indices, and the triangle order.
 void sah_mesh(const float* pos, uint32_t n, uint32_t node_base, uint32_t tri_base, std::vector<lux_bvh_node>& nodes,
-              std::vector<uint32_t>& perm) {
+              std::vector<uint32_t>& perm, std::atomic<bool>& failed) {
     std::vector<float> lo(4ull * n), hi(4ull * n);
     for (uint32_t i = 0; i < n; ++i) {
         const float* p = pos + 9ull * i;
@@ -655,9 +665,10 @@ void sah_mesh(const float* pos, uint32_t n, uint32_t node_base, uint32_t tri_bas
     }
     perm.resize(n);
     for (uint32_t i = 0; i < n; ++i) perm[i] = i;
-    SahBuilder b{lo.data(), hi.data(), perm.data(), {}};
+    SahBuilder b{lo.data(), hi.data(), perm.data(), {}, &failed};
     b.nodes.resize(1);
     b.build(0, 0, n, 0);
+    if (failed.load()) return;
     for (lux_bvh_node& nd : b.nodes) nd.first += nd.count == 0 ? node_base : tri_base;
     nodes.swap(b.nodes);
 }
@@ -695,19 +706,28 @@ static int build_blas(lux_ctx* ctx, bool sah) {
         const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         const uint32_t nth = std::min<uint32_t>(hw, ctx->n_meshes);
         std::atomic<uint32_t> next{0};
+        std::atomic<bool> failed{false};
+        // an exception (a failed allocation) stops every worker and is
+        // reported as LUX_ERR_OOM; none escapes a thread or this function
         auto work = [&]() {
-            for (uint32_t m = next++; m < ctx->n_meshes; m = next++)
-                sah_mesh(ctx->host_pos[m].data(), ctx->mesh_tri_count[m], ctx->mesh_blas_base[m],
-                         ctx->mesh_tri_base[m], sah_nodes[m], sah_perm[m]);
+            try {
+                for (uint32_t m = next++; m < ctx->n_meshes && !failed.load(); m = next++)
+                    sah_mesh(ctx->host_pos[m].data(), ctx->mesh_tri_count[m], ctx->mesh_blas_base[m],
+                             ctx->mesh_tri_base[m], sah_nodes[m], sah_perm[m], failed);
+            } catch (...) {
+                failed.store(true);
+            }
         };
         std::vector<std::thread> pool;
         try {
+            pool.reserve(nth);
             for (uint32_t t = 1; t < nth; ++t) pool.emplace_back(work);
         } catch (...) {
             // no more threads: the ones started and this one finish the work
         }
         work();
         for (std::thread& t : pool) t.join();
+        if (failed.load()) return lux_fail(ctx, LUX_ERR_OOM, "lux_bvh_build: host SAH build ran out of memory");
     }
     for (uint32_t m = 0; m < ctx->n_meshes && rc == LUX_OK; ++m) {
         uint32_t n = ctx->mesh_tri_count[m], tb = ctx->mesh_tri_base[m], bb = ctx->mesh_blas_base[m];
@@ -821,7 +841,13 @@ extern "C" int lux_bvh_build(lux_ctx* ctx, uint32_t mode) {
     if (!ctx->blas_built) {
         if ((mode & LUX_BVH_SAH) && ctx->host_pos.size() != ctx->n_meshes)
             return lux_fail(ctx, LUX_ERR_STATE, "lux_bvh_build: SAH needs the scene's host positions");
-        if (int rc = build_blas(ctx, (mode & LUX_BVH_SAH) != 0)) return rc;
+        int rc;
+        try {
+            rc = build_blas(ctx, (mode & LUX_BVH_SAH) != 0);
+        } catch (...) {  // host containers of the build; nothing may leave a C entry point
+            rc = lux_fail(ctx, LUX_ERR_OOM, "lux_bvh_build: host allocation failed");
+        }
+        if (rc) return rc;
         ctx->blas_sah = (mode & LUX_BVH_SAH) != 0;
         ctx->tlas_built = false;
         ctx->tlas_topology = false;

@@ -450,9 +450,18 @@ int prepare(lux_batch* b) {
             b->merge_base[row + s + 1] = b->merge_base[row + s] + cnt;
         }
         if (b->sur[s]) {
+            // the scene may have been reconfigured since the surrogate was set
+            if (S < 1 || S >= levels)
+                return bfail(b, LUX_ERR_STATE, std::string("scene ") + std::to_string(s) + ": surrogate level " +
+                                                   std::to_string(S) + " is no longer in 1..levels-1 (" +
+                                                   std::to_string(levels) + " levels); set the surrogate again");
             uint64_t q = c->gi_offset[S + 1] - c->gi_offset[S];
             b->query_max = std::max(b->query_max, q);
             if (q >= (1ull << 31)) return bfail(b, LUX_ERR_INVALID, "surrogate level too large");
+            // the fallback's probe list, count and errors follow the scene's current grid
+            if (b->sur_bound[s] < INFINITY)
+                if (int rc = lux_ensure(c, b->sur_fb[s], 4ull * lux_sur_fallback_words(c, S)))
+                    return bfail(b, rc, lux_last_error(c));
         }
     }
     const uint64_t items = b->item_base[n];
@@ -776,7 +785,10 @@ extern "C" int lux_batch_get_config(lux_batch* b, lux_trace_config* cfg) {
 
 // Autotuner of the batched launches: coordinate descent over the same knobs,
 // each candidate timed over whole batch updates (median of `repeats` after a
-// warm-up).  Drops a captured graph (capture again afterwards).
+// warm-up).  The timed updates skip the temporal history pass (it does not
+// depend on the knobs), so every scene's history, its validity and its
+// pending dirty regions are as they were before tuning.  Drops a captured
+// graph (capture again afterwards).
 extern "C" int lux_batch_autotune(lux_batch* b, uint32_t repeats, lux_trace_config* best, float* best_ms,
                                   uint32_t* evaluations) {
     if (!b) return LUX_ERR_INVALID;
@@ -785,6 +797,27 @@ extern "C" int lux_batch_autotune(lux_batch* b, uint32_t repeats, lux_trace_conf
     if (repeats > 15) repeats = 15;
     drop_graph(b);
     const uint32_t saved[4] = {b->wpc, b->depth, b->regroup, b->batch};
+    // history state restored on every return path
+    struct HistoryHold {
+        lux_batch* b;
+        bool any;
+        std::vector<uint32_t> dirty;
+        std::vector<char> valid;
+        explicit HistoryHold(lux_batch* bb) : b(bb), any(bb->any_history) {
+            for (lux_ctx* c : b->scenes) {
+                dirty.push_back(c->gi_dirty_n);
+                valid.push_back(c->gi_hist_valid ? 1 : 0);
+            }
+            b->any_history = false;
+        }
+        ~HistoryHold() {
+            b->any_history = any;
+            for (size_t s = 0; s < b->scenes.size(); ++s) {
+                b->scenes[s]->gi_dirty_n = dirty[s];
+                b->scenes[s]->gi_hist_valid = valid[s] != 0;
+            }
+        }
+    } hold(b);
     auto measure = [&](const uint32_t c[4], float& ms) -> int {
         b->wpc = c[0];
         b->depth = c[1];

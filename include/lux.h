@@ -594,6 +594,10 @@ int lux_batch_set_surrogate(lux_batch* b, uint32_t scene, lux_surrogate* s, uint
  * none.  Needs a surrogate on the scene.  Both calls drop a captured graph
  * (lux_batch_capture again). */
 int lux_batch_set_surrogate_bound(lux_batch* b, uint32_t scene, float error_bound);
+/* Rebuild the table after member scenes changed; the fallback words of a
+ * scene with a finite bound follow its current probe count.  Fails with
+ * LUX_ERR_STATE when a scene's surrogate level is no longer in
+ * 1..levels-1 (set or clear that scene's surrogate, then refresh). */
 int lux_batch_refresh(lux_batch* b);
 /* Frame index and seed of every scene (kept in device memory: a captured
  * graph picks them up); the member contexts' frame and seed follow. */
@@ -615,7 +619,9 @@ int lux_batch_get_config(lux_batch* b, lux_trace_config* cfg);
 /* Autotuner over whole batch updates (coordinate descent over waves per CU,
  * LDS stack depth, refill threshold, steps per refill; median of `repeats`);
  * installs the fastest and drops a captured graph.  Every configuration
- * writes identical results. */
+ * writes identical results.  The timed updates skip the temporal history
+ * pass: scenes' histories, their validity and pending dirty regions are left
+ * as they were (interval, merged and probe arrays hold a tuning update's). */
 int lux_batch_autotune(lux_batch* b, uint32_t repeats, lux_trace_config* best, float* best_ms,
                        uint32_t* evaluations);
 

@@ -14,9 +14,11 @@
  * 2. Binned SAH build_tree (src/scene.rs:2398-2554): 12 bins, f64 bin index,
  *    in-place swap partition, leaves <= 4, depth <= 48.  The median fallback
  *    for coincident centroids uses Rust's select_nth_unstable_by (by
- *    f32::total_cmp), whose element order within each side is
- *    implementation-defined; here it is a full sort by (total-order centre,
- *    index): the same split sets, the order inside them unpinned.
+ *    f32::total_cmp), which for a slice of at most 10 elements is a stable
+ *    insertion sort (insertion_sort_shift_left); here it is that insertion
+ *    sort for every size.  Pinned on 5-10 coincident triangles whose order a
+ *    parent's partition has permuted (tests/test_oracle_bvh.py); larger tied
+ *    groups (quickselect in the reference) are parity unpinned.
  */
 #include "lux_oracle.h"
 #include <math.h>
@@ -242,20 +244,26 @@ static float centre(const sah_ctx* c, uint32_t e, int axis) {
     return c->lo[4 * e + axis] * 0.5f + c->hi[4 * e + axis] * 0.5f;
 }
 
-static int g_med_axis;
-static const float *g_med_lo, *g_med_hi;
 /* f32::total_cmp as an unsigned key: -NaN < -inf < ... < -0 < +0 < ... < NaN */
 static uint32_t total_key(float f) {
     uint32_t u;
     memcpy(&u, &f, 4);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
-static int med_cmp(const void* a, const void* b) {
-    uint32_t i = *(const uint32_t*)a, j = *(const uint32_t*)b;
-    uint32_t ci = total_key(g_med_lo[4 * i + g_med_axis] * 0.5f + g_med_hi[4 * i + g_med_axis] * 0.5f);
-    uint32_t cj = total_key(g_med_lo[4 * j + g_med_axis] * 0.5f + g_med_hi[4 * j + g_med_axis] * 0.5f);
-    if (ci != cj) return ci < cj ? -1 : 1;
-    return i < j ? -1 : (i > j ? 1 : 0);
+
+/* Stable insertion sort of el[0..count) by the total-order centre on `axis`:
+ * each element shifts left past the elements of strictly greater key. */
+static void median_sort(const sah_ctx* c, uint32_t* el, uint32_t count, int axis) {
+    for (uint32_t i = 1; i < count; ++i) {
+        uint32_t e = el[i];
+        uint32_t k = total_key(centre(c, e, axis));
+        uint32_t j = i;
+        while (j > 0 && k < total_key(centre(c, el[j - 1], axis))) {
+            el[j] = el[j - 1];
+            --j;
+        }
+        el[j] = e;
+    }
 }
 
 static void sah_rec(sah_ctx* c, uint32_t index, uint32_t first, uint32_t count, uint32_t depth) {
@@ -330,8 +338,7 @@ static void sah_rec(sah_ctx* c, uint32_t index, uint32_t first, uint32_t count, 
     } else {
         float sx = cmax[0] - cmin[0], sy = cmax[1] - cmin[1], sz = cmax[2] - cmin[2];
         int axis = (sx >= sy && sx >= sz) ? 0 : (sy >= sz ? 1 : 2);
-        g_med_axis = axis; g_med_lo = c->lo; g_med_hi = c->hi;
-        qsort(el, count, sizeof(uint32_t), med_cmp);
+        median_sort(c, el, count, axis);
         mid = count / 2;
     }
     uint32_t children = c->n_nodes;

@@ -303,3 +303,53 @@ def test_gpu_batch_query_budget_per_scene(rooms):
     finally:
         p.query_limit = 0
         c.gi_configure(p)
+
+
+def test_gpu_batch_refresh_follows_a_reconfigured_surrogate_scene():
+    """A scene with a surrogate and a finite bound is reconfigured to a larger
+    grid: after lux_batch_refresh its fallback words cover the new probe count
+    and the batch update equals the scene's own lux_gi_update_surrogate; a
+    configuration without the surrogate's level makes the refresh fail."""
+    ctxs = [make_scene(20, SPECS[0]), make_scene(24, SPECS[4])]
+    c = ctxs[1]
+    tri, size, d, levels, R, ns = SPECS[4]
+    s = lux.Surrogate(c, lux.SurrogateParams.default((0, 0, 0), (size, size, size), seed=3))
+    level, bound = 1, 0.0                                           # bound 0: every probe of the level traced
+    try:
+        b = lux.Batch(ctxs, surrogates={1: (s, level)})
+        b.set_surrogate_bound(1, bound)
+        b.set_frame(2, 2)
+        b.update()
+        b.synchronize()
+        big = 2 * d                                                 # 8x the level's probes
+        sp = size / big
+        c.gi_configure(lux.gi_params((0.5 * sp, 0.5 * sp, 0.5 * sp), sp, (big, big, big), levels, base_phi=4,
+                                     base_y=4, interval_length=sp, emissive_samples=ns, rays_per_interval=R,
+                                     seed=4, terminal_tail=1))
+        b.refresh()
+        st = c.gi_update_surrogate(s, level, bound)
+        assert st.fallback_probes == st.probes
+        c.synchronize()
+        ref = state(c)
+        poison(c)
+        b.set_frame(2, 2)
+        c.gi_set_frame(2, 2)
+        c.gi_update_surrogate(s, level, bound, stats=False)
+        c.synchronize()
+        ref = state(c)
+        poison(c)
+        b.update()
+        offs = level_offsets(c)
+        assert_same(ref, batch_states(ctxs, b)[1], "reconfigured scene", upto=dict(intervals=offs[level],
+                                                                                   merged=offs[level + 1]))
+        c.gi_configure(lux.gi_params((0.5 * sp, 0.5 * sp, 0.5 * sp), sp, (big, big, big), 1, base_phi=4,
+                                     base_y=4, interval_length=sp, emissive_samples=ns, seed=4, terminal_tail=1))
+        with pytest.raises(lux.LuxError):
+            b.refresh()                                             # level 1 is not below 1 level
+        b.set_surrogate(1, None)
+        b.refresh()
+        b.close()
+    finally:
+        s.close()
+        for x in ctxs:
+            x.close()

@@ -243,3 +243,23 @@ def test_gpu_level_ranges_compose_to_the_full_trace(lux_ctx, mode):
         lux_ctx.gi_set_trace_mode(0)
     for key in ("intervals", "merged", "probes"):
         assert_bits(parts[key], full[key], key)
+
+
+@pytest.mark.parametrize("mode,terminal", [(1, 0), (1, 1), (0, 0), (0, 1)])
+def test_gpu_dense_fixture(lux_ctx, mode, terminal):
+    """The reference's dense 16^3 fixture (src/transport_tests.rs:3830-4106)
+    through the HIP interval trace: every interval equals the oracle's bit
+    for bit and, away from interval bounds and box edges, the f64 slab
+    intersection of its segment (J = (6, 2, 0.5) on the slab, 0 on other
+    boxes, beta = 1 for an empty non-terminal interval)."""
+    from tests.transport_cases import dense_expected, dense_fixture_params, dense_fixture_scene
+    sc = dense_fixture_scene()
+    p = dense_fixture_params(interval_mode=mode, terminal_tail=terminal)
+    g = run_gpu(lux_ctx, sc, p)
+    o = oracle_pipeline(sc, p)
+    for key in ("intervals", "merged", "probes"):
+        assert_bits(g[key], o[key], key)
+    want, close, _ = dense_expected(p)
+    ok = ~close
+    assert close.mean() < 0.01
+    assert np.array_equal(g["intervals"][ok], want[ok].astype(np.float32))

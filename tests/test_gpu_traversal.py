@@ -277,3 +277,24 @@ def test_gpu_traversal_work_equals_oracle(lux_ctx, colonnade, fixture, case):
     ohits, st = built.trace(rays, threads=16, stats=True)
     assert np.array_equal(h.numpy(), ohits)
     assert got == [st["node_fetches"], st["tri_tests"], st["inst_visits"]], (got, st)
+
+
+@pytest.mark.parametrize("group", [7, 10])
+def test_gpu_sah_median_keeps_partition_order(lux_ctx, group):
+    """The library's host SAH build takes the median of coincident centroids
+    in the order the parent's partition left (the reference's insertion sort
+    for slices of up to 10): its triangle order and nodes equal the oracle's
+    and a plain-loop model of build_tree."""
+    from tests.test_oracle_bvh import _model_build_tree, coincident_centroid_boxes
+    lo, hi = coincident_centroid_boxes(group)
+    tris = np.zeros((lo.shape[0], 9), np.float32)
+    tris[:, 0:3], tris[:, 3:6] = lo[:, :3], hi[:, :3]           # bounds of each triangle = the box
+    tris[:, 6:9] = np.stack([hi[:, 0], lo[:, 1], hi[:, 2]], axis=1)
+    mats = np.asarray([[0.5, 0.5, 0.5, 1.0, 0, 0, 0, 0]], np.float32)
+    sc = scenes.single_instance(tris, np.zeros(tris.shape[0], np.uint32), mats, ((0, 0, 0), (16, 4, 4)))
+    lux_ctx.upload_scene(sc)
+    lux_ctx.build_bvh(lux.LUX_BVH_SAH)
+    built = O.build_scene(sc, sah=2)
+    assert_same_scene(lux_ctx, built)
+    want_perm, _ = _model_build_tree(lo, hi)
+    assert lux_ctx.download_bvh()["tri_order"].tolist() == want_perm

@@ -9,6 +9,8 @@
 * SAH: src/scene.rs:3627-3674 (250k triangles, depth <= 48, cost < 3) and
   src/scene.rs:4440-4466 (cover every triangle once, contain children).
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -263,3 +265,139 @@ def test_sah_median_fallback_orders_by_total_cmp():
     perm, nodes = O.sah(lo, hi)
     assert nodes.shape[0] == 3 and nodes[0, 7] == 0 and nodes[1, 7] == 4 and nodes[2, 7] == 4
     assert perm.tolist() == [1, 4, 6, 0, 2, 3, 5, 7]
+
+
+def _model_build_tree(lo, hi, ties_by_index=False):
+    """build_tree (src/scene.rs:2398-2554) in plain Python loops over boxes:
+    12 bins with an f64 bin index, bin cost area(f64) x count, Hoare
+    partition, leaves <= 4, depth 48, and for coincident centroids the median
+    of select_nth_unstable_by(total_cmp), which for slices of at most 10
+    elements is Rust's stable insertion sort (insertion_sort_shift_left).
+    ties_by_index=True models the mistake of breaking key ties by index.
+    Returns (perm, nodes as (first, count, min3, max3))."""
+    f32 = np.float32
+    n = lo.shape[0]
+    perm = list(range(n))
+    nodes = [None]
+
+    def centre(e, a):
+        return f32(f32(lo[e, a]) * f32(0.5) + f32(hi[e, a]) * f32(0.5))
+
+    def key(e, a):
+        u = int(np.array([centre(e, a)], np.float32).view(np.uint32)[0])
+        return (~u & 0xFFFFFFFF) if u & 0x80000000 else (u | 0x80000000)
+
+    def area(mn, mx):
+        sx, sy, sz = (float(mx[a]) - float(mn[a]) for a in range(3))
+        return 2.0 * (sx * sy + sy * sz + sz * sx)
+
+    def build(index, first, count, depth):
+        el = perm[first:first + count]
+        mn = [min(float(lo[e, a]) for e in el) for a in range(3)]
+        mx = [max(float(hi[e, a]) for e in el) for a in range(3)]
+        nodes[index] = (first, count, mn, mx)
+        if count <= 4 or depth == 48:
+            return
+        cmin = [min(float(centre(e, a)) for e in el) for a in range(3)]
+        cmax = [max(float(centre(e, a)) for e in el) for a in range(3)]
+        best, split = math.inf, None
+        for axis in range(3):
+            span = cmax[axis] - cmin[axis]
+            if span <= 0.0:
+                continue
+            binof = lambda c: min(int((float(c) - cmin[axis]) / span * 12), 11)  # noqa: E731
+            bins = [[[math.inf] * 3, [-math.inf] * 3, 0] for _ in range(12)]
+            for e in el:
+                b = bins[binof(centre(e, axis))]
+                b[0] = [min(b[0][a], float(lo[e, a])) for a in range(3)]
+                b[1] = [max(b[1][a], float(hi[e, a])) for a in range(3)]
+                b[2] += 1
+
+            def join(x, y):
+                return [[min(x[0][a], y[0][a]) for a in range(3)],
+                        [max(x[1][a], y[1][a]) for a in range(3)], x[2] + y[2]]
+
+            def cost(b):
+                return 0.0 if b[2] == 0 else area(b[0], b[1]) * b[2]
+            right = [None] * 12
+            acc = [[math.inf] * 3, [-math.inf] * 3, 0]
+            for i in range(11, -1, -1):
+                acc = join(acc, bins[i])
+                right[i] = acc
+            left = [[math.inf] * 3, [-math.inf] * 3, 0]
+            for i in range(11):
+                left = join(left, bins[i])
+                c = cost(left) + cost(right[i + 1])
+                if left[2] > 0 and right[i + 1][2] > 0 and c < best:
+                    best, split = c, (axis, i, span)
+        if split is not None:
+            axis, last, span = split
+            l, r = 0, count
+            while l < r:
+                if min(int((float(centre(el[l], axis)) - cmin[axis]) / span * 12), 11) <= last:
+                    l += 1
+                else:
+                    r -= 1
+                    el[l], el[r] = el[r], el[l]
+            mid = l
+        else:
+            size = [cmax[a] - cmin[a] for a in range(3)]
+            axis = 0 if size[0] >= size[1] and size[0] >= size[2] else (1 if size[1] >= size[2] else 2)
+            assert count <= 10, "quickselect of larger slices is not modelled"
+            if ties_by_index:
+                el.sort(key=lambda e: (key(e, axis), e))
+            else:
+                for i in range(1, count):          # insertion_sort_shift_left
+                    j = i
+                    while j > 0 and key(el[j], axis) < key(el[j - 1], axis):
+                        el[j], el[j - 1] = el[j - 1], el[j]
+                        j -= 1
+            mid = count // 2
+        perm[first:first + count] = el
+        children = len(nodes)
+        nodes.extend([None, None])
+        nodes[index] = (children, 0, mn, mx)
+        build(children, first, mid, depth + 1)
+        build(children + 1, first + mid, count - mid, depth + 1)
+
+    build(0, 0, n, 0)
+    return perm, nodes
+
+
+def coincident_centroid_boxes(group):
+    """`group` boxes of different extents sharing the centre (1, 1, 1),
+    interleaved in input order with boxes spread along x = 10..15, so the root's
+    partition permutes the group before its median split."""
+    spread = 6
+    lo, hi = [], []
+    for i in range(max(group, spread)):
+        if i < group:
+            s = np.array([0.0625 * (i + 1), 0.125 * ((i * 5) % 7 + 1), 0.03125 * ((i * 3) % 5 + 1)])
+            lo.append(1.0 - s), hi.append(1.0 + s)
+        if i < spread:
+            c = np.array([10.0 + i, 2.0, 3.0])
+            lo.append(c - 0.25), hi.append(c + 0.25)
+    lo4 = np.zeros((len(lo), 4), np.float32)
+    hi4 = np.zeros((len(hi), 4), np.float32)
+    lo4[:, :3], hi4[:, :3] = lo, hi
+    return lo4, hi4
+
+
+@pytest.mark.parametrize("group", [5, 6, 7, 8, 9, 10])
+def test_sah_median_keeps_partition_order_of_coincident_centroids(group):
+    """The median split of 5-10 coincident centroids keeps the order a
+    parent's partition left (stable insertion sort), which gives other split
+    sets and node boxes than ordering the ties by index."""
+    lo, hi = coincident_centroid_boxes(group)
+    want_perm, want_nodes = _model_build_tree(lo, hi)
+    wrong_perm, wrong_nodes = _model_build_tree(lo, hi, ties_by_index=True)
+    assert want_perm != wrong_perm, "the fixture must reach a permuted median slice"
+    perm, nodes = O.sah(lo, hi)
+    assert perm.tolist() == want_perm
+    assert nodes.shape[0] == len(want_nodes)
+    f = nodes.view(np.float32)
+    for k, (first, count, mn, mx) in enumerate(want_nodes):
+        assert int(nodes[k, 3]) == first and int(nodes[k, 7]) == count, k
+        assert np.array_equal(f[k, :3], np.float32(mn)) and np.array_equal(f[k, 4:7], np.float32(mx)), k
+    if group >= 6:  # a median over >= 6 boxes: the wrong order moves a box to the other side
+        assert any(a[2:] != b[2:] for a, b in zip(want_nodes, wrong_nodes)), "node boxes must tell the orders apart"

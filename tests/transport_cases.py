@@ -77,3 +77,121 @@ def grid_receivers(n, lo, hi, seed=0):
     rec[:8, 3:] = 0.0           # not eligible
     rec[8:12, 3:] = np.nan      # not eligible
     return rec
+
+
+# Dense fixture of src/transport_tests.rs:3830-4106 (gpu_dense_16_cubed_sdf_...):
+# 16^3 probes at spacing 1 from -8, 4 levels, 32 * 4^L directions, intervals
+# [2^L - 1, 2^(L+1) - 1], emission (6, 2, 0.5) on the slab at y = 3, black
+# elsewhere.  The reference's SDF sphere becomes a box here (a triangle mesh
+# cannot hold a sphere exactly); the other three shapes are its boxes.
+DENSE_BOXES = [
+    # centre, half size, material (0 black, 1 emissive)
+    ((-3.0, 0.0, 0.0), (0.9, 0.9, 0.9), 0),
+    ((3.0, 1.0, 0.0), (0.7, 0.7, 0.7), 0),
+    ((0.0, 3.0, 0.0), (2.0, 0.04, 2.0), 1),
+    ((0.0, 0.0, 4.0), (2.0, 0.5, 0.03), 0),
+]
+DENSE_EMISSION = (6.0, 2.0, 0.5)
+
+
+def dense_fixture_scene():
+    from bevy_penumbra_amd import scenes
+    tris, mats = [], []
+    for centre, half, m in DENSE_BOXES:
+        tris.append(scenes.hexahedron(scenes.box_corners(centre, half)))
+        mats += [m] * 12
+    materials = np.asarray([[0, 0, 0, 1.0, 0, 0, 0, 0], [0, 0, 0, 1.0, *DENSE_EMISSION, 0]], np.float32)
+    return scenes.single_instance(np.concatenate(tris).astype(np.float32), np.asarray(mats, np.uint32),
+                                  materials, ((-8.0, -8.0, -8.0), (8.0, 8.0, 8.0)))
+
+
+def dense_fixture_params(interval_mode=1, terminal_tail=0, seed=0):
+    return O.gi_params((-8.0, -8.0, -8.0), 1.0, (16, 16, 16), 4, base_phi=8, base_y=4,
+                       interval_mode=interval_mode, interval_length=1.0, normal_bias=0.005,
+                       emissive_samples=2, seed=seed, terminal_tail=terminal_tail)
+
+
+def dense_interval_laws(mode, length, levels):
+    """Interval [start, end] per level by closed form: mode 1 ends at
+    len * (2^(L+1) - 1) (the fixture's [2^L - 1, 2^(L+1) - 1] at len 1, the
+    small configuration's [0,0.2],[0.2,0.6],[0.6,1.4],[1.4,3.0] at 0.2), mode 0
+    at len * 4^L (shaders/split.slang:224-231)."""
+    ends = [length * (2 ** (L + 1) - 1) if mode == 1 else length * 4.0 ** L for L in range(levels)]
+    return [(0.0 if L == 0 else ends[L - 1], ends[L]) for L in range(levels)]
+
+
+def dense_expected(p, margin=1e-3):
+    """Per interval (J, beta) from f64 slab intersections of each segment with
+    the boxes: the first box surface crossed after the interval start decides
+    the hit, if it lies before the interval end (any distance on the terminal
+    level with a terminal tail).  Also a mask of intervals too close to call
+    (a crossing within `margin` of an interval bound, or grazing a box edge)
+    and the box hit per interval (index into DENSE_BOXES, -1: none)."""
+    bounds = dense_interval_laws(p.interval_mode, p.interval_length, p.levels)
+    lo_b = np.array([np.subtract(c, h) for c, h, _ in DENSE_BOXES])
+    hi_b = np.array([np.add(c, h) for c, h, _ in DENSE_BOXES])
+    mat_b = np.array([m for _, _, m in DENSE_BOXES])
+    ox, oy = _dense_offset(p.seed)
+    out, amb, mats = [], [], []
+    for L in range(p.levels):
+        d = 16 >> L
+        nphi, ny = p.base_phi << L, p.base_y << L
+        dirs = np.stack([O.bin_direction(k, nphi, ny, ox, oy) for k in range(nphi * ny)]).astype(np.float64)
+        idx = np.arange(d)
+        sl = np.float32(p.spacing) * np.float32(2.0 ** L)
+        cc = (np.float32(-8.0) + (idx.astype(np.float32) + np.float32(0.5)) * sl).astype(np.float64)
+        Z, Y, X = np.meshgrid(cc, cc, cc, indexing="ij")
+        centres = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)          # probe-major, x fastest
+        C = np.repeat(centres, dirs.shape[0], axis=0)
+        D = np.tile(dirs, (centres.shape[0], 1))
+        start, end = bounds[L]
+        terminal = bool(p.terminal_tail) and L + 1 == p.levels
+        best = np.full(C.shape[0], np.inf)
+        best_m = np.full(C.shape[0], -1)
+        close = np.zeros(C.shape[0], bool)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            inv = 1.0 / D
+            for b in range(len(DENSE_BOXES)):
+                t0 = (lo_b[b] - C) * inv
+                t1 = (hi_b[b] - C) * inv
+                tn = np.nanmax(np.minimum(t0, t1), axis=1)
+                tf = np.nanmin(np.maximum(t0, t1), axis=1)
+                par = np.abs(D) < 1e-9
+                inside = (C >= lo_b[b]) & (C <= hi_b[b])
+                miss_par = np.any(par & ~inside, axis=1)
+                hit_line = (tn <= tf) & ~miss_par
+                for t in (tn, tf):
+                    ok = hit_line & (t > start)
+                    # crossing point near an edge of the box (two faces within margin)
+                    P = C + D * t[:, None]
+                    near = (np.abs(P - lo_b[b]) < margin) | (np.abs(P - hi_b[b]) < margin)
+                    edge = near.sum(axis=1) >= 2
+                    close |= hit_line & (t > start - margin) & (
+                        (np.abs(t - start) < margin) | ((not terminal) & (np.abs(t - end) < margin)) | edge)
+                    take = ok & (t < best)
+                    best = np.where(take, t, best)
+                    best_m = np.where(take, b, best_m)
+                close |= np.any(par & (np.abs(C - lo_b[b]) < margin) | par & (np.abs(C - hi_b[b]) < margin), axis=1)
+        hit = best <= (np.inf if terminal else end)
+        J = np.zeros((C.shape[0], 3))
+        J[hit & (mat_b[np.maximum(best_m, 0)] == 1)] = DENSE_EMISSION
+        beta = np.where(hit, 0.0, 0.0 if terminal else 1.0)
+        out.append(np.concatenate([J, beta[:, None]], axis=1))
+        amb.append(close)
+        mats.append(np.where(hit, best_m, -1))
+    return np.concatenate(out), np.concatenate(amb), np.concatenate(mats)
+
+
+def _dense_offset(seed):
+    """In-bin offset of ray 0: frac(0.5 + r2(transportHash(seed ^ 0x5a17c9e3)))."""
+    def th(v):
+        v ^= v >> 16; v = (v * 0x7feb352d) & 0xFFFFFFFF
+        v ^= v >> 15; v = (v * 0x846ca68b) & 0xFFFFFFFF
+        v ^= v >> 16
+        return v
+    h = th((seed ^ 0x5a17c9e3) & 0xFFFFFFFF)
+    jx = np.float32(((h * 3242174889) & 0xFFFFFFFF) >> 8) / np.float32(16777216.0)
+    jy = np.float32(((h * 2447445414) & 0xFFFFFFFF) >> 8) / np.float32(16777216.0)
+    fx = np.float32(np.float32(0.5) + jx); fx = fx - np.floor(fx)
+    fy = np.float32(np.float32(0.5) + jy); fy = fy - np.floor(fy)
+    return float(fx), float(fy)
