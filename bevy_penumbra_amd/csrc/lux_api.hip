@@ -273,6 +273,10 @@ extern "C" int lux_ctx_create(int device, lux_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->cu_count = prop.multiProcessorCount;
+    if (int rc = lux_trace_scratch_init(ctx)) {
+        lux_ctx_destroy(ctx);
+        return rc;
+    }
     *out = ctx;
     return LUX_OK;
 }

@@ -110,6 +110,9 @@ struct lux_ctx {
 constexpr float kTuneMargin = 0.985f;
 
 int lux_fail(lux_ctx* ctx, int code, const std::string& msg);
+// Looks up this device's copy of the allocation-free trace's scratch (once
+// per device; ctx->device must be current).
+int lux_trace_scratch_init(lux_ctx* ctx);
 int lux_hip_check(lux_ctx* ctx, hipError_t e, const char* what);
 int lux_ensure(lux_ctx* ctx, lux_buffer& b, size_t bytes);
 void lux_release(lux_buffer& b);

@@ -15,6 +15,7 @@
 // (shaders/split.slang:1266-1339 over shaders/traversal.slang:696-1046).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include "lux_internal.h"
 
 namespace {
@@ -142,6 +143,30 @@ __global__ void k_zero1(uint32_t* p) { *p = 0u; }
 
 }  // namespace
 
+// ---- scratch of the allocation-free trace (lux_trace_rays_device_noalloc) ----
+// The traversal kernels' only scratch is the global ray cursor.  The
+// allocation-free entry point keeps it in this file-scope device array, one
+// copy per GPU at its own address, looked up once per device when a context
+// is created on it.  Host code uses the looked-up address, never `scratch`
+// itself (the host-side name is not a device address).
+constexpr uint32_t kScratchWords = 64;          // 256 B: the cursor alone on its cache lines
+constexpr uint32_t kNoAllocMaxRays = 1u << 31;  // the cursor runs at most 64 per wave past n
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+static uint32_t* g_scratch_ptr[kScratchDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Called by lux_ctx_create with ctx->device current.
+int lux_trace_scratch_init(lux_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= kScratchDevices) return LUX_OK;  // the entry point refuses it
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[ctx->device]) return LUX_OK;
+    void* ptr = nullptr;
+    LUX_HIP(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[ctx->device] = static_cast<uint32_t*>(ptr);
+    return LUX_OK;
+}
+
 lux::DevScene lux_ctx::dev_scene() const {
     lux::DevScene s;
     s.tlas = static_cast<const float4*>(tlas.ptr);
@@ -155,22 +180,21 @@ lux::DevScene lux_ctx::dev_scene() const {
     return s;
 }
 
-static int trace_launch(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits, float* bary) {
-    if (!ctx->tlas_built) return lux_fail(ctx, LUX_ERR_STATE, "lux_trace: BVH not built");
-    if (n == 0) return LUX_OK;
-    if (!rays || !hits) return lux_fail(ctx, LUX_ERR_INVALID, "lux_trace: null buffer");
-    if (int rc = lux_ensure(ctx, ctx->scratch, 256)) return rc;
+// The launch of a trace with its cursor in `cursor` (4 bytes of device
+// memory); `timed` brackets it with the context's events.
+static int trace_launch_with(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits, float* bary,
+                             uint32_t* cursor, bool timed) {
     TraceArgs a;
     a.scene = ctx->dev_scene();
     a.rays = rays;
     a.hits = hits;
     a.bary = bary;
     a.n = n;
-    a.cursor = static_cast<uint32_t*>(ctx->scratch.ptr);
+    a.cursor = cursor;
     a.regroup = ctx->regroup;
     a.batch = ctx->trace_batch ? ctx->trace_batch : 8u;
     hipStream_t st = ctx->stream;
-    LUX_HIP(ctx, hipEventRecord(ctx->ev0, st));
+    if (timed) LUX_HIP(ctx, hipEventRecord(ctx->ev0, st));
     if (ctx->regroup > 0u && ctx->waves_per_cu != 1000u) {
         uint32_t wpc = ctx->waves_per_cu ? ctx->waves_per_cu : 20u;
         uint32_t blocks = (uint32_t)ctx->cu_count * wpc / (kTraceWg / 64);
@@ -196,14 +220,44 @@ static int trace_launch(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* 
         hipLaunchKernelGGL(k_trace_persistent, dim3(blocks), dim3(kTraceWg), 0, st, a);
     }
     LUX_HIP(ctx, hipGetLastError());
-    LUX_HIP(ctx, hipEventRecord(ctx->ev1, st));
+    if (timed) LUX_HIP(ctx, hipEventRecord(ctx->ev1, st));
     return LUX_OK;
+}
+
+static int trace_launch(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits, float* bary) {
+    if (!ctx->tlas_built) return lux_fail(ctx, LUX_ERR_STATE, "lux_trace: BVH not built");
+    if (n == 0) return LUX_OK;
+    if (!rays || !hits) return lux_fail(ctx, LUX_ERR_INVALID, "lux_trace: null buffer");
+    if (int rc = lux_ensure(ctx, ctx->scratch, 256)) return rc;
+    return trace_launch_with(ctx, rays, n, hits, bary, static_cast<uint32_t*>(ctx->scratch.ptr), true);
 }
 
 extern "C" int lux_trace_rays_device(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits,
                                      float* bary) {
     if (!ctx) return LUX_ERR_INVALID;
     return trace_launch(ctx, rays, n, hits, bary);
+}
+
+// lux_trace_rays_device without any allocation: the cursor lives in the
+// device's `scratch` copy (zeroed by a fill kernel on the context stream, not
+// a memset), no events are recorded.  Calls that overlap on one GPU share
+// that cursor and race.
+extern "C" int lux_trace_rays_device_noalloc(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits,
+                                             float* bary) {
+    if (!ctx) return LUX_ERR_INVALID;
+    if (!ctx->tlas_built) return lux_fail(ctx, LUX_ERR_STATE, "lux_trace_rays_device_noalloc: BVH not built");
+    if (n > kNoAllocMaxRays)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_trace_rays_device_noalloc: more than 2^31 rays in one call");
+    if (n == 0) return LUX_OK;
+    if (!rays || !hits) return lux_fail(ctx, LUX_ERR_INVALID, "lux_trace_rays_device_noalloc: null buffer");
+    uint32_t* cursor = nullptr;
+    if (ctx->device >= 0 && ctx->device < kScratchDevices) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        cursor = g_scratch_ptr[ctx->device];
+    }
+    if (!cursor)
+        return lux_fail(ctx, LUX_ERR_STATE, "lux_trace_rays_device_noalloc: no scratch address for this device");
+    return trace_launch_with(ctx, rays, n, hits, bary, cursor, false);
 }
 
 extern "C" int lux_trace_rays(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits) {

@@ -384,6 +384,7 @@ def load_library() -> C.CDLL:
         "lux_morton_codes_device": (i32, [vp, vp, vp, u32, vp]),
         "lux_trace_rays": (i32, [vp, vp, u32, vp]),
         "lux_trace_rays_device": (i32, [vp, vp, u32, vp, vp]),
+        "lux_trace_rays_device_noalloc": (i32, [vp, vp, u32, vp, vp]),
         "lux_trace_stats_device": (i32, [vp, vp, u32, vp, vp]),
         "lux_last_kernel_ms": (i32, [vp, C.POINTER(C.c_float)]),
         "lux_set_trace_tuning": (i32, [vp, u32, u32]),
@@ -681,6 +682,11 @@ class Context:
     def trace_device(self, rays, n: int, hits, bary=None):
         self._check(self.lib.lux_trace_rays_device(self.h, _ptr(rays), n, _ptr(hits), _ptr(bary)),
                     "lux_trace_rays_device")
+
+    def trace_device_noalloc(self, rays, n: int, hits, bary=None):
+        """lux_trace_rays_device_noalloc: the same trace with no allocation."""
+        self._check(self.lib.lux_trace_rays_device_noalloc(self.h, _ptr(rays), n, _ptr(hits), _ptr(bary)),
+                    "lux_trace_rays_device_noalloc")
 
     def trace_stats_device(self, rays, n: int, hits, counters):
         self._check(self.lib.lux_trace_stats_device(self.h, _ptr(rays), n, _ptr(hits),

@@ -230,6 +230,15 @@ int lux_trace_rays(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits)
  * (else 2 floats per ray: w1, w2). */
 int lux_trace_rays_device(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits,
                           float* bary);
+/* lux_trace_rays_device that allocates nothing, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  The ray cursor lives in a 256-byte
+ * library-owned device array per GPU (held until the process ends) instead
+ * of context memory; no timing events are recorded (lux_last_kernel_ms keeps
+ * the last timed trace).  n <= 2^31 (LUX_ERR_INVALID beyond).  Calls that
+ * overlap on one GPU share the cursor and race: serialise them. */
+int lux_trace_rays_device_noalloc(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits,
+                                  float* bary);
 
 /* Instrumented trace (same results): counters[0] += node fetches,
  * counters[1] += triangle tests, counters[2] += instance visits, summed over
