@@ -43,6 +43,7 @@ struct BatchWave {
     const uint64_t* probe_base;     // [n_scenes + 1] level-0 probes
     const uint64_t* merge_base;     // [kMaxBatchLevels][n_scenes + 1] intervals merged per level
     uint32_t nee_max;
+    uint32_t stride;  // items per NEE sample row of slots / vis (sample-major: smp * stride + item)
     uint4* prim;
     float4* shade;
     float4* slots;
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(kWg) void k_b_setup(BatchWave w) {
                 float3 origin = make_float3(0.0f, 0.0f, 0.0f), dir = origin, t3 = origin;
                 float sl = 0.0f;
                 bool q = nee_sample(S.a, c, smp, origin, dir, sl, t3);
-                uint32_t slot = i * w.nee_max + smp;
+                uint32_t slot = smp * w.stride + i;
                 w.slots[slot] = make_float4(t3.x, t3.y, t3.z, q ? 1.0f : 0.0f);
                 uint64_t mask = __ballot(q);
                 if (q) {
@@ -196,7 +197,7 @@ struct BatchShadow {
     __device__ bool begin(uint32_t i, lux::Walk& wk, St& st) {
         float4 r0 = w->shadow[2ull * i], r1 = w->shadow[2ull * i + 1];
         uint32_t slot = __float_as_uint(r1.w);
-        uint32_t s = find_scene(w->item_base, w->n_scenes, slot / w->nee_max);
+        uint32_t s = find_scene(w->item_base, w->n_scenes, slot % w->stride);  // the slot's item
         if (!admit_query_scene(*w, s)) {
             w->vis[slot] = 2u;
             return false;
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(kWg) void k_b_accum(BatchWave w) {
                 float3 sum = make_float3(0.0f, 0.0f, 0.0f);
                 bool ok = true;
                 for (uint32_t smp = 0; smp < count; ++smp) {
-                    uint32_t slot = i * w.nee_max + smp;
+                    uint32_t slot = smp * w.stride + i;
                     float4 s4 = w.slots[slot];
                     if (s4.w == 0.0f) continue;
                     uint32_t v = w.vis[slot];
@@ -505,6 +506,7 @@ BatchWave wave_of(lux_batch* b) {
     w.probe_base = bases + 2 * (n + 1);
     w.merge_base = bases + 3 * (n + 1);
     w.nee_max = b->nee_max;
+    w.stride = (uint32_t)b->item_base[n];
     w.prim = static_cast<uint4*>(b->prim.ptr);
     w.shade = static_cast<float4*>(b->shade.ptr);
     w.slots = static_cast<float4*>(b->slots.ptr);

@@ -160,9 +160,12 @@ struct Wave {
     uint32_t seed;               // kWaveSampled: sample seed
     float* sample_out;           // kWaveSampled: 12 floats per sample
     uint32_t R, nee;             // rays per interval, NEE slots per ray
+    uint32_t stride;             // items per NEE sample row of `slots` / `vis` (the chunk's capacity)
     uint4* prim;                 // per item: t bits, tri, inst, err
     float4* shade;               // per item: emission rgb, w = 1 when NEE runs
-    float4* slots;               // per slot: contribution rgb, w = 1 when a query was issued
+    float4* slots;               // per slot (sample-major: smp * stride + item, so a wave's
+                                 // lanes touch consecutive slots): contribution rgb, w = 1
+                                 // when a query was issued
     uint32_t* vis;               // per slot: 0 visible, 1 occluded, 2 unsupported
     float4* shadow;              // per occlusion ray: (origin, limit), (direction, slot bits)
     uint32_t* counters;          // [0] primary cursor, [1] shadow rays, [2] shadow cursor
@@ -286,7 +289,7 @@ __global__ __launch_bounds__(kWg) void k_gi_setup(GiArgs a, Wave w) {
                 float3 origin = make_float3(0.0f, 0.0f, 0.0f), dir = origin, t3 = origin;
                 float sl = 0.0f;
                 bool q = nee_sample(a, c, smp, origin, dir, sl, t3);
-                uint32_t slot = i * w.nee + smp;
+                uint32_t slot = smp * w.stride + i;
                 w.slots[slot] = make_float4(t3.x, t3.y, t3.z, q ? 1.0f : 0.0f);
                 uint64_t mask = __ballot(q);
                 if (q) {
@@ -367,7 +370,7 @@ __global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
                 float3 sum = make_float3(0.0f, 0.0f, 0.0f);
                 bool ok = true;
                 for (uint32_t smp = 0; smp < count; ++smp) {
-                    uint32_t slot = i * w.nee + smp;
+                    uint32_t slot = smp * w.stride + i;
                     float4 s4 = w.slots[slot];
                     if (s4.w == 0.0f) continue;
                     uint32_t v = w.vis[slot];
@@ -436,7 +439,7 @@ __global__ __launch_bounds__(kWg) void k_rx_wave_finish(GiArgs a, Wave w) {
         if (sh.w != 0.0f) {
             float3 sum = make_float3(0.0f, 0.0f, 0.0f);
             for (uint32_t smp = 0; smp < count; ++smp) {
-                const uint32_t slot = i * w.nee + smp;
+                const uint32_t slot = smp * w.stride + i;
                 const float4 s4 = w.slots[slot];
                 if (s4.w == 0.0f) continue;
                 const uint32_t v = w.vis[slot];
@@ -949,6 +952,7 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
     Wave w = proto;
     w.R = R;
     w.nee = nee;
+    w.stride = (uint32_t)items;
     w.prim = static_cast<uint4*>(ctx->gi_w_prim.ptr);
     w.shade = static_cast<float4*>(ctx->gi_w_shade.ptr);
     w.slots = static_cast<float4*>(ctx->gi_w_slots.ptr);
