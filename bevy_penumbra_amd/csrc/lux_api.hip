@@ -297,7 +297,7 @@ extern "C" void lux_ctx_destroy(lux_ctx* ctx) {
                           &ctx->rx_keys, &ctx->rx_flags, &ctx->rx_slot, &ctx->rx_lod, &ctx->rx_ord,
                           &ctx->rx_seq, &ctx->rx_dir, &ctx->rx_tickets, &ctx->rx_dep_val, &ctx->rx_pack, &ctx->rx_traced,
                           &ctx->rx_sort, &ctx->rx_scan, &ctx->rx_counters, &ctx->rx_cold,
-                          &ctx->screen_cache};
+                          &ctx->screen_cache, &ctx->range_ok};
     for (lux_buffer* b : bufs) lux_release(*b);
     for (hipEvent_t e : ctx->ev_phase)
         if (e) (void)hipEventDestroy(e);
@@ -678,6 +678,45 @@ void sah_mesh(const float* pos, uint32_t n, uint32_t node_base, uint32_t tri_bas
 }
 }  // namespace
 
+namespace {
+__global__ void k_range_flag(uint32_t* flag, uint32_t v) { *flag = v; }
+
+// Clears *flag when a node bound lies outside the range of the walk's
+// reciprocal slab divisions (lux::coord_in_range).
+__global__ void k_range_check(const float4* nodes, uint32_t n, uint32_t* flag) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = nodes[2ull * i], b = nodes[2ull * i + 1];
+    if (!(lux::coord_in_range(a.x) && lux::coord_in_range(a.y) && lux::coord_in_range(a.z) &&
+          lux::coord_in_range(b.x) && lux::coord_in_range(b.y) && lux::coord_in_range(b.z)))
+        *flag = 0u;
+}
+}  // namespace
+
+// Range flags of the built nodes (DevScene::range_ok): word 0 for the BLAS
+// nodes, word 1 for the TLAS nodes; stream-ordered, no host wait.
+static int range_check(lux_ctx* ctx, bool blas, bool tlas) {
+    if (!ctx->range_ok.ptr) {
+        if (int rc = lux_ensure(ctx, ctx->range_ok, 16)) return rc;
+        hipLaunchKernelGGL(k_range_flag, dim3(1), dim3(1), 0, ctx->stream, static_cast<uint32_t*>(ctx->range_ok.ptr), 0u);
+        hipLaunchKernelGGL(k_range_flag, dim3(1), dim3(1), 0, ctx->stream,
+                           static_cast<uint32_t*>(ctx->range_ok.ptr) + 1, 0u);
+    }
+    uint32_t* flags = static_cast<uint32_t*>(ctx->range_ok.ptr);
+    const bool which[2] = {blas, tlas};
+    const float4* nodes[2] = {static_cast<const float4*>(ctx->blas.ptr), static_cast<const float4*>(ctx->tlas.ptr)};
+    const uint32_t count[2] = {ctx->n_blas, ctx->n_tlas};
+    for (int k = 0; k < 2; ++k) {
+        if (!which[k]) continue;
+        hipLaunchKernelGGL(k_range_flag, dim3(1), dim3(1), 0, ctx->stream, flags + k, 1u);
+        if (count[k] && nodes[k])
+            hipLaunchKernelGGL(k_range_check, dim3(nblk(count[k], 256)), dim3(256), 0, ctx->stream, nodes[k], count[k],
+                               flags + k);
+    }
+    LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
 static int build_blas(lux_ctx* ctx, bool sah) {
     hipStream_t st = ctx->stream;
     uint32_t nt = ctx->n_tri;
@@ -760,6 +799,7 @@ static int build_blas(lux_ctx* ctx, bool sah) {
         if (e != hipSuccess) rc = lux_hip_check(ctx, e, "build_blas launch");
     }
     (void)hipStreamSynchronize(st);
+    if (rc == LUX_OK) rc = range_check(ctx, true, false);
     if (rc == LUX_OK) ctx->blas_built = true;
     return rc;
 }
@@ -773,7 +813,7 @@ static int build_tlas(lux_ctx* ctx) {
     if (int rc = lux_ensure(ctx, ctx->inst_order, 4ull * (ni ? ni : 1))) return rc;
     if (ni == 0) {
         ctx->tlas_built = true;
-        return LUX_OK;
+        return range_check(ctx, false, true);
     }
     // persistent temporaries (a rigid-motion frame rebuilds the TLAS every
     // frame: no allocation and no host wait here)
@@ -798,6 +838,7 @@ static int build_tlas(lux_ctx* ctx) {
                            static_cast<uint32_t*>(ctx->inst_order.ptr));
         if ((e = hipGetLastError()) != hipSuccess) rc = lux_hip_check(ctx, e, "build_tlas launch");
     } while (0);
+    if (rc == LUX_OK) rc = range_check(ctx, false, true);
     if (rc == LUX_OK) ctx->tlas_built = true;
     return rc;
 }
@@ -827,6 +868,7 @@ static int refit_tlas(lux_ctx* ctx) {
     hipLaunchKernelGGL(k_inst_records, dim3(nblk(ni, 128)), dim3(128), 0, st, d, order, ni, mb, mt, mc,
                        static_cast<float4*>(ctx->inst.ptr), static_cast<uint32_t*>(ctx->tlas_tmp.ptr) + 8ull * ni);
     LUX_HIP(ctx, hipGetLastError());
+    if (int rc = range_check(ctx, false, true)) return rc;
     ctx->tlas_built = true;
     return LUX_OK;
 }
@@ -948,6 +990,7 @@ extern "C" int lux_scene_upload_built(lux_ctx* ctx, const lux_bvh_node* blas, ui
     ctx->mesh_blas_base.clear();
     ctx->instances.clear();
     ctx->inst_material.clear();
+    if (int rc = range_check(ctx, true, true)) return rc;
     ctx->blas_built = true;
     ctx->tlas_built = true;
     ctx->tlas_topology = false;
@@ -1100,6 +1143,7 @@ extern "C" int lux_scene_upload_words(lux_ctx* ctx, const uint32_t* scene_words,
     ctx->mesh_tri_count.clear();
     ctx->mesh_tri_base.clear();
     ctx->mesh_blas_base.clear();
+    if (int rc = range_check(ctx, true, true)) return rc;
     ctx->blas_built = true;
     ctx->tlas_built = true;
     ctx->tlas_topology = false;

@@ -38,6 +38,10 @@ struct DevScene {
     const float4* blas;
     const float4* tri;  // 3 per triangle, interleaved: A (w = index in mesh), B (w = material), C
     uint32_t n_tlas, n_inst, n_blas, n_tri;
+    // [0] every BLAS node bound, [1] every TLAS node bound is zero or of
+    // magnitude in [2^-64, 2^39) (written after each build; null: unknown).
+    // With such bounds the resumable walk divides by reciprocals (div_fast).
+    const uint32_t* range_ok;
 };
 
 struct Node {
@@ -98,6 +102,56 @@ __device__ __forceinline__ bool box_interval(float3 o, float3 d, float3 mn, floa
 __device__ __forceinline__ float box_entry(float3 o, float3 d, const Node& n, float limit) {
     float e, x;
     return box_interval(o, d, n.mn, n.mx, limit, e, x) ? e : -1.0f;
+}
+
+// ---- the slab test's divisions from a per-ray reciprocal ----
+// For y = RN(1/d), q0 = RN(n*y) lies within one ulp of n/d, the residual
+// n - d*q0 is exact, and RN(q0 + r*y) = RN(n/d) (Markstein's theorem for
+// round-to-nearest), provided nothing underflows or overflows.  That holds
+// when the coordinates (node bounds, ray origin) are zero or of magnitude in
+// [2^-64, 2^39) and every direction component has magnitude in
+// [2^-60, 2^20): then n = bound - origin is zero or in [2^-87, 2^40), the
+// quotient in [2^-107, 2^100) and the residual on a grid no finer than
+// 2^-133.  For n = +-0 the quotient's zero sign may differ from the IEEE
+// quotient's; the slab test only compares such values, so its decisions,
+// entry distances' order and the hits are the same.
+__device__ __forceinline__ bool coord_in_range(float v) {
+    const uint32_t m = __float_as_uint(v) & 0x7fffffffu, e = m >> 23;
+    return m == 0u || (e >= 127u - 64u && e < 127u + 39u);
+}
+__device__ __forceinline__ bool dir_in_range(float v) {
+    const uint32_t e = (__float_as_uint(v) >> 23) & 255u;
+    return e >= 127u - 60u && e < 127u + 20u;
+}
+__device__ __forceinline__ bool ray_in_range(float3 o, float3 d) {
+    return coord_in_range(o.x) && coord_in_range(o.y) && coord_in_range(o.z) && dir_in_range(d.x) &&
+           dir_in_range(d.y) && dir_in_range(d.z);
+}
+__device__ __forceinline__ float div_fast(float n, float d, float y) {
+    const float q0 = n * y;
+    const float r = __builtin_fmaf(-q0, d, n);
+    return __builtin_fmaf(r, y, q0);
+}
+__device__ __forceinline__ float3 reciprocal(float3 d) {
+    return make_float3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+}
+
+// box_entry for a ray with nonzero, in-range direction components (the zero
+// branch of boxRayInterval cannot occur) and in-range coordinates; y = 1/d.
+__device__ __forceinline__ float box_entry_fast(float3 o, float3 d, float3 y, const Node& n, float limit) {
+    float near_ = 0.0f, far_ = limit;
+    const float os[3] = {o.x, o.y, o.z}, ds[3] = {d.x, d.y, d.z}, ys[3] = {y.x, y.y, y.z};
+    const float lo[3] = {n.mn.x, n.mn.y, n.mn.z}, hi[3] = {n.mx.x, n.mx.y, n.mx.z};
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        float a = div_fast(lo[axis] - os[axis], ds[axis], ys[axis]);
+        float b = div_fast(hi[axis] - os[axis], ds[axis], ys[axis]);
+        float ax = fmaxf(a, b);
+        near_ = fmaxf(near_, fminf(a, b));
+        float pad = fabsf(ax) * 3.5762791e-7f;
+        far_ = fminf(far_, ax + pad);
+    }
+    return near_ <= far_ ? near_ : -1.0f;
 }
 
 // orientedEdge (traversal.slang:379-392).
@@ -637,6 +691,10 @@ struct Walk {
     bool bready, binc;
     float3 lo, ld;
     TriRay ray;
+    // reciprocal slab divisions: scene bounds in range; world ray (TLAS) and
+    // local ray (current BLAS) in range, and their direction reciprocals
+    bool scene_fast, fast_t, fast_b;
+    float3 yt, yb;
 };
 
 template <class St>
@@ -655,7 +713,11 @@ __device__ __forceinline__ void walk_begin(const DevScene& s, Walk& w, float3 o,
     w.incomplete = false;
     w.tcount = 0;
     w.tsteps = 0;
+    w.fast_b = false;
     if (s.n_inst == 0) return;
+    w.scene_fast = s.range_ok != nullptr && s.range_ok[0] != 0u && s.range_ok[1] != 0u;
+    w.fast_t = w.scene_fast && ray_in_range(o, d);
+    if (w.fast_t) w.yt = reciprocal(d);
     w.node = load_node(s.tlas, s.n_tlas, 0);
     st.node(1);
     float re, rx;
@@ -701,8 +763,14 @@ __device__ __forceinline__ void walk_tlas(const DevScene& s, Walk& w, Stack& stk
             Node rn = load_node(s.tlas, s.n_tlas, ri);
             st.node(2);
             float lim = occl ? w.limit : w.h.t;
-            float le = box_entry(w.o, w.d, ln, lim);
-            float rre = box_entry(w.o, w.d, rn, lim);
+            float le, rre;
+            if (w.fast_t) {
+                le = box_entry_fast(w.o, w.d, w.yt, ln, lim);
+                rre = box_entry_fast(w.o, w.d, w.yt, rn, lim);
+            } else {
+                le = box_entry(w.o, w.d, ln, lim);
+                rre = box_entry(w.o, w.d, rn, lim);
+            }
             if (le >= 0.0f && rre >= 0.0f) {
                 if (le <= rre) {
                     stk.put(w.tcount, make_uint2(ri, occl ? 0u : __float_as_uint(rre)));
@@ -754,6 +822,8 @@ __device__ __forceinline__ void walk_instance(const DevScene& s, Walk& w, St& st
     w.gcount = g.w;
     w.lo = xform_point(i0, i1, i2, w.o);
     w.ld = xform_dir(i0, i1, i2, w.d);
+    w.fast_b = w.scene_fast && ray_in_range(w.lo, w.ld);
+    if (w.fast_b) w.yb = reciprocal(w.ld);
     st.inst();
     w.bcount = 0;
     w.node = load_node(s.blas, s.n_blas, g.y);
@@ -818,8 +888,14 @@ __device__ __forceinline__ void walk_blas(const DevScene& s, Walk& w, Stack& stk
             Node rn = load_node(s.blas, s.n_blas, ri);
             st.node(2);
             float lim = occl ? w.limit : w.h.t;
-            float le = box_entry(w.lo, w.ld, ln, lim);
-            float re = box_entry(w.lo, w.ld, rn, lim);
+            float le, re;
+            if (w.fast_b) {
+                le = box_entry_fast(w.lo, w.ld, w.yb, ln, lim);
+                re = box_entry_fast(w.lo, w.ld, w.yb, rn, lim);
+            } else {
+                le = box_entry(w.lo, w.ld, ln, lim);
+                re = box_entry(w.lo, w.ld, rn, lim);
+            }
             if (le >= 0.0f && re >= 0.0f) {
                 if (le <= re) {
                     stk.put(base + w.bcount, make_uint2(ri, occl ? 0u : __float_as_uint(re)));

@@ -40,6 +40,7 @@ struct lux_ctx {
     lux_buffer tlas;              // float4[2*n_tlas]
     lux_buffer inst;              // float4[8*n_inst]
     lux_buffer inst_order;        // uint32[n_inst]
+    lux_buffer range_ok;          // uint32[2]: BLAS / TLAS node bounds in the walk's division range
 
     // scratch
     lux_buffer scratch;

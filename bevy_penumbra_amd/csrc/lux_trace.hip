@@ -141,6 +141,46 @@ __global__ __launch_bounds__(kTraceWg) void k_trace_stats(TraceArgs a, unsigned 
 
 __global__ void k_zero1(uint32_t* p) { *p = 0u; }
 
+__device__ __forceinline__ uint32_t mix32(uint32_t v) {
+    v ^= v >> 16;
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return v;
+}
+
+// Self-check of the walk's reciprocal slab division: for `count` operand
+// pairs n (zero excluded, magnitude in [2^-87, 2^40)) and d (magnitude in
+// [2^-60, 2^20)) -- random significands, and every eighth pair with an
+// all-ones, all-zeros or one-ulp-from-either significand -- counts pairs
+// where lux::div_fast(n, d, 1/d) differs in any bit from the IEEE quotient.
+__global__ __launch_bounds__(256) void k_div_check(uint64_t count, uint32_t seed, unsigned long long* bad) {
+    unsigned long long local = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256u) {
+        const uint32_t h0 = mix32((uint32_t)i ^ mix32(seed)), h1 = mix32(h0 ^ (uint32_t)(i >> 32) ^ 0x9e3779b9u);
+        const uint32_t h2 = mix32(h1 + 0x632be5abu);
+        uint32_t mn = h0 & 0x7fffffu, md = h1 & 0x7fffffu;
+        switch (i & 7u) {
+        case 0: md = 0x7fffffu; break;
+        case 1: md = 0u; break;
+        case 2: md = 0x7ffffeu; break;
+        case 3: md = 1u; break;
+        case 4: mn = 0x7fffffu; break;
+        case 5: mn = 0u; md = 0x7fffffu; break;
+        default: break;
+        }
+        const uint32_t en = 127u - 87u + (h2 & 0xffffu) % 127u;  // 2^-87 .. 2^39
+        const uint32_t ed = 127u - 60u + (h2 >> 16) % 80u;       // 2^-60 .. 2^19
+        const float n = __uint_as_float((h0 & 0x80000000u) | (en << 23) | mn);
+        const float d = __uint_as_float((h1 & 0x80000000u) | (ed << 23) | md);
+        const float y = 1.0f / d;
+        if (__float_as_uint(lux::div_fast(n, d, y)) != __float_as_uint(n / d)) ++local;
+    }
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
+    if ((threadIdx.x & 63u) == 0u && local) atomicAdd(bad, local);
+}
+
 }  // namespace
 
 // ---- scratch of the allocation-free trace (lux_trace_rays_device_noalloc) ----
@@ -177,6 +217,7 @@ lux::DevScene lux_ctx::dev_scene() const {
     s.n_inst = n_inst;
     s.n_blas = n_blas;
     s.n_tri = n_tri;
+    s.range_ok = static_cast<const uint32_t*>(range_ok.ptr);
     return s;
 }
 
@@ -295,6 +336,20 @@ extern "C" int lux_trace_stats_device(lux_ctx* ctx, const lux_ray* rays, uint32_
     hipLaunchKernelGGL(k_trace_stats, dim3((n + kTraceWg - 1) / kTraceWg), dim3(kTraceWg), 0,
                        ctx->stream, a, counters);
     LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
+extern "C" int lux_check_reciprocal_division(lux_ctx* ctx, uint64_t pairs, uint32_t seed, uint64_t* mismatches) {
+    if (!ctx || !mismatches) return LUX_ERR_INVALID;
+    if (int rc = lux_ensure(ctx, ctx->scratch, 256)) return rc;
+    unsigned long long* bad = static_cast<unsigned long long*>(ctx->scratch.ptr) + 16;
+    LUX_HIP(ctx, hipMemsetAsync(bad, 0, 8, ctx->stream));
+    hipLaunchKernelGGL(k_div_check, dim3((uint32_t)ctx->cu_count * 32u), dim3(256), 0, ctx->stream, pairs, seed, bad);
+    LUX_HIP(ctx, hipGetLastError());
+    unsigned long long host = 0;
+    LUX_HIP(ctx, hipMemcpyAsync(&host, bad, 8, hipMemcpyDeviceToHost, ctx->stream));
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *mismatches = host;
     return LUX_OK;
 }
 

@@ -385,6 +385,7 @@ def load_library() -> C.CDLL:
         "lux_trace_rays": (i32, [vp, vp, u32, vp]),
         "lux_trace_rays_device": (i32, [vp, vp, u32, vp, vp]),
         "lux_trace_rays_device_noalloc": (i32, [vp, vp, u32, vp, vp]),
+        "lux_check_reciprocal_division": (i32, [vp, C.c_uint64, u32, C.POINTER(C.c_uint64)]),
         "lux_trace_stats_device": (i32, [vp, vp, u32, vp, vp]),
         "lux_last_kernel_ms": (i32, [vp, C.POINTER(C.c_float)]),
         "lux_set_trace_tuning": (i32, [vp, u32, u32]),
@@ -682,6 +683,13 @@ class Context:
     def trace_device(self, rays, n: int, hits, bary=None):
         self._check(self.lib.lux_trace_rays_device(self.h, _ptr(rays), n, _ptr(hits), _ptr(bary)),
                     "lux_trace_rays_device")
+
+    def check_reciprocal_division(self, pairs: int, seed: int = 0) -> int:
+        """Mismatches of the walk's reciprocal slab division against IEEE division."""
+        bad = C.c_uint64()
+        self._check(self.lib.lux_check_reciprocal_division(self.h, pairs, seed, C.byref(bad)),
+                    "lux_check_reciprocal_division")
+        return int(bad.value)
 
     def trace_device_noalloc(self, rays, n: int, hits, bary=None):
         """lux_trace_rays_device_noalloc: the same trace with no allocation."""

@@ -240,6 +240,15 @@ int lux_trace_rays_device(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit
 int lux_trace_rays_device_noalloc(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_hit* hits,
                                   float* bary);
 
+/* Self-check of the traversal's slab division (no reference counterpart):
+ * the resumable walk divides by a per-ray reciprocal with one correction
+ * (Markstein) when the scene's node bounds and the ray lie in the range where
+ * that quotient is the correctly rounded one; this counts, over `pairs`
+ * seeded operand pairs of that range (random and all-ones / all-zeros /
+ * near-boundary significands), those whose quotient differs from the IEEE
+ * division in any bit.  Synchronous. */
+int lux_check_reciprocal_division(lux_ctx* ctx, uint64_t pairs, uint32_t seed, uint64_t* mismatches);
+
 /* Instrumented trace (same results): counters[0] += node fetches,
  * counters[1] += triangle tests, counters[2] += instance visits, summed over
  * all rays.  counters is a device pointer to 3 uint64 (zeroed by caller). */

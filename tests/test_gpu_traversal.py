@@ -298,3 +298,34 @@ def test_gpu_sah_median_keeps_partition_order(lux_ctx, group):
     assert_same_scene(lux_ctx, built)
     want_perm, _ = _model_build_tree(lo, hi)
     assert lux_ctx.download_bvh()["tri_order"].tolist() == want_perm
+
+
+def test_gpu_reciprocal_slab_division_is_correctly_rounded(lux_ctx):
+    """The resumable walk's slab divisions by a per-ray reciprocal with one
+    FMA correction equal IEEE division bit for bit over 2^34 operand pairs of
+    the range the walk uses them in (random and adversarial significands)."""
+    assert lux_ctx.check_reciprocal_division(1 << 34, seed=1) == 0
+    assert lux_ctx.check_reciprocal_division(1 << 30, seed=2) == 0
+
+
+def test_gpu_walk_out_of_range_scene_takes_exact_division(lux_ctx):
+    """A scene whose coordinates leave the reciprocal division's range (a
+    mesh scaled to 2^45) and rays with zero or tiny direction components run
+    the walk's exact IEEE path: hits equal the oracle bit for bit."""
+    sc = scenes.traversal_fixture()
+    big = [m.astype(np.float64) * 2.0 ** 45 for m in sc.meshes]
+    sc.meshes = [m.astype(np.float32) for m in big]
+    rows = sc.inst_fwd.reshape(-1, 3, 4).astype(np.float64)
+    rows[:, :, 3] *= 2.0 ** 45
+    sc.inst_fwd = rows.reshape(-1, 12).astype(np.float32)
+    inv = sc.inst_inv.reshape(-1, 3, 4).astype(np.float64)
+    inv[:, :, 3] *= 2.0 ** 45
+    sc.inst_inv = inv.reshape(-1, 12).astype(np.float32)
+    rays = scenes.fixture_rays(scenes.traversal_fixture()).copy()
+    rays[:, :3] *= np.float32(2.0 ** 45)
+    rays[:, 3] *= np.float32(2.0 ** 45)
+    rays[::7, 4] = 0.0                                              # zero direction components
+    rays[1::7, 5] = np.float32(1e-30)
+    lux_ctx.upload_scene(sc)
+    lux_ctx.build_bvh()
+    assert np.array_equal(trace_gpu(lux_ctx, rays), O.build_scene(sc).trace(rays))
