@@ -70,14 +70,16 @@ def parse():
 TRAFFIC_PROFILE = ROOT / "profiles" / "pmc_traffic_colonnade_trace_and_gi.md"
 
 
-def measured_traffic(kernel: str, cfg: dict):
+def measured_traffic(kernel: str, cfg: dict, gi: bool = False):
     """HBM-side bytes per launch of `kernel` from the committed counter profile,
-    when it was taken with this traversal configuration and BLAS builder (the
-    profile's `config` line; `blas` absent means Morton-30); else None."""
+    when it was taken with this launch configuration (the traversal's, or with
+    gi=True the GI stages') and BLAS builder (the profile's `config` /
+    `gi_config` line; `blas` absent means Morton-30); else None."""
     if not TRAFFIC_PROFILE.exists():
         return None
     lines = TRAFFIC_PROFILE.read_text().splitlines()
-    conf = next((l for l in lines if l.startswith("config ")), None)
+    tag = "gi_config " if gi else "config "
+    conf = next((l for l in lines if l.startswith(tag)), None)
     if conf is None:
         return None
     want = dict(kv.split("=") for kv in conf.split()[1:])
@@ -91,8 +93,105 @@ def measured_traffic(kernel: str, cfg: dict):
     return None
 
 
+def gi_roofline(work: dict, intervals: int, probes: int, gp, update_ms: float, stage_ms: dict, cfg: dict,
+                args) -> dict:
+    """Roofline of the GI update's dominant kernel, the primary walks
+    (k_gi_primary): algorithmic bytes = per walk its 16-byte hit record plus
+    32 B per node fetch, 36 B per triangle test and 48 B per instance visit
+    (the walk generates its ray from the interval index, so no ray is read),
+    over the primary stage's HIP-event duration.  The whole update's
+    algorithmic bytes add the occlusion walks (32 B ray + 4 B visibility + the
+    same traversal bytes), 32 B per interval deposit, 544 B per merged interval
+    with children (J, beta; 32 child (I, support); out), 32 B per top-level
+    interval and 1,088 B per packed level-0 probe."""
+    prim = (16 * work["primary_walks"] + 32 * work["primary_nodes"] + 36 * work["primary_tris"]
+            + 48 * work["primary_insts"])
+    occl = (36 * work["occlusion_rays"] + 32 * work["occlusion_nodes"] + 36 * work["occlusion_tris"]
+            + 48 * work["occlusion_insts"])
+    dims = list(gp.dims)
+    top = 1
+    for a in range(3):
+        d = dims[a]
+        for _ in range(gp.levels - 1):
+            d = (d + 1) // 2
+        top *= d
+    top *= (gp.base_phi << (gp.levels - 1)) * (gp.base_y << (gp.levels - 1))
+    total = prim + occl + 32 * intervals + 544 * (intervals - top) + 32 * top + 1088 * probes
+    achieved = prim / (stage_ms["primary"] * 1e-3) / 1e9
+    kernel = "k_gi_primary<%d>" % cfg["lds_stack_depth"]
+    return {
+        "kernel": kernel, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": measured_traffic(kernel, dict(cfg, blas=args.blas), gi=True),
+        "algorithmic_bytes": prim,
+        "occlusion_walks": {"kernel": "k_gi_shadow<%d>" % cfg["lds_stack_depth"], "algorithmic_bytes": occl,
+                            "achieved": round(occl / (stage_ms["occlusion"] * 1e-3) / 1e9, 2)},
+        "update": {"algorithmic_bytes": total, "achieved": round(total / (update_ms * 1e-3) / 1e9, 2),
+                   "frac": round(total / (update_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
+    }
+
+
 def blas_mode(lux, args) -> int:
     return lux.LUX_BVH_SAH if args.blas == "sah" else lux.LUX_BVH_MORTON30
+
+
+def cpu_baseline(args, rays_np) -> dict:
+    """The oracle (C restatement of the reference CPU algorithms) timed on
+    this host in a child process (oracle/cpu_baseline.py), built for this CPU
+    (`make -C oracle native`: -O3 -march=native -ffp-contract=off); its hits
+    of the first rays must equal the portable build's bit for bit.  Falls
+    back to the portable build if the native one cannot be built."""
+    import subprocess
+    import tempfile
+    from oracle import oracle as O
+    tmp = Path(tempfile.mkdtemp(prefix="lux_cpu_"))
+    env = dict(os.environ)
+    build = "native (-O3 -march=native -ffp-contract=off)"
+    r = subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "native", f"NATIVE_DIR={tmp}"],
+                       capture_output=True, text=True)
+    if r.returncode == 0:
+        env["LUX_ORACLE_LIB"] = str(tmp / "liblux_oracle.so")
+    else:
+        build = "portable (-O2; the native build failed)"
+    m = min(args.cpu_sample, rays_np.shape[0])
+    hits_path = tmp / "hits.npy"
+    cmd = [sys.executable, str(ROOT / "oracle" / "cpu_baseline.py"), "--rays", str(m),
+           "--rays-one-thread", str(max(m // 16, 2)), "--hits", str(hits_path)]
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        return {"error": r.stderr[-400:]}
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    half = rays_np.shape[0] // 2
+    sample = np.concatenate([rays_np[: m // 2], rays_np[half: half + m // 2]])[:65536]
+    same = bool(np.array_equal(np.load(hits_path), O.build_scene(scenes_colonnade(), sah=True).trace(sample)))
+    lscpu = ""
+    try:
+        lscpu = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+    except OSError:
+        pass
+    model = next((l.split(":", 1)[1].strip() for l in lscpu.splitlines() if l.startswith("Model name")), "")
+    cpus = next((l.split(":", 1)[1].strip() for l in lscpu.splitlines() if l.startswith("CPU(s)")), "")
+    tr, gc, g2 = res["traversal"], res["gi_cornell"], res["gi_colonnade"]
+    return {
+        "value": tr["all_cores"]["mrays_s"],
+        "unit": "Mrays/s",
+        "cores": tr["all_cores"]["threads"],
+        "kind": "port",
+        "sample": f"{tr['all_cores']['rays']} rays of the same set (first half closest-hit, first half any-hit), "
+                  f"SAH build_tree BVH, C restatement of the reference CPU traversal, {build}, "
+                  f"{tr['all_cores']['threads']} threads = every core this process may use "
+                  f"(lscpu: {cpus} CPUs, {model})",
+        "one_thread": tr["one_thread"],
+        "native_hits_equal_portable": same,
+        "gi_cornell": gc,
+        "gi_colonnade": g2,
+        "gi_colonnade_ms_per_update": g2["all_cores"]["ms_per_update_est"],
+    }
+
+
+def scenes_colonnade():
+    from bevy_penumbra_amd import scenes
+    return scenes.colonnade()[0]
 
 
 def pmc_workload(lux, scenes, args) -> int:
@@ -747,8 +846,25 @@ def main() -> int:
         ctx.synchronize()
         gi_ms = [ctx.gi_update(timed=True) for _ in range(args.gi_steps)]
         ctx.synchronize()
+        # stage durations (HIP events between the stages, separate updates so
+        # the timed ones above carry no extra synchronisation) and the
+        # traversal work of one instrumented trace of the same grid
+        ctx.gi_set_stage_timing(True)
+        stages = []
+        for _ in range(args.gi_steps):
+            ctx.gi_update()
+            ctx.synchronize()
+            stages.append(ctx.gi_stage_times())
+        ctx.gi_set_stage_timing(False)
+        stage_ms = {k: round(float(np.median([x[k] for x in stages])), 4) for k in stages[0]}
+        work = ctx.gi_trace_stats()
+        gi_roof = gi_roofline(work, n_int, n_probes, gp, float(np.median(gi_ms)), stage_ms,
+                              ctx.gi_trace_config(), args)
         gi = {
             "ms_per_gi_update": round(float(np.median(gi_ms)), 4),
+            "stage_ms": stage_ms,
+            "traversal_work": work,
+            "roofline": gi_roof,
             "intervals": n_int,
             "interval_mrays_s": round(n_int / (float(np.median(gi_ms)) * 1e-3) / 1e6, 3),
             "probes": n_probes,
@@ -763,24 +879,9 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        threads = min(16, os.cpu_count() or 1)
-        built = O.build_scene(scene, sah=True)
-        m = min(args.cpu_sample, n)
-        half = n // 2
-        sample = np.concatenate([rays_np[: m // 2], rays_np[half: half + m // 2]])
-        t0 = time.perf_counter()
-        built.trace(sample, threads=threads)
-        dt = time.perf_counter() - t0
-        cpu = {
-            "value": round(sample.shape[0] / dt / 1e6, 4),
-            "unit": "Mrays/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{sample.shape[0]} rays of the same set (first {m // 2} closest-hit and "
-                      f"first {m // 2} any-hit), SAH build_tree BVH, C restatement of the "
-                      f"reference CPU traversal, {threads} threads",
-        }
+        cpu = cpu_baseline(args, rays_np)
+        if gi is not None and cpu.get("gi_colonnade_ms_per_update"):
+            gi["cpu_baseline"] = {k: cpu.pop(k) for k in list(cpu) if k.startswith("gi_")}
 
     out = {
         "metric": "interval Mrays/s + ms/GI update; surrogate vs plain trace at equal RMSE",

@@ -270,6 +270,12 @@ extern "C" int lux_ctx_create(int device, lux_ctx** out) {
             lux_ctx_destroy(ctx);
             return LUX_ERR_HIP;
         }
+    for (hipEvent_t& e : ctx->ev_stage)
+        if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+            lux_ctx_destroy(ctx);
+            return LUX_ERR_HIP;
+        }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->cu_count = prop.multiProcessorCount;
@@ -297,9 +303,11 @@ extern "C" void lux_ctx_destroy(lux_ctx* ctx) {
                           &ctx->rx_keys, &ctx->rx_flags, &ctx->rx_slot, &ctx->rx_lod, &ctx->rx_ord,
                           &ctx->rx_seq, &ctx->rx_dir, &ctx->rx_tickets, &ctx->rx_dep_val, &ctx->rx_pack, &ctx->rx_traced,
                           &ctx->rx_sort, &ctx->rx_scan, &ctx->rx_counters, &ctx->rx_cold,
-                          &ctx->screen_cache, &ctx->range_ok};
+                          &ctx->screen_cache, &ctx->range_ok, &ctx->gi_stats_buf};
     for (lux_buffer* b : bufs) lux_release(*b);
     for (hipEvent_t e : ctx->ev_phase)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->ev_stage)
         if (e) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

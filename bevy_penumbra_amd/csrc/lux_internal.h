@@ -75,6 +75,15 @@ struct lux_ctx {
     lux_buffer gi_w_prim, gi_w_shade, gi_w_slots, gi_w_vis, gi_w_shadow, gi_w_counters;
     lux_buffer mesh_info;    // uint32 per mesh: BLAS base | triangle base | count
     hipEvent_t ev_phase[5] = {};
+    // opt-in stage timing of the wavefront trace (lux_gi_set_stage_timing):
+    // primary walks, setup + occlusion-ray packing, occlusion walks,
+    // accumulation, summed over the chunks of the last traced update
+    bool gi_stage_timing = false;
+    hipEvent_t ev_stage[5] = {};
+    float gi_stage_ms[4] = {};
+    // instrumented pass (lux_gi_trace_stats): device counters, occlusion rays
+    lux_buffer gi_stats_buf;
+    uint64_t gi_stats_rays = 0;
     // temporal history of raw intervals: on when a blend is positive; sample
     // counts, two float4 per interval, state words (valid, dirty boxes)
     bool blas_sah = false;   // the built BLASes came from the host SAH build

@@ -180,7 +180,23 @@ struct Wave {
     uint32_t count, frame;
     float4* rx_out;
     uint32_t* rx_status;
+    // instrumented pass (lux_gi_trace_stats): node fetches, triangle tests and
+    // instance visits of the primary walks [0..2] and occlusion walks [3..5]
+    unsigned long long* stats;
 };
+
+// Adds a wave's traversal work counts to w.stats[off..off+2] (the whole wave
+// calls it; NoStats: nothing).
+__device__ __forceinline__ void stats_flush(const lux::NoStats&, unsigned long long*, uint32_t) {}
+__device__ __forceinline__ void stats_flush(const lux::Counting& st, unsigned long long* out, uint32_t off) {
+    unsigned long long v[3] = {st.nodes, st.tris, st.insts};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        unsigned long long x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (__lane_id() == 0u) atomicAdd(&out[off + k], x);
+    }
+}
 
 // Intervals of the chunk (kWaveListed: listed probes beyond the list's
 // device-side length contribute none).
@@ -250,15 +266,16 @@ struct PrimarySource {
     }
 };
 
-template <int Depth>
+template <int Depth, class St = lux::NoStats>
 __global__ __launch_bounds__(kWg) void k_gi_primary(GiArgs a, Wave w) {
     __shared__ uint2 lds[Depth][kWg];
     lux::LaneStack<Depth, kWg> stk;
     stk.col = &lds[0][threadIdx.x];
-    lux::NoStats st;
+    St st;
     PrimarySource src{&a, &w, wave_intervals(a, w) * w.R, 0u};
     lux::regroup_walks(a.scene, src, &w.counters[0], w.regroup, w.batch, stk, st);
     query_tally_flush(a, src.tally);
+    stats_flush(st, w.stats, 0u);
 }
 
 __global__ __launch_bounds__(kWg) void k_gi_setup(GiArgs a, Wave w) {
@@ -332,15 +349,16 @@ struct ShadowSource {
     }
 };
 
-template <int Depth>
+template <int Depth, class St = lux::NoStats>
 __global__ __launch_bounds__(kWg) void k_gi_shadow(GiArgs a, Wave w) {
     __shared__ uint2 lds[Depth][kWg];
     lux::LaneStack<Depth, kWg> stk;
     stk.col = &lds[0][threadIdx.x];
-    lux::NoStats st;
+    St st;
     ShadowSource src{&a, &w, w.counters[1], 0u};
     lux::regroup_walks(a.scene, src, &w.counters[2], w.regroup, w.batch, stk, st);
     query_tally_flush(a, src.tally);
+    stats_flush(st, w.stats, 3u);
 }
 
 __global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
@@ -981,25 +999,54 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
         if (int rc = lux_device_zero(ctx, w.counters, 16)) return rc;
         uint32_t need = (max_items + kWg - 1) / kWg;
         uint32_t pb = pblocks < need ? pblocks : need;
-        switch (ctx->gi_depth) {
-        case 4: hipLaunchKernelGGL(k_gi_primary<4>, dim3(pb), dim3(kWg), 0, st, a, w); break;
-        case 16: hipLaunchKernelGGL(k_gi_primary<16>, dim3(pb), dim3(kWg), 0, st, a, w); break;
-        default: hipLaunchKernelGGL(k_gi_primary<8>, dim3(pb), dim3(kWg), 0, st, a, w); break;
+        const bool timing = ctx->gi_stage_timing && !w.stats;
+        if (timing) LUX_HIP(ctx, hipEventRecord(ctx->ev_stage[0], st));
+        if (w.stats) {
+            hipLaunchKernelGGL((k_gi_primary<8, lux::Counting>), dim3(pb), dim3(kWg), 0, st, a, w);
+        } else {
+            switch (ctx->gi_depth) {
+            case 4: hipLaunchKernelGGL(k_gi_primary<4>, dim3(pb), dim3(kWg), 0, st, a, w); break;
+            case 16: hipLaunchKernelGGL(k_gi_primary<16>, dim3(pb), dim3(kWg), 0, st, a, w); break;
+            default: hipLaunchKernelGGL(k_gi_primary<8>, dim3(pb), dim3(kWg), 0, st, a, w); break;
+            }
         }
+        if (timing) LUX_HIP(ctx, hipEventRecord(ctx->ev_stage[1], st));
         hipLaunchKernelGGL(k_gi_setup, dim3(need), dim3(kWg), 0, st, a, w);
         const uint32_t nw = need * (kWg / 64u);
         hipLaunchKernelGGL(k_fill_u32_one, dim3(1), dim3(1), 0, st, w.wave_cnt + nw, 0u);
         if (int rc = lux_scan_exclusive_on(ctx, st, w.wave_cnt, nw + 1u, scan_sums, &w.counters[1])) return rc;
         hipLaunchKernelGGL(k_shadow_compact, dim3((nw + 3u) / 4u), dim3(256), 0, st, w.slab, w.wave_cnt, nw,
                            64u * w.nee, w.shadow);
-        switch (ctx->gi_depth) {
-        case 4: hipLaunchKernelGGL(k_gi_shadow<4>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
-        case 16: hipLaunchKernelGGL(k_gi_shadow<16>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
-        default: hipLaunchKernelGGL(k_gi_shadow<8>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
+        if (timing) LUX_HIP(ctx, hipEventRecord(ctx->ev_stage[2], st));
+        if (w.stats) {
+            hipLaunchKernelGGL((k_gi_shadow<8, lux::Counting>), dim3(pblocks), dim3(kWg), 0, st, a, w);
+        } else {
+            switch (ctx->gi_depth) {
+            case 4: hipLaunchKernelGGL(k_gi_shadow<4>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
+            case 16: hipLaunchKernelGGL(k_gi_shadow<16>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
+            default: hipLaunchKernelGGL(k_gi_shadow<8>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
+            }
         }
+        if (timing) LUX_HIP(ctx, hipEventRecord(ctx->ev_stage[3], st));
         if (rx) hipLaunchKernelGGL(k_rx_wave_finish, dim3((max_intervals + kWg - 1) / kWg), dim3(kWg), 0, st, a, w);
         else hipLaunchKernelGGL(k_gi_accum, dim3((max_intervals + kWg - 1) / kWg), dim3(kWg), 0, st, a, w);
         LUX_HIP(ctx, hipGetLastError());
+        if (timing) {
+            LUX_HIP(ctx, hipEventRecord(ctx->ev_stage[4], st));
+            LUX_HIP(ctx, hipEventSynchronize(ctx->ev_stage[4]));
+            for (int k = 0; k < 4; ++k) {
+                float ms = 0.0f;
+                LUX_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_stage[k], ctx->ev_stage[k + 1]));
+                ctx->gi_stage_ms[k] += ms;
+            }
+        }
+        if (w.stats) {
+            // occlusion rays of the chunk (the walks' count), accumulated on the host
+            uint32_t rays = 0;
+            LUX_HIP(ctx, hipMemcpyAsync(&rays, &w.counters[1], 4, hipMemcpyDeviceToHost, st));
+            LUX_HIP(ctx, hipStreamSynchronize(st));
+            ctx->gi_stats_rays += rays;
+        }
     }
     return LUX_OK;
 }
@@ -1156,12 +1203,52 @@ extern "C" int lux_gi_trace_levels(lux_ctx* ctx, uint32_t first, uint32_t last) 
     if (first > last || last >= ctx->gi.levels)
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_trace_levels: level range outside the grid");
     uint64_t g0 = ctx->gi_offset[first], g1 = ctx->gi_offset[last + 1];
+    for (float& t : ctx->gi_stage_ms) t = 0.0f;
     if (g1 == g0) return LUX_OK;
     if (ctx->gi_trace_mode == 0u) return trace_wavefront(ctx, g0, g1);
     GiArgs a = gi_args(ctx);
     uint32_t blocks = (uint32_t)((g1 - g0 + kWg - 1) / kWg);
     hipLaunchKernelGGL(k_gi_trace, dim3(blocks), dim3(kWg), 0, ctx->stream, a, g0, g1);
     LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
+extern "C" int lux_gi_set_stage_timing(lux_ctx* ctx, int on) {
+    if (!ctx) return LUX_ERR_INVALID;
+    ctx->gi_stage_timing = on != 0;
+    return LUX_OK;
+}
+
+extern "C" int lux_gi_stage_times(lux_ctx* ctx, float* ms) {
+    if (!ctx || !ms) return LUX_ERR_INVALID;
+    for (int k = 0; k < 4; ++k) ms[k] = ctx->gi_stage_ms[k];
+    return LUX_OK;
+}
+
+extern "C" int lux_gi_trace_stats(lux_ctx* ctx, uint64_t* counters) {
+    if (!ctx || !counters) return LUX_ERR_INVALID;
+    if (int rc = gi_check(ctx)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->gi_stats_buf, 64)) return rc;
+    LUX_HIP(ctx, hipMemsetAsync(ctx->gi_stats_buf.ptr, 0, 48, ctx->stream));
+    if (int rc = lux_device_zero(ctx, ctx->gi_counters.ptr, 8)) return rc;
+    ctx->gi_stats_rays = 0;
+    Wave w = wave_proto(kWaveRange);
+    w.g0 = 0;
+    w.g1 = ctx->gi_total;
+    w.stats = static_cast<unsigned long long*>(ctx->gi_stats_buf.ptr);
+    if (int rc = run_wave(ctx, w, ctx->gi_total, 1)) return rc;
+    unsigned long long v[6] = {};
+    LUX_HIP(ctx, hipMemcpyAsync(v, ctx->gi_stats_buf.ptr, 48, hipMemcpyDeviceToHost, ctx->stream));
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint64_t R = ctx->gi.rays_per_interval ? ctx->gi.rays_per_interval : 1u;
+    counters[0] = ctx->gi_total * R;
+    counters[1] = v[0];
+    counters[2] = v[1];
+    counters[3] = v[2];
+    counters[4] = ctx->gi_stats_rays;
+    counters[5] = v[3];
+    counters[6] = v[4];
+    counters[7] = v[5];
     return LUX_OK;
 }
 

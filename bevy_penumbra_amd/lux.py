@@ -390,6 +390,9 @@ def load_library() -> C.CDLL:
         "lux_last_kernel_ms": (i32, [vp, C.POINTER(C.c_float)]),
         "lux_set_trace_tuning": (i32, [vp, u32, u32]),
         "lux_gi_configure": (i32, [vp, vp]),
+        "lux_gi_set_stage_timing": (i32, [vp, i32]),
+        "lux_gi_stage_times": (i32, [vp, vp]),
+        "lux_gi_trace_stats": (i32, [vp, vp]),
         "lux_gi_set_frame": (i32, [vp, u32, u32]),
         "lux_gi_trace": (i32, [vp]),
         "lux_gi_merge": (i32, [vp]),
@@ -734,6 +737,23 @@ class Context:
     def gi_configure(self, params: GiParams):
         self._gi = params
         self._check(self.lib.lux_gi_configure(self.h, C.byref(params)), "lux_gi_configure")
+
+    def gi_set_stage_timing(self, on: bool):
+        self._check(self.lib.lux_gi_set_stage_timing(self.h, 1 if on else 0), "lux_gi_set_stage_timing")
+
+    def gi_stage_times(self) -> dict:
+        """Milliseconds of the last traced update's wavefront stages (stage timing on)."""
+        ms = (C.c_float * 4)()
+        self._check(self.lib.lux_gi_stage_times(self.h, ms), "lux_gi_stage_times")
+        return dict(primary=ms[0], setup_pack=ms[1], occlusion=ms[2], accumulate=ms[3])
+
+    def gi_trace_stats(self) -> dict:
+        """Traversal work of one instrumented interval trace (lux_gi_trace_stats)."""
+        c = (C.c_uint64 * 8)()
+        self._check(self.lib.lux_gi_trace_stats(self.h, c), "lux_gi_trace_stats")
+        keys = ("primary_walks", "primary_nodes", "primary_tris", "primary_insts",
+                "occlusion_rays", "occlusion_nodes", "occlusion_tris", "occlusion_insts")
+        return {k: int(v) for k, v in zip(keys, c)}
 
     def gi_set_frame(self, frame: int, seed: int):
         self._check(self.lib.lux_gi_set_frame(self.h, frame, seed), "lux_gi_set_frame")

@@ -300,6 +300,18 @@ int lux_gi_merge(lux_ctx* ctx);
 int lux_gi_pack(lux_ctx* ctx);
 /* trace + merge + pack; *ms (nullable) = GPU time of the three phases. */
 int lux_gi_update(lux_ctx* ctx, float* ms);
+/* Opt-in stage timing of the wavefront interval trace (HIP events between
+ * the stages; each chunk synchronizes): lux_gi_stage_times gives the last
+ * traced update's milliseconds of {primary walks, NEE setup + occlusion-ray
+ * packing, occlusion walks, accumulation}.  No reference counterpart. */
+int lux_gi_set_stage_timing(lux_ctx* ctx, int on);
+int lux_gi_stage_times(lux_ctx* ctx, float* ms);
+/* One instrumented interval trace of the configured grid (wavefront, the
+ * same results as lux_gi_trace; synchronous), counting the work that
+ * defines its algorithmic bytes: counters[0] primary walks, [1..3] their
+ * node fetches, triangle tests and instance visits, [4] occlusion rays,
+ * [5..7] their node fetches, triangle tests and instance visits. */
+int lux_gi_trace_stats(lux_ctx* ctx, uint64_t* counters);
 /* Total intervals of all levels and level-0 probe count. */
 int lux_gi_sizes(lux_ctx* ctx, uint64_t* intervals, uint32_t* probes, uint32_t* lights);
 /* Host copies (each nullable): intervals/merged float[4*intervals],

@@ -210,6 +210,9 @@ void orc_cos_sin_2pi(float u, float* c, float* s);
 void orc_gi_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                   const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                   float* intervals, int threads);
+void orc_gi_trace_stride(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
+                         const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
+                         float* intervals, int threads, uint64_t stride);
 void orc_gi_merge(const orc_gi_params* p, const float* intervals, float* merged);
 /* Temporal history (mergeEntry's history part): settings and state. */
 typedef struct {

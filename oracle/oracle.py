@@ -6,6 +6,7 @@ module; the product path (bevy_penumbra_amd) never does.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -17,12 +18,15 @@ _lib = None
 
 
 def load() -> C.CDLL:
+    """The oracle library: oracle/liblux_oracle.so, or the build named by the
+    LUX_ORACLE_LIB environment variable (the CPU baseline's host-tuned build)."""
     global _lib
     if _lib is not None:
         return _lib
-    if not LIB.exists():
+    path = Path(os.environ["LUX_ORACLE_LIB"]) if os.environ.get("LUX_ORACLE_LIB") else LIB
+    if not path.exists():
         subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
-    lib = C.CDLL(str(LIB))
+    lib = C.CDLL(str(path))
     vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int
     sig = {
         "orc_box_interval": (i32, [vp, vp, vp, vp, C.c_float, vp, vp]),
@@ -261,6 +265,7 @@ def _tsig():
         ("orc_bin_direction", None, [u32, u32, u32, C.c_float, C.c_float, vp]),
         ("orc_cos_sin_2pi", None, [C.c_float, vp, vp]),
         ("orc_gi_trace", None, [vp, vp, u32, vp, u32, vp, vp, i32]),
+        ("orc_gi_trace_stride", None, [vp, vp, u32, vp, u32, vp, vp, i32, C.c_uint64]),
         ("orc_gi_trace_f64", None, [vp, vp, u32, vp, u32, vp, vp, i32]),
         ("orc_gi_merge", None, [vp, vp, vp]),
         ("orc_gi_trace_samples", None, [vp, vp, u32, vp, u32, vp, vp, vp, i32]),
@@ -382,6 +387,18 @@ def gi_trace(built: BuiltScene, lights: np.ndarray, materials: np.ndarray, p: Gi
     fn = lib.orc_gi_trace_f64 if f64 else lib.orc_gi_trace
     fn(C.byref(built._s), lights.ctypes.data, lights.shape[0], mats.ctypes.data, mats.shape[0],
        C.byref(p), out.ctypes.data, threads)
+    return out
+
+
+def gi_trace_stride(built: BuiltScene, lights: np.ndarray, materials: np.ndarray, p: GiParams, stride: int,
+                    threads: int = 8) -> np.ndarray:
+    """Intervals (n, 4) with only every stride-th one traced (the rest zero)."""
+    lib = _tsig()
+    lights = np.ascontiguousarray(lights, np.float32)
+    mats = np.ascontiguousarray(materials, np.float32)
+    out = np.zeros((total_intervals(p), 4), np.float32)
+    lib.orc_gi_trace_stride(C.byref(built._s), lights.ctypes.data, lights.shape[0], mats.ctypes.data,
+                            mats.shape[0], C.byref(p), out.ctypes.data, threads, stride)
     return out
 
 

@@ -387,17 +387,24 @@ void orc_lod_select(const orc_lod_view* v, const float* position, uint32_t* low,
 void orc_gi_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                   const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                   float* intervals, int threads) {
-    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads);
+    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads, 1);
+}
+/* Every stride-th interval (g = 0, stride, 2 stride, ...) only; the others
+ * are left as they are (CPU timing of a bounded fraction of an update). */
+void orc_gi_trace_stride(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
+                         const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
+                         float* intervals, int threads, uint64_t stride) {
+    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads, stride);
 }
 void orc_gi_trace_f64(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                       const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                       double* intervals, int threads) {
-    gi_trace_f64(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads);
+    gi_trace_f64(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads, 1);
 }
 void orc_gi_trace_samples(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                           const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                           float* intervals, uint32_t* samples, int threads) {
-    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, samples, threads);
+    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, samples, threads, 1);
 }
 void orc_gi_merge_history(const orc_gi_params* p, const float* intervals, const float* history, float* merged) {
     gi_merge_f32(p, intervals, history, merged);

@@ -187,7 +187,8 @@ typedef struct {
     const orc_gi_params* p;
     REAL* intervals;
     uint32_t* samples; /* nullable: supported rays per interval */
-    uint64_t begin, end;
+    uint64_t begin, end; /* positions k of the traced intervals g = k * stride */
+    uint64_t stride;
 } SFX(trace_job);
 
 static void* SFX(trace_worker)(void* arg) {
@@ -196,7 +197,8 @@ static void* SFX(trace_worker)(void* arg) {
     uint32_t levels = p->levels;
     uint64_t off[17];
     for (uint32_t l = 0; l <= levels; ++l) off[l] = orc_gi_level_offset(p, l);
-    for (uint64_t g = j->begin; g < j->end; ++g) {
+    for (uint64_t kpos = j->begin; kpos < j->end; ++kpos) {
+        const uint64_t g = kpos * j->stride;
         uint32_t L = 0;
         while (L + 1 < levels && g >= off[L + 1]) ++L;
         uint64_t local = g - off[L];
@@ -264,8 +266,9 @@ static void* SFX(trace_worker)(void* arg) {
 
 static void SFX(gi_trace)(const orc_scene* s, const orc_light* lights, uint32_t nl,
                           const orc_material* mats, uint32_t nm, const orc_gi_params* p,
-                          REAL* intervals, uint32_t* samples, int threads) {
-    uint64_t total = orc_gi_level_offset(p, p->levels);
+                          REAL* intervals, uint32_t* samples, int threads, uint64_t stride) {
+    if (stride < 1) stride = 1;
+    uint64_t total = (orc_gi_level_offset(p, p->levels) + stride - 1) / stride;
     if (threads < 1) threads = 1;
     if (threads > 64) threads = 64;
     SFX(trace_job) jobs[64];
@@ -276,6 +279,7 @@ static void SFX(gi_trace)(const orc_scene* s, const orc_light* lights, uint32_t 
         jobs[t].nm = nm; jobs[t].p = p; jobs[t].intervals = intervals; jobs[t].samples = samples;
         jobs[t].begin = per * (uint64_t)t < total ? per * (uint64_t)t : total;
         jobs[t].end = jobs[t].begin + per < total ? jobs[t].begin + per : total;
+        jobs[t].stride = stride;
     }
     for (int t = 1; t < threads; ++t) pthread_create(&tids[t], NULL, SFX(trace_worker), &jobs[t]);
     SFX(trace_worker)(&jobs[0]);

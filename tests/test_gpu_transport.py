@@ -263,3 +263,27 @@ def test_gpu_dense_fixture(lux_ctx, mode, terminal):
     ok = ~close
     assert close.mean() < 0.01
     assert np.array_equal(g["intervals"][ok], want[ok].astype(np.float32))
+
+
+def test_gpu_gi_trace_stats_and_stage_times(lux_ctx):
+    """The instrumented interval trace writes the same intervals as the plain
+    one and counts its work (a primary walk per interval, node fetches at
+    least one per walk, occlusion rays at most the NEE samples per walk);
+    stage timing reports positive stage durations without changing results."""
+    sc, _ = scenes.cornell()
+    p = cornell_params()
+    g = run_gpu(lux_ctx, sc, p)
+    w = lux_ctx.gi_trace_stats()
+    lux_ctx.synchronize()
+    iv = lux_ctx.gi_download()["intervals"]
+    assert_bits(iv, g["intervals"], "intervals after the instrumented trace")
+    n = O.total_intervals(p)
+    assert w["primary_walks"] == n and w["primary_nodes"] >= n and w["primary_tris"] > 0
+    assert 0 < w["occlusion_rays"] <= n * (p.emissive_samples // 2) and w["occlusion_nodes"] >= w["occlusion_rays"]
+    lux_ctx.gi_set_stage_timing(True)
+    lux_ctx.gi_update()
+    lux_ctx.synchronize()
+    t = lux_ctx.gi_stage_times()
+    lux_ctx.gi_set_stage_timing(False)
+    assert all(v > 0 for v in t.values()), t
+    assert_bits(lux_ctx.gi_download()["intervals"], g["intervals"], "intervals with stage timing")
