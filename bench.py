@@ -346,7 +346,10 @@ def dynamic_section(lux, scenes, args) -> dict:
     update (plain, or levels 0-1 traced with the surrogate answering level 2
     and trained online on 65,536 traced samples per frame), and the world
     resolve of all 2,073,600 receivers.  Wall time per frame (host waits for
-    the frame's last kernel), p50/p99 over the frames."""
+    the frame's last kernel), p50/p99 over the frames.  With the temporal
+    history on (blends 0.88 / 0.98, persistent), the share of intervals whose
+    history each frame reuses -- every instance moving, and only every 32nd
+    (8 instances) moving."""
     import time
     from bevy_penumbra_amd.lux import DeviceBuffer
     sc, motion = scenes.dynamic()
@@ -391,12 +394,20 @@ def dynamic_section(lux, scenes, args) -> dict:
            "full_rebuild_ms": round(float(np.median(full)), 3),
            "frame_blas": ("binned SAH on the host" if args.blas == "sah" else "Morton-30") +
                          " (%.1f ms, once)" % blas_ms}
-    for mode in ("plain", "plain_refit", "surrogate"):
-        times, tlas = [], []
+    inv0, fwd0 = scenes.dynamic_transforms(motion, 0)
+    n_int = ctx.gi_sizes()[0]
+    for mode in ("plain", "plain_refit", "surrogate", "history", "history_8_moving"):
+        times, tlas, reused = [], [], []
+        if mode.startswith("history"):
+            ctx.gi_set_history(0.88, 0.98, 24, 1)
         for f in range(-5, args.dynamic_frames):
             fr = max(f, 0)
             t0 = time.perf_counter()
             inv, fwd = scenes.dynamic_transforms(motion, fr)
+            if mode == "history_8_moving":
+                # only every 32nd instance moves; the others stay where frame 0 put them
+                keep = np.arange(inv.shape[0]) % 32 != 0
+                inv[keep], fwd[keep] = inv0[keep], fwd0[keep]
             ctx.set_transforms(inv, fwd)
             tb = time.perf_counter()
             ctx.build_bvh(bmode | (lux.LUX_BVH_REFIT if mode == "plain_refit" else 0))
@@ -415,11 +426,17 @@ def dynamic_section(lux, scenes, args) -> dict:
             if f >= 0:
                 times.append((time.perf_counter() - t0) * 1e3)
                 tlas.append((te - tb) * 1e3)
+                if mode.startswith("history"):
+                    reused.append(ctx.gi_history_reused() / n_int)
         r = out.numpy()
         res[mode] = {"frames": len(times), "p50_ms": round(float(np.percentile(times, 50)), 3),
                      "p99_ms": round(float(np.percentile(times, 99)), 3),
                      "tlas_p50_ms": round(float(np.percentile(tlas, 50)), 3),
                      "supported_receivers": int((r[:, 3] >= 1.0).sum())}
+        if reused:
+            res[mode]["history_reused_median"] = round(float(np.median(reused)), 4)
+            res[mode]["history_reused_min"] = round(float(np.min(reused)), 4)
+            ctx.gi_history_off()
     sur.close()
     ctx.close()
     return res

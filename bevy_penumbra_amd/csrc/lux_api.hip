@@ -303,7 +303,7 @@ extern "C" void lux_ctx_destroy(lux_ctx* ctx) {
                           &ctx->rx_keys, &ctx->rx_flags, &ctx->rx_slot, &ctx->rx_lod, &ctx->rx_ord,
                           &ctx->rx_seq, &ctx->rx_dir, &ctx->rx_tickets, &ctx->rx_dep_val, &ctx->rx_pack, &ctx->rx_traced,
                           &ctx->rx_sort, &ctx->rx_scan, &ctx->rx_counters, &ctx->rx_cold,
-                          &ctx->screen_cache, &ctx->range_ok, &ctx->gi_stats_buf};
+                          &ctx->screen_cache, &ctx->range_ok, &ctx->gi_stats_buf, &ctx->gi_dep, &ctx->gi_w_dep};
     for (lux_buffer* b : bufs) lux_release(*b);
     for (hipEvent_t e : ctx->ev_phase)
         if (e) (void)hipEventDestroy(e);
@@ -379,6 +379,7 @@ extern "C" int lux_scene_upload_soa(lux_ctx* ctx, const lux_mesh_desc* meshes, u
         }
     }
     ctx->host_pos.assign(mesh_count, {});
+    ctx->mesh_bounds.clear();
     ctx->host_mat.assign(mesh_count, {});
     for (uint32_t m = 0; m < mesh_count; ++m) {
         uint32_t n = meshes[m].triangle_count;
@@ -421,42 +422,70 @@ extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_wo
     if (instance_count && (!local_from_world || !world_from_local))
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_scene_set_transforms: null array");
     // dirty regions for the GI history: the world boxes of every moved
-    // instance before and after the move (a whole-history reset past the
-    // dirty-box capacity)
+    // instance before and after the move.  A measured-empty dependency (a ray
+    // that missed the scene's root box) stays valid only while the root does
+    // not grow: a moved box leaving the previous root resets the history.
     std::vector<float> dirty;
     bool reset_history = false;
     if (ctx->gi_hist_on) {
-        uint32_t moved = 0;
-        for (uint32_t i = 0; i < instance_count; ++i)
-            if (memcmp(ctx->instances[i].world_from_local, world_from_local + 12ull * i, 48) != 0) ++moved;
-        if (2 * moved > 16) {
-            reset_history = true;
-        } else {
-            for (uint32_t i = 0; i < instance_count; ++i) {
-                const float* nf = world_from_local + 12ull * i;
-                if (memcmp(ctx->instances[i].world_from_local, nf, 48) == 0) continue;
-                uint32_t m = ctx->instances[i].mesh;
-                if (m >= ctx->host_pos.size() || ctx->host_pos[m].empty()) continue;
+        if (ctx->mesh_bounds.size() != 6ull * ctx->host_pos.size()) {
+            ctx->mesh_bounds.assign(6ull * ctx->host_pos.size(), 0.0f);
+            for (size_t m = 0; m < ctx->host_pos.size(); ++m) {
                 const std::vector<float>& pos = ctx->host_pos[m];
-                float lo[3] = {pos[0], pos[1], pos[2]}, hi[3] = {pos[0], pos[1], pos[2]};
+                float* mb = &ctx->mesh_bounds[6 * m];
+                for (int q = 0; q < 3; ++q) {
+                    mb[q] = pos.empty() ? 0.0f : pos[q];
+                    mb[3 + q] = mb[q];
+                }
                 for (size_t v = 0; v < pos.size(); v += 3)
                     for (int q = 0; q < 3; ++q) {
-                        lo[q] = std::min(lo[q], pos[v + q]);
-                        hi[q] = std::max(hi[q], pos[v + q]);
+                        mb[q] = std::min(mb[q], pos[v + q]);
+                        mb[3 + q] = std::max(mb[3 + q], pos[v + q]);
                     }
-                for (const float* f : {static_cast<const float*>(ctx->instances[i].world_from_local), nf}) {
-                    float wlo[3] = {INFINITY, INFINITY, INFINITY}, whi[3] = {-INFINITY, -INFINITY, -INFINITY};
-                    for (int c = 0; c < 8; ++c) {
-                        float pl[3] = {(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
-                        for (int r = 0; r < 3; ++r) {
-                            float w = f[4 * r] * pl[0] + f[4 * r + 1] * pl[1] + f[4 * r + 2] * pl[2] + f[4 * r + 3];
-                            wlo[r] = std::min(wlo[r], w);
-                            whi[r] = std::max(whi[r], w);
-                        }
-                    }
-                    dirty.insert(dirty.end(), {wlo[0], wlo[1], wlo[2], whi[0], whi[1], whi[2]});
+            }
+        }
+        auto world_box = [&](uint32_t m, const float* f, float* out) {
+            const float* mb = &ctx->mesh_bounds[6ull * m];
+            for (int r = 0; r < 3; ++r) {
+                out[r] = INFINITY;
+                out[3 + r] = -INFINITY;
+            }
+            for (int c = 0; c < 8; ++c) {
+                const float pl[3] = {(c & 1) ? mb[3] : mb[0], (c & 2) ? mb[4] : mb[1], (c & 4) ? mb[5] : mb[2]};
+                for (int r = 0; r < 3; ++r) {
+                    const float w = ((f[4 * r] * pl[0] + f[4 * r + 1] * pl[1]) + f[4 * r + 2] * pl[2]) + f[4 * r + 3];
+                    out[r] = std::min(out[r], w);
+                    out[3 + r] = std::max(out[3 + r], w);
                 }
             }
+        };
+        float root[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = 0; i < instance_count; ++i) {
+            const uint32_t m = ctx->instances[i].mesh;
+            if (m >= ctx->host_pos.size() || ctx->host_pos[m].empty()) continue;
+            float wb[6];
+            world_box(m, ctx->instances[i].world_from_local, wb);
+            for (int q = 0; q < 3; ++q) {
+                root[q] = std::min(root[q], wb[q]);
+                root[3 + q] = std::max(root[3 + q], wb[3 + q]);
+            }
+        }
+        for (uint32_t i = 0; i < instance_count && !reset_history; ++i) {
+            const float* nf = world_from_local + 12ull * i;
+            if (memcmp(ctx->instances[i].world_from_local, nf, 48) == 0) continue;
+            const uint32_t m = ctx->instances[i].mesh;
+            if (m >= ctx->host_pos.size() || ctx->host_pos[m].empty()) continue;
+            float ob[6], nb[6];
+            world_box(m, ctx->instances[i].world_from_local, ob);
+            world_box(m, nf, nb);
+            for (int q = 0; q < 3; ++q) {
+                // inside the previous root, with a relative margin for the
+                // device's own rounding of the root box
+                const float pad = 1e-5f * std::max(1.0f, std::max(std::fabs(root[q]), std::fabs(root[3 + q])));
+                if (!(nb[q] >= root[q] + pad && nb[3 + q] <= root[3 + q] - pad)) reset_history = true;
+            }
+            dirty.insert(dirty.end(), ob, ob + 6);
+            dirty.insert(dirty.end(), nb, nb + 6);
         }
     }
     std::vector<InstDesc> idesc(instance_count);
@@ -475,9 +504,7 @@ extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_wo
     ctx->tlas_built = false;
     ctx->gi_lights_dirty = true;
     if (reset_history) return lux_gi_reset_history(ctx);
-    for (size_t b = 0; b + 6 <= dirty.size(); b += 6)
-        if (int rc = lux_gi_invalidate_history(ctx, &dirty[b], &dirty[b + 3])) return rc;
-    return LUX_OK;
+    return lux_gi_invalidate_boxes(ctx, dirty.data(), (uint32_t)(dirty.size() / 6));
 }
 
 namespace {
@@ -1114,6 +1141,7 @@ extern "C" int lux_scene_upload_words(lux_ctx* ctx, const uint32_t* scene_words,
     LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     // host copies for the emitter table: one mesh per distinct triangle range
     ctx->host_pos.assign(ranges.size(), {});
+    ctx->mesh_bounds.clear();
     ctx->host_mat.assign(ranges.size(), {});
     for (size_t m = 0; m < ranges.size(); ++m) {
         std::vector<float>& hp = ctx->host_pos[m];

@@ -53,6 +53,7 @@ struct BatchWave {
     float4* slab;        // per setup wave: its occlusion rays (64 * nee_max)
     uint32_t* wave_cnt;  // per setup wave: its ray count, then scanned into offsets
     uint32_t regroup, batch;
+    float4* dep_item;    // per item of a history scene: its NEE segments' dependency (null: no history)
 };
 
 // Largest s with base[s] <= x (base[0] = 0, base[n] = total; empty scenes
@@ -167,10 +168,13 @@ __global__ __launch_bounds__(kWg) void k_b_setup(BatchWave w) {
         sh = make_float4(rad.x, rad.y, rad.z, nee ? 1.0f : 0.0f);
         if (nee) {
             const uint32_t lane = __lane_id();
+            const bool track = w.dep_item != nullptr && S.a.dep != nullptr;
+            Dep nd = dep_empty();
             for (uint32_t smp = 0; smp < c.count; ++smp) {
                 float3 origin = make_float3(0.0f, 0.0f, 0.0f), dir = origin, t3 = origin;
                 float sl = 0.0f;
                 bool q = nee_sample(S.a, c, smp, origin, dir, sl, t3);
+                if (q && track) dep_ray(nd, S.a.scene, origin, dir, sl, false, 0.0f);
                 uint32_t slot = smp * w.stride + i;
                 w.slots[slot] = make_float4(t3.x, t3.y, t3.z, q ? 1.0f : 0.0f);
                 uint64_t mask = __ballot(q);
@@ -182,6 +186,7 @@ __global__ __launch_bounds__(kWg) void k_b_setup(BatchWave w) {
                 }
                 running += (uint32_t)__popcll(mask);
             }
+            if (track) dep_store(w.dep_item + 2ull * i, nd);
         }
     }
     if (live) w.shade[i] = sh;
@@ -242,16 +247,20 @@ __global__ __launch_bounds__(kWg) void k_b_accum(BatchWave w) {
     float3 jsum = make_float3(0.0f, 0.0f, 0.0f);
     float bsum = 0.0f;
     uint32_t n = 0;
+    const bool track = a.dep != nullptr && w.dep_item != nullptr;
+    Dep dep = dep_empty();
     for (uint32_t r = 0; r < S.R; ++r) {
         uint32_t i = (uint32_t)(w.item_base[s] + g * S.R + r);
         uint4 pr = w.prim[i];
         if (pr.w & lux::kErrUnsupported) continue;
         float3 rad = make_float3(0.0f, 0.0f, 0.0f);
         float beta;
+        bool nee_ran = false;
         if (pr.y != lux::kEmpty) {
             float4 sh = w.shade[i];
             float3 rr = make_float3(sh.x, sh.y, sh.z);
             if (sh.w != 0.0f) {
+                nee_ran = true;
                 float3 sum = make_float3(0.0f, 0.0f, 0.0f);
                 bool ok = true;
                 for (uint32_t smp = 0; smp < count; ++smp) {
@@ -277,6 +286,17 @@ __global__ __launch_bounds__(kWg) void k_b_accum(BatchWave w) {
         } else {
             beta = terminal ? 0.0f : 1.0f;
         }
+        if (track) {
+            // the ray's segment, then its NEE occlusion segments
+            float3 ro, rdir;
+            float rlim;
+            bool rterm;
+            item_ray(a, g, r, false, ro, rdir, rlim, rterm);
+            Dep rd = dep_empty();
+            dep_ray(rd, a.scene, ro, rdir, rlim, pr.y != lux::kEmpty, __uint_as_float(pr.x));
+            if (nee_ran) dep_union(rd, dep_load(w.dep_item + 2ull * i));
+            dep_union(dep, rd);
+        }
         jsum = make_float3(jsum.x + rad.x, jsum.y + rad.y, jsum.z + rad.z);
         bsum = bsum + beta;
         n++;
@@ -290,6 +310,7 @@ __global__ __launch_bounds__(kWg) void k_b_accum(BatchWave w) {
     }
     a.intervals[g] = out;
     if (a.samples) a.samples[g] = n;
+    if (track) dep_store(a.dep + 2 * g, dep);
 }
 
 // Temporal history of every traced interval of the scenes that keep one.
@@ -379,7 +400,7 @@ struct lux_batch {
     uint64_t query_max = 0;
     uint64_t setup_waves = 0;  // waves of the NEE setup launch (occlusion slabs)
     lux_ctx* owner = nullptr;  // allocations go through the first scene's context
-    lux_buffer table, bases, prim, shade, slots, vis, shadow, counters, queries, sur_scratch;
+    lux_buffer table, bases, prim, shade, slots, vis, shadow, counters, queries, sur_scratch, dep_item;
     uint32_t wpc = 16, depth = 16, regroup = 16, batch = 8;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -475,6 +496,8 @@ int prepare(lux_batch* b) {
     if (int rc = ensure(b, b->shade, 16 * items + 16)) return rc;
     if (int rc = ensure(b, b->slots, 16 * items * b->nee_max + 16)) return rc;
     if (int rc = ensure(b, b->vis, 4 * items * b->nee_max + 16)) return rc;
+    if (b->any_history)
+        if (int rc = ensure(b, b->dep_item, 32 * items + 32)) return rc;
     // occlusion rays: per setup wave a slab of 64 * nee_max, then the packed list
     b->setup_waves = ((items + kWg - 1) / kWg) * (kWg / 64u);
     if (b->setup_waves * 64u * b->nee_max >= (1ull << 31))
@@ -517,6 +540,7 @@ BatchWave wave_of(lux_batch* b) {
     w.slab = w.shadow + 2ull * b->setup_waves * 64u * b->nee_max;
     w.regroup = b->regroup ? b->regroup : 1u;
     w.batch = b->batch ? b->batch : 8u;
+    w.dep_item = b->any_history ? static_cast<float4*>(b->dep_item.ptr) : nullptr;
     return w;
 }
 
@@ -625,7 +649,7 @@ extern "C" void lux_batch_destroy(lux_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
     lux_buffer* bufs[] = {&b->table, &b->bases, &b->prim, &b->shade, &b->slots,
-                          &b->vis, &b->shadow, &b->counters, &b->queries, &b->sur_scratch};
+                          &b->vis, &b->shadow, &b->counters, &b->queries, &b->sur_scratch, &b->dep_item};
     for (lux_buffer* x : bufs) lux_release(*x);
     for (lux_buffer& x : b->sur_fb) lux_release(x);
     if (b->ev0) (void)hipEventDestroy(b->ev0);

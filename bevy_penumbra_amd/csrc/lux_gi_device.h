@@ -36,16 +36,21 @@ struct GiArgs {
     // temporal history of raw intervals (null when off): supported rays per
     // interval of this update; per interval (J.rgb, radiance mass) and
     // (beta, transmittance mass, last sampled frame bits, 0); state words
-    // [0] history valid, [1] dirty boxes, [2..7] emitter bounds (lo, hi),
-    // then 8 floats per dirty box (lo, pad, hi, pad) from word 8
+    // [0] history valid, [1] dirty boxes, [2] intervals whose history the last
+    // pass reused, [3..7] unused, then 8 floats per dirty box (lo, pad, hi,
+    // pad) from word 8
     uint32_t* samples;
     float4* history;
     uint32_t* hist_state;
     float hist_rb, hist_tb;
     uint32_t hist_frames, hist_persistent;
+    // this update's dependency per interval (history on): (lo.xyz, known),
+    // (hi.xyz, has bounds) -- the union of the world segments its supported
+    // rays and their NEE occlusion rays traversed
+    float4* dep;
 };
 
-constexpr uint32_t kMaxDirtyBoxes = 16;
+constexpr uint32_t kMaxDirtyBoxes = 1024;
 
 __device__ bool admit_query(const GiArgs& a);
 
@@ -100,6 +105,99 @@ __device__ __forceinline__ float sat(float x) { return x < 0.0f ? 0.0f : (x > 1.
 __device__ __forceinline__ float fracf(float x) { return x - floorf(x); }
 __device__ __forceinline__ float pow2i(int e) { return __uint_as_float((uint32_t)(127 + e) << 23); }
 __device__ __forceinline__ float dot3(float3 a, float3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+// ---- dependencies (DependencyContext, shaders/common.slang:138-242) ----
+// The union of world-space segments that established a value: known /
+// unknown, and measured-empty versus bounded.
+struct Dep {
+    float3 lo, hi;
+    bool known, has;
+};
+__device__ __forceinline__ Dep dep_empty() {
+    Dep d;
+    d.lo = d.hi = make_float3(0.0f, 0.0f, 0.0f);
+    d.known = true;
+    d.has = false;
+    return d;
+}
+__device__ __forceinline__ Dep dep_unknown() {
+    Dep d = dep_empty();
+    d.known = false;
+    return d;
+}
+__device__ __forceinline__ bool finite3(float3 v) {
+    return lux::isfinite_bits(v.x) && lux::isfinite_bits(v.y) && lux::isfinite_bits(v.z);
+}
+// dependencyIncludeBounds
+__device__ __forceinline__ void dep_include_bounds(Dep& d, float3 mn, float3 mx) {
+    if (!d.known) return;
+    if (!finite3(mn) || !finite3(mx) || mn.x > mx.x || mn.y > mx.y || mn.z > mx.z) {
+        d = dep_unknown();
+        return;
+    }
+    if (!d.has) {
+        d.lo = mn;
+        d.hi = mx;
+        d.has = true;
+    } else {
+        d.lo = make_float3(smin(d.lo.x, mn.x), smin(d.lo.y, mn.y), smin(d.lo.z, mn.z));
+        d.hi = make_float3(smax(d.hi.x, mx.x), smax(d.hi.y, mx.y), smax(d.hi.z, mx.z));
+    }
+}
+// dependencyIncludeSegment with no cone padding: endpoints widened by the
+// float round-off slack max(scale * 1e-6, 2^-23), scale = max(1, |coords|).
+__device__ __forceinline__ void dep_include_segment(Dep& d, float3 s, float3 e) {
+    if (!finite3(s) || !finite3(e)) {
+        d = dep_unknown();
+        return;
+    }
+    float scale = smax(1.0f, smax(smax(fabsf(s.x), fabsf(s.y)), fabsf(s.z)));
+    scale = smax(scale, smax(smax(fabsf(e.x), fabsf(e.y)), fabsf(e.z)));
+    const float r = smax(scale * 1e-6f, 1.1920929e-7f);
+    dep_include_bounds(d, make_float3(smin(s.x, e.x) - r, smin(s.y, e.y) - r, smin(s.z, e.z) - r),
+                       make_float3(smax(s.x, e.x) + r, smax(s.y, e.y) + r, smax(s.z, e.z) + r));
+}
+// dependencyUnion
+__device__ __forceinline__ void dep_union(Dep& d, const Dep& o) {
+    if (!d.known || !o.known) {
+        d = dep_unknown();
+        return;
+    }
+    if (o.has) dep_include_bounds(d, o.lo, o.hi);
+}
+// The segment a ray traversed (traceRayTracked / traceOccludedTracked,
+// shaders/traversal.slang:696-810,1097-1150): its part inside the scene's root
+// box, from max(root entry, 0) to the hit (closest-hit rays that hit) or the
+// root exit, which is clipped to the ray's limit; a ray missing the root
+// traversed nothing (measured empty).
+__device__ __forceinline__ void dep_ray(Dep& d, const lux::DevScene& s, float3 o, float3 dir, float limit, bool hit,
+                                        float t) {
+    if (s.n_inst == 0) return;
+    const lux::Node root = lux::load_node(s.tlas, s.n_tlas, 0);
+    float e, x;
+    if (!lux::box_interval(o, dir, root.mn, root.mx, limit, e, x)) return;
+    const float s0 = smax(e, 0.0f);
+    const float s1 = hit ? smin(t, x) : x;
+    dep_include_segment(d, make_float3(o.x + dir.x * s0, o.y + dir.y * s0, o.z + dir.z * s0),
+                        make_float3(o.x + dir.x * s1, o.y + dir.y * s1, o.z + dir.z * s1));
+}
+__device__ __forceinline__ void dep_store(float4* out, const Dep& d) {
+    out[0] = make_float4(d.lo.x, d.lo.y, d.lo.z, d.known ? 1.0f : 0.0f);
+    out[1] = make_float4(d.hi.x, d.hi.y, d.hi.z, d.has ? 1.0f : 0.0f);
+}
+// storedDependency: w > 0.5 flags; a malformed bounded set is unknown.
+__device__ __forceinline__ Dep dep_load(const float4* in) {
+    const float4 a = in[0], b = in[1];
+    Dep d;
+    d.lo = make_float3(a.x, a.y, a.z);
+    d.hi = make_float3(b.x, b.y, b.z);
+    d.known = a.w > 0.5f;
+    d.has = b.w > 0.5f;
+    if (d.known && d.has && (!finite3(d.lo) || !finite3(d.hi) || d.lo.x > d.hi.x || d.lo.y > d.hi.y ||
+                             d.lo.z > d.hi.z))
+        d.known = false;
+    return d;
+}
 
 __device__ __forceinline__ uint32_t level_dim(uint32_t d, uint32_t level) {
     for (uint32_t l = 0; l < level; ++l) d = (d + 1u) / 2u;
@@ -385,7 +483,7 @@ __device__ __forceinline__ float3 shade_end(float3 rad) {
 }
 
 __device__ bool shade(const GiArgs& a, float3 o, float3 d, const lux::HitState& h, Stack& stk,
-                      float3& out) {
+                      float3& out, Dep* dep = nullptr) {
     float3 rad;
     NeeCtx c;
     if (shade_begin(a, o, d, h, rad, c)) {
@@ -398,6 +496,7 @@ __device__ bool shade(const GiArgs& a, float3 o, float3 d, const lux::HitState& 
             lux::HitState vh;
             lux::NoStats ns;
             lux::trace_occluded(a.scene, origin, dir, lim, vh, stk, ns);
+            if (dep) dep_ray(*dep, a.scene, origin, dir, lim, false, 0.0f);
             if (vh.err & lux::kHitOccluded) continue;
             if (vh.err & lux::kErrUnsupported) return false;
             sum.x = sum.x + term.x;
@@ -432,7 +531,7 @@ __device__ __forceinline__ float3 probe_centre(const lux_gi_params& p, uint32_t 
 // J = mean shaded radiance, beta = mean transmittance (1 on a miss of a
 // bounded interval), beta = -1 when no ray was supported.
 __device__ float4 trace_interval(const GiArgs& a, uint32_t L, float3 c, uint32_t k, bool terminal,
-                                 Stack& stk, uint32_t* n_out = nullptr) {
+                                 Stack& stk, uint32_t* n_out = nullptr, Dep* dep = nullptr) {
     const lux_gi_params& p = a.p;
     uint32_t nphi = p.base_phi << L, ny = p.base_y << L;
     float end = interval_end(p, L);
@@ -453,12 +552,17 @@ __device__ float4 trace_interval(const GiArgs& a, uint32_t L, float3 c, uint32_t
         if (h.err & lux::kErrUnsupported) continue;
         float3 rad = make_float3(0.0f, 0.0f, 0.0f);
         float beta;
+        // the ray's segment, then its NEE occlusion segments (a sample that
+        // is dropped contributes no dependency)
+        Dep rd = dep_empty();
+        if (dep) dep_ray(rd, a.scene, o, d, limit, h.tri != lux::kEmpty, h.t);
         if (h.tri != lux::kEmpty) {
-            if (!shade(a, o, d, h, stk, rad)) continue;
+            if (!shade(a, o, d, h, stk, rad, dep ? &rd : nullptr)) continue;
             beta = 0.0f;
         } else {
             beta = terminal ? 0.0f : 1.0f;
         }
+        if (dep) dep_union(*dep, rd);
         jsum = make_float3(jsum.x + rad.x, jsum.y + rad.y, jsum.z + rad.z);
         bsum = bsum + beta;
         n++;
@@ -575,7 +679,7 @@ __device__ void merge_interval(const GiArgs& a, uint32_t L, uint64_t local) {
     bool has;
     float tr;
     if (a.history) {
-        float rm = a.history[2 * g].w, tm = a.history[2 * g + 1].y;
+        float rm = a.history[4 * g].w, tm = a.history[4 * g + 1].y;
         tr = tm > 0.0f ? sat(iv.w) : 0.0f;
         has = rm > 0.0f && tm > 0.0f;
     } else {
@@ -630,29 +734,26 @@ __device__ __forceinline__ float valid_mass(float m) { return (lux::isfinite_bit
 // the box of probe centre +- interval end joined with the emitter bounds
 // (unbounded for a terminal interval).  Writes the blended (J, beta) in
 // place and the new history.
-__device__ void history_blend(const GiArgs& a, uint64_t g, uint32_t L, uint64_t local) {
+__device__ bool history_blend(const GiArgs& a, uint64_t g, uint32_t L, uint64_t local) {
     const lux_gi_params& p = a.p;
     const uint32_t* st = a.hist_state;
     float4 iv = a.intervals[g];
     uint32_t n = a.samples[g];
-    float4 h0 = a.history[2 * g], h1 = a.history[2 * g + 1];
+    float4 h0 = a.history[4 * g], h1 = a.history[4 * g + 1];
     uint32_t last = __float_as_uint(h1.z);
     bool reuse = st[0] != 0u && a.hist_frames > 0u;
     if (reuse) reuse = a.hist_persistent != 0u || p.frame - last <= a.hist_frames;
     uint32_t nd = st[1] < kMaxDirtyBoxes ? st[1] : kMaxDirtyBoxes;
+    const Dep old = dep_load(a.history + 4 * g + 2);
     if (reuse && nd > 0u) {
-        bool terminal = p.terminal_tail != 0u && L + 1 == p.levels;
-        uint32_t D = (p.base_phi << L) * (p.base_y << L);
-        float3 c = probe_centre(p, L, local / D);
-        float e = interval_end(p, L);
-        const float* em = reinterpret_cast<const float*>(st + 2);
-        float dlo[3] = {smin(c.x - e, em[0]), smin(c.y - e, em[1]), smin(c.z - e, em[2])};
-        float dhi[3] = {smax(c.x + e, em[3]), smax(c.y + e, em[4]), smax(c.z + e, em[5])};
-        for (uint32_t b = 0; b < nd && reuse; ++b) {
+        // dependencyIntersectsDirty (split.slang:13-25) against each dirty box:
+        // an unknown dependency meets every region, a measured-empty one none
+        if (!old.known) reuse = false;
+        for (uint32_t b = 0; b < nd && reuse && old.has; ++b) {
             const float* box = reinterpret_cast<const float*>(st + 8) + 8 * b;
-            bool hit = terminal || (dlo[0] <= box[4] && dhi[0] >= box[0] && dlo[1] <= box[5] &&
-                                    dhi[1] >= box[1] && dlo[2] <= box[6] && dhi[2] >= box[2]);
-            if (hit) reuse = false;
+            if (old.lo.x <= box[4] && old.hi.x >= box[0] && old.lo.y <= box[5] && old.hi.y >= box[1] &&
+                old.lo.z <= box[6] && old.hi.z >= box[2])
+                reuse = false;
         }
     }
     const float rbr = a.hist_rb, tbr = a.hist_tb;
@@ -696,8 +797,18 @@ __device__ void history_blend(const GiArgs& a, uint64_t g, uint32_t L, uint64_t 
         }
     }
     a.intervals[g] = make_float4(J.x, J.y, J.z, beta);
-    a.history[2 * g] = make_float4(J.x, J.y, J.z, mass);
-    a.history[2 * g + 1] = make_float4(beta, ts, __uint_as_float(last), 0.0f);
+    a.history[4 * g] = make_float4(J.x, J.y, J.z, mass);
+    a.history[4 * g + 1] = make_float4(beta, ts, __uint_as_float(last), 0.0f);
+    // the retained value depends on this update's segments and, when history
+    // was reused, on everything the history depended on
+    Dep cur = dep_load(a.dep + 2 * g);
+    if (reuse) {
+        Dep u = old;
+        dep_union(u, cur);
+        cur = u;
+    }
+    dep_store(a.history + 4 * g + 2, cur);
+    return reuse && (om > 0.0f || otm > 0.0f);  // history contributed
 }
 
 // decodeNormal (common.slang:370-375).
@@ -878,6 +989,7 @@ inline GiArgs gi_args(lux_ctx* ctx) {
     a.samples = hist ? static_cast<uint32_t*>(ctx->gi_samples.ptr) : nullptr;
     a.history = hist ? static_cast<float4*>(ctx->gi_history.ptr) : nullptr;
     a.hist_state = hist ? static_cast<uint32_t*>(ctx->gi_hist_state.ptr) : nullptr;
+    a.dep = hist ? static_cast<float4*>(ctx->gi_dep.ptr) : nullptr;
     a.hist_rb = ctx->gi_hist.radiance_blend;
     a.hist_tb = ctx->gi_hist.transmittance_blend;
     a.hist_frames = ctx->gi_hist.history_frames;

@@ -51,6 +51,7 @@ struct lux_ctx {
     std::vector<std::vector<float>> host_pos;
     std::vector<std::vector<uint32_t>> host_mat;
     std::vector<lux_material> host_materials;
+    std::vector<float> mesh_bounds;  // per mesh: local box (lo, hi) of host_pos, computed on demand
 
     // transport state
     bool gi_ready = false;
@@ -73,6 +74,7 @@ struct lux_ctx {
     // item setup is heavier than a ray-list load)
     uint32_t gi_wpc = 16, gi_depth = 16, gi_regroup = 16, gi_batch = 8;
     lux_buffer gi_w_prim, gi_w_shade, gi_w_slots, gi_w_vis, gi_w_shadow, gi_w_counters;
+    lux_buffer gi_w_dep;     // per item: NEE segments' dependency (history on)
     lux_buffer mesh_info;    // uint32 per mesh: BLAS base | triangle base | count
     hipEvent_t ev_phase[5] = {};
     // opt-in stage timing of the wavefront trace (lux_gi_set_stage_timing):
@@ -91,8 +93,11 @@ struct lux_ctx {
     bool gi_hist_valid = false;  // a history pass ran since the last reset
     lux_gi_history_params gi_hist{};
     lux_buffer gi_samples, gi_history, gi_hist_state;
+    lux_buffer gi_dep;  // per interval: this update's dependency (2 float4)
+    // host copy of the emitter sampling table (probability, CDF per light):
+    // a change of either resets the history (every NEE sample may move)
+    std::vector<float> gi_light_cdf;
     uint32_t gi_dirty_n = 0;                 // dirty boxes pending on the device
-    float gi_emit_lo[3] = {}, gi_emit_hi[3] = {};  // emitter bounds (world)
     // receiver-driven updates (lux_gi_update_receivers, state on the first
     // LOD's context): descriptor keys / flags / directory slots, per
     // (variant, receiver) LOD, ordinal and sequence, the probe directory
@@ -168,6 +173,11 @@ int lux_surrogate_forward_into(lux_surrogate* s, hipStream_t st, const float* qu
                                const float** Y);
 // GI state check of a context (configured, BVH built, emitter table current).
 int lux_gi_check_internal(lux_ctx* ctx);
+// Dirty boxes (6 floats each: lo, hi) for the next history pass.
+int lux_gi_invalidate_boxes(lux_ctx* ctx, const float* boxes, uint32_t count);
+// This update's dependency of every interval set to unknown (updates whose
+// deposits record no segments: receiver-driven updates).
+int lux_gi_dep_unknown(lux_ctx* ctx, hipStream_t st);
 // History blend (when on), merge and pack of a context's intervals on stream st.
 int lux_gi_finish_internal(lux_ctx* ctx, hipStream_t st);
 uint64_t lux_sur_fallback_words(const lux_ctx* ctx, uint32_t level);

@@ -43,8 +43,10 @@ __global__ __launch_bounds__(kWg) void k_gi_trace(GiArgs a, uint64_t g_begin, ui
     float3 c = probe_centre(p, L, local / D);
     bool terminal = p.terminal_tail != 0u && L + 1 == p.levels;
     uint32_t ns = 0;
-    a.intervals[g] = trace_interval(a, L, c, k, terminal, stk, &ns);
+    Dep dep = dep_empty();
+    a.intervals[g] = trace_interval(a, L, c, k, terminal, stk, &ns, a.dep ? &dep : nullptr);
     if (a.samples) a.samples[g] = ns;
+    if (a.dep) dep_store(a.dep + 2 * g, dep);
 }
 
 __global__ __launch_bounds__(kWg) void k_sur_queries(GiArgs a, uint32_t L, uint64_t count, float* out) {
@@ -163,6 +165,7 @@ struct Wave {
     uint32_t stride;             // items per NEE sample row of `slots` / `vis` (the chunk's capacity)
     uint4* prim;                 // per item: t bits, tri, inst, err
     float4* shade;               // per item: emission rgb, w = 1 when NEE runs
+    float4* dep_item;            // per item (history on, kWaveRange): its NEE segments' dependency
     float4* slots;               // per slot (sample-major: smp * stride + item, so a wave's
                                  // lanes touch consecutive slots): contribution rgb, w = 1
                                  // when a query was issued
@@ -302,10 +305,12 @@ __global__ __launch_bounds__(kWg) void k_gi_setup(GiArgs a, Wave w) {
         sh = make_float4(rad.x, rad.y, rad.z, nee ? 1.0f : 0.0f);
         if (nee) {
             const uint32_t lane = __lane_id();
+            Dep nd = dep_empty();
             for (uint32_t smp = 0; smp < c.count; ++smp) {
                 float3 origin = make_float3(0.0f, 0.0f, 0.0f), dir = origin, t3 = origin;
                 float sl = 0.0f;
                 bool q = nee_sample(a, c, smp, origin, dir, sl, t3);
+                if (q && w.dep_item) dep_ray(nd, a.scene, origin, dir, sl, false, 0.0f);
                 uint32_t slot = smp * w.stride + i;
                 w.slots[slot] = make_float4(t3.x, t3.y, t3.z, q ? 1.0f : 0.0f);
                 uint64_t mask = __ballot(q);
@@ -317,6 +322,7 @@ __global__ __launch_bounds__(kWg) void k_gi_setup(GiArgs a, Wave w) {
                 }
                 running += (uint32_t)__popcll(mask);
             }
+            if (w.dep_item) dep_store(w.dep_item + 2ull * i, nd);
         }
     }
     if (live) w.shade[i] = sh;
@@ -375,16 +381,20 @@ __global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
     float3 jsum = make_float3(0.0f, 0.0f, 0.0f);
     float bsum = 0.0f;
     uint32_t n = 0;
+    const bool track = a.dep != nullptr && w.kind == kWaveRange;
+    Dep dep = dep_empty();
     for (uint32_t r = 0; r < w.R; ++r) {
         uint32_t i = j * w.R + r;
         uint4 pr = w.prim[i];
         if (pr.w & lux::kErrUnsupported) continue;
+        bool nee_ran = false;
         float3 rad = make_float3(0.0f, 0.0f, 0.0f);
         float beta;
         if (pr.y != lux::kEmpty) {
             float4 sh = w.shade[i];
             float3 rr = make_float3(sh.x, sh.y, sh.z);
             if (sh.w != 0.0f) {
+                nee_ran = true;
                 float3 sum = make_float3(0.0f, 0.0f, 0.0f);
                 bool ok = true;
                 for (uint32_t smp = 0; smp < count; ++smp) {
@@ -410,6 +420,17 @@ __global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
         } else {
             beta = terminal ? 0.0f : 1.0f;
         }
+        if (track) {
+            // the ray's segment, then its NEE occlusion segments
+            float3 ro, rdir;
+            float rlim;
+            bool rterm;
+            item_ray(a, g, r, force, ro, rdir, rlim, rterm);
+            Dep rd = dep_empty();
+            dep_ray(rd, a.scene, ro, rdir, rlim, pr.y != lux::kEmpty, __uint_as_float(pr.x));
+            if (nee_ran) dep_union(rd, dep_load(w.dep_item + 2ull * i));
+            dep_union(dep, rd);
+        }
         jsum = make_float3(jsum.x + rad.x, jsum.y + rad.y, jsum.z + rad.z);
         bsum = bsum + beta;
         n++;
@@ -424,6 +445,7 @@ __global__ __launch_bounds__(kWg) void k_gi_accum(GiArgs a, Wave w) {
     if (w.kind == kWaveRange) {
         a.intervals[g] = out;
         if (a.samples) a.samples[g] = n;
+        if (track) dep_store(a.dep + 2 * g, dep);
     } else if (w.kind == kWaveListed) {
         a.merged[g] = merge_leaf(out);
     } else {
@@ -482,10 +504,14 @@ __global__ __launch_bounds__(kWg) void k_rx_wave_finish(GiArgs a, Wave w) {
 // Temporal blend of intervals [0, count) (levels are contiguous).
 __global__ __launch_bounds__(kWg) void k_gi_history(GiArgs a, uint64_t count) {
     uint64_t g = (uint64_t)blockIdx.x * kWg + threadIdx.x;
-    if (g >= count) return;
-    uint32_t L = 0;
-    while (L + 1 < a.p.levels && g >= a.offset[L + 1]) ++L;
-    history_blend(a, g, L, g - a.offset[L]);
+    bool kept = false;
+    if (g < count) {
+        uint32_t L = 0;
+        while (L + 1 < a.p.levels && g >= a.offset[L + 1]) ++L;
+        kept = history_blend(a, g, L, g - a.offset[L]);
+    }
+    const uint64_t m = __ballot(kept);
+    if (__lane_id() == 0u && m) atomicAdd(&a.hist_state[2], (uint32_t)__popcll(m));
 }
 
 // After a history pass: the history is valid and the dirty boxes consumed.
@@ -843,32 +869,25 @@ std::vector<Light> build_lights(const lux_ctx* ctx) {
 int upload_lights(lux_ctx* ctx) {
     std::vector<Light> lights = build_lights(ctx);
     ctx->gi_lights = (uint32_t)lights.size();
-    for (int q = 0; q < 3; ++q) {
-        ctx->gi_emit_lo[q] = INFINITY;
-        ctx->gi_emit_hi[q] = -INFINITY;
-    }
-    for (const Light& l : lights) {
-        const float a[3] = {l.position.x, l.position.y, l.position.z};
-        const float e1[3] = {l.edge1.x, l.edge1.y, l.edge1.z}, e2[3] = {l.edge2.x, l.edge2.y, l.edge2.z};
-        for (int q = 0; q < 3; ++q) {
-            const float v[3] = {a[q], a[q] + e1[q], a[q] + e2[q]};
-            for (float x : v) {
-                ctx->gi_emit_lo[q] = std::min(ctx->gi_emit_lo[q], x);
-                ctx->gi_emit_hi[q] = std::max(ctx->gi_emit_hi[q], x);
-            }
-        }
-    }
     if (int rc = lux_ensure(ctx, ctx->gi_light_buf, sizeof(Light) * (lights.size() + 1))) return rc;
     LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->gi_hist_state.ptr) {
-        float em[6] = {ctx->gi_emit_lo[0], ctx->gi_emit_lo[1], ctx->gi_emit_lo[2],
-                       ctx->gi_emit_hi[0], ctx->gi_emit_hi[1], ctx->gi_emit_hi[2]};
-        LUX_HIP(ctx, hipMemcpy(static_cast<char*>(ctx->gi_hist_state.ptr) + 8, em, 24, hipMemcpyHostToDevice));
-    }
     if (!lights.empty())
         LUX_HIP(ctx, hipMemcpy(ctx->gi_light_buf.ptr, lights.data(), sizeof(Light) * lights.size(),
                                hipMemcpyHostToDevice));
     ctx->gi_lights_dirty = false;
+    // A moved emitter invalidates the intervals whose NEE segments reach its
+    // old or new place (dirty boxes); a changed choice of emitters (their
+    // number, or a sampling probability off by more than 1 %, i.e. a change of
+    // emitted power rather than the rounding of a rigid motion) may move
+    // every NEE sample: the whole history goes.
+    bool changed = ctx->gi_light_cdf.size() != lights.size();
+    for (size_t i = 0; i < lights.size() && !changed; ++i) {
+        const float p0 = ctx->gi_light_cdf[i], p1 = lights[i].edge1.w;
+        if (std::fabs(p1 - p0) > 0.01f * std::max(p0, 1.0f / 16777216.0f)) changed = true;
+    }
+    ctx->gi_light_cdf.resize(lights.size());
+    for (size_t i = 0; i < lights.size(); ++i) ctx->gi_light_cdf[i] = lights[i].edge1.w;
+    if (changed && ctx->gi_hist_valid) return lux_gi_reset_history(ctx);
     return LUX_OK;
 }
 
@@ -960,6 +979,9 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
     if (int rc = lux_ensure(ctx, ctx->gi_w_shade, 16ull * items)) return rc;
     if (int rc = lux_ensure(ctx, ctx->gi_w_slots, 16ull * items * nee)) return rc;
     if (int rc = lux_ensure(ctx, ctx->gi_w_vis, 4ull * items * nee)) return rc;
+    const bool track = ctx->gi_hist_on && proto.kind == kWaveRange;
+    if (track)
+        if (int rc = lux_ensure(ctx, ctx->gi_w_dep, 32ull * items)) return rc;
     // occlusion rays: per setup wave a slab of 64 * nee, then the packed list
     const uint64_t setup_waves = ((items + kWg - 1) / kWg) * (kWg / 64u);
     if (int rc = lux_ensure(ctx, ctx->gi_w_shadow, 32ull * setup_waves * 64u * nee * 2u)) return rc;
@@ -975,6 +997,7 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
     w.shade = static_cast<float4*>(ctx->gi_w_shade.ptr);
     w.slots = static_cast<float4*>(ctx->gi_w_slots.ptr);
     w.vis = static_cast<uint32_t*>(ctx->gi_w_vis.ptr);
+    w.dep_item = track ? static_cast<float4*>(ctx->gi_w_dep.ptr) : nullptr;
     w.shadow = static_cast<float4*>(ctx->gi_w_shadow.ptr);
     w.counters = static_cast<uint32_t*>(ctx->gi_w_counters.ptr);
     w.wave_cnt = w.counters + 4;
@@ -1310,6 +1333,7 @@ int history_pass(lux_ctx* ctx, uint64_t count, hipStream_t st = nullptr) {
     if (!ctx->gi_hist_on || count == 0) return LUX_OK;
     if (!st) st = ctx->stream;
     GiArgs a = gi_args(ctx);
+    hipLaunchKernelGGL(k_fill_u32_one, dim3(1), dim3(1), 0, st, a.hist_state + 2, 0u);
     hipLaunchKernelGGL(k_gi_history, dim3((uint32_t)((count + kWg - 1) / kWg)), dim3(kWg), 0, st, a, count);
     hipLaunchKernelGGL(k_gi_history_done, dim3(1), dim3(64), 0, st, a.hist_state);
     LUX_HIP(ctx, hipGetLastError());
@@ -1322,12 +1346,11 @@ int history_alloc(lux_ctx* ctx) {
     ctx->gi_hist_valid = false;
     if (!ctx->gi_hist_on || !ctx->gi_ready) return LUX_OK;
     if (int rc = lux_ensure(ctx, ctx->gi_samples, 4ull * ctx->gi_total)) return rc;
-    if (int rc = lux_ensure(ctx, ctx->gi_history, 32ull * ctx->gi_total)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->gi_history, 64ull * ctx->gi_total)) return rc;
+    if (int rc = lux_ensure(ctx, ctx->gi_dep, 32ull * ctx->gi_total)) return rc;
     if (int rc = lux_ensure(ctx, ctx->gi_hist_state, 32ull + 32ull * kMaxDirtyBoxes)) return rc;
     ctx->gi_dirty_n = 0;
-    uint32_t st[8] = {0u, 0u};
-    std::memcpy(st + 2, ctx->gi_emit_lo, 12);
-    std::memcpy(st + 5, ctx->gi_emit_hi, 12);
+    uint32_t st[8] = {};
     LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     LUX_HIP(ctx, hipMemcpy(ctx->gi_hist_state.ptr, st, 32, hipMemcpyHostToDevice));
     return LUX_OK;
@@ -1352,26 +1375,42 @@ extern "C" int lux_gi_reset_history(lux_ctx* ctx) {
     return lux_device_zero(ctx, ctx->gi_hist_state.ptr, 8);
 }
 
+// Appends dirty boxes (6 floats each: lo, hi) for the next history pass;
+// empty boxes are skipped; past the capacity the whole history is reset.
+int lux_gi_invalidate_boxes(lux_ctx* ctx, const float* boxes, uint32_t count) {
+    if (!ctx->gi_hist_on || !ctx->gi_hist_state.ptr) return LUX_OK;
+    std::vector<float> add;
+    for (uint32_t b = 0; b < count; ++b) {
+        const float* lo = boxes + 6ull * b;
+        const float* hi = lo + 3;
+        if (!(lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2])) continue;
+        add.insert(add.end(), {lo[0], lo[1], lo[2], 0.0f, hi[0], hi[1], hi[2], 0.0f});
+    }
+    const uint32_t n_add = (uint32_t)(add.size() / 8);
+    if (n_add == 0) return LUX_OK;
+    if (ctx->gi_dirty_n + n_add > kMaxDirtyBoxes) return lux_gi_reset_history(ctx);
+    const uint32_t n = ctx->gi_dirty_n + n_add;
+    char* st = static_cast<char*>(ctx->gi_hist_state.ptr);
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LUX_HIP(ctx, hipMemcpy(st + 32 + 32ull * ctx->gi_dirty_n, add.data(), 32ull * n_add, hipMemcpyHostToDevice));
+    LUX_HIP(ctx, hipMemcpy(st + 4, &n, 4, hipMemcpyHostToDevice));
+    ctx->gi_dirty_n = n;
+    return LUX_OK;
+}
+
 extern "C" int lux_gi_invalidate_history(lux_ctx* ctx, const float* lo, const float* hi) {
     if (!ctx) return LUX_ERR_INVALID;
     if (!lo || !hi) return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_invalidate_history: null box");
+    const float box[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
+    return lux_gi_invalidate_boxes(ctx, box, 1);
+}
+
+extern "C" int lux_gi_history_reused(lux_ctx* ctx, uint32_t* reused) {
+    if (!ctx || !reused) return LUX_ERR_INVALID;
+    *reused = 0;
     if (!ctx->gi_hist_on || !ctx->gi_hist_state.ptr) return LUX_OK;
-    bool empty = false, emitters = ctx->gi_lights > 0;
-    for (int q = 0; q < 3; ++q) {
-        if (!(lo[q] <= hi[q])) empty = true;
-        if (!(lo[q] <= ctx->gi_emit_hi[q] && hi[q] >= ctx->gi_emit_lo[q])) emitters = false;
-    }
-    if (empty) return LUX_OK;
-    // a change that may touch an emitter reaches every interval through its
-    // NEE occlusion rays; past the box capacity the whole history goes too
-    if (emitters || ctx->gi_dirty_n >= kMaxDirtyBoxes) return lux_gi_reset_history(ctx);
-    float box[8] = {lo[0], lo[1], lo[2], 0.0f, hi[0], hi[1], hi[2], 0.0f};
-    uint32_t n = ctx->gi_dirty_n + 1;
-    char* st = static_cast<char*>(ctx->gi_hist_state.ptr);
     LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    LUX_HIP(ctx, hipMemcpy(st + 32 + 32ull * ctx->gi_dirty_n, box, 32, hipMemcpyHostToDevice));
-    LUX_HIP(ctx, hipMemcpy(st + 4, &n, 4, hipMemcpyHostToDevice));
-    ctx->gi_dirty_n = n;
+    LUX_HIP(ctx, hipMemcpy(reused, static_cast<char*>(ctx->gi_hist_state.ptr) + 8, 4, hipMemcpyDeviceToHost));
     return LUX_OK;
 }
 
@@ -1401,7 +1440,25 @@ extern "C" int lux_gi_update(lux_ctx* ctx, float* ms) {
 
 // History blend (when on), merge and pack of a context's intervals on a
 // given stream: the end of an update whose intervals another pass wrote.
+namespace {
+__global__ void k_zero_f4(float4* p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+}  // namespace
+
+int lux_gi_dep_unknown(lux_ctx* ctx, hipStream_t st) {
+    if (!ctx->gi_hist_on || !ctx->gi_dep.ptr || ctx->gi_total == 0) return LUX_OK;
+    const uint64_t n = 2ull * ctx->gi_total;  // known flag 0: unknown
+    hipLaunchKernelGGL(k_zero_f4, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
+                       static_cast<float4*>(ctx->gi_dep.ptr), n);
+    LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
 int lux_gi_finish_internal(lux_ctx* ctx, hipStream_t st) {
+    // receiver deposits record no segments: their dependency is unknown
+    if (int rc = lux_gi_dep_unknown(ctx, st)) return rc;
     if (int rc = history_pass(ctx, ctx->gi_total, st)) return rc;
     if (int rc = merge_levels(ctx, (int)ctx->gi.levels - 1, st)) return rc;
     GiArgs a = gi_args(ctx);

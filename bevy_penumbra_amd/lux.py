@@ -445,6 +445,7 @@ def load_library() -> C.CDLL:
         "lux_gi_reset_history": (i32, [vp]),
         "lux_gi_invalidate_history": (i32, [vp, vp, vp]),
         "lux_gi_history_arrays": (i32, [vp, vp, vp]),
+        "lux_gi_history_reused": (i32, [vp, C.POINTER(C.c_uint32)]),
         "lux_radiance_cascades_history_params": (i32, [vp, vp]),
         "lux_radiance_cascades_receiver_grid": (i32, [vp, u32, u32, vp, vp]),
         "lux_radiance_cascades_update_budget": (i32, [vp, u32, u32, vp]),
@@ -890,15 +891,23 @@ class Context:
         b = (C.c_float * 3)(*[float(x) for x in hi])
         self._check(self.lib.lux_gi_invalidate_history(self.h, a, b), "lux_gi_invalidate_history")
 
+    def gi_history_reused(self) -> int:
+        """Intervals whose history the last history pass reused."""
+        r = C.c_uint32()
+        self._check(self.lib.lux_gi_history_reused(self.h, C.byref(r)), "lux_gi_history_reused")
+        return int(r.value)
+
     def gi_history_download(self):
-        """(history (n, 8) float32, samples (n,) uint32) of the last update."""
+        """(history (n, 16) float32, samples (n,) uint32) of the last update:
+        J.rgb, radiance mass, beta, transmittance mass, last frame, 0, then the
+        retained dependency (lo.xyz, known, hi.xyz, has bounds)."""
         hp, sp = C.c_void_p(), C.c_void_p()
         self._check(self.lib.lux_gi_history_arrays(self.h, C.byref(hp), C.byref(sp)), "lux_gi_history_arrays")
         if not hp.value:
             return None, None
         n = self.gi_sizes()[0]
         self.synchronize()
-        hist = np.zeros((n, 8), np.float32)
+        hist = np.zeros((n, 16), np.float32)
         ns = np.zeros(n, np.uint32)
         self._check(self.lib.lux_copy_to_host(self.h, hist.ctypes.data, hp.value, hist.nbytes), "lux_copy_to_host")
         self._check(self.lib.lux_copy_to_host(self.h, ns.ctypes.data, sp.value, ns.nbytes), "lux_copy_to_host")

@@ -371,16 +371,27 @@ typedef struct {
 int lux_gi_set_history(lux_ctx* ctx, const lux_gi_history_params* params);
 int lux_gi_reset_history(lux_ctx* ctx);
 /* Dirty region (world AABB): the next update reuses no history of an
- * interval whose dependency box meets it -- the box of probe centre +-
- * interval end joined with the emitter bounds (every terminal interval).  A
- * region touching an emitter, or more than 16 pending regions, drops the
- * whole history.  lux_scene_set_transforms marks the old and new boxes of
- * every moved instance (more than 8 moved instances: the whole history). */
+ * interval whose retained dependency meets it (dependencyIntersectsDirty,
+ * shaders/split.slang:13-25).  An interval's dependency is the union of the
+ * world segments its supported rays traversed (from the probe-side start to
+ * the hit, clipped to the scene's root box) and, for rays that hit, their NEE
+ * occlusion segments, joined over the updates its history kept
+ * (DependencyContext, shaders/common.slang:138-242; deposit,
+ * shaders/split.slang:1085-1105); receiver-driven updates record an unknown
+ * dependency, which any region meets.  More than 1024 pending regions drop the
+ * whole history.  lux_scene_set_transforms marks the old and new boxes of every
+ * moved instance; a moved box leaving the previous scene root, or a change of
+ * the emitters' sampling probabilities (beyond the rounding of a rigid
+ * motion), drops the whole history. */
 int lux_gi_invalidate_history(lux_ctx* ctx, const float* lo, const float* hi);
-/* Device addresses of the history (8 floats per interval: J.rgb, radiance
- * mass, beta, transmittance mass, last sampled frame bits, 0) and of the
+/* Device addresses of the history (16 floats per interval: J.rgb, radiance
+ * mass, beta, transmittance mass, last sampled frame bits, 0, then the
+ * retained dependency lo.xyz, known, hi.xyz, has bounds) and of the
  * supported-ray counts of the last update; NULL when history is off. */
 int lux_gi_history_arrays(lux_ctx* ctx, float** history, uint32_t** samples);
+/* Intervals whose history the last history pass reused (valid, alive, with
+ * a dependency meeting no dirty region, and holding mass); synchronous. */
+int lux_gi_history_reused(lux_ctx* ctx, uint32_t* reused);
 
 /* ---- receiver-driven updates (the sparse Split path, shaders/split.slang:
  *      567-1476, src/render.rs:9640-9794 for a frame's dispatches) ----

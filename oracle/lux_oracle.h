@@ -223,8 +223,12 @@ typedef struct {
 void orc_gi_trace_samples(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                           const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                           float* intervals, uint32_t* samples, int threads);
-void orc_gi_history(const orc_gi_params* p, const orc_history_params* h, const float* emit, const float* boxes,
-                    uint32_t n_boxes, uint64_t count, float* intervals, const uint32_t* samples, float* history);
+void orc_gi_history(const orc_gi_params* p, const orc_history_params* h, const float* boxes,
+                    uint32_t n_boxes, uint64_t count, float* intervals, const uint32_t* samples, const float* dep,
+                    float* history);
+void orc_gi_trace_dep(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
+                      const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
+                      float* intervals, uint32_t* samples, float* dep, int threads);
 void orc_gi_merge_history(const orc_gi_params* p, const float* intervals, const float* history, float* merged);
 void orc_gi_merge_f64(const orc_gi_params* p, const double* intervals, double* merged);
 /* 36 octahedral texels (irradiance/pi rgb, coverage) per level-0 probe. */

@@ -269,7 +269,8 @@ def _tsig():
         ("orc_gi_trace_f64", None, [vp, vp, u32, vp, u32, vp, vp, i32]),
         ("orc_gi_merge", None, [vp, vp, vp]),
         ("orc_gi_trace_samples", None, [vp, vp, u32, vp, u32, vp, vp, vp, i32]),
-        ("orc_gi_history", None, [vp, vp, vp, vp, u32, C.c_uint64, vp, vp, vp]),
+        ("orc_gi_history", None, [vp, vp, vp, u32, C.c_uint64, vp, vp, vp, vp]),
+        ("orc_gi_trace_dep", None, [vp, vp, u32, vp, u32, vp, vp, vp, vp, i32]),
         ("orc_gi_merge_history", None, [vp, vp, vp, vp]),
         ("orc_gi_merge_f64", None, [vp, vp, vp]),
         ("orc_gi_pack", None, [vp, vp, vp]),
@@ -372,12 +373,21 @@ def build_lights(scene, cap: int = 1 << 20) -> np.ndarray:
 
 
 def gi_trace(built: BuiltScene, lights: np.ndarray, materials: np.ndarray, p: GiParams,
-             threads: int = 8, f64: bool = False, samples: bool = False):
-    """Intervals (n, 4); with samples=True also the supported rays per interval."""
+             threads: int = 8, f64: bool = False, samples: bool = False, dep: bool = False):
+    """Intervals (n, 4); with samples=True also the supported rays per
+    interval; with dep=True also (samples and) each interval's dependency
+    (n, 8: lo.xyz, known, hi.xyz, has bounds)."""
     lib = _tsig()
     lights = np.ascontiguousarray(lights, np.float32)
     mats = np.ascontiguousarray(materials, np.float32)
     out = np.zeros((total_intervals(p), 4), np.float64 if f64 else np.float32)
+    if dep:
+        assert not f64
+        ns = np.zeros(total_intervals(p), np.uint32)
+        dp = np.zeros((total_intervals(p), 8), np.float32)
+        lib.orc_gi_trace_dep(C.byref(built._s), lights.ctypes.data, lights.shape[0], mats.ctypes.data,
+                             mats.shape[0], C.byref(p), out.ctypes.data, ns.ctypes.data, dp.ctypes.data, threads)
+        return out, ns, dp
     if samples:
         assert not f64
         ns = np.zeros(total_intervals(p), np.uint32)
@@ -423,23 +433,24 @@ class HistoryParams(C.Structure):
 
 def gi_history(p: GiParams, radiance_blend: float, transmittance_blend: float, history_frames: int,
                persistent: int, valid: bool, intervals: np.ndarray, samples: np.ndarray, history: np.ndarray,
-               emit=None, boxes=(), count: int | None = None):
+               dep: np.ndarray, boxes=(), count: int | None = None):
     """One temporal history pass over intervals [0, count): returns the
-    blended intervals and the new history (n, 8); emit = emitter bounds
-    (lo, hi) or None (no emitters); boxes = dirty (lo, hi) pairs."""
+    blended intervals and the new history (n, 16: J.rgb, radiance mass, beta,
+    transmittance mass, last frame, 0, then the retained dependency lo.xyz,
+    known, hi.xyz, has bounds); dep (n, 8) = this trace's dependency
+    (gi_trace(..., dep=True)); boxes = dirty (lo, hi) pairs."""
     lib = _tsig()
     h = HistoryParams(radiance_blend, transmittance_blend, history_frames, persistent, int(bool(valid)))
     iv = np.array(intervals, np.float32, copy=True, order="C")
     hs = np.array(history, np.float32, copy=True, order="C")
     ns = np.ascontiguousarray(samples, np.uint32)
-    em = np.array([np.inf] * 3 + [-np.inf] * 3, np.float32) if emit is None else \
-        np.concatenate([np.asarray(emit[0], np.float32), np.asarray(emit[1], np.float32)])
+    dp = np.ascontiguousarray(dep, np.float32)
     bx = np.zeros((max(1, len(boxes)), 6), np.float32)
     for i, (lo, hi) in enumerate(boxes):
         bx[i, :3], bx[i, 3:] = lo, hi
     n = iv.shape[0] if count is None else count
-    lib.orc_gi_history(C.byref(p), C.byref(h), em.ctypes.data, bx.ctypes.data, len(boxes), n, iv.ctypes.data,
-                       ns.ctypes.data, hs.ctypes.data)
+    lib.orc_gi_history(C.byref(p), C.byref(h), bx.ctypes.data, len(boxes), n, iv.ctypes.data,
+                       ns.ctypes.data, dp.ctypes.data, hs.ctypes.data)
     return iv, hs
 
 

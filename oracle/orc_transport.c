@@ -373,6 +373,102 @@ void orc_lod_select(const orc_lod_view* v, const float* position, uint32_t* low,
     }
 }
 
+/* ---------------- dependencies (f32) ----------------
+ * DependencyContext (shaders/common.slang:138-242): the union of world
+ * segments that established a value, known or unknown, measured-empty or
+ * bounded.  A ray's segment is its part inside the scene's root box, from
+ * max(root entry, 0) to the hit (closest-hit rays that hit) or to the root
+ * exit clipped at the ray's limit (traceRayTracked / traceOccludedTracked,
+ * shaders/traversal.slang:696-810,1097-1150); endpoints are widened by
+ * max(scale * 1e-6, 2^-23), scale = max(1, |coords|) (dependencyIncludeSegment). */
+typedef struct {
+    float lo[3], hi[3];
+    int known, has;
+} orc_dep;
+
+static inline float dmin_(float a, float b) { return b < a ? b : a; }
+static inline float dmax_(float a, float b) { return b > a ? b : a; }
+
+static orc_dep dep_empty(void) {
+    orc_dep d;
+    memset(&d, 0, sizeof d);
+    d.known = 1;
+    return d;
+}
+static orc_dep dep_unknown(void) {
+    orc_dep d = dep_empty();
+    d.known = 0;
+    return d;
+}
+static int finite3_(const float v[3]) { return finitef_(v[0]) && finitef_(v[1]) && finitef_(v[2]); }
+static void dep_include_bounds(orc_dep* d, const float mn[3], const float mx[3]) {
+    if (!d->known) return;
+    if (!finite3_(mn) || !finite3_(mx) || mn[0] > mx[0] || mn[1] > mx[1] || mn[2] > mx[2]) {
+        *d = dep_unknown();
+        return;
+    }
+    if (!d->has) {
+        memcpy(d->lo, mn, 12);
+        memcpy(d->hi, mx, 12);
+        d->has = 1;
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            d->lo[a] = dmin_(d->lo[a], mn[a]);
+            d->hi[a] = dmax_(d->hi[a], mx[a]);
+        }
+    }
+}
+static void dep_include_segment(orc_dep* d, const float st[3], const float en[3]) {
+    if (!finite3_(st) || !finite3_(en)) {
+        *d = dep_unknown();
+        return;
+    }
+    float scale = dmax_(1.0f, dmax_(dmax_(fabsf(st[0]), fabsf(st[1])), fabsf(st[2])));
+    scale = dmax_(scale, dmax_(dmax_(fabsf(en[0]), fabsf(en[1])), fabsf(en[2])));
+    const float r = dmax_(scale * 1e-6f, 1.1920929e-7f);
+    float mn[3], mx[3];
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = dmin_(st[a], en[a]) - r;
+        mx[a] = dmax_(st[a], en[a]) + r;
+    }
+    dep_include_bounds(d, mn, mx);
+}
+static void dep_union(orc_dep* d, const orc_dep* o) {
+    if (!d->known || !o->known) {
+        *d = dep_unknown();
+        return;
+    }
+    if (o->has) dep_include_bounds(d, o->lo, o->hi);
+}
+static void dep_ray(orc_dep* d, const orc_scene* s, const float o[3], const float dir[3], float limit, int hit,
+                    float t) {
+    if (s->n_inst == 0) return;
+    float e, x;
+    if (!orc_box_interval(o, dir, s->tlas[0].mn, s->tlas[0].mx, limit, &e, &x)) return;
+    const float s0 = dmax_(e, 0.0f), s1 = hit ? dmin_(t, x) : x;
+    float a[3], b[3];
+    for (int q = 0; q < 3; ++q) {
+        a[q] = o[q] + dir[q] * s0;
+        b[q] = o[q] + dir[q] * s1;
+    }
+    dep_include_segment(d, a, b);
+}
+static void dep_store(float* out, const orc_dep* d) {
+    out[0] = d->lo[0]; out[1] = d->lo[1]; out[2] = d->lo[2]; out[3] = d->known ? 1.0f : 0.0f;
+    out[4] = d->hi[0]; out[5] = d->hi[1]; out[6] = d->hi[2]; out[7] = d->has ? 1.0f : 0.0f;
+}
+static orc_dep dep_load(const float* in) {
+    orc_dep d;
+    memcpy(d.lo, in, 12);
+    memcpy(d.hi, in + 4, 12);
+    d.known = in[3] > 0.5f;
+    d.has = in[7] > 0.5f;
+    if (d.known && d.has && (!finite3_(d.lo) || !finite3_(d.hi) || d.lo[0] > d.hi[0] || d.lo[1] > d.hi[1] ||
+                             d.lo[2] > d.hi[2]))
+        d.known = 0;
+    return d;
+}
+
 #define REAL float
 #define SFX(name) name##_f32
 #include "orc_transport_impl.h"
@@ -387,24 +483,32 @@ void orc_lod_select(const orc_lod_view* v, const float* position, uint32_t* low,
 void orc_gi_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                   const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                   float* intervals, int threads) {
-    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads, 1);
+    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads, 1, NULL);
+}
+/* The interval trace with, per interval, its supported rays and the
+ * dependency of this trace (8 floats: lo.xyz, known, hi.xyz, has bounds):
+ * each supported ray's segment, then its NEE occlusion segments, in order. */
+void orc_gi_trace_dep(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
+                      const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
+                      float* intervals, uint32_t* samples, float* dep, int threads) {
+    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, samples, threads, 1, dep);
 }
 /* Every stride-th interval (g = 0, stride, 2 stride, ...) only; the others
  * are left as they are (CPU timing of a bounded fraction of an update). */
 void orc_gi_trace_stride(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                          const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                          float* intervals, int threads, uint64_t stride) {
-    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads, stride);
+    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads, stride, NULL);
 }
 void orc_gi_trace_f64(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                       const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                       double* intervals, int threads) {
-    gi_trace_f64(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads, 1);
+    gi_trace_f64(s, lights, n_lights, mats, n_mats, p, intervals, NULL, threads, 1, NULL);
 }
 void orc_gi_trace_samples(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                           const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,
                           float* intervals, uint32_t* samples, int threads) {
-    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, samples, threads, 1);
+    gi_trace_f32(s, lights, n_lights, mats, n_mats, p, intervals, samples, threads, 1, NULL);
 }
 void orc_gi_merge_history(const orc_gi_params* p, const float* intervals, const float* history, float* merged) {
     gi_merge_f32(p, intervals, history, merged);
@@ -427,8 +531,9 @@ static inline float hmass(float m) { return (finitef_(m) && m >= 0.0f) ? m : 0.0
  * n) for an integer n is a product of n factors.  history: 8 floats per
  * interval (J.rgb, radiance mass, beta, transmittance mass, last sampled
  * frame bits, 0).  emit: lo.xyz, hi.xyz; boxes: lo.xyz, hi.xyz each. */
-void orc_gi_history(const orc_gi_params* p, const orc_history_params* h, const float* emit, const float* boxes,
-                    uint32_t n_boxes, uint64_t count, float* intervals, const uint32_t* samples, float* history) {
+void orc_gi_history(const orc_gi_params* p, const orc_history_params* h, const float* boxes,
+                    uint32_t n_boxes, uint64_t count, float* intervals, const uint32_t* samples, const float* dep,
+                    float* history) {
     uint64_t off[17];
     for (uint32_t l = 0; l <= p->levels; ++l) off[l] = orc_gi_level_offset(p, l);
     const float rb = hmin(hsat(h->radiance_blend), 0.999f), tb = hmin(hsat(h->transmittance_blend), 0.999f);
@@ -437,35 +542,22 @@ void orc_gi_history(const orc_gi_params* p, const orc_history_params* h, const f
         uint32_t L = 0;
         while (L + 1 < p->levels && g >= off[L + 1]) ++L;
         float* iv = intervals + 4 * g;
-        float* hs = history + 8 * g;
+        float* hs = history + 16 * g;
         uint32_t n = samples[g];
         uint32_t last;
         memcpy(&last, hs + 6, 4);
         int reuse = h->valid && h->history_frames > 0;
         if (reuse) reuse = h->persistent || p->frame - last <= h->history_frames;
+        /* dependencyIntersectsDirty (split.slang:13-25) per dirty box: an
+         * unknown dependency meets every region, a measured-empty one none */
+        const orc_dep old = dep_load(hs + 8);
         if (reuse && n_boxes) {
-            uint32_t dims[3];
-            orc_gi_level_dims(p, L, dims);
-            uint32_t D = orc_gi_level_directions(p, L);
-            uint64_t probe = (g - off[L]) / D;
-            uint32_t ix = (uint32_t)(probe % dims[0]), iy = (uint32_t)((probe / dims[0]) % dims[1]);
-            uint32_t iz = (uint32_t)(probe / ((uint64_t)dims[0] * dims[1]));
-            float sl = p->spacing * ldexpf(1.0f, (int)L);
-            float c[3] = {p->origin[0] + ((float)ix + 0.5f) * sl, p->origin[1] + ((float)iy + 0.5f) * sl,
-                          p->origin[2] + ((float)iz + 0.5f) * sl};
-            float start, e;
-            orc_gi_interval_bounds(p, L, &start, &e);
-            int terminal = p->terminal_tail && L + 1 == p->levels;
-            float dlo[3], dhi[3];
-            for (int q = 0; q < 3; ++q) {
-                dlo[q] = hmin(c[q] - e, emit[q]);
-                dhi[q] = hmax(c[q] + e, emit[3 + q]);
-            }
-            for (uint32_t b = 0; b < n_boxes && reuse; ++b) {
+            if (!old.known) reuse = 0;
+            for (uint32_t b = 0; b < n_boxes && reuse && old.has; ++b) {
                 const float* bx = boxes + 6 * b;
-                int hit = terminal || (dlo[0] <= bx[3] && dhi[0] >= bx[0] && dlo[1] <= bx[4] && dhi[1] >= bx[1] &&
-                                       dlo[2] <= bx[5] && dhi[2] >= bx[2]);
-                if (hit) reuse = 0;
+                if (old.lo[0] <= bx[3] && old.hi[0] >= bx[0] && old.lo[1] <= bx[4] && old.hi[1] >= bx[1] &&
+                    old.lo[2] <= bx[5] && old.hi[2] >= bx[2])
+                    reuse = 0;
             }
         }
         int reuse_r = reuse && h->radiance_blend > 0.0f, reuse_t = reuse && h->transmittance_blend > 0.0f;
@@ -511,6 +603,15 @@ void orc_gi_history(const orc_gi_params* p, const orc_history_params* h, const f
         hs[4] = beta; hs[5] = ts;
         memcpy(hs + 6, &last, 4);
         hs[7] = 0.0f;
+        /* the retained value depends on this trace's segments and, when the
+         * history was reused, on everything the history depended on */
+        orc_dep cur = dep_load(dep + 8 * g);
+        if (reuse) {
+            orc_dep u = old;
+            dep_union(&u, &cur);
+            cur = u;
+        }
+        dep_store(hs + 8, &cur);
     }
 }
 
@@ -666,7 +767,7 @@ void orc_rx_trace(const orc_scene* s, const orc_light* lights, uint32_t n_lights
             if (h.flags & ORC_ERR_UNSUPPORTED) { stats[1]++; continue; }
             float rad[3] = {0.0f, 0.0f, 0.0f};
             const int hit = h.triangle != ORC_EMPTY;
-            if (hit && !shade_f32(s, lights, n_lights, mats, n_mats, p0, o, d, h.t, ts, is, rad)) {
+            if (hit && !shade_f32(s, lights, n_lights, mats, n_mats, p0, o, d, h.t, ts, is, rad, NULL)) {
                 stats[1]++;
                 continue;
             }

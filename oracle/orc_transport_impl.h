@@ -53,7 +53,7 @@ static REAL SFX(disney)(const REAL n[3], const REAL l[3], const REAL v[3], REAL 
 static int SFX(shade)(const orc_scene* s, const orc_light* lights, uint32_t nl,
                       const orc_material* mats, uint32_t nm, const orc_gi_params* p,
                       const float o[3], const float d[3], float t, uint32_t tri_slot,
-                      uint32_t inst_slot, REAL out[3]) {
+                      uint32_t inst_slot, REAL out[3], orc_dep* dep) {
     const orc_instance* in = &s->inst[inst_slot];
     const float* tri = s->tri + 12ull * tri_slot;
     /* a nonzero instance material (scenes given as scene words) wins */
@@ -158,6 +158,10 @@ static int SFX(shade)(const orc_scene* s, const orc_light* lights, uint32_t nl,
                 ray.flags = ORC_RAY_ANY_HIT;
                 orc_hit vh;
                 orc_trace_ray(s, &ray, &vh, NULL, NULL, NULL);
+                if (dep) {
+                    const float ro[3] = {ray.ox, ray.oy, ray.oz}, rd[3] = {ray.dx, ray.dy, ray.dz};
+                    dep_ray(dep, s, ro, rd, ray.tmax, 0, 0.0f);
+                }
                 if (vh.flags & ORC_HIT_OCCLUDED) continue;
                 if (vh.flags & ORC_ERR_UNSUPPORTED) return 0;
                 REAL dm = SFX(disney)(nrm, Ld, view, rough) / (REAL)ORC_PI_F;
@@ -189,6 +193,7 @@ typedef struct {
     uint32_t* samples; /* nullable: supported rays per interval */
     uint64_t begin, end; /* positions k of the traced intervals g = k * stride */
     uint64_t stride;
+    float* dep; /* nullable (f32 only): 8 floats per interval, this trace's dependency */
 } SFX(trace_job);
 
 static void* SFX(trace_worker)(void* arg) {
@@ -220,6 +225,7 @@ static void* SFX(trace_worker)(void* arg) {
         float limit = terminal ? 1e20f : end - start;
         REAL jsum[3] = {K(0.0), K(0.0), K(0.0)}, bsum = K(0.0);
         uint32_t n = 0;
+        orc_dep idep = dep_empty();
         uint32_t R = p->rays_per_interval ? p->rays_per_interval : 1u;
         for (uint32_t r = 0; r < R; ++r) {
             float ouv[2], d[3];
@@ -238,19 +244,25 @@ static void* SFX(trace_worker)(void* arg) {
             if (h.flags & ORC_ERR_UNSUPPORTED) continue;
             REAL rad[3] = {K(0.0), K(0.0), K(0.0)};
             REAL beta;
+            /* the ray's segment, then its NEE occlusion segments */
+            orc_dep rdep = dep_empty();
+            float o3[3] = {ray.ox, ray.oy, ray.oz};
+            if (j->dep) dep_ray(&rdep, j->s, o3, d, ray.tmax, h.triangle != ORC_EMPTY, h.t);
             if (h.triangle != ORC_EMPTY) {
-                float o3[3] = {ray.ox, ray.oy, ray.oz};
-                if (!SFX(shade)(j->s, j->lights, j->nl, j->mats, j->nm, p, o3, d, h.t, ts, is, rad))
+                if (!SFX(shade)(j->s, j->lights, j->nl, j->mats, j->nm, p, o3, d, h.t, ts, is, rad,
+                                j->dep ? &rdep : NULL))
                     continue;
                 beta = K(0.0);
             } else {
                 beta = terminal ? K(0.0) : K(1.0); /* escape: black environment */
             }
+            if (j->dep) dep_union(&idep, &rdep);
             for (int a = 0; a < 3; ++a) jsum[a] = jsum[a] + rad[a];
             bsum = bsum + beta;
             n++;
         }
         if (j->samples) j->samples[g] = n;
+        if (j->dep) dep_store(j->dep + 8 * g, &idep);
         REAL* out = j->intervals + 4 * g;
         if (n == 0) {
             out[0] = out[1] = out[2] = K(0.0);
@@ -266,7 +278,7 @@ static void* SFX(trace_worker)(void* arg) {
 
 static void SFX(gi_trace)(const orc_scene* s, const orc_light* lights, uint32_t nl,
                           const orc_material* mats, uint32_t nm, const orc_gi_params* p,
-                          REAL* intervals, uint32_t* samples, int threads, uint64_t stride) {
+                          REAL* intervals, uint32_t* samples, int threads, uint64_t stride, float* dep) {
     if (stride < 1) stride = 1;
     uint64_t total = (orc_gi_level_offset(p, p->levels) + stride - 1) / stride;
     if (threads < 1) threads = 1;
@@ -280,6 +292,7 @@ static void SFX(gi_trace)(const orc_scene* s, const orc_light* lights, uint32_t 
         jobs[t].begin = per * (uint64_t)t < total ? per * (uint64_t)t : total;
         jobs[t].end = jobs[t].begin + per < total ? jobs[t].begin + per : total;
         jobs[t].stride = stride;
+        jobs[t].dep = dep;
     }
     for (int t = 1; t < threads; ++t) pthread_create(&tids[t], NULL, SFX(trace_worker), &jobs[t]);
     SFX(trace_worker)(&jobs[0]);
@@ -355,7 +368,7 @@ static void SFX(gi_merge)(const orc_gi_params* p, const REAL* intervals, const f
                 int has;
                 REAL tr;
                 if (history) {
-                    float rm = history[8 * g + 3], tm = history[8 * g + 5];
+                    float rm = history[16 * g + 3], tm = history[16 * g + 5];
                     tr = tm > 0.0f ? SFX(sat)(iv[3]) : K(0.0);
                     has = rm > 0.0f && tm > 0.0f;
                 } else {

@@ -179,9 +179,8 @@ def test_gpu_receiver_updates_with_history_match_oracle():
         g0 = to_oracle(p0)
         built = O.build_scene(sc)
         lights = O.build_lights(sc)
-        pts = np.concatenate([np.stack([l[0:3], l[0:3] + l[12:15], l[0:3] + l[16:19]]) for l in lights])
-        emit = (pts.min(axis=0), pts.max(axis=0))
-        hist = np.zeros((O.total_intervals(g0), 8), np.float32)
+        hist = np.zeros((O.total_intervals(g0), 16), np.float32)
+        unknown = np.zeros((O.total_intervals(g0), 8), np.float32)   # receiver deposits record no segments
         sched = lux.CandidateSchedule()
         for f in range(3):
             cursor, quota = sched.phase_for(W, H, W * H, 2000)
@@ -198,7 +197,7 @@ def test_gpu_receiver_updates_with_history_match_oracle():
             g0.frame = rp.frame
             ivs, sms, _ = O.rx_trace(built, lights, sc.materials, [g0], rec, lods, sel["sequence"], sel["tickets"],
                                      rp.frame)
-            blended, hist = O.gi_history(g0, RB, TB, FRAMES, 1, f > 0, ivs[0], sms[0], hist, emit=emit)
+            blended, hist = O.gi_history(g0, RB, TB, FRAMES, 1, f > 0, ivs[0], sms[0], hist, unknown)
             merged = O.gi_merge(g0, blended, hist)
             g = c.gi_download()
             gh, gs = c.gi_history_download()

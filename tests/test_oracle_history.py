@@ -7,13 +7,23 @@ saturate both masses at 1 / (1 - blend); one opposite observation moves J to
 bit for bit; four observations at once retain j^4 and b^4; each channel's
 history is used only when its blend is positive.  Beyond those: the masses
 follow the geometric series, J is the geometric-weighted mean of the
-observations, a dirty box drops exactly the intervals whose dependency box
-meets it, and the merge of a first update with history is the plain merge.
+observations, a dirty box drops exactly the intervals whose retained
+dependency meets it (hand-built dependencies: bounded, unknown, measured
+empty; a reused value's dependency is the union of the old and the new), the
+trace's dependencies are its segments clipped to the scene's root box, and the
+merge of a first update with history is the plain merge.
 """
 import numpy as np
 import pytest
 
 from oracle import oracle as O
+
+
+def known_empty(n):
+    """Dependencies of n intervals: known, measured empty."""
+    d = np.zeros((n, 8), np.float32)
+    d[:, 3] = 1.0
+    return d
 
 
 def one_interval():
@@ -27,13 +37,13 @@ class Hist:
     def __init__(self, rb, tb, frames=30, persistent=1):
         self.p = one_interval()
         self.rb, self.tb, self.frames, self.persistent = rb, tb, frames, persistent
-        self.h = np.zeros((1, 8), np.float32)
+        self.h = np.zeros((1, 16), np.float32)
         self.valid = False
 
     def advance(self, radiance, beta, count):
         iv = np.array([[radiance] * 3 + [beta if count else -1.0]], np.float32)
         out, self.h = O.gi_history(self.p, self.rb, self.tb, self.frames, self.persistent, self.valid, iv,
-                                   np.array([count], np.uint32), self.h)
+                                   np.array([count], np.uint32), self.h, known_empty(1))
         self.valid = True
         return out[0]
 
@@ -121,40 +131,114 @@ def test_history_off_or_expired_starts_over():
     assert e.advance(4.0, 0.0, 0)[3] == 1.0               # 7 frames since the last sample: expired
 
 
-def test_dirty_box_drops_exactly_the_intervals_it_meets():
-    """Level-0 probe centres x = i + 0.5 (8 x 1 x 1 probes, spacing 1, one
-    direction) with interval end 0.4 have dependency boxes [i + 0.1, i +
-    0.9] in x and [0.1, 0.9] in y, z; a dirty box at x in [6.0, 6.15] meets
-    probe 6's only.  Level 1 is terminal: always dropped.  With emitter bounds
-    [0, 0.2]^3 every dependency box reaches back to x = 0, so probes 6 and 7
-    (boxes reaching x >= 6.0) are dropped."""
-    p = O.gi_params((0.0, 0.0, 0.0), 1.0, (8, 1, 1), 2, base_phi=1, base_y=1, interval_length=0.4,
-                    terminal_tail=1)
+def test_dirty_box_drops_exactly_the_intervals_whose_dependency_it_meets():
+    """Hand-built dependencies of eight intervals: boxes [i + 0.1, i + 0.9]
+    (x) x [0.1, 0.9]^2 for i = 0..4, an unknown one, a measured-empty one and
+    a box over x in [3, 6].  A dirty box over x in [3.95, 4.05] lies between
+    the boxes of intervals 3 and 4: only the unknown dependency and the box
+    over [3, 6] are dropped.  A second dirty box over [4.5, 4.6] drops
+    interval 4 as well.  Without dirty boxes everything is reused, the unknown
+    dependency included.  A box touching a dependency at one face meets it."""
+    p = O.gi_params((0.0, 0.0, 0.0), 1.0, (8, 1, 1), 1, base_phi=1, base_y=1, terminal_tail=0)
     n = O.total_intervals(p)
-    start, end = O.interval_bounds(p, 0)
-    assert abs(end - 0.4) < 1e-6
+    assert n == 8
+    dep = np.zeros((n, 8), np.float32)
+    for i in range(5):
+        dep[i] = [i + 0.1, 0.1, 0.1, 1.0, i + 0.9, 0.9, 0.9, 1.0]
+    dep[5] = 0.0                                                    # unknown
+    dep[6] = [0, 0, 0, 1.0, 0, 0, 0, 0.0]                           # measured empty
+    dep[7] = [3.0, 0.2, 0.2, 1.0, 6.0, 0.8, 0.8, 1.0]
     iv = np.zeros((n, 4), np.float32)
     iv[:, 0], iv[:, 3] = 3.0, 0.5
     ns = np.ones(n, np.uint32)
-    h0 = np.zeros((n, 8), np.float32)
-    _, h1 = O.gi_history(p, 0.9, 0.9, 30, 1, False, iv, ns, h0)
+    _, h1 = O.gi_history(p, 0.9, 0.9, 30, 1, False, iv, ns, np.zeros((n, 16), np.float32), dep)
+    assert np.array_equal(h1[:, 8:16], dep)                         # the first pass stores this trace's
     iv2 = iv.copy()
     iv2[:, 0] = 1.0
-    box = [((6.0, 0.4, 0.4), (6.15, 0.6, 0.6))]
-    out, _ = O.gi_history(p, 0.9, 0.9, 30, 1, True, iv2, ns, h1, boxes=box)
+    blended = (1.0 + 3.0 * 0.9) / 1.9
+    box = ((3.95, 0.0, 0.0), (4.05, 1.0, 1.0))
+    out, h2 = O.gi_history(p, 0.9, 0.9, 30, 1, True, iv2, ns, h1, known_empty(n), boxes=[box])
     fresh = out[:, 0] == 1.0
-    assert fresh.tolist() == [False] * 6 + [True] + [False] + [True] * (n - 8)
-    assert abs(out[0, 0] - (1.0 + 3.0 * 0.9) / 1.9) < 1e-6
-    out2, _ = O.gi_history(p, 0.9, 0.9, 30, 1, True, iv2, ns, h1, emit=((0.0, 0.0, 0.0), (0.2, 0.2, 0.2)),
-                           boxes=box)
-    assert (out2[:8, 0] == 1.0).tolist() == [False] * 6 + [True, True]
+    assert fresh.tolist() == [False, False, False, False, False, True, False, True]
+    assert abs(out[0, 0] - blended) < 1e-6
+    out, _ = O.gi_history(p, 0.9, 0.9, 30, 1, True, iv2, ns, h1, known_empty(n),
+                          boxes=[box, ((4.5, 0.4, 0.4), (4.6, 0.6, 0.6))])
+    assert (out[:, 0] == 1.0).tolist() == [False, False, False, False, True, True, False, True]
+    out, _ = O.gi_history(p, 0.9, 0.9, 30, 1, True, iv2, ns, h1, known_empty(n))
+    assert np.all(out[:, 0] != 1.0)                                 # no dirty box: all reused
+    out, _ = O.gi_history(p, 0.9, 0.9, 30, 1, True, iv2, ns, h1, known_empty(n),
+                          boxes=[((2.9, 0.0, 0.0), (2.9, 0.5, 0.5))])
+    assert (out[:, 0] == 1.0).tolist() == [False, False, True, False, False, True, False, False]
+    # a reused value keeps the old dependency joined with the new; a dropped
+    # one takes the new alone
+    new = np.zeros((n, 8), np.float32)
+    new[:] = [10.0, 0.0, 0.0, 1.0, 11.0, 1.0, 1.0, 1.0]
+    _, h3 = O.gi_history(p, 0.9, 0.9, 30, 1, True, iv2, ns, h1, new, boxes=[box])
+    assert np.array_equal(h3[0, 8:16], np.float32([0.1, 0.0, 0.0, 1.0, 11.0, 1.0, 1.0, 1.0]))
+    assert np.array_equal(h3[5, 8:16], new[5]) and np.array_equal(h3[7, 8:16], new[7])
+    assert np.array_equal(h3[6, 8:16], new[6])                      # empty joined with new = new
+
+
+def test_trace_dependencies_are_the_segments_clipped_to_the_root():
+    """The dense fixture (boxes in [-3.9, 2.0] x [-0.9, 3.04] x [-2.0, 4.03]):
+    every interval that missed (beta = 1) depends on its own segment from
+    c + d * start, clipped to the root box and to the interval's length,
+    widened by max(scale * 1e-6, 2^-23) -- checked in f64 with a tolerance;
+    a segment that misses the root is measured empty; an interval that hit
+    has a bounded dependency."""
+    from tests.transport_cases import dense_fixture_params, dense_fixture_scene
+    sc = dense_fixture_scene()
+    built = O.build_scene(sc)
+    p = dense_fixture_params()
+    iv, ns, dep = O.gi_trace(built, O.build_lights(sc), sc.materials, p, threads=8, dep=True)
+    assert np.all(dep[:, 3] == 1.0)                                  # every dependency known
+    root = built.tlas[0].view(np.float32)
+    lo, hi = root[:3].astype(np.float64), root[4:7].astype(np.float64)
+    checked = empty = hits = 0
+    for L in range(p.levels):
+        a, b = O.level_offset(p, L), O.level_offset(p, L + 1)
+        start, end = O.interval_bounds(p, L)
+        nd, dims = O.level_dims(p, L)
+        D = O.level_directions(p, L)
+        rng = np.random.default_rng(L)
+        for g in rng.choice(np.arange(a, b), size=400, replace=False):
+            local = g - a
+            probe, k = divmod(local, D)
+            ix, iy, iz = probe % dims[0], (probe // dims[0]) % dims[1], probe // (dims[0] * dims[1])
+            sl = 2.0 ** L
+            c = np.array([-8.0 + (ix + 0.5) * sl, -8.0 + (iy + 0.5) * sl, -8.0 + (iz + 0.5) * sl])
+            from tests.transport_cases import _dense_offset
+            d = O.bin_direction(k, p.base_phi << L, p.base_y << L, *_dense_offset(p.seed)).astype(np.float64)
+            o = c + d * start
+            with np.errstate(divide="ignore", invalid="ignore"):
+                t0, t1 = (lo - o) / d, (hi - o) / d
+            tn = max(0.0, np.nanmax(np.minimum(t0, t1)))
+            tf = min(end - start, np.nanmin(np.maximum(t0, t1)))
+            if iv[g, 3] == 1.0:
+                if tn > tf + 1e-4:
+                    assert dep[g, 7] == 0.0, g                       # missed the root: empty
+                    empty += 1
+                    continue
+                if tf - tn < 1e-3:
+                    continue                                         # grazing: too close to call
+                s0, s1 = o + d * tn, o + d * tf
+                r = max(np.abs(np.concatenate([s0, s1])).max(), 1.0) * 1e-6
+                assert dep[g, 7] == 1.0
+                assert np.allclose(dep[g, :3], np.minimum(s0, s1) - r, atol=1e-4), (g, dep[g], s0, s1)
+                assert np.allclose(dep[g, 4:7], np.maximum(s0, s1) + r, atol=1e-4), (g, dep[g], s0, s1)
+                checked += 1
+            elif iv[g, 3] == 0.0:
+                assert dep[g, 7] == 1.0 and np.all(dep[g, :3] <= dep[g, 4:7])   # a hit is bounded
+                hits += 1
+    assert checked > 50 and empty > 50 and hits > 10, (checked, empty, hits)
 
 
 def test_merge_of_a_first_update_with_history_is_the_plain_merge():
     from tests.transport_cases import four_cascade_case
     p, iv, _ = four_cascade_case()
     ns = (iv[:, 3] >= 0).astype(np.uint32)
-    blended, hist = O.gi_history(p, 0.88, 0.98, 24, 1, False, iv, ns, np.zeros((iv.shape[0], 8), np.float32))
+    blended, hist = O.gi_history(p, 0.88, 0.98, 24, 1, False, iv, ns, np.zeros((iv.shape[0], 16), np.float32),
+                                 known_empty(iv.shape[0]))
     a = O.gi_merge(p, iv)
     b = O.gi_merge(p, blended, hist)
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
@@ -168,11 +252,12 @@ def test_merge_uses_transmittance_without_radiance_mass():
     from tests.transport_cases import four_cascade_case
     p, iv, idx = four_cascade_case()
     ns = (iv[:, 3] >= 0).astype(np.uint32)
-    _, hist = O.gi_history(p, 0.0, 0.9, 24, 1, False, iv, ns, np.zeros((iv.shape[0], 8), np.float32))
+    _, hist = O.gi_history(p, 0.0, 0.9, 24, 1, False, iv, ns, np.zeros((iv.shape[0], 16), np.float32),
+                           known_empty(iv.shape[0]))
     g0 = idx["g0"]
     ns2 = ns.copy()
     ns2[g0] = 0
-    blended, hist2 = O.gi_history(p, 0.0, 0.9, 24, 1, True, iv, ns2, hist)
+    blended, hist2 = O.gi_history(p, 0.0, 0.9, 24, 1, True, iv, ns2, hist, known_empty(iv.shape[0]))
     assert hist2[g0, 3] == 0.0 and hist2[g0, 5] > 0.0
     m = O.gi_merge(p, blended, hist2)
     plain = O.gi_merge(p, iv)
