@@ -424,9 +424,13 @@ extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_wo
     // dirty regions for the GI history: the world boxes of every moved
     // instance before and after the move.  A measured-empty dependency (a ray
     // that missed the scene's root box) stays valid only while the root does
-    // not grow: a moved box leaving the previous root resets the history.
+    // not grow: a moved box leaving the previous root marks the next history
+    // pass to drop measured-empty dependencies (the reference's root-revision
+    // precondition, shaders/split.slang:19-21).  A moved instance that emits
+    // changes the light every NEE sample may reach, sampled or not: the
+    // whole history goes.
     std::vector<float> dirty;
-    bool reset_history = false;
+    bool root_grew = false, reset_history = false;
     if (ctx->gi_hist_on) {
         if (ctx->mesh_bounds.size() != 6ull * ctx->host_pos.size()) {
             ctx->mesh_bounds.assign(6ull * ctx->host_pos.size(), 0.0f);
@@ -470,19 +474,28 @@ extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_wo
                 root[3 + q] = std::max(root[3 + q], wb[3 + q]);
             }
         }
-        for (uint32_t i = 0; i < instance_count && !reset_history; ++i) {
+        for (uint32_t i = 0; i < instance_count; ++i) {
             const float* nf = world_from_local + 12ull * i;
             if (memcmp(ctx->instances[i].world_from_local, nf, 48) == 0) continue;
             const uint32_t m = ctx->instances[i].mesh;
             if (m >= ctx->host_pos.size() || ctx->host_pos[m].empty()) continue;
+            if (m < ctx->host_mat.size())
+                for (uint32_t mat : ctx->host_mat[m]) {
+                    if (mat >= ctx->host_materials.size()) continue;
+                    const float* e = ctx->host_materials[mat].emission;
+                    if (e[0] > 0.0f || e[1] > 0.0f || e[2] > 0.0f) {
+                        reset_history = true;
+                        break;
+                    }
+                }
             float ob[6], nb[6];
             world_box(m, ctx->instances[i].world_from_local, ob);
             world_box(m, nf, nb);
             for (int q = 0; q < 3; ++q) {
-                // inside the previous root, with a relative margin for the
-                // device's own rounding of the root box
+                // outside the previous root beyond a relative margin (the
+                // device rounds its own root box)
                 const float pad = 1e-5f * std::max(1.0f, std::max(std::fabs(root[q]), std::fabs(root[3 + q])));
-                if (!(nb[q] >= root[q] + pad && nb[3 + q] <= root[3 + q] - pad)) reset_history = true;
+                if (!(nb[q] >= root[q] - pad && nb[3 + q] <= root[3 + q] + pad)) root_grew = true;
             }
             dirty.insert(dirty.end(), ob, ob + 6);
             dirty.insert(dirty.end(), nb, nb + 6);
@@ -504,6 +517,8 @@ extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_wo
     ctx->tlas_built = false;
     ctx->gi_lights_dirty = true;
     if (reset_history) return lux_gi_reset_history(ctx);
+    if (root_grew)
+        if (int rc = lux_gi_root_grew(ctx)) return rc;
     return lux_gi_invalidate_boxes(ctx, dirty.data(), (uint32_t)(dirty.size() / 6));
 }
 

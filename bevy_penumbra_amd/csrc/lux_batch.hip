@@ -331,6 +331,7 @@ __global__ void k_b_history_done(const BatchScene* sc, uint32_t n) {
     if (i < n && sc[i].a.hist_state) {
         sc[i].a.hist_state[0] = 1u;
         sc[i].a.hist_state[1] = 0u;
+        sc[i].a.hist_state[3] = 0u;
     }
 }
 

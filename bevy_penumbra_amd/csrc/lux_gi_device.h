@@ -37,8 +37,9 @@ struct GiArgs {
     // interval of this update; per interval (J.rgb, radiance mass) and
     // (beta, transmittance mass, last sampled frame bits, 0); state words
     // [0] history valid, [1] dirty boxes, [2] intervals whose history the last
-    // pass reused, [3..7] unused, then 8 floats per dirty box (lo, pad, hi,
-    // pad) from word 8
+    // pass reused, [3] the root box grew (measured-empty dependencies are
+    // dropped), [4..7] unused, then 8 floats per dirty box (lo, pad, hi, pad)
+    // from word 8
     uint32_t* samples;
     float4* history;
     uint32_t* hist_state;
@@ -745,6 +746,8 @@ __device__ bool history_blend(const GiArgs& a, uint64_t g, uint32_t L, uint64_t 
     if (reuse) reuse = a.hist_persistent != 0u || p.frame - last <= a.hist_frames;
     uint32_t nd = st[1] < kMaxDirtyBoxes ? st[1] : kMaxDirtyBoxes;
     const Dep old = dep_load(a.history + 4 * g + 2);
+    // a measured-empty dependency holds only while the scene's root box did not grow
+    if (reuse && st[3] != 0u && old.known && !old.has) reuse = false;
     if (reuse && nd > 0u) {
         // dependencyIntersectsDirty (split.slang:13-25) against each dirty box:
         // an unknown dependency meets every region, a measured-empty one none

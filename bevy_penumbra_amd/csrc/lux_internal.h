@@ -94,9 +94,6 @@ struct lux_ctx {
     lux_gi_history_params gi_hist{};
     lux_buffer gi_samples, gi_history, gi_hist_state;
     lux_buffer gi_dep;  // per interval: this update's dependency (2 float4)
-    // host copy of the emitter sampling table (probability, CDF per light):
-    // a change of either resets the history (every NEE sample may move)
-    std::vector<float> gi_light_cdf;
     uint32_t gi_dirty_n = 0;                 // dirty boxes pending on the device
     // receiver-driven updates (lux_gi_update_receivers, state on the first
     // LOD's context): descriptor keys / flags / directory slots, per
@@ -175,6 +172,8 @@ int lux_surrogate_forward_into(lux_surrogate* s, hipStream_t st, const float* qu
 int lux_gi_check_internal(lux_ctx* ctx);
 // Dirty boxes (6 floats each: lo, hi) for the next history pass.
 int lux_gi_invalidate_boxes(lux_ctx* ctx, const float* boxes, uint32_t count);
+// The scene's root box grew: the next history pass drops measured-empty dependencies.
+int lux_gi_root_grew(lux_ctx* ctx);
 // This update's dependency of every interval set to unknown (updates whose
 // deposits record no segments: receiver-driven updates).
 int lux_gi_dep_unknown(lux_ctx* ctx, hipStream_t st);

@@ -519,6 +519,7 @@ __global__ void k_gi_history_done(uint32_t* state) {
     if (threadIdx.x == 0) {
         state[0] = 1u;
         state[1] = 0u;
+        state[3] = 0u;
     }
 }
 
@@ -875,19 +876,6 @@ int upload_lights(lux_ctx* ctx) {
         LUX_HIP(ctx, hipMemcpy(ctx->gi_light_buf.ptr, lights.data(), sizeof(Light) * lights.size(),
                                hipMemcpyHostToDevice));
     ctx->gi_lights_dirty = false;
-    // A moved emitter invalidates the intervals whose NEE segments reach its
-    // old or new place (dirty boxes); a changed choice of emitters (their
-    // number, or a sampling probability off by more than 1 %, i.e. a change of
-    // emitted power rather than the rounding of a rigid motion) may move
-    // every NEE sample: the whole history goes.
-    bool changed = ctx->gi_light_cdf.size() != lights.size();
-    for (size_t i = 0; i < lights.size() && !changed; ++i) {
-        const float p0 = ctx->gi_light_cdf[i], p1 = lights[i].edge1.w;
-        if (std::fabs(p1 - p0) > 0.01f * std::max(p0, 1.0f / 16777216.0f)) changed = true;
-    }
-    ctx->gi_light_cdf.resize(lights.size());
-    for (size_t i = 0; i < lights.size(); ++i) ctx->gi_light_cdf[i] = lights[i].edge1.w;
-    if (changed && ctx->gi_hist_valid) return lux_gi_reset_history(ctx);
     return LUX_OK;
 }
 
@@ -1395,6 +1383,16 @@ int lux_gi_invalidate_boxes(lux_ctx* ctx, const float* boxes, uint32_t count) {
     LUX_HIP(ctx, hipMemcpy(st + 32 + 32ull * ctx->gi_dirty_n, add.data(), 32ull * n_add, hipMemcpyHostToDevice));
     LUX_HIP(ctx, hipMemcpy(st + 4, &n, 4, hipMemcpyHostToDevice));
     ctx->gi_dirty_n = n;
+    return LUX_OK;
+}
+
+// The scene's root box grew: the next history pass drops measured-empty
+// dependencies (state word 3, cleared after the pass).
+int lux_gi_root_grew(lux_ctx* ctx) {
+    if (!ctx->gi_hist_on || !ctx->gi_hist_state.ptr) return LUX_OK;
+    const uint32_t one = 1u;
+    LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LUX_HIP(ctx, hipMemcpy(static_cast<char*>(ctx->gi_hist_state.ptr) + 12, &one, 4, hipMemcpyHostToDevice));
     return LUX_OK;
 }
 

@@ -366,13 +366,15 @@ def _uv_sphere(stacks: int, slices: int) -> np.ndarray:
     return np.stack([t1, t2], axis=2).reshape(-1, 3, 3)
 
 
-def dynamic(seed: int = 0, meshes: int = 256, instances: int = 256, stacks: int = 32, slices: int = 64):
+def dynamic(seed: int = 0, meshes: int = 256, instances: int = 256, stacks: int = 32, slices: int = 64,
+            emissive_every: int = 1):
     """The dynamic large frame: ``instances`` rigid instances of ``meshes``
     procedural meshes of 2 * stacks * slices triangles (defaults: one mesh per
     instance, 256 x 4096 = 1,048,576 triangles) moving inside a 30 x 12 x 15 m
     volume.  Each mesh is a
     UV sphere with seeded radial relief, scaled to an ellipsoid of 0.3-1.6 m
-    radii; the band below its top pole (2 * slices triangles) is emissive.  Returns
+    radii; the band below its top pole (2 * slices triangles) is emissive (on
+    every emissive_every-th mesh).  Returns
     (Scene at frame 0, motion) where ``motion`` feeds ``dynamic_transforms``."""
     rng = np.random.default_rng(seed)
     base = _uv_sphere(stacks, slices)
@@ -389,7 +391,8 @@ def dynamic(seed: int = 0, meshes: int = 256, instances: int = 256, stacks: int 
         tri = (p * r[:, None] * radii[None, :]).reshape(-1, 9).astype(np.float32)
         mesh_list.append(tri)
         mats = np.full(tri.shape[0], m % 3, np.uint32)
-        mats[2 * slices: 4 * slices] = 3
+        if m % emissive_every == 0:
+            mats[2 * slices: 4 * slices] = 3
         mat_list.append(mats)
     materials = np.asarray([
         [0.75, 0.75, 0.75, 1.0, 0, 0, 0, 0],

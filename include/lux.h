@@ -380,9 +380,10 @@ int lux_gi_reset_history(lux_ctx* ctx);
  * shaders/split.slang:1085-1105); receiver-driven updates record an unknown
  * dependency, which any region meets.  More than 1024 pending regions drop the
  * whole history.  lux_scene_set_transforms marks the old and new boxes of every
- * moved instance; a moved box leaving the previous scene root, or a change of
- * the emitters' sampling probabilities (beyond the rounding of a rigid
- * motion), drops the whole history. */
+ * moved instance; a moved instance with emissive triangles drops the whole
+ * history (the light every NEE sample may reach moved), and a moved box
+ * leaving the previous scene root makes the next pass drop measured-empty
+ * dependencies. */
 int lux_gi_invalidate_history(lux_ctx* ctx, const float* lo, const float* hi);
 /* Device addresses of the history (16 floats per interval: J.rgb, radiance
  * mass, beta, transmittance mass, last sampled frame bits, 0, then the

@@ -219,6 +219,7 @@ typedef struct {
     float radiance_blend, transmittance_blend;
     uint32_t history_frames, persistent;
     uint32_t valid;  /* a previous update wrote the history */
+    uint32_t root_grew; /* the scene's root box grew: measured-empty dependencies are dropped */
 } orc_history_params;
 void orc_gi_trace_samples(const orc_scene* s, const orc_light* lights, uint32_t n_lights,
                           const orc_material* mats, uint32_t n_mats, const orc_gi_params* p,

@@ -428,19 +428,23 @@ def gi_merge(p: GiParams, intervals: np.ndarray, history: np.ndarray | None = No
 
 class HistoryParams(C.Structure):
     _fields_ = [("radiance_blend", C.c_float), ("transmittance_blend", C.c_float),
-                ("history_frames", C.c_uint32), ("persistent", C.c_uint32), ("valid", C.c_uint32)]
+                ("history_frames", C.c_uint32), ("persistent", C.c_uint32), ("valid", C.c_uint32),
+                ("root_grew", C.c_uint32)]
 
 
 def gi_history(p: GiParams, radiance_blend: float, transmittance_blend: float, history_frames: int,
                persistent: int, valid: bool, intervals: np.ndarray, samples: np.ndarray, history: np.ndarray,
-               dep: np.ndarray, boxes=(), count: int | None = None):
+               dep: np.ndarray, boxes=(), count: int | None = None, root_grew: bool = False):
     """One temporal history pass over intervals [0, count): returns the
     blended intervals and the new history (n, 16: J.rgb, radiance mass, beta,
     transmittance mass, last frame, 0, then the retained dependency lo.xyz,
     known, hi.xyz, has bounds); dep (n, 8) = this trace's dependency
-    (gi_trace(..., dep=True)); boxes = dirty (lo, hi) pairs."""
+    (gi_trace(..., dep=True)); boxes = dirty (lo, hi) pairs; root_grew: the
+    scene's root box grew since the history was written (measured-empty
+    dependencies are then not reused)."""
     lib = _tsig()
-    h = HistoryParams(radiance_blend, transmittance_blend, history_frames, persistent, int(bool(valid)))
+    h = HistoryParams(radiance_blend, transmittance_blend, history_frames, persistent, int(bool(valid)),
+                      int(bool(root_grew)))
     iv = np.array(intervals, np.float32, copy=True, order="C")
     hs = np.array(history, np.float32, copy=True, order="C")
     ns = np.ascontiguousarray(samples, np.uint32)

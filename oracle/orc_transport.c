@@ -551,6 +551,7 @@ void orc_gi_history(const orc_gi_params* p, const orc_history_params* h, const f
         /* dependencyIntersectsDirty (split.slang:13-25) per dirty box: an
          * unknown dependency meets every region, a measured-empty one none */
         const orc_dep old = dep_load(hs + 8);
+        if (reuse && h->root_grew && old.known && !old.has) reuse = 0;
         if (reuse && n_boxes) {
             if (!old.known) reuse = 0;
             for (uint32_t b = 0; b < n_boxes && reuse && old.has; ++b) {

@@ -143,11 +143,13 @@ def world_box(mesh_tris, fwd_row):
 
 
 def test_gpu_history_after_rigid_motion_matches_oracle(lux_ctx):
-    """Six emissive-banded meshes; after one update one instance moves: the
-    library marks its old and new boxes dirty and the next update's history
-    (with every interval's retained dependency) equals the oracle's history
-    pass over the same boxes; most intervals keep their history."""
-    sc, motion = scenes.dynamic(seed=3, meshes=6, instances=6, stacks=6, slices=8)
+    """Six meshes, every other one with an emissive band; after one update a
+    non-emissive instance moves: the library marks its old and new boxes
+    dirty and the next update's history (with every interval's retained
+    dependency) equals the oracle's history pass over the same boxes; most
+    intervals keep their history.  Moving an emissive instance drops the whole
+    history."""
+    sc, motion = scenes.dynamic(seed=3, meshes=6, instances=6, stacks=6, slices=8, emissive_every=2)
     p = O.gi_params((0.5, 0.5, 0.5), 1.0, (29, 11, 14), 3, base_phi=4, base_y=4, interval_mode=0,
                     interval_length=1.0, emissive_samples=8, frame=0, seed=0, terminal_tail=1)
     lux_ctx.upload_scene(sc)
@@ -171,7 +173,8 @@ def test_gpu_history_after_rigid_motion_matches_oracle(lux_ctx):
         rlo = np.min([b[0] for b in boxes], axis=0) + 1e-3
         rhi = np.max([b[1] for b in boxes], axis=0) - 1e-3
         step = np.float32([0.25, 0.125, 0.0])
-        k = next(i for i, (lo, hi) in enumerate(boxes) if np.all(lo + step >= rlo) and np.all(hi + step <= rhi))
+        k = next(i for i, (lo, hi) in enumerate(boxes)
+                 if sc.inst_mesh[i] % 2 == 1 and np.all(lo + step >= rlo) and np.all(hi + step <= rhi))
         m = int(sc.inst_mesh[k])
         old_box = boxes[k]
         fwd[k, 3] += step[0]
@@ -201,5 +204,15 @@ def test_gpu_history_after_rigid_motion_matches_oracle(lux_ctx):
         bits_equal(g["merged"], merged, "merged after the motion")
         bits_equal(g["probes"], O.gi_pack(p, merged), "probes after the motion")
         assert 0.5 * n < reused < n, (reused, n)
+        e = next(i for i in range(len(fwd)) if sc.inst_mesh[i] % 2 == 0)   # an emitting instance
+        fwd2, inv2 = fwd.copy(), inv.copy()
+        fwd2[e, 7] += np.float32(0.0625)
+        inv_m = np.linalg.inv(np.vstack([fwd2[e].reshape(3, 4).astype(np.float64), [0, 0, 0, 1]]))
+        inv2[e] = inv_m[:3].reshape(12).astype(np.float32)
+        lux_ctx.set_transforms(inv2, fwd2)
+        lux_ctx.build_bvh()
+        lux_ctx.gi_set_frame(2, 2)
+        lux_ctx.gi_update()
+        assert lux_ctx.gi_history_reused() == 0
     finally:
         lux_ctx.gi_history_off()

@@ -169,6 +169,8 @@ def test_dirty_box_drops_exactly_the_intervals_whose_dependency_it_meets():
     out, _ = O.gi_history(p, 0.9, 0.9, 30, 1, True, iv2, ns, h1, known_empty(n),
                           boxes=[((2.9, 0.0, 0.0), (2.9, 0.5, 0.5))])
     assert (out[:, 0] == 1.0).tolist() == [False, False, True, False, False, True, False, False]
+    out, _ = O.gi_history(p, 0.9, 0.9, 30, 1, True, iv2, ns, h1, known_empty(n), root_grew=True)
+    assert (out[:, 0] == 1.0).tolist() == [False] * 6 + [True, False]   # a grown root: empty is stale
     # a reused value keeps the old dependency joined with the new; a dropped
     # one takes the new alone
     new = np.zeros((n, 8), np.float32)
