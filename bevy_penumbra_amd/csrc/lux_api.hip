@@ -48,24 +48,7 @@ namespace {
 __device__ __forceinline__ float sel_min(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float sel_max(float a, float b) { return b > a ? b : a; }
 
-// Triangle boxes of one mesh (input order).
-__global__ void k_tri_boxes(const float* pos, uint32_t n, float4* lo, float4* hi) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* p = pos + 9ull * i;
-    float4 l, h;
-    l.x = sel_min(sel_min(p[0], p[3]), p[6]);
-    l.y = sel_min(sel_min(p[1], p[4]), p[7]);
-    l.z = sel_min(sel_min(p[2], p[5]), p[8]);
-    h.x = sel_max(sel_max(p[0], p[3]), p[6]);
-    h.y = sel_max(sel_max(p[1], p[4]), p[7]);
-    h.z = sel_max(sel_max(p[2], p[5]), p[8]);
-    l.w = h.w = 0.0f;
-    lo[i] = l;
-    hi[i] = h;
-}
-
-// Gather triangles into BLAS leaf order.
+// Gather one mesh's triangles into BLAS leaf order (SAH builds).
 __global__ void k_tri_gather(const float* pos, const uint32_t* mat, const uint32_t* perm, uint32_t n,
                              float4* tri, uint32_t* order, uint32_t tri_base) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -778,22 +761,42 @@ static int build_blas(lux_ctx* ctx, bool sah) {
     info.insert(info.end(), ctx->mesh_blas_base.begin(), ctx->mesh_blas_base.end());
     info.insert(info.end(), ctx->mesh_tri_base.begin(), ctx->mesh_tri_base.end());
     info.insert(info.end(), ctx->mesh_tri_count.begin(), ctx->mesh_tri_count.end());
+    // then the segmented build's bottom-up tiles
+    std::vector<uint32_t> tiles;
+    lux_segment_tiles(ctx->mesh_tri_count.data(), ctx->n_meshes, tiles);
+    const size_t tiles_at = info.size();
+    if (!sah) info.insert(info.end(), tiles.begin(), tiles.end());
     if (int rc = lux_ensure(ctx, ctx->mesh_info, 4ull * info.size() + 16)) return rc;
     LUX_HIP(ctx, hipStreamSynchronize(st));
     LUX_HIP(ctx, hipMemcpy(ctx->mesh_info.ptr, info.data(), 4ull * info.size(), hipMemcpyHostToDevice));
-    // per-mesh temporaries: boxes + perm (largest mesh)
+    int rc = LUX_OK;
+    if (!sah) {
+        // Morton-30: every mesh's tree in one segmented pass
+        rc = lux_blas_build_segmented(ctx, static_cast<const float*>(ctx->raw_pos.ptr),
+                                      static_cast<const uint32_t*>(ctx->raw_mat.ptr), nt, ctx->n_meshes,
+                                      static_cast<const uint32_t*>(ctx->mesh_info.ptr),
+                                      static_cast<const uint32_t*>(ctx->mesh_info.ptr) + tiles_at,
+                                      (uint32_t)tiles.size() - 1u, 4u,
+                                      static_cast<float4*>(ctx->blas.ptr), static_cast<float4*>(ctx->tris.ptr),
+                                      static_cast<uint32_t*>(ctx->tri_order.ptr));
+        if (rc == LUX_OK) {
+            const hipError_t e = hipStreamSynchronize(st);
+            if (e != hipSuccess) rc = lux_hip_check(ctx, e, "build_blas");
+        }
+        if (rc == LUX_OK) rc = range_check(ctx, true, false);
+        if (rc == LUX_OK) ctx->blas_built = true;
+        return rc;
+    }
+    // SAH: every mesh's tree on the host first, meshes spread over threads,
+    // then uploaded mesh by mesh with the largest mesh's permutation buffer
     uint32_t maxn = 1;
     for (uint32_t n : ctx->mesh_tri_count) maxn = n > maxn ? n : maxn;
     lux_buffer& tmp = ctx->tlas_tmp;
-    if (int rc = lux_ensure(ctx, tmp, 36ull * maxn + 64)) return rc;
-    float4* lo = static_cast<float4*>(tmp.ptr);
-    float4* hi = lo + maxn;
-    uint32_t* perm = reinterpret_cast<uint32_t*>(hi + maxn);
-    int rc = LUX_OK;
-    // SAH: every mesh's tree on the host first, meshes spread over threads
+    if (int rc = lux_ensure(ctx, tmp, 4ull * maxn + 64)) return rc;
+    uint32_t* perm = static_cast<uint32_t*>(tmp.ptr);
     std::vector<std::vector<lux_bvh_node>> sah_nodes;
     std::vector<std::vector<uint32_t>> sah_perm;
-    if (sah) {
+    {
         sah_nodes.resize(ctx->n_meshes);
         sah_perm.resize(ctx->n_meshes);
         const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -827,21 +830,15 @@ static int build_blas(lux_ctx* ctx, bool sah) {
         const float* pos = static_cast<const float*>(ctx->raw_pos.ptr) + 9ull * tb;
         const uint32_t* mat = static_cast<const uint32_t*>(ctx->raw_mat.ptr) + tb;
         float4* nodes = static_cast<float4*>(ctx->blas.ptr) + 2ull * bb;
-        if (sah) {
-            // host build from the uploaded positions; unused slots of the
-            // mesh's 2n - 1 region stay zero
-            const std::vector<lux_bvh_node>& hn = sah_nodes[m];
-            const std::vector<uint32_t>& hp = sah_perm[m];
-            const uint32_t region = n > 1 ? 2 * n - 1 : 1;
-            LUX_HIP(ctx, hipStreamSynchronize(st));
-            LUX_HIP(ctx, hipMemset(nodes, 0, 32ull * region));
-            LUX_HIP(ctx, hipMemcpy(nodes, hn.data(), 32ull * hn.size(), hipMemcpyHostToDevice));
-            LUX_HIP(ctx, hipMemcpy(perm, hp.data(), 4ull * n, hipMemcpyHostToDevice));
-        } else {
-            hipLaunchKernelGGL(k_tri_boxes, dim3(nblk(n, 256)), dim3(256), 0, st, pos, n, lo, hi);
-            rc = lux_lbvh_build_device(ctx, lo, hi, n, 4u, nodes, perm, bb, tb);
-            if (rc) break;
-        }
+        // host build from the uploaded positions; unused slots of the
+        // mesh's 2n - 1 region stay zero
+        const std::vector<lux_bvh_node>& hn = sah_nodes[m];
+        const std::vector<uint32_t>& hp = sah_perm[m];
+        const uint32_t region = n > 1 ? 2 * n - 1 : 1;
+        LUX_HIP(ctx, hipStreamSynchronize(st));
+        LUX_HIP(ctx, hipMemset(nodes, 0, 32ull * region));
+        LUX_HIP(ctx, hipMemcpy(nodes, hn.data(), 32ull * hn.size(), hipMemcpyHostToDevice));
+        LUX_HIP(ctx, hipMemcpy(perm, hp.data(), 4ull * n, hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_tri_gather, dim3(nblk(n, 256)), dim3(256), 0, st, pos, mat, perm, n,
                            static_cast<float4*>(ctx->tris.ptr) + 3ull * tb,
                            static_cast<uint32_t*>(ctx->tri_order.ptr) + tb, tb);

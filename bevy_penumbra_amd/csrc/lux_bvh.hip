@@ -96,10 +96,8 @@ __device__ __forceinline__ uint32_t quantize10(float c, float lo, float ext) {
     return (uint32_t)q;
 }
 
-__global__ __launch_bounds__(256) void k_morton(const float4* lo, const float4* hi, uint32_t n,
-                                                const uint32_t* bounds, uint32_t* codes) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// Morton code of the box centre inside bounds[0..5] (ordered images).
+__device__ __forceinline__ uint32_t morton_code(float4 l, float4 h, const uint32_t* bounds) {
     float cmin[3], ext[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -108,12 +106,18 @@ __global__ __launch_bounds__(256) void k_morton(const float4* lo, const float4* 
         cmin[a] = mnv;
         ext[a] = mxv - mnv;
     }
-    float4 l = lo[i], h = hi[i];
     float c0 = l.x * 0.5f + h.x * 0.5f, c1 = l.y * 0.5f + h.y * 0.5f, c2 = l.z * 0.5f + h.z * 0.5f;
     uint32_t q0 = quantize10(c0, cmin[0], ext[0]);
     uint32_t q1 = quantize10(c1, cmin[1], ext[1]);
     uint32_t q2 = quantize10(c2, cmin[2], ext[2]);
-    codes[i] = (spread10(q0) << 2) | (spread10(q1) << 1) | spread10(q2);
+    return (spread10(q0) << 2) | (spread10(q1) << 1) | spread10(q2);
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float4* lo, const float4* hi, uint32_t n,
+                                                const uint32_t* bounds, uint32_t* codes) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    codes[i] = morton_code(lo[i], hi[i], bounds);
 }
 
 // ---------------- stable LSD radix sort, 8-bit digits ----------------
@@ -245,12 +249,13 @@ __device__ __forceinline__ int kdelta(const uint32_t* k, int64_t n, int64_t i, i
     return __clz(a ^ b);
 }
 
-// children[k] = gamma; range lo/hi; parents of leaves/internal nodes.
-__global__ __launch_bounds__(256) void k_karras(const uint32_t* sk, uint32_t n, uint32_t* gamma_out,
-                                                uint32_t* rlo, uint32_t* rhi, uint32_t* parent_leaf,
-                                                uint32_t* parent_int) {
-    uint32_t ii = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ii + 1 >= n) return;
+// Internal node ii of one primitive set's radix tree (keys sk[0..n), ties
+// broken by index): gamma, range lo/hi, and the parents of its children.
+// Every pointer is relative to the set, so one set of a segmented build
+// runs exactly the code of a single build.
+__device__ __forceinline__ void karras_node(const uint32_t* sk, uint32_t n, uint32_t ii, uint32_t* gamma_out,
+                                            uint32_t* rlo, uint32_t* rhi, uint32_t* parent_leaf,
+                                            uint32_t* parent_int) {
     int64_t i = ii, N = n;
     int d = kdelta(sk, N, i, i + 1) - kdelta(sk, N, i, i - 1) > 0 ? 1 : -1;
     int dmin = kdelta(sk, N, i, i - d);
@@ -278,6 +283,15 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* sk, uint32_t n, 
     else parent_int[g + 1] = ii;
 }
 
+// children[k] = gamma; range lo/hi; parents of leaves/internal nodes.
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* sk, uint32_t n, uint32_t* gamma_out,
+                                                uint32_t* rlo, uint32_t* rhi, uint32_t* parent_leaf,
+                                                uint32_t* parent_int) {
+    uint32_t ii = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii + 1 >= n) return;
+    karras_node(sk, n, ii, gamma_out, rlo, rhi, parent_leaf, parent_int);
+}
+
 __device__ __forceinline__ float4 ld_relaxed(const float4* p) {
     const uint32_t* u = reinterpret_cast<const uint32_t*>(p);
     float4 r;
@@ -294,43 +308,150 @@ __device__ __forceinline__ void st_relaxed(float4* p, float4 v) {
     __hip_atomic_store(u + 2, __float_as_uint(v.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Sorted primitive boxes, then walk up: the second arrival at a node unions
-// its children (left first) and continues.
-__global__ __launch_bounds__(256) void k_bottom_up(const float4* lo, const float4* hi,
-                                                   const uint32_t* perm, uint32_t n,
-                                                   const uint32_t* gamma, const uint32_t* rlo,
-                                                   const uint32_t* rhi, const uint32_t* parent_leaf,
-                                                   const uint32_t* parent_int, float4* plo,
-                                                   float4* phi, float4* ilo, float4* ihi,
-                                                   uint32_t* arrivals) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    uint32_t src = perm[j];
-    float4 l = lo[src], h = hi[src];
-    l.w = 0.0f;
-    h.w = 0.0f;
-    st_relaxed(&plo[j], l);
-    st_relaxed(&phi[j], h);
-    if (n < 2) return;
+// Bottom-up boxes: every sorted leaf walks up from its parent; the second
+// arrival at a node unions its children (left first) and continues.  The
+// sorted positions are cut into tiles of at most kBuTile leaves, one
+// workgroup each (two leaves per thread).  A node whose leaf range lies
+// inside its tile only ever sees arrivals from that workgroup: its counter,
+// records and boxes live in LDS and order at workgroup scope.  A node
+// reaching outside the tile (and every node above it) counts in global
+// memory with agent-scope acquire/release RMWs, its boxes stored and loaded
+// as agent-coherent atomics.  Every box is also stored to plo/phi/ilo/ihi
+// for node emission.  Seg: sets are segments of the sorted positions
+// (seg_of, tri_base, count; sets of at most leaf_max are skipped) and tile k
+// covers [tiles[k], tiles[k+1]) with no tile boundary other than at a
+// segment start or kBuTile multiples into one segment; otherwise one set of
+// n_all in tiles of kBuTile.  Indices inside a set are local, as in a single
+// build of that set.
+constexpr uint32_t kBuThreads = 1024;
+constexpr uint32_t kBuTile = 2 * kBuThreads;
+
+struct Box3 {
+    float x, y, z;
+};
+__device__ __forceinline__ Box3 box3(float4 v) { return Box3{v.x, v.y, v.z}; }
+__device__ __forceinline__ float4 box4(Box3 v) { return make_float4(v.x, v.y, v.z, 0.0f); }
+
+struct BuTile {
+    Box3 plo[kBuTile], phi[kBuTile], ilo[kBuTile], ihi[kBuTile];
+    uint32_t cnt[kBuTile], gamma[kBuTile], rlo[kBuTile], rhi[kBuTile], pint[kBuTile];
+};
+
+// The walk of sorted leaf j (tile slot j - start) of a set at tb of n.
+__device__ __forceinline__ void bu_walk(BuTile& T, uint32_t j, uint32_t start, uint32_t len, uint32_t tb,
+                                        const uint32_t* gamma, const uint32_t* rlo, const uint32_t* rhi,
+                                        const uint32_t* parent_leaf, const uint32_t* parent_int, float4* plo,
+                                        float4* phi, float4* ilo, float4* ihi, uint32_t* arrivals) {
+    // the set's arrays, indexed by local node / leaf index
+    const uint32_t* g_gamma = gamma + tb;
+    const uint32_t* g_rlo = rlo + tb;
+    const uint32_t* g_rhi = rhi + tb;
+    const uint32_t* g_pint = parent_int + tb;
+    float4* g_plo = plo + tb;
+    float4* g_phi = phi + tb;
+    float4* g_ilo = ilo + tb;
+    float4* g_ihi = ihi + tb;
+    // local index x of the set sits at tile slot tb + x - start
+    const int64_t off = (int64_t)tb - (int64_t)start, L = (int64_t)len;
     uint32_t p = parent_leaf[j];
+    bool in_tile = true;
     while (true) {
-        __atomic_thread_fence(__ATOMIC_RELEASE);  // publish this thread's boxes
-        uint32_t old = __hip_atomic_fetch_add(&arrivals[p], 1u, __ATOMIC_ACQ_REL,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0u) return;
-        __atomic_thread_fence(__ATOMIC_ACQUIRE);
-        uint32_t g = gamma[p];
+        // p lies in its own range, so a node inside the tile has its records
+        // in the tile
+        const int64_t sp = off + (int64_t)p;
+        const bool p_here = in_tile && sp >= 0 && sp < L;
+        const uint32_t r0 = p_here ? T.rlo[sp] : g_rlo[p];
+        const uint32_t r1 = p_here ? T.rhi[sp] : g_rhi[p];
+        const uint32_t g = p_here ? T.gamma[sp] : g_gamma[p];
+        in_tile = p_here && off + (int64_t)r0 >= 0 && off + (int64_t)r1 < L;
         float4 alo, ahi, blo, bhi;
-        if (g == rlo[p]) { alo = ld_relaxed(&plo[g]); ahi = ld_relaxed(&phi[g]); }
-        else { alo = ld_relaxed(&ilo[g]); ahi = ld_relaxed(&ihi[g]); }
-        if (g + 1 == rhi[p]) { blo = ld_relaxed(&plo[g + 1]); bhi = ld_relaxed(&phi[g + 1]); }
-        else { blo = ld_relaxed(&ilo[g + 1]); bhi = ld_relaxed(&ihi[g + 1]); }
+        if (in_tile) {
+            const uint32_t old = __hip_atomic_fetch_add(&T.cnt[sp], 1u, __ATOMIC_ACQ_REL,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0u) return;
+            if (g == r0) { alo = box4(T.plo[off + g]); ahi = box4(T.phi[off + g]); }
+            else { alo = box4(T.ilo[off + g]); ahi = box4(T.ihi[off + g]); }
+            if (g + 1 == r1) { blo = box4(T.plo[off + g + 1]); bhi = box4(T.phi[off + g + 1]); }
+            else { blo = box4(T.ilo[off + g + 1]); bhi = box4(T.ihi[off + g + 1]); }
+        } else {
+            const uint32_t old = __hip_atomic_fetch_add(&arrivals[tb + p], 1u, __ATOMIC_ACQ_REL,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            if (old == 0u) return;
+            if (g == r0) { alo = ld_relaxed(&g_plo[g]); ahi = ld_relaxed(&g_phi[g]); }
+            else { alo = ld_relaxed(&g_ilo[g]); ahi = ld_relaxed(&g_ihi[g]); }
+            if (g + 1 == r1) { blo = ld_relaxed(&g_plo[g + 1]); bhi = ld_relaxed(&g_phi[g + 1]); }
+            else { blo = ld_relaxed(&g_ilo[g + 1]); bhi = ld_relaxed(&g_ihi[g + 1]); }
+        }
         float4 olo = make_float4(sel_min(alo.x, blo.x), sel_min(alo.y, blo.y), sel_min(alo.z, blo.z), 0.0f);
         float4 ohi = make_float4(sel_max(ahi.x, bhi.x), sel_max(ahi.y, bhi.y), sel_max(ahi.z, bhi.z), 0.0f);
-        st_relaxed(&ilo[p], olo);
-        st_relaxed(&ihi[p], ohi);
+        if (in_tile) {
+            T.ilo[sp] = box3(olo);
+            T.ihi[sp] = box3(ohi);
+        }
+        st_relaxed(&g_ilo[p], olo);
+        st_relaxed(&g_ihi[p], ohi);
         if (p == 0) return;
-        p = parent_int[p];
+        p = p_here ? T.pint[sp] : g_pint[p];
+    }
+}
+
+template <bool Seg>
+__global__ __launch_bounds__(kBuThreads) void k_bottom_up_t(const float4* lo, const float4* hi, const uint32_t* perm,
+                                                            uint32_t n_all, const uint32_t* tiles,
+                                                            const uint32_t* seg_of, const uint32_t* tri_base,
+                                                            const uint32_t* count, uint32_t leaf_max,
+                                                            const uint32_t* gamma, const uint32_t* rlo,
+                                                            const uint32_t* rhi, const uint32_t* parent_leaf,
+                                                            const uint32_t* parent_int, float4* plo, float4* phi,
+                                                            float4* ilo, float4* ihi, uint32_t* arrivals) {
+    __shared__ BuTile T;
+    const uint32_t t = threadIdx.x;
+    uint32_t start, end;
+    if (Seg) {
+        start = tiles[blockIdx.x];
+        end = tiles[blockIdx.x + 1];
+    } else {
+        start = blockIdx.x * kBuTile;
+        end = n_all - start < kBuTile ? n_all : start + kBuTile;
+    }
+    const uint32_t len = end - start;
+    // the tile's counters, tree records and leaf boxes (slot x = position
+    // start + x; positions that are no internal node of a set hold records
+    // that no in-tile lookup reads)
+#pragma unroll
+    for (uint32_t r = 0; r < 2; ++r) {
+        const uint32_t x = t + r * kBuThreads;
+        T.cnt[x] = 0u;
+        if (x >= len) continue;
+        const uint32_t j = start + x;
+        T.gamma[x] = gamma[j];
+        T.rlo[x] = rlo[j];
+        T.rhi[x] = rhi[j];
+        T.pint[x] = parent_int[j];
+        const uint32_t src = perm[j];
+        float4 l = lo[src], h = hi[src];
+        l.w = 0.0f;
+        h.w = 0.0f;
+        T.plo[x] = box3(l);
+        T.phi[x] = box3(h);
+        st_relaxed(&plo[j], l);
+        st_relaxed(&phi[j], h);
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (uint32_t r = 0; r < 2; ++r) {
+        const uint32_t x = t + r * kBuThreads;
+        if (x >= len) break;
+        const uint32_t j = start + x;
+        uint32_t tb = 0, n = n_all;
+        if (Seg) {
+            const uint32_t m = seg_of[j];
+            tb = tri_base[m];
+            n = count[m];
+            if (n <= leaf_max) continue;
+        }
+        if (n < 2) continue;
+        bu_walk(T, j, start, len, tb, gamma, rlo, rhi, parent_leaf, parent_int, plo, phi, ilo, ihi, arrivals);
     }
 }
 
@@ -340,13 +461,12 @@ __device__ __forceinline__ void put_node(float4* nodes, uint32_t slot, float4 mn
     nodes[2u * slot + 1u] = make_float4(mx.x, mx.y, mx.z, __uint_as_float(count));
 }
 
-__global__ __launch_bounds__(256) void k_emit(uint32_t n, uint32_t leaf_max, const uint32_t* gamma,
-                                              const uint32_t* rlo, const uint32_t* rhi,
-                                              const float4* plo, const float4* phi,
-                                              const float4* ilo, const float4* ihi, float4* nodes,
-                                              uint32_t node_off, uint32_t leaf_off) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k + 1 >= n) return;
+// Internal node k of one set writes its two children into slots 1+2k,
+// 2+2k (node 0 also writes the root); pointers are relative to the set.
+__device__ __forceinline__ void emit_node(uint32_t n, uint32_t leaf_max, uint32_t k, const uint32_t* gamma,
+                                          const uint32_t* rlo, const uint32_t* rhi, const float4* plo,
+                                          const float4* phi, const float4* ilo, const float4* ihi,
+                                          float4* nodes, uint32_t node_off, uint32_t leaf_off) {
     if (k == 0) put_node(nodes, 0, ilo[0], ihi[0], 1u + node_off, 0u);
     uint32_t g = gamma[k];
     uint32_t cl[2] = {rlo[k], g + 1}, ch[2] = {g, rhi[k]};
@@ -368,9 +488,20 @@ __global__ __launch_bounds__(256) void k_emit(uint32_t n, uint32_t leaf_max, con
     }
 }
 
-__global__ void k_small_root(const float4* lo, const float4* hi, const uint32_t* perm, uint32_t n,
-                             float4* nodes, uint32_t leaf_off) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(256) void k_emit(uint32_t n, uint32_t leaf_max, const uint32_t* gamma,
+                                              const uint32_t* rlo, const uint32_t* rhi,
+                                              const float4* plo, const float4* phi,
+                                              const float4* ilo, const float4* ihi, float4* nodes,
+                                              uint32_t node_off, uint32_t leaf_off) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k + 1 >= n) return;
+    emit_node(n, leaf_max, k, gamma, rlo, rhi, plo, phi, ilo, ihi, nodes, node_off, leaf_off);
+}
+
+// A set of at most leaf_max primitives: the root is a leaf over all of
+// them, its box the union in sorted order.
+__device__ __forceinline__ void small_root(const float4* lo, const float4* hi, const uint32_t* perm, uint32_t n,
+                                           float4* nodes, uint32_t leaf_off) {
     float4 mn = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
     float4 mx = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
     for (uint32_t i = 0; i < n; ++i) {
@@ -379,6 +510,169 @@ __global__ void k_small_root(const float4* lo, const float4* hi, const uint32_t*
         mx = make_float4(sel_max(mx.x, h.x), sel_max(mx.y, h.y), sel_max(mx.z, h.z), 0.0f);
     }
     put_node(nodes, 0, mn, mx, leaf_off, n);
+}
+
+__global__ void k_small_root(const float4* lo, const float4* hi, const uint32_t* perm, uint32_t n,
+                             float4* nodes, uint32_t leaf_off) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    small_root(lo, hi, perm, n, nodes, leaf_off);
+}
+
+// ---------------- segmented build: every mesh's tree in one pass ----------------
+// Segment m holds primitives [tri_base[m], tri_base[m] + count[m]) (every
+// segment non-empty, so tri_base is strictly increasing); its nodes start at
+// slot blas_base[m].  Each segment's tree is the tree a single build of that
+// segment writes: Morton codes inside its own centre bounds, order by
+// (code, index within the segment), the same Karras/bottom-up/emit code on
+// pointers offset to the segment.
+
+// Largest m with tri_base[m] <= i.
+__device__ __forceinline__ uint32_t seg_find(const uint32_t* tri_base, uint32_t n_seg, uint32_t i) {
+    uint32_t a = 0, b = n_seg;
+    while (b - a > 1) {
+        const uint32_t mid = (a + b) >> 1;
+        if (tri_base[mid] <= i) a = mid;
+        else b = mid;
+    }
+    return a;
+}
+
+__global__ void k_seg_bounds_init(uint32_t* bounds, uint32_t n_seg) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 6u * n_seg) bounds[i] = (i % 6u) < 3u ? 0xffffffffu : 0u;
+}
+
+// Triangle boxes (a.min(b).min(c) as compare-selects, as the per-mesh
+// build), segment ids, and per-segment centre bounds: a wave whose lanes all
+// lie in one segment reduces in registers and issues one atomic per word.
+__global__ __launch_bounds__(256) void k_seg_boxes(const float* pos, uint32_t n, const uint32_t* tri_base,
+                                                   uint32_t n_seg, float4* lo, float4* hi, uint32_t* seg_of,
+                                                   uint32_t* bounds) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t m = 0;
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+    if (valid) {
+        const float* p = pos + 9ull * i;
+        float4 l, h;
+        l.x = sel_min(sel_min(p[0], p[3]), p[6]);
+        l.y = sel_min(sel_min(p[1], p[4]), p[7]);
+        l.z = sel_min(sel_min(p[2], p[5]), p[8]);
+        h.x = sel_max(sel_max(p[0], p[3]), p[6]);
+        h.y = sel_max(sel_max(p[1], p[4]), p[7]);
+        h.z = sel_max(sel_max(p[2], p[5]), p[8]);
+        l.w = h.w = 0.0f;
+        lo[i] = l;
+        hi[i] = h;
+        m = seg_find(tri_base, n_seg, i);
+        seg_of[i] = m;
+        const float c[3] = {l.x * 0.5f + h.x * 0.5f, l.y * 0.5f + h.y * 0.5f, l.z * 0.5f + h.z * 0.5f};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (c[a] != c[a]) continue;  // NaN centres do not widen the bounds
+            mn[a] = f2ord(c[a]);
+            mx[a] = mn[a];
+        }
+    }
+    // lanes of one wave are consecutive triangles: valid lanes are a prefix
+    const uint32_t m0 = __shfl(m, 0, 64);
+    const uint64_t act = __ballot(valid);
+    if (act == 0ull) return;
+    const bool uniform = __ballot(valid && m == m0) == act;
+    if (uniform) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const uint32_t om = __shfl_xor(mn[a], off, 64), ox = __shfl_xor(mx[a], off, 64);
+                mn[a] = om < mn[a] ? om : mn[a];
+                mx[a] = ox > mx[a] ? ox : mx[a];
+            }
+        }
+        if ((threadIdx.x & 63u) == 0u) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (mn[a] != 0xffffffffu) atomicMin(&bounds[6u * m0 + a], mn[a]);
+                if (mx[a] != 0u) atomicMax(&bounds[6u * m0 + 3u + a], mx[a]);
+            }
+        }
+    } else if (valid) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (mn[a] != 0xffffffffu) atomicMin(&bounds[6u * m + a], mn[a]);
+            if (mx[a] != 0u) atomicMax(&bounds[6u * m + 3u + a], mx[a]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_seg_morton(const float4* lo, const float4* hi, uint32_t n,
+                                                    const uint32_t* seg_of, const uint32_t* bounds,
+                                                    uint32_t* codes) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    codes[i] = morton_code(lo[i], hi[i], bounds + 6u * seg_of[i]);
+}
+
+__global__ void k_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* dst) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) dst[j] = src[idx[j]];
+}
+
+__global__ __launch_bounds__(256) void k_seg_karras(const uint32_t* sk, uint32_t n_all, const uint32_t* seg_of,
+                                                    const uint32_t* tri_base, const uint32_t* count,
+                                                    uint32_t leaf_max, uint32_t* gamma, uint32_t* rlo,
+                                                    uint32_t* rhi, uint32_t* parent_leaf, uint32_t* parent_int) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_all) return;
+    const uint32_t m = seg_of[j], tb = tri_base[m], n = count[m], ii = j - tb;
+    if (n <= leaf_max || ii + 1 >= n) return;
+    karras_node(sk + tb, n, ii, gamma + tb, rlo + tb, rhi + tb, parent_leaf + tb, parent_int + tb);
+}
+
+// Segments above leaf_max emit as a single build; smaller ones write a leaf
+// root and zero the rest of their 2n - 1 slots.
+__global__ __launch_bounds__(256) void k_seg_emit(uint32_t n_all, uint32_t leaf_max, const uint32_t* seg_of,
+                                                  const uint32_t* blas_base, const uint32_t* tri_base,
+                                                  const uint32_t* count, const uint32_t* gamma,
+                                                  const uint32_t* rlo, const uint32_t* rhi, const float4* plo,
+                                                  const float4* phi, const float4* ilo, const float4* ihi,
+                                                  const float4* lo, const float4* hi, const uint32_t* perm,
+                                                  float4* nodes) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_all) return;
+    const uint32_t m = seg_of[j], tb = tri_base[m], n = count[m], bb = blas_base[m], k = j - tb;
+    float4* nm = nodes + 2ull * bb;
+    if (n <= leaf_max) {
+        if (k == 0) {
+            small_root(lo, hi, perm + tb, n, nm, tb);
+        } else {
+            const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            nm[2u * (2u * k - 1u)] = z;
+            nm[2u * (2u * k - 1u) + 1u] = z;
+            nm[2u * (2u * k)] = z;
+            nm[2u * (2u * k) + 1u] = z;
+        }
+        return;
+    }
+    if (k + 1 >= n) return;
+    emit_node(n, leaf_max, k, gamma + tb, rlo + tb, rhi + tb, plo + tb, phi + tb, ilo + tb, ihi + tb, nm, bb,
+              tb);
+}
+
+// Triangles in leaf order: w of the first vertex holds the index within
+// the mesh, of the second the material; order[j] is the global index.
+__global__ __launch_bounds__(256) void k_seg_gather(const float* pos, const uint32_t* mat, const uint32_t* perm,
+                                                    const uint32_t* seg_of, const uint32_t* tri_base, uint32_t n,
+                                                    float4* tri, uint32_t* order) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t s = perm[j];
+    const float* p = pos + 9ull * s;
+    float4* t = tri + 3ull * j;
+    t[0] = make_float4(p[0], p[1], p[2], __uint_as_float(s - tri_base[seg_of[j]]));
+    t[1] = make_float4(p[3], p[4], p[5], __uint_as_float(mat ? mat[s] : 0u));
+    t[2] = make_float4(p[6], p[7], p[8], 0.0f);
+    order[j] = s;
 }
 
 inline uint32_t blocks_for(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
@@ -412,6 +706,12 @@ size_t lux_sort_hist_words(uint32_t n) {
 int lux_sort_pairs_impl(lux_ctx* ctx, const uint32_t* keys_in, uint32_t n, uint32_t* keys_out,
                         uint32_t* vals_out, uint32_t* tmp_keys, uint32_t* tmp_vals,
                         uint32_t* hist, uint32_t bits) {
+    return lux_sort_pairs_vals_impl(ctx, keys_in, nullptr, n, keys_out, vals_out, tmp_keys, tmp_vals, hist, bits);
+}
+
+int lux_sort_pairs_vals_impl(lux_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t n,
+                             uint32_t* keys_out, uint32_t* vals_out, uint32_t* tmp_keys, uint32_t* tmp_vals,
+                             uint32_t* hist, uint32_t bits) {
     if (n == 0) return LUX_OK;
     hipStream_t st = ctx->stream;
     uint32_t nb = (n + kRsTile - 1) / kRsTile;
@@ -420,7 +720,7 @@ int lux_sort_pairs_impl(lux_ctx* ctx, const uint32_t* keys_in, uint32_t n, uint3
     const int passes = bits == 0 ? 1 : (int)((bits + 7u) / 8u);
     // ping-pong so that the last pass writes keys_out / vals_out
     const uint32_t* kin = keys_in;
-    const uint32_t* vin = nullptr;
+    const uint32_t* vin = vals_in;
     for (int pass = 0; pass < passes; ++pass) {
         uint32_t shift = 8u * (uint32_t)pass;
         const bool last_parity = ((passes - 1 - pass) & 1) == 0;
@@ -489,10 +789,97 @@ int lux_lbvh_build_device(lux_ctx* ctx, const float4* lo, const float4* hi, uint
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, arrivals, 0u, n);
     hipLaunchKernelGGL(k_karras, dim3(blocks_for(n, 256)), dim3(256), 0, st, sk, n, gamma, rlo, rhi,
                        parent_leaf, parent_int);
-    hipLaunchKernelGGL(k_bottom_up, dim3(blocks_for(n, 256)), dim3(256), 0, st, lo, hi, perm, n,
-                       gamma, rlo, rhi, parent_leaf, parent_int, plo, phi, ilo, ihi, arrivals);
+    hipLaunchKernelGGL(k_bottom_up_t<false>, dim3(blocks_for(n, kBuTile)), dim3(kBuThreads), 0, st, lo, hi, perm,
+                       n, nullptr, nullptr, nullptr, nullptr, leaf_max, gamma, rlo, rhi, parent_leaf, parent_int, plo,
+                       phi, ilo, ihi, arrivals);
     hipLaunchKernelGGL(k_emit, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, leaf_max, gamma, rlo,
                        rhi, plo, phi, ilo, ihi, nodes, node_first_offset, leaf_first_offset);
+    LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
+void lux_segment_tiles(const uint32_t* count, uint32_t n_seg, std::vector<uint32_t>& tiles) {
+    // whole segments pack into a tile while they fit; a larger segment
+    // starts a tile and is cut every kBuTile leaves, its last piece left
+    // open for the segments that follow
+    tiles.clear();
+    uint32_t pos = 0, fill = 0;
+    for (uint32_t m = 0; m < n_seg; ++m) {
+        const uint32_t c = count[m];
+        if (!tiles.empty() && fill + c <= kBuTile) {
+            fill += c;
+        } else {
+            uint32_t k = 0;
+            while (c - k * kBuTile > kBuTile) tiles.push_back(pos + kBuTile * k++);
+            tiles.push_back(pos + kBuTile * k);
+            fill = c - k * kBuTile;
+        }
+        pos += c;
+    }
+    tiles.push_back(pos);
+}
+
+int lux_blas_build_segmented(lux_ctx* ctx, const float* pos, const uint32_t* mat, uint32_t n, uint32_t n_seg,
+                             const uint32_t* seg_info, const uint32_t* tiles, uint32_t n_tiles, uint32_t leaf_max,
+                             float4* nodes, float4* tris, uint32_t* order) {
+    if (n == 0 || n_seg == 0) return LUX_OK;
+    hipStream_t st = ctx->stream;
+    const uint32_t* blas_base = seg_info;
+    const uint32_t* tri_base = seg_info + n_seg;
+    const uint32_t* count = seg_info + 2ull * n_seg;
+    const size_t hw = lux_sort_hist_words(n);
+    const size_t words = 6ull * n_seg + 16ull * n + hw + 4 + 4ull * 6 * n;
+    if (int rc = lux_ensure(ctx, ctx->scratch, words * 4 + 1024)) return rc;
+    uint32_t* w = static_cast<uint32_t*>(ctx->scratch.ptr);
+    uint32_t* bounds = w;
+    uint32_t* seg_of = bounds + 6ull * n_seg;
+    uint32_t* codes = seg_of + n;
+    uint32_t* sk = codes + n;
+    uint32_t* perm1 = sk + n;
+    uint32_t* key2 = perm1 + n;
+    uint32_t* perm = key2 + n;
+    uint32_t* tk = perm + n;
+    uint32_t* tv = tk + n;
+    uint32_t* gamma = tv + n;
+    uint32_t* rlo = gamma + n;
+    uint32_t* rhi = rlo + n;
+    uint32_t* parent_leaf = rhi + n;
+    uint32_t* parent_int = parent_leaf + n;
+    uint32_t* arrivals = parent_int + n;
+    uint32_t* hist = arrivals + n;
+    uintptr_t fa = reinterpret_cast<uintptr_t>(hist + hw);
+    fa = (fa + 15) & ~uintptr_t(15);
+    float4* lo = reinterpret_cast<float4*>(fa);
+    float4* hi = lo + n;
+    float4* plo = hi + n;
+    float4* phi = plo + n;
+    float4* ilo = phi + n;
+    float4* ihi = ilo + n;
+    const uint32_t nb = blocks_for(n, 256);
+
+    hipLaunchKernelGGL(k_seg_bounds_init, dim3(blocks_for(6u * n_seg, 256)), dim3(256), 0, st, bounds, n_seg);
+    hipLaunchKernelGGL(k_seg_boxes, dim3(nb), dim3(256), 0, st, pos, n, tri_base, n_seg, lo, hi, seg_of, bounds);
+    hipLaunchKernelGGL(k_seg_morton, dim3(nb), dim3(256), 0, st, lo, hi, n, seg_of, bounds, codes);
+    // stable by code (ties by global index), then stable by segment: within
+    // a segment the order is by (code, index within the segment)
+    if (n_seg == 1) {
+        if (int rc = lux_sort_pairs_impl(ctx, codes, n, sk, perm, tk, tv, hist, 32)) return rc;
+    } else {
+        if (int rc = lux_sort_pairs_impl(ctx, codes, n, sk, perm1, tk, tv, hist, 32)) return rc;
+        hipLaunchKernelGGL(k_gather_u32, dim3(nb), dim3(256), 0, st, seg_of, perm1, n, key2);
+        const uint32_t seg_bits = 32u - (uint32_t)__builtin_clz(n_seg - 1u);
+        if (int rc = lux_sort_pairs_vals_impl(ctx, key2, perm1, n, sk, perm, tk, tv, hist, seg_bits)) return rc;
+        hipLaunchKernelGGL(k_gather_u32, dim3(nb), dim3(256), 0, st, codes, perm, n, sk);
+    }
+    hipLaunchKernelGGL(k_fill_u32, dim3(nb), dim3(256), 0, st, arrivals, 0u, n);
+    hipLaunchKernelGGL(k_seg_karras, dim3(nb), dim3(256), 0, st, sk, n, seg_of, tri_base, count, leaf_max, gamma,
+                       rlo, rhi, parent_leaf, parent_int);
+    hipLaunchKernelGGL(k_bottom_up_t<true>, dim3(n_tiles), dim3(kBuThreads), 0, st, lo, hi, perm, n, tiles, seg_of,
+                       tri_base, count, leaf_max, gamma, rlo, rhi, parent_leaf, parent_int, plo, phi, ilo, ihi,
+                       arrivals);
+    hipLaunchKernelGGL(k_seg_emit, dim3(nb), dim3(256), 0, st, n, leaf_max, seg_of, blas_base, tri_base, count,
+                       gamma, rlo, rhi, plo, phi, ilo, ihi, lo, hi, perm, nodes);
+    hipLaunchKernelGGL(k_seg_gather, dim3(nb), dim3(256), 0, st, pos, mat, perm, seg_of, tri_base, n, tris, order);
     LUX_HIP(ctx, hipGetLastError());
     return LUX_OK;
 }

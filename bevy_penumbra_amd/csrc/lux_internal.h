@@ -147,7 +147,25 @@ int lux_lbvh_build_device(lux_ctx* ctx, const float4* lo, const float4* hi, uint
 int lux_sort_pairs_impl(lux_ctx* ctx, const uint32_t* keys_in, uint32_t n, uint32_t* keys_out,
                         uint32_t* vals_out, uint32_t* tmp_keys, uint32_t* tmp_vals,
                         uint32_t* hist, uint32_t bits);
+// The same sort carrying vals_in (nullptr: input positions) as values.
+int lux_sort_pairs_vals_impl(lux_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t n,
+                             uint32_t* keys_out, uint32_t* vals_out, uint32_t* tmp_keys, uint32_t* tmp_vals,
+                             uint32_t* hist, uint32_t bits);
 size_t lux_sort_hist_words(uint32_t n);
+// Every mesh's BLAS in one pass over all n triangles (device positions
+// float[9n], materials u32[n] or null).  seg_info (device) holds n_seg BLAS
+// node bases, then n_seg triangle bases, then n_seg triangle counts (each
+// count > 0).  Each mesh's nodes, and its triangles in leaf order (tris
+// float4[3n], order u32[n]), are those a single LBVH build of that mesh
+// writes with node_first_offset = its BLAS base and leaf_first_offset = its
+// triangle base.  tiles (device, n_tiles + 1 words) is the bottom-up tile
+// table lux_segment_tiles makes from the same counts.  Stream-ordered; no
+// host wait.
+int lux_blas_build_segmented(lux_ctx* ctx, const float* pos, const uint32_t* mat, uint32_t n, uint32_t n_seg,
+                             const uint32_t* seg_info, const uint32_t* tiles, uint32_t n_tiles, uint32_t leaf_max,
+                             float4* nodes, float4* tris, uint32_t* order);
+// Bottom-up tile starts over segments of the given counts, then the total.
+void lux_segment_tiles(const uint32_t* count, uint32_t n_seg, std::vector<uint32_t>& tiles);
 // Exclusive scan of n words in place on the context stream; sums holds
 // lux_scan_sums_words(n) words, *total (device) receives the sum.
 int lux_scan_exclusive_impl(lux_ctx* ctx, uint32_t* data, uint32_t n, uint32_t* sums, uint32_t* total);

@@ -144,6 +144,60 @@ def test_gpu_sah_blas_build_matches_oracle(lux_ctx, colonnade, fixture):
     assert O.tree_cost(O.build_scene(sc, sah=2).blas) < O.tree_cost(O.build_scene(sc).blas)
 
 
+def uneven_meshes(seed=11, count=300):
+    """Meshes of 1..9 triangles (leaf-sized and just above), a few large ones,
+    meshes whose triangles share one centre (all Morton codes tie), a point
+    mesh (zero centre extent) and signed zeros; one instance per mesh on a
+    grid.  Over 256 meshes, so the segment-id sort takes two digit passes."""
+    rng = np.random.default_rng(seed)
+    sizes = [1, 2, 3, 4, 5, 6, 7, 8, 9, 17, 64, 1000, 5000]
+    meshes, mats = [], []
+    for m in range(count):
+        n = sizes[m % len(sizes)]
+        tri = rng.normal(size=(n, 9)).astype(np.float32)
+        if m % 7 == 3:
+            tri[:] = tri[0]                                  # one shared centre
+        if m % 11 == 5:
+            tri[:] = 0.0                                     # a point
+            tri[::2, :3] = -0.0
+        if m % 13 == 6:
+            tri[:, 0] = -0.0                                 # signed zeros in a coordinate
+        meshes.append(tri)
+        mats.append(rng.integers(0, 2, n).astype(np.uint32))
+    eye = np.eye(4)
+    inv = np.zeros((count, 12), np.float32)
+    fwd = np.zeros((count, 12), np.float32)
+    for i in range(count):
+        t = np.array([i % 20, (i // 20) % 5, i // 100], float) * 6.0
+        f, v = eye.copy(), eye.copy()
+        f[:3, 3] = t
+        v[:3, 3] = -t
+        fwd[i] = f[:3, :4].reshape(12)
+        inv[i] = v[:3, :4].reshape(12)
+    materials = np.asarray([[0.5, 0.5, 0.5, 1.0, 0, 0, 0, 0], [0.8, 0.8, 0.8, 1.0, 2.0, 2.0, 2.0, 0]], np.float32)
+    return scenes.Scene(meshes=meshes, mesh_materials=mats, inst_inv=inv, inst_fwd=fwd,
+                        inst_mesh=np.arange(count, dtype=np.uint32), inst_flags=np.zeros(count, np.uint32),
+                        materials=materials, bounds=((-4.0, -4.0, -4.0), (118.0, 28.0, 16.0)))
+
+
+def test_gpu_segmented_blas_build_matches_oracle_per_mesh(lux_ctx):
+    """Every mesh's BLAS comes from one segmented pass; each must equal the
+    oracle's build of that mesh alone: node words at the mesh's BLAS base,
+    triangle order (global index = mesh base + index in the mesh), hits."""
+    sc = uneven_meshes()
+    lux_ctx.upload_scene(sc)
+    lux_ctx.build_bvh(lux.LUX_BVH_MORTON30 | lux.LUX_BVH_FORCE_BLAS)
+    built = O.build_scene(sc)
+    g = lux_ctx.download_bvh()
+    assert np.array_equal(g["blas"], built.blas)
+    assert np.array_equal(g["tlas"], built.tlas)
+    tri_base = np.concatenate([[0], np.cumsum([m.shape[0] for m in sc.meshes])[:-1]]).astype(np.uint32)
+    mesh_of = np.repeat(np.arange(len(sc.meshes)), [m.shape[0] for m in sc.meshes])
+    assert np.array_equal(g["tri_order"], tri_base[mesh_of] + built.tri[:, 3].view(np.uint32))
+    rays = scenes.microbench_rays(sc.bounds, n=200_000, seed=9)
+    assert np.array_equal(trace_gpu(lux_ctx, rays), built.trace(rays, threads=16))
+
+
 def test_gpu_trace_colonnade_1m_bit_exact(lux_ctx, colonnade):
     sc, built = colonnade
     lux_ctx.upload_scene(sc)
