@@ -507,7 +507,9 @@ extern "C" int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_wo
 
 namespace {
 // Binned SAH over one mesh's triangle boxes on the host: build_tree
-// (src/scene.rs:2398-2554) -- 12 bins on the centroid span per axis with an
+// (src/scene.rs:2398-2554) -- box and centroid folds with glam's Vec3::min /
+// max per component (a < b ? a : b: the later of equal values wins), 12 bins
+// on the centroid span per axis with an
 // f64 bin index, bin costs area(f64) x count, the cheapest split with both
 // sides non-empty (axis order x, y, z, first strictly cheaper wins), Hoare
 // partition by bin, leaves of at most 4 and depth 48, and for coincident
@@ -527,6 +529,8 @@ struct SahBuilder {
     std::atomic<bool>* failed = nullptr;
 
     float centre(uint32_t e, int a) const { return lo[4 * e + a] * 0.5f + hi[4 * e + a] * 0.5f; }
+    static float vmin(float a, float b) { return a < b ? a : b; }
+    static float vmax(float a, float b) { return a > b ? a : b; }
 
     static double area(const float* mn, const float* mx) {
         const double sx = (double)mx[0] - (double)mn[0], sy = (double)mx[1] - (double)mn[1];
@@ -546,8 +550,8 @@ struct SahBuilder {
         }
         void add(const Bin& o) {
             for (int a = 0; a < 3; ++a) {
-                mn[a] = std::fmin(mn[a], o.mn[a]);
-                mx[a] = std::fmax(mx[a], o.mx[a]);
+                mn[a] = vmin(mn[a], o.mn[a]);
+                mx[a] = vmax(mx[a], o.mx[a]);
             }
             count += o.count;
         }
@@ -564,8 +568,8 @@ struct SahBuilder {
         float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (uint32_t i = 0; i < count; ++i)
             for (int a = 0; a < 3; ++a) {
-                mn[a] = std::fmin(mn[a], lo[4 * el[i] + a]);
-                mx[a] = std::fmax(mx[a], hi[4 * el[i] + a]);
+                mn[a] = vmin(mn[a], lo[4 * el[i] + a]);
+                mx[a] = vmax(mx[a], hi[4 * el[i] + a]);
             }
         lux_bvh_node& nd = nodes[index];
         std::memcpy(nd.mn, mn, 12);
@@ -577,8 +581,8 @@ struct SahBuilder {
         for (uint32_t i = 0; i < count; ++i)
             for (int a = 0; a < 3; ++a) {
                 const float c = centre(el[i], a);
-                cmin[a] = std::fmin(cmin[a], c);
-                cmax[a] = std::fmax(cmax[a], c);
+                cmin[a] = vmin(cmin[a], c);
+                cmax[a] = vmax(cmax[a], c);
             }
         double best = INFINITY, best_span = 0.0;
         int best_axis = -1;
@@ -750,7 +754,7 @@ static int range_check(lux_ctx* ctx, bool blas, bool tlas) {
     return LUX_OK;
 }
 
-static int build_blas(lux_ctx* ctx, bool sah) {
+static int build_blas(lux_ctx* ctx, uint32_t builder) {
     hipStream_t st = ctx->stream;
     uint32_t nt = ctx->n_tri;
     if (int rc = lux_ensure(ctx, ctx->tris, 48ull * nt)) return rc;
@@ -765,12 +769,27 @@ static int build_blas(lux_ctx* ctx, bool sah) {
     std::vector<uint32_t> tiles;
     lux_segment_tiles(ctx->mesh_tri_count.data(), ctx->n_meshes, tiles);
     const size_t tiles_at = info.size();
-    if (!sah) info.insert(info.end(), tiles.begin(), tiles.end());
+    if (builder == 0) info.insert(info.end(), tiles.begin(), tiles.end());
     if (int rc = lux_ensure(ctx, ctx->mesh_info, 4ull * info.size() + 16)) return rc;
     LUX_HIP(ctx, hipStreamSynchronize(st));
     LUX_HIP(ctx, hipMemcpy(ctx->mesh_info.ptr, info.data(), 4ull * info.size(), hipMemcpyHostToDevice));
     int rc = LUX_OK;
-    if (!sah) {
+    if (builder == 1) {
+        // binned SAH on the GPU: every mesh's tree level by level
+        rc = lux_blas_build_sah(ctx, static_cast<const float*>(ctx->raw_pos.ptr),
+                                static_cast<const uint32_t*>(ctx->raw_mat.ptr), nt, ctx->n_meshes,
+                                static_cast<const uint32_t*>(ctx->mesh_info.ptr), ctx->n_blas,
+                                static_cast<float4*>(ctx->blas.ptr), static_cast<float4*>(ctx->tris.ptr),
+                                static_cast<uint32_t*>(ctx->tri_order.ptr));
+        if (rc == LUX_OK) {
+            const hipError_t e = hipStreamSynchronize(st);
+            if (e != hipSuccess) rc = lux_hip_check(ctx, e, "build_blas");
+        }
+        if (rc == LUX_OK) rc = range_check(ctx, true, false);
+        if (rc == LUX_OK) ctx->blas_built = true;
+        return rc;
+    }
+    if (builder == 0) {
         // Morton-30: every mesh's tree in one segmented pass
         rc = lux_blas_build_segmented(ctx, static_cast<const float*>(ctx->raw_pos.ptr),
                                       static_cast<const uint32_t*>(ctx->raw_mat.ptr), nt, ctx->n_meshes,
@@ -787,8 +806,12 @@ static int build_blas(lux_ctx* ctx, bool sah) {
         if (rc == LUX_OK) ctx->blas_built = true;
         return rc;
     }
-    // SAH: every mesh's tree on the host first, meshes spread over threads,
+    // SAH on the host: every mesh's tree first, meshes spread over threads,
     // then uploaded mesh by mesh with the largest mesh's permutation buffer
+    for (const std::vector<float>& hp : ctx->host_pos)
+        for (float v : hp)
+            if (!(std::fabs(v) <= 3.402823466e38f))
+                return lux_fail(ctx, LUX_ERR_INVALID, "lux_bvh_build: SAH needs finite triangle positions");
     uint32_t maxn = 1;
     for (uint32_t n : ctx->mesh_tri_count) maxn = n > maxn ? n : maxn;
     lux_buffer& tmp = ctx->tlas_tmp;
@@ -922,26 +945,28 @@ static int refit_tlas(lux_ctx* ctx) {
 
 extern "C" int lux_bvh_build(lux_ctx* ctx, uint32_t mode) {
     if (!ctx) return LUX_ERR_INVALID;
-    if ((mode & ~(LUX_BVH_FORCE_BLAS | LUX_BVH_REFIT | LUX_BVH_SAH)) != LUX_BVH_MORTON30)
+    if ((mode & ~(LUX_BVH_FORCE_BLAS | LUX_BVH_REFIT | LUX_BVH_SAH | LUX_BVH_SAH_HOST)) != LUX_BVH_MORTON30 ||
+        ((mode & LUX_BVH_SAH) && (mode & LUX_BVH_SAH_HOST)))
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_bvh_build: unknown mode");
+    const uint32_t builder = (mode & LUX_BVH_SAH) ? 1u : ((mode & LUX_BVH_SAH_HOST) ? 2u : 0u);
     // a change of BLAS builder rebuilds the BLASes
-    if (ctx->blas_built && ctx->blas_sah != ((mode & LUX_BVH_SAH) != 0)) ctx->blas_built = false;
+    if (ctx->blas_built && ctx->blas_builder != builder) ctx->blas_built = false;
     if (mode & LUX_BVH_FORCE_BLAS) ctx->blas_built = false;
     if (ctx->n_meshes == 0 && ctx->n_inst == 0)
         return lux_fail(ctx, LUX_ERR_STATE, "lux_bvh_build: no scene uploaded");
     if (ctx->prebuilt) return LUX_OK;  // installed built: nothing to build
     LUX_HIP(ctx, hipSetDevice(ctx->device));
     if (!ctx->blas_built) {
-        if ((mode & LUX_BVH_SAH) && ctx->host_pos.size() != ctx->n_meshes)
+        if (builder == 2 && ctx->host_pos.size() != ctx->n_meshes)
             return lux_fail(ctx, LUX_ERR_STATE, "lux_bvh_build: SAH needs the scene's host positions");
         int rc;
         try {
-            rc = build_blas(ctx, (mode & LUX_BVH_SAH) != 0);
+            rc = build_blas(ctx, builder);
         } catch (...) {  // host containers of the build; nothing may leave a C entry point
             rc = lux_fail(ctx, LUX_ERR_OOM, "lux_bvh_build: host allocation failed");
         }
         if (rc) return rc;
-        ctx->blas_sah = (mode & LUX_BVH_SAH) != 0;
+        ctx->blas_builder = builder;
         ctx->tlas_built = false;
         ctx->tlas_topology = false;
     }

@@ -884,6 +884,26 @@ int lux_blas_build_segmented(lux_ctx* ctx, const float* pos, const uint32_t* mat
     return LUX_OK;
 }
 
+int lux_seg_prepare(lux_ctx* ctx, const float* pos, uint32_t n, const uint32_t* tri_base, uint32_t n_seg,
+                    float4* lo, float4* hi, uint32_t* seg_of, uint32_t* bounds) {
+    if (n == 0 || n_seg == 0) return LUX_OK;
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(k_seg_bounds_init, dim3(blocks_for(6u * n_seg, 256)), dim3(256), 0, st, bounds, n_seg);
+    hipLaunchKernelGGL(k_seg_boxes, dim3(blocks_for(n, 256)), dim3(256), 0, st, pos, n, tri_base, n_seg, lo, hi,
+                       seg_of, bounds);
+    LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
+int lux_seg_gather(lux_ctx* ctx, const float* pos, const uint32_t* mat, const uint32_t* perm, const uint32_t* seg_of,
+                   const uint32_t* tri_base, uint32_t n, float4* tris, uint32_t* order) {
+    if (n == 0) return LUX_OK;
+    hipLaunchKernelGGL(k_seg_gather, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, pos, mat, perm, seg_of,
+                       tri_base, n, tris, order);
+    LUX_HIP(ctx, hipGetLastError());
+    return LUX_OK;
+}
+
 extern "C" int lux_morton_codes_device(lux_ctx* ctx, const float* lo4, const float* hi4, uint32_t n,
                                        uint32_t* codes) {
     if (!ctx) return LUX_ERR_INVALID;

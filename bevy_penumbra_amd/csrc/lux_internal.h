@@ -88,7 +88,7 @@ struct lux_ctx {
     uint64_t gi_stats_rays = 0;
     // temporal history of raw intervals: on when a blend is positive; sample
     // counts, two float4 per interval, state words (valid, dirty boxes)
-    bool blas_sah = false;   // the built BLASes came from the host SAH build
+    uint32_t blas_builder = 0;  // builder of the built BLASes: 0 Morton-30, 1 SAH on the GPU, 2 SAH on the host
     bool gi_hist_on = false;
     bool gi_hist_valid = false;  // a history pass ran since the last reset
     lux_gi_history_params gi_hist{};
@@ -164,6 +164,19 @@ size_t lux_sort_hist_words(uint32_t n);
 int lux_blas_build_segmented(lux_ctx* ctx, const float* pos, const uint32_t* mat, uint32_t n, uint32_t n_seg,
                              const uint32_t* seg_info, const uint32_t* tiles, uint32_t n_tiles, uint32_t leaf_max,
                              float4* nodes, float4* tris, uint32_t* order);
+// Triangle boxes (compare-select a.min(b).min(c)), segment ids and
+// per-segment centre bounds of n triangles (device positions float[9n]).
+int lux_seg_prepare(lux_ctx* ctx, const float* pos, uint32_t n, const uint32_t* tri_base, uint32_t n_seg,
+                    float4* lo, float4* hi, uint32_t* seg_of, uint32_t* bounds);
+// Triangles in leaf order from a permutation of global triangle indices:
+// tris float4[3n] (w: index in the mesh, material, 0), order u32[n].
+int lux_seg_gather(lux_ctx* ctx, const float* pos, const uint32_t* mat, const uint32_t* perm, const uint32_t* seg_of,
+                   const uint32_t* tri_base, uint32_t n, float4* tris, uint32_t* order);
+// Every mesh's binned-SAH BLAS on the GPU (the reference's build_tree;
+// nodes, triangle order and tris as the host builder writes them);
+// seg_info as lux_blas_build_segmented.  Positions must be finite.
+int lux_blas_build_sah(lux_ctx* ctx, const float* pos, const uint32_t* mat, uint32_t n, uint32_t n_seg,
+                       const uint32_t* seg_info, uint32_t n_blas, float4* nodes, float4* tris, uint32_t* order);
 // Bottom-up tile starts over segments of the given counts, then the total.
 void lux_segment_tiles(const uint32_t* count, uint32_t n_seg, std::vector<uint32_t>& tiles);
 // Exclusive scan of n words in place on the context stream; sums holds

@@ -27,6 +27,7 @@ LUX_BVH_MORTON30 = 0
 LUX_BVH_FORCE_BLAS = 0x100
 LUX_BVH_REFIT = 0x200
 LUX_BVH_SAH = 0x1
+LUX_BVH_SAH_HOST = 0x2
 
 
 class LuxError(RuntimeError):

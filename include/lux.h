@@ -157,14 +157,16 @@ int lux_scene_set_transforms(lux_ctx* ctx, const float* local_from_world,
  * src/scene.rs:1040-1058,2033-2047); otherwise build as usual. */
 #define LUX_BVH_REFIT 0x200u
 /* or-ed into the mode: build each BLAS with the reference's binned SAH
- * (build_tree, src/scene.rs:2398-2554) on the host from the uploaded
- * positions -- the reference's own tree, fewer node visits than Morton-30;
- * meshes build in parallel on up to 16 host threads and a large mesh's top
- * subtrees in parallel too (one 262k-triangle mesh in about 55 ms), with the
- * sequential node order -- instead of Morton-30 on the GPU;
- * the TLAS stays Morton-30 on the GPU.  A change of builder rebuilds the
- * BLASes. */
+ * (build_tree, src/scene.rs:2398-2554) on the GPU -- the reference's own
+ * tree node for node (fewer node visits than Morton-30), every mesh in one
+ * level-by-level pass -- instead of Morton-30; the TLAS stays Morton-30.
+ * Triangle positions must be finite (LUX_ERR_INVALID otherwise).  A change
+ * of builder rebuilds the BLASes. */
 #define LUX_BVH_SAH 0x1u
+/* or-ed into the mode instead of LUX_BVH_SAH: the same binned-SAH tree built
+ * on the host from the uploaded positions (meshes on up to 16 threads, a
+ * large mesh's top subtrees in parallel), for comparison with the GPU build. */
+#define LUX_BVH_SAH_HOST 0x2u
 /* Builds what is stale: the BLASes after lux_scene_upload_soa, the TLAS after
  * lux_scene_set_transforms (Morton-30 over instance boxes; no host wait). */
 int lux_bvh_build(lux_ctx* ctx, uint32_t mode);

@@ -12,7 +12,13 @@
  *    Pinned by: std-sort comparison of the order (tests), tree invariants
  *    (every primitive covered once, children contained, ranges contiguous).
  * 2. Binned SAH build_tree (src/scene.rs:2398-2554): 12 bins, f64 bin index,
- *    in-place swap partition, leaves <= 4, depth <= 48.  The median fallback
+ *    in-place swap partition, leaves <= 4, depth <= 48.  Box and centroid
+ *    folds and bin joins are glam 0.33.12's Vec3::min/max (the reference's
+ *    `min.min(a)`), restated as the per-component select a < b ? a : b
+ *    (a > b ? a : b): of equal values the later one is kept, which shows only
+ *    on signed zeros (positions are finite; non-finite ones are rejected by
+ *    the builders).  This tie rule is parity unpinned: no reference test or
+ *    fixture holds a signed-zero tie, and glam is not vendored.  The median fallback
  *    for coincident centroids uses Rust's select_nth_unstable_by (by
  *    f32::total_cmp), which for a slice of at most 10 elements is a stable
  *    insertion sort (insertion_sort_shift_left); here it is that insertion
@@ -216,6 +222,10 @@ uint32_t orc_lbvh_build(const float* lo, const float* hi, uint32_t n, uint32_t l
 /* ---------------- binned SAH (src/scene.rs:2398-2554) ---------------- */
 typedef struct { float mn[3], mx[3]; size_t count; } sah_bin;
 
+/* glam Vec3::min / max per component */
+static inline float vmin(float a, float b) { return a < b ? a : b; }
+static inline float vmax(float a, float b) { return a > b ? a : b; }
+
 static double bounds_area(const float* mn, const float* mx) {
     double sx = (double)mx[0] - (double)mn[0];
     double sy = (double)mx[1] - (double)mn[1];
@@ -227,7 +237,7 @@ static void bin_empty(sah_bin* b) {
     b->count = 0;
 }
 static void bin_join(sah_bin* o, const sah_bin* x) {
-    for (int a = 0; a < 3; ++a) { o->mn[a] = fminf(o->mn[a], x->mn[a]); o->mx[a] = fmaxf(o->mx[a], x->mx[a]); }
+    for (int a = 0; a < 3; ++a) { o->mn[a] = vmin(o->mn[a], x->mn[a]); o->mx[a] = vmax(o->mx[a], x->mx[a]); }
     o->count += x->count;
 }
 static double bin_cost(const sah_bin* b) { return b->count == 0 ? 0.0 : bounds_area(b->mn, b->mx) * (double)b->count; }
@@ -271,8 +281,8 @@ static void sah_rec(sah_ctx* c, uint32_t index, uint32_t first, uint32_t count, 
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = 0; i < count; ++i)
         for (int a = 0; a < 3; ++a) {
-            mn[a] = fminf(mn[a], c->lo[4 * el[i] + a]);
-            mx[a] = fmaxf(mx[a], c->hi[4 * el[i] + a]);
+            mn[a] = vmin(mn[a], c->lo[4 * el[i] + a]);
+            mx[a] = vmax(mx[a], c->hi[4 * el[i] + a]);
         }
     orc_node* nd = &c->nodes[index];
     memcpy(nd->mn, mn, 12);
@@ -285,8 +295,8 @@ static void sah_rec(sah_ctx* c, uint32_t index, uint32_t first, uint32_t count, 
     for (uint32_t i = 0; i < count; ++i)
         for (int a = 0; a < 3; ++a) {
             float ce = centre(c, el[i], a);
-            cmin[a] = fminf(cmin[a], ce);
-            cmax[a] = fmaxf(cmax[a], ce);
+            cmin[a] = vmin(cmin[a], ce);
+            cmax[a] = vmax(cmax[a], ce);
         }
     double best = INFINITY;
     int split_axis = -1, split_bin = 0;

@@ -124,13 +124,18 @@ def test_gpu_traverses_sah_built_scene(lux_ctx, colonnade):
     assert np.array_equal(trace_gpu(lux_ctx, rays), sah.trace(rays, threads=16))
 
 
-def test_gpu_sah_blas_build_matches_oracle(lux_ctx, colonnade, fixture):
-    """LUX_BVH_SAH: the reference's binned-SAH BLASes built on the host,
-    Morton-30 TLAS; node arrays, triangle order and traversal equal the
-    oracle's SAH-BLAS scene; switching back rebuilds the LBVH BLASes."""
+SAH_BUILDERS = [pytest.param(lux.LUX_BVH_SAH, id="gpu"), pytest.param(lux.LUX_BVH_SAH_HOST, id="host")]
+
+
+@pytest.mark.parametrize("builder", SAH_BUILDERS)
+def test_gpu_sah_blas_build_matches_oracle(lux_ctx, colonnade, fixture, builder):
+    """LUX_BVH_SAH (on the GPU) and LUX_BVH_SAH_HOST: the reference's
+    binned-SAH BLASes, Morton-30 TLAS; node arrays, triangle order and
+    traversal equal the oracle's SAH-BLAS scene; switching back rebuilds the
+    LBVH BLASes."""
     for sc in (colonnade[0], fixture[0]):
         lux_ctx.upload_scene(sc)
-        lux_ctx.build_bvh(lux.LUX_BVH_SAH)
+        lux_ctx.build_bvh(builder)
         built = O.build_scene(sc, sah=2)
         assert_same_scene(lux_ctx, built)
         if len(sc.meshes) == 1:                           # one mesh: the order is the permutation itself
@@ -196,6 +201,67 @@ def test_gpu_segmented_blas_build_matches_oracle_per_mesh(lux_ctx):
     assert np.array_equal(g["tri_order"], tri_base[mesh_of] + built.tri[:, 3].view(np.uint32))
     rays = scenes.microbench_rays(sc.bounds, n=200_000, seed=9)
     assert np.array_equal(trace_gpu(lux_ctx, rays), built.trace(rays, threads=16))
+
+
+def signed_zero_meshes(seed=4):
+    """Meshes whose vertex coordinates are +0, -0 or positive at random, so
+    node box minima tie between the zeros at every level (the later
+    element's zero is kept), and a large mesh of one repeated triangle plus
+    a few others (median splits on the level-built part)."""
+    rng = np.random.default_rng(seed)
+    meshes, mats = [], []
+    for n in (3000, 200, 40, 7):
+        tri = rng.choice(np.array([0.0, -0.0, 0.5, 1.0, 2.0], np.float32), size=(n, 9)).astype(np.float32)
+        tri += (rng.random((n, 9)) < 0.3) * rng.random((n, 9)).astype(np.float32)
+        meshes.append(tri.astype(np.float32))
+    same = np.tile(np.array([[0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, -0.0]], np.float32), (700, 1))
+    same[::3, 8] = 0.0
+    same[5] = [2.0, 2.0, 2.0, 3.0, 2.0, 2.0, 2.0, 3.0, 2.0]
+    meshes.append(same)
+    for m in meshes:
+        mats.append(np.zeros(m.shape[0], np.uint32))
+    count = len(meshes)
+    eye = np.eye(4)[:3, :4].reshape(12).astype(np.float32)
+    return scenes.Scene(meshes=meshes, mesh_materials=mats, inst_inv=np.tile(eye, (count, 1)),
+                        inst_fwd=np.tile(eye, (count, 1)), inst_mesh=np.arange(count, dtype=np.uint32),
+                        inst_flags=np.zeros(count, np.uint32),
+                        materials=np.asarray([[0.5, 0.5, 0.5, 1.0, 0, 0, 0, 0]], np.float32),
+                        bounds=((-1.0, -1.0, -1.0), (4.0, 4.0, 4.0)))
+
+
+@pytest.mark.parametrize("builder", SAH_BUILDERS)
+def test_gpu_sah_uneven_meshes_match_oracle(lux_ctx, builder):
+    """Binned SAH over meshes of 1..5000 triangles (the GPU build's
+    sequential small subtrees and level-built large nodes), coincident
+    centroids (medians, a 5000-triangle one among them), points, signed-zero
+    ties in node boxes: nodes, triangle order and hits equal the oracle's."""
+    for sc in (uneven_meshes(), signed_zero_meshes()):
+        lux_ctx.upload_scene(sc)
+        lux_ctx.build_bvh(builder)
+        built = O.build_scene(sc, sah=2)
+        g = lux_ctx.download_bvh()
+        bad = np.nonzero(np.any(g["blas"] != built.blas, axis=1))[0]
+        assert bad.size == 0, (bad[:5], g["blas"][bad[:3]], built.blas[bad[:3]])
+        assert np.array_equal(g["tlas"], built.tlas)
+        tri_base = np.concatenate([[0], np.cumsum([m.shape[0] for m in sc.meshes])[:-1]]).astype(np.uint32)
+        mesh_of = np.repeat(np.arange(len(sc.meshes)), [m.shape[0] for m in sc.meshes])
+        assert np.array_equal(g["tri_order"], tri_base[mesh_of] + built.tri[:, 3].view(np.uint32))
+        rays = scenes.microbench_rays(sc.bounds, n=100_000, seed=3)
+        assert np.array_equal(trace_gpu(lux_ctx, rays), built.trace(rays, threads=16))
+
+
+@pytest.mark.parametrize("builder", SAH_BUILDERS)
+def test_gpu_sah_rejects_non_finite_positions(lux_ctx, builder):
+    """The SAH builds need finite positions (their tie rules differ from the
+    reference's on NaN); Morton-30 still builds the scene."""
+    sc = uneven_meshes(count=20)
+    sc.meshes[12] = sc.meshes[12].copy()
+    sc.meshes[12][3, 4] = np.nan
+    lux_ctx.upload_scene(sc)
+    with pytest.raises(lux.LuxError):
+        lux_ctx.build_bvh(builder)
+    lux_ctx.build_bvh()
+    assert np.array_equal(lux_ctx.download_bvh()["blas"], O.build_scene(sc).blas)
 
 
 def test_gpu_trace_colonnade_1m_bit_exact(lux_ctx, colonnade):
@@ -333,9 +399,10 @@ def test_gpu_traversal_work_equals_oracle(lux_ctx, colonnade, fixture, case):
     assert got == [st["node_fetches"], st["tri_tests"], st["inst_visits"]], (got, st)
 
 
+@pytest.mark.parametrize("builder", SAH_BUILDERS)
 @pytest.mark.parametrize("group", [7, 10])
-def test_gpu_sah_median_keeps_partition_order(lux_ctx, group):
-    """The library's host SAH build takes the median of coincident centroids
+def test_gpu_sah_median_keeps_partition_order(lux_ctx, group, builder):
+    """Both SAH builds take the median of coincident centroids
     in the order the parent's partition left (the reference's insertion sort
     for slices of up to 10): its triangle order and nodes equal the oracle's
     and a plain-loop model of build_tree."""
@@ -347,7 +414,7 @@ def test_gpu_sah_median_keeps_partition_order(lux_ctx, group):
     mats = np.asarray([[0.5, 0.5, 0.5, 1.0, 0, 0, 0, 0]], np.float32)
     sc = scenes.single_instance(tris, np.zeros(tris.shape[0], np.uint32), mats, ((0, 0, 0), (16, 4, 4)))
     lux_ctx.upload_scene(sc)
-    lux_ctx.build_bvh(lux.LUX_BVH_SAH)
+    lux_ctx.build_bvh(builder)
     built = O.build_scene(sc, sah=2)
     assert_same_scene(lux_ctx, built)
     want_perm, _ = _model_build_tree(lo, hi)

@@ -291,15 +291,27 @@ def _model_build_tree(lo, hi, ties_by_index=False):
         sx, sy, sz = (float(mx[a]) - float(mn[a]) for a in range(3))
         return 2.0 * (sx * sy + sy * sz + sz * sx)
 
+    def vmin(vals):  # glam Vec3::min per component: a < b ? a : b, folded in order
+        acc = math.inf
+        for v in vals:
+            acc = acc if acc < v else v
+        return acc
+
+    def vmax(vals):
+        acc = -math.inf
+        for v in vals:
+            acc = acc if acc > v else v
+        return acc
+
     def build(index, first, count, depth):
         el = perm[first:first + count]
-        mn = [min(float(lo[e, a]) for e in el) for a in range(3)]
-        mx = [max(float(hi[e, a]) for e in el) for a in range(3)]
+        mn = [vmin(float(lo[e, a]) for e in el) for a in range(3)]
+        mx = [vmax(float(hi[e, a]) for e in el) for a in range(3)]
         nodes[index] = (first, count, mn, mx)
         if count <= 4 or depth == 48:
             return
-        cmin = [min(float(centre(e, a)) for e in el) for a in range(3)]
-        cmax = [max(float(centre(e, a)) for e in el) for a in range(3)]
+        cmin = [vmin(float(centre(e, a)) for e in el) for a in range(3)]
+        cmax = [vmax(float(centre(e, a)) for e in el) for a in range(3)]
         best, split = math.inf, None
         for axis in range(3):
             span = cmax[axis] - cmin[axis]
@@ -401,3 +413,25 @@ def test_sah_median_keeps_partition_order_of_coincident_centroids(group):
         assert np.array_equal(f[k, :3], np.float32(mn)) and np.array_equal(f[k, 4:7], np.float32(mx)), k
     if group >= 6:  # a median over >= 6 boxes: the wrong order moves a box to the other side
         assert any(a[2:] != b[2:] for a, b in zip(want_nodes, wrong_nodes)), "node boxes must tell the orders apart"
+
+
+def test_sah_box_keeps_the_later_of_equal_zeros():
+    """Node boxes fold glam's Vec3::min/max (a < b ? a : b): of +0 and -0 the
+    later element's zero is kept -- in the oracle and in the plain-loop model;
+    a fold that keeps the first one (fminf on some libms, Python's min) fails."""
+    lo = np.zeros((2, 4), np.float32)
+    hi = np.ones((2, 4), np.float32)
+    hi[:, 3] = lo[:, 3] = 0
+    for first, second in ((0.0, -0.0), (-0.0, 0.0)):
+        lo[0, 0], lo[1, 0] = first, second
+        hi[0, 1], hi[1, 1] = first, second                   # max over y: both zeros
+        lo[:, 1] = -1.0
+        perm, nodes = O.sah(lo, hi)
+        assert perm.tolist() == [0, 1]
+        mn_x = nodes[0, 0:1].view(np.float32)[0]
+        mx_y = nodes[0, 5:6].view(np.float32)[0]
+        assert mn_x == 0.0 and np.signbit(mn_x) == np.signbit(np.float32(second))
+        assert mx_y == 0.0 and np.signbit(mx_y) == np.signbit(np.float32(second))
+        _, mnodes = _model_build_tree(lo, hi)
+        assert np.signbit(np.float32(mnodes[0][2][0])) == np.signbit(np.float32(second))
+        assert np.signbit(np.float32(mnodes[0][3][1])) == np.signbit(np.float32(second))
