@@ -574,11 +574,19 @@ __global__ __launch_bounds__(256) void k_seg_boxes(const float* pos, uint32_t n,
             mx[a] = mn[a];
         }
     }
-    // lanes of one wave are consecutive triangles: valid lanes are a prefix
+    // a block whose valid triangles all lie in one segment reduces through
+    // LDS and issues one atomic per word; otherwise each wave of one segment
+    // reduces in registers, and lanes of mixed waves go one by one
+    __shared__ uint32_t s_m0;
+    __shared__ uint32_t s_b[6];
+    if (threadIdx.x == 0) s_m0 = m;  // the block's first triangle is valid
+    if (threadIdx.x < 6) s_b[threadIdx.x] = threadIdx.x < 3 ? 0xffffffffu : 0u;
+    __syncthreads();
+    const uint32_t mb = s_m0;
+    const bool block_uniform = !__syncthreads_or(valid && m != mb);
     const uint32_t m0 = __shfl(m, 0, 64);
     const uint64_t act = __ballot(valid);
-    if (act == 0ull) return;
-    const bool uniform = __ballot(valid && m == m0) == act;
+    const bool uniform = act != 0ull && __ballot(valid && m == m0) == act;
     if (uniform) {
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
@@ -592,8 +600,13 @@ __global__ __launch_bounds__(256) void k_seg_boxes(const float* pos, uint32_t n,
         if ((threadIdx.x & 63u) == 0u) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                if (mn[a] != 0xffffffffu) atomicMin(&bounds[6u * m0 + a], mn[a]);
-                if (mx[a] != 0u) atomicMax(&bounds[6u * m0 + 3u + a], mx[a]);
+                if (block_uniform) {
+                    atomicMin(&s_b[a], mn[a]);
+                    atomicMax(&s_b[3 + a], mx[a]);
+                } else {
+                    if (mn[a] != 0xffffffffu) atomicMin(&bounds[6u * m0 + a], mn[a]);
+                    if (mx[a] != 0u) atomicMax(&bounds[6u * m0 + 3u + a], mx[a]);
+                }
             }
         }
     } else if (valid) {
@@ -601,6 +614,16 @@ __global__ __launch_bounds__(256) void k_seg_boxes(const float* pos, uint32_t n,
         for (int a = 0; a < 3; ++a) {
             if (mn[a] != 0xffffffffu) atomicMin(&bounds[6u * m + a], mn[a]);
             if (mx[a] != 0u) atomicMax(&bounds[6u * m + 3u + a], mx[a]);
+        }
+    }
+    if (!block_uniform) return;
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const uint32_t v = s_b[threadIdx.x];
+        if (threadIdx.x < 3) {
+            if (v != 0xffffffffu) atomicMin(&bounds[6u * mb + threadIdx.x], v);
+        } else if (v != 0u) {
+            atomicMax(&bounds[6u * mb + threadIdx.x], v);
         }
     }
 }

@@ -97,6 +97,8 @@ __device__ __forceinline__ uint32_t bin_of(float c, float cmin, double span) {
 
 inline uint32_t nblocks(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
+__device__ __forceinline__ uint64_t lanes_below(uint32_t k) { return k >= 64 ? ~0ull : ((1ull << k) - 1ull); }
+
 __global__ void k_sah_finite(const float* pos, uint64_t words, uint32_t* bad) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < words && !(fabsf(pos[i]) <= 3.402823466e38f)) *bad = 1u;
@@ -143,84 +145,73 @@ __global__ void k_sah_level_init(const SahRec* lv, uint32_t L, unsigned long lon
     }
 }
 
+// Runs of one node in a block of 256 positions: a node's positions are
+// contiguous and a large node holds more than kSmall of them, so a block
+// meets at most kRunSlots nodes.  slot: the run of this position (kNone
+// for positions outside large nodes); run_v[k]: the node of run k.
+constexpr uint32_t kRunSlots = 256 / (kSmall + 1) + 2;
+
+__device__ __forceinline__ uint32_t block_runs(uint32_t v, uint32_t* s_v, uint32_t* s_heads, uint32_t* run_v,
+                                               uint32_t* n_runs) {
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    s_v[t] = v;
+    __syncthreads();
+    const bool head = v != kNone && (t == 0 || s_v[t - 1] != v);
+    const uint64_t hm = __ballot(head);
+    if (lane == 0) s_heads[wave] = (uint32_t)__popcll(hm);
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t w = 0; w < wave; ++w) base += s_heads[w];
+    const uint32_t upto = base + (uint32_t)__popcll(hm & lanes_below(lane + 1));
+    // more runs than slots cannot happen (see above); such positions would
+    // take kNone here and be counted in *n_runs no further
+    const uint32_t slot = v != kNone && upto - 1 < kRunSlots ? upto - 1 : kNone;
+    if (head && slot != kNone) run_v[slot] = v;
+    if (t == 255) *n_runs = upto < kRunSlots ? upto : kRunSlots;
+    __syncthreads();
+    return slot;
+}
+
 // Per active position: the node box keys (value, position) and the
-// centroid bounds.  A block whose active positions all belong to one node
-// reduces through LDS first.
+// centroid bounds, gathered per node run in LDS, then one atomic per word
+// and run.
 __global__ __launch_bounds__(256) void k_sah_bounds(uint32_t n, const uint32_t* pos_node, const uint32_t* perm,
                                                     const float4* lo, const float4* hi,
                                                     unsigned long long* keys, uint32_t* cb) {
-    __shared__ unsigned long long s_key[4][6];
-    __shared__ uint32_t s_cb[4][6];
-    __shared__ uint32_t s_v0;
-    const uint32_t p = blockIdx.x * 256u + threadIdx.x, lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    __shared__ uint32_t s_v[256], s_heads[4], s_run_v[kRunSlots], s_nruns;
+    __shared__ unsigned long long s_key[kRunSlots][6];
+    __shared__ uint32_t s_cb[kRunSlots][6];
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
     const uint32_t v = p < n ? pos_node[p] : kNone;
-    unsigned long long k[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
-    uint32_t c[6] = {kOrdInf, kOrdInf, kOrdInf, kOrdNegInf, kOrdNegInf, kOrdNegInf};
-    if (v != kNone) {
+    if (threadIdx.x < 6 * kRunSlots) {
+        const uint32_t k = threadIdx.x / 6, a = threadIdx.x % 6;
+        s_key[k][a] = a < 3 ? ~0ull : 0ull;
+        s_cb[k][a] = a < 3 ? kOrdInf : kOrdNegInf;
+    }
+    const uint32_t slot = block_runs(v, s_v, s_heads, s_run_v, &s_nruns);
+    if (slot != kNone) {
         const uint32_t e = perm[p];
         const float4 l = lo[e], h = hi[e];
 #pragma unroll
         for (uint32_t a = 0; a < 3; ++a) {
-            k[a] = ((unsigned long long)f2ord_z(comp(l, a)) << 32) | (0xffffffffu - p);
-            k[3 + a] = ((unsigned long long)f2ord_z(comp(h, a)) << 32) | p;
-            c[a] = c[3 + a] = f2ord(centre(l, h, a));
+            atomicMin(&s_key[slot][a], ((unsigned long long)f2ord_z(comp(l, a)) << 32) | (0xffffffffu - p));
+            atomicMax(&s_key[slot][3 + a], ((unsigned long long)f2ord_z(comp(h, a)) << 32) | p);
+            const uint32_t c = f2ord(centre(l, h, a));
+            atomicMin(&s_cb[slot][a], c);
+            atomicMax(&s_cb[slot][3 + a], c);
         }
     }
-    if (threadIdx.x == 0) s_v0 = v;
     __syncthreads();
-    const uint32_t v0 = s_v0;
-    const bool uniform = v0 != kNone && !__syncthreads_or(v != v0);
-    if (uniform) {
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const unsigned long long kn = __shfl_xor(k[a], off, 64), kx = __shfl_xor(k[3 + a], off, 64);
-                k[a] = kn < k[a] ? kn : k[a];
-                k[3 + a] = kx > k[3 + a] ? kx : k[3 + a];
-                const uint32_t cn = __shfl_xor(c[a], off, 64), cx = __shfl_xor(c[3 + a], off, 64);
-                c[a] = cn < c[a] ? cn : c[a];
-                c[3 + a] = cx > c[3 + a] ? cx : c[3 + a];
-            }
+    const uint32_t runs = s_nruns;
+    if (threadIdx.x < 6 * runs) {
+        const uint32_t k = threadIdx.x / 6, a = threadIdx.x % 6, v0 = s_run_v[k];
+        if (a < 3) {
+            atomicMin(&keys[6ull * v0 + a], s_key[k][a]);
+            atomicMin(&cb[6ull * v0 + a], s_cb[k][a]);
+        } else {
+            atomicMax(&keys[6ull * v0 + a], s_key[k][a]);
+            atomicMax(&cb[6ull * v0 + a], s_cb[k][a]);
         }
-        if (lane == 0) {
-#pragma unroll
-            for (int a = 0; a < 6; ++a) {
-                s_key[wave][a] = k[a];
-                s_cb[wave][a] = c[a];
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x < 6) {
-            const uint32_t a = threadIdx.x;
-            unsigned long long kk = s_key[0][a];
-            uint32_t cc = s_cb[0][a];
-            for (int w = 1; w < 4; ++w) {
-                if (a < 3) {
-                    kk = s_key[w][a] < kk ? s_key[w][a] : kk;
-                    cc = s_cb[w][a] < cc ? s_cb[w][a] : cc;
-                } else {
-                    kk = s_key[w][a] > kk ? s_key[w][a] : kk;
-                    cc = s_cb[w][a] > cc ? s_cb[w][a] : cc;
-                }
-            }
-            if (a < 3) {
-                atomicMin(&keys[6ull * v0 + a], kk);
-                atomicMin(&cb[6ull * v0 + a], cc);
-            } else {
-                atomicMax(&keys[6ull * v0 + a], kk);
-                atomicMax(&cb[6ull * v0 + a], cc);
-            }
-        }
-        return;
-    }
-    if (v == kNone) return;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        atomicMin(&keys[6ull * v + a], k[a]);
-        atomicMax(&keys[6ull * v + 3 + a], k[3 + a]);
-        atomicMin(&cb[6ull * v + a], c[a]);
-        atomicMax(&cb[6ull * v + 3 + a], c[3 + a]);
     }
 }
 
@@ -244,27 +235,22 @@ __global__ void k_sah_node(SahRec* lv, uint32_t L, const unsigned long long* key
 }
 
 // Per active position of a large inner node: the element into its bin on
-// every axis with a positive span.  A block of one node gathers its bins in
-// LDS first.
+// every axis with a positive span, gathered per node run in LDS, then
+// flushed (non-empty bins only) to the node's bins.
 __global__ __launch_bounds__(256) void k_sah_bin(uint32_t n, const uint32_t* pos_node, const SahRec* lv,
                                                  const uint32_t* perm, const float4* lo, const float4* hi,
                                                  uint32_t* bins) {
-    __shared__ uint32_t s_bins[kNodeBinWords];
-    __shared__ uint32_t s_v0;
+    __shared__ uint32_t s_v[256], s_heads[4], s_run_v[kRunSlots], s_nruns;
+    __shared__ uint32_t s_bins[kRunSlots * kNodeBinWords];
     const uint32_t p = blockIdx.x * 256u + threadIdx.x;
     uint32_t v = p < n ? pos_node[p] : kNone;
     if (v != kNone && lv[v].kind != kLarge) v = kNone;
-    if (threadIdx.x == 0) s_v0 = v;
-    for (uint32_t w = threadIdx.x; w < kNodeBinWords; w += 256u) {
-        const uint32_t f = w % kBinWords;
+    for (uint32_t w = threadIdx.x; w < kRunSlots * kNodeBinWords; w += 256u) {
+        const uint32_t f = (w % kNodeBinWords) % kBinWords;
         s_bins[w] = f == 0 ? 0u : (f < 4 ? kOrdInf : kOrdNegInf);
     }
-    __syncthreads();
-    const uint32_t v0 = s_v0;
-    const bool uniform = v0 != kNone && !__syncthreads_or(v != v0);
-    if (v == kNone && !uniform) return;
-    uint32_t* dst = uniform ? s_bins : bins + (uint64_t)kNodeBinWords * v;
-    if (v != kNone) {
+    const uint32_t slot = block_runs(v, s_v, s_heads, s_run_v, &s_nruns);
+    if (slot != kNone) {
         const SahRec& r = lv[v];
         const uint32_t e = perm[p];
         const float4 l = lo[e], h = hi[e];
@@ -274,7 +260,7 @@ __global__ __launch_bounds__(256) void k_sah_bin(uint32_t n, const uint32_t* pos
             const double span = (double)r.cmax[axis] - (double)r.cmin[axis];
             if (!(span > 0.0)) continue;
             const uint32_t b = bin_of(centre(l, h, axis), r.cmin[axis], span);
-            uint32_t* bw = dst + (axis * kBins + b) * kBinWords;
+            uint32_t* bw = s_bins + slot * kNodeBinWords + (axis * kBins + b) * kBinWords;
             atomicAdd(&bw[0], 1u);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -283,18 +269,16 @@ __global__ __launch_bounds__(256) void k_sah_bin(uint32_t n, const uint32_t* pos
             }
         }
     }
-    if (!uniform) return;
     __syncthreads();
-    uint32_t* g = bins + (uint64_t)kNodeBinWords * v0;
-    for (uint32_t w = threadIdx.x; w < kNodeBinWords; w += 256u) {
-        const uint32_t f = w % kBinWords, val = s_bins[w];
-        if (f == 0) {
-            if (val) atomicAdd(&g[w], val);
-        } else if (f < 4) {
-            if (val != kOrdInf) atomicMin(&g[w], val);
-        } else if (val != kOrdNegInf) {
-            atomicMax(&g[w], val);
-        }
+    const uint32_t runs = s_nruns;
+    for (uint32_t w = threadIdx.x; w < runs * kNodeBinWords; w += 256u) {
+        const uint32_t k = w / kNodeBinWords, bw = w % kNodeBinWords, f = bw % kBinWords;
+        if (s_bins[k * kNodeBinWords + bw - f] == 0u) continue;  // an empty bin
+        uint32_t* g = bins + (uint64_t)kNodeBinWords * s_run_v[k] + bw;
+        const uint32_t val = s_bins[w];
+        if (f == 0) atomicAdd(g, val);
+        else if (f < 4) atomicMin(g, val);
+        else atomicMax(g, val);
     }
 }
 
@@ -518,126 +502,223 @@ __global__ void k_sah_posmap(uint32_t n, uint32_t* pos_node, const SahRec* lv, c
     pos_node[p] = recs[c].kind == kLarge ? c - next_begin : kNone;
 }
 
-// The sequential recursion over one subtree of at most kSmall elements,
-// one thread; nodes in local pre-order at out[0 ..), the element order in
-// perm.
-__global__ void k_sah_small(SahRec* recs, const uint32_t* small_list, uint32_t small_n, uint32_t* perm,
-                            const float4* lo, const float4* hi, SmallNode* snodes) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= small_n) return;
-    SahRec& root = recs[small_list[s]];
-    SmallNode* out = snodes + 2ull * root.first;
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+static_assert(kSmall <= 64, "a small subtree holds one element per lane");
+
+// The recursion over one subtree of at most kSmall elements, one wave: lane
+// j holds the element at offset j, a node is a run of lanes.  Each node
+// computes what the sequential recursion computes -- box by the select fold
+// (the wave minimum, its value taken from the last lane holding it), bins in
+// LDS, the sweep one lane per axis (the first strictly cheapest (axis, bin)
+// in order), the partition's closed form from ballots, the median as a
+// stable rank by key -- and nodes go out in local pre-order; the subtree's
+// element order returns to perm.
+__global__ __launch_bounds__(64) void k_sah_small(SahRec* recs, const uint32_t* small_list, uint32_t small_n,
+                                                  uint32_t* perm, const float4* lo, const float4* hi,
+                                                  SmallNode* snodes) {
     struct Task {
-        uint32_t index, first, count, depth;
+        uint32_t index, a, cnt, depth;
     };
-    Task stack[kSmall + 2];  // pending right subtrees along one path, then the current one
-    int top = 0;
-    uint32_t next = 1;
-    stack[top++] = Task{0u, root.first, root.count, root.depth};
+    __shared__ uint32_t s_e[64];
+    __shared__ float4 s_l[64], s_h[64];
+    __shared__ uint32_t s_bins[kNodeBinWords];
+    __shared__ uint32_t s_tl[64], s_tr[64];
+    __shared__ Task s_stack[kSmall + 2];
+    if (blockIdx.x >= small_n) return;
+    SahRec& root = recs[small_list[blockIdx.x]];
+    const uint32_t j = threadIdx.x, first = root.first, count = root.count;
+    SmallNode* out = snodes + 2ull * first;
+    uint32_t e = 0;
+    float4 l = make_float4(0.0f, 0.0f, 0.0f, 0.0f), h = l;
+    if (j < count) {
+        e = perm[first + j];
+        l = lo[e];
+        h = hi[e];
+    }
+    if (j == 0) s_stack[0] = Task{0u, 0u, count, root.depth};
+    uint32_t top = 1, next = 1;
+    __syncthreads();
     while (top > 0) {
-        const Task t = stack[--top];
-        uint32_t* el = perm + t.first;
-        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (uint32_t i = 0; i < t.count; ++i) {
-            const float4 l = lo[el[i]], h = hi[el[i]];
-            for (uint32_t a = 0; a < 3; ++a) {
-                mn[a] = gmin(mn[a], comp(l, a));
-                mx[a] = gmax(mx[a], comp(h, a));
+        --top;
+        const Task t = s_stack[top];
+        const bool act = j >= t.a && j < t.a + t.cnt;
+        const uint64_t node_mask = lanes_below(t.a + t.cnt) & ~lanes_below(t.a);
+        float box[6];
+#pragma unroll
+        for (uint32_t a = 0; a < 3; ++a) {
+            const uint32_t on = act ? f2ord_z(comp(l, a)) : 0xffffffffu;
+            const uint32_t mn = wave_min(on);
+            const uint64_t wn = __ballot(act && on == mn);
+            box[a] = __shfl(comp(l, a), 63 - __clzll(wn), 64);
+            const uint32_t ox = act ? f2ord_z(comp(h, a)) : 0u;
+            const uint32_t mx = wave_max(ox);
+            const uint64_t wx = __ballot(act && ox == mx);
+            box[3 + a] = __shfl(comp(h, a), 63 - __clzll(wx), 64);
+        }
+        const bool leaf = t.cnt <= kLeaf || t.depth == kMaxDepth;
+        if (j == 0) {
+            SmallNode& nd = out[t.index];
+            for (int a = 0; a < 6; ++a) nd.box[a] = box[a];
+            nd.first = leaf ? first + t.a : next;
+            nd.count = leaf ? t.cnt : 0u;
+        }
+        if (leaf) {
+            __syncthreads();
+            continue;
+        }
+        float c[3], cmin[3], cmax[3];
+#pragma unroll
+        for (uint32_t a = 0; a < 3; ++a) {
+            c[a] = centre(l, h, a);
+            cmin[a] = ord2f(wave_min(act ? f2ord(c[a]) : kOrdInf));
+            cmax[a] = ord2f(wave_max(act ? f2ord(c[a]) : kOrdNegInf));
+        }
+        for (uint32_t w = j; w < kNodeBinWords; w += 64) {
+            const uint32_t f = w % kBinWords;
+            s_bins[w] = f == 0 ? 0u : (f < 4 ? kOrdInf : kOrdNegInf);
+        }
+        __syncthreads();
+        if (act) {
+            const uint32_t ol[3] = {f2ord(l.x), f2ord(l.y), f2ord(l.z)};
+            const uint32_t oh[3] = {f2ord(h.x), f2ord(h.y), f2ord(h.z)};
+            for (uint32_t axis = 0; axis < 3; ++axis) {
+                const double span = (double)cmax[axis] - (double)cmin[axis];
+                if (!(span > 0.0)) continue;
+                uint32_t* bw = s_bins + (axis * kBins + bin_of(c[axis], cmin[axis], span)) * kBinWords;
+                atomicAdd(&bw[0], 1u);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    atomicMin(&bw[1 + q], ol[q]);
+                    atomicMax(&bw[4 + q], oh[q]);
+                }
             }
         }
-        SmallNode& nd = out[t.index];
-        for (int a = 0; a < 3; ++a) {
-            nd.box[a] = mn[a];
-            nd.box[3 + a] = mx[a];
-        }
-        nd.first = t.first;
-        nd.count = t.count;
-        if (t.count <= kLeaf || t.depth == kMaxDepth) continue;
-        float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (uint32_t i = 0; i < t.count; ++i) {
-            const float4 l = lo[el[i]], h = hi[el[i]];
-            for (uint32_t a = 0; a < 3; ++a) {
-                const float c = centre(l, h, a);
-                cmin[a] = gmin(cmin[a], c);
-                cmax[a] = gmax(cmax[a], c);
+        __syncthreads();
+        // lane `axis` sweeps its axis: the cheapest bin, the first of equal cost
+        double my_cost = INFINITY;
+        uint32_t my_bin = 0;
+        if (j < 3) {
+            const double span = (double)cmax[j] - (double)cmin[j];
+            if (span > 0.0) {
+                Bin b[kBins], right[kBins];
+                for (uint32_t k = 0; k < kBins; ++k) {
+                    const uint32_t* src = s_bins + (j * kBins + k) * kBinWords;
+                    b[k].count = src[0];
+                    for (int q = 0; q < 3; ++q) {
+                        b[k].mn[q] = ord2f(src[1 + q]);
+                        b[k].mx[q] = ord2f(src[4 + q]);
+                    }
+                }
+                Bin sum;
+                sum.clear();
+                for (int k = (int)kBins - 1; k >= 0; --k) {
+                    sum.join(b[k]);
+                    right[k] = sum;
+                }
+                Bin left;
+                left.clear();
+                for (uint32_t k = 0; k + 1 < kBins; ++k) {
+                    left.join(b[k]);
+                    const double cost = left.cost() + right[k + 1].cost();
+                    if (left.count > 0 && right[k + 1].count > 0 && cost < my_cost) {
+                        my_cost = cost;
+                        my_bin = k;
+                    }
+                }
             }
         }
-        double best = INFINITY, best_span = 0.0;
+        double best = INFINITY;
         int best_axis = -1;
         uint32_t best_bin = 0;
-        for (uint32_t axis = 0; axis < 3; ++axis) {
-            const double span = (double)cmax[axis] - (double)cmin[axis];
-            if (span <= 0.0) continue;
-            Bin b[kBins], right[kBins], sum, left;
-            for (uint32_t k = 0; k < kBins; ++k) b[k].clear();
-            for (uint32_t i = 0; i < t.count; ++i) {
-                const float4 l = lo[el[i]], h = hi[el[i]];
-                Bin one;
-                for (uint32_t a = 0; a < 3; ++a) {
-                    one.mn[a] = comp(l, a);
-                    one.mx[a] = comp(h, a);
-                }
-                one.count = 1;
-                b[bin_of(centre(l, h, axis), cmin[axis], span)].join(one);
-            }
-            sum.clear();
-            for (int k = (int)kBins - 1; k >= 0; --k) {
-                sum.join(b[k]);
-                right[k] = sum;
-            }
-            left.clear();
-            for (uint32_t k = 0; k + 1 < kBins; ++k) {
-                left.join(b[k]);
-                const double cost = left.cost() + right[k + 1].cost();
-                if (left.count > 0 && right[k + 1].count > 0 && cost < best) {
-                    best = cost;
-                    best_axis = (int)axis;
-                    best_bin = k;
-                    best_span = span;
-                }
+#pragma unroll
+        for (int axis = 0; axis < 3; ++axis) {
+            const double ca = __shfl(my_cost, axis, 64);
+            const uint32_t ba = __shfl(my_bin, axis, 64);
+            if (ca < best) {
+                best = ca;
+                best_axis = axis;
+                best_bin = ba;
             }
         }
-        uint32_t mid;
+        const uint32_t i = j - t.a;
+        uint32_t mid, dst = i;
         if (best_axis >= 0) {
-            uint32_t l = 0, r = t.count;
-            while (l < r) {
-                const uint32_t e = el[l];
-                if (bin_of(centre(lo[e], hi[e], (uint32_t)best_axis), cmin[best_axis], best_span) <= best_bin) {
-                    ++l;
-                } else {
-                    --r;
-                    el[l] = el[r];
-                    el[r] = e;
+            const double span = (double)cmax[best_axis] - (double)cmin[best_axis];
+            const bool left = act && bin_of(c[best_axis], cmin[best_axis], span) <= best_bin;
+            const uint64_t lm = __ballot(left), rm = node_mask & ~lm;
+            mid = (uint32_t)__popcll(lm);
+            const uint32_t front = mid == t.cnt ? mid : mid + (uint32_t)(((lm >> (t.a + mid)) & 1ull) ^ 1ull);
+            const uint64_t front_mask = node_mask & lanes_below(t.a + front);
+            uint32_t kr = 0, kl = 0;
+            if (act) {
+                if (i < front && !left) {
+                    kr = (uint32_t)__popcll(rm & front_mask & lanes_below(j + 1));
+                    s_tr[kr - 1] = i;
+                } else if (i >= front && left) {
+                    kl = (uint32_t)__popcll(lm & ~front_mask & ~lanes_below(j));
+                    s_tl[kl - 1] = i;
                 }
             }
-            mid = l;
+            __syncthreads();
+            if (act) {
+                if (i < front) dst = left ? i : (kr == 1 ? t.cnt - 1 : s_tl[kr - 2] - 1);
+                else dst = left ? s_tr[kl - 1] : i - 1;
+            }
         } else {
             const float sx = cmax[0] - cmin[0], sy = cmax[1] - cmin[1], sz = cmax[2] - cmin[2];
             const uint32_t axis = (sx >= sy && sx >= sz) ? 0u : (sy >= sz ? 1u : 2u);
-            // stable insertion sort by the total-order key of the centre
-            for (uint32_t i = 1; i < t.count; ++i) {
-                const uint32_t e = el[i];
-                const uint32_t ke = f2ord(centre(lo[e], hi[e], axis));
-                uint32_t j = i;
-                while (j > 0) {
-                    const uint32_t d = el[j - 1];
-                    if (f2ord(centre(lo[d], hi[d], axis)) <= ke) break;
-                    el[j] = d;
-                    --j;
+            const uint32_t key = f2ord(c[axis]);
+            s_tl[j] = key;
+            __syncthreads();
+            if (act) {
+                uint32_t r = 0;
+                for (uint32_t q = t.a; q < t.a + t.cnt; ++q) {
+                    const uint32_t kq = s_tl[q];
+                    r += (kq < key || (kq == key && q < j)) ? 1u : 0u;
                 }
-                el[j] = e;
+                dst = r;
             }
-            mid = t.count / 2;
+            mid = t.cnt / 2;
+        }
+        // move the node's elements to their places
+        if (act) {
+            s_e[t.a + dst] = e;
+            s_l[t.a + dst] = l;
+            s_h[t.a + dst] = h;
+        }
+        __syncthreads();
+        if (act) {
+            e = s_e[j];
+            l = s_l[j];
+            h = s_h[j];
         }
         const uint32_t children = next;
         next += 2;
-        nd.first = children;
-        nd.count = 0;
         // the left subtree runs first, as in the recursion
-        stack[top++] = Task{children + 1, t.first + mid, t.count - mid, t.depth + 1};
-        stack[top++] = Task{children, t.first, mid, t.depth + 1};
+        if (j == 0) {
+            s_stack[top] = Task{children + 1, t.a + mid, t.cnt - mid, t.depth + 1};
+            s_stack[top + 1] = Task{children, t.a, mid, t.depth + 1};
+        }
+        top += 2;
+        __syncthreads();
     }
-    root.desc = next - 1;
+    if (j < count) perm[first + j] = e;
+    if (j == 0) root.desc = next - 1;
 }
 
 __global__ void k_sah_desc(SahRec* recs, uint32_t lv_begin, uint32_t L) {
@@ -850,8 +931,8 @@ int lux_blas_build_sah(lux_ctx* ctx, const float* pos, const uint32_t* mat, uint
     const uint32_t n_recs = lv_begin.back() + lv_size.back();
     const uint32_t small_n = h_ctl[1];
     if (small_n)
-        hipLaunchKernelGGL(k_sah_small, dim3(nblocks(small_n, 64)), dim3(64), 0, st, recs, small_list, small_n, perm,
-                           lo, hi, snodes);
+        hipLaunchKernelGGL(k_sah_small, dim3(small_n), dim3(64), 0, st, recs, small_list, small_n, perm, lo, hi,
+                           snodes);
     for (int k = (int)lv; k >= 0; --k)
         hipLaunchKernelGGL(k_sah_desc, dim3(nblocks(lv_size[k], 128)), dim3(128), 0, st, recs, lv_begin[k],
                            lv_size[k]);
