@@ -58,12 +58,18 @@ def parse():
     p.add_argument("--no-receivers", action="store_true")
     p.add_argument("--no-screen", action="store_true")
     p.add_argument("--blas", choices=("sah", "morton"), default="sah",
-                   help="BLAS builder of the traversal scene: the reference's binned SAH on the host "
+                   help="BLAS builder of the traversal scene: the reference's binned SAH built on the GPU "
                         "(LUX_BVH_SAH) or Morton-30 on the GPU")
     p.add_argument("--receiver-steps", type=int, default=10)
     p.add_argument("--pmc-workload", action="store_true",
                    help="run only the counter-profiling workload (1 GiB device copy x2, three "
-                        "traversal launches, two GI updates; default launch configurations)")
+                        "traversal launches, two GI updates; default launch configurations unless "
+                        "--trace-config / --gi-config name others)")
+    p.add_argument("--trace-config", default="",
+                   help="traversal launch configuration of the counter workload: "
+                        "waves_per_cu,lds_stack_depth,regroup_threshold,steps_per_refill")
+    p.add_argument("--gi-config", default="",
+                   help="GI stage launch configuration of the counter workload, same four numbers")
     return p.parse_args()
 
 
@@ -71,23 +77,24 @@ TRAFFIC_PROFILE = ROOT / "profiles" / "pmc_traffic_colonnade_trace_and_gi.md"
 
 
 def measured_traffic(kernel: str, cfg: dict, gi: bool = False):
-    """HBM-side bytes per launch of `kernel` from the committed counter profile,
-    when it was taken with this launch configuration (the traversal's, or with
-    gi=True the GI stages') and BLAS builder (the profile's `config` /
-    `gi_config` line; `blas` absent means Morton-30); else None."""
+    """HBM-side bytes per launch of `kernel` from the committed counter profile:
+    its blocks each start with a `config` (traversal) and a `gi_config` (GI
+    stages) line giving the launch configuration and BLAS builder they were
+    measured with (`blas` absent means Morton-30); the kernel's `traffic`
+    line is taken from the first block whose configuration of that kind (gi)
+    equals cfg.  None when no block matches."""
     if not TRAFFIC_PROFILE.exists():
         return None
-    lines = TRAFFIC_PROFILE.read_text().splitlines()
-    tag = "gi_config " if gi else "config "
-    conf = next((l for l in lines if l.startswith(tag)), None)
-    if conf is None:
-        return None
-    want = dict(kv.split("=") for kv in conf.split()[1:])
-    if any(str(want.get(k, "morton" if k == "blas" else -1)) != str(v) for k, v in cfg.items()):
-        return None
-    for l in lines:
+    tag = "gi_config" if gi else "config"
+    match = False
+    for l in TRAFFIC_PROFILE.read_text().splitlines():
         parts = l.split()
-        if len(parts) == 4 and parts[0] == "traffic" and parts[1] == kernel:
+        if not parts:
+            continue
+        if parts[0] == tag:
+            want = dict(kv.split("=") for kv in parts[1:])
+            match = all(str(want.get(k, "morton" if k == "blas" else -1)) == str(v) for k, v in cfg.items())
+        elif parts[0] == "traffic" and match and len(parts) == 4 and parts[1] == kernel:
             vals = dict(kv.split("=") for kv in parts[2:])
             return int(vals["fetch_bytes"]) + int(vals["write_bytes"])
     return None
@@ -209,9 +216,13 @@ def pmc_workload(lux, scenes, args) -> int:
     n = rays_np.shape[0]
     rays = DeviceBuffer.from_array(ctx, rays_np)
     hits = DeviceBuffer.empty(ctx, (n, 4), np.uint32)
+    if args.trace_config:
+        ctx.set_trace_config(*[int(v) for v in args.trace_config.split(",")])
     for _ in range(3):
         ctx.trace_device(rays, n, hits)
     ctx.gi_configure(colonnade_gi_params(lux))
+    if args.gi_config:
+        ctx.gi_set_trace_config(*[int(v) for v in args.gi_config.split(",")])
     for _ in range(2):
         ctx.gi_update()
     ctx.synchronize()
@@ -392,7 +403,7 @@ def dynamic_section(lux, scenes, args) -> dict:
            "first_build_ms": round(first_ms, 3),
            "gi_trace_config": ctx.gi_trace_config(),
            "full_rebuild_ms": round(float(np.median(full)), 3),
-           "frame_blas": ("binned SAH on the host" if args.blas == "sah" else "Morton-30") +
+           "frame_blas": ("binned SAH on the GPU" if args.blas == "sah" else "Morton-30") +
                          " (%.1f ms, once)" % blas_ms}
     inv0, fwd0 = scenes.dynamic_transforms(motion, 0)
     n_int = ctx.gi_sizes()[0]
@@ -930,7 +941,7 @@ def main() -> int:
         "config": {
             "workload": "traversal microbench: 262,144-triangle colonnade, 4,194,304 rays "
                         "(half closest-hit, half any-hit), "
-                        + ("binned-SAH BLAS (host), Morton-30 TLAS" if args.blas == "sah" else "Morton-30 LBVH"),
+                        + ("binned-SAH BLAS (GPU build), Morton-30 TLAS" if args.blas == "sah" else "Morton-30 LBVH"),
             "rays": n,
             "triangles": scene.triangle_count,
             "bvh_build_ms": round(build_ms, 3),
