@@ -47,7 +47,7 @@ def parse():
     p.add_argument("--gi-steps", type=int, default=5)
     p.add_argument("--no-gi", action="store_true")
     p.add_argument("--no-surrogate", action="store_true")
-    p.add_argument("--train-frames", type=int, default=100)
+    p.add_argument("--train-frames", type=int, default=400)
     p.add_argument("--surrogate-levels", default="2,1")
     p.add_argument("--no-dynamic", action="store_true")
     p.add_argument("--dynamic-frames", type=int, default=120)
@@ -254,15 +254,18 @@ def surrogate_section(ctx, lux, gp, args) -> dict:
     largest predicted-error quantile whose probe RMSE against that frame's
     plain update is <= 2 %), evaluated on held-out frame 999 against its plain
     update.  Also reported: cascades truncated at S (the same traced work
-    without the network) and the noise floor (levels >= S taken from another
-    frame's trace)."""
+    without the network), the noise floor (levels >= S taken from another
+    frame's trace), and the error curve of training: the network's
+    no-fallback update of held-out frame 997 against that frame's plain
+    update after 1, 5, 10, 25, 50, 100 ... frames (outside the timed
+    training)."""
     import ctypes
     import time
     n = 262144
     buf = lux.DeviceBuffer.empty(ctx, (n, 12), np.float32)
     ctx.gi_configure(gp)
     refs, ms_plain = {}, []
-    for f in (998, 999):
+    for f in (997, 998, 999):
         ctx.gi_set_frame(f, f)
         ms_plain.append(ctx.gi_update(timed=True))
         refs[f] = ctx.gi_download()["probes"]
@@ -295,7 +298,8 @@ def surrogate_section(ctx, lux, gp, args) -> dict:
         res["noise_floor_rmse"] = round(probe_rmse(ctx.gi_download()["probes"], refs[999]), 5)
         s = lux.Surrogate(ctx, lux.SurrogateParams.default(origin, hi))
         t_samp = t_train = 0.0
-        loss = []
+        loss, curve = [], []
+        checkpoints = {1, 5, 10, 25, 50, 100, 200, 400, args.train_frames}
         for f in range(args.train_frames):
             ctx.gi_set_frame(f, f)
             ctx.synchronize()
@@ -307,7 +311,12 @@ def surrogate_section(ctx, lux, gp, args) -> dict:
             t2 = time.perf_counter()
             t_samp += t1 - t0
             t_train += t2 - t1
+            if f + 1 in checkpoints:
+                ctx.gi_set_frame(997, 997)
+                ctx.gi_update_surrogate(s, level, float("inf"))
+                curve.append([f + 1, round(probe_rmse(ctx.gi_download()["probes"], refs[997]), 5)])
         res["train"] = {"frames": args.train_frames, "samples_per_frame": n,
+                        "heldout_rmse_curve": curve,
                         "sample_trace_ms": round(t_samp / args.train_frames * 1e3, 3),
                         "train_step_ms": round(t_train / args.train_frames * 1e3, 3),
                         "loss_first_last10": [round(float(np.mean(loss[:10])), 4), round(float(np.mean(loss[-10:])), 4)]}

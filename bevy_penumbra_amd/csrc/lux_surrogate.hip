@@ -401,16 +401,59 @@ __global__ __launch_bounds__(64) void k_sur_wgrad(const short* A, uint32_t lda, 
 }
 
 // ---------------------------------------------------------------- hash backward
-__global__ __launch_bounds__(256) void k_sur_hash_bwd(SurDev d, const float* rec, uint32_t stride,
-                                                      uint32_t n, const float* dXh, float* grad) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+// Sort key of a sample: a hash of its unit-cube position bits, so samples
+// at one position sort next to each other.
+__global__ __launch_bounds__(256) void k_sur_pos_key(SurDev d, const float* rec, uint32_t stride, uint32_t n,
+                                                     uint32_t* key) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const float* r = rec + (size_t)stride * s;
     float p[3];
-    unit_position(d, r, p);
+    unit_position(d, rec + (size_t)stride * s, p);
+    uint32_t h = __float_as_uint(p[0]) * 0x9e3779b1u;
+    h = (h ^ (h >> 15) ^ __float_as_uint(p[1])) * 0x85ebca77u;
+    h = (h ^ (h >> 13) ^ __float_as_uint(p[2])) * 0xc2b2ae3du;
+    key[s] = h ^ (h >> 16);
+}
+
+// Trilinear scatter of dX into the FP32 table gradient, samples taken in
+// the order `order` (samples at one position adjacent).  Lanes of a wave
+// whose positions are bitwise equal form a run: the corner weights are the
+// run's, so a segmented scan sums the run's 32 gradient values and only its
+// last lane adds them to the table (one atomic per word and run instead of
+// one per sample; training samples concentrate on few positions).
+__global__ __launch_bounds__(256) void k_sur_hash_bwd(SurDev d, const float* rec, uint32_t stride,
+                                                      uint32_t n, const uint32_t* order, const float* dXh,
+                                                      float* grad) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63u;
+    const bool valid = j < n;
+    const uint32_t s = valid ? (order ? order[j] : j) : 0u;
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    if (valid) unit_position(d, rec + (size_t)stride * s, p);
+    // run heads: the first lane, an invalid neighbour, or a new position
+    const uint32_t px = __float_as_uint(p[0]), py = __float_as_uint(p[1]), pz = __float_as_uint(p[2]);
+    const uint32_t qx = __shfl_up(px, 1, 64), qy = __shfl_up(py, 1, 64), qz = __shfl_up(pz, 1, 64);
+    const bool prev_valid = __shfl_up(valid ? 1u : 0u, 1, 64) != 0u;
+    const bool head = lane == 0 || !valid || !prev_valid || px != qx || py != qy || pz != qz;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const uint32_t my_head = 63u - (uint32_t)__clzll(heads & upto);
+    const bool tail = valid && (lane == 63 || ((heads >> (lane + 1)) & 1ull) != 0ull);
+    float g[32];
     const float* gs = dXh + (size_t)s * 32;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) g[k] = valid ? gs[k] : 0.0f;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const bool take = lane >= (uint32_t)off && lane - (uint32_t)off >= my_head;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const float v = __shfl_up(g[k], off, 64);
+            if (take) g[k] += v;
+        }
+    }
+    if (!tail) return;
     for (int l = 0; l < kLevels; ++l) {
-        float g0 = gs[2 * l], g1 = gs[2 * l + 1];
+        const float g0 = g[2 * l], g1 = g[2 * l + 1];
         if (g0 == 0.0f && g1 == 0.0f) continue;
         float res = (float)d.res[l];
         float gx = p[0] * res, gy = p[1] * res, gz = p[2] * res;
@@ -519,6 +562,7 @@ struct lux_surrogate {
     lux_buffer sh_const;
     // per-call scratch sized for the largest batch seen
     lux_buffer X, H1, H2, Y, dYb, dH1, dH2, dXh, loss;
+    lux_buffer bwd_sort;  // position keys, sorted keys, order, sort temporaries and histogram
     uint32_t cap = 0;
     uint32_t step = 0;
 };
@@ -774,7 +818,7 @@ extern "C" void lux_surrogate_destroy(lux_surrogate* s) {
                           &s->mlp_f32,   &s->mlp_m,   &s->mlp_v,   &s->mlp_g,   &s->mlp_bf,
                           &s->sh_const,  &s->X,       &s->H1,      &s->H2,      &s->Y,
                           &s->dYb,       &s->dH1,     &s->dH2,     &s->dXh,     &s->loss,
-                          &s->table_ema, &s->table_ema_h, &s->mlp_ema, &s->mlp_ema_bf};
+                          &s->table_ema, &s->table_ema_h, &s->mlp_ema, &s->mlp_ema_bf, &s->bwd_sort};
     for (lux_buffer* b : bufs) lux_release(*b);
     delete s;
 }
@@ -840,8 +884,20 @@ extern "C" int lux_surrogate_train_step(lux_surrogate* s, const float* samples, 
                        static_cast<const short*>(s->H2.ptr), (uint32_t)kHid,
                        static_cast<const short*>(s->dYb.ptr), (uint32_t)kOut, n, (uint32_t)(kHid / 16),
                        (uint32_t)(kOut / 16), chunk, g + kW1 + kW2, (uint32_t)kOut);
+    // the scatter in position order (a stable sort of position hashes)
+    const size_t hw = lux_sort_hist_words(n);
+    if (int rc = lux_ensure(ctx, s->bwd_sort, 4ull * (5ull * n + hw) + 64)) return rc;
+    uint32_t* key = static_cast<uint32_t*>(s->bwd_sort.ptr);
+    uint32_t* skey = key + n;
+    uint32_t* order = skey + n;
+    uint32_t* tk = order + n;
+    uint32_t* tv = tk + n;
+    uint32_t* hist = tv + n;
+    hipLaunchKernelGGL(k_sur_pos_key, dim3(grid_for(n, 256)), dim3(256), 0, st, s->dev_train, samples, 12u, n, key);
+    if (int rc = lux_sort_pairs_impl(ctx, key, n, skey, order, tk, tv, hist, 32)) return rc;
     hipLaunchKernelGGL(k_sur_hash_bwd, dim3(grid_for(n, 256)), dim3(256), 0, st, s->dev_train, samples, 12u, n,
-                       static_cast<const float*>(s->dXh.ptr), static_cast<float*>(s->table_g.ptr));
+                       static_cast<const uint32_t*>(order), static_cast<const float*>(s->dXh.ptr),
+                       static_cast<float*>(s->table_g.ptr));
     LUX_HIP(ctx, hipGetLastError());
     if (s->prm.freeze_updates == 0u) {
         s->step += 1;
