@@ -965,13 +965,19 @@ __device__ __forceinline__ void walk_tri(const DevScene& s, Walk& w, St& st) {
     if (w.mode == kWalkFirst && w.h.tri != kEmpty) walk_instance_end(w, true);
 }
 
-// One iteration; returns true once the ray is finished.
+// One iteration; returns true once the ray is finished.  The phases run in
+// walk order and each falls through to the next within the iteration: a ray
+// that starts, or that ends its TLAS leaf's previous instance, enters its
+// BLAS and takes its first node step at once, and a node step that reaches a
+// leaf tests the leaf's first triangle at once.  A wave therefore runs the
+// node-step and triangle bodies with more of its lanes in each, while every
+// lane still performs its own steps in the same order.
 template <class Stack, class St>
 __device__ __forceinline__ bool walk_step(const DevScene& s, Walk& w, Stack& stk, St& st) {
+    if (w.phase == kPhaseTlas) walk_tlas(s, w, stk, st);
+    if (w.phase == kPhaseLeaf) walk_instance(s, w, st);
+    if (w.phase == kPhaseBlas) walk_blas(s, w, stk, st);
     if (w.phase == kPhaseTri) walk_tri(s, w, st);
-    else if (w.phase == kPhaseBlas) walk_blas(s, w, stk, st);
-    else if (w.phase == kPhaseTlas) walk_tlas(s, w, stk, st);
-    else if (w.phase == kPhaseLeaf) walk_instance(s, w, st);
     return w.phase == kPhaseDone;
 }
 

@@ -437,7 +437,7 @@ extern "C" int lux_autotune_trace(lux_ctx* ctx, const lux_ray* rays, uint32_t n,
     static const uint32_t cand_batch[] = {2, 4, 8, 16};
     const uint32_t* cands[4] = {cand_wpc, cand_depth, cand_regroup, cand_batch};
     const uint32_t ncand[4] = {5, 3, 6, 4};
-    uint32_t cur[4] = {20, 8, 8, 8};
+    uint32_t cur[4] = {20, 8, 8, 16};
     float cur_ms = 0.0f;
     uint32_t evals = 0;
     int rc = measure(cur, cur_ms);

@@ -1163,7 +1163,7 @@ extern "C" int lux_autotune_gi(lux_ctx* ctx, uint32_t repeats, lux_trace_config*
     static const uint32_t cand_batch[] = {4, 8, 16};
     const uint32_t* cands[4] = {cand_wpc, cand_depth, cand_regroup, cand_batch};
     const uint32_t ncand[4] = {4, 3, 5, 3};
-    uint32_t cur[4] = {20, 8, 8, 8};
+    uint32_t cur[4] = {20, 8, 16, 8};
     float cur_ms = 0.0f;
     uint32_t evals = 1;
     int rc = measure(cur, cur_ms);
