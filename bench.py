@@ -49,6 +49,10 @@ def parse():
     p.add_argument("--no-surrogate", action="store_true")
     p.add_argument("--train-frames", type=int, default=400)
     p.add_argument("--surrogate-levels", default="2,1")
+    p.add_argument("--reference-frames", type=int, default=64,
+                   help="plain updates averaged into the converged reference of the surrogate section")
+    p.add_argument("--reference-eval", type=int, default=8,
+                   help="held-out frames scored against the converged reference")
     p.add_argument("--no-dynamic", action="store_true")
     p.add_argument("--dynamic-frames", type=int, default=120)
     p.add_argument("--no-batch", action="store_true")
@@ -246,6 +250,23 @@ def probe_rmse(a, b) -> float:
                          np.mean(b[..., :3].astype(np.float64) ** 2)))
 
 
+def converged_reference(ctx, frames: int, first: int):
+    """Mean packed level-0 probe radiance (RGB, f64) of `frames` plain updates
+    seeded first .. first + frames - 1, and the relative RMS standard error of
+    that mean (its own noise, which every RMSE against it carries in
+    quadrature)."""
+    acc = acc2 = None
+    for f in range(first, first + frames):
+        ctx.gi_set_frame(f, f)
+        ctx.gi_update()
+        p = ctx.gi_download_probes()[..., :3].astype(np.float64)
+        acc = p if acc is None else acc + p
+        acc2 = p * p if acc2 is None else acc2 + p * p
+    mu = acc / frames
+    var = np.maximum(acc2 / frames - mu * mu, 0.0) * frames / (frames - 1)
+    return mu, float(np.sqrt(np.mean(var) / frames / np.mean(mu ** 2)))
+
+
 def surrogate_section(ctx, lux, gp, args) -> dict:
     """Equal-RMSE comparison of the surrogate-assisted update with the plain
     update on the same grid.  Per replaced level S: online training over
@@ -253,12 +274,20 @@ def surrogate_section(ctx, lux, gp, args) -> dict:
     step per frame), fallback bound calibrated on held-out frame 998 (the
     largest predicted-error quantile whose probe RMSE against that frame's
     plain update is <= 2 %), evaluated on held-out frame 999 against its plain
-    update.  Also reported: cascades truncated at S (the same traced work
-    without the network), the noise floor (levels >= S taken from another
+    update.  Also reported: cascades truncated at S (levels 0 .. S-1 only,
+    without a terminal tail: the same traced work without the network and
+    without any far field), the noise floor (levels >= S taken from another
     frame's trace), and the error curve of training: the network's
     no-fallback update of held-out frame 997 against that frame's plain
     update after 1, 5, 10, 25, 50, 100 ... frames (outside the timed
-    training)."""
+    training).
+
+    Second view ("converged"): every method's probe RMSE against the mean of
+    --reference-frames plain updates (the expected radiance; the plain update
+    equals the oracle frame by frame), on --reference-eval held-out frames,
+    as the RMS over those frames.  There one frame of the plain update has
+    its own sampling noise, and a method is at equal RMSE when its RMS is no
+    higher than the plain update's."""
     import ctypes
     import time
     n = 262144
@@ -272,12 +301,29 @@ def surrogate_section(ctx, lux, gp, args) -> dict:
     for _ in range(2):
         ms_plain.append(ctx.gi_update(timed=True))
     ms_plain = float(np.median(ms_plain))
+    mu, mu_noise = converged_reference(ctx, args.reference_frames, 10000)
+    eval_frames = list(range(20000, 20000 + args.reference_eval))
+
+    def conv(run) -> dict:
+        e, ms = [], []
+        for f in eval_frames:
+            ctx.gi_set_frame(f, f)
+            ms.append(run())
+            e.append(probe_rmse(ctx.gi_download_probes(), mu))
+        return {"rmse_rms": round(float(np.sqrt(np.mean(np.square(e)))), 5),
+                "rmse_frames": [round(x, 5) for x in e], "ms": round(float(np.median(ms)), 3)}
+
+    conv_plain = conv(lambda: ctx.gi_update(timed=True))
     origin = np.array(gp.origin, np.float64)
     hi = origin + gp.spacing * np.array(gp.dims, np.float64)
     out = {"protocol": "probe RMSE vs the plain update of the same held-out frame (which equals "
                        "the oracle within the parity tests' bounds); bound calibrated on frame 998, "
                        "evaluated on frame 999",
-           "plain_ms": round(ms_plain, 3), "rmse_target": 0.02, "levels": {}}
+           "plain_ms": round(ms_plain, 3), "rmse_target": 0.02, "levels": {},
+           "converged": {"protocol": "probe RMSE against the mean of %d plain updates (seeds 10000+), "
+                                     "RMS over held-out frames %d-%d" % (args.reference_frames, eval_frames[0],
+                                                                        eval_frames[-1]),
+                         "reference_noise": round(mu_noise, 5), "plain": conv_plain}}
     for level in [int(x) for x in args.surrogate_levels.split(",") if x]:
         res = {}
         tp = lux.GiParams()
@@ -288,6 +334,7 @@ def surrogate_section(ctx, lux, gp, args) -> dict:
         ctx.gi_set_frame(999, 999)
         t_ms = float(np.median([ctx.gi_update(timed=True) for _ in range(3)]))
         res["truncated"] = {"ms": round(t_ms, 3), "rmse": round(probe_rmse(ctx.gi_download()["probes"], refs[999]), 5)}
+        conv_trunc = conv(lambda: ctx.gi_update(timed=True))
         ctx.gi_configure(gp)
         ctx.gi_set_frame(998, 998)
         ctx.gi_trace_levels(level, gp.levels - 1)
@@ -351,6 +398,14 @@ def surrogate_section(ctx, lux, gp, args) -> dict:
             "speedup_with_online_training": round(
                 ms_plain / (ms + res["train"]["sample_trace_ms"] + res["train"]["train_step_ms"]), 3),
         })
+        cv = {"truncated": conv_trunc,
+              "no_fallback": conv(lambda: ctx.gi_update_surrogate(s, level, float("inf")).ms_total),
+              "calibrated": conv(lambda: ctx.gi_update_surrogate(s, level, chosen).ms_total)}
+        equal = [cv[k]["ms"] for k in ("no_fallback", "calibrated")
+                 if cv[k]["rmse_rms"] <= conv_plain["rmse_rms"]]
+        cv["equal_rmse_speedup"] = round(conv_plain["ms"] / min(equal), 3) if equal else None
+        cv["truncated_at_equal_rmse"] = conv_trunc["rmse_rms"] <= conv_plain["rmse_rms"]
+        res["converged"] = cv
         s.close()
         out["levels"][f"replace_from_{level}"] = res
     ctx.gi_configure(gp)

@@ -797,6 +797,13 @@ class Context:
                                              li.ctypes.data), "lux_gi_download")
         return dict(intervals=iv, merged=mg, probes=pr, lights=li[:l])
 
+    def gi_download_probes(self) -> np.ndarray:
+        """The packed level-0 probes only, (probes, 36, 4) float32."""
+        _, p, _ = self.gi_sizes()
+        pr = np.zeros((p, 36, 4), np.float32)
+        self._check(self.lib.lux_gi_download(self.h, None, None, pr.ctypes.data, None), "lux_gi_download")
+        return pr
+
     def gi_upload_intervals(self, intervals: np.ndarray):
         iv = np.ascontiguousarray(intervals, np.float32)
         self._check(self.lib.lux_gi_upload_intervals(self.h, iv.ctypes.data), "lux_gi_upload_intervals")
