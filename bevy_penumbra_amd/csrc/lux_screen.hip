@@ -148,6 +148,19 @@ __device__ __forceinline__ bool dirty(const lux_screen_params& p, float3 q) {
            q.x <= p.dirty_max[0] && q.y <= p.dirty_max[1] && q.z <= p.dirty_max[2];
 }
 
+// This workgroup's 8 x 8 tile.  Workgroups go to the 8 XCDs in turn and each
+// XCD has its own L2; numbering the tiles so that XCD k takes the k-th eighth
+// of the image (row-major) keeps the texels that neighbouring tiles' taps
+// share in one L2 instead of fetching them into all eight.
+__device__ __forceinline__ void xcd_tile(uint32_t& bx, uint32_t& by) {
+    const uint32_t gx = gridDim.x, n = gridDim.x * gridDim.y;
+    const uint32_t id = blockIdx.y * gx + blockIdx.x;
+    const uint32_t xcd = id % 8u, within = id / 8u, per = n / 8u, rem = n % 8u;
+    const uint32_t t = xcd * per + (xcd < rem ? xcd : rem) + within;
+    bx = t % gx;
+    by = t / gx;
+}
+
 // The full-resolution representative of a resolve texel.
 __device__ __forceinline__ uint64_t full_of(const ScreenArgs& a, int tx, int ty, uint32_t& fx, uint32_t& fy) {
     uint32_t lx = (uint32_t)tx * a.rs + a.rs / 2u, ly = (uint32_t)ty * a.rs + a.rs / 2u;
@@ -167,8 +180,10 @@ __global__ __launch_bounds__(64) void k_screen_cache_raster(ScreenArgs a, const 
                                                             const uint4* __restrict__ gbuffer,
                                                             const uint4* __restrict__ raster,
                                                             float4* __restrict__ cache) {
-    const int tx = (int)(blockIdx.x * kTile + threadIdx.x % kTile);
-    const int ty = (int)(blockIdx.y * kTile + threadIdx.x / kTile);
+    uint32_t bx, by;
+    xcd_tile(bx, by);
+    const int tx = (int)(bx * kTile + threadIdx.x % kTile);
+    const int ty = (int)(by * kTile + threadIdx.x / kTile);
     if (tx >= (int)a.rw || ty >= (int)a.rh) return;
     const uint64_t pix = (uint64_t)ty * a.rw + tx;
     uint32_t fx, fy;
@@ -191,8 +206,10 @@ __global__ __launch_bounds__(64) void k_screen_temporal(
     const float4* __restrict__ prev_recv, const float4* __restrict__ prev_bg, const float4* __restrict__ cache,
     float4* __restrict__ out_irr, float4* __restrict__ out_meta, float4* __restrict__ out_recv,
     float4* __restrict__ out_bg) {
-    const int tx = (int)(blockIdx.x * kTile + threadIdx.x % kTile);
-    const int ty = (int)(blockIdx.y * kTile + threadIdx.x / kTile);
+    uint32_t bx, by;
+    xcd_tile(bx, by);
+    const int tx = (int)(bx * kTile + threadIdx.x % kTile);
+    const int ty = (int)(by * kTile + threadIdx.x / kTile);
     if (tx >= (int)a.rw || ty >= (int)a.rh) return;
     const lux_screen_params& P = a.p;
     const uint32_t RW = a.rw, RH = a.rh, RS = a.rs;
@@ -460,8 +477,10 @@ __global__ __launch_bounds__(64) void k_screen_denoise(ScreenArgs a, int step_x,
                                                        const float4* __restrict__ in_recv,
                                                        const float* __restrict__ depth, float4* __restrict__ out,
                                                        float4* __restrict__ out_recv) {
-    const int tx = (int)(blockIdx.x * kTile + threadIdx.x % kTile);
-    const int ty = (int)(blockIdx.y * kTile + threadIdx.x / kTile);
+    uint32_t bx, by;
+    xcd_tile(bx, by);
+    const int tx = (int)(bx * kTile + threadIdx.x % kTile);
+    const int ty = (int)(by * kTile + threadIdx.x / kTile);
     if (tx >= (int)a.rw || ty >= (int)a.rh) return;
     const lux_screen_params& P = a.p;
     const uint32_t RW = a.rw, RH = a.rh;
