@@ -9,8 +9,11 @@
 // pixel) and the 3 x 3 neighbourhood reads those (L2-resident); it then reads
 // up to four previous texels with their metadata, receiver words and
 // background.  Denoise (a 5 x 5 footprint without its four far corners)
-// decodes in its taps: a decode cache there, or straight-line taps with all
-// loads in flight, both measured slower than early rejection.
+// stages each 16 x 16 tile's footprint in LDS and decodes every staged
+// texel's normal and world position there once (steps up to 4; larger steps
+// read and decode in the taps): a decode cache in global memory made the
+// taps wait on its loads, and staging without decoding left the decode as
+// the cost.
 //
 // Restatement choices shared with the oracle: row-major matrices with
 // h_i = ((m_i0 x + m_i1 y) + m_i2 z) + m_i3; dot products summed x, y, z;
@@ -471,17 +474,62 @@ __device__ __forceinline__ float pow32(float x) {
     return x16 * x16;
 }
 
-__global__ __launch_bounds__(64) void k_screen_denoise(ScreenArgs a, int step_x, int step_y,
-                                                       const float4* __restrict__ in_irr,
-                                                       const float4* __restrict__ moments,
-                                                       const float4* __restrict__ in_recv,
-                                                       const float* __restrict__ depth, float4* __restrict__ out,
-                                                       float4* __restrict__ out_recv) {
-    uint32_t bx, by;
-    xcd_tile(bx, by);
-    const int tx = (int)(bx * kTile + threadIdx.x % kTile);
-    const int ty = (int)(by * kTile + threadIdx.x / kTile);
-    if (tx >= (int)a.rw || ty >= (int)a.rh) return;
+// Where a denoise tap reads its neighbour: the images themselves, or a tile of
+// them staged in LDS (the images' values, so the taps compute the same).
+struct ImageTaps {
+    const ScreenArgs* a;
+    const float4* irr;
+    const float4* mom;
+    const float4* recv;
+    __device__ __forceinline__ float4 value(int ox, int oy) const { return irr[(uint64_t)oy * a->rw + ox]; }
+    __device__ __forceinline__ float2 moment(int ox, int oy) const {
+        const float4 m = mom[(uint64_t)oy * a->rw + ox];
+        return make_float2(m.z, m.w);
+    }
+    __device__ __forceinline__ float4 receiver(int ox, int oy) const { return recv[(uint64_t)oy * a->rw + ox]; }
+    __device__ __forceinline__ float3 normal(int, int, float4 orr) const { return recv_normal(orr); }
+    __device__ __forceinline__ float3 position(int ox, int oy, float w) const {
+        uint32_t ofx, ofy;
+        full_of(*a, ox, oy, ofx, ofy);
+        return world_pos(a->p.world_from_clip, a->p, (float)ofx + 0.5f, (float)ofy + 0.5f, w);
+    }
+};
+// A tile's taps with each texel's receiver normal and world position decoded
+// once when it is staged (the same expressions as ImageTaps evaluates).
+struct TileTaps {
+    const float4* val;
+    const float4* recv;  // receiver words x, y, w and the decoded normal's x
+    const float4* geo;   // decoded normal y, z; world position x, y
+    const float2* aux;   // world position z; moments w
+    const float* momz;   // moments z
+    int x0, y0, fw;      // image position of the tile's first slot, slots per row
+    __device__ __forceinline__ int at(int ox, int oy) const { return (oy - y0) * fw + (ox - x0); }
+    __device__ __forceinline__ float4 value(int ox, int oy) const { return val[at(ox, oy)]; }
+    __device__ __forceinline__ float2 moment(int ox, int oy) const {
+        const int i = at(ox, oy);
+        return make_float2(momz[i], aux[i].y);
+    }
+    __device__ __forceinline__ float4 receiver(int ox, int oy) const {
+        const float4 r = recv[at(ox, oy)];
+        return make_float4(r.x, r.y, 0.0f, r.z);
+    }
+    __device__ __forceinline__ float3 normal(int ox, int oy, float4) const {
+        const int i = at(ox, oy);
+        return make_float3(recv[i].w, geo[i].x, geo[i].y);
+    }
+    __device__ __forceinline__ float3 position(int ox, int oy, float) const {
+        const int i = at(ox, oy);
+        return make_float3(geo[i].z, geo[i].w, aux[i].x);
+    }
+};
+
+// One denoise pass at resolve texel (tx, ty) (shaders/cascades.slang:624-711).
+template <class Taps>
+__device__ __forceinline__ void denoise_texel(const ScreenArgs& a, int tx, int ty, int step_x, int step_y,
+                                              const Taps& taps, const float4* __restrict__ in_irr,
+                                              const float4* __restrict__ moments,
+                                              const float4* __restrict__ in_recv, const float* __restrict__ depth,
+                                              float4* __restrict__ out, float4* __restrict__ out_recv) {
     const lux_screen_params& P = a.p;
     const uint32_t RW = a.rw, RH = a.rh;
     const uint64_t pix = (uint64_t)ty * RW + tx;
@@ -513,27 +561,24 @@ __global__ __launch_bounds__(64) void k_screen_denoise(ScreenArgs a, int step_x,
         for (int x = -xr; x <= xr; ++x) {
             const int ox = tx + x * step_x, oy = ty + y * step_y;
             if (ox < 0 || oy < 0 || ox >= (int)RW || oy >= (int)RH) continue;
-            const uint64_t op = (uint64_t)oy * RW + ox;
-            const float4 value = in_irr[op];
+            const float4 value = taps.value(ox, oy);
             if (value.w <= 0.0f) continue;
-            const float4 om = moments[op];
-            if (om.z <= 0.0f) continue;
-            const float4 orr = in_recv[op];
+            const float2 om = taps.moment(ox, oy);
+            if (om.x <= 0.0f) continue;
+            const float4 orr = taps.receiver(ox, oy);
             if (!recv_same(receiver, orr)) continue;
-            const float3 on = recv_normal(orr);
+            const float3 on = taps.normal(ox, oy, orr);
             if (!(dot3(gn, on) > 0.97f)) continue;
             const uint32_t omask = support_mask(orr);
             if (cmask != 0u && omask != cmask) continue;
             if (cmask == 0u && omask == 0u) continue;
-            uint32_t ofx, ofy;
-            full_of(a, ox, oy, ofx, ofy);
-            const float3 opos = world_pos(P.world_from_clip, P, (float)ofx + 0.5f, (float)ofy + 0.5f, value.w);
+            const float3 opos = taps.position(ox, oy, value.w);
             if (dot3(gn, on) < 0.95f) continue;
             const float pd = plane_dist(sub3(opos, position), gn, on);
             if (pd > tol) continue;
             float w = binomial(x) * binomial(y) * pow32(fmaxf(0.0f, dot3(normal, on))) * expf(-pd / falloff);
             if (csup) {
-                const float scale = fmaxf(0.02f, 2.0f * sqrtf(fmaxf(variance + om.w, 0.0f)));
+                const float scale = fmaxf(0.02f, 2.0f * sqrtf(fmaxf(variance + om.y, 0.0f)));
                 w *= expf(-fabsf(lum(value.x, value.y, value.z) - center_lum) / scale);
             }
             sum.x += value.x * w; sum.y += value.y * w; sum.z += value.z * w;
@@ -548,6 +593,69 @@ __global__ __launch_bounds__(64) void k_screen_denoise(ScreenArgs a, int step_x,
     o.w = has_light || csup || fsup ? d : 0.0f;
     out[pix] = o;
     out_recv[pix] = with_support(receiver, o.w > 0.0f ? 1u : 0u);
+}
+
+// Taps straight from the images: 8 x 8 tiles, one wave each (any step).
+__global__ __launch_bounds__(64) void k_screen_denoise(ScreenArgs a, int step_x, int step_y,
+                                                       const float4* __restrict__ in_irr,
+                                                       const float4* __restrict__ moments,
+                                                       const float4* __restrict__ in_recv,
+                                                       const float* __restrict__ depth, float4* __restrict__ out,
+                                                       float4* __restrict__ out_recv) {
+    uint32_t bx, by;
+    xcd_tile(bx, by);
+    const int tx = (int)(bx * kTile + threadIdx.x % kTile);
+    const int ty = (int)(by * kTile + threadIdx.x / kTile);
+    if (tx >= (int)a.rw || ty >= (int)a.rh) return;
+    const ImageTaps taps{&a, in_irr, moments, in_recv};
+    denoise_texel(a, tx, ty, step_x, step_y, taps, in_irr, moments, in_recv, depth, out, out_recv);
+}
+
+// Taps from LDS: a 16 x 16 tile of texels per workgroup stages its whole
+// footprint (the tile and two steps on each side) once -- irradiance, the
+// receiver words a tap compares, the decoded receiver normal, the world
+// position at the texel's irradiance depth and the moments' (z, w), 60 bytes
+// per texel -- so each texel is read and decoded once per workgroup instead
+// of by each of its up to 21 taps.  Dynamic LDS of 60 bytes per slot.
+constexpr int kDTile = 16;
+__global__ __launch_bounds__(256) void k_screen_denoise_tile(ScreenArgs a, int step_x, int step_y,
+                                                             const float4* __restrict__ in_irr,
+                                                             const float4* __restrict__ moments,
+                                                             const float4* __restrict__ in_recv,
+                                                             const float* __restrict__ depth,
+                                                             float4* __restrict__ out, float4* __restrict__ out_recv) {
+    extern __shared__ float4 lds_taps[];
+    uint32_t bx, by;
+    xcd_tile(bx, by);
+    const int fw = kDTile + 4 * step_x, n = fw * (kDTile + 4 * step_y);
+    const int x0 = (int)bx * kDTile - 2 * step_x, y0 = (int)by * kDTile - 2 * step_y;
+    float4* val = lds_taps;
+    float4* recv = val + n;
+    float4* geo = recv + n;
+    float2* aux = reinterpret_cast<float2*>(geo + n);
+    float* momz = reinterpret_cast<float*>(aux + n);
+    const ImageTaps image{&a, in_irr, moments, in_recv};
+    for (int i = (int)threadIdx.x; i < n; i += 256) {
+        const int ox = x0 + i % fw, oy = y0 + i / fw;
+        if (ox >= 0 && oy >= 0 && ox < (int)a.rw && oy < (int)a.rh) {
+            const float4 v = image.value(ox, oy);
+            const float4 r = image.receiver(ox, oy);
+            const float2 m = image.moment(ox, oy);
+            const float3 nn = image.normal(ox, oy, r);
+            const float3 q = v.w > 0.0f ? image.position(ox, oy, v.w) : make_float3(0.0f, 0.0f, 0.0f);
+            val[i] = v;
+            recv[i] = make_float4(r.x, r.y, r.w, nn.x);
+            geo[i] = make_float4(nn.y, nn.z, q.x, q.y);
+            aux[i] = make_float2(q.z, m.y);
+            momz[i] = m.x;
+        }
+    }
+    __syncthreads();
+    const int tx = (int)bx * kDTile + (int)(threadIdx.x % kDTile);
+    const int ty = (int)by * kDTile + (int)(threadIdx.x / kDTile);
+    if (tx >= (int)a.rw || ty >= (int)a.rh) return;
+    const TileTaps taps{val, recv, geo, aux, momz, x0, y0, fw};
+    denoise_texel(a, tx, ty, step_x, step_y, taps, in_irr, moments, in_recv, depth, out, out_recv);
 }
 
 int screen_args(lux_ctx* ctx, const lux_screen_params* p, const char* who, ScreenArgs* a) {
@@ -648,6 +756,17 @@ extern "C" int lux_screen_denoise(lux_ctx* ctx, const lux_screen_params* p, uint
         step[q] = pass == 0 ? 1 : (int)(b * (1u << pass));
     }
     LUX_HIP(ctx, hipSetDevice(ctx->device));
+    // the staged tile while its footprint fits 64 KB of LDS (steps up to 4)
+    const uint64_t lds = (kDTile + 4ull * (uint64_t)step[0]) * (kDTile + 4ull * (uint64_t)step[1]) * 60ull;
+    if (lds <= 65536u) {
+        const dim3 tgrid((a.rw + kDTile - 1) / kDTile, (a.rh + kDTile - 1) / kDTile);
+        hipLaunchKernelGGL(k_screen_denoise_tile, tgrid, dim3(256), (size_t)lds, ctx->stream, a, step[0], step[1],
+                           reinterpret_cast<const float4*>(irradiance), reinterpret_cast<const float4*>(moments),
+                           reinterpret_cast<const float4*>(receiver), depth, reinterpret_cast<float4*>(out),
+                           reinterpret_cast<float4*>(out_receiver));
+        LUX_HIP(ctx, hipGetLastError());
+        return LUX_OK;
+    }
     const dim3 grid((a.rw + kTile - 1) / kTile, (a.rh + kTile - 1) / kTile);
     hipLaunchKernelGGL(k_screen_denoise, grid, dim3(64), 0, ctx->stream, a, step[0], step[1],
                        reinterpret_cast<const float4*>(irradiance), reinterpret_cast<const float4*>(moments),
