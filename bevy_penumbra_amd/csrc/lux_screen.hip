@@ -9,11 +9,11 @@
 // pixel) and the 3 x 3 neighbourhood reads those (L2-resident); it then reads
 // up to four previous texels with their metadata, receiver words and
 // background.  Denoise (a 5 x 5 footprint without its four far corners)
-// stages each 16 x 16 tile's footprint in LDS and decodes every staged
-// texel's normal and world position there once (steps up to 4; larger steps
-// read and decode in the taps): a decode cache in global memory made the
-// taps wait on its loads, and staging without decoding left the decode as
-// the cost.
+// decodes the normal and world position of every texel of a 16 x 16 tile's
+// footprint once into LDS (steps up to 8; larger steps decode in the taps)
+// and its taps read the images' values from L2: a decode cache in global
+// memory made the taps wait on its loads, staging the raw values as well
+// cost occupancy, and staging without decoding left the decode as the cost.
 //
 // Restatement choices shared with the oracle: row-major matrices with
 // h_i = ((m_i0 x + m_i1 y) + m_i2 z) + m_i3; dot products summed x, y, z;
@@ -494,32 +494,25 @@ struct ImageTaps {
         return world_pos(a->p.world_from_clip, a->p, (float)ofx + 0.5f, (float)ofy + 0.5f, w);
     }
 };
-// A tile's taps with each texel's receiver normal and world position decoded
-// once when it is staged (the same expressions as ImageTaps evaluates).
+// A tile's taps: the images' values, with each texel's receiver normal and
+// world position decoded once when the tile is staged (the same expressions
+// as ImageTaps evaluates).
 struct TileTaps {
-    const float4* val;
-    const float4* recv;  // receiver words x, y, w and the decoded normal's x
-    const float4* geo;   // decoded normal y, z; world position x, y
-    const float2* aux;   // world position z; moments w
-    const float* momz;   // moments z
+    ImageTaps image;
+    const float4* geo;   // decoded normal x, y, z; world position x
+    const float2* geo2;  // world position y, z
     int x0, y0, fw;      // image position of the tile's first slot, slots per row
     __device__ __forceinline__ int at(int ox, int oy) const { return (oy - y0) * fw + (ox - x0); }
-    __device__ __forceinline__ float4 value(int ox, int oy) const { return val[at(ox, oy)]; }
-    __device__ __forceinline__ float2 moment(int ox, int oy) const {
-        const int i = at(ox, oy);
-        return make_float2(momz[i], aux[i].y);
-    }
-    __device__ __forceinline__ float4 receiver(int ox, int oy) const {
-        const float4 r = recv[at(ox, oy)];
-        return make_float4(r.x, r.y, 0.0f, r.z);
-    }
+    __device__ __forceinline__ float4 value(int ox, int oy) const { return image.value(ox, oy); }
+    __device__ __forceinline__ float2 moment(int ox, int oy) const { return image.moment(ox, oy); }
+    __device__ __forceinline__ float4 receiver(int ox, int oy) const { return image.receiver(ox, oy); }
     __device__ __forceinline__ float3 normal(int ox, int oy, float4) const {
-        const int i = at(ox, oy);
-        return make_float3(recv[i].w, geo[i].x, geo[i].y);
+        const float4 g = geo[at(ox, oy)];
+        return make_float3(g.x, g.y, g.z);
     }
     __device__ __forceinline__ float3 position(int ox, int oy, float) const {
         const int i = at(ox, oy);
-        return make_float3(geo[i].z, geo[i].w, aux[i].x);
+        return make_float3(geo[i].w, geo2[i].x, geo2[i].y);
     }
 };
 
@@ -611,12 +604,12 @@ __global__ __launch_bounds__(64) void k_screen_denoise(ScreenArgs a, int step_x,
     denoise_texel(a, tx, ty, step_x, step_y, taps, in_irr, moments, in_recv, depth, out, out_recv);
 }
 
-// Taps from LDS: a 16 x 16 tile of texels per workgroup stages its whole
-// footprint (the tile and two steps on each side) once -- irradiance, the
-// receiver words a tap compares, the decoded receiver normal, the world
-// position at the texel's irradiance depth and the moments' (z, w), 60 bytes
-// per texel -- so each texel is read and decoded once per workgroup instead
-// of by each of its up to 21 taps.  Dynamic LDS of 60 bytes per slot.
+// Taps with decoded geometry in LDS: a 16 x 16 tile of texels per workgroup
+// decodes the receiver normal and the world position (at the texel's
+// irradiance depth) of every texel of its footprint -- the tile and two steps
+// on each side -- once, 24 bytes per texel, so a texel is decoded once per
+// workgroup instead of by each of its up to 21 taps; the taps read the
+// images' values (L2-resident).  Dynamic LDS of 24 bytes per slot.
 constexpr int kDTile = 16;
 __global__ __launch_bounds__(256) void k_screen_denoise_tile(ScreenArgs a, int step_x, int step_y,
                                                              const float4* __restrict__ in_irr,
@@ -629,32 +622,24 @@ __global__ __launch_bounds__(256) void k_screen_denoise_tile(ScreenArgs a, int s
     xcd_tile(bx, by);
     const int fw = kDTile + 4 * step_x, n = fw * (kDTile + 4 * step_y);
     const int x0 = (int)bx * kDTile - 2 * step_x, y0 = (int)by * kDTile - 2 * step_y;
-    float4* val = lds_taps;
-    float4* recv = val + n;
-    float4* geo = recv + n;
-    float2* aux = reinterpret_cast<float2*>(geo + n);
-    float* momz = reinterpret_cast<float*>(aux + n);
+    float4* geo = lds_taps;
+    float2* geo2 = reinterpret_cast<float2*>(geo + n);
     const ImageTaps image{&a, in_irr, moments, in_recv};
     for (int i = (int)threadIdx.x; i < n; i += 256) {
         const int ox = x0 + i % fw, oy = y0 + i / fw;
         if (ox >= 0 && oy >= 0 && ox < (int)a.rw && oy < (int)a.rh) {
-            const float4 v = image.value(ox, oy);
-            const float4 r = image.receiver(ox, oy);
-            const float2 m = image.moment(ox, oy);
-            const float3 nn = image.normal(ox, oy, r);
-            const float3 q = v.w > 0.0f ? image.position(ox, oy, v.w) : make_float3(0.0f, 0.0f, 0.0f);
-            val[i] = v;
-            recv[i] = make_float4(r.x, r.y, r.w, nn.x);
-            geo[i] = make_float4(nn.y, nn.z, q.x, q.y);
-            aux[i] = make_float2(q.z, m.y);
-            momz[i] = m.x;
+            const float w = image.value(ox, oy).w;
+            const float3 nn = image.normal(ox, oy, image.receiver(ox, oy));
+            const float3 q = w > 0.0f ? image.position(ox, oy, w) : make_float3(0.0f, 0.0f, 0.0f);
+            geo[i] = make_float4(nn.x, nn.y, nn.z, q.x);
+            geo2[i] = make_float2(q.y, q.z);
         }
     }
     __syncthreads();
     const int tx = (int)bx * kDTile + (int)(threadIdx.x % kDTile);
     const int ty = (int)by * kDTile + (int)(threadIdx.x / kDTile);
     if (tx >= (int)a.rw || ty >= (int)a.rh) return;
-    const TileTaps taps{val, recv, geo, aux, momz, x0, y0, fw};
+    const TileTaps taps{image, geo, geo2, x0, y0, fw};
     denoise_texel(a, tx, ty, step_x, step_y, taps, in_irr, moments, in_recv, depth, out, out_recv);
 }
 
@@ -756,8 +741,8 @@ extern "C" int lux_screen_denoise(lux_ctx* ctx, const lux_screen_params* p, uint
         step[q] = pass == 0 ? 1 : (int)(b * (1u << pass));
     }
     LUX_HIP(ctx, hipSetDevice(ctx->device));
-    // the staged tile while its footprint fits 64 KB of LDS (steps up to 4)
-    const uint64_t lds = (kDTile + 4ull * (uint64_t)step[0]) * (kDTile + 4ull * (uint64_t)step[1]) * 60ull;
+    // the staged tile while its footprint fits 64 KB of LDS (steps up to 8)
+    const uint64_t lds = (kDTile + 4ull * (uint64_t)step[0]) * (kDTile + 4ull * (uint64_t)step[1]) * 24ull;
     if (lds <= 65536u) {
         const dim3 tgrid((a.rw + kDTile - 1) / kDTile, (a.rh + kDTile - 1) / kDTile);
         hipLaunchKernelGGL(k_screen_denoise_tile, tgrid, dim3(256), (size_t)lds, ctx->stream, a, step[0], step[1],
