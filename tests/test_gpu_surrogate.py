@@ -176,17 +176,21 @@ def test_gpu_training_fits_a_smooth_field(lux_ctx):
 
     before = held_error()
     buf = DeviceBuffer.empty(lux_ctx, (16384, 12), np.float32)
-    batches = [batch(16384) for _ in range(40)]
+    curve = []
     for step in range(1000):
-        buf.upload(batches[step % 40])
+        # fresh samples every step, as online training draws them: cycling a
+        # few fixed batches lets the fine hash levels memorise their positions
+        # (the numpy oracle of this network plateaus at 10-20 % held-out error
+        # on 40 cycled batches and reaches 2.8 % after 400 fresh steps of 4096)
+        buf.upload(batch(16384))
         s.train_step(buf, 16384, want_loss=False)
+        if step + 1 in (100, 250, 500):
+            curve.append(round(held_error(), 4))
     after = held_error()
-    print(f"held-out relative error {before:.4f} -> {after:.4f}")
-    # the same network trained in numpy (f64 or with the BF16 rounding points)
-    # follows the same curve (about 10 % after 150 steps of 4096 samples);
-    # on the GPU 1000 steps end between 5 % and 12 % (FP32 atomics make runs
-    # differ)
-    assert after < 0.15 and after < 0.2 * before, (before, after)
+    print(f"held-out relative error {before:.4f} -> {curve} -> {after:.4f}")
+    # on the GPU: about 23 %, 5.5 %, 2.7 % and 1.4 % after 100, 250, 500, 1000
+    assert curve[0] > curve[1] > curve[2] > after, (before, curve, after)
+    assert after < 0.05 and after < 0.2 * before, (before, curve, after)
     s.close()
 
 
