@@ -47,7 +47,7 @@ def parse():
     p.add_argument("--gi-steps", type=int, default=5)
     p.add_argument("--no-gi", action="store_true")
     p.add_argument("--no-surrogate", action="store_true")
-    p.add_argument("--train-frames", type=int, default=400)
+    p.add_argument("--train-frames", type=int, default=1600)
     p.add_argument("--surrogate-levels", default="2,1")
     p.add_argument("--reference-frames", type=int, default=64,
                    help="plain updates averaged into the converged reference of the surrogate section")
@@ -57,7 +57,7 @@ def parse():
     p.add_argument("--dynamic-frames", type=int, default=120)
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--batch-scenes", type=int, default=64)
-    p.add_argument("--batch-train", type=int, default=100,
+    p.add_argument("--batch-train", type=int, default=400,
                    help="online training steps of each batch scene's surrogate")
     p.add_argument("--no-receivers", action="store_true")
     p.add_argument("--no-screen", action="store_true")
@@ -346,7 +346,7 @@ def surrogate_section(ctx, lux, gp, args) -> dict:
         s = lux.Surrogate(ctx, lux.SurrogateParams.default(origin, hi))
         t_samp = t_train = 0.0
         loss, curve = [], []
-        checkpoints = {1, 5, 10, 25, 50, 100, 200, 400, args.train_frames}
+        checkpoints = {1, 5, 10, 25, 50, 100, 200, 400, 800, 1600, args.train_frames}
         for f in range(args.train_frames):
             ctx.gi_set_frame(f, f)
             ctx.synchronize()
@@ -408,6 +408,17 @@ def surrogate_section(ctx, lux, gp, args) -> dict:
         res["converged"] = cv
         s.close()
         out["levels"][f"replace_from_{level}"] = res
+    # every method of the converged view by cost: the cheapest at or below
+    # the plain update's RMSE is the equal-RMSE choice
+    methods = [{"method": "plain", "ms": conv_plain["ms"], "rmse_rms": conv_plain["rmse_rms"]}]
+    for key, res in out["levels"].items():
+        for m in ("truncated", "no_fallback", "calibrated"):
+            cv = res["converged"][m]
+            methods.append({"method": f"{key} {m}", "ms": cv["ms"], "rmse_rms": cv["rmse_rms"]})
+    methods.sort(key=lambda m: m["ms"])
+    out["converged"]["by_cost"] = methods
+    ok = [m for m in methods if m["rmse_rms"] <= conv_plain["rmse_rms"]]
+    out["converged"]["cheapest_at_plain_rmse"] = dict(ok[0], speedup=round(conv_plain["ms"] / ok[0]["ms"], 3))
     ctx.gi_configure(gp)
     return out
 
