@@ -561,6 +561,7 @@ int enqueue(lux_batch* b) {
         switch (b->depth) {
         case 4: hipLaunchKernelGGL(k_b_primary<4>, dim3(pb), dim3(kWg), 0, st, w); break;
         case 8: hipLaunchKernelGGL(k_b_primary<8>, dim3(pb), dim3(kWg), 0, st, w); break;
+        case 12: hipLaunchKernelGGL(k_b_primary<12>, dim3(pb), dim3(kWg), 0, st, w); break;
         default: hipLaunchKernelGGL(k_b_primary<16>, dim3(pb), dim3(kWg), 0, st, w); break;
         }
         hipLaunchKernelGGL(k_b_setup, dim3(need), dim3(kWg), 0, st, w);
@@ -574,6 +575,7 @@ int enqueue(lux_batch* b) {
         switch (b->depth) {
         case 4: hipLaunchKernelGGL(k_b_shadow<4>, dim3(pblocks), dim3(kWg), 0, st, w); break;
         case 8: hipLaunchKernelGGL(k_b_shadow<8>, dim3(pblocks), dim3(kWg), 0, st, w); break;
+        case 12: hipLaunchKernelGGL(k_b_shadow<12>, dim3(pblocks), dim3(kWg), 0, st, w); break;
         default: hipLaunchKernelGGL(k_b_shadow<16>, dim3(pblocks), dim3(kWg), 0, st, w); break;
         }
         hipLaunchKernelGGL(k_b_accum, dim3((uint32_t)((intervals + kWg - 1) / kWg)), dim3(kWg), 0, st, w);
@@ -790,7 +792,7 @@ extern "C" int lux_batch_set_config(lux_batch* b, const lux_trace_config* cfg) {
     if (!b || !cfg) return LUX_ERR_INVALID;
     if (cfg->waves_per_cu > 32 || cfg->regroup_threshold > 64 || cfg->steps_per_refill > 256 ||
         (cfg->lds_stack_depth != 0 && cfg->lds_stack_depth != 4 && cfg->lds_stack_depth != 8 &&
-         cfg->lds_stack_depth != 16))
+         cfg->lds_stack_depth != 12 && cfg->lds_stack_depth != 16))
         return bfail(b, LUX_ERR_INVALID, "lux_batch_set_config: value out of range");
     b->wpc = cfg->waves_per_cu;
     b->depth = cfg->lds_stack_depth;
@@ -861,11 +863,11 @@ extern "C" int lux_batch_autotune(lux_batch* b, uint32_t repeats, lux_trace_conf
         return LUX_OK;
     };
     static const uint32_t cand_wpc[] = {16, 20, 24, 32};
-    static const uint32_t cand_depth[] = {4, 8, 16};
+    static const uint32_t cand_depth[] = {4, 8, 12, 16};
     static const uint32_t cand_regroup[] = {4, 8, 16, 32, 48};
     static const uint32_t cand_batch[] = {4, 8, 16};
     const uint32_t* cands[4] = {cand_wpc, cand_depth, cand_regroup, cand_batch};
-    const uint32_t ncand[4] = {4, 3, 5, 3};
+    const uint32_t ncand[4] = {4, 4, 5, 3};
     uint32_t cur[4] = {16, 16, 16, 8};
     float cur_ms = 0.0f;
     uint32_t evals = 1;

@@ -245,6 +245,7 @@ static int trace_launch_with(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_
         hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, a.cursor);
         switch (ctx->trace_depth) {
         case 4: hipLaunchKernelGGL(k_trace_regroup<4>, dim3(blocks), dim3(kTraceWg), 0, st, a); break;
+        case 12: hipLaunchKernelGGL(k_trace_regroup<12>, dim3(blocks), dim3(kTraceWg), 0, st, a); break;
         case 16: hipLaunchKernelGGL(k_trace_regroup<16>, dim3(blocks), dim3(kTraceWg), 0, st, a); break;
         default: hipLaunchKernelGGL(k_trace_regroup<8>, dim3(blocks), dim3(kTraceWg), 0, st, a); break;
         }
@@ -368,8 +369,8 @@ extern "C" int lux_set_trace_config(lux_ctx* ctx, const lux_trace_config* cfg) {
     if (cfg->regroup_threshold > 64)
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_set_trace_config: regroup threshold > 64");
     if (cfg->lds_stack_depth != 0 && cfg->lds_stack_depth != 4 && cfg->lds_stack_depth != 8 &&
-        cfg->lds_stack_depth != 16)
-        return lux_fail(ctx, LUX_ERR_INVALID, "lux_set_trace_config: LDS stack depth must be 4, 8 or 16");
+        cfg->lds_stack_depth != 12 && cfg->lds_stack_depth != 16)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_set_trace_config: LDS stack depth must be 4, 8, 12 or 16");
     if (cfg->steps_per_refill > 256)
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_set_trace_config: steps per refill > 256");
     ctx->waves_per_cu = cfg->waves_per_cu;
@@ -432,12 +433,12 @@ extern "C" int lux_autotune_trace(lux_ctx* ctx, const lux_ray* rays, uint32_t n,
         return LUX_OK;
     };
     static const uint32_t cand_wpc[] = {12, 16, 20, 24, 32};
-    static const uint32_t cand_depth[] = {4, 8, 16};
+    static const uint32_t cand_depth[] = {4, 8, 12, 16};
     static const uint32_t cand_regroup[] = {0, 1, 4, 8, 16, 32};
     static const uint32_t cand_batch[] = {2, 4, 8, 16};
     const uint32_t* cands[4] = {cand_wpc, cand_depth, cand_regroup, cand_batch};
-    const uint32_t ncand[4] = {5, 3, 6, 4};
-    uint32_t cur[4] = {20, 8, 8, 16};
+    const uint32_t ncand[4] = {5, 4, 6, 4};
+    uint32_t cur[4] = {20, 12, 8, 16};
     float cur_ms = 0.0f;
     uint32_t evals = 0;
     int rc = measure(cur, cur_ms);

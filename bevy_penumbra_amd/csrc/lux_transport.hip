@@ -1017,6 +1017,7 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
         } else {
             switch (ctx->gi_depth) {
             case 4: hipLaunchKernelGGL(k_gi_primary<4>, dim3(pb), dim3(kWg), 0, st, a, w); break;
+            case 12: hipLaunchKernelGGL(k_gi_primary<12>, dim3(pb), dim3(kWg), 0, st, a, w); break;
             case 16: hipLaunchKernelGGL(k_gi_primary<16>, dim3(pb), dim3(kWg), 0, st, a, w); break;
             default: hipLaunchKernelGGL(k_gi_primary<8>, dim3(pb), dim3(kWg), 0, st, a, w); break;
             }
@@ -1034,6 +1035,7 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
         } else {
             switch (ctx->gi_depth) {
             case 4: hipLaunchKernelGGL(k_gi_shadow<4>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
+            case 12: hipLaunchKernelGGL(k_gi_shadow<12>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
             case 16: hipLaunchKernelGGL(k_gi_shadow<16>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
             default: hipLaunchKernelGGL(k_gi_shadow<8>, dim3(pblocks), dim3(kWg), 0, st, a, w); break;
             }
@@ -1104,8 +1106,8 @@ extern "C" int lux_gi_set_trace_config(lux_ctx* ctx, const lux_trace_config* cfg
     if (cfg->regroup_threshold > 64)
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_set_trace_config: regroup threshold > 64");
     if (cfg->lds_stack_depth != 0 && cfg->lds_stack_depth != 4 && cfg->lds_stack_depth != 8 &&
-        cfg->lds_stack_depth != 16)
-        return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_set_trace_config: LDS stack depth must be 4, 8 or 16");
+        cfg->lds_stack_depth != 12 && cfg->lds_stack_depth != 16)
+        return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_set_trace_config: LDS stack depth must be 4, 8, 12 or 16");
     if (cfg->steps_per_refill > 256)
         return lux_fail(ctx, LUX_ERR_INVALID, "lux_gi_set_trace_config: steps per refill > 256");
     ctx->gi_wpc = cfg->waves_per_cu;
@@ -1158,12 +1160,12 @@ extern "C" int lux_autotune_gi(lux_ctx* ctx, uint32_t repeats, lux_trace_config*
         return LUX_OK;
     };
     static const uint32_t cand_wpc[] = {16, 20, 24, 32};
-    static const uint32_t cand_depth[] = {4, 8, 16};
+    static const uint32_t cand_depth[] = {4, 8, 12, 16};
     static const uint32_t cand_regroup[] = {4, 8, 16, 32, 48};
     static const uint32_t cand_batch[] = {4, 8, 16};
     const uint32_t* cands[4] = {cand_wpc, cand_depth, cand_regroup, cand_batch};
-    const uint32_t ncand[4] = {4, 3, 5, 3};
-    uint32_t cur[4] = {20, 8, 16, 8};
+    const uint32_t ncand[4] = {4, 4, 5, 3};
+    uint32_t cur[4] = {20, 12, 16, 8};
     float cur_ms = 0.0f;
     uint32_t evals = 1;
     int rc = measure(cur, cur_ms);

@@ -104,9 +104,9 @@ typedef struct lux_ctx lux_ctx;
  * 64 rays to completion; 1..64 runs resumable walks and refills idle lanes
  * once that many are idle, checking every steps_per_refill walk steps
  * (0 = 8).  lds_stack_depth: traversal-stack slots per lane kept in LDS
- * (4, 8 or 16; 0 = 8), deeper slots spill to private memory.
+ * (4, 8, 12 or 16; 0 = 8), deeper slots spill to private memory.
  * waves_per_cu: persistent grid size (0 = 20; 1000 = one ray per thread,
- * no persistence).  A new context starts at {20, 8, 8, 16}.  Results do not
+ * no persistence).  A new context starts at {20, 12, 8, 16}.  Results do not
  * depend on the configuration. */
 typedef struct {
     uint32_t waves_per_cu;
@@ -477,7 +477,7 @@ int lux_gi_trace_levels(lux_ctx* ctx, uint32_t first, uint32_t last);
 int lux_gi_set_trace_mode(lux_ctx* ctx, uint32_t mode);
 /* Launch configuration of the wavefront stages' walks (same fields and
  * limits as lux_trace_config; waves_per_cu 1..32; a new context starts at
- * {20, 8, 16, 8}) and its autotuner, which times full interval traces of the
+ * {20, 12, 16, 8}) and its autotuner, which times full interval traces of the
  * configured grid and installs the fastest (overwrites the interval array).
  * No counterpart in the reference. */
 int lux_gi_set_trace_config(lux_ctx* ctx, const lux_trace_config* cfg);

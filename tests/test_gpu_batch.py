@@ -113,6 +113,12 @@ def test_gpu_batch_equals_per_scene_updates(rooms):
     b.update()
     for i, (r, g) in enumerate(zip(refs8, batch_states(rooms, b))):
         assert_same(r, g, f"scene {i}, frame 8, config 2")
+    b.set_config(waves_per_cu=20, lds_stack_depth=12, regroup_threshold=16, steps_per_refill=16)
+    for c in rooms:
+        poison(c)
+    b.update()
+    for i, (r, g) in enumerate(zip(refs8, batch_states(rooms, b))):
+        assert_same(r, g, f"scene {i}, frame 8, config 3")
     b.close()
 
 

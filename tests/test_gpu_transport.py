@@ -220,6 +220,14 @@ def test_gpu_gi_autotuner_keeps_results(lux_ctx):
     got = lux_ctx.gi_download()
     for key in ("intervals", "merged", "probes"):
         assert_bits(got[key], ref[key], key)
+    # every LDS stack depth of the wavefront walks gives the same bits
+    for cfg in [(16, 4, 4, 2), (20, 8, 16, 8), (20, 12, 48, 16), (16, 16, 16, 8)]:
+        lux_ctx.gi_set_trace_config(*cfg)
+        lux_ctx.gi_update()
+        lux_ctx.synchronize()
+        got = lux_ctx.gi_download()
+        for key in ("intervals", "merged", "probes"):
+            assert_bits(got[key], ref[key], (key, cfg))
     lux_ctx.gi_set_trace_config(**saved)
 
 

@@ -286,13 +286,13 @@ def test_gpu_launch_shapes_agree(lux_ctx, colonnade):
         lux_ctx.set_trace_tuning(wpc, rg)
         assert np.array_equal(trace_gpu(lux_ctx, rays), ref), (wpc, rg)
     # regrouping walks: every LDS depth, refill threshold and batch
-    for wpc, depth, rg, batch in [(16, 16, 16, 8), (16, 8, 1, 1), (8, 4, 64, 32), (20, 16, 32, 4)]:
+    for wpc, depth, rg, batch in [(16, 16, 16, 8), (16, 8, 1, 1), (8, 4, 64, 32), (20, 16, 32, 4), (20, 12, 8, 16)]:
         lux_ctx.set_trace_config(wpc, depth, rg, batch)
         assert np.array_equal(trace_gpu(lux_ctx, rays), ref), (wpc, depth, rg, batch)
     lux_ctx.set_trace_config(**saved)
 
 
-REGROUP_CONFIGS = [(0, 0, 0, 0), (16, 16, 16, 8), (16, 4, 1, 1)]
+REGROUP_CONFIGS = [(0, 0, 0, 0), (16, 16, 16, 8), (16, 4, 1, 1), (20, 12, 8, 16)]
 
 
 @pytest.mark.parametrize("cfg", REGROUP_CONFIGS)
