@@ -81,27 +81,36 @@ TRAFFIC_PROFILE = ROOT / "profiles" / "pmc_traffic_colonnade_trace_and_gi.md"
 
 
 def measured_traffic(kernel: str, cfg: dict, gi: bool = False):
-    """HBM-side bytes per launch of `kernel` from the committed counter profile:
-    its blocks each start with a `config` (traversal) and a `gi_config` (GI
+    """HBM-side bytes per launch of `kernel` from the committed counter profile,
+    and the launch configuration they were measured with.  The profile's
+    blocks each start with a `config` (traversal) and a `gi_config` (GI
     stages) line giving the launch configuration and BLAS builder they were
-    measured with (`blas` absent means Morton-30); the kernel's `traffic`
+    measured with (`blas` absent means Morton-30).  The kernel's `traffic`
     line is taken from the first block whose configuration of that kind (gi)
-    equals cfg.  None when no block matches."""
+    equals cfg, else from the first block with the same kernel and BLAS
+    builder (the same LDS stack depth: the other launch parameters move the
+    traffic by a few percent).  (None, None) when no block has the kernel."""
     if not TRAFFIC_PROFILE.exists():
-        return None
+        return None, None
     tag = "gi_config" if gi else "config"
-    match = False
+    exact = fallback = None
+    want = {}
     for l in TRAFFIC_PROFILE.read_text().splitlines():
         parts = l.split()
         if not parts:
             continue
         if parts[0] == tag:
             want = dict(kv.split("=") for kv in parts[1:])
-            match = all(str(want.get(k, "morton" if k == "blas" else -1)) == str(v) for k, v in cfg.items())
-        elif parts[0] == "traffic" and match and len(parts) == 4 and parts[1] == kernel:
+            want.setdefault("blas", "morton")
+        elif parts[0] == "traffic" and len(parts) == 4 and parts[1] == kernel and want:
             vals = dict(kv.split("=") for kv in parts[2:])
-            return int(vals["fetch_bytes"]) + int(vals["write_bytes"])
-    return None
+            got = (int(vals["fetch_bytes"]) + int(vals["write_bytes"]),
+                   {k: (v if k == "blas" else int(v)) for k, v in want.items()})
+            if exact is None and all(str(want.get(k)) == str(v) for k, v in cfg.items()):
+                exact = got
+            if fallback is None and str(want.get("blas")) == str(cfg.get("blas")):
+                fallback = got
+    return exact or fallback or (None, None)
 
 
 def gi_roofline(work: dict, intervals: int, probes: int, gp, update_ms: float, stage_ms: dict, cfg: dict,
@@ -130,10 +139,12 @@ def gi_roofline(work: dict, intervals: int, probes: int, gp, update_ms: float, s
     total = prim + occl + 32 * intervals + 544 * (intervals - top) + 32 * top + 1088 * probes
     achieved = prim / (stage_ms["primary"] * 1e-3) / 1e9
     kernel = "k_gi_primary<%d>" % cfg["lds_stack_depth"]
+    traffic, traffic_cfg = measured_traffic(kernel, dict(cfg, blas=args.blas), gi=True)
     return {
         "kernel": kernel, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": measured_traffic(kernel, dict(cfg, blas=args.blas), gi=True),
+        "traffic": traffic,
+        "traffic_config": traffic_cfg,
         "algorithmic_bytes": prim,
         "occlusion_walks": {"kernel": "k_gi_shadow<%d>" % cfg["lds_stack_depth"], "algorithmic_bytes": occl,
                             "achieved": round(occl / (stage_ms["occlusion"] * 1e-3) / 1e9, 2)},
@@ -948,6 +959,8 @@ def main() -> int:
     mrays = n / (ms_per_step * 1e-3) / 1e6
     achieved = algo_bytes / (avg_kernel_ms * 1e-3) / 1e9
 
+    traffic, traffic_cfg = measured_traffic("k_trace_regroup<%d>" % trace_cfg["lds_stack_depth"],
+                                            dict(trace_cfg, blas=args.blas))
     gi = None
     if not args.no_gi:
         gp = colonnade_gi_params(lux)
@@ -1032,8 +1045,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic("k_trace_regroup<%d>" % trace_cfg["lds_stack_depth"],
-                                        dict(trace_cfg, blas=args.blas)),
+            "traffic": traffic,
+            "traffic_config": traffic_cfg,
         },
         "cpu_baseline": cpu,
         "gi_update": gi,
