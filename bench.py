@@ -55,6 +55,9 @@ def parse():
                    help="held-out frames scored against the converged reference")
     p.add_argument("--no-dynamic", action="store_true")
     p.add_argument("--dynamic-frames", type=int, default=120)
+    p.add_argument("--dynamic-pretrain", type=int, default=400,
+                   help="frames of 262,144 traced samples the level-1 surrogate of the dynamic frame trains on "
+                        "before the timed frames")
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--batch-scenes", type=int, default=64)
     p.add_argument("--batch-train", type=int, default=400,
@@ -440,9 +443,12 @@ def dynamic_section(lux, scenes, args) -> dict:
     builder, built once).  One frame = new transforms + Morton-30 TLAS
     rebuild (or, in the refit run, a refit of the first frame's TLAS,
     LUX_BVH_REFIT), a 1920 x 1080 camera pass producing the receivers, the cascade
-    update (plain, or levels 0-1 traced with the surrogate answering level 2
-    and trained online on 65,536 traced samples per frame), and the world
-    resolve of all 2,073,600 receivers.  Wall time per frame (host waits for
+    update (plain; or levels 0-1 traced with the surrogate answering level 2,
+    trained online on 65,536 traced samples per frame; or level 0 traced with
+    a surrogate answering level 1, trained beforehand on --dynamic-pretrain
+    other frames of the animation and online as above; both scored every
+    10th frame against that frame's plain update, outside the timed frame),
+    and the world resolve of all 2,073,600 receivers.  Wall time per frame (host waits for
     the frame's last kernel), p50/p99 over the frames.  With the temporal
     history on (blends 0.88 / 0.98, persistent), the share of intervals whose
     history each frame reuses -- every instance moving, and only every 32nd
@@ -504,11 +510,30 @@ def dynamic_section(lux, scenes, args) -> dict:
     if res_tune:
         ctx8.gi_set_trace_config(**{k: res_tune[k] for k in ("waves_per_cu", "lds_stack_depth",
                                                                "regroup_threshold", "steps_per_refill")})
-    for mode in ("plain", "plain_refit", "surrogate", "history", "history_8_moving"):
-        times, tlas, reused = [], [], []
+    sur1 = lux.Surrogate(ctx, lux.SurrogateParams.default(origin, hi))
+    for mode in ("plain", "plain_refit", "surrogate", "surrogate_level1", "history", "history_8_moving"):
+        times, tlas, reused, errs, noise = [], [], [], [], []
         c = ctx8 if mode == "history_8_moving" else ctx
         if mode.startswith("history"):
             c.gi_set_history(0.88, 0.98, 24, 1)
+        sur_mode = {"surrogate": (sur, 2), "surrogate_level1": (sur1, 1)}.get(mode)
+        if mode == "surrogate_level1":
+            # trained beforehand on other frames of the same animation (seeds
+            # and frames 10000 + k), as by the frames a running application
+            # rendered before these; then online on each frame as it goes
+            big = DeviceBuffer.empty(ctx, (262144, 12), np.float32)
+            t_pre = time.perf_counter()
+            for k in range(args.dynamic_pretrain):
+                inv, fwd = scenes.dynamic_transforms(motion, 10000 + k)
+                ctx.set_transforms(inv, fwd)
+                ctx.build_bvh(bmode)
+                ctx.gi_set_frame(10000 + k, 10000 + k)
+                ctx.gi_surrogate_samples(1, 262144, 10000 + k, lux.LUX_SAMPLES_TRACE, big)
+                sur1.train_step(big, 262144, want_loss=False)
+            ctx.synchronize()
+            res["surrogate_level1_pretrain"] = {"frames": args.dynamic_pretrain, "samples_per_frame": 262144,
+                                                "s": round(time.perf_counter() - t_pre, 2)}
+            big.free()
         for f in range(-5, args.dynamic_frames):
             fr = max(f, 0)
             t0 = time.perf_counter()
@@ -525,12 +550,13 @@ def dynamic_section(lux, scenes, args) -> dict:
             te = time.perf_counter()
             c.camera_receivers(cam, W, H, recv)
             c.gi_set_frame(fr, fr)
-            if mode != "surrogate":
+            if sur_mode is None:
                 c.gi_update()
             else:
-                ctx.gi_update_surrogate(sur, 2, float("inf"), stats=False)
-                ctx.gi_surrogate_samples(2, ns, fr, lux.LUX_SAMPLES_TRACE, buf)
-                sur.train_step(buf, ns, want_loss=False)
+                s_net, s_level = sur_mode
+                ctx.gi_update_surrogate(s_net, s_level, float("inf"), stats=False)
+                ctx.gi_surrogate_samples(s_level, ns, fr, lux.LUX_SAMPLES_TRACE, buf)
+                s_net.train_step(buf, ns, want_loss=False)
             c.world_resolve_device(recv, W * H, out)
             c.synchronize()
             if f >= 0:
@@ -538,6 +564,18 @@ def dynamic_section(lux, scenes, args) -> dict:
                 tlas.append((te - tb) * 1e3)
                 if mode.startswith("history"):
                     reused.append(c.gi_history_reused() / n_int)
+                if sur_mode is not None and f % 10 == 0:
+                    # outside the timed frame: this frame's plain update as
+                    # the reference of the surrogate-assisted probes
+                    got = ctx.gi_download_probes()
+                    ctx.gi_update()
+                    ref = ctx.gi_download_probes()
+                    errs.append(probe_rmse(got, ref))
+                    # the plain update's own noise: another seed of the same
+                    # frame differs from it by about sqrt(2) of it
+                    ctx.gi_set_frame(fr, fr + 50000)
+                    ctx.gi_update()
+                    noise.append(probe_rmse(ctx.gi_download_probes(), ref) / np.sqrt(2.0))
         r = out.numpy()
         res[mode] = {"frames": len(times), "p50_ms": round(float(np.percentile(times, 50)), 3),
                      "p99_ms": round(float(np.percentile(times, 99)), 3),
@@ -547,7 +585,13 @@ def dynamic_section(lux, scenes, args) -> dict:
             res[mode]["history_reused_median"] = round(float(np.median(reused)), 4)
             res[mode]["history_reused_min"] = round(float(np.min(reused)), 4)
             c.gi_history_off()
+        if errs:
+            res[mode]["probe_rmse_vs_plain_median"] = round(float(np.median(errs)), 5)
+            res[mode]["probe_rmse_vs_plain_max"] = round(float(np.max(errs)), 5)
+            res[mode]["probe_rmse_frames"] = len(errs)
+            res[mode]["plain_noise_rmse_median"] = round(float(np.median(noise)), 5)
     sur.close()
+    sur1.close()
     ctx8.close()
     ctx.close()
     return res
