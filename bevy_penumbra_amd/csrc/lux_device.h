@@ -175,8 +175,10 @@ struct TriRay {
 
 __device__ __forceinline__ float comp(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
-// triangleRay (traversal.slang:399-413).
-__device__ __forceinline__ TriRay make_tri_ray(float3 d) {
+// triangleRay (traversal.slang:399-413).  With y = 1/d for a direction in
+// div_fast's range (every component nonzero), the shear constants divide by
+// the reciprocal of d[kz] (the same quotients).
+__device__ __forceinline__ TriRay make_tri_ray(float3 d, const float3* y = nullptr) {
     TriRay r;
     r.kx = r.ky = 0;
     r.sx = r.sy = 0.0f;
@@ -192,8 +194,14 @@ __device__ __forceinline__ TriRay make_tri_ray(float3 d) {
         r.kx = r.ky;
         r.ky = s;
     }
-    r.sx = comp(d, r.kx) / dz;
-    r.sy = comp(d, r.ky) / dz;
+    if (y) {
+        const float ydz = comp(*y, r.kz);
+        r.sx = div_fast(comp(d, r.kx), dz, ydz);
+        r.sy = div_fast(comp(d, r.ky), dz, ydz);
+    } else {
+        r.sx = comp(d, r.kx) / dz;
+        r.sy = comp(d, r.ky) / dz;
+    }
     return r;
 }
 
@@ -721,7 +729,14 @@ __device__ __forceinline__ void walk_begin(const DevScene& s, Walk& w, float3 o,
     w.node = load_node(s.tlas, s.n_tlas, 0);
     st.node(1);
     float re, rx;
-    bool root = box_interval(o, d, w.node.mn, w.node.mx, tmax, re, rx);
+    // box_entry_fast is box_interval's entry (or -1 on a miss) for in-range rays
+    bool root;
+    if (w.fast_t) {
+        re = box_entry_fast(o, d, w.yt, w.node, tmax);
+        root = re >= 0.0f;
+    } else {
+        root = box_interval(o, d, w.node.mn, w.node.mx, tmax, re, rx);
+    }
     if (mode == kWalkOccluded && root) w.h.err |= kHitRoot;
     if (!root) return;
     w.tready = re >= 0.0f;
@@ -830,11 +845,12 @@ __device__ __forceinline__ void walk_instance(const DevScene& s, Walk& w, St& st
     st.node(1);
     float e, x;
     float lim = w.mode == kWalkOccluded ? w.limit : w.h.t;
-    w.bready = box_interval(w.lo, w.ld, w.node.mn, w.node.mx, lim, e, x) && e >= 0.0f;
+    if (w.fast_b) w.bready = box_entry_fast(w.lo, w.ld, w.yb, w.node, lim) >= 0.0f;
+    else w.bready = box_interval(w.lo, w.ld, w.node.mn, w.node.mx, lim, e, x) && e >= 0.0f;
     w.ray.valid = false;
     w.ray.kx = w.ray.ky = w.ray.kz = 0;
     w.ray.sx = w.ray.sy = 0.0f;
-    if (w.bready) w.ray = make_tri_ray(w.ld);
+    if (w.bready) w.ray = make_tri_ray(w.ld, w.fast_b ? &w.yb : nullptr);
     w.bsteps = 0;
     w.binc = false;
     w.phase = kPhaseBlas;
