@@ -154,17 +154,20 @@ __device__ __forceinline__ float box_entry_fast(float3 o, float3 d, float3 y, co
     return near_ <= far_ ? near_ : -1.0f;
 }
 
-// orientedEdge (traversal.slang:379-392).
+// orientedEdge (traversal.slang:379-392).  The reference orders the two
+// points (the larger x, then the larger y, second), takes the edge of the
+// ordered pair and negates it when it swapped them.  The swapped products
+// are the same products (RN(b.x a.y) = RN(a.y b.x)), and RN(-v) = -RN(v), so
+// the negated edge of the swapped pair is a.x b.y - a.y b.x bit for bit and
+// its zeroing test is the same; only a zeroed edge keeps the swap as its
+// sign, -0.
 __device__ __forceinline__ float oriented_edge(float ax, float ay, float bx, float by) {
-    bool reverse = ax > bx || (ax == bx && ay > by);
-    float fx = reverse ? bx : ax, fy = reverse ? by : ay;
-    float sx = reverse ? ax : bx, sy = reverse ? ay : by;
-    float pa = fx * sy;
-    float pb = fy * sx;
-    float edge = pa - pb;
-    float err = (fabsf(pa) + fabsf(pb)) * 2.3841858e-7f;
-    if (fabsf(edge) <= err) edge = 0.0f;
-    return reverse ? -edge : edge;
+    const bool reverse = ax > bx || (ax == bx && ay > by);
+    const float pa = ax * by;
+    const float pb = ay * bx;
+    const float edge = pa - pb;
+    const float err = (fabsf(pa) + fabsf(pb)) * 2.3841858e-7f;
+    return fabsf(edge) <= err ? (reverse ? -0.0f : 0.0f) : edge;
 }
 
 struct TriRay {

@@ -68,3 +68,47 @@ def test_triangle_miss_and_degenerate():
     assert t == 5.0
     # barycentric of the ray point (0,0): analytic weights for b and c
     assert abs(w1 - 0.25) < 1e-7 and abs(w2 - 0.5) < 1e-7
+
+
+def _edge_reference(ax, ay, bx, by):
+    """orientedEdge as the reference writes it (traversal.slang:379-392), in
+    float32: order the points, edge of the ordered pair, negate if swapped."""
+    f32 = np.float32
+    reverse = (ax > bx) | ((ax == bx) & (ay > by))
+    fx, fy = np.where(reverse, bx, ax), np.where(reverse, by, ay)
+    sx, sy = np.where(reverse, ax, bx), np.where(reverse, ay, by)
+    pa, pb = fx * sy, fy * sx
+    edge = pa - pb
+    err = (np.abs(pa) + np.abs(pb)) * f32(2.3841858e-7)
+    edge = np.where(np.abs(edge) <= err, f32(0.0), edge)
+    return np.where(reverse, -edge, edge).astype(f32)
+
+
+def _edge_unordered(ax, ay, bx, by):
+    """The kernels' form: a.x b.y - a.y b.x, a zeroed edge signed by the swap."""
+    f32 = np.float32
+    reverse = (ax > bx) | ((ax == bx) & (ay > by))
+    pa, pb = ax * by, ay * bx
+    edge = pa - pb
+    err = (np.abs(pa) + np.abs(pb)) * f32(2.3841858e-7)
+    return np.where(np.abs(edge) <= err, np.where(reverse, f32(-0.0), f32(0.0)), edge).astype(f32)
+
+
+def test_oriented_edge_needs_no_point_ordering():
+    """The swapped products are the same products and rounding is symmetric,
+    so the kernels' unordered edge equals the reference's ordered one bit for
+    bit, zero signs included (the barycentric outputs divide by them)."""
+    rng = np.random.default_rng(11)
+    n = 1 << 20
+    v = (rng.uniform(-2, 2, (4, n)) * np.exp2(rng.integers(-40, 1, (4, n)))).astype(np.float32)
+    # ties on x, equal points, zeros and collinear pairs
+    v[2, : n // 8] = v[0, : n // 8]
+    v[3, : n // 16] = v[1, : n // 16]
+    v[:, n // 8: n // 8 + 4096] = 0.0
+    k = np.float32(rng.uniform(-3, 3))
+    v[2, n // 4: n // 2], v[3, n // 4: n // 2] = v[0, n // 4: n // 2] * k, v[1, n // 4: n // 2] * k
+    with np.errstate(all="ignore"):
+        a = _edge_reference(*v)
+        b = _edge_unordered(*v)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert (a == 0).sum() > n // 8 and (np.signbit(a) & (a == 0)).sum() > 0
