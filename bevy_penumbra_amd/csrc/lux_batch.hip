@@ -135,7 +135,7 @@ template <int Depth>
 __global__ __launch_bounds__(kWg) void k_b_primary(BatchWave w) {
     __shared__ uint2 lds[Depth][kWg];
     lux::LaneStack<Depth, kWg> stk;
-    stk.col = &lds[0][threadIdx.x];
+    stk.bind(&lds[0][threadIdx.x]);
     lux::NoStats st;
     BatchPrimary src{&w, (uint32_t)w.item_base[w.n_scenes]};
     lux::regroup_walks_multi(src, &w.counters[0], w.regroup, w.batch, stk, st);
@@ -225,7 +225,7 @@ template <int Depth>
 __global__ __launch_bounds__(kWg) void k_b_shadow(BatchWave w) {
     __shared__ uint2 lds[Depth][kWg];
     lux::LaneStack<Depth, kWg> stk;
-    stk.col = &lds[0][threadIdx.x];
+    stk.bind(&lds[0][threadIdx.x]);
     lux::NoStats st;
     BatchShadow src{&w, w.counters[1]};
     lux::regroup_walks_multi(src, &w.counters[2], w.regroup, w.batch, stk, st);

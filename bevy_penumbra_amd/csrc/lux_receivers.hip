@@ -666,7 +666,7 @@ __global__ __launch_bounds__(kWg) void k_rx_trace(GiArgs a, RxGrids grids, const
                                                   float4* dval, uint32_t* dcount, uint32_t* counters) {
     __shared__ uint2 lds[kLdsDepth][kWg];
     Stack stk;
-    stk.col = &lds[0][threadIdx.x];
+    stk.bind(&lds[0][threadIdx.x]);
     const uint32_t i = blockIdx.x * kWg + threadIdx.x;
     bool traced = false, unsupported = false;
     uint32_t kept = 0, outside = 0;

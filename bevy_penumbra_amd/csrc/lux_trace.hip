@@ -46,7 +46,7 @@ __device__ __forceinline__ void write_hit(const TraceArgs& a, uint32_t idx, cons
 
 template <class St>
 __device__ __forceinline__ void trace_one(const TraceArgs& a, uint32_t idx,
-                                          lux::LaneStack<kLdsDepth, kTraceWg>& stk, St& st) {
+                                          lux::LaneStack<kLdsDepth, kTraceWg, lux::FlatWord>& stk, St& st) {
     const float4* rp = reinterpret_cast<const float4*>(a.rays) + 2u * idx;
     float4 r0 = rp[0], r1 = rp[1];
     float3 o = make_float3(r0.x, r0.y, r0.z);
@@ -67,8 +67,8 @@ __device__ __forceinline__ void trace_one(const TraceArgs& a, uint32_t idx,
 // One ray per lane, rays fetched in wave-sized batches from a cursor.
 __global__ __launch_bounds__(kTraceWg) void k_trace_persistent(TraceArgs a) {
     __shared__ uint2 lds[kLdsDepth][kTraceWg];
-    lux::LaneStack<kLdsDepth, kTraceWg> stk;
-    stk.col = &lds[0][threadIdx.x];
+    lux::LaneStack<kLdsDepth, kTraceWg, lux::FlatWord> stk;
+    stk.bind(&lds[0][threadIdx.x]);
     const uint32_t lane = threadIdx.x & 63u;
     while (true) {
         uint32_t base = 0;
@@ -105,8 +105,8 @@ struct RaySource {
 template <int Depth>
 __global__ __launch_bounds__(kTraceWg) void k_trace_regroup(TraceArgs a) {
     __shared__ uint2 lds[Depth][kTraceWg];
-    lux::LaneStack<Depth, kTraceWg> stk;
-    stk.col = &lds[0][threadIdx.x];
+    lux::LaneStack<Depth, kTraceWg, lux::FlatWord> stk;
+    stk.bind(&lds[0][threadIdx.x]);
     lux::NoStats st;
     RaySource src{&a};
     lux::regroup_walks(a.scene, src, a.cursor, a.regroup, a.batch, stk, st);
@@ -115,8 +115,8 @@ __global__ __launch_bounds__(kTraceWg) void k_trace_regroup(TraceArgs a) {
 // Plain grid: one ray per thread.
 __global__ __launch_bounds__(kTraceWg) void k_trace_grid(TraceArgs a) {
     __shared__ uint2 lds[kLdsDepth][kTraceWg];
-    lux::LaneStack<kLdsDepth, kTraceWg> stk;
-    stk.col = &lds[0][threadIdx.x];
+    lux::LaneStack<kLdsDepth, kTraceWg, lux::FlatWord> stk;
+    stk.bind(&lds[0][threadIdx.x]);
     uint32_t idx = blockIdx.x * kTraceWg + threadIdx.x;
     lux::NoStats st;
     if (idx < a.n) trace_one(a, idx, stk, st);
@@ -125,8 +125,8 @@ __global__ __launch_bounds__(kTraceWg) void k_trace_grid(TraceArgs a) {
 // Instrumented grid: tallies traversal work per ray (outside timed runs).
 __global__ __launch_bounds__(kTraceWg) void k_trace_stats(TraceArgs a, unsigned long long* counters) {
     __shared__ uint2 lds[kLdsDepth][kTraceWg];
-    lux::LaneStack<kLdsDepth, kTraceWg> stk;
-    stk.col = &lds[0][threadIdx.x];
+    lux::LaneStack<kLdsDepth, kTraceWg, lux::FlatWord> stk;
+    stk.bind(&lds[0][threadIdx.x]);
     uint32_t idx = blockIdx.x * kTraceWg + threadIdx.x;
     lux::Counting st;
     if (idx < a.n) trace_one(a, idx, stk, st);

@@ -31,7 +31,7 @@ namespace {
 __global__ __launch_bounds__(kWg) void k_gi_trace(GiArgs a, uint64_t g_begin, uint64_t g_end) {
     __shared__ uint2 lds[kLdsDepth][kWg];
     Stack stk;
-    stk.col = &lds[0][threadIdx.x];
+    stk.bind(&lds[0][threadIdx.x]);
     uint64_t g = g_begin + (uint64_t)blockIdx.x * kWg + threadIdx.x;
     const lux_gi_params& p = a.p;
     if (g >= g_end) return;
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kWg) void k_sur_samples(GiArgs a, uint32_t L, uint3
                                                      uint32_t source, float* out) {
     __shared__ uint2 lds[kLdsDepth][kWg];
     Stack stk;
-    stk.col = &lds[0][threadIdx.x];
+    stk.bind(&lds[0][threadIdx.x]);
     uint32_t i = blockIdx.x * kWg + threadIdx.x;
     if (i >= n) return;
     const lux_gi_params& p = a.p;
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kWg) void k_sur_fallback(GiArgs a, uint32_t L, cons
                                                       const uint32_t* count) {
     __shared__ uint2 lds[kLdsDepth][kWg];
     Stack stk;
-    stk.col = &lds[0][threadIdx.x];
+    stk.bind(&lds[0][threadIdx.x]);
     const lux_gi_params& p = a.p;
     uint32_t D = (p.base_phi << L) * (p.base_y << L);
     uint64_t g = (uint64_t)blockIdx.x * kWg + threadIdx.x;
@@ -273,7 +273,7 @@ template <int Depth, class St = lux::NoStats>
 __global__ __launch_bounds__(kWg) void k_gi_primary(GiArgs a, Wave w) {
     __shared__ uint2 lds[Depth][kWg];
     lux::LaneStack<Depth, kWg> stk;
-    stk.col = &lds[0][threadIdx.x];
+    stk.bind(&lds[0][threadIdx.x]);
     St st;
     PrimarySource src{&a, &w, wave_intervals(a, w) * w.R, 0u};
     lux::regroup_walks(a.scene, src, &w.counters[0], w.regroup, w.batch, stk, st);
@@ -359,7 +359,7 @@ template <int Depth, class St = lux::NoStats>
 __global__ __launch_bounds__(kWg) void k_gi_shadow(GiArgs a, Wave w) {
     __shared__ uint2 lds[Depth][kWg];
     lux::LaneStack<Depth, kWg> stk;
-    stk.col = &lds[0][threadIdx.x];
+    stk.bind(&lds[0][threadIdx.x]);
     St st;
     ShadowSource src{&a, &w, w.counters[1], 0u};
     lux::regroup_walks(a.scene, src, &w.counters[2], w.regroup, w.batch, stk, st);
@@ -747,7 +747,7 @@ __global__ __launch_bounds__(kWg) void k_camera_receivers(lux::DevScene scene, C
                                                           uint32_t h, float* out) {
     __shared__ uint2 lds[kLdsDepth][kWg];
     Stack stk;
-    stk.col = &lds[0][threadIdx.x];
+    stk.bind(&lds[0][threadIdx.x]);
     uint32_t i = blockIdx.x * kWg + threadIdx.x;
     if (i >= w * h) return;
     uint32_t x = i % w, y = i / w;
@@ -1165,7 +1165,7 @@ extern "C" int lux_autotune_gi(lux_ctx* ctx, uint32_t repeats, lux_trace_config*
     static const uint32_t cand_batch[] = {4, 8, 16};
     const uint32_t* cands[4] = {cand_wpc, cand_depth, cand_regroup, cand_batch};
     const uint32_t ncand[4] = {4, 4, 5, 3};
-    uint32_t cur[4] = {20, 12, 16, 8};
+    uint32_t cur[4] = {20, 12, 16, 16};
     float cur_ms = 0.0f;
     uint32_t evals = 1;
     int rc = measure(cur, cur_ms);
