@@ -27,6 +27,9 @@
 namespace {
 
 constexpr uint32_t kMaxBatchLevels = 8;
+// Scenes whose unbounded admission counts a walk workgroup tallies in LDS.
+constexpr uint32_t kTallyScenes = 256;
+typedef __attribute__((address_space(3))) uint32_t LdsU32;
 
 
 struct BatchScene {
@@ -70,8 +73,12 @@ __device__ __forceinline__ uint32_t find_scene(const uint64_t* base, uint32_t n,
 
 // admit_query for lanes of possibly different scenes: the active lanes are
 // served in groups of one scene each (one atomic per group on that scene's
-// counters), with admitTransportQuery's semantics per scene.
-__device__ bool admit_query_scene(const BatchWave& w, uint32_t s) {
+// counters), with admitTransportQuery's semantics per scene.  A scene with
+// an unbounded budget admits every query and only counts it: its groups add
+// to the workgroup's LDS tally instead (scenes below kTallyScenes), which
+// tally_flush adds to the scene's counters once, with the same totals.  The
+// global atomics would make every refill of a wave wait for their return.
+__device__ bool admit_query_scene(const BatchWave& w, uint32_t s, LdsU32* tally) {
     const GiArgs& a = w.sc[s].a;
     const uint32_t lane = __lane_id();
     const uint64_t below = (1ull << lane) - 1ull;
@@ -86,6 +93,12 @@ __device__ bool admit_query_scene(const BatchWave& w, uint32_t s) {
             uint64_t group = __ballot(1);
             uint32_t count = (uint32_t)__popcll(group);
             uint32_t rank = (uint32_t)__popcll(group & below);
+            if (a.admit_limit == 0u && s < kTallyScenes) {
+                if (lane == leader)
+                    __hip_atomic_fetch_add(&tally[s], count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                done = true;
+                continue;
+            }
             uint32_t base = 0;
             if (lane == leader) {
                 atomicAdd(&a.counters[0], count);
@@ -103,9 +116,28 @@ __device__ bool admit_query_scene(const BatchWave& w, uint32_t s) {
     return ok;
 }
 
+__device__ __forceinline__ void tally_clear(uint32_t* tally) {
+    for (uint32_t k = threadIdx.x; k < kTallyScenes; k += kWg) tally[k] = 0u;
+    __syncthreads();
+}
+
+// Called by every thread of the workgroup once its walks are done.
+__device__ __forceinline__ void tally_flush(const BatchWave& w, const uint32_t* tally) {
+    __syncthreads();
+    const uint32_t n = w.n_scenes < kTallyScenes ? w.n_scenes : kTallyScenes;
+    for (uint32_t k = threadIdx.x; k < n; k += kWg) {
+        const uint32_t v = tally[k];
+        if (v) {
+            atomicAdd(&w.sc[k].a.counters[0], v);
+            atomicAdd(&w.sc[k].a.counters[1], v);
+        }
+    }
+}
+
 struct BatchPrimary {
     const BatchWave* w;
     uint32_t n;
+    LdsU32* tally;
     __device__ uint32_t count() const { return n; }
     template <class St>
     __device__ bool begin(uint32_t i, lux::Walk& wk, St& st) {
@@ -116,7 +148,7 @@ struct BatchPrimary {
         float lim;
         bool term;
         item_ray(S.a, local / S.R, (uint32_t)(local % S.R), false, o, d, lim, term);
-        if (!admit_query_scene(*w, s)) {
+        if (!admit_query_scene(*w, s, tally)) {
             w->prim[i] = make_uint4(0u, lux::kEmpty, lux::kEmpty, lux::kErrUnsupported);
             return false;
         }
@@ -137,8 +169,11 @@ __global__ __launch_bounds__(kWg) void k_b_primary(BatchWave w) {
     lux::LaneStack<Depth, kWg> stk;
     stk.bind(&lds[0][threadIdx.x]);
     lux::NoStats st;
-    BatchPrimary src{&w, (uint32_t)w.item_base[w.n_scenes]};
+    __shared__ uint32_t tally[kTallyScenes];
+    tally_clear(tally);
+    BatchPrimary src{&w, (uint32_t)w.item_base[w.n_scenes], (LdsU32*)tally};
     lux::regroup_walks_multi(src, &w.counters[0], w.regroup, w.batch, stk, st);
+    tally_flush(w, tally);
 }
 
 __global__ __launch_bounds__(kWg) void k_b_setup(BatchWave w) {
@@ -197,13 +232,14 @@ __global__ __launch_bounds__(kWg) void k_b_setup(BatchWave w) {
 struct BatchShadow {
     const BatchWave* w;
     uint32_t n;
+    LdsU32* tally;
     __device__ uint32_t count() const { return n; }
     template <class St>
     __device__ bool begin(uint32_t i, lux::Walk& wk, St& st) {
         float4 r0 = w->shadow[2ull * i], r1 = w->shadow[2ull * i + 1];
         uint32_t slot = __float_as_uint(r1.w);
         uint32_t s = find_scene(w->item_base, w->n_scenes, slot % w->stride);  // the slot's item
-        if (!admit_query_scene(*w, s)) {
+        if (!admit_query_scene(*w, s, tally)) {
             w->vis[slot] = 2u;
             return false;
         }
@@ -227,8 +263,11 @@ __global__ __launch_bounds__(kWg) void k_b_shadow(BatchWave w) {
     lux::LaneStack<Depth, kWg> stk;
     stk.bind(&lds[0][threadIdx.x]);
     lux::NoStats st;
-    BatchShadow src{&w, w.counters[1]};
+    __shared__ uint32_t tally[kTallyScenes];
+    tally_clear(tally);
+    BatchShadow src{&w, w.counters[1], (LdsU32*)tally};
     lux::regroup_walks_multi(src, &w.counters[2], w.regroup, w.batch, stk, st);
+    tally_flush(w, tally);
 }
 
 __global__ __launch_bounds__(kWg) void k_b_accum(BatchWave w) {

@@ -48,6 +48,15 @@ constexpr uint32_t kRxLodMask = 31u << 18;
 constexpr uint32_t kRxMaxLods = 8;
 constexpr uint32_t kScanBlock = 1024;
 
+// Descriptors (variant, receiver, level) are stored level-major: descriptor
+// j = variant * count + receiver at level L is d = L * m + j (m = 2 * count),
+// so the seed's stores and the directory insert's loads of one level are
+// contiguous across a wave.  Canonical indices compare the same way as in
+// the receiver-major order of the reference's dispatch within one level
+// (lower j, lower d).  lux_receivers_download returns the receiver-major
+// order, (j * levels + L).
+__device__ __forceinline__ uint64_t rx_desc(uint64_t j, uint32_t L, uint64_t m) { return (uint64_t)L * m + j; }
+
 // counters: [0] eligible receivers, [1] tickets, [2] rays, [3] unsupported,
 // [4] deposits kept, [5] deposits outside, [6] root total, [7] overflow
 __device__ __forceinline__ void wave_count(uint32_t* c, bool pred) {
@@ -142,11 +151,17 @@ __device__ __forceinline__ int4 rx_cell(float3 p, float base, uint32_t level, ui
 
 __global__ __launch_bounds__(kWg) void k_rx_seed(RxSeed s, const float* recv, int4* keys, uint32_t* flags,
                                                  uint32_t* lods, uint32_t* counters) {
-    const uint32_t i = blockIdx.x * kWg + threadIdx.x;
-    bool ok = false;
-    if (i < s.count) {
+    // grid-stride over the receivers; the eligible count goes to the global
+    // counter once per block (one counter word taking an atomic from every
+    // wave of a 2M-receiver grid costs more than the kernel's stores)
+    __shared__ uint32_t block_eligible;
+    if (threadIdx.x == 0) block_eligible = 0u;
+    __syncthreads();
+    uint32_t eligible = 0;
+    for (uint32_t i = blockIdx.x * kWg + threadIdx.x; i < s.count; i += gridDim.x * kWg) {
         float3 pos, n;
-        ok = rx_surface(recv + 6ull * i, pos, n);
+        const bool ok = rx_surface(recv + 6ull * i, pos, n);
+        eligible += ok ? 1u : 0u;
         uint32_t low = 0, high = 0;
         float blend = 0.0f;
         float pp[3] = {pos.x, pos.y, pos.z};
@@ -162,7 +177,7 @@ __global__ __launch_bounds__(kWg) void k_rx_seed(RxSeed s, const float* recv, in
         for (uint32_t v = 0; v < 2; ++v) {
             lods[(uint64_t)v * s.count + i] = v ? high : low;
             for (uint32_t L = 0; L < s.levels; ++L) {
-                uint64_t d = ((uint64_t)v * s.count + i) * s.levels + L;
+                const uint64_t d = rx_desc((uint64_t)v * s.count + i, L, 2ull * s.count);
                 if (!ok || (v == 1 && high == low)) {
                     keys[d] = make_int4(0, 0, 0, 0);
                     flags[d] = 0;
@@ -173,7 +188,9 @@ __global__ __launch_bounds__(kWg) void k_rx_seed(RxSeed s, const float* recv, in
             }
         }
     }
-    wave_count(&counters[0], ok);
+    if (eligible) atomicAdd(&block_eligible, eligible);
+    __syncthreads();
+    if (threadIdx.x == 0 && block_eligible) atomicAdd(&counters[0], block_eligible);
 }
 
 // ---------------- probe directory ----------------
@@ -212,6 +229,7 @@ struct RxDir {
     uint32_t* count;
     uint32_t* offset;
     uint32_t cap, levels;
+    uint32_t m;  // descriptors per level, 2 * count
 };
 
 __global__ __launch_bounds__(kWg) void k_rx_dir_clear(RxDir D) {
@@ -260,7 +278,7 @@ __global__ __launch_bounds__(kWg) void k_rx_insert(RxDir D, const int4* keys, co
     int4 k = make_int4(0, 0, 0, 0);
     if (t < (uint64_t)m * D.levels) {
         L = (uint32_t)(t / m);
-        d = (uint32_t)((t % m) * D.levels + L);
+        d = (uint32_t)t;  // rx_desc(t % m, L, m)
         valid = (flags[d] & kRxValid) != 0u;
         if (valid) k = keys[d];
         else slot_out[d] = kRxEmpty;
@@ -290,7 +308,7 @@ __global__ __launch_bounds__(kWg) void k_rx_ray_keys(uint32_t* flags, const uint
                                                      uint32_t levels, uint32_t none, uint32_t cold, uint32_t* out) {
     const uint32_t j = blockIdx.x * kWg + threadIdx.x;
     if (j >= m) return;
-    const uint64_t d0 = (uint64_t)j * levels;
+    const uint64_t d0 = rx_desc(j, 0u, m);
     uint32_t f = flags[d0];
     const uint32_t b = slot[d0];
     if (cold && (f & kRxRay)) {
@@ -351,7 +369,7 @@ __global__ __launch_bounds__(kWg) void k_rx_cold_nominate(RxSeed sd, const uint3
     bool e = false;
     uint32_t b = kRxEmpty, pr = 0u;
     if (j < m) {
-        const uint64_t d0 = (uint64_t)j * sd.levels;
+        const uint64_t d0 = rx_desc(j, 0u, m);
         b = slot[d0];
         e = (flags[d0] & kRxRay) && b != kRxEmpty;
         if (e) pr = 0xffffffffu - rank_of(j % sd.count, sd.width, sd.height, sd.count, sd.frame * 0x9e3779b9u);
@@ -417,7 +435,7 @@ __global__ __launch_bounds__(kWg) void k_rx_cold_order(RxSeed sd, RxDir D, const
     const uint32_t boundary = cw.table[64];
     uint32_t f = 0u;
     if (boundary < 32u) {
-        const uint64_t d0 = (uint64_t)rank_mapped(sd, p) * sd.levels;
+        const uint64_t d0 = rx_desc(rank_mapped(sd, p), 0u, m);
         const uint32_t b = slot[d0];
         if (b != kRxEmpty && D.canon[b] == (uint32_t)d0 && cw.pop[b] != 0u && cold_bin(cw.pop[b]) == boundary) f = 1u;
     }
@@ -431,13 +449,14 @@ __global__ __launch_bounds__(kWg) void k_rx_cold_admit(RxSeed sd, RxDir D, const
     const uint32_t b = blockIdx.x * kWg + threadIdx.x;
     if (b >= D.cap || cw.prio[b] == 0u) return;
     const uint32_t ns = sample_of(0xffffffffu - cw.prio[b], sd.width, sd.height, sd.count, sd.frame * 0x9e3779b9u);
-    uint64_t nom = (uint64_t)ns * sd.levels;
-    if (slot[nom] != b) nom = ((uint64_t)sd.count + ns) * sd.levels;
+    const uint64_t m = 2ull * sd.count;
+    uint64_t nom = rx_desc(ns, 0u, m);
+    if (slot[nom] != b) nom = rx_desc((uint64_t)sd.count + ns, 0u, m);
     const uint32_t bin = cold_bin(cw.pop[b]);
     const uint32_t q = cw.table[32 + bin];
     bool admitted = q > 0u;
     if (bin == cw.table[64]) {
-        const uint32_t jc = D.canon[b] / sd.levels;
+        const uint32_t jc = D.canon[b];  // a level-0 descriptor: rx_desc(jc, 0, m) = jc
         const uint32_t sc = jc % sd.count, vc = jc / sd.count;
         admitted = cw.pos[rank_of(sc, sd.width, sd.height, sd.count, 0u) * 2u + vc] < q;
     }
@@ -456,7 +475,8 @@ struct RxWindow {
 __device__ __forceinline__ uint64_t window_descriptor(const RxSeed& sd, const RxWindow& w, uint32_t t) {
     const uint32_t i = t >> 1;
     const uint32_t r = i < w.wrapped ? i : w.start + (i - w.wrapped);
-    return ((uint64_t)(t & 1u) * sd.count + sample_of(r, sd.width, sd.height, sd.count, 0u)) * sd.levels;
+    return rx_desc((uint64_t)(t & 1u) * sd.count + sample_of(r, sd.width, sd.height, sd.count, 0u), 0u,
+                   2ull * sd.count);
 }
 
 __device__ __forceinline__ bool cold_fill_candidate(uint32_t f, uint32_t b) {
@@ -506,7 +526,7 @@ __global__ __launch_bounds__(kWg) void k_rx_root_scatter(RxDir D, uint32_t* root
     if (b >= D.cap) return;
     const uint64_t top = (uint64_t)(D.levels - 1u) * D.cap + b;
     if (D.owner[top] == kRxEmpty) return;
-    root_arr[D.canon[top] / D.levels] = D.count[top];
+    root_arr[D.canon[top] - (uint64_t)(D.levels - 1u) * D.m] = D.count[top];
 }
 
 __global__ __launch_bounds__(kWg) void k_rx_root_offsets(RxDir D, const uint32_t* root_scan) {
@@ -514,7 +534,7 @@ __global__ __launch_bounds__(kWg) void k_rx_root_offsets(RxDir D, const uint32_t
     if (b >= D.cap) return;
     const uint64_t top = (uint64_t)(D.levels - 1u) * D.cap + b;
     if (D.owner[top] == kRxEmpty) return;
-    D.offset[top] = root_scan[D.canon[top] / D.levels];
+    D.offset[top] = root_scan[D.canon[top] - (uint64_t)(D.levels - 1u) * D.m];
 }
 
 // prefix (split.slang:924-961) below the roots.
@@ -550,7 +570,7 @@ __global__ __launch_bounds__(kWg) void k_rx_generate(const uint32_t* flags, cons
     uint32_t rays = 0;
     for (uint32_t v = 0; v < 2; ++v) {
         const uint64_t j = (uint64_t)v * count + s;
-        const uint64_t d0 = j * levels;
+        const uint64_t d0 = rx_desc(j, 0u, 2ull * count);
         uint32_t q = kRxEmpty;
         if ((flags[d0] & kRxRay) && slot[d0] != kRxEmpty) {
             q = ord[j] + offset0[slot[d0]];
@@ -825,17 +845,28 @@ __global__ __launch_bounds__(kWg) void k_rx_reduce_long(const uint32_t* sk, cons
     }
 }
 
-// Descriptor view for downloads: canonical, count and offset of each
-// descriptor's probe.
-__global__ __launch_bounds__(kWg) void k_rx_describe(RxDir D, const uint32_t* slot, uint64_t nd, uint32_t* canon,
-                                                     uint32_t* cnt, uint32_t* off) {
-    const uint64_t d = (uint64_t)blockIdx.x * kWg + threadIdx.x;
-    if (d >= nd) return;
+// Descriptor view for downloads, in the receiver-major order p = j * levels
+// + L: key, flags, and the canonical descriptor (in the same numbering),
+// count and offset of each descriptor's probe.
+__global__ __launch_bounds__(kWg) void k_rx_describe(RxDir D, const int4* keys, const uint32_t* flags,
+                                                     const uint32_t* slot, uint64_t nd, int4* key_out,
+                                                     uint32_t* flag_out, uint32_t* canon, uint32_t* cnt,
+                                                     uint32_t* off) {
+    const uint64_t p = (uint64_t)blockIdx.x * kWg + threadIdx.x;
+    if (p >= nd) return;
+    const uint64_t j = p / D.levels;
+    const uint32_t L = (uint32_t)(p % D.levels);
+    const uint64_t d = rx_desc(j, L, D.m);
+    key_out[p] = keys[d];
+    flag_out[p] = flags[d];
     const uint32_t b = slot[d];
-    const uint64_t L = d % D.levels;
-    canon[d] = b == kRxEmpty ? kRxEmpty : D.canon[L * D.cap + b];
-    cnt[d] = b == kRxEmpty ? 0u : D.count[L * D.cap + b];
-    off[d] = b == kRxEmpty ? kRxEmpty : D.offset[L * D.cap + b];
+    const uint64_t e = (uint64_t)L * D.cap + b;
+    uint32_t c = kRxEmpty;
+    if (b != kRxEmpty && D.canon[e] != kRxEmpty)
+        c = (uint32_t)((D.canon[e] - (uint64_t)L * D.m) * D.levels + L);
+    canon[p] = c;
+    cnt[p] = b == kRxEmpty ? 0u : D.count[e];
+    off[p] = b == kRxEmpty ? kRxEmpty : D.offset[e];
 }
 
 uint32_t directory_capacity(uint32_t count) {
@@ -856,6 +887,7 @@ RxDir dir_of(lux_ctx* c) {
     D.offset = w + 3 * lc;
     D.cap = c->rx_cap;
     D.levels = c->rx_levels;
+    D.m = 2u * c->rx_count;
     return D;
 }
 
@@ -1006,8 +1038,8 @@ extern "C" int lux_gi_update_receivers(lux_ctx* const* lods, uint32_t lod_count,
     sd.cold = cold ? 1u : 0u;
     sd.spacing = c->gi.spacing;
     if (view) sd.view = *view;
-    hipLaunchKernelGGL(k_rx_seed, dim3(blocks_for(count)), dim3(kWg), 0, st, sd, receivers, keys, flags, lodv,
-                       counters);
+    hipLaunchKernelGGL(k_rx_seed, dim3(std::min<uint32_t>(blocks_for(count), 2048u)), dim3(kWg), 0, st, sd, receivers,
+                       keys, flags, lodv, counters);
     // rank: directory, canonical descriptors, ordinals and level-0 counts
     RxDir D = dir_of(c);
     const uint32_t clear_blocks = std::min<uint32_t>(blocks_for((uint64_t)levels * cap), 4096u);
@@ -1181,19 +1213,23 @@ extern "C" int lux_receivers_download(lux_ctx* ctx, int32_t* keys, uint32_t* fla
     const uint64_t m = 2ull * ctx->rx_count;
     LUX_HIP(ctx, hipSetDevice(ctx->device));
     LUX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (keys) LUX_HIP(ctx, hipMemcpy(keys, ctx->rx_keys.ptr, 16ull * nd, hipMemcpyDeviceToHost));
-    if (flags) LUX_HIP(ctx, hipMemcpy(flags, ctx->rx_flags.ptr, 4ull * nd, hipMemcpyDeviceToHost));
-    if (canonical || probe_count || probe_offset) {
+    if (keys || flags || canonical || probe_count || probe_offset) {
+        // the device holds descriptors level-major; the view is receiver-major
         lux_buffer tmp;
-        if (int rc = lux_ensure(ctx, tmp, 12ull * nd)) return rc;
-        uint32_t* t = static_cast<uint32_t*>(tmp.ptr);
+        if (int rc = lux_ensure(ctx, tmp, 32ull * nd)) return rc;
+        int4* tk = static_cast<int4*>(tmp.ptr);
+        uint32_t* t = reinterpret_cast<uint32_t*>(tk + nd);
         hipLaunchKernelGGL(k_rx_describe, dim3(blocks_for(nd)), dim3(kWg), 0, ctx->stream, dir_of(ctx),
-                           static_cast<const uint32_t*>(ctx->rx_slot.ptr), nd, t, t + nd, t + 2 * nd);
+                           static_cast<const int4*>(ctx->rx_keys.ptr), static_cast<const uint32_t*>(ctx->rx_flags.ptr),
+                           static_cast<const uint32_t*>(ctx->rx_slot.ptr), nd, tk, t, t + nd, t + 2 * nd, t + 3 * nd);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e == hipSuccess && canonical) e = hipMemcpy(canonical, t, 4ull * nd, hipMemcpyDeviceToHost);
-        if (e == hipSuccess && probe_count) e = hipMemcpy(probe_count, t + nd, 4ull * nd, hipMemcpyDeviceToHost);
-        if (e == hipSuccess && probe_offset) e = hipMemcpy(probe_offset, t + 2 * nd, 4ull * nd, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && keys) e = hipMemcpy(keys, tk, 16ull * nd, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && flags) e = hipMemcpy(flags, t, 4ull * nd, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && canonical) e = hipMemcpy(canonical, t + nd, 4ull * nd, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && probe_count) e = hipMemcpy(probe_count, t + 2 * nd, 4ull * nd, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && probe_offset)
+            e = hipMemcpy(probe_offset, t + 3 * nd, 4ull * nd, hipMemcpyDeviceToHost);
         lux_release(tmp);
         if (e != hipSuccess) return lux_hip_check(ctx, e, "lux_receivers_download");
     }
