@@ -164,7 +164,7 @@ struct BatchPrimary {
 };
 
 template <int Depth>
-__global__ __launch_bounds__(kWg) void k_b_primary(BatchWave w) {
+__global__ __launch_bounds__(kWg, lux::walk_min_waves(Depth, kWg, 4 * kTallyScenes)) void k_b_primary(BatchWave w) {
     __shared__ uint2 lds[Depth][kWg];
     lux::LaneStack<Depth, kWg> stk;
     stk.bind(&lds[0][threadIdx.x]);
@@ -258,7 +258,7 @@ struct BatchShadow {
 };
 
 template <int Depth>
-__global__ __launch_bounds__(kWg) void k_b_shadow(BatchWave w) {
+__global__ __launch_bounds__(kWg, lux::walk_min_waves(Depth, kWg, 4 * kTallyScenes)) void k_b_shadow(BatchWave w) {
     __shared__ uint2 lds[Depth][kWg];
     lux::LaneStack<Depth, kWg> stk;
     stk.bind(&lds[0][threadIdx.x]);
@@ -441,7 +441,7 @@ struct lux_batch {
     uint64_t setup_waves = 0;  // waves of the NEE setup launch (occlusion slabs)
     lux_ctx* owner = nullptr;  // allocations go through the first scene's context
     lux_buffer table, bases, prim, shade, slots, vis, shadow, counters, queries, sur_scratch, dep_item;
-    uint32_t wpc = 16, depth = 16, regroup = 16, batch = 8;
+    uint32_t wpc = 24, depth = 12, regroup = 48, batch = 16;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool ready = false;
@@ -591,7 +591,7 @@ int enqueue(lux_batch* b) {
     BatchWave w = wave_of(b);
     const uint64_t items = b->item_base[n], intervals = b->interval_base[n], probes = b->probe_base[n];
     int cu = b->owner->cu_count;
-    uint32_t pblocks = (uint32_t)cu * (b->wpc ? b->wpc : 16u) / (kWg / 64);
+    uint32_t pblocks = (uint32_t)cu * (b->wpc ? b->wpc : 24u) / (kWg / 64);
     hipLaunchKernelGGL(k_b_zero, dim3((n + 4 + 255) / 256), dim3(256), 0, st, static_cast<BatchScene*>(b->table.ptr),
                        n, w.counters);
     if (items) {
@@ -907,7 +907,7 @@ extern "C" int lux_batch_autotune(lux_batch* b, uint32_t repeats, lux_trace_conf
     static const uint32_t cand_batch[] = {4, 8, 16};
     const uint32_t* cands[4] = {cand_wpc, cand_depth, cand_regroup, cand_batch};
     const uint32_t ncand[4] = {4, 4, 5, 3};
-    uint32_t cur[4] = {16, 16, 16, 8};
+    uint32_t cur[4] = {24, 12, 48, 16};
     float cur_ms = 0.0f;
     uint32_t evals = 1;
     int rc = measure(cur, cur_ms);

@@ -284,6 +284,18 @@ struct HitState {
     float b1, b2;
 };
 
+// Waves per SIMD the persistent walk kernels are compiled for: the most of
+// 6, 5, 4 whose workgroups (wg lanes, each with `depth` 8-byte LDS stack
+// slots, plus `extra_lds` bytes) fit a CU's 160 KB of LDS.  At six the
+// register budget is 80 VGPRs (512 per SIMD lane / 6, in granules of 8); the
+// walks then spill some state to scratch, and still run faster with the
+// sixth wave (traversal 550 against 522 Mrays/s at 92 VGPRs, five waves).
+constexpr int walk_min_waves(int depth, int wg, int extra_lds) {
+    return 6 * (depth * 8 * wg + extra_lds) <= 160 * 1024   ? 6
+           : 5 * (depth * 8 * wg + extra_lds) <= 160 * 1024 ? 5
+                                                             : 4;
+}
+
 // Per-lane traversal stack: the first LdsDepth slots live in LDS
 // (lane-contiguous columns, so a wave's access is one conflict-free row),
 // deeper slots spill to a private array.  TLAS entries occupy the bottom of

@@ -270,7 +270,7 @@ struct PrimarySource {
 };
 
 template <int Depth, class St = lux::NoStats>
-__global__ __launch_bounds__(kWg) void k_gi_primary(GiArgs a, Wave w) {
+__global__ __launch_bounds__(kWg, lux::walk_min_waves(Depth, kWg, 0)) void k_gi_primary(GiArgs a, Wave w) {
     __shared__ uint2 lds[Depth][kWg];
     lux::LaneStack<Depth, kWg> stk;
     stk.bind(&lds[0][threadIdx.x]);
@@ -356,7 +356,7 @@ struct ShadowSource {
 };
 
 template <int Depth, class St = lux::NoStats>
-__global__ __launch_bounds__(kWg) void k_gi_shadow(GiArgs a, Wave w) {
+__global__ __launch_bounds__(kWg, lux::walk_min_waves(Depth, kWg, 0)) void k_gi_shadow(GiArgs a, Wave w) {
     __shared__ uint2 lds[Depth][kWg];
     lux::LaneStack<Depth, kWg> stk;
     stk.bind(&lds[0][threadIdx.x]);
@@ -993,7 +993,7 @@ int run_wave(lux_ctx* ctx, Wave proto, uint64_t total, uint64_t per_unit) {
     w.slab = w.shadow + 2ull * setup_waves * 64u * nee;
     w.regroup = ctx->gi_regroup ? ctx->gi_regroup : 8u;
     w.batch = ctx->gi_batch ? ctx->gi_batch : 8u;
-    const uint32_t wpc = ctx->gi_wpc ? ctx->gi_wpc : 20u;
+    const uint32_t wpc = ctx->gi_wpc ? ctx->gi_wpc : 24u;
     const uint32_t pblocks = (uint32_t)ctx->cu_count * wpc / (kWg / 64);
     hipStream_t st = ctx->stream;
     for (uint64_t u0 = 0; u0 < total; u0 += chunk_units) {
@@ -1165,7 +1165,7 @@ extern "C" int lux_autotune_gi(lux_ctx* ctx, uint32_t repeats, lux_trace_config*
     static const uint32_t cand_batch[] = {4, 8, 16};
     const uint32_t* cands[4] = {cand_wpc, cand_depth, cand_regroup, cand_batch};
     const uint32_t ncand[4] = {4, 4, 5, 3};
-    uint32_t cur[4] = {20, 12, 16, 16};
+    uint32_t cur[4] = {24, 12, 16, 16};
     float cur_ms = 0.0f;
     uint32_t evals = 1;
     int rc = measure(cur, cur_ms);

@@ -837,8 +837,8 @@ class Context:
         """0 wavefront (default), 1 fused; results are identical."""
         self._check(self.lib.lux_gi_set_trace_mode(self.h, mode), "lux_gi_set_trace_mode")
 
-    def gi_set_trace_config(self, waves_per_cu: int = 20, lds_stack_depth: int = 8,
-                            regroup_threshold: int = 8, steps_per_refill: int = 8):
+    def gi_set_trace_config(self, waves_per_cu: int = 24, lds_stack_depth: int = 12,
+                            regroup_threshold: int = 16, steps_per_refill: int = 16):
         cfg = (C.c_uint32 * 8)(waves_per_cu, lds_stack_depth, regroup_threshold, steps_per_refill,
                                0, 0, 0, 0)
         self._check(self.lib.lux_gi_set_trace_config(self.h, cfg), "lux_gi_set_trace_config")
@@ -1078,8 +1078,8 @@ class Batch:
                     "lux_batch_sizes")
         return dict(items=int(it.value), intervals=int(iv.value), probes=int(pr.value), scenes=int(sc.value))
 
-    def set_config(self, waves_per_cu: int = 16, lds_stack_depth: int = 16, regroup_threshold: int = 16,
-                   steps_per_refill: int = 8):
+    def set_config(self, waves_per_cu: int = 24, lds_stack_depth: int = 12, regroup_threshold: int = 48,
+                   steps_per_refill: int = 16):
         cfg = (C.c_uint32 * 8)(waves_per_cu, lds_stack_depth, regroup_threshold, steps_per_refill, 0, 0, 0, 0)
         self._check(self.lib.lux_batch_set_config(self.h, cfg), "lux_batch_set_config")
 

@@ -105,8 +105,8 @@ typedef struct lux_ctx lux_ctx;
  * once that many are idle, checking every steps_per_refill walk steps
  * (0 = 8).  lds_stack_depth: traversal-stack slots per lane kept in LDS
  * (4, 8, 12 or 16; 0 = 8), deeper slots spill to private memory.
- * waves_per_cu: persistent grid size (0 = 20; 1000 = one ray per thread,
- * no persistence).  A new context starts at {20, 12, 8, 16}.  Results do not
+ * waves_per_cu: persistent grid size (0 = 24; 1000 = one ray per thread,
+ * no persistence).  A new context starts at {24, 12, 8, 16}.  Results do not
  * depend on the configuration. */
 typedef struct {
     uint32_t waves_per_cu;
@@ -477,7 +477,7 @@ int lux_gi_trace_levels(lux_ctx* ctx, uint32_t first, uint32_t last);
 int lux_gi_set_trace_mode(lux_ctx* ctx, uint32_t mode);
 /* Launch configuration of the wavefront stages' walks (same fields and
  * limits as lux_trace_config; waves_per_cu 1..32; a new context starts at
- * {20, 12, 16, 16}) and its autotuner, which times full interval traces of the
+ * {24, 12, 16, 16}) and its autotuner, which times full interval traces of the
  * configured grid and installs the fastest (overwrites the interval array).
  * No counterpart in the reference. */
 int lux_gi_set_trace_config(lux_ctx* ctx, const lux_trace_config* cfg);
@@ -657,7 +657,8 @@ int lux_batch_synchronize(lux_batch* b);
  * intervals, level-0 probes, scenes.  Each pointer nullable. */
 int lux_batch_sizes(lux_batch* b, uint64_t* items, uint64_t* intervals, uint64_t* probes, uint32_t* scenes);
 /* Launch configuration of the two persistent walks (lux_trace_config
- * meaning); setting it drops a captured graph. */
+ * meaning; a new batch starts at {24, 12, 48, 16}); setting it drops a
+ * captured graph. */
 int lux_batch_set_config(lux_batch* b, const lux_trace_config* cfg);
 int lux_batch_get_config(lux_batch* b, lux_trace_config* cfg);
 /* Autotuner over whole batch updates (coordinate descent over waves per CU,
