@@ -363,6 +363,7 @@ extern "C" int lux_scene_upload_soa(lux_ctx* ctx, const lux_mesh_desc* meshes, u
     }
     ctx->host_pos.assign(mesh_count, {});
     ctx->mesh_bounds.clear();
+    ctx->mesh_emissive.clear();
     ctx->host_mat.assign(mesh_count, {});
     for (uint32_t m = 0; m < mesh_count; ++m) {
         uint32_t n = meshes[m].triangle_count;
@@ -1179,6 +1180,7 @@ extern "C" int lux_scene_upload_words(lux_ctx* ctx, const uint32_t* scene_words,
     // host copies for the emitter table: one mesh per distinct triangle range
     ctx->host_pos.assign(ranges.size(), {});
     ctx->mesh_bounds.clear();
+    ctx->mesh_emissive.clear();
     ctx->host_mat.assign(ranges.size(), {});
     for (size_t m = 0; m < ranges.size(); ++m) {
         std::vector<float>& hp = ctx->host_pos[m];

@@ -52,6 +52,7 @@ struct lux_ctx {
     std::vector<std::vector<uint32_t>> host_mat;
     std::vector<lux_material> host_materials;
     std::vector<float> mesh_bounds;  // per mesh: local box (lo, hi) of host_pos, computed on demand
+    std::vector<std::vector<uint32_t>> mesh_emissive;  // per mesh: its emissive triangles, computed on demand
 
     // transport state
     bool gi_ready = false;

@@ -785,11 +785,31 @@ inline uint32_t level_dim_host(uint32_t d, uint32_t level) {
 
 // build_emissive_lights (src/scene.rs:2311-2372) over the uploaded scene, in
 // caller instance order and mesh triangle order.
-std::vector<Light> build_lights(const lux_ctx* ctx) {
+std::vector<Light> build_lights(lux_ctx* ctx) {
     std::vector<Light> out;
     std::vector<double> w;
     uint32_t nm = (uint32_t)ctx->host_materials.size();
     if (nm == 0) return out;
+    // A triangle whose material's luminance is not positive has zero (or no
+    // finite) power under every transform and never enters the table, so
+    // each frame transforms only the emissive triangles: per mesh, those of
+    // its own materials (cached until the scene changes); an instance with a
+    // material of its own emits from all of its triangles or none.
+    auto luminance = [&](uint32_t mt) {
+        const lux_material& m = ctx->host_materials[mt < nm ? mt : nm - 1u];
+        float em[3];
+        for (int c = 0; c < 3; ++c) em[c] = m.emission[c] > 0.0f ? m.emission[c] : 0.0f;
+        return (em[0] * 0.2126f + em[1] * 0.7152f) + em[2] * 0.0722f;
+    };
+    if (ctx->mesh_emissive.size() != ctx->host_pos.size()) {
+        ctx->mesh_emissive.assign(ctx->host_pos.size(), {});
+        for (size_t m = 0; m < ctx->host_pos.size(); ++m) {
+            const std::vector<uint32_t>& mat = ctx->host_mat[m];
+            const uint32_t nt = (uint32_t)(ctx->host_pos[m].size() / 9);
+            for (uint32_t t = 0; t < nt; ++t)
+                if (luminance(t < mat.size() ? mat[t] : 0u) > 0.0f) ctx->mesh_emissive[m].push_back(t);
+        }
+    }
     for (uint32_t k = 0; k < (uint32_t)ctx->instances.size(); ++k) {
         const lux_instance_desc& in = ctx->instances[k];
         if (in.mesh >= ctx->host_pos.size()) continue;
@@ -799,7 +819,11 @@ std::vector<Light> build_lights(const lux_ctx* ctx) {
         const std::vector<uint32_t>& mat = ctx->host_mat[in.mesh];
         uint32_t nt = (uint32_t)(pos.size() / 9);
         const bool inst_mat = k < ctx->inst_material.size();
-        for (uint32_t t = 0; t < nt; ++t) {
+        if (inst_mat && !(luminance(ctx->inst_material[k]) > 0.0f)) continue;
+        const std::vector<uint32_t>& emissive = ctx->mesh_emissive[in.mesh];
+        const uint32_t count = inst_mat ? nt : (uint32_t)emissive.size();
+        for (uint32_t e = 0; e < count; ++e) {
+            const uint32_t t = inst_mat ? e : emissive[e];
             uint32_t mt = inst_mat ? ctx->inst_material[k] : mat[t];
             uint32_t mi = mt < nm ? mt : nm - 1u;
             const lux_material& m = ctx->host_materials[mi];
