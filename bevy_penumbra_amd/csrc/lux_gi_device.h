@@ -11,7 +11,11 @@
 namespace {
 
 constexpr int kWg = 256;
-constexpr int kLdsDepth = 16;
+// LDS stack slots per lane of the one-ray-per-thread kernels (camera
+// receivers, surrogate samples and fallback, the fused traces): 16 KB per
+// workgroup, so LDS leaves room for eight waves per SIMD (the 1080p camera
+// pass: 1.93 against 2.32 ms with 16 slots, five waves; same receivers).
+constexpr int kLdsDepth = 8;
 constexpr float kPi = 3.14159265358979323846f;
 
 struct Light {
