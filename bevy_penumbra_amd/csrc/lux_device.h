@@ -154,6 +154,13 @@ __device__ __forceinline__ float box_entry_fast(float3 o, float3 d, float3 y, co
     return near_ <= far_ ? near_ : -1.0f;
 }
 
+// Whether the resumable and nested walks may divide by reciprocals: every
+// BLAS and TLAS node bound is in div_fast's range (range_ok, written after
+// each build; null: unknown).
+__device__ __forceinline__ bool scene_in_range(const DevScene& s) {
+    return s.range_ok != nullptr && s.range_ok[0] != 0u && s.range_ok[1] != 0u;
+}
+
 // orientedEdge (traversal.slang:379-392).  The reference orders the two
 // points (the larger x, then the larger y, second), takes the edge of the
 // ordered pair and negates it when it swapped them.  The swapped products
@@ -207,6 +214,24 @@ __device__ __forceinline__ TriRay make_tri_ray(float3 d, const float3* y = nullp
     }
     return r;
 }
+
+// The slab tests and triangle shear of one ray of the nested traversals:
+// by the reciprocals of its direction (div_fast, the same quotients) when the
+// scene and the ray are in range, else by IEEE division.
+struct SlabRay {
+    float3 o, d, y;
+    bool fast;
+    __device__ __forceinline__ SlabRay(const DevScene& s, float3 o_, float3 d_, bool allow = true)
+        : o(o_), d(d_), y(make_float3(0.0f, 0.0f, 0.0f)),
+          fast(allow && scene_in_range(s) && ray_in_range(o_, d_)) {
+        if (fast) y = reciprocal(d_);
+    }
+    // boxRayInterval's entry, or -1 when the box is missed
+    __device__ __forceinline__ float entry(const Node& n, float limit) const {
+        return fast ? box_entry_fast(o, d, y, n, limit) : box_entry(o, d, n, limit);
+    }
+    __device__ __forceinline__ TriRay tri_ray() const { return make_tri_ray(d, fast ? &y : nullptr); }
+};
 
 // intersectTriangle (traversal.slang:415-443).  Returns (t, w1, w2), t = -1 on miss.
 __device__ __forceinline__ float3 intersect_tri(float3 o, float3 d, const TriRay& r, float3 ta,
@@ -339,7 +364,7 @@ struct LaneStack {
 template <class Stack, class St>
 __device__ __forceinline__ void trace_instance(const DevScene& s, float3 o, float3 d, uint32_t ii,
                                                HitState& h, Stack& stk, uint32_t base, St& st,
-                                               bool any_hit) {
+                                               bool any_hit, bool by_reciprocal = true) {
     const float4* rec = s.inst + 8u * ii;
     float4 i0 = rec[0], i1 = rec[1], i2 = rec[2];
     uint4 g = *reinterpret_cast<const uint4*>(rec + 6);
@@ -347,16 +372,16 @@ __device__ __forceinline__ void trace_instance(const DevScene& s, float3 o, floa
     float3 lo = xform_point(i0, i1, i2, o);
     float3 ld = xform_dir(i0, i1, i2, d);
     st.inst();
+    const SlabRay sr(s, lo, ld, by_reciprocal);
     uint32_t count = 0;
     Node node = load_node(s.blas, s.n_blas, root);
     st.node(1);
-    float e, x;
-    bool ready = box_interval(lo, ld, node.mn, node.mx, h.t, e, x) && e >= 0.0f;
+    bool ready = sr.entry(node, h.t) >= 0.0f;
     TriRay ray;
     ray.valid = false;
     ray.kx = ray.ky = ray.kz = 0;
     ray.sx = ray.sy = 0.0f;
-    if (ready) ray = make_tri_ray(ld);
+    if (ready) ray = sr.tri_ray();
     uint32_t steps = 0;
     while (ready || count != 0) {
         if (++steps > kMaxSteps) {
@@ -382,8 +407,8 @@ __device__ __forceinline__ void trace_instance(const DevScene& s, float3 o, floa
                 Node ln = load_node(s.blas, s.n_blas, li);
                 Node rn = load_node(s.blas, s.n_blas, ri);
                 st.node(2);
-                float le = box_entry(lo, ld, ln, h.t);
-                float re = box_entry(lo, ld, rn, h.t);
+                float le = sr.entry(ln, h.t);
+                float re = sr.entry(rn, h.t);
                 if (le >= 0.0f && re >= 0.0f) {
                     if (le <= re) {
                         stk.put(base + count, make_uint2(ri, __float_as_uint(re)));
@@ -436,9 +461,12 @@ __device__ __forceinline__ void trace_instance(const DevScene& s, float3 o, floa
 }
 
 // traceRayTracked without dependency or cone terms (traversal.slang:696-811).
+// by_reciprocal = false keeps every slab test an IEEE division (same results;
+// a caller passing the constant compiles the reciprocal path out).
 template <class Stack, class St>
 __device__ __forceinline__ void trace_closest(const DevScene& s, float3 o, float3 d, float tmax,
-                                              HitState& h, Stack& stk, St& st, bool any_hit) {
+                                              HitState& h, Stack& stk, St& st, bool any_hit,
+                                              bool by_reciprocal = true) {
     h.t = tmax;
     h.tri = kEmpty;
     h.inst = kEmpty;
@@ -446,11 +474,12 @@ __device__ __forceinline__ void trace_closest(const DevScene& s, float3 o, float
     h.b1 = h.b2 = 0.0f;
     if (s.n_inst == 0) return;
     uint32_t count = 0;
+    const SlabRay sr(s, o, d, by_reciprocal);
     Node node = load_node(s.tlas, s.n_tlas, 0);
     st.node(1);
-    float re, rx;
-    if (!box_interval(o, d, node.mn, node.mx, h.t, re, rx)) return;
-    bool ready = re >= 0.0f;
+    // the root's interval, when met, starts at or after 0
+    if (sr.entry(node, h.t) < 0.0f) return;
+    bool ready = true;
     uint32_t steps = 0, ninst = s.n_inst;
     while (ready || count != 0) {
         if (++steps > kMaxSteps) {
@@ -476,8 +505,8 @@ __device__ __forceinline__ void trace_closest(const DevScene& s, float3 o, float
                 Node ln = load_node(s.tlas, s.n_tlas, li);
                 Node rn = load_node(s.tlas, s.n_tlas, ri);
                 st.node(2);
-                float le = box_entry(o, d, ln, h.t);
-                float rre = box_entry(o, d, rn, h.t);
+                float le = sr.entry(ln, h.t);
+                float rre = sr.entry(rn, h.t);
                 if (le >= 0.0f && rre >= 0.0f) {
                     if (le <= rre) {
                         stk.put(count, make_uint2(ri, __float_as_uint(rre)));
@@ -508,7 +537,7 @@ __device__ __forceinline__ void trace_closest(const DevScene& s, float3 o, float
         truncated = truncated || be < ie;
         if (truncated) h.err |= kErrUnsupported;
         for (uint32_t i = ib; i < be; ++i) {
-            trace_instance(s, o, d, i, h, stk, count, st, any_hit);
+            trace_instance(s, o, d, i, h, stk, count, st, any_hit, by_reciprocal);
             if (any_hit && h.tri != kEmpty) {
                 // an accepted hit proves an any-hit query (traversal.slang:786-792)
                 h.err &= ~kErrUnsupported;
@@ -532,16 +561,16 @@ __device__ __forceinline__ bool occluded_instance(const DevScene& s, float3 o, f
     float3 lo = xform_point(i0, i1, i2, o);
     float3 ld = xform_dir(i0, i1, i2, d);
     st.inst();
+    const SlabRay sr(s, lo, ld);
     uint32_t count = 0;
     Node node = load_node(s.blas, s.n_blas, root);
     st.node(1);
-    float e, x;
-    bool ready = box_interval(lo, ld, node.mn, node.mx, limit, e, x) && e >= 0.0f;
+    bool ready = sr.entry(node, limit) >= 0.0f;
     TriRay ray;
     ray.valid = false;
     ray.kx = ray.ky = ray.kz = 0;
     ray.sx = ray.sy = 0.0f;
-    if (ready) ray = make_tri_ray(ld);
+    if (ready) ray = sr.tri_ray();
     uint32_t steps = 0;
     bool incomplete = false;
     while (ready || count != 0) {
@@ -565,8 +594,8 @@ __device__ __forceinline__ bool occluded_instance(const DevScene& s, float3 o, f
                 Node ln = load_node(s.blas, s.n_blas, li);
                 Node rn = load_node(s.blas, s.n_blas, ri);
                 st.node(2);
-                float le = box_entry(lo, ld, ln, limit);
-                float re = box_entry(lo, ld, rn, limit);
+                float le = sr.entry(ln, limit);
+                float re = sr.entry(rn, limit);
                 if (le >= 0.0f && re >= 0.0f) {
                     if (le <= re) {
                         stk.put(base + count, make_uint2(ri, 0u));
@@ -633,12 +662,13 @@ __device__ __forceinline__ void trace_occluded(const DevScene& s, float3 o, floa
     h.b1 = h.b2 = 0.0f;
     if (s.n_inst == 0) return;
     uint32_t count = 0;
+    const SlabRay sr(s, o, d);
     Node node = load_node(s.tlas, s.n_tlas, 0);
     st.node(1);
-    float re, rx;
-    bool root = box_interval(o, d, node.mn, node.mx, limit, re, rx);
+    // the root's interval, when met, starts at or after 0
+    bool root = sr.entry(node, limit) >= 0.0f;
     if (root) h.err |= kHitRoot;
-    bool ready = root && re >= 0.0f;
+    bool ready = root;
     if (!ready) return;
     uint32_t steps = 0, ninst = s.n_inst;
     bool incomplete = false;
@@ -663,8 +693,8 @@ __device__ __forceinline__ void trace_occluded(const DevScene& s, float3 o, floa
                 Node ln = load_node(s.tlas, s.n_tlas, li);
                 Node rn = load_node(s.tlas, s.n_tlas, ri);
                 st.node(2);
-                float le = box_entry(o, d, ln, limit);
-                float rre = box_entry(o, d, rn, limit);
+                float le = sr.entry(ln, limit);
+                float rre = sr.entry(rn, limit);
                 if (le >= 0.0f && rre >= 0.0f) {
                     if (le <= rre) {
                         stk.put(count, make_uint2(ri, 0u));
@@ -757,7 +787,7 @@ __device__ __forceinline__ void walk_begin(const DevScene& s, Walk& w, float3 o,
     w.tsteps = 0;
     w.fast_b = false;
     if (s.n_inst == 0) return;
-    w.scene_fast = s.range_ok != nullptr && s.range_ok[0] != 0u && s.range_ok[1] != 0u;
+    w.scene_fast = scene_in_range(s);
     w.fast_t = w.scene_fast && ray_in_range(o, d);
     if (w.fast_t) w.yt = reciprocal(d);
     w.node = load_node(s.tlas, s.n_tlas, 0);

@@ -760,7 +760,9 @@ __global__ __launch_bounds__(kWg) void k_camera_receivers(lux::DevScene scene, C
     float3 o = make_float3(cam.o[0], cam.o[1], cam.o[2]);
     lux::HitState hs;
     lux::NoStats ns;
-    lux::trace_closest(scene, o, d, 1e20f, hs, stk, ns, false);
+    // IEEE slab tests: with the reciprocal path the kernel needs 73 VGPRs
+    // (six waves per SIMD) and the coherent camera rays ran 3 % slower
+    lux::trace_closest(scene, o, d, 1e20f, hs, stk, ns, false, false);
     float* r = out + 6ull * i;
     if (hs.tri == lux::kEmpty || (hs.err & lux::kErrUnsupported)) {
         r[0] = o.x; r[1] = o.y; r[2] = o.z;
