@@ -311,6 +311,41 @@ def test_gpu_batch_query_budget_per_scene(rooms):
         c.gi_configure(p)
 
 
+def test_gpu_batch_counters_past_the_lds_tally():
+    """The batch walks tally unbounded admissions per workgroup in LDS for the
+    first 256 scenes and use the scenes' counters directly past them: with 300
+    small rooms every unbounded scene's intervals, probes and query counters
+    equal its own lux_gi_update, and a bounded scene on either side of 256
+    keeps its budget (never more than max(limit / 2, 1) admitted)."""
+    ctxs, bounded = [], (10, 290)
+    try:
+        for i in range(300):
+            ctxs.append(make_scene(1000 + i, (120, 3.0, 2, 2, 1, 2)))
+        for j in bounded:
+            p = lux.GiParams()
+            ctypes.memmove(ctypes.byref(p), ctypes.byref(ctxs[j]._gi), ctypes.sizeof(p))
+            p.query_limit = ctxs[j].gi_sizes()[0]
+            ctxs[j].gi_configure(p)
+        refs = per_scene(ctxs, 4, 2)
+        for c in ctxs:
+            poison(c)
+        b = lux.Batch(ctxs)
+        assert b.sizes()["scenes"] == 300
+        b.set_frame(4, 2)
+        b.update()
+        got = batch_states(ctxs, b)
+        b.close()
+        for i, (r, g) in enumerate(zip(refs, got)):
+            if i in bounded:
+                attempts, admitted = g["counters"]
+                assert admitted <= ctxs[i].gi_sizes()[0] // 2 < attempts, (i, attempts, admitted)
+            else:
+                assert_same(r, g, f"scene {i}")
+    finally:
+        for c in ctxs:
+            c.close()
+
+
 def test_gpu_batch_refresh_follows_a_reconfigured_surrogate_scene():
     """A scene with a surrogate and a finite bound is reconfigured to a larger
     grid: after lux_batch_refresh its fallback words cover the new probe count
