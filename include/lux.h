@@ -592,7 +592,7 @@ int lux_camera_receivers(lux_ctx* ctx, const float* camera, uint32_t width, uint
 /* Kernel timing of the last device trace (HIP events on the context stream). */
 int lux_last_kernel_ms(lux_ctx* ctx, float* ms);
 
-/* Launch tuning: persistent-grid waves per CU (0 = 20) and refill threshold
+/* Launch tuning: persistent-grid waves per CU (0 = 24) and refill threshold
  * (idle lanes before a wave fetches new rays; 0 = run each wave's 64 rays to
  * completion).  Keeps the LDS stack depth and steps per refill. */
 int lux_set_trace_tuning(lux_ctx* ctx, uint32_t waves_per_cu, uint32_t regroup_threshold);
