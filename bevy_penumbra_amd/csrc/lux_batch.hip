@@ -44,6 +44,7 @@ struct BatchWave {
     const uint64_t* item_base;      // [n_scenes + 1] (interval, ray) items
     const uint64_t* interval_base;  // [n_scenes + 1] traced intervals
     const uint64_t* probe_base;     // [n_scenes + 1] level-0 probes
+    const uint64_t* pack_block_base;  // [n_scenes + 1] k_b_pack blocks (kPackBlockProbes probes each)
     const uint64_t* merge_base;     // [kMaxBatchLevels][n_scenes + 1] intervals merged per level
     uint32_t nee_max;
     uint32_t stride;  // items per NEE sample row of slots / vis (sample-major: smp * stride + item)
@@ -382,11 +383,25 @@ __global__ __launch_bounds__(kWg) void k_b_merge(BatchWave w, uint32_t L) {
     merge_interval(w.sc[s].a, L, x - base[s]);
 }
 
-__global__ __launch_bounds__(64) void k_b_pack(BatchWave w) {
-    __shared__ float4 smp[kMaxPackDirections];
-    __shared__ float3 dirs[kMaxPackDirections];
-    uint32_t s = find_scene(w.probe_base, w.n_scenes, blockIdx.x);
-    pack_probe(w.sc[s].a, (uint32_t)(blockIdx.x - w.probe_base[s]), smp, dirs);
+// Each block packs up to kPackBlockProbes level-0 probes of one scene with
+// that scene's cosine table (k_gi_pack's layout).
+__global__ __launch_bounds__(256) void k_b_pack(BatchWave w) {
+    extern __shared__ float pack_lds[];
+    const uint32_t s = find_scene(w.pack_block_base, w.n_scenes, blockIdx.x);
+    const GiArgs& a = w.sc[s].a;
+    const uint32_t D = a.p.base_phi * a.p.base_y;
+    float* ctab = pack_lds;
+    float* nom = ctab + 36u * D;
+    float3* dirs = reinterpret_cast<float3*>(nom + 36u);
+    float4* smp = reinterpret_cast<float4*>(pack_lds + pack_smp_offset(D));
+    pack_table(a.p, ctab, nom, dirs);
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint64_t first = (blockIdx.x - w.pack_block_base[s]) * kPackBlockProbes;
+    const uint64_t probes = w.probe_base[s + 1] - w.probe_base[s];
+    for (uint32_t j = 0; j < kPackBlockProbes / 4u; ++j) {
+        const uint64_t probe = first + wv * (kPackBlockProbes / 4u) + j;
+        if (probe < probes) pack_probe_table(a, (uint32_t)probe, ctab, nom, smp + wv * D);
+    }
 }
 
 __global__ void k_b_zero(BatchScene* sc, uint32_t n, uint32_t* counters) {
@@ -434,7 +449,8 @@ struct lux_batch {
     std::vector<float> sur_bound;  // per scene: the fallback's error bound (+inf: none)
     std::vector<lux_buffer> sur_fb;  // per scene: the batch's own fallback words
     std::vector<BatchScene> host_table;
-    std::vector<uint64_t> item_base, interval_base, probe_base, merge_base;
+    std::vector<uint64_t> item_base, interval_base, probe_base, merge_base, pack_block_base;
+    uint32_t pack_dirs = 1;  // most level-0 directions of a scene (k_b_pack's LDS)
     uint32_t nee_max = 1, max_levels = 1;
     bool any_history = false;
     uint64_t query_max = 0;
@@ -485,6 +501,8 @@ int prepare(lux_batch* b) {
     b->interval_base.assign(n + 1, 0);
     b->probe_base.assign(n + 1, 0);
     b->merge_base.assign((size_t)kMaxBatchLevels * (n + 1), 0);
+    b->pack_block_base.assign(n + 1, 0);
+    b->pack_dirs = 1;
     b->nee_max = 1;
     b->max_levels = 1;
     b->query_max = 0;
@@ -506,6 +524,8 @@ int prepare(lux_batch* b) {
         b->interval_base[s + 1] = b->interval_base[s] + traced;
         b->item_base[s + 1] = b->item_base[s] + traced * bs.R;
         b->probe_base[s + 1] = b->probe_base[s] + c->gi_probes0;
+        b->pack_block_base[s + 1] = b->pack_block_base[s] + (c->gi_probes0 + kPackBlockProbes - 1) / kPackBlockProbes;
+        b->pack_dirs = std::max(b->pack_dirs, c->gi.base_phi * c->gi.base_y);
         for (uint32_t L = 0; L < kMaxBatchLevels; ++L) {
             uint64_t cnt = (L < S) ? c->gi_offset[L + 1] - c->gi_offset[L] : 0;
             size_t row = (size_t)L * (n + 1);
@@ -530,7 +550,7 @@ int prepare(lux_batch* b) {
     if (items * b->nee_max >= (1ull << 31) || b->probe_base[n] >= (1ull << 31))
         return bfail(b, LUX_ERR_INVALID, "batch too large for one launch (2^31 NEE slots or probes)");
     if (int rc = ensure(b, b->table, sizeof(BatchScene) * n)) return rc;
-    size_t nb = 3 * (n + 1) + (size_t)kMaxBatchLevels * (n + 1);
+    size_t nb = 4 * (n + 1) + (size_t)kMaxBatchLevels * (n + 1);
     if (int rc = ensure(b, b->bases, 8 * nb)) return rc;
     if (int rc = ensure(b, b->prim, 16 * items + 16)) return rc;
     if (int rc = ensure(b, b->shade, 16 * items + 16)) return rc;
@@ -552,6 +572,7 @@ int prepare(lux_batch* b) {
     all.insert(all.end(), b->interval_base.begin(), b->interval_base.end());
     all.insert(all.end(), b->probe_base.begin(), b->probe_base.end());
     all.insert(all.end(), b->merge_base.begin(), b->merge_base.end());
+    all.insert(all.end(), b->pack_block_base.begin(), b->pack_block_base.end());
     LUX_BHIP(b, hipMemcpy(b->table.ptr, b->host_table.data(), sizeof(BatchScene) * n, hipMemcpyHostToDevice));
     LUX_BHIP(b, hipMemcpy(b->bases.ptr, all.data(), 8 * all.size(), hipMemcpyHostToDevice));
     b->ready = true;
@@ -568,6 +589,7 @@ BatchWave wave_of(lux_batch* b) {
     w.interval_base = bases + (n + 1);
     w.probe_base = bases + 2 * (n + 1);
     w.merge_base = bases + 3 * (n + 1);
+    w.pack_block_base = bases + 3 * (n + 1) + (size_t)kMaxBatchLevels * (n + 1);
     w.nee_max = b->nee_max;
     w.stride = (uint32_t)b->item_base[n];
     w.prim = static_cast<uint4*>(b->prim.ptr);
@@ -648,7 +670,9 @@ int enqueue(lux_batch* b) {
         if (cnt == 0) continue;
         hipLaunchKernelGGL(k_b_merge, dim3((uint32_t)((cnt + kWg - 1) / kWg)), dim3(kWg), 0, st, w, (uint32_t)L);
     }
-    if (probes) hipLaunchKernelGGL(k_b_pack, dim3((uint32_t)probes), dim3(64), 0, st, w);
+    if (probes)
+        hipLaunchKernelGGL(k_b_pack, dim3((uint32_t)b->pack_block_base[n]), dim3(256), pack_lds_bytes(b->pack_dirs), st,
+                           w);
     LUX_BHIP(b, hipGetLastError());
     return LUX_OK;
 }

@@ -832,42 +832,75 @@ __device__ __forceinline__ float3 decode_normal(float px, float py) {
 
 constexpr uint32_t kMaxPackDirections = 256;
 
-// packProbes for one level-0 probe by one 64-lane block; smp/dirs are the
-// block's LDS staging arrays (kMaxPackDirections entries).
-__device__ void pack_probe(const GiArgs& a, uint32_t probe, float4* smp, float3* dirs) {
+// 16 probes per 256-lane block, 4 per wave, sharing the block's cosine table.
+// Dynamic LDS (floats): the table (36 * D), the nominal sums (36), the
+// directions (3 * D), then from a 16-byte boundary four staging arrays.
+constexpr uint32_t kPackBlockProbes = 16;
+__host__ __device__ constexpr uint32_t pack_smp_offset(uint32_t D) { return (40u * D + 36u + 3u) & ~3u; }
+inline size_t pack_lds_bytes(uint32_t D) { return 4ull * (pack_smp_offset(D) + 16ull * D); }
+
+// packProbes over many probes of one direction set: the per-texel cosines
+// max(n . dir, 0) and their sum (the nominal weight) do not depend on the
+// probe, so a block computes them once (in the same order, so the same
+// bits) into `ctab` (direction-major, D * 36: the lanes of a wave read
+// consecutive words) and `nom` (36); each wave then packs its probes
+// with `smp` (D) as its staging array.  Lanes 0..35 are the texels.
+__device__ void pack_table(const lux_gi_params& p, float* ctab, float* nom, float3* dirs) {
+    const uint32_t D = p.base_phi * p.base_y;
+    for (uint32_t k = threadIdx.x; k < D; k += blockDim.x) dirs[k] = bin_direction(k, p.base_phi, p.base_y, 0.5f, 0.5f);
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < 36u; t += blockDim.x) {
+        float3 n = decode_normal(((float)(t % 6u) + 0.5f) / 6.0f, ((float)(t / 6u) + 0.5f) / 6.0f);
+        float nominal = 0.0f;
+        for (uint32_t k = 0; k < D; ++k) {
+            float c = smax(dot3(n, dirs[k]), 0.0f);
+            ctab[k * 36u + t] = c;
+            nominal = nominal + c;
+        }
+        nom[t] = nominal;
+    }
+    __syncthreads();
+}
+
+__device__ void pack_probe_table(const GiArgs& a, uint32_t probe, const float* ctab, const float* nom, float4* smp) {
     const lux_gi_params& p = a.p;
-    uint32_t D = p.base_phi * p.base_y;
+    const uint32_t D = p.base_phi * p.base_y;
+    const uint32_t lane = threadIdx.x & 63u;
     const float4* m = a.merged + (uint64_t)probe * D;
-    for (uint32_t k = threadIdx.x; k < D; k += 64u) {
+    for (uint32_t k = lane; k < D; k += 64u) {
         float4 v = m[k];
         bool ok = v.w > 0.0f && lux::isfinite_bits(v.x) && lux::isfinite_bits(v.y) &&
                   lux::isfinite_bits(v.z) && lux::isfinite_bits(v.w);
         smp[k] = ok ? v : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        dirs[k] = bin_direction(k, p.base_phi, p.base_y, 0.5f, 0.5f);
     }
-    __syncthreads();
-    uint32_t t = threadIdx.x;
-    if (t >= 36u) return;
-    float3 n = decode_normal(((float)(t % 6u) + 0.5f) / 6.0f, ((float)(t / 6u) + 0.5f) / 6.0f);
-    float3 sum = make_float3(0.0f, 0.0f, 0.0f);
-    float nominal = 0.0f, observed = 0.0f;
-    for (uint32_t k = 0; k < D; ++k) {
-        float c = smax(dot3(n, dirs[k]), 0.0f);
-        nominal = nominal + c;
-        if (c <= 0.0f) continue;
-        float4 v = smp[k];
-        if (!(v.w > 0.0f)) continue;
-        float w = c * sat(v.w);
-        sum = make_float3(sum.x + v.x * w, sum.y + v.y * w, sum.z + v.z * w);
-        observed = observed + w;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < 36u) {
+        const float* c = ctab + lane;
+        float3 sum = make_float3(0.0f, 0.0f, 0.0f);
+        float observed = 0.0f;
+        for (uint32_t k = 0; k < D; ++k) {
+            const float ck = c[k * 36u];
+            if (ck <= 0.0f) continue;
+            float4 v = smp[k];
+            if (!(v.w > 0.0f)) continue;
+            float w = ck * sat(v.w);
+            sum = make_float3(sum.x + v.x * w, sum.y + v.y * w, sum.z + v.z * w);
+            observed = observed + w;
+        }
+        const float nominal = nom[lane];
+        float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (nominal > 0.0f && observed > 0.0f) {
+            float sc = nominal * (4.0f / (float)D);
+            out = make_float4((sum.x / observed) * sc, (sum.y / observed) * sc, (sum.z / observed) * sc,
+                              sat(observed / nominal));
+        }
+        a.probes[36ull * probe + lane] = out;
     }
-    float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (nominal > 0.0f && observed > 0.0f) {
-        float sc = nominal * (4.0f / (float)D);
-        out = make_float4((sum.x / observed) * sc, (sum.y / observed) * sc, (sum.z / observed) * sc,
-                          sat(observed / nominal));
-    }
-    a.probes[36ull * probe + t] = out;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // splitLodBlendForParameters (shaders/common.slang:431-461).  floor(log2(x))

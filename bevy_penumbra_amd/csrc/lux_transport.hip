@@ -11,8 +11,9 @@
 //                k_gi_accum) or the fused k_gi_trace, one thread per interval
 //   k_gi_merge   per level, top down: support-aware Eq. 7 merge with 4 child
 //                directions x 8 trilinear corners (split.slang:389-479,1523-1725)
-//   k_gi_pack    one wave per level-0 probe: directions staged in LDS, 36
-//                octahedral texels (split.slang:1788-1918)
+//   k_gi_pack    16 level-0 probes per block (4 per wave) over the block's
+//                table of texel-direction cosines, 36 octahedral texels
+//                (split.slang:1788-1918)
 // and k_world_resolve per receiver (shaders/world_resolve.slang:1-196).
 //
 // Layout in HBM: level-major, then probe-major (x fastest), then direction;
@@ -530,11 +531,21 @@ __global__ __launch_bounds__(kWg) void k_gi_merge(GiArgs a, uint32_t L) {
 }
 
 // One 64-lane wave per level-0 probe (packProbes, split.slang:1788-1918).
-__global__ __launch_bounds__(64) void k_gi_pack(GiArgs a) {
-    __shared__ float4 smp[kMaxPackDirections];
-    __shared__ float3 dirs[kMaxPackDirections];
-    pack_probe(a, blockIdx.x, smp, dirs);
+__global__ __launch_bounds__(256) void k_gi_pack(GiArgs a, uint32_t probes) {
+    extern __shared__ float pack_lds[];
+    const uint32_t D = a.p.base_phi * a.p.base_y;
+    float* ctab = pack_lds;
+    float* nom = ctab + 36u * D;
+    float3* dirs = reinterpret_cast<float3*>(nom + 36u);
+    float4* smp = reinterpret_cast<float4*>(pack_lds + pack_smp_offset(D));
+    pack_table(a.p, ctab, nom, dirs);
+    const uint32_t wv = threadIdx.x >> 6;
+    for (uint32_t j = 0; j < kPackBlockProbes / 4u; ++j) {
+        const uint32_t probe = blockIdx.x * kPackBlockProbes + wv * (kPackBlockProbes / 4u) + j;
+        if (probe < probes) pack_probe_table(a, probe, ctab, nom, smp + wv * D);
+    }
 }
+
 
 __device__ __forceinline__ void wrap_texel(int32_t& x, int32_t& y) {
     int32_t px = x, py = y;
@@ -1321,7 +1332,8 @@ extern "C" int lux_gi_pack(lux_ctx* ctx) {
     if (!ctx) return LUX_ERR_INVALID;
     if (!ctx->gi_ready) return lux_fail(ctx, LUX_ERR_STATE, "GI not configured (lux_gi_configure)");
     GiArgs a = gi_args(ctx);
-    hipLaunchKernelGGL(k_gi_pack, dim3(ctx->gi_probes0), dim3(64), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_gi_pack, dim3((ctx->gi_probes0 + kPackBlockProbes - 1) / kPackBlockProbes), dim3(256),
+                       pack_lds_bytes(ctx->gi.base_phi * ctx->gi.base_y), ctx->stream, a, ctx->gi_probes0);
     LUX_HIP(ctx, hipGetLastError());
     return LUX_OK;
 }
@@ -1488,7 +1500,8 @@ int lux_gi_finish_internal(lux_ctx* ctx, hipStream_t st) {
     if (int rc = history_pass(ctx, ctx->gi_total, st)) return rc;
     if (int rc = merge_levels(ctx, (int)ctx->gi.levels - 1, st)) return rc;
     GiArgs a = gi_args(ctx);
-    hipLaunchKernelGGL(k_gi_pack, dim3(ctx->gi_probes0), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_gi_pack, dim3((ctx->gi_probes0 + kPackBlockProbes - 1) / kPackBlockProbes), dim3(256),
+                       pack_lds_bytes(ctx->gi.base_phi * ctx->gi.base_y), st, a, ctx->gi_probes0);
     LUX_HIP(ctx, hipGetLastError());
     return LUX_OK;
 }
