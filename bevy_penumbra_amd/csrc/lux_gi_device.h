@@ -998,12 +998,16 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 __global__ void k_fill_u32_one(uint32_t* p, uint32_t v) { *p = v; }
 
 // One wave per setup wave: its rays [0, count) from the slab to the list.
-__global__ __launch_bounds__(256) void k_shadow_compact(const float4* slab, const uint32_t* off, uint32_t n_waves,
-                                                        uint32_t slab_rays, float4* list) {
+// The slab and the list are distinct buffers (restrict): the copies of one
+// wave's rays need not wait for each other's stores.
+__global__ __launch_bounds__(256) void k_shadow_compact(const float4* __restrict__ slab,
+                                                        const uint32_t* __restrict__ off, uint32_t n_waves,
+                                                        uint32_t slab_rays, float4* __restrict__ list) {
     const uint32_t gw = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (gw >= n_waves) return;
     const uint32_t base = off[gw], cnt = off[gw + 1] - base;
-    const float4* src = slab + 2ull * gw * slab_rays;
+    const float4* __restrict__ src = slab + 2ull * gw * slab_rays;
+#pragma unroll 4
     for (uint32_t j = __lane_id(); j < cnt; j += 64u) {
         list[2ull * (base + j)] = src[2ull * j];
         list[2ull * (base + j) + 1] = src[2ull * j + 1];
