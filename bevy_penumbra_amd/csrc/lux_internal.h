@@ -112,7 +112,7 @@ struct lux_ctx {
     // trace tuning
     // traversal launch configuration (defaults: the autotuner's choice on the
     // 262,144-triangle colonnade microbenchmark)
-    uint32_t waves_per_cu = 24, regroup = 8, trace_depth = 12, trace_batch = 16;
+    uint32_t waves_per_cu = 28, regroup = 6, trace_depth = 12, trace_batch = 24;
     float last_ms = 0.0f;
 
     lux::DevScene dev_scene() const;

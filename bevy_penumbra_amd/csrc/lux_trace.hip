@@ -237,7 +237,7 @@ static int trace_launch_with(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_
     hipStream_t st = ctx->stream;
     if (timed) LUX_HIP(ctx, hipEventRecord(ctx->ev0, st));
     if (ctx->regroup > 0u && ctx->waves_per_cu != 1000u) {
-        uint32_t wpc = ctx->waves_per_cu ? ctx->waves_per_cu : 24u;
+        uint32_t wpc = ctx->waves_per_cu ? ctx->waves_per_cu : 28u;
         uint32_t blocks = (uint32_t)ctx->cu_count * wpc / (kTraceWg / 64);
         uint32_t need = (n + kTraceWg - 1) / kTraceWg;
         if (blocks > need) blocks = need;
@@ -253,7 +253,7 @@ static int trace_launch_with(lux_ctx* ctx, const lux_ray* rays, uint32_t n, lux_
         uint32_t blocks = (n + kTraceWg - 1) / kTraceWg;
         hipLaunchKernelGGL(k_trace_grid, dim3(blocks), dim3(kTraceWg), 0, st, a);
     } else {
-        uint32_t wpc = ctx->waves_per_cu ? ctx->waves_per_cu : 24u;
+        uint32_t wpc = ctx->waves_per_cu ? ctx->waves_per_cu : 28u;
         uint32_t blocks = (uint32_t)ctx->cu_count * wpc / (kTraceWg / 64);
         uint32_t need = (n + kTraceWg - 1) / kTraceWg;
         if (blocks > need) blocks = need;
@@ -432,13 +432,13 @@ extern "C" int lux_autotune_trace(lux_ctx* ctx, const lux_ray* rays, uint32_t n,
         ms = t[repeats / 2];
         return LUX_OK;
     };
-    static const uint32_t cand_wpc[] = {12, 16, 20, 24, 32};
+    static const uint32_t cand_wpc[] = {16, 20, 24, 28, 32};
     static const uint32_t cand_depth[] = {4, 8, 12, 16};
-    static const uint32_t cand_regroup[] = {0, 1, 4, 8, 16, 32};
-    static const uint32_t cand_batch[] = {2, 4, 8, 16};
+    static const uint32_t cand_regroup[] = {0, 1, 4, 6, 8, 12, 16, 32};
+    static const uint32_t cand_batch[] = {4, 8, 16, 24, 32};
     const uint32_t* cands[4] = {cand_wpc, cand_depth, cand_regroup, cand_batch};
-    const uint32_t ncand[4] = {5, 4, 6, 4};
-    uint32_t cur[4] = {24, 12, 8, 16};
+    const uint32_t ncand[4] = {5, 4, 8, 5};
+    uint32_t cur[4] = {28, 12, 6, 24};
     float cur_ms = 0.0f;
     uint32_t evals = 0;
     int rc = measure(cur, cur_ms);

@@ -105,8 +105,8 @@ typedef struct lux_ctx lux_ctx;
  * once that many are idle, checking every steps_per_refill walk steps
  * (0 = 8).  lds_stack_depth: traversal-stack slots per lane kept in LDS
  * (4, 8, 12 or 16; 0 = 8), deeper slots spill to private memory.
- * waves_per_cu: persistent grid size (0 = 24; 1000 = one ray per thread,
- * no persistence).  A new context starts at {24, 12, 8, 16}.  Results do not
+ * waves_per_cu: persistent grid size (0 = 28; 1000 = one ray per thread,
+ * no persistence).  A new context starts at {28, 12, 6, 24}.  Results do not
  * depend on the configuration. */
 typedef struct {
     uint32_t waves_per_cu;
@@ -592,7 +592,7 @@ int lux_camera_receivers(lux_ctx* ctx, const float* camera, uint32_t width, uint
 /* Kernel timing of the last device trace (HIP events on the context stream). */
 int lux_last_kernel_ms(lux_ctx* ctx, float* ms);
 
-/* Launch tuning: persistent-grid waves per CU (0 = 24) and refill threshold
+/* Launch tuning: persistent-grid waves per CU (0 = 28) and refill threshold
  * (idle lanes before a wave fetches new rays; 0 = run each wave's 64 rays to
  * completion).  Keeps the LDS stack depth and steps per refill. */
 int lux_set_trace_tuning(lux_ctx* ctx, uint32_t waves_per_cu, uint32_t regroup_threshold);
