@@ -73,7 +73,7 @@ struct lux_ctx {
     uint32_t gi_trace_mode = 0;
     // launch configuration of the wavefront stages' walks (own tuning: their
     // item setup is heavier than a ray-list load)
-    uint32_t gi_wpc = 24, gi_depth = 12, gi_regroup = 16, gi_batch = 16;
+    uint32_t gi_wpc = 24, gi_depth = 12, gi_regroup = 12, gi_batch = 24;
     lux_buffer gi_w_prim, gi_w_shade, gi_w_slots, gi_w_vis, gi_w_shadow, gi_w_counters;
     lux_buffer gi_w_dep;     // per item: NEE segments' dependency (history on)
     lux_buffer mesh_info;    // uint32 per mesh: BLAS base | triangle base | count

@@ -1196,13 +1196,13 @@ extern "C" int lux_autotune_gi(lux_ctx* ctx, uint32_t repeats, lux_trace_config*
         ms = t[repeats / 2];
         return LUX_OK;
     };
-    static const uint32_t cand_wpc[] = {16, 20, 24, 32};
+    static const uint32_t cand_wpc[] = {16, 20, 24, 28, 32};
     static const uint32_t cand_depth[] = {4, 8, 12, 16};
-    static const uint32_t cand_regroup[] = {4, 8, 16, 32, 48};
-    static const uint32_t cand_batch[] = {4, 8, 16};
+    static const uint32_t cand_regroup[] = {4, 8, 12, 16, 32, 48};
+    static const uint32_t cand_batch[] = {4, 8, 16, 24, 32};
     const uint32_t* cands[4] = {cand_wpc, cand_depth, cand_regroup, cand_batch};
-    const uint32_t ncand[4] = {4, 4, 5, 3};
-    uint32_t cur[4] = {24, 12, 16, 16};
+    const uint32_t ncand[4] = {5, 4, 6, 5};
+    uint32_t cur[4] = {24, 12, 12, 24};
     float cur_ms = 0.0f;
     uint32_t evals = 1;
     int rc = measure(cur, cur_ms);

@@ -838,7 +838,7 @@ class Context:
         self._check(self.lib.lux_gi_set_trace_mode(self.h, mode), "lux_gi_set_trace_mode")
 
     def gi_set_trace_config(self, waves_per_cu: int = 24, lds_stack_depth: int = 12,
-                            regroup_threshold: int = 16, steps_per_refill: int = 16):
+                            regroup_threshold: int = 12, steps_per_refill: int = 24):
         cfg = (C.c_uint32 * 8)(waves_per_cu, lds_stack_depth, regroup_threshold, steps_per_refill,
                                0, 0, 0, 0)
         self._check(self.lib.lux_gi_set_trace_config(self.h, cfg), "lux_gi_set_trace_config")

@@ -477,7 +477,7 @@ int lux_gi_trace_levels(lux_ctx* ctx, uint32_t first, uint32_t last);
 int lux_gi_set_trace_mode(lux_ctx* ctx, uint32_t mode);
 /* Launch configuration of the wavefront stages' walks (same fields and
  * limits as lux_trace_config; waves_per_cu 1..32; a new context starts at
- * {24, 12, 16, 16}) and its autotuner, which times full interval traces of the
+ * {24, 12, 12, 24}) and its autotuner, which times full interval traces of the
  * configured grid and installs the fastest (overwrites the interval array).
  * No counterpart in the reference. */
 int lux_gi_set_trace_config(lux_ctx* ctx, const lux_trace_config* cfg);
